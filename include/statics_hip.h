@@ -1,0 +1,178 @@
+/*
+ * statics_hip: an MI355X (gfx950) backend for the statics crate's solve:
+ * the forces in a voxel structure under load, all nodes at once, by
+ * conjugate gradients preconditioned with an octree multigrid V-cycle.
+ *
+ * The model is the reference's face-node/beam lattice (statics/src/model.rs,
+ * statics/src/lattice.rs): every free cell's six faces are nodes with six
+ * degrees of freedom (translation, rotation); the cell joins each pair of its
+ * faces by a beam of four springs ("bonds"). A face of a held cell is held.
+ * Cells are stored on the device in 8^3 bricks aligned to multiples of 8 of
+ * the lattice (octree level 3 above the lattice's level).
+ *
+ * Every entry point returns an StStatus. Output buffers are owned by the
+ * caller. One context owns one device stream and its buffers; calls on one
+ * context are serialised by the caller (as a Rust &mut would be).
+ *
+ * Node order. Nodes are the faces of the free cells. Each is named by its
+ * face: orientation o (0 x, 1 y, 2 z) and the lattice coordinates of the
+ * cell on its + side (the greater of the two cells it lies between, which is
+ * the reference's around[v][1]). st_node_faces lists them in the device's
+ * order; st_lookup_faces maps faces to device indices, so a caller can keep
+ * the reference's own node order (statics/src/lattice.rs:124-163).
+ */
+#ifndef STATICS_HIP_H
+#define STATICS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  ST_OK = 0,
+  ST_BAD_INPUT = 1,     /* inconsistent or out-of-range arguments */
+  ST_NO_CONVERGENCE = 2,/* the iteration cap met before the tolerance (x holds the last iterate) */
+  ST_NOT_SPD = 3,       /* a non-positive curvature p.Ap or a singular block */
+  ST_NAN = 4,           /* a NaN or Inf in the iteration */
+  ST_OOM = 5,           /* a device allocation failed */
+  ST_DEVICE = 6,        /* any other HIP error */
+  ST_NO_STRUCTURE = 7   /* an entry point needing uploaded cells called before st_upload */
+} StStatus;
+
+typedef struct StCtx StCtx;
+
+/* One structure's cells, as the reference's `solids` after hold_bulk
+ * (statics/src/lattice.rs:50-58, 87-119): each solid cell, its site in
+ * lattice units (the reference's site >> level), whether it is held, and for
+ * free cells the stiffness of its material along (e) and across (g) a bond,
+ * in pascals. beams: per cell a 15-bit mask of its beams that are alive, bit
+ * q for the q-th face pair (a, b), a < b in VOXEL_FACES order
+ * (lattice/src/lib.rs:64); NULL means all alive. Cells may come in any order;
+ * a site may appear once. */
+typedef struct {
+  int64_t n_cells;
+  const int32_t* sites; /* 3 per cell: x, y, z */
+  const uint8_t* held;  /* 1 per cell */
+  const double* e;      /* 1 per cell (ignored for held cells) */
+  const double* g;      /* 1 per cell */
+  const uint16_t* beams;/* 1 per cell, or NULL */
+  double spacing;       /* metres per cell: UNIT * 2^level (statics/src/lattice.rs:34-36) */
+} StCells;
+
+typedef enum { ST_FP64 = 0, ST_MIXED = 1 } StPrecision;
+
+typedef struct {
+  int32_t precision;   /* StPrecision */
+  double tol;          /* stop when ||f - K x||_2 <= tol * ||f||_2 (statics/src/bodies.rs:350-352) */
+  int32_t max_iter;    /* cap on CG iterations (the reference: 100,000) */
+  int32_t refresh;     /* ST_MIXED: recompute the FP64 true residual every `refresh` iterations (0: 50) */
+  int32_t warm_start;  /* nonzero: start from x_out's contents instead of 0 */
+} StSolveOpts;
+
+typedef struct {
+  int32_t iters;          /* CG iterations taken */
+  int32_t levels;         /* multigrid levels, the finest included */
+  int32_t refreshes;      /* FP64 true-residual recomputations (ST_MIXED) */
+  int32_t status;         /* StStatus of the solve */
+  double rel_residual;    /* final ||f - K x|| / ||f||, recomputed in FP64 */
+  double setup_ms;        /* classification, numbering, hierarchy */
+  double solve_ms;        /* the iterations */
+  int64_t nodes;          /* face nodes (held and gone included) */
+  int64_t dof_nodes;      /* free nodes: not held, not gone */
+  int64_t cells;          /* free cells */
+  int64_t bricks;         /* bricks on the device */
+} StStats;
+
+/* ---- context ---------------------------------------------------------- */
+int st_create(StCtx** ctx, int device);
+int st_destroy(StCtx* ctx);
+/* The last error's text for this context (static storage, overwritten by the next call). */
+const char* st_last_error(const StCtx* ctx);
+
+/* ---- structure -------------------------------------------------------- */
+/* Replaces: lattice::build (statics/src/lattice.rs:124-163). Uploads the cells
+ * into device bricks and numbers the face nodes; no lattice of bonds is ever
+ * materialised. */
+int st_upload(StCtx* ctx, const StCells* cells);
+/* Counts of what st_upload built. */
+int st_sizes(StCtx* ctx, int64_t* nodes, int64_t* cells, int64_t* bricks);
+/* Each node's face, 4 int32 per node: o, x, y, z (the + side cell), in device order. */
+int st_node_faces(StCtx* ctx, int32_t* faces_out);
+/* Device index of each face (4 int32 each, as st_node_faces); -1 where no node. */
+int st_lookup_faces(StCtx* ctx, int64_t n, const int32_t* faces, int64_t* index_out);
+/* Per node: 1 if held (a face of a held cell; statics/src/lattice.rs:138). */
+int st_node_held(StCtx* ctx, uint8_t* held_out);
+
+/* ---- classification --------------------------------------------------- */
+/* Replaces: bodies::joined_to_held (statics/src/bodies.rs:557-574). Per node,
+ * 1 if a chain of alive beams joins it to a held node. beams_alive: per cell,
+ * in st_upload's cell order, the 15-bit alive mask (NULL: the uploaded masks).
+ * The result also becomes the context's `gone` (gone = !joined) unless
+ * keep_gone is nonzero. */
+int st_joined_to_held(StCtx* ctx, const uint16_t* beams_alive, uint8_t* joined_out, int keep_gone);
+/* Sets the nodes taken out of the solve (the `gone` argument of motions,
+ * statics/src/bodies.rs:429); NULL clears it. Beams touching a gone node are
+ * not live (statics/src/bodies.rs:434). */
+int st_set_gone(StCtx* ctx, const uint8_t* gone);
+/* Replaces the per-cell alive masks (settle's broken beams, statics/src/bodies.rs:525-533). */
+int st_set_beams(StCtx* ctx, const uint16_t* beams_alive);
+
+/* ---- operator --------------------------------------------------------- */
+/* Replaces: Level::apply at level 0 (statics/src/bodies.rs:193-206): y = K x
+ * over the live beams, 6 doubles per node in device order; rows of held and
+ * gone nodes are 0 and their x is ignored. */
+int st_apply(StCtx* ctx, const double* x, double* y);
+
+/* ---- solve ------------------------------------------------------------ */
+/* Replaces: bodies::motions (statics/src/bodies.rs:429-468). loads: 6 doubles
+ * per node in device order (force, moment), read for free nodes only. x_out:
+ * 6 doubles per node; 0 for held and gone nodes. */
+int st_solve(StCtx* ctx, const double* loads, const StSolveOpts* opts, double* x_out, StStats* stats);
+
+/* Replaces: bodies::moved (statics/src/bodies.rs:420-423) for every live bond:
+ * per cell (in upload order) 15 beams x 4 bonds x 3 doubles, the force on
+ * the j side of each bond, bonds in the reference's order
+ * (statics/src/lattice.rs:76-84); 0 for dead beams. x: 6 per node. */
+int st_bond_forces(StCtx* ctx, const double* x, double* forces_out);
+
+/* ---- gradients (no counterpart in the reference) ---------------------- */
+typedef enum { ST_COMPLIANCE = 0, ST_SOFTMAX_STRESS = 1 } StObjective;
+/* After st_solve: J and dJ/dtheta per free cell (upload order; 0 for held
+ * cells), where theta is the cell's log-stiffness (e and g both scale by
+ * exp(theta)). ST_COMPLIANCE: J = f.x. ST_SOFTMAX_STRESS: J = (1/beta) log
+ * sum_b exp(beta sigma_b) over live bonds b, sigma_b = (f_b . n_b)/a_b, the
+ * bond's normal stress. The adjoint solve reuses the operator and rebuilds
+ * the hierarchy; it runs with the precision and tolerance of opts. */
+int st_backward(StCtx* ctx, int objective, double beta, const StSolveOpts* opts, double* J_out,
+                double* dJ_dtheta_out, StStats* stats);
+
+/* ---- batches ---------------------------------------------------------- */
+/* Many independent structures in one set of launches: every kernel covers
+ * all structures' bricks; CG scalars are per structure and a structure that
+ * has converged stops updating. cells[s] and loads[s] as for st_upload and
+ * st_solve (loads may be NULL: gravity along -z of weight 1 per free cell,
+ * a sixth on each face). x_out[s] may be NULL. stats: one per structure. */
+int st_solve_batch(StCtx* ctx, int32_t n, const StCells* const* cells, const double* const* loads,
+                   const StSolveOpts* opts, double* const* x_out, StStats* stats);
+
+/* ---- synthetic structures (the workloads the benchmarks are quoted on) -- */
+typedef struct {
+  int32_t nx, ny, nz;      /* lattice extent; z is up */
+  uint64_t seed;
+  double pillar_spacing;   /* Poisson-disk radius between pillar centres, cells (8) */
+  int32_t pillar_min, pillar_max; /* pillar width range, cells (2, 6) */
+  int32_t slab_min, slab_max;     /* gap between slabs, cells (8, 16) */
+  double hole_p;           /* probability a 2x2 tile of a slab is a hole (0.2) */
+  double sigma;            /* log-stiffness standard deviation (0: unit stiffness) */
+} StGenParams;
+/* Fills occ (nx*ny*nz bytes, index (x*ny + y)*nz + z): 0 empty, 1 free, 2
+ * held; theta (same size, may be NULL): log-stiffness per cell, N(0, sigma^2). */
+int st_generate(const StGenParams* p, uint8_t* occ, float* theta);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1118 @@
+// The oracle: a CPU FP64 restatement of the reference's statics crate, for
+// tests and the CPU baseline only. It is test infrastructure: nothing in the
+// product links, loads or calls it.
+//
+// Followed, function by function (all paths relative to the reference repo):
+//   statics/src/model.rs    Mat, Node, Bond::{force, used, stiffness}, Mat::used
+//   statics/src/lattice.rs  spacing, centre, coarse_site, class, beam, hold_bulk, build
+//   statics/src/bodies.rs   shift, each, times*, Level::{new, of, most_asked, apply,
+//                           jacobi, ask, coarser}, merge, Cholesky, conjugate_gradients,
+//                           cycle, forces, moved, motions, bodies_of, settle, used,
+//                           used_standing, joined_to_held, invert, find
+// Rust semantics mirrored: sums are sequential left folds; `max_by` keeps the
+// LAST maximum (invert's pivot); `>>` on i64 floors; f64::round is half away
+// from zero (std::round); HashMaps are only looked up, never iterated.
+//
+// Pinned by tests/test_oracle.py: closed forms (SECTION identity, a single
+// bond, principal stresses), invariants (symmetry, rigid motions, equilibrium,
+// f.x = x.K.x), brute-force dense solves on small lattices, independent BFS
+// and hold_bulk restatements, numpy inverses/Cholesky, and the level-0
+// "used" numbers the reference documents in docs/statics.md:236-245.
+// The exact iteration path at the reference's hardcoded 1e-6 (iteration counts,
+// Work totals, most_asked's 12-step value) cannot be pinned without running
+// the Rust code, which has no toolchain here; parity is defined on the
+// converged solution.
+//
+// Additions that the reference does not have (oracle for new functionality):
+//   orc_grad_*     compliance and soft-max stress gradients w.r.t. per-cell
+//                  log-stiffness (adjoint), pinned by central differences and
+//                  the scaling invariants.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+using V3 = std::array<double, 3>;
+using Six = std::array<double, 6>;
+using M6 = std::array<std::array<double, 6>, 6>;
+using M3 = std::array<std::array<double, 3>, 3>;
+using Site = std::array<int64_t, 3>;
+
+// model.rs:16-36
+V3 add(V3 a, V3 b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+V3 sub(V3 a, V3 b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+V3 mul(V3 a, double s) { return {a[0] * s, a[1] * s, a[2] * s}; }
+double dot(V3 a, V3 b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+V3 cross(V3 a, V3 b) {
+  return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+double norm(V3 a) { return std::sqrt(dot(a, a)); }
+V3 unit(V3 a) { return mul(a, 1.0 / norm(a)); }
+
+constexpr double GRAVITY = 9.81;  // model.rs:38
+
+// model.rs:43-51 (name dropped: it is only printed by the examples)
+struct Mat {
+  double rho, e, g, ft, fc, fs;
+  // model.rs:66-68
+  double used(double s1, double s3) const {
+    return std::max(std::max(s1 / ft, -s3 / fc), (s1 - s3) / (2.0 * fs));
+  }
+};
+
+// model.rs:92-98
+struct Node {
+  V3 x;
+  double m;
+  Mat mat;
+  bool held;
+};
+
+// model.rs:101-111
+struct Bond {
+  size_t i, j;
+  V3 p, n;
+  double a, d;
+  Mat mat;
+  // model.rs:115-119
+  V3 force(V3 delta) const {
+    double along = dot(delta, n);
+    double k = a / d;
+    return add(mul(n, mat.e * k * along), mul(sub(delta, mul(n, along)), mat.g * k));
+  }
+  // model.rs:123-129
+  double used(V3 f) const {
+    V3 t = mul(f, 1.0 / a);
+    double sigma = dot(t, n);
+    double tau2 = std::max(dot(t, t) - sigma * sigma, 0.0);
+    double r = std::sqrt(sigma * sigma / 4.0 + tau2);
+    return mat.used(sigma / 2.0 + r, sigma / 2.0 - r);
+  }
+  // model.rs:139-149
+  M3 stiffness() const {
+    double k = a / d;
+    M3 s{};
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        double nn = n[r] * n[c];
+        s[r][c] = mat.e * k * nn + mat.g * k * ((r == c ? 1.0 : 0.0) - nn);
+      }
+    return s;
+  }
+};
+
+// lattice.rs:16-25
+constexpr double UNIT = 0.125;
+constexpr size_t PER_BEAM = 4;
+constexpr double SECTION[2] = {0.336, 0.336};
+// lattice/src/lib.rs:64
+constexpr int VOXEL_FACES[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+
+// lattice.rs:29-32
+size_t cls(V3 a, V3 b, V3 c) {
+  V3 u = sub(a, c), v = sub(b, c);
+  return (size_t)std::round(1.0 + dot(u, v) / (norm(u) * norm(v)));
+}
+// lattice.rs:34-36
+double spacing(uint32_t level) { return UNIT * (double)(uint64_t(1) << level); }
+// lattice.rs:39-41
+V3 centre(Site site, uint32_t level) {
+  V3 c;
+  for (int k = 0; k < 3; k++) c[k] = ((double)site[k] + 0.5 * (double)(uint64_t(1) << level)) * UNIT;
+  return c;
+}
+// lattice.rs:45-47: arithmetic shift floors, as Rust's >> on i64
+Site coarse_site(Site p, uint32_t l) {
+  Site q;
+  for (int k = 0; k < 3; k++) q[k] = (p[k] >> l) << l;
+  return q;
+}
+
+struct Solid {
+  Site site;
+  Mat mat;
+  bool held;
+};
+
+using Around = std::array<Site, 2>;
+struct AroundHash {
+  size_t operator()(const Around& a) const {
+    uint64_t h = 1469598103934665603ull;
+    for (auto& s : a)
+      for (auto v : s) h = (h ^ (uint64_t)v) * 1099511628211ull;
+    return (size_t)h;
+  }
+};
+struct SiteHash {
+  size_t operator()(const Site& s) const {
+    uint64_t h = 1469598103934665603ull;
+    for (auto v : s) h = (h ^ (uint64_t)v) * 1099511628211ull;
+    return (size_t)h;
+  }
+};
+
+// lattice.rs:61-71 (+ which cell emitted each beam, for the gradient oracle)
+struct Lattice {
+  uint32_t level;
+  std::vector<Node> nodes;
+  std::vector<Bond> bonds;
+  std::vector<Around> around;
+  std::vector<size_t> beam_cell;  // per beam (PER_BEAM bonds), the index into the solids
+};
+
+// lattice.rs:76-84
+void beam(size_t i, size_t j, V3 a, V3 b, double area, Mat mat, std::vector<Bond>& out) {
+  V3 n = unit(sub(b, a));
+  V3 e1 = unit(cross(n, std::fabs(n[0]) < 0.9 ? V3{1.0, 0.0, 0.0} : V3{0.0, 1.0, 0.0}));
+  V3 e2 = cross(n, e1);
+  double r = std::sqrt(area / (2.0 * M_PI));
+  V3 middle = mul(add(a, b), 0.5);
+  double d = norm(sub(b, a));
+  for (V3 o : {mul(e1, r), mul(e1, -r), mul(e2, r), mul(e2, -r)})
+    out.push_back(Bond{i, j, add(middle, o), n, area / 4.0, d, mat});
+}
+
+// lattice.rs:107-119
+void hold_bulk(uint32_t level, std::vector<Solid>& solids, double radius) {
+  int64_t r = (int64_t)std::ceil(radius / spacing(level));
+  std::vector<Site> near;
+  for (int64_t a = -r; a <= r; a++)
+    for (int64_t b = -r; b <= r; b++)
+      for (int64_t c = -r; c <= r; c++)
+        if (std::sqrt((double)(a * a + b * b + c * c)) * spacing(level) <= radius)
+          near.push_back({a << level, b << level, c << level});
+  std::unordered_map<Site, char, SiteHash> filled;
+  for (auto& c : solids) filled[c.site] = 1;
+  std::vector<char> held(solids.size());
+  for (size_t i = 0; i < solids.size(); i++) {
+    bool all = true;
+    for (auto& o : near) {
+      Site s{solids[i].site[0] + o[0], solids[i].site[1] + o[1], solids[i].site[2] + o[2]};
+      if (!filled.count(s)) {
+        all = false;
+        break;
+      }
+    }
+    held[i] = all;
+  }
+  for (size_t i = 0; i < solids.size(); i++) solids[i].held = held[i];
+}
+
+// lattice.rs:124-163
+Lattice build(uint32_t level, const std::vector<Solid>& solids) {
+  double s = spacing(level);
+  std::unordered_map<Site, char, SiteHash> held;
+  for (auto& c : solids)
+    if (c.held) held[c.site] = 1;
+  std::unordered_map<Around, size_t, AroundHash> index;
+  Lattice lat;
+  lat.level = level;
+  for (size_t ci = 0; ci < solids.size(); ci++) {
+    const Solid& cell = solids[ci];
+    if (cell.held) continue;
+    double m = cell.mat.rho * s * s * s / 6.0;
+    size_t ends[6];
+    for (int f = 0; f < 6; f++) {
+      Site other;
+      for (int k = 0; k < 3; k++) other[k] = cell.site[k] + ((int64_t)VOXEL_FACES[f][k] << level);
+      Around key = cell.site < other ? Around{cell.site, other} : Around{other, cell.site};
+      auto it = index.find(key);
+      size_t k;
+      if (it == index.end()) {
+        V3 x = mul(add(centre(cell.site, level), centre(other, level)), 0.5);
+        lat.nodes.push_back(Node{x, 0.0, cell.mat, held.count(other) > 0});
+        lat.around.push_back(key);
+        k = lat.nodes.size() - 1;
+        index.emplace(key, k);
+      } else {
+        k = it->second;
+      }
+      Node& node = lat.nodes[k];
+      node.m += m;
+      if (cell.mat.rho > node.mat.rho) node.mat = cell.mat;
+      ends[f] = k;
+    }
+    V3 middle = centre(cell.site, level);
+    for (int a = 0; a < 6; a++)
+      for (int b = a + 1; b < 6; b++)
+        if (!(lat.nodes[ends[a]].held && lat.nodes[ends[b]].held)) {
+          V3 x = lat.nodes[ends[a]].x, y = lat.nodes[ends[b]].x;
+          double area = SECTION[cls(x, y, middle)] * s * s;
+          beam(ends[a], ends[b], x, y, area, cell.mat, lat.bonds);
+          lat.beam_cell.push_back(ci);
+        }
+  }
+  return lat;
+}
+
+// ---------------------------------------------------------------- bodies.rs
+
+// bodies.rs:44-47
+Six shift(Six x, V3 r) {
+  V3 u = add({x[0], x[1], x[2]}, cross({x[3], x[4], x[5]}, r));
+  return {u[0], u[1], u[2], x[3], x[4], x[5]};
+}
+
+// bodies.rs:51-69: contiguous chunks over the host's hardware threads; serial below 256.
+int g_threads = 0;  // 0: std::thread::hardware_concurrency()
+int threads_now() {
+  int t = g_threads > 0 ? g_threads : (int)std::thread::hardware_concurrency();
+  return t < 1 ? 1 : t;
+}
+template <class T, class F>
+std::vector<T> each(size_t n, F f) {
+  int threads = threads_now();
+  std::vector<T> out(n);
+  if (n < 256 || threads == 1) {
+    for (size_t i = 0; i < n; i++) out[i] = f(i);
+    return out;
+  }
+  size_t size = (n + threads - 1) / threads;
+  std::vector<std::thread> pool;
+  for (size_t c = 0; c * size < n; c++) {
+    pool.emplace_back([&, c]() {
+      size_t hi = std::min(n, (c + 1) * size);
+      for (size_t k = c * size; k < hi; k++) out[k] = f(k);
+    });
+  }
+  for (auto& t : pool) t.join();
+  return out;
+}
+
+// bodies.rs:71-77
+Six times(const M6& a, const Six& x) {
+  Six y;
+  for (int s = 0; s < 6; s++) {
+    double acc = 0.0;
+    for (int t = 0; t < 6; t++) acc += a[s][t] * x[t];
+    y[s] = acc;
+  }
+  return y;
+}
+Six times_transposed(const M6& a, const Six& x) {
+  Six y;
+  for (int s = 0; s < 6; s++) {
+    double acc = 0.0;
+    for (int t = 0; t < 6; t++) acc += a[t][s] * x[t];
+    y[s] = acc;
+  }
+  return y;
+}
+Six add6v(const Six& a, const Six& b) {
+  Six y;
+  for (int s = 0; s < 6; s++) y[s] = a[s] + b[s];
+  return y;
+}
+// bodies.rs:83-89
+M6 product(const M6& a, const M6& b) {
+  M6 c;
+  for (int r = 0; r < 6; r++)
+    for (int q = 0; q < 6; q++) {
+      double acc = 0.0;
+      for (int k = 0; k < 6; k++) acc += a[r][k] * b[k][q];
+      c[r][q] = acc;
+    }
+  return c;
+}
+M6 transpose(const M6& a) {
+  M6 t;
+  for (int r = 0; r < 6; r++)
+    for (int c = 0; c < 6; c++) t[r][c] = a[c][r];
+  return t;
+}
+// bodies.rs:91-97
+void add6(M6& to, const M6& a, double by) {
+  for (int r = 0; r < 6; r++)
+    for (int c = 0; c < 6; c++) to[r][c] += by * a[r][c];
+}
+// bodies.rs:100-109
+template <class F>
+M6 matrix(F f) {
+  M6 a{};
+  for (int c = 0; c < 6; c++) {
+    Six e{};
+    e[c] = 1.0;
+    Six col = f(e);
+    for (int r = 0; r < 6; r++) a[r][c] = col[r];
+  }
+  return a;
+}
+using Row = std::vector<std::pair<size_t, M6>>;
+// bodies.rs:112-121
+void merge(Row& row, size_t c, const M6& m, double by) {
+  for (auto& e : row)
+    if (e.first == c) {
+      add6(e.second, m, by);
+      return;
+    }
+  M6 to{};
+  add6(to, m, by);
+  row.emplace_back(c, to);
+}
+
+// bodies.rs:576-602: Gauss-Jordan, the pivot the LAST greatest |a[x][col]| (Rust max_by).
+M6 invert(const M6& m) {
+  M6 a = m, inv{};
+  for (int k = 0; k < 6; k++) inv[k][k] = 1.0;
+  for (int col = 0; col < 6; col++) {
+    int piv = col;
+    for (int x = col + 1; x < 6; x++)
+      if (!(std::fabs(a[x][col]) < std::fabs(a[piv][col]))) piv = x;  // ties move to the later row
+    std::swap(a[col], a[piv]);
+    std::swap(inv[col], inv[piv]);
+    double d = a[col][col];
+    for (int k = 0; k < 6; k++) {
+      a[col][k] /= d;
+      inv[col][k] /= d;
+    }
+    for (int r = 0; r < 6; r++) {
+      double f = a[r][col];
+      if (r != col && f != 0.0)
+        for (int k = 0; k < 6; k++) {
+          a[r][k] -= f * a[col][k];
+          inv[r][k] -= f * inv[col][k];
+        }
+    }
+  }
+  return inv;
+}
+
+double dot6(const std::vector<Six>& a, const std::vector<Six>& b) {
+  double acc = 0.0;
+  for (size_t k = 0; k < a.size(); k++) {
+    double s6 = 0.0;
+    for (int s = 0; s < 6; s++) s6 += a[k][s] * b[k][s];
+    acc += s6;
+  }
+  return acc;
+}
+
+struct Cholesky {
+  size_t n = 0;
+  std::vector<double> l;
+  std::vector<Six> solve(const std::vector<Six>& r) const;
+};
+
+struct Pair {
+  size_t i, j;
+  M6 m;
+};
+
+struct Level {
+  std::vector<V3> centre;
+  std::vector<M6> own;
+  std::vector<Pair> pairs;
+  std::vector<std::vector<std::tuple<size_t, size_t, bool>>> rows;
+  std::vector<M6> inverse;
+  double step = 1.0;
+  std::vector<Row> down;
+  std::vector<std::vector<std::pair<size_t, size_t>>> from;
+  bool has_direct = false;
+  Cholesky direct;
+
+  // bodies.rs:193-206
+  std::vector<Six> apply(const std::vector<Six>& x, double& work) const {
+    work += (double)(x.size() + pairs.size());
+    return each<Six>(x.size(), [&](size_t i) {
+      Six y = times(own[i], x[i]);
+      for (auto& [o, p, tr] : rows[i]) {
+        const M6& m = pairs[p].m;
+        Six a = tr ? times_transposed(m, x[o]) : times(m, x[o]);
+        for (int s = 0; s < 6; s++) y[s] += a[s];
+      }
+      return y;
+    });
+  }
+  // bodies.rs:208-210
+  std::vector<Six> jacobi(const std::vector<Six>& r) const {
+    return each<Six>(r.size(), [&](size_t i) { return times(inverse[i], r[i]); });
+  }
+  // bodies.rs:180-190
+  double most_asked() const {
+    std::vector<Six> v(own.size());
+    for (size_t k = 0; k < own.size(); k++)
+      for (int s = 0; s < 6; s++) v[k][s] = 1.0 + (double)((7 * k + 3 * s) % 11) / 11.0;
+    double most = 1.0;
+    for (int it = 0; it < 12; it++) {
+      double w0 = 0.0;
+      std::vector<Six> w = jacobi(apply(v, w0));
+      most = std::sqrt(dot6(w, w)) / std::sqrt(dot6(v, v));
+      double scale = 1.0 / std::sqrt(dot6(w, w));
+      for (auto& x : w)
+        for (auto& a : x) a *= scale;
+      v = std::move(w);
+    }
+    return most;
+  }
+  // bodies.rs:213-225
+  template <class F>
+  Row ask(size_t k, F m) const {
+    Row row;
+    for (auto& [c, b] : m(k)) merge(row, c, product(own[k], b), 1.0);
+    for (auto& [o, p, tr] : rows[k]) {
+      M6 a = tr ? transpose(pairs[p].m) : pairs[p].m;
+      for (auto& [c, b] : m(o)) merge(row, c, product(a, b), 1.0);
+    }
+    return row;
+  }
+  Level coarser(const std::vector<size_t>& up, size_t count);
+};
+
+// bodies.rs:166-176
+Level level_of(std::vector<V3> centre, std::vector<M6> own, std::vector<Pair> pairs) {
+  Level lv;
+  lv.rows.assign(own.size(), {});
+  for (size_t p = 0; p < pairs.size(); p++) {
+    lv.rows[pairs[p].i].emplace_back(pairs[p].j, p, false);
+    lv.rows[pairs[p].j].emplace_back(pairs[p].i, p, true);
+  }
+  lv.inverse.resize(own.size());
+  for (size_t k = 0; k < own.size(); k++) lv.inverse[k] = invert(own[k]);
+  lv.centre = std::move(centre);
+  lv.own = std::move(own);
+  lv.pairs = std::move(pairs);
+  lv.step = 4.0 / 3.0 / lv.most_asked();
+  return lv;
+}
+
+struct PairHash {
+  size_t operator()(const std::pair<size_t, size_t>& p) const { return p.first * 1000003u ^ p.second; }
+};
+
+// bodies.rs:126-164
+Level level_new(const Lattice& lat, const std::vector<int64_t>& part, size_t parts, const std::vector<size_t>& live) {
+  std::vector<V3> centre(parts, V3{0, 0, 0});
+  for (size_t v = 0; v < part.size(); v++)
+    if (part[v] >= 0) centre[part[v]] = lat.nodes[v].x;
+  std::vector<M6> own(parts, M6{});
+  std::vector<Pair> pairs;
+  std::unordered_map<std::pair<size_t, size_t>, size_t, PairHash> pair;
+  using T3 = std::array<std::array<double, 6>, 3>;
+  for (size_t b : live) {
+    const Bond& bond = lat.bonds[b];
+    M3 s = bond.stiffness();
+    auto seen = [&](size_t v) -> T3 {
+      V3 r = sub(bond.p, lat.nodes[v].x);
+      return T3{{{1.0, 0.0, 0.0, 0.0, r[2], -r[1]}, {0.0, 1.0, 0.0, -r[2], 0.0, r[0]}, {0.0, 0.0, 1.0, r[1], -r[0], 0.0}}};
+    };
+    auto through = [&](const T3& a, const T3& bb) -> M6 {
+      T3 sb;
+      for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 6; c++) {
+          double acc = 0.0;
+          for (int k = 0; k < 3; k++) acc += s[r][k] * bb[k][c];
+          sb[r][c] = acc;
+        }
+      M6 out;
+      for (int r = 0; r < 6; r++)
+        for (int c = 0; c < 6; c++) {
+          double acc = 0.0;
+          for (int k = 0; k < 3; k++) acc += a[k][r] * sb[k][c];
+          out[r][c] = acc;
+        }
+      return out;
+    };
+    int64_t pi = part[bond.i], pj = part[bond.j];
+    T3 ti, tj;
+    if (pi >= 0) ti = seen(bond.i);
+    if (pj >= 0) tj = seen(bond.j);
+    if (pi >= 0) add6(own[pi], through(ti, ti), 1.0);
+    if (pj >= 0) add6(own[pj], through(tj, tj), 1.0);
+    if (pi >= 0 && pj >= 0) {
+      size_t i = pi, j = pj;
+      const T3* a = &ti;
+      const T3* bb = &tj;
+      if (!(i < j)) {
+        std::swap(i, j);
+        std::swap(a, bb);
+      }
+      auto it = pair.find({i, j});
+      size_t k;
+      if (it == pair.end()) {
+        pairs.push_back(Pair{i, j, M6{}});
+        k = pairs.size() - 1;
+        pair.emplace(std::make_pair(i, j), k);
+      } else {
+        k = it->second;
+      }
+      add6(pairs[k].m, through(*a, *bb), -1.0);
+    }
+  }
+  return level_of(std::move(centre), std::move(own), std::move(pairs));
+}
+
+// bodies.rs:228-282
+Level Level::coarser(const std::vector<size_t>& up, size_t count) {
+  std::vector<V3> c(count, V3{0, 0, 0});
+  std::vector<double> n(count, 0.0);
+  for (size_t k = 0; k < up.size(); k++) {
+    c[up[k]] = add(c[up[k]], centre[k]);
+    n[up[k]] += 1.0;
+  }
+  for (size_t q = 0; q < count; q++) c[q] = mul(c[q], 1.0 / n[q]);
+  std::vector<M6> rigid(up.size());
+  for (size_t k = 0; k < up.size(); k++) {
+    V3 r = sub(centre[k], c[up[k]]);
+    rigid[k] = matrix([&](Six y) { return shift(y, r); });
+  }
+  auto rigid_row = [&](size_t k) { return Row{{up[k], rigid[k]}}; };
+  down = each<Row>(up.size(), [&](size_t k) {
+    Row row = rigid_row(k);
+    for (auto& [cc, m] : ask(k, rigid_row)) merge(row, cc, product(inverse[k], m), -step);
+    return row;
+  });
+  from.assign(count, {});
+  for (size_t k = 0; k < down.size(); k++)
+    for (size_t q = 0; q < down[k].size(); q++) from[down[k][q].first].emplace_back(k, q);
+  const Level& fine = *this;
+  std::vector<Row> kp = each<Row>(up.size(), [&](size_t k) { return fine.ask(k, [&](size_t o) { return fine.down[o]; }); });
+  std::vector<Row> rows_c = each<Row>(count, [&](size_t cc) {
+    Row row;
+    for (auto& [k, q] : fine.from[cc]) {
+      M6 pt = transpose(fine.down[k][q].second);
+      for (auto& [d, m] : kp[k])
+        if (d >= cc) merge(row, d, product(pt, m), 1.0);
+    }
+    return row;
+  });
+  std::vector<M6> own_c(count, M6{});
+  std::vector<Pair> pairs_c;
+  for (size_t cc = 0; cc < count; cc++)
+    for (auto& [d, m] : rows_c[cc]) {
+      if (d == cc)
+        own_c[cc] = m;
+      else
+        pairs_c.push_back(Pair{cc, d, m});
+    }
+  return level_of(std::move(c), std::move(own_c), std::move(pairs_c));
+}
+
+// bodies.rs:293-315
+Cholesky cholesky_of(const Level& level) {
+  size_t parts = level.centre.size();
+  size_t n = 6 * parts;
+  std::vector<double> a(n * n, 0.0);
+  double w = 0.0;
+  for (size_t c = 0; c < n; c++) {
+    std::vector<Six> e(parts, Six{});
+    e[c / 6][c % 6] = 1.0;
+    std::vector<Six> v = level.apply(e, w);
+    for (size_t k = 0; k < parts; k++)
+      for (int s = 0; s < 6; s++) a[(6 * k + s) * n + c] = v[k][s];
+  }
+  for (size_t j = 0; j < n; j++) {
+    double acc = 0.0;
+    for (size_t k = 0; k < j; k++) acc += a[j * n + k] * a[j * n + k];
+    double d = std::sqrt(a[j * n + j] - acc);
+    a[j * n + j] = d;
+    for (size_t i = j + 1; i < n; i++) {
+      double acc2 = 0.0;
+      for (size_t k = 0; k < j; k++) acc2 += a[i * n + k] * a[j * n + k];
+      a[i * n + j] = (a[i * n + j] - acc2) / d;
+    }
+  }
+  Cholesky ch;
+  ch.n = n;
+  ch.l = std::move(a);
+  return ch;
+}
+
+// bodies.rs:317-327
+std::vector<Six> Cholesky::solve(const std::vector<Six>& r) const {
+  std::vector<double> y;
+  y.reserve(n);
+  for (auto& v : r)
+    for (double a : v) y.push_back(a);
+  for (size_t i = 0; i < n; i++) {
+    double acc = 0.0;
+    for (size_t k = 0; k < i; k++) acc += l[i * n + k] * y[k];
+    y[i] = (y[i] - acc) / l[i * n + i];
+  }
+  for (size_t i = n; i-- > 0;) {
+    double acc = 0.0;
+    for (size_t k = i + 1; k < n; k++) acc += l[k * n + i] * y[k];
+    y[i] = (y[i] - acc) / l[i * n + i];
+  }
+  std::vector<Six> out(n / 6);
+  for (size_t k = 0; k < n / 6; k++)
+    for (int s = 0; s < 6; s++) out[k][s] = y[6 * k + s];
+  return out;
+}
+
+size_t find(std::vector<size_t>& p, size_t a) {  // bodies.rs:628-634
+  while (p[a] != a) {
+    p[a] = p[p[a]];
+    a = p[a];
+  }
+  return a;
+}
+
+// bodies.rs:472-496
+size_t bodies_of(const Level& level, const std::vector<int64_t>& part, const std::vector<Site>& key,
+                 std::vector<size_t>& up, std::vector<int64_t>& next) {
+  size_t parts = level.centre.size();
+  std::vector<Site> their(parts, Site{0, 0, 0});
+  for (size_t v = 0; v < part.size(); v++)
+    if (part[v] >= 0) their[part[v]] = key[v];
+  std::vector<size_t> parent(parts);
+  for (size_t k = 0; k < parts; k++) parent[k] = k;
+  for (auto& pr : level.pairs)
+    if (their[pr.i] == their[pr.j]) {
+      size_t a = find(parent, pr.i), c = find(parent, pr.j);
+      parent[a] = c;
+    }
+  std::unordered_map<size_t, size_t> id;
+  up.assign(parts, 0);
+  for (size_t k = 0; k < parts; k++) {
+    size_t n = id.size();
+    auto it = id.emplace(find(parent, k), n).first;
+    up[k] = it->second;
+  }
+  next.assign(part.size(), -1);
+  for (size_t v = 0; v < part.size(); v++)
+    if (part[v] >= 0) next[v] = (int64_t)up[part[v]];
+  return id.size();
+}
+
+// bodies.rs:377-403
+std::vector<Six> cycle(const std::vector<Level>& levels, size_t l, const std::vector<Six>& r, double& work) {
+  const Level& level = levels[l];
+  if (l + 1 == levels.size()) return level.direct.solve(r);
+  auto rest = [&](const std::vector<Six>& x) {
+    std::vector<Six> ax = level.apply(x, work);
+    std::vector<Six> out(r.size());
+    for (size_t k = 0; k < r.size(); k++)
+      for (int s = 0; s < 6; s++) out[k][s] = r[k][s] - ax[k][s];
+    return out;
+  };
+  std::vector<Six> x = level.jacobi(r);
+  for (auto& v : x)
+    for (auto& a : v) a = a * level.step;
+  std::vector<Six> left = rest(x);
+  std::vector<Six> coarse = each<Six>(level.from.size(), [&](size_t c) {
+    Six a{};
+    for (auto& [k, q] : level.from[c]) a = add6v(a, times_transposed(level.down[k][q].second, left[k]));
+    return a;
+  });
+  std::vector<Six> y = cycle(levels, l + 1, coarse, work);
+  std::vector<Six> moved = each<Six>(x.size(), [&](size_t k) {
+    Six a{};
+    for (auto& [c, p] : level.down[k]) a = add6v(a, times(p, y[c]));
+    return a;
+  });
+  for (size_t k = 0; k < x.size(); k++) x[k] = add6v(x[k], moved[k]);
+  left = rest(x);
+  std::vector<Six> d = level.jacobi(left);
+  for (size_t k = 0; k < x.size(); k++)
+    for (int s = 0; s < 6; s++) x[k][s] += level.step * d[k][s];
+  return x;
+}
+
+// bodies.rs:338-373 (tol and the iteration cap are parameters here; the
+// reference hardcodes 1e-6 and 100,000)
+std::vector<Six> conjugate_gradients(const Level& level, const std::vector<Six>& loads,
+                                     const std::function<std::vector<Six>(const std::vector<Six>&)>& pre,
+                                     double tol, int max_iter, double& work, int& iters) {
+  std::vector<Six> x(loads.size(), Six{});
+  std::vector<Six> r = loads;
+  std::vector<Six> z = pre(r);
+  std::vector<Six> p = z;
+  double rz = dot6(r, z);
+  double goal = tol * std::sqrt(dot6(loads, loads));
+  iters = 0;
+  for (int it = 0; it < max_iter; it++) {
+    if (std::sqrt(dot6(r, r)) <= goal) break;
+    std::vector<Six> ap = level.apply(p, work);
+    double alpha = rz / dot6(p, ap);
+    for (size_t k = 0; k < x.size(); k++)
+      for (int s = 0; s < 6; s++) {
+        x[k][s] += alpha * p[k][s];
+        r[k][s] -= alpha * ap[k][s];
+      }
+    z = pre(r);
+    double next = dot6(r, z);
+    for (size_t k = 0; k < x.size(); k++)
+      for (int s = 0; s < 6; s++) p[k][s] = z[k][s] + next / rz * p[k][s];
+    rz = next;
+    iters = it + 1;
+  }
+  return x;
+}
+
+struct MotionStats {
+  int iters = 0;
+  double work = 0.0;
+  double setup_s = 0.0, solve_s = 0.0;
+  int levels = 0;
+  size_t top_parts = 0;
+};
+
+// bodies.rs:429-468
+std::vector<Six> motions(const Lattice& lat, const std::vector<char>& alive, const std::vector<char>& gone,
+                         const std::function<Six(size_t)>& loads, double tol, int max_iter, MotionStats& st) {
+  auto t0 = std::chrono::steady_clock::now();
+  constexpr size_t FEW = 64;
+  const auto& nodes = lat.nodes;
+  const auto& bonds = lat.bonds;
+  std::vector<size_t> live;
+  for (size_t b = 0; b < bonds.size(); b++)
+    if (alive[b] && !gone[bonds[b].i] && !gone[bonds[b].j]) live.push_back(b);
+  std::vector<int64_t> part(nodes.size(), -1);
+  size_t count = 0;
+  for (size_t v = 0; v < nodes.size(); v++)
+    if (!nodes[v].held && !gone[v]) part[v] = (int64_t)count++;
+  std::vector<int64_t> node = part;
+  std::vector<Level> levels;
+  levels.push_back(level_new(lat, part, count, live));
+  std::vector<Site> key(nodes.size());
+  for (size_t v = 0; v < nodes.size(); v++) key[v] = lat.around[v][0];
+  uint32_t l = lat.level;
+  while (count > FEW) {
+    l += 1;
+    for (auto& k : key) k = coarse_site(k, l);
+    std::vector<size_t> up;
+    std::vector<int64_t> next;
+    size_t bodies = bodies_of(levels.back(), part, key, up, next);
+    if (bodies == count) break;
+    Level up_level = levels.back().coarser(up, bodies);
+    levels.push_back(std::move(up_level));
+    part = std::move(next);
+    count = bodies;
+  }
+  Level& top = levels.back();
+  top.direct = cholesky_of(top);
+  top.has_direct = true;
+  std::vector<Six> ld;
+  for (size_t v = 0; v < nodes.size(); v++)
+    if (node[v] >= 0) ld.push_back(loads(v));
+  auto t1 = std::chrono::steady_clock::now();
+  double inner = 0.0;
+  auto vcyc = [&](const std::vector<Six>& r) { return cycle(levels, 0, r, inner); };
+  std::vector<Six> x = conjugate_gradients(levels[0], ld, vcyc, tol, max_iter, st.work, st.iters);
+  st.work += inner;
+  auto t2 = std::chrono::steady_clock::now();
+  st.setup_s = std::chrono::duration<double>(t1 - t0).count();
+  st.solve_s = std::chrono::duration<double>(t2 - t1).count();
+  st.levels = (int)levels.size();
+  st.top_parts = levels.back().centre.size();
+  std::vector<Six> out(nodes.size(), Six{});
+  for (size_t v = 0; v < nodes.size(); v++)
+    if (node[v] >= 0) out[v] = x[node[v]];
+  return out;
+}
+
+// bodies.rs:420-423
+std::vector<V3> moved(const Lattice& lat, const std::vector<Six>& x) {
+  std::vector<V3> f(lat.bonds.size());
+  for (size_t b = 0; b < lat.bonds.size(); b++) {
+    const Bond& bd = lat.bonds[b];
+    Six aj = shift(x[bd.j], sub(bd.p, lat.nodes[bd.j].x));
+    Six ai = shift(x[bd.i], sub(bd.p, lat.nodes[bd.i].x));
+    f[b] = bd.force(sub({aj[0], aj[1], aj[2]}, {ai[0], ai[1], ai[2]}));
+  }
+  return f;
+}
+
+// bodies.rs:407-417
+std::vector<V3> forces(const Lattice& lat, const std::vector<char>& alive, const std::vector<char>& gone, double tol,
+                       MotionStats& st) {
+  auto weight = [&](size_t v) { return Six{0.0, -lat.nodes[v].m * GRAVITY, 0.0, 0.0, 0.0, 0.0}; };
+  std::vector<Six> x = motions(lat, alive, gone, weight, tol, 100000, st);
+  std::vector<V3> f = moved(lat, x);
+  for (size_t b = 0; b < lat.bonds.size(); b++)
+    if (!alive[b] || gone[lat.bonds[b].i] || gone[lat.bonds[b].j]) f[b] = V3{0, 0, 0};
+  return f;
+}
+
+// bodies.rs:544-546
+std::vector<double> used(const Lattice& lat, const std::vector<V3>& f) {
+  std::vector<double> u(lat.bonds.size());
+  for (size_t b = 0; b < lat.bonds.size(); b++) u[b] = lat.bonds[b].used(f[b]);
+  return u;
+}
+
+// bodies.rs:557-574
+std::vector<char> joined_to_held(const std::vector<Node>& nodes, const std::vector<Bond>& bonds,
+                                 const std::vector<char>& alive) {
+  std::vector<std::vector<size_t>> next(nodes.size());
+  for (size_t b = 0; b < bonds.size(); b++)
+    if (alive[b]) {
+      next[bonds[b].i].push_back(bonds[b].j);
+      next[bonds[b].j].push_back(bonds[b].i);
+    }
+  std::vector<char> held(nodes.size());
+  std::vector<size_t> stack;
+  for (size_t v = 0; v < nodes.size(); v++) {
+    held[v] = nodes[v].held;
+    if (held[v]) stack.push_back(v);
+  }
+  while (!stack.empty()) {
+    size_t v = stack.back();
+    stack.pop_back();
+    for (size_t w : next[v])
+      if (!held[w]) {
+        held[w] = 1;
+        stack.push_back(w);
+      }
+  }
+  return held;
+}
+
+struct Outcome {
+  double fallen;
+  std::vector<char> gone;
+  int rounds;
+  double work;
+};
+
+// bodies.rs:511-541
+Outcome settle(const Lattice& lat, double tol) {
+  const auto& nodes = lat.nodes;
+  const auto& bonds = lat.bonds;
+  std::vector<char> alive(bonds.size(), 1), gone(nodes.size(), 0);
+  double work = 0.0;
+  int rounds = 0;
+  for (;;) {
+    rounds += 1;
+    std::vector<char> held = joined_to_held(nodes, bonds, alive);
+    for (size_t v = 0; v < nodes.size(); v++) gone[v] |= !held[v];
+    MotionStats st;
+    std::vector<V3> f = forces(lat, alive, gone, tol, st);
+    work += st.work;
+    std::vector<double> u = used(lat, f);
+    bool broke = false;
+    for (size_t beam0 = 0; beam0 < bonds.size(); beam0 += PER_BEAM) {
+      bool any = false;
+      for (size_t b = beam0; b < beam0 + PER_BEAM && b < bonds.size(); b++)
+        any |= alive[b] && !gone[bonds[b].i] && !gone[bonds[b].j] && u[b] > 1.0;
+      if (any) {
+        for (size_t b = beam0; b < beam0 + PER_BEAM && b < bonds.size(); b++) alive[b] = 0;
+        broke = true;
+      }
+    }
+    if (!broke) break;
+  }
+  double freem = 0.0, fell = 0.0;
+  for (auto& v : nodes)
+    if (!v.held) freem += v.m;
+  for (size_t v = 0; v < nodes.size(); v++)
+    if (gone[v]) fell += nodes[v].m;
+  return Outcome{freem > 0.0 ? fell / freem : 0.0, gone, rounds, work};
+}
+
+// bodies.rs:550-554
+std::vector<double> used_standing(const Lattice& lat, double tol) {
+  std::vector<char> alive(lat.bonds.size(), 1);
+  std::vector<char> held = joined_to_held(lat.nodes, lat.bonds, alive);
+  std::vector<char> gone(held.size());
+  for (size_t v = 0; v < held.size(); v++) gone[v] = !held[v];
+  MotionStats st;
+  return used(lat, forces(lat, alive, gone, tol, st));
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+// Used from Python (ctypes) by tests and by bench.py's CPU-baseline leg.
+
+extern "C" {
+
+struct OrcLattice;
+
+void orc_set_threads(int t) { g_threads = t; }
+int orc_threads() { return threads_now(); }
+
+// solids: n sites (3 int64 each, level-0 units), mats n x 6 (rho, e, g, ft, fc, fs), held n bytes.
+OrcLattice* orc_build(uint32_t level, size_t n, const int64_t* sites, const double* mats, const uint8_t* held) {
+  std::vector<Solid> s(n);
+  for (size_t i = 0; i < n; i++) {
+    s[i].site = {sites[3 * i], sites[3 * i + 1], sites[3 * i + 2]};
+    const double* m = mats + 6 * i;
+    s[i].mat = Mat{m[0], m[1], m[2], m[3], m[4], m[5]};
+    s[i].held = held[i] != 0;
+  }
+  return reinterpret_cast<OrcLattice*>(new Lattice(build(level, s)));
+}
+
+void orc_free(OrcLattice* h) { delete reinterpret_cast<Lattice*>(h); }
+
+void orc_hold_bulk(uint32_t level, size_t n, const int64_t* sites, double radius, uint8_t* held_out) {
+  std::vector<Solid> s(n);
+  for (size_t i = 0; i < n; i++) s[i].site = {sites[3 * i], sites[3 * i + 1], sites[3 * i + 2]};
+  hold_bulk(level, s, radius);
+  for (size_t i = 0; i < n; i++) held_out[i] = s[i].held;
+}
+
+void orc_sizes(const OrcLattice* h, size_t* nodes, size_t* bonds) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  *nodes = L->nodes.size();
+  *bonds = L->bonds.size();
+}
+
+// x: 3 per node, m: 1, held: 1 byte, around: 6 int64 (lesser site, then greater)
+void orc_nodes(const OrcLattice* h, double* x, double* m, uint8_t* held, int64_t* around) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  for (size_t v = 0; v < L->nodes.size(); v++) {
+    for (int k = 0; k < 3; k++) x[3 * v + k] = L->nodes[v].x[k];
+    m[v] = L->nodes[v].m;
+    held[v] = L->nodes[v].held;
+    for (int a = 0; a < 2; a++)
+      for (int k = 0; k < 3; k++) around[6 * v + 3 * a + k] = L->around[v][a][k];
+  }
+}
+
+// ij: 2 per bond, p and n: 3, a, d: 1, e, g: 1, beam_cell: one per beam
+void orc_bonds(const OrcLattice* h, int64_t* ij, double* p, double* n, double* a, double* d, double* e, double* g,
+               int64_t* beam_cell) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  for (size_t b = 0; b < L->bonds.size(); b++) {
+    const Bond& bd = L->bonds[b];
+    ij[2 * b] = (int64_t)bd.i;
+    ij[2 * b + 1] = (int64_t)bd.j;
+    for (int k = 0; k < 3; k++) {
+      p[3 * b + k] = bd.p[k];
+      n[3 * b + k] = bd.n[k];
+    }
+    a[b] = bd.a;
+    d[b] = bd.d;
+    e[b] = bd.mat.e;
+    g[b] = bd.mat.g;
+  }
+  for (size_t q = 0; q < L->beam_cell.size(); q++) beam_cell[q] = (int64_t)L->beam_cell[q];
+}
+
+void orc_joined_to_held(const OrcLattice* h, const uint8_t* alive, uint8_t* out) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  std::vector<char> al(alive, alive + L->bonds.size());
+  auto r = joined_to_held(L->nodes, L->bonds, al);
+  for (size_t v = 0; v < r.size(); v++) out[v] = r[v];
+}
+
+// loads: 6 per node (read for parts only). Returns 0, or 1 if the iteration cap was met.
+int orc_motions(const OrcLattice* h, const uint8_t* alive, const uint8_t* gone, const double* loads, double tol,
+                int max_iter, double* x_out, int* iters, double* work, double* setup_s, double* solve_s,
+                int* levels) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  std::vector<char> al(alive, alive + L->bonds.size()), gn(gone, gone + L->nodes.size());
+  MotionStats st;
+  auto ld = [&](size_t v) {
+    Six s;
+    for (int k = 0; k < 6; k++) s[k] = loads[6 * v + k];
+    return s;
+  };
+  auto x = motions(*L, al, gn, ld, tol, max_iter, st);
+  for (size_t v = 0; v < x.size(); v++)
+    for (int k = 0; k < 6; k++) x_out[6 * v + k] = x[v][k];
+  if (iters) *iters = st.iters;
+  if (work) *work = st.work;
+  if (setup_s) *setup_s = st.setup_s;
+  if (solve_s) *solve_s = st.solve_s;
+  if (levels) *levels = st.levels;
+  return st.iters >= max_iter ? 1 : 0;
+}
+
+// Level 0's stiffness applied to x (6 per node, zero rows for held and gone nodes), as Level::apply.
+void orc_apply(const OrcLattice* h, const uint8_t* alive, const uint8_t* gone, const double* x, double* y) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  std::vector<size_t> live;
+  for (size_t b = 0; b < L->bonds.size(); b++)
+    if (alive[b] && !gone[L->bonds[b].i] && !gone[L->bonds[b].j]) live.push_back(b);
+  std::vector<int64_t> part(L->nodes.size(), -1);
+  size_t count = 0;
+  for (size_t v = 0; v < L->nodes.size(); v++)
+    if (!L->nodes[v].held && !gone[v]) part[v] = (int64_t)count++;
+  Level lv = level_new(*L, part, count, live);
+  std::vector<Six> xs(count);
+  for (size_t v = 0; v < L->nodes.size(); v++)
+    if (part[v] >= 0)
+      for (int k = 0; k < 6; k++) xs[part[v]][k] = x[6 * v + k];
+  double w = 0.0;
+  auto ys = lv.apply(xs, w);
+  for (size_t v = 0; v < L->nodes.size(); v++)
+    for (int k = 0; k < 6; k++) y[6 * v + k] = part[v] >= 0 ? ys[part[v]][k] : 0.0;
+}
+
+void orc_moved(const OrcLattice* h, const double* x, double* f_out) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  std::vector<Six> xs(L->nodes.size());
+  for (size_t v = 0; v < xs.size(); v++)
+    for (int k = 0; k < 6; k++) xs[v][k] = x[6 * v + k];
+  auto f = moved(*L, xs);
+  for (size_t b = 0; b < f.size(); b++)
+    for (int k = 0; k < 3; k++) f_out[3 * b + k] = f[b][k];
+}
+
+void orc_forces(const OrcLattice* h, const uint8_t* alive, const uint8_t* gone, double tol, double* f_out) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  std::vector<char> al(alive, alive + L->bonds.size()), gn(gone, gone + L->nodes.size());
+  MotionStats st;
+  auto f = forces(*L, al, gn, tol, st);
+  for (size_t b = 0; b < f.size(); b++)
+    for (int k = 0; k < 3; k++) f_out[3 * b + k] = f[b][k];
+}
+
+// Bond::used with the bond's own material; mats_used lets a test give the strengths (ft, fc, fs) per bond.
+void orc_used(const OrcLattice* h, const double* f, const double* strength, double* u_out) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  for (size_t b = 0; b < L->bonds.size(); b++) {
+    Bond bd = L->bonds[b];
+    if (strength) {
+      bd.mat.ft = strength[3 * b];
+      bd.mat.fc = strength[3 * b + 1];
+      bd.mat.fs = strength[3 * b + 2];
+    }
+    u_out[b] = bd.used({f[3 * b], f[3 * b + 1], f[3 * b + 2]});
+  }
+}
+
+void orc_used_standing(const OrcLattice* h, double tol, double* u_out) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  auto u = used_standing(*L, tol);
+  for (size_t b = 0; b < u.size(); b++) u_out[b] = u[b];
+}
+
+void orc_settle(const OrcLattice* h, double tol, uint8_t* gone_out, double* fallen, int* rounds, double* work) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  Outcome o = settle(*L, tol);
+  for (size_t v = 0; v < o.gone.size(); v++) gone_out[v] = o.gone[v];
+  *fallen = o.fallen;
+  *rounds = o.rounds;
+  *work = o.work;
+}
+
+// Scales the stiffness (e and g) of every bond of cell `cell` by `by` (a log-stiffness step of log(by)).
+void orc_scale_cell(OrcLattice* h, int64_t cell, double by) {
+  auto* L = reinterpret_cast<Lattice*>(h);
+  for (size_t q = 0; q < L->beam_cell.size(); q++)
+    if ((int64_t)L->beam_cell[q] == cell)
+      for (size_t b = q * PER_BEAM; b < (q + 1) * PER_BEAM; b++) {
+        L->bonds[b].mat.e *= by;
+        L->bonds[b].mat.g *= by;
+      }
+}
+
+// Test hooks for the small dense routines.
+void orc_invert6(const double* m, double* out) {
+  M6 a;
+  for (int r = 0; r < 6; r++)
+    for (int c = 0; c < 6; c++) a[r][c] = m[6 * r + c];
+  M6 inv = invert(a);
+  for (int r = 0; r < 6; r++)
+    for (int c = 0; c < 6; c++) out[6 * r + c] = inv[r][c];
+}
+
+void orc_coarse_site(const int64_t* p, uint32_t l, int64_t* out) {
+  Site s = coarse_site({p[0], p[1], p[2]}, l);
+  for (int k = 0; k < 3; k++) out[k] = s[k];
+}
+
+double orc_spacing(uint32_t level) { return spacing(level); }
+}

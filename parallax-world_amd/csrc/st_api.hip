@@ -1,0 +1,481 @@
+// Entry points of the C ABI for the solve, its batches, classification, bond
+// forces and the gradients.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+
+#include "st_solver.hpp"
+
+namespace st {
+namespace {
+
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)std::max<int64_t>(1, (n + b - 1) / b); }
+
+// e1, e2 of pair q as in lattice.rs:78-80, from the unit direction n.
+__device__ __forceinline__ void pair_frame(const double* n, double* e1, double* e2) {
+  double ref[3] = {0, 0, 0};
+  if (fabs(n[0]) < 0.9)
+    ref[0] = 1.0;
+  else
+    ref[1] = 1.0;
+  double c[3] = {n[1] * ref[2] - n[2] * ref[1], n[2] * ref[0] - n[0] * ref[2], n[0] * ref[1] - n[1] * ref[0]};
+  const double inv = 1.0 / sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  for (int k = 0; k < 3; k++) e1[k] = c[k] * inv;
+  e2[0] = n[1] * e1[2] - n[2] * e1[1];
+  e2[1] = n[2] * e1[0] - n[0] * e1[2];
+  e2[2] = n[0] * e1[1] - n[1] * e1[0];
+}
+
+struct BeamState {
+  double n[3], hd, K4, R2, k;  // k = a/d per bond
+  double r;                    // offset radius sqrt(A / 2 pi)
+};
+__device__ __forceinline__ void beam_state(const BeamGeo& G, int q, BeamState& b) {
+  const int cls = pair_opposite(q) ? 0 : 1;
+  for (int k = 0; k < 3; k++) b.n[k] = G.nscale[cls] * (double)pair_dir(q, k);
+  b.hd = G.hd[cls];
+  b.K4 = G.K4[cls];
+  b.R2 = G.R2[cls];
+  b.k = 0.25 * G.K4[cls];
+  // A = K4 * d ; r = sqrt(A / (2 pi))
+  b.r = sqrt(G.K4[cls] * 2.0 * G.hd[cls] / (2.0 * kPi));
+}
+
+// middle relative motion delta and dw of a beam between end motions xi, xj
+__device__ __forceinline__ void beam_delta(const BeamState& b, const double* xi, const double* xj, double* d,
+                                           double* dw) {
+  const double w[3] = {xi[3] + xj[3], xi[4] + xj[4], xi[5] + xj[5]};
+  d[0] = xj[0] - xi[0] + b.hd * (b.n[1] * w[2] - b.n[2] * w[1]);
+  d[1] = xj[1] - xi[1] + b.hd * (b.n[2] * w[0] - b.n[0] * w[2]);
+  d[2] = xj[2] - xi[2] + b.hd * (b.n[0] * w[1] - b.n[1] * w[0]);
+  for (int k = 0; k < 3; k++) dw[k] = xj[3 + k] - xi[3 + k];
+}
+
+__device__ __forceinline__ void load_faces(const uint32_t* cfaces, int64_t c, const double* x, double (*xv)[6]) {
+  for (int f = 0; f < 6; f++) {
+    const uint32_t v = cfaces[6 * c + f];
+    for (int k = 0; k < 6; k++) xv[f][k] = (v & kNotFree) ? 0.0 : x[6 * (size_t)v + k];
+  }
+}
+
+// Per bond force on the j side (statics/src/bodies.rs:420-423 with model.rs:115-119):
+// out[c][q][k][3], zero for dead beams.
+__global__ void k_bond_forces(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, const double* ce,
+                              const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* x,
+                              double* out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  double xv[6][6];
+  load_faces(cfaces, c, x, xv);
+  const BeamGeo G = bgeo[cstruct[c]];
+  const double e = ce[c], g = cg[c];
+  const uint32_t m = cmask[c];
+  for (int q = 0; q < 15; q++) {
+    double* o = out + ((size_t)c * 15 + q) * 12;
+    if (!(m & (1u << q))) {
+      for (int i = 0; i < 12; i++) o[i] = 0.0;
+      continue;
+    }
+    BeamState b;
+    beam_state(G, q, b);
+    double d[3], dw[3], e1[3], e2[3];
+    beam_delta(b, xv[kPairA[q]], xv[kPairB[q]], d, dw);
+    pair_frame(b.n, e1, e2);
+    for (int k = 0; k < 4; k++) {
+      const double* ek = k < 2 ? e1 : e2;
+      const double sg = (k & 1) ? -b.r : b.r;
+      const double off[3] = {sg * ek[0], sg * ek[1], sg * ek[2]};
+      const double dl[3] = {d[0] + dw[1] * off[2] - dw[2] * off[1], d[1] + dw[2] * off[0] - dw[0] * off[2],
+                            d[2] + dw[0] * off[1] - dw[1] * off[0]};
+      const double along = dl[0] * b.n[0] + dl[1] * b.n[1] + dl[2] * b.n[2];
+      for (int t = 0; t < 3; t++) o[3 * k + t] = b.n[t] * (e * b.k * along) + (dl[t] - b.n[t] * along) * (g * b.k);
+    }
+  }
+}
+
+// sigma of bond k of beam q: (e/d)(n.delta + dw.(o_k x n))
+__device__ __forceinline__ double bond_sigma(const BeamState& b, double e, const double* d, const double* dw,
+                                             const double* ox) {
+  return (e / (2.0 * b.hd)) * (b.n[0] * d[0] + b.n[1] * d[1] + b.n[2] * d[2] + dw[0] * ox[0] + dw[1] * ox[1] +
+                               dw[2] * ox[2]);
+}
+__device__ __forceinline__ void bond_oxn(const BeamState& b, int k, double* ox) {
+  double e1[3], e2[3];
+  pair_frame(b.n, e1, e2);
+  const double* ek = k < 2 ? e1 : e2;
+  const double sg = (k & 1) ? -b.r : b.r;
+  const double o[3] = {sg * ek[0], sg * ek[1], sg * ek[2]};
+  ox[0] = o[1] * b.n[2] - o[2] * b.n[1];
+  ox[1] = o[2] * b.n[0] - o[0] * b.n[2];
+  ox[2] = o[0] * b.n[1] - o[1] * b.n[0];
+}
+
+// pass 0: per-cell max sigma; pass 1: per-cell sum exp(beta (sigma - m))
+__global__ void k_softmax_stats(int pass, int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask,
+                                const double* ce, const uint32_t* cstruct, const BeamGeo* bgeo, const double* x,
+                                double beta, const double* mref, double* out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  double xv[6][6];
+  load_faces(cfaces, c, x, xv);
+  const BeamGeo G = bgeo[cstruct[c]];
+  const double e = ce[c];
+  const uint32_t m = cmask[c];
+  double acc = pass == 0 ? -INFINITY : 0.0;
+  for (int q = 0; q < 15; q++) {
+    if (!(m & (1u << q))) continue;
+    BeamState b;
+    beam_state(G, q, b);
+    double d[3], dw[3];
+    beam_delta(b, xv[kPairA[q]], xv[kPairB[q]], d, dw);
+    for (int k = 0; k < 4; k++) {
+      double ox[3];
+      bond_oxn(b, k, ox);
+      const double s = bond_sigma(b, e, d, dw, ox);
+      if (pass == 0)
+        acc = fmax(acc, s);
+      else
+        acc += exp(beta * (s - mref[0]));
+    }
+  }
+  out[c] = acc;
+}
+
+// d J / d x for the soft-max: per bond weight w = exp(beta(sigma - m)) / Z, scattered to the
+// bond's end nodes (global FP64 atomics; run once per backward).
+__global__ void k_softmax_rhs(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, const double* ce,
+                              const uint32_t* cstruct, const BeamGeo* bgeo, const double* x, double beta,
+                              const double* mz, double* g) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  double xv[6][6];
+  load_faces(cfaces, c, x, xv);
+  const BeamGeo G = bgeo[cstruct[c]];
+  const double e = ce[c];
+  const uint32_t m = cmask[c];
+  const double mm = mz[0], Z = mz[1];
+  for (int q = 0; q < 15; q++) {
+    if (!(m & (1u << q))) continue;
+    BeamState b;
+    beam_state(G, q, b);
+    double d[3], dw[3];
+    beam_delta(b, xv[kPairA[q]], xv[kPairB[q]], d, dw);
+    double gu[3] = {0, 0, 0}, gw[3] = {0, 0, 0};
+    for (int k = 0; k < 4; k++) {
+      double ox[3];
+      bond_oxn(b, k, ox);
+      const double w = exp(beta * (bond_sigma(b, e, d, dw, ox) - mm)) / Z;
+      const double sc = w * e / (2.0 * b.hd);
+      for (int t = 0; t < 3; t++) {
+        gu[t] += sc * b.n[t];
+        gw[t] += sc * ox[t];
+      }
+    }
+    const uint32_t vi = cfaces[6 * c + kPairA[q]], vj = cfaces[6 * c + kPairB[q]];
+    if (!(vj & kNotFree))
+      for (int t = 0; t < 3; t++) {
+        atomicAdd(&g[6 * (size_t)vj + t], gu[t]);
+        atomicAdd(&g[6 * (size_t)vj + 3 + t], gw[t]);
+      }
+    if (!(vi & kNotFree))
+      for (int t = 0; t < 3; t++) {
+        atomicAdd(&g[6 * (size_t)vi + t], -gu[t]);
+        atomicAdd(&g[6 * (size_t)vi + 3 + t], -gw[t]);
+      }
+  }
+}
+
+// Per-cell sensitivity: -lambda_c^T K_c x_c (+ the soft-max's explicit sum w sigma when beta > 0).
+__global__ void k_sensitivity(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, const double* ce,
+                              const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* x,
+                              const double* lam, int softmax, double beta, const double* mz, double* out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  double xv[6][6], lv[6][6];
+  load_faces(cfaces, c, x, xv);
+  load_faces(cfaces, c, lam, lv);
+  const BeamGeo G = bgeo[cstruct[c]];
+  const double e = ce[c], g = cg[c];
+  const uint32_t m = cmask[c];
+  double acc = 0.0;
+  for (int q = 0; q < 15; q++) {
+    if (!(m & (1u << q))) continue;
+    BeamState b;
+    beam_state(G, q, b);
+    double d[3], dw[3], dl[3], dwl[3];
+    beam_delta(b, xv[kPairA[q]], xv[kPairB[q]], d, dw);
+    beam_delta(b, lv[kPairA[q]], lv[kPairB[q]], dl, dwl);
+    const double nd = b.n[0] * d[0] + b.n[1] * d[1] + b.n[2] * d[2];
+    const double nq = b.n[0] * dw[0] + b.n[1] * dw[1] + b.n[2] * dw[2];
+    double F[3], M[3];
+    for (int t = 0; t < 3; t++) {
+      F[t] = b.K4 * (g * d[t] + (e - g) * nd * b.n[t]);
+      M[t] = b.R2 * (e * dw[t] + (g + g - e) * nq * b.n[t]);
+    }
+    acc -= dl[0] * F[0] + dl[1] * F[1] + dl[2] * F[2] + dwl[0] * M[0] + dwl[1] * M[1] + dwl[2] * M[2];
+    if (softmax)
+      for (int k = 0; k < 4; k++) {
+        double ox[3];
+        bond_oxn(b, k, ox);
+        const double s = bond_sigma(b, e, d, dw, ox);
+        acc += exp(beta * (s - mz[0])) / mz[1] * s;
+      }
+  }
+  out[c] = acc;
+}
+
+__global__ void k_gravity(int64_t nnodes, const int32_t* ncell, const uint8_t* nflags, double* f) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nnodes) return;
+  const int cnt = (ncell[2 * v] >= 0) + (ncell[2 * v + 1] >= 0);
+  for (int k = 0; k < 6; k++) f[6 * v + k] = 0.0;
+  if (!(nflags[v] & (kHeld | kGone))) f[6 * v + 2] = -(double)cnt / 6.0;
+}
+
+__global__ void k_mask_rows2(int64_t n, const uint8_t* nflags, double* v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (nflags[i] & (kHeld | kGone))
+    for (int k = 0; k < 6; k++) v[6 * i + k] = 0.0;
+}
+
+void check_opts(const StSolveOpts* o) {
+  if (!o) throw StError{ST_BAD_INPUT, "opts is NULL"};
+  if (o->precision != ST_FP64 && o->precision != ST_MIXED) throw StError{ST_BAD_INPUT, "opts.precision"};
+  if (!(o->tol >= 0) || o->max_iter < 0 || o->refresh < 0) throw StError{ST_BAD_INPUT, "opts: tol, max_iter or refresh"};
+}
+
+int worst(const std::vector<StStats>& s) {
+  for (auto& t : s)
+    if (t.status != ST_OK) return t.status;
+  return ST_OK;
+}
+
+// Host reduction of per-cell partials (fixed order).
+double host_sum(const std::vector<double>& v) {
+  double s = 0;
+  for (double a : v) s += a;
+  return s;
+}
+
+}  // namespace
+}  // namespace st
+
+using namespace st;
+
+extern "C" {
+
+int st_solve(StCtx* c, const double* loads, const StSolveOpts* opts, double* x_out, StStats* stats) {
+  std::vector<StStats> st_out;
+  int rc = guard(c, [&] {
+    need_upload(c);
+    check_opts(opts);
+    if (!loads || !x_out) throw StError{ST_BAD_INPUT, "solve: loads or x_out is NULL"};
+    ST_HIP(hipSetDevice(c->device));
+    const int64_t nn = c->geo.nnodes;
+    const size_t bytes = (size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double);
+    c->f64.ensure(bytes);
+    c->x64.ensure(bytes);
+    if (nn) {
+      ST_HIP(hipMemcpyAsync(c->f64.p, loads, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->f64.as<double>());
+      if (opts->warm_start) {
+        ST_HIP(hipMemcpyAsync(c->x64.p, x_out, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->x64.as<double>());
+      } else {
+        ST_HIP(hipMemsetAsync(c->x64.p, 0, nn * 6 * sizeof(double), c->stream));
+      }
+    }
+    solve_all(*c, *opts, st_out);
+    if (nn) ST_HIP(hipMemcpyAsync(x_out, c->x64.p, nn * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    c->last_x.assign(x_out, x_out + nn * 6);
+    c->last_loads.assign(loads, loads + nn * 6);
+    c->last_opts = *opts;
+    c->have_solution = true;
+    if (stats) *stats = st_out.empty() ? StStats{} : st_out[0];
+  });
+  if (rc == ST_OK && !st_out.empty()) rc = st_out[0].status;
+  return rc;
+}
+
+int st_solve_batch(StCtx* c, int32_t n, const StCells* const* cells, const double* const* loads,
+                   const StSolveOpts* opts, double* const* x_out, StStats* stats) {
+  std::vector<StStats> st_out;
+  int rc = guard(c, [&] {
+    check_opts(opts);
+    if (n <= 0 || !cells) throw StError{ST_BAD_INPUT, "batch: n <= 0 or cells is NULL"};
+    for (int i = 0; i < n; i++)
+      if (!cells[i]) throw StError{ST_BAD_INPUT, "batch: a cells entry is NULL"};
+    ST_HIP(hipSetDevice(c->device));
+    build_host_geo(c->host, cells, n);
+    copy_geo_to_device(*c);
+    c->gone_host.assign(c->geo.nnodes, 0);
+    refresh_cell_flags(*c);
+    c->uploaded = true;
+    const int64_t nn = c->geo.nnodes;
+    const size_t bytes = (size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double);
+    c->f64.ensure(bytes);
+    c->x64.ensure(bytes);
+    if (nn) {
+      k_gravity<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.ncell.as<int32_t>(), c->geo.nflags.as<uint8_t>(),
+                                                 c->f64.as<double>());
+      for (int i = 0; i < n; i++)
+        if (loads && loads[i]) {
+          const StructInfo& si = c->host.structs[i];
+          ST_HIP(hipMemcpyAsync(c->f64.as<double>() + 6 * si.node0, loads[i], si.nnodes * 6 * sizeof(double),
+                                hipMemcpyHostToDevice, c->stream));
+        }
+      k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->f64.as<double>());
+      ST_HIP(hipMemsetAsync(c->x64.p, 0, nn * 6 * sizeof(double), c->stream));
+    }
+    solve_all(*c, *opts, st_out);
+    for (int i = 0; i < n; i++)
+      if (x_out && x_out[i]) {
+        const StructInfo& si = c->host.structs[i];
+        ST_HIP(hipMemcpyAsync(x_out[i], c->x64.as<double>() + 6 * si.node0, si.nnodes * 6 * sizeof(double),
+                              hipMemcpyDeviceToHost, c->stream));
+      }
+    ST_HIP(hipStreamSynchronize(c->stream));
+    c->have_solution = false;
+    if (stats)
+      for (int i = 0; i < n; i++) stats[i] = st_out[i];
+  });
+  if (rc == ST_OK) rc = worst(st_out);
+  return rc;
+}
+
+int st_joined_to_held(StCtx* c, const uint16_t* beams, uint8_t* joined_out, int keep_gone) {
+  return guard(c, [&] {
+    need_upload(c);
+    ST_HIP(hipSetDevice(c->device));
+    if (beams) set_beam_masks(*c, beams);
+    std::vector<uint8_t> j;
+    joined_to_held(*c, j);
+    if (joined_out) std::memcpy(joined_out, j.data(), j.size());
+    if (!keep_gone) {
+      std::vector<uint8_t> gone(j.size());
+      for (size_t v = 0; v < j.size(); v++) gone[v] = !j[v];
+      set_gone_flags(*c, gone.data());
+    }
+  });
+}
+
+int st_bond_forces(StCtx* c, const double* x, double* out) {
+  return guard(c, [&] {
+    need_upload(c);
+    if (!x || !out) throw StError{ST_BAD_INPUT, "bond forces: x or out is NULL"};
+    ST_HIP(hipSetDevice(c->device));
+    const int64_t nn = c->geo.nnodes, ncl = c->geo.ncells;
+    DBuf xd, od;
+    xd.ensure(std::max<int64_t>(nn, 1) * 6 * sizeof(double));
+    od.ensure(std::max<int64_t>(ncl, 1) * 180 * sizeof(double));
+    if (nn) ST_HIP(hipMemcpyAsync(xd.p, x, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (ncl)
+      k_bond_forces<<<nblk(ncl), 128, 0, c->stream>>>(ncl, c->geo.cfaces.as<uint32_t>(), c->geo.cmask.as<uint16_t>(),
+                                                      c->geo.ce64.as<double>(), c->geo.cg64.as<double>(),
+                                                      c->geo.cstruct.as<uint32_t>(), c->geo.bgeo.as<BeamGeo>(),
+                                                      xd.as<double>(), od.as<double>());
+    ST_HIP(hipGetLastError());
+    std::vector<double> h((size_t)ncl * 180);
+    if (ncl) ST_HIP(hipMemcpyAsync(h.data(), od.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    xd.release();
+    od.release();
+    // upload order: one 180-block per input cell; held cells get zeros
+    const auto& coi = c->host.cell_of_input;
+    for (size_t i = 0; i < coi.size(); i++) {
+      if (coi[i] < 0)
+        std::memset(out + 180 * i, 0, 180 * sizeof(double));
+      else
+        std::memcpy(out + 180 * i, h.data() + 180 * (size_t)coi[i], 180 * sizeof(double));
+    }
+  });
+}
+
+int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, double* J_out, double* dJ,
+                StStats* stats) {
+  std::vector<StStats> st_out;
+  int rc = guard(c, [&] {
+    need_upload(c);
+    check_opts(opts);
+    if (!c->have_solution) throw StError{ST_BAD_INPUT, "backward: no solve to differentiate (call st_solve first)"};
+    if (objective != ST_COMPLIANCE && objective != ST_SOFTMAX_STRESS) throw StError{ST_BAD_INPUT, "objective"};
+    if (objective == ST_SOFTMAX_STRESS && !(beta > 0)) throw StError{ST_BAD_INPUT, "beta must be > 0"};
+    ST_HIP(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    DevGeo& g = c->geo;
+    const int64_t nn = g.nnodes, ncl = g.ncells;
+    const size_t bytes = (size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double);
+    DBuf xd, lam, cellv, mz;
+    xd.ensure(bytes);
+    lam.ensure(bytes);
+    cellv.ensure(std::max<int64_t>(ncl, 1) * sizeof(double));
+    mz.ensure(2 * sizeof(double));
+    if (nn) ST_HIP(hipMemcpyAsync(xd.p, c->last_x.data(), nn * 6 * sizeof(double), hipMemcpyHostToDevice, s));
+    double J = 0.0, mzh[2] = {0.0, 1.0};
+    std::vector<double> hc(ncl);
+    const bool soft = objective == ST_SOFTMAX_STRESS;
+    if (!soft) {
+      // compliance f.x; self-adjoint: lambda = x
+      for (int64_t i = 0; i < nn * 6; i++) J += c->last_loads[i] * c->last_x[i];
+      if (nn) ST_HIP(hipMemcpyAsync(lam.p, xd.p, nn * 6 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    } else {
+      // J = m + log(sum exp(beta (sigma - m))) / beta, m the greatest sigma
+      if (ncl) {
+        k_softmax_stats<<<nblk(ncl), 128, 0, s>>>(0, ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(),
+                                                  g.ce64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
+                                                  xd.as<double>(), beta, nullptr, cellv.as<double>());
+        ST_HIP(hipMemcpyAsync(hc.data(), cellv.p, ncl * sizeof(double), hipMemcpyDeviceToHost, s));
+        ST_HIP(hipStreamSynchronize(s));
+      }
+      double m = -INFINITY;
+      for (double v : hc) m = std::max(m, v);
+      if (!std::isfinite(m)) throw StError{ST_BAD_INPUT, "backward: no live bond"};
+      mzh[0] = m;
+      ST_HIP(hipMemcpyAsync(mz.p, mzh, 2 * sizeof(double), hipMemcpyHostToDevice, s));
+      k_softmax_stats<<<nblk(ncl), 128, 0, s>>>(1, ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(),
+                                                g.ce64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
+                                                xd.as<double>(), beta, mz.as<double>(), cellv.as<double>());
+      ST_HIP(hipMemcpyAsync(hc.data(), cellv.p, ncl * sizeof(double), hipMemcpyDeviceToHost, s));
+      ST_HIP(hipStreamSynchronize(s));
+      mzh[1] = host_sum(hc);
+      J = m + std::log(mzh[1]) / beta;
+      ST_HIP(hipMemcpyAsync(mz.p, mzh, 2 * sizeof(double), hipMemcpyHostToDevice, s));
+      // adjoint: K lambda = dJ/dx, solved with the same operator and a rebuilt hierarchy
+      c->f64.ensure(bytes);
+      c->x64.ensure(bytes);
+      ST_HIP(hipMemsetAsync(c->f64.p, 0, bytes, s));
+      k_softmax_rhs<<<nblk(ncl), 128, 0, s>>>(ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(), g.ce64.as<double>(),
+                                              g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), xd.as<double>(), beta,
+                                              mz.as<double>(), c->f64.as<double>());
+      k_mask_rows2<<<nblk(nn), 256, 0, s>>>(nn, g.nflags.as<uint8_t>(), c->f64.as<double>());
+      ST_HIP(hipMemsetAsync(c->x64.p, 0, bytes, s));
+      solve_all(*c, *opts, st_out);
+      ST_HIP(hipMemcpyAsync(lam.p, c->x64.p, bytes, hipMemcpyDeviceToDevice, s));
+    }
+    if (ncl)
+      k_sensitivity<<<nblk(ncl), 128, 0, s>>>(ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(), g.ce64.as<double>(),
+                                              g.cg64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
+                                              xd.as<double>(), lam.as<double>(), soft ? 1 : 0, beta, mz.as<double>(),
+                                              cellv.as<double>());
+    ST_HIP(hipGetLastError());
+    if (ncl) ST_HIP(hipMemcpyAsync(hc.data(), cellv.p, ncl * sizeof(double), hipMemcpyDeviceToHost, s));
+    ST_HIP(hipStreamSynchronize(s));
+    xd.release();
+    lam.release();
+    cellv.release();
+    mz.release();
+    if (J_out) *J_out = J;
+    if (dJ) {
+      const auto& coi = c->host.cell_of_input;
+      for (size_t i = 0; i < coi.size(); i++) dJ[i] = coi[i] < 0 ? 0.0 : hc[coi[i]];
+    }
+    if (stats) *stats = st_out.empty() ? StStats{} : st_out[0];
+  });
+  if (rc == ST_OK && !st_out.empty()) rc = st_out[0].status;
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,269 @@
+// Context, upload and the operator entry points of the C ABI.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+#include "st_kernels.hpp"
+#include "st_solver.hpp"
+
+namespace st {
+
+namespace {
+
+template <class V>
+void up(DBuf& d, const std::vector<V>& h, hipStream_t s) {
+  d.ensure(std::max<size_t>(h.size(), 1) * sizeof(V));
+  if (!h.empty()) ST_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(V), hipMemcpyHostToDevice, s));
+}
+
+// Marks faces that are not free (held or gone) in each cell's face list, and
+// drops beams touching a gone node (statics/src/bodies.rs:434).
+__global__ void k_cell_flags(int64_t ncells, uint32_t* cfaces, const uint8_t* nflags, const uint16_t* mask_user,
+                             uint16_t* mask) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  uint32_t m = mask_user[c];
+#pragma unroll
+  for (int f = 0; f < 6; f++) {
+    uint32_t v = cfaces[6 * c + f] & ~kNotFree;
+    uint8_t fl = nflags[v];
+    cfaces[6 * c + f] = v | ((fl & (kHeld | kGone)) ? kNotFree : 0u);
+    if (fl & kGone) m &= ~face_beams(f);
+  }
+  mask[c] = (uint16_t)m;
+}
+
+__global__ void k_to_float(int64_t n, const double* a, float* b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+}  // namespace
+
+void copy_geo_to_device(Ctx& c) {
+  HostGeo& h = c.host;
+  DevGeo& d = c.geo;
+  hipStream_t s = c.stream;
+  d.nbricks = (int64_t)h.cbase.size() - 1;
+  d.ncells = (int64_t)h.ce.size();
+  d.nnodes = (int64_t)h.nflags.size();
+  d.nhalo = (int64_t)h.hcell.size();
+  d.nstruct = (int32_t)h.structs.size();
+  d.max_brick_nodes = h.max_brick_nodes;
+  up(d.cbase, h.cbase, s);
+  up(d.nbase, h.nbase, s);
+  up(d.hbase, h.hbase, s);
+  up(d.bstruct, h.bstruct, s);
+  up(d.hcell, h.hcell, s);
+  up(d.ho, h.ho, s);
+  up(d.ce64, h.ce, s);
+  up(d.cg64, h.cg, s);
+  up(d.cmask_user, h.cmask, s);
+  up(d.cmask, h.cmask, s);
+  up(d.cfaces, h.cfaces, s);
+  up(d.ccoord, h.ccoord, s);
+  up(d.cstruct, h.cstruct, s);
+  up(d.nflags, h.nflags, s);
+  up(d.ncell, h.ncell, s);
+  up(d.nkind, h.nkind, s);
+  up(d.nkey, h.nkey, s);
+  std::vector<BeamGeo> bg;
+  for (auto& si : h.structs) bg.push_back(beam_geo(si.spacing));
+  up(d.bgeo, bg, s);
+  d.ce32.ensure(std::max<int64_t>(d.ncells, 1) * sizeof(float));
+  d.cg32.ensure(std::max<int64_t>(d.ncells, 1) * sizeof(float));
+  if (d.ncells) {
+    const int nb = (int)((d.ncells + 255) / 256);
+    k_to_float<<<nb, 256, 0, s>>>(d.ncells, d.ce64.as<double>(), d.ce32.as<float>());
+    k_to_float<<<nb, 256, 0, s>>>(d.ncells, d.cg64.as<double>(), d.cg32.as<float>());
+    ST_HIP(hipGetLastError());
+  }
+}
+
+// Applies the node flags (held, gone) to the cells' face lists and live masks.
+void refresh_cell_flags(Ctx& c) {
+  DevGeo& d = c.geo;
+  if (!d.ncells) return;
+  k_cell_flags<<<(unsigned)((d.ncells + 255) / 256), 256, 0, c.stream>>>(d.ncells, d.cfaces.as<uint32_t>(), d.nflags.as<uint8_t>(),
+                                                                   d.cmask_user.as<uint16_t>(), d.cmask.as<uint16_t>());
+  ST_HIP(hipGetLastError());
+}
+
+template <class T>
+GeoD<T> geo_view(Ctx& c) {
+  DevGeo& d = c.geo;
+  GeoD<T> g;
+  g.cbase = d.cbase.as<uint32_t>();
+  g.nbase = d.nbase.as<uint32_t>();
+  g.hbase = d.hbase.as<uint32_t>();
+  g.hcell = d.hcell.as<uint32_t>();
+  g.bstruct = d.bstruct.as<uint32_t>();
+  g.cfaces = d.cfaces.as<uint32_t>();
+  g.ho = d.ho.as<uint8_t>();
+  g.ce = std::is_same<T, double>::value ? (const T*)d.ce64.p : (const T*)d.ce32.p;
+  g.cg = std::is_same<T, double>::value ? (const T*)d.cg64.p : (const T*)d.cg32.p;
+  g.cmask = d.cmask.as<uint16_t>();
+  g.bgeo = d.bgeo.as<BeamGeo>();
+  return g;
+}
+template GeoD<double> geo_view<double>(Ctx&);
+template GeoD<float> geo_view<float>(Ctx&);
+
+size_t apply_smem(const Ctx& c, size_t tbytes) { return std::max<size_t>(16, (size_t)c.geo.max_brick_nodes * 6 * tbytes); }
+
+template <class T, class Epi>
+void launch_apply(Ctx& c, const T* x, const Epi& epi) {
+  if (!c.geo.nbricks) return;
+  const size_t smem = apply_smem(c, sizeof(T));
+  static size_t attr_bytes = 0;  // one per instantiation
+  if (smem > attr_bytes) {
+    ST_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply<T, Epi>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)smem));
+    attr_bytes = smem;
+  }
+  k_apply<T, Epi><<<(unsigned)c.geo.nbricks, kBlock, smem, c.stream>>>(geo_view<T>(c), x, epi);
+  ST_HIP(hipGetLastError());
+}
+
+// Explicit instantiations used by the solver.
+#define ST_INST_APPLY(T, E) template void launch_apply<T, E>(Ctx&, const T*, const E&);
+ST_INST_APPLY(double, EpiStore<double>)
+ST_INST_APPLY(float, EpiStore<float>)
+ST_INST_APPLY(double, EpiDot<double>)
+ST_INST_APPLY(float, EpiDot<float>)
+ST_INST_APPLY(double, SINGLE_ARG(EpiResid<double, double>))
+ST_INST_APPLY(float, SINGLE_ARG(EpiResid<float, float>))
+ST_INST_APPLY(float, SINGLE_ARG(EpiResid<float, double>))
+ST_INST_APPLY(double, EpiSmooth<double>)
+ST_INST_APPLY(float, EpiSmooth<float>)
+
+}  // namespace st
+
+using namespace st;
+
+extern "C" {
+
+int st_create(StCtx** out, int device) {
+  if (!out) return ST_BAD_INPUT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return ST_DEVICE;
+  if (device < 0 || device >= n) return ST_BAD_INPUT;
+  auto* c = new StCtx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return ST_DEVICE;
+  }
+  *out = c;
+  return ST_OK;
+}
+
+int st_destroy(StCtx* c) {
+  if (!c) return ST_BAD_INPUT;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  release_all(*c);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return ST_OK;
+}
+
+const char* st_last_error(const StCtx* c) { return c ? c->err.c_str() : "null context"; }
+
+int st_upload(StCtx* c, const StCells* cells) {
+  return guard(c, [&] {
+    if (!cells) throw StError{ST_BAD_INPUT, "cells is NULL"};
+    ST_HIP(hipSetDevice(c->device));
+    const StCells* list[1] = {cells};
+    build_host_geo(c->host, list, 1);
+    copy_geo_to_device(*c);
+    c->gone_host.assign(c->geo.nnodes, 0);
+    refresh_cell_flags(*c);
+    ST_HIP(hipStreamSynchronize(c->stream));
+    c->uploaded = true;
+    c->have_solution = false;
+  });
+}
+
+int st_sizes(StCtx* c, int64_t* nodes, int64_t* cells, int64_t* bricks) {
+  return guard(c, [&] {
+    need_upload(c);
+    if (nodes) *nodes = c->geo.nnodes;
+    if (cells) *cells = c->geo.ncells;
+    if (bricks) *bricks = c->geo.nbricks;
+  });
+}
+
+int st_node_faces(StCtx* c, int32_t* faces) {
+  return guard(c, [&] {
+    need_upload(c);
+    if (!faces) throw StError{ST_BAD_INPUT, "faces is NULL"};
+    std::memcpy(faces, c->host.nface.data(), c->host.nface.size() * sizeof(int32_t));
+  });
+}
+
+int st_lookup_faces(StCtx* c, int64_t n, const int32_t* faces, int64_t* index) {
+  return guard(c, [&] {
+    need_upload(c);
+    if (n < 0 || (n && (!faces || !index))) throw StError{ST_BAD_INPUT, "lookup: bad arguments"};
+    struct H {
+      size_t operator()(const std::array<int32_t, 4>& a) const {
+        uint64_t h = 1469598103934665603ull;
+        for (auto v : a) h = (h ^ (uint32_t)v) * 1099511628211ull;
+        return (size_t)h;
+      }
+    };
+    std::unordered_map<std::array<int32_t, 4>, int64_t, H> m;
+    m.reserve(c->geo.nnodes * 2);
+    const auto& nf = c->host.nface;
+    for (int64_t v = 0; v < c->geo.nnodes; v++) m[{nf[4 * v], nf[4 * v + 1], nf[4 * v + 2], nf[4 * v + 3]}] = v;
+    for (int64_t i = 0; i < n; i++) {
+      auto it = m.find({faces[4 * i], faces[4 * i + 1], faces[4 * i + 2], faces[4 * i + 3]});
+      index[i] = it == m.end() ? -1 : it->second;
+    }
+  });
+}
+
+int st_node_held(StCtx* c, uint8_t* held) {
+  return guard(c, [&] {
+    need_upload(c);
+    if (!held) throw StError{ST_BAD_INPUT, "held is NULL"};
+    for (int64_t v = 0; v < c->geo.nnodes; v++) held[v] = (c->host.nflags[v] & kHeld) ? 1 : 0;
+  });
+}
+
+int st_set_gone(StCtx* c, const uint8_t* gone) {
+  return guard(c, [&] {
+    need_upload(c);
+    set_gone_flags(*c, gone);
+  });
+}
+
+int st_set_beams(StCtx* c, const uint16_t* beams) {
+  return guard(c, [&] {
+    need_upload(c);
+    set_beam_masks(*c, beams);
+  });
+}
+
+int st_apply(StCtx* c, const double* x, double* y) {
+  return guard(c, [&] {
+    need_upload(c);
+    if (!x || !y) throw StError{ST_BAD_INPUT, "apply: x or y is NULL"};
+    ST_HIP(hipSetDevice(c->device));
+    const size_t bytes = (size_t)c->geo.nnodes * 6 * sizeof(double);
+    c->t64.ensure(bytes);
+    c->u64.ensure(bytes);
+    ST_HIP(hipMemcpyAsync(c->t64.p, x, bytes, hipMemcpyHostToDevice, c->stream));
+    EpiStore<double> e;
+    e.y = c->u64.as<double>();
+    launch_apply<double>(*c, c->t64.as<double>(), e);
+    ST_HIP(hipMemcpyAsync(y, c->u64.p, bytes, hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+}  // extern "C"

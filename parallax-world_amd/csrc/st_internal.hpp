@@ -1,0 +1,167 @@
+// Internal layout of the statics_hip library.
+//
+// Device layout (one set of arrays for all structures of a context; a batch
+// concatenates them, structure after structure):
+//   bricks   8^3 cells each, numbered in Morton order within a structure; only
+//            bricks holding a free cell exist. Per brick: the range of its free
+//            cells [cbase[b], cbase[b+1]), of its nodes [nbase[b], nbase[b+1]),
+//            and of the halo cells it recomputes [hbase[b], hbase[b+1]).
+//   cells    the free cells, brick-major, slot order within a brick (slot
+//            ((z&7)*8 + (y&7))*8 + (x&7), lattice/src/lib.rs:310-314). Per cell:
+//            e, g (FP64 and FP32), the live-beam mask, its six face nodes
+//            (VOXEL_FACES order; bit 31 set where the node is not free), and its
+//            coordinates.
+//   nodes    every face of a free cell, brick-major. A face lies between cells
+//            c- and c+ = c- + e_o; it is stored with c+ ("lower" slot of c+) when
+//            c+ is free, else with c- ("upper" slot of c-). So every node is kept
+//            by a free cell next to it: that cell is its key in the octree.
+//   vectors  6 values per node (u, w), zero on held and gone nodes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/statics_hip.h"
+#include "st_beam.hpp"
+
+namespace st {
+
+constexpr uint32_t kNotFree = 0x80000000u;  // in a cell's face list: held or gone node
+constexpr uint8_t kHeld = 1, kGone = 2;
+
+struct StError {
+  int code;
+  std::string what;
+};
+
+#define ST_HIP(call)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      throw ::st::StError{e_ == hipErrorOutOfMemory ? ST_OOM : ST_DEVICE,                      \
+                          std::string(#call) + ": " + hipGetErrorString(e_)};                 \
+  } while (0)
+
+// A device buffer that grows on demand and is never shrunk (freed at context end).
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t n) {
+    if (n <= bytes) return;
+    if (p) ST_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+    ST_HIP(hipMalloc(&p, n < 256 ? 256 : n));
+    bytes = n < 256 ? 256 : n;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// Host-side description of one uploaded structure (its share of the device arrays).
+struct StructInfo {
+  int64_t brick0, nbricks;
+  int64_t cell0, ncells;
+  int64_t node0, nnodes;
+  int32_t origin[3];  // lattice coordinates of local (0,0,0)
+  int32_t ext[3];     // extent in cells (local coordinates lie in [0, ext))
+  int32_t levels_cap; // ceil(log2(max extent))
+  double spacing;
+};
+
+// What the host builds from cells (st_upload); copied to the device as is.
+struct HostGeo {
+  std::vector<uint32_t> cbase, nbase, hbase;  // per brick + 1
+  std::vector<uint32_t> bstruct;              // per brick: structure
+  std::vector<uint32_t> hcell;                // halo cell (compact id)
+  std::vector<uint8_t> ho;                    // halo orientation
+  std::vector<double> ce, cg;                 // per free cell
+  std::vector<uint16_t> cmask;                // per free cell, alive beams
+  std::vector<uint32_t> cfaces;               // 6 per free cell
+  std::vector<uint32_t> ccoord;               // per free cell: x | y << 10 | z << 20 (local)
+  std::vector<uint32_t> cstruct;              // per free cell: structure
+  std::vector<int64_t> cell_of_input;         // per input cell: compact id or -1 (held)
+  std::vector<uint8_t> nflags;                // per node: kHeld
+  std::vector<int32_t> ncell;                 // 2 per node: c-, c+ (compact ids, -1 absent)
+  std::vector<uint8_t> nkind;                 // per node: 2*o + side (side 1: stored with c-)
+  std::vector<uint32_t> nkey;                 // per node: its key cell (the cell keeping it), packed as ccoord
+  std::vector<int32_t> nface;                 // 4 per node: o, x, y, z (+ side cell, lattice units)
+  std::vector<StructInfo> structs;
+  uint32_t max_brick_nodes = 0;
+  void clear() { *this = HostGeo(); }
+};
+
+// Device copies of the geometry.
+struct DevGeo {
+  int64_t nbricks = 0, ncells = 0, nnodes = 0, nhalo = 0;
+  int32_t nstruct = 0;
+  uint32_t max_brick_nodes = 0;
+  DBuf cbase, nbase, hbase, bstruct, hcell, ho;
+  DBuf ce64, cg64, ce32, cg32, cmask_user, cmask, cfaces, ccoord, cstruct;
+  DBuf nflags, ncell, nkind, nkey;
+  DBuf bgeo;  // BeamGeo per structure
+};
+
+// One coarse level (aggregates of the level below, 6 DOF each), block-sparse.
+struct Level {
+  int64_t n = 0;        // aggregates
+  int64_t nblocks = 0;  // stored blocks incl. diagonal
+  int32_t cube = 0;     // octree level of the cubes its aggregates lie in
+  DBuf rowptr;          // n+1 (int32): blocks of row I at [rowptr[I], rowptr[I+1]); diagonal first
+  DBuf col;             // nblocks (int32)
+  DBuf a64, a32;        // 36 per block, row-major
+  DBuf dinv64, dinv32;  // 36 per aggregate
+  DBuf centre;          // 3 per aggregate (double, metres, structure-local)
+  DBuf parent;          // per aggregate of this level: its aggregate in the next level (-1 none)
+  DBuf rep;             // per aggregate: one fine node of it (int32)
+  DBuf key;             // per aggregate: cube coordinates packed (uint64: struct, x, y, z)
+  DBuf kid_ptr, kids;   // children (aggregates of the level below) grouped by this level's aggregate
+  DBuf agg_struct;      // per aggregate: structure
+  std::vector<double> step;  // smoother step per structure
+  DBuf step_d;               // per structure (double)
+  // V-cycle vectors at this level
+  DBuf r64, x64, t64, r32, x32, t32;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  HostGeo host;
+  DevGeo geo;
+  bool uploaded = false;
+  std::vector<uint8_t> gone_host;  // per node, last set
+  // fine-level smoother
+  DBuf dinv64, dinv32;              // 36 per node (row-major), zero for non-free nodes
+  std::vector<double> step0;        // per structure
+  DBuf step0_d;
+  // fine-node aggregation (level 1) and the union-find scratch
+  DBuf label1, uf, uf2, flag, scan_tmp, sort_tmp;
+  DBuf nodes_by_agg, agg_ptr;
+  std::vector<Level> levels;        // coarse levels 1..
+  // solver vectors
+  DBuf f64, x64, r64, p64, ap64, z64, t64, u64;
+  DBuf x32, r32, p32, ap32, z32, t32;
+  DBuf partial, scal;               // reductions: per-block partials, per-structure scalars
+  DBuf done;                        // per structure: 1 once converged
+  std::vector<double> last_x;       // solution of the last solve (device order, 6 per node)
+  bool have_solution = false;
+  std::vector<double> last_loads;
+  StSolveOpts last_opts{};
+  double last_ms[2] = {0, 0};
+};
+
+// host upload (st_upload.cpp)
+void build_host_geo(HostGeo& hg, const StCells* const* cells, int32_t n);
+void copy_geo_to_device(Ctx& c);
+
+}  // namespace st

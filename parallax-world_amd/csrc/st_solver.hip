@@ -1,0 +1,1693 @@
+// The solve: conjugate gradients preconditioned by an octree multigrid
+// V-cycle, all on the device (statics/src/bodies.rs:338-403, 429-468).
+//
+// Hierarchy. Level 1's aggregates are the connected pieces of the free nodes
+// keyed in each 2^3 cube of cells, level l's those of level l-1's aggregates
+// in each 2^l cube: "bodies within bodies" as in bodies_of
+// (statics/src/bodies.rs:472-496), here with every node keyed by the free
+// cell that keeps it. Pieces are found by a lock-free union-find over the
+// fine nodes whose edges are the live beams inside each cube. Coarsening goes
+// on until a cube holds the whole structure, where the aggregates are the
+// structure's connected pieces and the top operator is block diagonal, so the
+// top solve is exact by 6x6 inverses.
+//
+// Transfer. Each aggregate moves rigidly about its centre (the reference's
+// P0 = shift, statics/src/bodies.rs:44-47, 239); the coarse operator is the
+// Galerkin P0^T K P0. Level 1's is assembled directly from the beams that
+// cross aggregates (a beam inside an aggregate moves rigidly and adds nothing),
+// coarser ones from the level below. Unlike the reference the prolongator is
+// not smoothed: the reference's smoothing step widens every coarse stencil
+// (about 20 blocks a row at level 1, 40 at level 2) and would need a sparse
+// triple product per level on the device.
+//
+// Smoother: damped block Jacobi with the reference's step 4/3 over the
+// largest eigenvalue of D^-1 A (statics/src/bodies.rs:172-190), estimated by
+// power iteration per level and structure.
+//
+// Precision: ST_FP64 runs everything in FP64. ST_MIXED runs the V-cycle and
+// the CG vectors in FP32 with FP64 inner products and step lengths, and
+// replaces the recursive residual by the true FP64 residual f - K x (x kept in
+// FP64) every `refresh` iterations or when the residual has fallen tenfold
+// since the last time (reliable updates), which acts as iterative refinement.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+#include "st_solver.hpp"
+
+namespace st {
+
+namespace {
+
+constexpr int kScal = 16;  // doubles of CG state per structure
+enum { S_RZ = 0, S_RR, S_PAP, S_ALPHA, S_BETA, S_GOAL2, S_FF, S_RZNEW, S_RRTRUE, S_RRLAST, S_LAM_A, S_LAM_B };
+constexpr int kMaxJ = 96;  // distinct neighbours of one aggregate
+
+inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)std::max<int64_t>(1, (n + b - 1) / b); }
+
+// --------------------------------------------------------------- geometry
+__device__ __forceinline__ void node_pos(uint32_t v, const uint32_t* nkey, const uint8_t* nkind, double s, double* p) {
+  const uint32_t k = nkey[v];
+  const int kind = nkind[v], o = kind >> 1;
+  p[0] = (double)(k & 1023u) + 0.5;
+  p[1] = (double)((k >> 10) & 1023u) + 0.5;
+  p[2] = (double)((k >> 20) & 1023u) + 0.5;
+  p[o] += (kind & 1) ? 0.5 : -0.5;
+  p[0] *= s;
+  p[1] *= s;
+  p[2] *= s;
+}
+
+// Rigid motion y about a centre seen at offset r: [u + w x r; w] (statics/src/bodies.rs:44-47)
+template <class T>
+__device__ __forceinline__ void shift_add(const T* y, const double* r, T* out) {
+  const T r0 = (T)r[0], r1 = (T)r[1], r2 = (T)r[2];
+  out[0] += y[0] + (y[4] * r2 - y[5] * r1);
+  out[1] += y[1] + (y[5] * r0 - y[3] * r2);
+  out[2] += y[2] + (y[3] * r1 - y[4] * r0);
+  out[3] += y[3];
+  out[4] += y[4];
+  out[5] += y[5];
+}
+// Its transpose: a load [f; m] at offset r taken to the centre: [f; m + r x f]
+template <class T>
+__device__ __forceinline__ void shiftT_add(const T* f, const double* r, T* out) {
+  const T r0 = (T)r[0], r1 = (T)r[1], r2 = (T)r[2];
+  out[0] += f[0];
+  out[1] += f[1];
+  out[2] += f[2];
+  out[3] += f[3] + (r1 * f[2] - r2 * f[1]);
+  out[4] += f[4] + (r2 * f[0] - r0 * f[2]);
+  out[5] += f[5] + (r0 * f[1] - r1 * f[0]);
+}
+
+// 6x6 inverse by Gauss-Jordan with partial pivoting (FP64). Returns false if singular.
+__device__ bool invert6(const double* a_in, double* inv) {
+  double a[36];
+  for (int i = 0; i < 36; i++) {
+    a[i] = a_in[i];
+    inv[i] = (i % 7 == 0) ? 1.0 : 0.0;
+  }
+  for (int col = 0; col < 6; col++) {
+    int piv = col;
+    for (int r = col + 1; r < 6; r++)
+      if (fabs(a[6 * r + col]) > fabs(a[6 * piv + col])) piv = r;
+    if (!(fabs(a[6 * piv + col]) > 0.0)) return false;
+    if (piv != col)
+      for (int k = 0; k < 6; k++) {
+        double t = a[6 * col + k];
+        a[6 * col + k] = a[6 * piv + k];
+        a[6 * piv + k] = t;
+        t = inv[6 * col + k];
+        inv[6 * col + k] = inv[6 * piv + k];
+        inv[6 * piv + k] = t;
+      }
+    const double d = 1.0 / a[6 * col + col];
+    for (int k = 0; k < 6; k++) {
+      a[6 * col + k] *= d;
+      inv[6 * col + k] *= d;
+    }
+    for (int r = 0; r < 6; r++) {
+      if (r == col) continue;
+      const double f = a[6 * r + col];
+      if (f != 0.0)
+        for (int k = 0; k < 6; k++) {
+          a[6 * r + k] -= f * a[6 * col + k];
+          inv[6 * r + k] -= f * inv[6 * col + k];
+        }
+    }
+  }
+  return true;
+}
+
+// The beam geometry of pair q at spacing geo: n (unit), class.
+__device__ __forceinline__ void pair_n(int q, const BeamGeo& G, double* n, int& cls) {
+  cls = pair_opposite(q) ? 0 : 1;
+  const double ns = G.nscale[cls];
+  for (int k = 0; k < 3; k++) n[k] = ns * (double)pair_dir(q, k);
+}
+
+// Own block of one beam end (6x6, row-major) added into D: sgn -1 for the beam's i end (face a), +1 for j.
+__device__ void beam_own_add(const BeamGeo& G, int q, int sgn, double e, double g, double* D) {
+  double n[3];
+  int cls;
+  pair_n(q, G, n, cls);
+  const double K4 = G.K4[cls], R2 = G.R2[cls], hd = G.hd[cls];
+  const double kg = K4 * g, ke = K4 * (e - g);
+  double nx[9] = {0, -n[2], n[1], n[2], 0, -n[0], -n[1], n[0], 0};  // [n]x
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      const double nn = n[r] * n[c], id = r == c ? 1.0 : 0.0;
+      D[6 * r + c] += kg * id + ke * nn;                          // uu
+      const double uw = sgn * hd * kg * nx[3 * r + c];            // uw = sgn hd kg [n]x
+      D[6 * r + 3 + c] += uw;
+      D[6 * (3 + c) + r] += uw;                                   // wu = uw^T
+      D[6 * (3 + r) + 3 + c] += hd * hd * kg * (id - nn) + R2 * (e * id + (g + g - e) * nn);
+    }
+}
+
+// G_v = B_v Sh(r_v) = [sgn I, hd[n]x - sgn [r]x]   (3x6)
+__device__ __forceinline__ void gmat(int sgn, const double* n, double hd, const double* r, double* Gm) {
+  const double nx[9] = {0, -n[2], n[1], n[2], 0, -n[0], -n[1], n[0], 0};
+  const double rx[9] = {0, -r[2], r[1], r[2], 0, -r[0], -r[1], r[0], 0};
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      Gm[6 * a + b] = a == b ? (double)sgn : 0.0;
+      Gm[6 * a + 3 + b] = hd * nx[3 * a + b] - sgn * rx[3 * a + b];
+    }
+}
+
+// out += Gv^T Sf Gw + sv*sw*diag(0, Rm), Sf = K4(gI + (e-g)nn^T), Rm = R2(eI + (2g-e)nn^T)
+__device__ void beam_coarse_add(const double* Gv, const double* Gw, int sv, int sw, const double* n, double K4,
+                                double R2, double e, double g, double* out) {
+  double Sf[9], Rm[9];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      const double nn = n[r] * n[c], id = r == c ? 1.0 : 0.0;
+      Sf[3 * r + c] = K4 * (g * id + (e - g) * nn);
+      Rm[3 * r + c] = R2 * (e * id + (g + g - e) * nn);
+    }
+  double SG[18];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 6; c++) {
+      double s = 0;
+      for (int k = 0; k < 3; k++) s += Sf[3 * r + k] * Gw[6 * k + c];
+      SG[6 * r + c] = s;
+    }
+  for (int r = 0; r < 6; r++)
+    for (int c = 0; c < 6; c++) {
+      double s = 0;
+      for (int k = 0; k < 3; k++) s += Gv[6 * k + r] * SG[6 * k + c];
+      out[6 * r + c] += s;
+    }
+  const double ss = (double)(sv * sw);
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) out[6 * (3 + r) + 3 + c] += ss * Rm[3 * r + c];
+}
+
+// ------------------------------------------------------------- union-find
+__device__ __forceinline__ int uf_find(int* p, int x) {
+  volatile int* vp = p;
+  for (;;) {
+    const int px = vp[x];
+    if (px == x) return x;
+    const int ppx = vp[px];
+    if (ppx != px) vp[x] = ppx;  // path halving: only ever points further up
+    x = ppx;
+  }
+}
+__device__ __forceinline__ void uf_unite(int* p, int a, int b) {
+  for (;;) {
+    a = uf_find(p, a);
+    b = uf_find(p, b);
+    if (a == b) return;
+    if (a < b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    if (atomicCAS(&p[a], a, b) == a) return;
+  }
+}
+
+__global__ void k_iota(int64_t n, int* p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (int)i;
+}
+
+__device__ __forceinline__ bool same_cube(uint32_t a, uint32_t b, int L) {
+  return (((a & 1023u) >> L) == ((b & 1023u) >> L)) && ((((a >> 10) & 1023u) >> L) == (((b >> 10) & 1023u) >> L)) &&
+         ((((a >> 20) & 1023u) >> L) == (((b >> 20) & 1023u) >> L));
+}
+
+// Unites the free ends of every live beam whose ends are keyed in one 2^L cube
+// but not in one 2^(L-1) cube (those were united before).
+__global__ void k_unite_level(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, const uint32_t* nkey, int L,
+                              int* uf) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  uint32_t fi[6], key[6];
+  for (int f = 0; f < 6; f++) {
+    fi[f] = cfaces[6 * c + f];
+    key[f] = (fi[f] & kNotFree) ? 0u : nkey[fi[f]];
+  }
+  const uint32_t m = cmask[c];
+  for (int q = 0; q < 15; q++) {
+    if (!(m & (1u << q))) continue;
+    const int a = kPairA[q], b = kPairB[q];
+    if ((fi[a] | fi[b]) & kNotFree) continue;
+    if (!same_cube(key[a], key[b], L)) continue;
+    if (L > 1 && same_cube(key[a], key[b], L - 1)) continue;
+    uf_unite(uf, (int)fi[a], (int)fi[b]);
+  }
+}
+
+// Every alive beam (any ends): for reachability from held nodes.
+__global__ void k_unite_all(int64_t ncells, const uint32_t* cfaces, const uint16_t* mask, int* uf) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const uint32_t m = mask[c];
+  for (int q = 0; q < 15; q++)
+    if (m & (1u << q))
+      uf_unite(uf, (int)(cfaces[6 * c + kPairA[q]] & ~kNotFree), (int)(cfaces[6 * c + kPairB[q]] & ~kNotFree));
+}
+
+__global__ void k_compress(int64_t n, int* uf) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) uf[v] = uf_find(uf, (int)v);
+}
+
+// flag[v] = 1 for free nodes that are roots
+__global__ void k_root_flags(int64_t n, const int* uf, const uint8_t* nflags, int* flag) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  flag[v] = (!(nflags[v] & (kHeld | kGone)) && uf[v] == (int)v) ? 1 : 0;
+}
+
+// label[v] = id of v's root (ids = exclusive scan of root flags); rep[id] = root node
+__global__ void k_label(int64_t n, const int* uf, const uint8_t* nflags, const int* ids, int* label, int* rep) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  if (nflags[v] & (kHeld | kGone)) {
+    label[v] = -1;
+    return;
+  }
+  const int r = uf[v];
+  label[v] = ids[r];
+  if (r == (int)v) rep[ids[r]] = (int)v;
+}
+
+__global__ void k_held_roots(int64_t n, const int* uf, const uint8_t* nflags, int* flag) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n && (nflags[v] & kHeld)) flag[uf[v]] = 1;
+}
+__global__ void k_joined(int64_t n, const int* uf, const int* flag, uint8_t* joined) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < n) joined[v] = flag[uf[v]] ? 1 : 0;
+}
+
+__global__ void k_fill_int(int64_t n, int* p, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// parent[I] = label[rep[I]]
+__global__ void k_parent(int64_t n, const int* rep, const int* label, int* parent) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) parent[i] = label[rep[i]];
+}
+
+// segment starts of sorted keys: ptr[key[i]] = i for the first of each key; ptr[nseg] = count
+__global__ void k_seg_ptr(int64_t count, const int* keys, int* ptr, int nseg) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) {
+    if (i == 0 || keys[i] != keys[i - 1]) ptr[keys[i]] = (int)i;
+  }
+  if (i == 0) ptr[nseg] = (int)count;
+}
+
+// sort key: label for free nodes, INT_MAX otherwise
+__global__ void k_sort_keys(int64_t n, const int* label, int* keys, int* vals) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  keys[v] = label[v] >= 0 ? label[v] : 0x7fffffff;
+  vals[v] = (int)v;
+}
+
+// ------------------------------------------------------- fine smoother D^-1
+// D_v = sum of the own blocks of the live beams at node v (statics/src/bodies.rs:151-153).
+__global__ void k_dinv(int64_t nnodes, const uint8_t* nflags, const int32_t* ncell, const uint8_t* nkind,
+                       const double* ce, const double* cg, const uint16_t* cmask, const uint32_t* cstruct,
+                       const BeamGeo* bgeo, double* dinv64, float* dinv32, int* bad) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nnodes) return;
+  double D[36], inv[36];
+  for (int i = 0; i < 36; i++) D[i] = 0.0;
+  const bool isfree = !(nflags[v] & (kHeld | kGone));
+  if (isfree) {
+    const int o = nkind[v] >> 1;
+    for (int side = 0; side < 2; side++) {
+      const int c = ncell[2 * v + side];
+      if (c < 0) continue;
+      const int f = side == 0 ? 2 * o + 1 : 2 * o;
+      const BeamGeo G = bgeo[cstruct[c]];
+      const double e = ce[c], g = cg[c];
+      const uint32_t m = (uint32_t)cmask[c] & face_beams(f);
+      for (int q = 0; q < 15; q++)
+        if (m & (1u << q)) beam_own_add(G, q, kPairA[q] == f ? -1 : 1, e, g, D);
+    }
+    if (!invert6(D, inv)) {
+      atomicExch(bad, 1);
+      for (int i = 0; i < 36; i++) inv[i] = 0.0;
+    }
+  } else {
+    for (int i = 0; i < 36; i++) inv[i] = 0.0;
+  }
+  for (int i = 0; i < 36; i++) {
+    dinv64[36 * v + i] = inv[i];
+    dinv32[36 * v + i] = (float)inv[i];
+  }
+}
+
+// ----------------------------------------------------------- brick vector ops
+// All fine-level vector kernels run one workgroup per brick so a structure's
+// share is a contiguous run of workgroups and per-structure sums are exact
+// brick by brick in a fixed order.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_jacobi_fine(const uint32_t* nbase, const uint32_t* bstruct, const uint8_t* done,
+                                                        const T* dinv, const double* step, const T* r, T* x) {
+  const uint32_t b = blockIdx.x, s = bstruct[b];
+  if (done && done[s]) return;
+  const T w = step ? (T)step[s] : T(1);
+  for (uint32_t n = nbase[b] + threadIdx.x; n < nbase[b + 1]; n += blockDim.x) {
+    T rv[6], d[6];
+    load6<T>(r + 6 * (size_t)n, rv);
+    mat6v<T>(dinv + 36 * (size_t)n, rv, d);
+    for (int k = 0; k < 6; k++) d[k] *= w;
+    store6<T>(x + 6 * (size_t)n, d);
+  }
+}
+
+// x += a p ; r -= a ap ; partial r.r      (TX may be wider than T)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_update_xr(const uint32_t* nbase, const uint32_t* bstruct, const uint8_t* done,
+                                                      const double* scal, T* x, T* r, const T* p, const T* ap,
+                                                      double* partial) {
+  __shared__ double red[kBlock / 64];
+  const uint32_t b = blockIdx.x, s = bstruct[b];
+  if (done[s]) {
+    if (threadIdx.x == 0) partial[b] = 0.0;
+    return;
+  }
+  const double a = scal[kScal * s + S_ALPHA];
+  double acc = 0.0;
+  for (uint32_t i = 6 * nbase[b] + threadIdx.x; i < 6 * nbase[b + 1]; i += blockDim.x) {
+    x[i] = (T)((double)x[i] + a * (double)p[i]);
+    const T rn = (T)((double)r[i] - a * (double)ap[i]);
+    r[i] = rn;
+    acc += (double)rn * (double)rn;
+  }
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) partial[b] = t;
+}
+
+// p = z + beta p
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_update_p(const uint32_t* nbase, const uint32_t* bstruct, const uint8_t* done,
+                                                     const double* scal, const T* z, T* p) {
+  const uint32_t b = blockIdx.x, s = bstruct[b];
+  if (done[s]) return;
+  const double be = scal[kScal * s + S_BETA];
+  for (uint32_t i = 6 * nbase[b] + threadIdx.x; i < 6 * nbase[b + 1]; i += blockDim.x)
+    p[i] = (T)((double)z[i] + be * (double)p[i]);
+}
+
+// partial a.b per brick
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dot_fine(const uint32_t* nbase, const T* a, const T* bb, double* partial) {
+  __shared__ double red[kBlock / 64];
+  const uint32_t b = blockIdx.x;
+  double acc = 0.0;
+  for (uint32_t i = 6 * nbase[b] + threadIdx.x; i < 6 * nbase[b + 1]; i += blockDim.x)
+    acc += (double)a[i] * (double)bb[i];
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) partial[b] = t;
+}
+
+// dst = src (converting), brick-blocked, skipping done structures
+template <class TD, class TS>
+__global__ __launch_bounds__(kBlock) void k_copy_fine(const uint32_t* nbase, const uint32_t* bstruct, const uint8_t* done,
+                                                      const TS* src, TD* dst) {
+  const uint32_t b = blockIdx.x, s = bstruct[b];
+  if (done && done[s]) return;
+  for (uint32_t i = 6 * nbase[b] + threadIdx.x; i < 6 * nbase[b + 1]; i += blockDim.x) dst[i] = (TD)src[i];
+}
+
+// x64 += x32; x32 = 0
+__global__ __launch_bounds__(kBlock) void k_flush(const uint32_t* nbase, const uint32_t* bstruct, const uint8_t* done,
+                                                  double* x64, float* x32) {
+  const uint32_t b = blockIdx.x, s = bstruct[b];
+  if (done[s]) return;
+  for (uint32_t i = 6 * nbase[b] + threadIdx.x; i < 6 * nbase[b + 1]; i += blockDim.x) {
+    x64[i] += (double)x32[i];
+    x32[i] = 0.0f;
+  }
+}
+
+// zero rows of held and gone nodes (a load on them is not read)
+__global__ void k_mask_rows(int64_t n, const uint8_t* nflags, double* v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (nflags[i] & (kHeld | kGone))
+    for (int k = 0; k < 6; k++) v[6 * i + k] = 0.0;
+}
+
+// out[s * stride + slot] = sum of partial over [seg[s], seg[s+1]) (fixed order)
+__global__ __launch_bounds__(kBlock) void k_reduce_seg(const double* partial, const int64_t* seg, double* out, int stride,
+                                                       int slot) {
+  __shared__ double red[kBlock / 64];
+  const int s = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t i = seg[s] + threadIdx.x; i < seg[s + 1]; i += blockDim.x) acc += partial[i];
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) out[(int64_t)s * stride + slot] = t;
+}
+
+// ------------------------------------------------------------- CG scalars
+__global__ void k_cg_start(int ns, double* scal, uint8_t* done, int* status, int* iters, double tol, int mixed,
+                           uint8_t* want) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  double* S = scal + kScal * s;
+  S[S_GOAL2] = tol * tol * S[S_FF];
+  S[S_RRLAST] = S[S_RR];
+  iters[s] = 0;
+  status[s] = 0;
+  want[s] = 0;
+  done[s] = (S[S_RR] <= S[S_GOAL2]) ? 1 : 0;
+  if (!isfinite(S[S_RR])) {
+    done[s] = 1;
+    status[s] = ST_NAN;
+  }
+  (void)mixed;
+}
+__global__ void k_cg_alpha(int ns, double* scal, uint8_t* done, int* status) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns || done[s]) return;
+  double* S = scal + kScal * s;
+  const double pap = S[S_PAP];
+  if (!isfinite(pap)) {
+    status[s] = ST_NAN;
+    done[s] = 1;
+    S[S_ALPHA] = 0.0;
+  } else if (!(pap > 0.0)) {
+    status[s] = ST_NOT_SPD;
+    done[s] = 1;
+    S[S_ALPHA] = 0.0;
+  } else {
+    S[S_ALPHA] = S[S_RZ] / pap;
+  }
+}
+// after x, r update. mixed: ask for a true residual instead of stopping
+__global__ void k_cg_check(int ns, double* scal, uint8_t* done, int* status, int* iters, int mixed, uint8_t* want,
+                           double drop2) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns || done[s]) return;
+  double* S = scal + kScal * s;
+  iters[s] += 1;
+  const double rr = S[S_RR];
+  if (!isfinite(rr)) {
+    status[s] = ST_NAN;
+    done[s] = 1;
+    return;
+  }
+  if (mixed) {
+    if (rr <= S[S_GOAL2] || rr <= drop2 * S[S_RRLAST]) want[s] = 1;
+  } else if (rr <= S[S_GOAL2]) {
+    done[s] = 1;
+  }
+}
+__global__ void k_cg_beta(int ns, double* scal, const uint8_t* done) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns || done[s]) return;
+  double* S = scal + kScal * s;
+  const double rzn = S[S_RZNEW];
+  S[S_BETA] = S[S_RZ] != 0.0 ? rzn / S[S_RZ] : 0.0;
+  S[S_RZ] = rzn;
+}
+// after a true residual: it is the residual from now on
+__global__ void k_cg_true(int ns, double* scal, uint8_t* done, int* status, uint8_t* want, int* refreshes) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns || done[s]) return;
+  double* S = scal + kScal * s;
+  refreshes[s] += 1;
+  S[S_RR] = S[S_RRTRUE];
+  S[S_RRLAST] = S[S_RRTRUE];
+  want[s] = 0;
+  if (!isfinite(S[S_RRTRUE])) {
+    status[s] = ST_NAN;
+    done[s] = 1;
+  } else if (S[S_RRTRUE] <= S[S_GOAL2]) {
+    done[s] = 1;
+  }
+}
+
+// ------------------------------------------------------------ level 1 setup
+// centre of each level-1 aggregate: mean of its nodes' positions (fixed order)
+__global__ void k_centre1(int64_t n1, const int* agg_ptr, const int* nodes, const uint32_t* nkey, const uint8_t* nkind,
+                          const uint32_t* nstruct_of_agg, const BeamGeo* bgeo, double* centre) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n1) return;
+  const double s = 2.0 * bgeo[nstruct_of_agg[I]].hd[0];
+  double c[3] = {0, 0, 0};
+  const int b = agg_ptr[I], e = agg_ptr[I + 1];
+  for (int i = b; i < e; i++) {
+    double p[3];
+    node_pos((uint32_t)nodes[i], nkey, nkind, s, p);
+    c[0] += p[0];
+    c[1] += p[1];
+    c[2] += p[2];
+  }
+  const double inv = 1.0 / (double)(e - b);
+  centre[3 * I] = c[0] * inv;
+  centre[3 * I + 1] = c[1] * inv;
+  centre[3 * I + 2] = c[2] * inv;
+}
+
+// structure of each aggregate (from its representative node's first cell)
+__global__ void k_agg_struct(int64_t n, const int* rep, const int32_t* ncell, const uint32_t* cstruct, uint32_t* out) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  const int v = rep[I];
+  int c = ncell[2 * v];
+  if (c < 0) c = ncell[2 * v + 1];
+  out[I] = cstruct[c];
+}
+
+// Level-1 Galerkin rows from the beams crossing aggregates. pass 0: count
+// distinct neighbours; pass 1: fill blocks (row's diagonal first).
+__global__ void k_a1(int pass, int64_t n1, const int* agg_ptr, const int* nodes, const int* label, const double* centre,
+                     const uint32_t* agg_struct, const uint32_t* nkey, const uint8_t* nkind, const int32_t* ncell,
+                     const uint32_t* cfaces, const double* ce, const double* cg, const uint16_t* cmask,
+                     const BeamGeo* bgeo, int* rowcnt, const int* rowptr, int* col, double* A, int* overflow) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n1) return;
+  const BeamGeo G = bgeo[agg_struct[I]];
+  const double s = 2.0 * G.hd[0];
+  const double cI[3] = {centre[3 * I], centre[3 * I + 1], centre[3 * I + 2]};
+  int js[kMaxJ];
+  int nj = 0;
+  int base = 0, len = 0;
+  if (pass == 1) {
+    base = rowptr[I];
+    len = rowptr[I + 1] - base;
+    col[base] = (int)I;
+    for (int i = 0; i < 36 * len; i++) A[36 * (int64_t)base + i] = 0.0;
+    nj = 1;  // filled by the slots already in col
+  }
+  for (int idx = agg_ptr[I]; idx < agg_ptr[I + 1]; idx++) {
+    const uint32_t v = (uint32_t)nodes[idx];
+    const int o = nkind[v] >> 1;
+    double pv[3], rv[3];
+    node_pos(v, nkey, nkind, s, pv);
+    for (int k = 0; k < 3; k++) rv[k] = pv[k] - cI[k];
+    for (int side = 0; side < 2; side++) {
+      const int c = ncell[2 * v + side];
+      if (c < 0) continue;
+      const int f = side == 0 ? 2 * o + 1 : 2 * o;
+      const uint32_t m = (uint32_t)cmask[c] & face_beams(f);
+      if (!m) continue;
+      const double e = ce[c], g = cg[c];
+      for (int q = 0; q < 15; q++) {
+        if (!(m & (1u << q))) continue;
+        const int f2 = kPairA[q] == f ? kPairB[q] : kPairA[q];
+        const uint32_t wf = cfaces[6 * c + f2];
+        const int J = (wf & kNotFree) ? -1 : label[wf];
+        if (J == (int)I) continue;
+        if (pass == 0) {
+          if (J >= 0) {
+            bool seen = false;
+            for (int t = 0; t < nj; t++) seen |= js[t] == J;
+            if (!seen) {
+              if (nj < kMaxJ)
+                js[nj++] = J;
+              else
+                atomicExch(overflow, 1);
+            }
+          }
+          continue;
+        }
+        double n[3];
+        int cls;
+        pair_n(q, G, n, cls);
+        const int sv = kPairA[q] == f ? -1 : 1, sw = -sv;
+        double Gv[18];
+        gmat(sv, n, G.hd[cls], rv, Gv);
+        beam_coarse_add(Gv, Gv, sv, sv, n, G.K4[cls], G.R2[cls], e, g, A + 36 * (int64_t)base);
+        if (J >= 0) {
+          double pw[3], rw[3];
+          node_pos(wf, nkey, nkind, s, pw);
+          for (int k = 0; k < 3; k++) rw[k] = pw[k] - centre[3 * (int64_t)J + k];
+          double Gw[18];
+          gmat(sw, n, G.hd[cls], rw, Gw);
+          int slot = -1;
+          for (int t = 1; t < len; t++)
+            if (col[base + t] == J) slot = t;
+          if (slot < 0) {
+            // first time: claim the next free slot (filled in order)
+            for (int t = 1; t < len; t++)
+              if (col[base + t] == -1) {
+                col[base + t] = J;
+                slot = t;
+                break;
+              }
+          }
+          if (slot > 0) beam_coarse_add(Gv, Gw, sv, sw, n, G.K4[cls], G.R2[cls], e, g, A + 36 * ((int64_t)base + slot));
+        }
+      }
+    }
+  }
+  if (pass == 0) rowcnt[I] = 1 + nj;
+}
+
+// Galerkin from the level below: rows of parent J = sum over children i, blocks (i, j):
+// Sh(r_i)^T A_ij Sh(r_j) into (J, parent(j)).
+__device__ void shT_A_sh(const double* A, const double* ri, const double* rj, double* out) {
+  // M = A Sh(rj): columns 3..5 get  -A[:,0:3][rj]x
+  double M[36];
+  const double rx[9] = {0, -rj[2], rj[1], rj[2], 0, -rj[0], -rj[1], rj[0], 0};
+  for (int r = 0; r < 6; r++) {
+    for (int c = 0; c < 3; c++) M[6 * r + c] = A[6 * r + c];
+    for (int c = 0; c < 3; c++) {
+      double s = A[6 * r + 3 + c];
+      for (int k = 0; k < 3; k++) s -= A[6 * r + k] * rx[3 * k + c];
+      M[6 * r + 3 + c] = s;
+    }
+  }
+  // Sh(ri)^T M: rows 3..5 += [ri]x rows 0..2
+  const double qx[9] = {0, -ri[2], ri[1], ri[2], 0, -ri[0], -ri[1], ri[0], 0};
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 6; c++) out[6 * r + c] += M[6 * r + c];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 6; c++) {
+      double s = M[6 * (3 + r) + c];
+      for (int k = 0; k < 3; k++) s += qx[3 * r + k] * M[6 * k + c];
+      out[6 * (3 + r) + c] += s;
+    }
+}
+
+__global__ void k_galerkin(int pass, int64_t n, const int* kid_ptr, const int* kids, const double* centre,
+                           const double* centre_c, const int* parent_c, const int* rowptr_c, const int* col_c,
+                           const double* A_c, int* rowcnt, const int* rowptr, int* col, double* A, int* overflow) {
+  const int64_t J = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= n) return;
+  const double cJ[3] = {centre[3 * J], centre[3 * J + 1], centre[3 * J + 2]};
+  int js[kMaxJ];
+  int nj = 0;
+  int base = 0, len = 0;
+  if (pass == 1) {
+    base = rowptr[J];
+    len = rowptr[J + 1] - base;
+    col[base] = (int)J;
+    for (int i = 0; i < 36 * len; i++) A[36 * (int64_t)base + i] = 0.0;
+  }
+  for (int t = kid_ptr[J]; t < kid_ptr[J + 1]; t++) {
+    const int i = kids[t];
+    double ri[3];
+    for (int k = 0; k < 3; k++) ri[k] = centre_c[3 * (int64_t)i + k] - cJ[k];
+    for (int bq = rowptr_c[i]; bq < rowptr_c[i + 1]; bq++) {
+      const int j = col_c[bq];
+      const int P = parent_c[j];
+      if (pass == 0) {
+        if (P != (int)J) {
+          bool seen = false;
+          for (int u = 0; u < nj; u++) seen |= js[u] == P;
+          if (!seen) {
+            if (nj < kMaxJ)
+              js[nj++] = P;
+            else
+              atomicExch(overflow, 1);
+          }
+        }
+        continue;
+      }
+      double rj[3];
+      for (int k = 0; k < 3; k++) rj[k] = centre_c[3 * (int64_t)j + k] - centre[3 * (int64_t)P + k];
+      int slot = 0;
+      if (P != (int)J) {
+        slot = -1;
+        for (int u = 1; u < len; u++)
+          if (col[base + u] == P) slot = u;
+        if (slot < 0)
+          for (int u = 1; u < len; u++)
+            if (col[base + u] == -1) {
+              col[base + u] = P;
+              slot = u;
+              break;
+            }
+      }
+      if (slot >= 0) shT_A_sh(A_c + 36 * (int64_t)bq, ri, rj, A + 36 * ((int64_t)base + slot));
+    }
+  }
+  if (pass == 0) rowcnt[J] = 1 + nj;
+}
+
+// centre of a coarse aggregate: mean of its children's centres (statics/src/bodies.rs:229-237)
+__global__ void k_centre_c(int64_t n, const int* kid_ptr, const int* kids, const double* centre_c, double* centre) {
+  const int64_t J = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= n) return;
+  double c[3] = {0, 0, 0};
+  const int b = kid_ptr[J], e = kid_ptr[J + 1];
+  for (int t = b; t < e; t++)
+    for (int k = 0; k < 3; k++) c[k] += centre_c[3 * (int64_t)kids[t] + k];
+  for (int k = 0; k < 3; k++) centre[3 * J + k] = c[k] / (double)(e - b);
+}
+
+__global__ void k_coarse_dinv(int64_t n, const int* rowptr, const double* A, double* d64, float* d32, int* bad) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  double inv[36];
+  if (!invert6(A + 36 * (int64_t)rowptr[I], inv)) {
+    atomicExch(bad, 1);
+    for (int i = 0; i < 36; i++) inv[i] = 0.0;
+  }
+  for (int i = 0; i < 36; i++) {
+    d64[36 * I + i] = inv[i];
+    d32[36 * I + i] = (float)inv[i];
+  }
+}
+
+__global__ void k_to_f32(int64_t n, const double* a, float* b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void k_fill_neg(int64_t n, int* p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = -1;
+}
+
+// --------------------------------------------------------- coarse vector ops
+// Every coarse kernel: one thread per aggregate, skipping structures that are done.
+template <class T>
+__global__ void k_c_jacobi(int64_t n, const uint32_t* ast, const uint8_t* done, const T* dinv, const double* step,
+                           const T* r, T* x) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  const uint32_t s = ast[I];
+  if (done && done[s]) return;
+  T rv[6], d[6];
+  load6<T>(r + 6 * I, rv);
+  mat6v<T>(dinv + 36 * I, rv, d);
+  const T w = step ? (T)step[s] : T(1);
+  for (int k = 0; k < 6; k++) d[k] *= w;
+  store6<T>(x + 6 * I, d);
+}
+
+// t = r - A x  (mode 0)   or   z = x + step D^-1 (r - A x)  (mode 1)
+template <class T>
+__global__ void k_c_resid(int mode, int64_t n, const uint32_t* ast, const uint8_t* done, const int* rowptr, const int* col,
+                          const T* A, const T* dinv, const double* step, const T* r, const T* x, T* out) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  const uint32_t s = ast[I];
+  if (done && done[s]) return;
+  T acc[6];
+  load6<T>(r + 6 * I, acc);
+  for (int b = rowptr[I]; b < rowptr[I + 1]; b++) {
+    const T* M = A + 36 * (int64_t)b;
+    T xv[6];
+    load6<T>(x + 6 * (int64_t)col[b], xv);
+    for (int i = 0; i < 6; i++) {
+      T t = 0;
+      for (int j = 0; j < 6; j++) t += M[6 * i + j] * xv[j];
+      acc[i] -= t;
+    }
+  }
+  if (mode == 0) {
+    store6<T>(out + 6 * I, acc);
+  } else {
+    T d[6], xv[6];
+    mat6v<T>(dinv + 36 * I, acc, d);
+    load6<T>(x + 6 * I, xv);
+    const T w = (T)step[s];
+    for (int k = 0; k < 6; k++) xv[k] += w * d[k];
+    store6<T>(out + 6 * I, xv);
+  }
+}
+
+// y = A x (for power iterations), then partial sums of x.x per aggregate are not needed: w = D^-1 A x
+template <class T>
+__global__ void k_c_dinvA(int64_t n, const int* rowptr, const int* col, const T* A, const T* dinv, const T* x, T* w) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  T acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int b = rowptr[I]; b < rowptr[I + 1]; b++) {
+    const T* M = A + 36 * (int64_t)b;
+    T xv[6];
+    load6<T>(x + 6 * (int64_t)col[b], xv);
+    for (int i = 0; i < 6; i++)
+      for (int j = 0; j < 6; j++) acc[i] += M[6 * i + j] * xv[j];
+  }
+  T d[6];
+  mat6v<T>(dinv + 36 * I, acc, d);
+  store6<T>(w + 6 * I, d);
+}
+
+template <class T>
+__global__ void k_c_sq(int64_t n, const T* x, double* out) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  double s = 0;
+  for (int k = 0; k < 6; k++) s += (double)x[6 * I + k] * (double)x[6 * I + k];
+  out[I] = s;
+}
+
+__global__ void k_c_init(int64_t n, double* x) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  for (int k = 0; k < 6; k++) x[6 * I + k] = 1.0 + (double)((7 * I + 3 * k) % 11) / 11.0;
+}
+
+// restrict fine -> level 1: c[I] = sum over nodes v of I of Sh(r_v)^T t[v]
+template <class T>
+__global__ void k_restrict1(int64_t n1, const uint32_t* ast, const uint8_t* done, const int* agg_ptr, const int* nodes,
+                            const double* centre, const uint32_t* nkey, const uint8_t* nkind, const BeamGeo* bgeo,
+                            const T* t, T* c) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n1) return;
+  const uint32_t s = ast[I];
+  if (done && done[s]) return;
+  const double sp = 2.0 * bgeo[s].hd[0];
+  T acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = agg_ptr[I]; i < agg_ptr[I + 1]; i++) {
+    const uint32_t v = (uint32_t)nodes[i];
+    double p[3];
+    node_pos(v, nkey, nkind, sp, p);
+    for (int k = 0; k < 3; k++) p[k] -= centre[3 * I + k];
+    T tv[6];
+    load6<T>(t + 6 * (size_t)v, tv);
+    shiftT_add<T>(tv, p, acc);
+  }
+  store6<T>(c + 6 * I, acc);
+}
+
+// prolong level 1 -> fine (brick-blocked): x[v] += Sh(r_v) y[label[v]]
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prolong1(const uint32_t* nbase, const uint32_t* bstruct, const uint8_t* done,
+                                                     const int* label, const double* centre, const uint32_t* nkey,
+                                                     const uint8_t* nkind, const BeamGeo* bgeo, const T* y, T* x) {
+  const uint32_t b = blockIdx.x, s = bstruct[b];
+  if (done && done[s]) return;
+  const double sp = 2.0 * bgeo[s].hd[0];
+  for (uint32_t v = nbase[b] + threadIdx.x; v < nbase[b + 1]; v += blockDim.x) {
+    const int I = label[v];
+    if (I < 0) continue;
+    double p[3];
+    node_pos(v, nkey, nkind, sp, p);
+    for (int k = 0; k < 3; k++) p[k] -= centre[3 * (int64_t)I + k];
+    T xv[6], yv[6];
+    load6<T>(x + 6 * (size_t)v, xv);
+    load6<T>(y + 6 * (int64_t)I, yv);
+    shift_add<T>(yv, p, xv);
+    store6<T>(x + 6 * (size_t)v, xv);
+  }
+}
+
+// restrict level l -> l+1 over children lists
+template <class T>
+__global__ void k_restrict_c(int64_t n, const uint32_t* ast, const uint8_t* done, const int* kid_ptr, const int* kids,
+                             const double* centre, const double* centre_c, const T* t, T* c) {
+  const int64_t J = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= n) return;
+  if (done && done[ast[J]]) return;
+  T acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int u = kid_ptr[J]; u < kid_ptr[J + 1]; u++) {
+    const int i = kids[u];
+    double r[3];
+    for (int k = 0; k < 3; k++) r[k] = centre_c[3 * (int64_t)i + k] - centre[3 * J + k];
+    T tv[6];
+    load6<T>(t + 6 * (int64_t)i, tv);
+    shiftT_add<T>(tv, r, acc);
+  }
+  store6<T>(c + 6 * J, acc);
+}
+
+// prolong level l+1 -> l: x[i] += Sh(r_i) y[parent[i]]
+template <class T>
+__global__ void k_prolong_c(int64_t n, const uint32_t* ast, const uint8_t* done, const int* parent, const double* centre,
+                            const double* centre_p, const T* y, T* x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (done && done[ast[i]]) return;
+  const int P = parent[i];
+  double r[3];
+  for (int k = 0; k < 3; k++) r[k] = centre[3 * i + k] - centre_p[3 * (int64_t)P + k];
+  T xv[6], yv[6];
+  load6<T>(x + 6 * i, xv);
+  load6<T>(y + 6 * (int64_t)P, yv);
+  shift_add<T>(yv, r, xv);
+  store6<T>(x + 6 * i, xv);
+}
+
+// structure ranges of an aggregate numbering: out[s] = ids[first node of structure s]
+__global__ void k_seg_from_ids(int ns, const int64_t* node0, const int* ids, const int* flag, int64_t nnodes,
+                               int64_t* out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > ns) return;
+  if (s < ns)
+    out[s] = node0[s] < nnodes ? ids[node0[s]] : (nnodes ? ids[nnodes - 1] + flag[nnodes - 1] : 0);
+  else
+    out[s] = nnodes ? ids[nnodes - 1] + flag[nnodes - 1] : 0;
+}
+
+template <class T>
+T* ptr(DBuf& b) {
+  return b.as<T>();
+}
+
+}  // namespace
+
+// ============================================================== host side
+
+struct Solver {
+  Ctx& c;
+  hipStream_t st;
+  int ns;
+  DevGeo& g;
+  std::vector<int64_t> sbrick;  // per structure: brick offsets (ns+1)
+  DBuf sbrick_d, node0_d;
+  DBuf sstatus, siters, srefresh, swant, sbad;
+  std::vector<std::vector<int64_t>> lseg;  // per level: aggregate offsets per structure
+  std::vector<DBuf*> lseg_d;
+  DBuf seg_tmp;
+  int64_t ndof = 0;
+  explicit Solver(Ctx& cc) : c(cc), st(cc.stream), ns(cc.geo.nstruct), g(cc.geo) {}
+  ~Solver() {
+    sbrick_d.release();
+    node0_d.release();
+    sstatus.release();
+    siters.release();
+    srefresh.release();
+    swant.release();
+    sbad.release();
+    seg_tmp.release();
+  }
+
+  void reduce_bricks(DBuf& partial, int slot) {
+    k_reduce_seg<<<ns, kBlock, 0, st>>>(partial.as<double>(), sbrick_d.as<int64_t>(), c.scal.as<double>(), kScal, slot);
+  }
+
+  void scan(int* in, int* out, int64_t n) {
+    size_t bytes = 0;
+    ST_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, st));
+    c.scan_tmp.ensure(bytes + 16);
+    ST_HIP(hipcub::DeviceScan::ExclusiveSum(c.scan_tmp.p, bytes, in, out, (int)n, st));
+  }
+  void sort_pairs(const int* kin, int* kout, const int* vin, int* vout, int64_t n) {
+    size_t bytes = 0;
+    ST_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0, 32, st));
+    c.sort_tmp.ensure(bytes + 16);
+    ST_HIP(hipcub::DeviceRadixSort::SortPairs(c.sort_tmp.p, bytes, kin, kout, vin, vout, (int)n, 0, 32, st));
+  }
+  int read_int(const int* d) {
+    int h = 0;
+    ST_HIP(hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipStreamSynchronize(st));
+    return h;
+  }
+
+  void init_struct_arrays() {
+    sbrick.assign(ns + 1, 0);
+    std::vector<int64_t> node0(ns + 1, 0);
+    for (int s = 0; s < ns; s++) {
+      sbrick[s] = c.host.structs[s].brick0;
+      node0[s] = c.host.structs[s].node0;
+    }
+    sbrick[ns] = g.nbricks;
+    node0[ns] = g.nnodes;
+    sbrick_d.ensure((ns + 1) * sizeof(int64_t));
+    node0_d.ensure((ns + 1) * sizeof(int64_t));
+    ST_HIP(hipMemcpyAsync(sbrick_d.p, sbrick.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    ST_HIP(hipMemcpyAsync(node0_d.p, node0.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    sstatus.ensure(ns * sizeof(int));
+    siters.ensure(ns * sizeof(int));
+    srefresh.ensure(ns * sizeof(int));
+    swant.ensure(ns);
+    sbad.ensure(sizeof(int));
+    c.scal.ensure((size_t)ns * kScal * sizeof(double));
+    c.done.ensure(ns);
+    c.partial.ensure(std::max<int64_t>(g.nbricks, 1) * sizeof(double));
+    ST_HIP(hipMemsetAsync(c.scal.p, 0, (size_t)ns * kScal * sizeof(double), st));
+    ST_HIP(hipMemsetAsync(srefresh.p, 0, ns * sizeof(int), st));
+  }
+
+  // ---------------------------------------------------------------- setup
+  void setup_fine() {
+    const int64_t nn = g.nnodes;
+    c.dinv64.ensure(std::max<int64_t>(nn, 1) * 36 * sizeof(double));
+    c.dinv32.ensure(std::max<int64_t>(nn, 1) * 36 * sizeof(float));
+    ST_HIP(hipMemsetAsync(sbad.p, 0, sizeof(int), st));
+    if (nn)
+      k_dinv<<<nblk(nn), 256, 0, st>>>(nn, g.nflags.as<uint8_t>(), g.ncell.as<int32_t>(), g.nkind.as<uint8_t>(),
+                                       g.ce64.as<double>(), g.cg64.as<double>(), g.cmask.as<uint16_t>(),
+                                       g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), c.dinv64.as<double>(),
+                                       c.dinv32.as<float>(), sbad.as<int>());
+    ST_HIP(hipGetLastError());
+    if (read_int(sbad.as<int>())) throw StError{ST_NOT_SPD, "a free node has a singular stiffness block"};
+    // smoother step: 4/3 over the greatest eigenvalue of D^-1 K (statics/src/bodies.rs:174, 180-190)
+    const size_t vb = (size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double);
+    c.t64.ensure(vb);
+    c.u64.ensure(vb);
+    c.z64.ensure(vb);
+    double* v = c.t64.as<double>();
+    double* w = c.u64.as<double>();
+    double* tmp = c.z64.as<double>();
+    if (nn) {
+      k_c_init<<<nblk(nn), 256, 0, st>>>(nn, v);
+      k_mask_rows<<<nblk(nn), 256, 0, st>>>(nn, g.nflags.as<uint8_t>(), v);
+    }
+    const int iters = 10;
+    std::vector<double> nrm_prev(ns, 0.0), nrm(ns, 0.0);
+    for (int it = 0; it < iters; it++) {
+      EpiStore<double> e;
+      e.y = tmp;
+      launch_apply<double>(c, v, e);
+      if (g.nbricks)
+        k_jacobi_fine<double><<<(unsigned)g.nbricks, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(),
+                                                                      nullptr, c.dinv64.as<double>(), nullptr, tmp, w);
+      if (it >= iters - 2) {
+        if (g.nbricks)
+          k_dot_fine<double><<<(unsigned)g.nbricks, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), w, w, c.partial.as<double>());
+        reduce_bricks(c.partial, it == iters - 2 ? S_LAM_A : S_LAM_B);
+      }
+      std::swap(v, w);
+    }
+    std::vector<double> sc((size_t)ns * kScal);
+    ST_HIP(hipMemcpyAsync(sc.data(), c.scal.p, sc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipStreamSynchronize(st));
+    c.step0.assign(ns, 1.0);
+    for (int s = 0; s < ns; s++) {
+      const double a = sc[(size_t)s * kScal + S_LAM_A], b = sc[(size_t)s * kScal + S_LAM_B];
+      const double lam = (a > 0 && b > 0) ? std::sqrt(b / a) : 1.0;
+      c.step0[s] = 4.0 / 3.0 / lam;
+    }
+    c.step0_d.ensure(ns * sizeof(double));
+    ST_HIP(hipMemcpyAsync(c.step0_d.p, c.step0.data(), ns * sizeof(double), hipMemcpyHostToDevice, st));
+  }
+
+  // Labels at cube level L over the fine union-find (already holding level L-1's unions).
+  // Returns the aggregate count; ids/rep/label written.
+  int64_t label_level(int L, int* label, DBuf& rep, std::vector<int64_t>& seg) {
+    const int64_t nn = g.nnodes;
+    int* uf = c.uf.as<int>();
+    if (g.ncells)
+      k_unite_level<<<nblk(g.ncells), 256, 0, st>>>(g.ncells, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(),
+                                                    g.nkey.as<uint32_t>(), L, uf);
+    k_compress<<<nblk(nn), 256, 0, st>>>(nn, uf);
+    int* flag = c.flag.as<int>();
+    int* ids = c.uf2.as<int>();
+    k_root_flags<<<nblk(nn), 256, 0, st>>>(nn, uf, g.nflags.as<uint8_t>(), flag);
+    scan(flag, ids, nn);
+    seg.assign(ns + 1, 0);
+    seg_tmp.ensure((ns + 1) * sizeof(int64_t));
+    k_seg_from_ids<<<nblk(ns + 1), 256, 0, st>>>(ns, node0_d.as<int64_t>(), ids, flag, nn, seg_tmp.as<int64_t>());
+    ST_HIP(hipMemcpyAsync(seg.data(), seg_tmp.p, (ns + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipStreamSynchronize(st));
+    const int64_t count = seg[ns];
+    rep.ensure(std::max<int64_t>(count, 1) * sizeof(int));
+    k_label<<<nblk(nn), 256, 0, st>>>(nn, uf, g.nflags.as<uint8_t>(), ids, label, rep.as<int>());
+    ST_HIP(hipGetLastError());
+    return count;
+  }
+
+  void build_rows_a1(Level& lv) {
+    const int64_t n = lv.n;
+    DBuf cnt;
+    cnt.ensure((n + 1) * sizeof(int));
+    lv.rowptr.ensure((n + 1) * sizeof(int));
+    ST_HIP(hipMemsetAsync(sbad.p, 0, sizeof(int), st));
+    ST_HIP(hipMemsetAsync(cnt.p, 0, (n + 1) * sizeof(int), st));
+    auto launch = [&](int pass) {
+      k_a1<<<nblk(n, 64), 64, 0, st>>>(pass, n, c.agg_ptr.as<int>(), c.nodes_by_agg.as<int>(), c.label1.as<int>(),
+                                      lv.centre.as<double>(), lv.agg_struct.as<uint32_t>(), g.nkey.as<uint32_t>(),
+                                      g.nkind.as<uint8_t>(), g.ncell.as<int32_t>(), g.cfaces.as<uint32_t>(),
+                                      g.ce64.as<double>(), g.cg64.as<double>(), g.cmask.as<uint16_t>(),
+                                      g.bgeo.as<BeamGeo>(), cnt.as<int>(), lv.rowptr.as<int>(), lv.col.as<int>(),
+                                      lv.a64.as<double>(), sbad.as<int>());
+      ST_HIP(hipGetLastError());
+    };
+    launch(0);
+    finish_rows(lv, cnt, launch);
+    cnt.release();
+  }
+
+  template <class F>
+  void finish_rows(Level& lv, DBuf& cnt, F launch) {
+    const int64_t n = lv.n;
+    if (read_int(sbad.as<int>())) throw StError{ST_DEVICE, "an aggregate has more neighbours than the row capacity"};
+    scan(cnt.as<int>(), lv.rowptr.as<int>(), n + 1);
+    int nb = 0;
+    ST_HIP(hipMemcpyAsync(&nb, lv.rowptr.as<int>() + n, sizeof(int), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipStreamSynchronize(st));
+    lv.nblocks = nb;
+    lv.col.ensure(std::max(nb, 1) * sizeof(int));
+    lv.a64.ensure(std::max(nb, 1) * 36 * sizeof(double));
+    lv.a32.ensure(std::max(nb, 1) * 36 * sizeof(float));
+    k_fill_neg<<<nblk(nb), 256, 0, st>>>(nb, lv.col.as<int>());
+    launch(1);
+    k_to_f32<<<nblk((int64_t)nb * 36), 256, 0, st>>>((int64_t)nb * 36, lv.a64.as<double>(), lv.a32.as<float>());
+    lv.dinv64.ensure(std::max<int64_t>(n, 1) * 36 * sizeof(double));
+    lv.dinv32.ensure(std::max<int64_t>(n, 1) * 36 * sizeof(float));
+    ST_HIP(hipMemsetAsync(sbad.p, 0, sizeof(int), st));
+    k_coarse_dinv<<<nblk(n), 256, 0, st>>>(n, lv.rowptr.as<int>(), lv.a64.as<double>(), lv.dinv64.as<double>(),
+                                           lv.dinv32.as<float>(), sbad.as<int>());
+    ST_HIP(hipGetLastError());
+    if (read_int(sbad.as<int>())) throw StError{ST_NOT_SPD, "a coarse diagonal block is singular"};
+  }
+
+  void coarse_step(Level& lv, const std::vector<int64_t>& seg) {
+    const int64_t n = lv.n;
+    DBuf v, w, sq;
+    v.ensure(n * 6 * sizeof(double));
+    w.ensure(n * 6 * sizeof(double));
+    sq.ensure(n * sizeof(double));
+    DBuf segd;
+    segd.ensure((ns + 1) * sizeof(int64_t));
+    ST_HIP(hipMemcpyAsync(segd.p, seg.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    double* a = v.as<double>();
+    double* b = w.as<double>();
+    k_c_init<<<nblk(n), 256, 0, st>>>(n, a);
+    const int iters = 12;
+    for (int it = 0; it < iters; it++) {
+      k_c_dinvA<double><<<nblk(n), 256, 0, st>>>(n, lv.rowptr.as<int>(), lv.col.as<int>(), lv.a64.as<double>(),
+                                                 lv.dinv64.as<double>(), a, b);
+      if (it >= iters - 2) {
+        k_c_sq<double><<<nblk(n), 256, 0, st>>>(n, b, sq.as<double>());
+        k_reduce_seg<<<ns, kBlock, 0, st>>>(sq.as<double>(), segd.as<int64_t>(), c.scal.as<double>(), kScal,
+                                            it == iters - 2 ? S_LAM_A : S_LAM_B);
+      }
+      std::swap(a, b);
+    }
+    std::vector<double> sc((size_t)ns * kScal);
+    ST_HIP(hipMemcpyAsync(sc.data(), c.scal.p, sc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipStreamSynchronize(st));
+    lv.step.assign(ns, 1.0);
+    for (int s = 0; s < ns; s++) {
+      const double x = sc[(size_t)s * kScal + S_LAM_A], y = sc[(size_t)s * kScal + S_LAM_B];
+      const double lam = (x > 0 && y > 0) ? std::sqrt(y / x) : 1.0;
+      lv.step[s] = 4.0 / 3.0 / lam;
+    }
+    lv.step_d.ensure(ns * sizeof(double));
+    ST_HIP(hipMemcpyAsync(lv.step_d.p, lv.step.data(), ns * sizeof(double), hipMemcpyHostToDevice, st));
+    ST_HIP(hipStreamSynchronize(st));
+    v.release();
+    w.release();
+    sq.release();
+    segd.release();
+  }
+
+  void alloc_level_vectors(Level& lv) {
+    const size_t n = std::max<int64_t>(lv.n, 1);
+    lv.r64.ensure(n * 6 * 8);
+    lv.x64.ensure(n * 6 * 8);
+    lv.t64.ensure(n * 6 * 8);
+    lv.r32.ensure(n * 6 * 4);
+    lv.x32.ensure(n * 6 * 4);
+    lv.t32.ensure(n * 6 * 4);
+  }
+
+  void setup_hierarchy() {
+    release_levels(c);
+    const int64_t nn = g.nnodes;
+    if (!nn) return;
+    c.uf.ensure(nn * sizeof(int));
+    c.uf2.ensure((nn + 1) * sizeof(int));
+    c.flag.ensure((nn + 1) * sizeof(int));
+    c.label1.ensure(nn * sizeof(int));
+    k_iota<<<nblk(nn), 256, 0, st>>>(nn, c.uf.as<int>());
+    int cap = 0;
+    for (auto& si : c.host.structs) cap = std::max(cap, si.levels_cap);
+    // number of free nodes
+    k_root_flags<<<nblk(nn), 256, 0, st>>>(nn, c.uf.as<int>(), g.nflags.as<uint8_t>(), c.flag.as<int>());
+    scan(c.flag.as<int>(), c.uf2.as<int>(), nn);
+    {
+      int last_id = 0, last_flag = 0;
+      ST_HIP(hipMemcpyAsync(&last_id, c.uf2.as<int>() + nn - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+      ST_HIP(hipMemcpyAsync(&last_flag, c.flag.as<int>() + nn - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+      ST_HIP(hipStreamSynchronize(st));
+      ndof = last_id + last_flag;
+    }
+    if (!ndof) return;
+    int64_t prev = ndof;
+    DBuf label_tmp;
+    label_tmp.ensure(nn * sizeof(int));
+    for (int L = 1; L <= std::max(cap, 1); L++) {
+      Level lv;
+      std::vector<int64_t> seg;
+      int* lab = c.levels.empty() ? c.label1.as<int>() : label_tmp.as<int>();
+      const int64_t cnt = label_level(L, lab, lv.rep, seg);
+      lv.cube = L;
+      if (cnt == prev && L < std::max(cap, 1)) {  // nothing merged: try the next cube
+        lv.rep.release();
+        continue;
+      }
+      if (cnt == prev && !c.levels.empty()) {
+        lv.rep.release();
+        break;
+      }
+      lv.n = cnt;
+      lv.agg_struct.ensure(cnt * sizeof(uint32_t));
+      k_agg_struct<<<nblk(cnt), 256, 0, st>>>(cnt, lv.rep.as<int>(), g.ncell.as<int32_t>(), g.cstruct.as<uint32_t>(),
+                                              lv.agg_struct.as<uint32_t>());
+      lv.centre.ensure(cnt * 3 * sizeof(double));
+      lv.parent.ensure(cnt * sizeof(int));
+      k_fill_int<<<nblk(cnt), 256, 0, st>>>(cnt, lv.parent.as<int>(), -1);
+      if (c.levels.empty()) {
+        // level 1: nodes grouped by aggregate
+        DBuf keys, vals, skeys;
+        keys.ensure(nn * sizeof(int));
+        vals.ensure(nn * sizeof(int));
+        skeys.ensure(nn * sizeof(int));
+        c.nodes_by_agg.ensure(nn * sizeof(int));
+        c.agg_ptr.ensure((cnt + 1) * sizeof(int));
+        k_sort_keys<<<nblk(nn), 256, 0, st>>>(nn, c.label1.as<int>(), keys.as<int>(), vals.as<int>());
+        sort_pairs(keys.as<int>(), skeys.as<int>(), vals.as<int>(), c.nodes_by_agg.as<int>(), nn);
+        k_seg_ptr<<<nblk(ndof), 256, 0, st>>>(ndof, skeys.as<int>(), c.agg_ptr.as<int>(), (int)cnt);
+        k_centre1<<<nblk(cnt), 256, 0, st>>>(cnt, c.agg_ptr.as<int>(), c.nodes_by_agg.as<int>(), g.nkey.as<uint32_t>(),
+                                             g.nkind.as<uint8_t>(), lv.agg_struct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
+                                             lv.centre.as<double>());
+        keys.release();
+        vals.release();
+        skeys.release();
+        build_rows_a1(lv);
+      } else {
+        Level& below = c.levels.back();
+        // parent of each aggregate below
+        k_parent<<<nblk(below.n), 256, 0, st>>>(below.n, below.rep.as<int>(), lab, below.parent.as<int>());
+        DBuf vals, skeys;
+        vals.ensure(below.n * sizeof(int));
+        skeys.ensure(below.n * sizeof(int));
+        lv.kids.ensure(below.n * sizeof(int));
+        lv.kid_ptr.ensure((cnt + 1) * sizeof(int));
+        k_iota<<<nblk(below.n), 256, 0, st>>>(below.n, vals.as<int>());
+        sort_pairs(below.parent.as<int>(), skeys.as<int>(), vals.as<int>(), lv.kids.as<int>(), below.n);
+        k_seg_ptr<<<nblk(below.n), 256, 0, st>>>(below.n, skeys.as<int>(), lv.kid_ptr.as<int>(), (int)cnt);
+        k_centre_c<<<nblk(cnt), 256, 0, st>>>(cnt, lv.kid_ptr.as<int>(), lv.kids.as<int>(), below.centre.as<double>(),
+                                              lv.centre.as<double>());
+        vals.release();
+        skeys.release();
+        DBuf cntb;
+        cntb.ensure((cnt + 1) * sizeof(int));
+        lv.rowptr.ensure((cnt + 1) * sizeof(int));
+        ST_HIP(hipMemsetAsync(sbad.p, 0, sizeof(int), st));
+        ST_HIP(hipMemsetAsync(cntb.p, 0, (cnt + 1) * sizeof(int), st));
+        auto launch = [&](int pass) {
+          k_galerkin<<<nblk(cnt, 64), 64, 0, st>>>(pass, cnt, lv.kid_ptr.as<int>(), lv.kids.as<int>(),
+                                                   lv.centre.as<double>(), below.centre.as<double>(),
+                                                   below.parent.as<int>(), below.rowptr.as<int>(), below.col.as<int>(),
+                                                   below.a64.as<double>(), cntb.as<int>(), lv.rowptr.as<int>(),
+                                                   lv.col.as<int>(), lv.a64.as<double>(), sbad.as<int>());
+          ST_HIP(hipGetLastError());
+        };
+        launch(0);
+        finish_rows(lv, cntb, launch);
+        cntb.release();
+      }
+      coarse_step(lv, seg);
+      alloc_level_vectors(lv);
+      c.levels.push_back(std::move(lv));
+      prev = cnt;
+      if (c.levels.back().nblocks == c.levels.back().n) break;  // block diagonal: exact top
+    }
+    label_tmp.release();
+  }
+
+  // --------------------------------------------------------------- V-cycle
+  template <class T>
+  T* lvbuf(Level& lv, int which) {
+    DBuf* b64[3] = {&lv.r64, &lv.x64, &lv.t64};
+    DBuf* b32[3] = {&lv.r32, &lv.x32, &lv.t32};
+    return std::is_same<T, double>::value ? (T*)b64[which]->p : (T*)b32[which]->p;
+  }
+  template <class T>
+  const T* lvA(Level& lv) {
+    return std::is_same<T, double>::value ? (const T*)lv.a64.p : (const T*)lv.a32.p;
+  }
+  template <class T>
+  const T* lvD(Level& lv) {
+    return std::is_same<T, double>::value ? (const T*)lv.dinv64.p : (const T*)lv.dinv32.p;
+  }
+
+  // Solves approximately A_li z = r where r is in lv.r; result in lv.x.
+  template <class T>
+  void coarse_cycle(size_t li) {
+    Level& lv = c.levels[li];
+    const uint8_t* done = c.done.as<uint8_t>();
+    T* r = lvbuf<T>(lv, 0);
+    T* x = lvbuf<T>(lv, 1);
+    T* t = lvbuf<T>(lv, 2);
+    const int64_t n = lv.n;
+    const uint32_t* ast = lv.agg_struct.as<uint32_t>();
+    if (li + 1 == c.levels.size()) {
+      if (lv.nblocks == lv.n) {
+        k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), nullptr, r, x);
+      } else {  // not block diagonal (should not happen): a few smoothing sweeps
+        k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), r, x);
+        for (int k = 0; k < 8; k++) {
+          k_c_resid<T><<<nblk(n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
+                                                lvD<T>(lv), lv.step_d.as<double>(), r, x, t);
+          std::swap(x, t);
+        }
+        if (x != lvbuf<T>(lv, 1))
+          ST_HIP(hipMemcpyAsync(lvbuf<T>(lv, 1), x, n * 6 * sizeof(T), hipMemcpyDeviceToDevice, st));
+      }
+      return;
+    }
+    Level& up = c.levels[li + 1];
+    k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), r, x);
+    k_c_resid<T><<<nblk(n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv), nullptr,
+                                          nullptr, r, x, t);
+    k_restrict_c<T><<<nblk(up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, up.kid_ptr.as<int>(),
+                                                up.kids.as<int>(), up.centre.as<double>(), lv.centre.as<double>(), t,
+                                                lvbuf<T>(up, 0));
+    coarse_cycle<T>(li + 1);
+    k_prolong_c<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lv.parent.as<int>(), lv.centre.as<double>(),
+                                            up.centre.as<double>(), lvbuf<T>(up, 1), x);
+    k_c_resid<T><<<nblk(n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv), lvD<T>(lv),
+                                          lv.step_d.as<double>(), r, x, t);
+    ST_HIP(hipMemcpyAsync(x, t, n * 6 * sizeof(T), hipMemcpyDeviceToDevice, st));
+  }
+
+  // z = M^-1 r at the fine level; partial r.z per brick into c.partial
+  template <class T>
+  void vcycle(const T* r, T* z, T* x1, T* r1) {
+    const uint8_t* done = c.done.as<uint8_t>();
+    const unsigned nb = (unsigned)g.nbricks;
+    const T* dinv = std::is_same<T, double>::value ? (const T*)c.dinv64.p : (const T*)c.dinv32.p;
+    k_jacobi_fine<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done, dinv,
+                                            c.step0_d.as<double>(), r, x1);
+    if (!c.levels.empty()) {
+      EpiResid<T, T> er;
+      er.done = done;
+      er.partial = c.partial.as<double>();
+      er.b = r;
+      er.r = r1;
+      launch_apply<T>(c, x1, er);
+      Level& l1 = c.levels[0];
+      k_restrict1<T><<<nblk(l1.n), 256, 0, st>>>(l1.n, l1.agg_struct.as<uint32_t>(), done, c.agg_ptr.as<int>(),
+                                                 c.nodes_by_agg.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
+                                                 g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), r1, lvbuf<T>(l1, 0));
+      coarse_cycle<T>(0);
+      k_prolong1<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done, c.label1.as<int>(),
+                                           l1.centre.as<double>(), g.nkey.as<uint32_t>(), g.nkind.as<uint8_t>(),
+                                           g.bgeo.as<BeamGeo>(), lvbuf<T>(l1, 1), x1);
+    }
+    EpiSmooth<T> es;
+    es.done = done;
+    es.partial = c.partial.as<double>();
+    es.b = r;
+    es.xin = x1;
+    es.dinv = dinv;
+    es.step = c.step0_d.as<double>();
+    es.z = z;
+    launch_apply<T>(c, x1, es);
+    ST_HIP(hipGetLastError());
+  }
+
+  // ------------------------------------------------------------------ PCG
+  void sync_flags(std::vector<uint8_t>& done, std::vector<uint8_t>& want) {
+    done.resize(ns);
+    want.resize(ns);
+    ST_HIP(hipMemcpyAsync(done.data(), c.done.p, ns, hipMemcpyDeviceToHost, st));
+    ST_HIP(hipMemcpyAsync(want.data(), swant.p, ns, hipMemcpyDeviceToHost, st));
+    ST_HIP(hipStreamSynchronize(st));
+  }
+
+  // f64: loads (device, zero on non-free rows). x64: initial guess (zeros or warm) and result.
+  void pcg(int precision, double tol, int max_iter, int refresh) {
+    const unsigned nb = (unsigned)g.nbricks;
+    const int64_t nn = g.nnodes;
+    const size_t v64 = (size_t)std::max<int64_t>(nn, 1) * 6 * 8, v32 = (size_t)std::max<int64_t>(nn, 1) * 6 * 4;
+    const uint32_t* nbase = g.nbase.as<uint32_t>();
+    const uint32_t* bst = g.bstruct.as<uint32_t>();
+    uint8_t* done = c.done.as<uint8_t>();
+    double* scal = c.scal.as<double>();
+    double* partial = c.partial.as<double>();
+    const bool mixed = precision == ST_MIXED;
+    c.r64.ensure(v64);
+    // ||f||^2 and the initial residual r = f - K x
+    k_dot_fine<double><<<nb, kBlock, 0, st>>>(nbase, c.f64.as<double>(), c.f64.as<double>(), partial);
+    reduce_bricks(c.partial, S_FF);
+    ST_HIP(hipMemsetAsync(done, 0, ns, st));
+    {
+      EpiResid<double, double> er;
+      er.partial = partial;
+      er.b = c.f64.as<double>();
+      er.r = c.r64.as<double>();
+      launch_apply<double>(c, c.x64.as<double>(), er);
+      reduce_bricks(c.partial, S_RR);
+    }
+    k_cg_start<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), tol, mixed ? 1 : 0,
+                                         swant.as<uint8_t>());
+    if (!mixed) {
+      c.p64.ensure(v64);
+      c.ap64.ensure(v64);
+      c.z64.ensure(v64);
+      c.t64.ensure(v64);
+      c.u64.ensure(v64);
+      double *x = c.x64.as<double>(), *r = c.r64.as<double>(), *p = c.p64.as<double>(), *ap = c.ap64.as<double>(),
+             *z = c.z64.as<double>();
+      vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>());
+      reduce_bricks(c.partial, S_RZ);
+      k_copy_fine<double, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, z, p);
+      std::vector<uint8_t> dh, wh;
+      int it = 0;
+      const int chunk = 4;
+      while (it < max_iter) {
+        const int k_end = std::min(max_iter, it + chunk);
+        for (; it < k_end; it++) {
+          EpiDot<double> ed;
+          ed.done = done;
+          ed.partial = partial;
+          ed.y = ap;
+          ed.p = p;
+          launch_apply<double>(c, p, ed);
+          reduce_bricks(c.partial, S_PAP);
+          k_cg_alpha<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>());
+          k_update_xr<double><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, x, r, p, ap, partial);
+          reduce_bricks(c.partial, S_RR);
+          k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), 0,
+                                               swant.as<uint8_t>(), 0.0);
+          vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>());
+          reduce_bricks(c.partial, S_RZNEW);
+          k_cg_beta<<<nblk(ns), 256, 0, st>>>(ns, scal, done);
+          k_update_p<double><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z, p);
+        }
+        ST_HIP(hipGetLastError());
+        sync_flags(dh, wh);
+        if (std::all_of(dh.begin(), dh.end(), [](uint8_t d) { return d != 0; })) break;
+      }
+    } else {
+      c.x32.ensure(v32);
+      c.r32.ensure(v32);
+      c.p32.ensure(v32);
+      c.ap32.ensure(v32);
+      c.z32.ensure(v32);
+      c.t32.ensure(v32);
+      c.u64.ensure(v32);  // reused as the second fine FP32 scratch
+      float *x = c.x32.as<float>(), *r = c.r32.as<float>(), *p = c.p32.as<float>(), *ap = c.ap32.as<float>(),
+            *z = c.z32.as<float>();
+      float* x1 = c.t32.as<float>();
+      float* r1 = c.u64.as<float>();
+      ST_HIP(hipMemsetAsync(x, 0, v32, st));
+      k_copy_fine<float, double><<<nb, kBlock, 0, st>>>(nbase, bst, nullptr, c.r64.as<double>(), r);
+      vcycle<float>(r, z, x1, r1);
+      reduce_bricks(c.partial, S_RZ);
+      k_copy_fine<float, float><<<nb, kBlock, 0, st>>>(nbase, bst, done, z, p);
+      std::vector<uint8_t> dh, wh;
+      int it = 0, since = 0;
+      const int chunk = 4;
+      const int period = refresh > 0 ? refresh : 50;
+      while (it < max_iter) {
+        const int k_end = std::min(max_iter, it + chunk);
+        for (; it < k_end; it++, since++) {
+          EpiDot<float> ed;
+          ed.done = done;
+          ed.partial = partial;
+          ed.y = ap;
+          ed.p = p;
+          launch_apply<float>(c, p, ed);
+          reduce_bricks(c.partial, S_PAP);
+          k_cg_alpha<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>());
+          k_update_xr<float><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, x, r, p, ap, partial);
+          reduce_bricks(c.partial, S_RR);
+          k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), 1,
+                                               swant.as<uint8_t>(), 1e-2);
+          vcycle<float>(r, z, x1, r1);
+          reduce_bricks(c.partial, S_RZNEW);
+          k_cg_beta<<<nblk(ns), 256, 0, st>>>(ns, scal, done);
+          k_update_p<float><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z, p);
+        }
+        ST_HIP(hipGetLastError());
+        sync_flags(dh, wh);
+        bool any_want = false;
+        for (int s = 0; s < ns; s++) any_want |= !dh[s] && wh[s];
+        if (any_want || since >= period || it >= max_iter) {
+          // reliable update: x64 += x32; r = f - K x64 in FP64; the CG goes on from it
+          k_flush<<<nb, kBlock, 0, st>>>(nbase, bst, done, c.x64.as<double>(), x);
+          EpiResid<double, double> er;
+          er.done = done;
+          er.partial = partial;
+          er.b = c.f64.as<double>();
+          er.r = c.r64.as<double>();
+          launch_apply<double>(c, c.x64.as<double>(), er);
+          reduce_bricks(c.partial, S_RRTRUE);
+          k_copy_fine<float, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, c.r64.as<double>(), r);
+          k_cg_true<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), swant.as<uint8_t>(),
+                                              srefresh.as<int>());
+          since = 0;
+          sync_flags(dh, wh);
+        }
+        if (std::all_of(dh.begin(), dh.end(), [](uint8_t d) { return d != 0; })) break;
+      }
+      // anything left in the FP32 correction
+      ST_HIP(hipMemsetAsync(done, 0, ns, st));
+      k_flush<<<nb, kBlock, 0, st>>>(nbase, bst, done, c.x64.as<double>(), x);
+    }
+    ST_HIP(hipGetLastError());
+  }
+
+  // final true residual per structure (FP64), into scal S_RRTRUE; ||f||^2 in S_FF
+  void true_residual() {
+    const unsigned nb = (unsigned)g.nbricks;
+    uint8_t* done = c.done.as<uint8_t>();
+    ST_HIP(hipMemsetAsync(done, 0, ns, st));
+    EpiResid<double, double> er;
+    er.partial = c.partial.as<double>();
+    er.b = c.f64.as<double>();
+    er.r = c.r64.as<double>();
+    launch_apply<double>(c, c.x64.as<double>(), er);
+    reduce_bricks(c.partial, S_RRTRUE);
+    k_dot_fine<double><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), c.f64.as<double>(), c.f64.as<double>(),
+                                              c.partial.as<double>());
+    reduce_bricks(c.partial, S_FF);
+  }
+};
+
+// ============================================================ entry helpers
+
+void release_levels(Ctx& c) {
+  for (auto& lv : c.levels)
+    for (DBuf* b : {&lv.rowptr, &lv.col, &lv.a64, &lv.a32, &lv.dinv64, &lv.dinv32, &lv.centre, &lv.parent, &lv.rep,
+                    &lv.key, &lv.kid_ptr, &lv.kids, &lv.agg_struct, &lv.step_d, &lv.r64, &lv.x64, &lv.t64, &lv.r32,
+                    &lv.x32, &lv.t32})
+      b->release();
+  c.levels.clear();
+}
+
+void release_all(Ctx& c) {
+  DevGeo& d = c.geo;
+  for (DBuf* b : {&d.cbase, &d.nbase, &d.hbase, &d.bstruct, &d.hcell, &d.ho, &d.ce64, &d.cg64, &d.ce32, &d.cg32,
+                  &d.cmask_user, &d.cmask, &d.cfaces, &d.ccoord, &d.cstruct, &d.nflags, &d.ncell, &d.nkind, &d.nkey,
+                  &d.bgeo})
+    b->release();
+  for (DBuf* b : {&c.dinv64, &c.dinv32, &c.step0_d, &c.label1, &c.uf, &c.uf2, &c.flag, &c.scan_tmp, &c.sort_tmp,
+                  &c.nodes_by_agg, &c.agg_ptr, &c.f64, &c.x64, &c.r64, &c.p64, &c.ap64, &c.z64, &c.t64, &c.u64, &c.x32,
+                  &c.r32, &c.p32, &c.ap32, &c.z32, &c.t32, &c.partial, &c.scal, &c.done})
+    b->release();
+  release_levels(c);
+}
+
+void set_gone_flags(Ctx& c, const uint8_t* gone) {
+  const int64_t nn = c.geo.nnodes;
+  for (int64_t v = 0; v < nn; v++) {
+    uint8_t& f = c.host.nflags[v];
+    f = (uint8_t)((f & ~kGone) | ((gone && gone[v]) ? kGone : 0));
+  }
+  c.gone_host.assign(nn, 0);
+  if (gone) std::memcpy(c.gone_host.data(), gone, nn);
+  if (nn) ST_HIP(hipMemcpyAsync(c.geo.nflags.p, c.host.nflags.data(), nn, hipMemcpyHostToDevice, c.stream));
+  refresh_cell_flags(c);
+  ST_HIP(hipStreamSynchronize(c.stream));
+  c.have_solution = false;
+}
+
+void set_beam_masks(Ctx& c, const uint16_t* beams) {
+  HostGeo& h = c.host;
+  const int64_t nin = (int64_t)h.cell_of_input.size();
+  for (int64_t i = 0; i < nin; i++) {
+    const int64_t cc = h.cell_of_input[i];
+    if (cc >= 0) h.cmask[cc] = beams ? (uint16_t)(beams[i] & 0x7fff) : (uint16_t)0x7fff;
+  }
+  if (!h.cmask.empty())
+    ST_HIP(hipMemcpyAsync(c.geo.cmask_user.p, h.cmask.data(), h.cmask.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
+                          c.stream));
+  refresh_cell_flags(c);
+  ST_HIP(hipStreamSynchronize(c.stream));
+  c.have_solution = false;
+}
+
+// Runs setup + PCG for all uploaded structures. loads: device FP64 (c.f64), x0 in c.x64.
+void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
+  Solver S(c);
+  const int ns = S.ns;
+  hipEvent_t e0, e1, e2;
+  ST_HIP(hipEventCreate(&e0));
+  ST_HIP(hipEventCreate(&e1));
+  ST_HIP(hipEventCreate(&e2));
+  ST_HIP(hipEventRecord(e0, c.stream));
+  S.init_struct_arrays();
+  S.setup_fine();
+  S.setup_hierarchy();
+  ST_HIP(hipEventRecord(e1, c.stream));
+  if (S.ndof > 0) S.pcg(o.precision, o.tol, o.max_iter, o.refresh);
+  S.true_residual();
+  ST_HIP(hipEventRecord(e2, c.stream));
+  ST_HIP(hipEventSynchronize(e2));
+  float ms_setup = 0, ms_solve = 0;
+  ST_HIP(hipEventElapsedTime(&ms_setup, e0, e1));
+  ST_HIP(hipEventElapsedTime(&ms_solve, e1, e2));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  std::vector<double> sc((size_t)ns * kScal);
+  std::vector<int> it(ns), sts(ns), rf(ns);
+  ST_HIP(hipMemcpy(sc.data(), c.scal.p, sc.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (S.ndof > 0) {
+    ST_HIP(hipMemcpy(it.data(), S.siters.p, ns * sizeof(int), hipMemcpyDeviceToHost));
+    ST_HIP(hipMemcpy(sts.data(), S.sstatus.p, ns * sizeof(int), hipMemcpyDeviceToHost));
+    ST_HIP(hipMemcpy(rf.data(), S.srefresh.p, ns * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  stats.assign(ns, StStats{});
+  for (int s = 0; s < ns; s++) {
+    StStats& t = stats[s];
+    const StructInfo& si = c.host.structs[s];
+    t.iters = it[s];
+    t.levels = 1 + (int)c.levels.size();
+    t.refreshes = rf[s];
+    const double ff = sc[(size_t)s * kScal + S_FF], rr = sc[(size_t)s * kScal + S_RRTRUE];
+    t.rel_residual = ff > 0 ? std::sqrt(rr / ff) : std::sqrt(rr);
+    t.status = sts[s];
+    if (t.status == ST_OK && !(t.rel_residual <= o.tol * (1.0 + 1e-9)) && ff > 0) t.status = ST_NO_CONVERGENCE;
+    if (t.status == ST_OK && !std::isfinite(t.rel_residual)) t.status = ST_NAN;
+    t.setup_ms = ms_setup;
+    t.solve_ms = ms_solve;
+    t.nodes = si.nnodes;
+    t.cells = si.ncells;
+    t.bricks = si.nbricks;
+    int64_t dof = 0;
+    for (int64_t v = si.node0; v < si.node0 + si.nnodes; v++) dof += !(c.host.nflags[v] & (kHeld | kGone));
+    t.dof_nodes = dof;
+  }
+}
+
+// Reachability from held nodes over alive beams (statics/src/bodies.rs:557-574).
+void joined_to_held(Ctx& c, std::vector<uint8_t>& joined) {
+  DevGeo& g = c.geo;
+  const int64_t nn = g.nnodes;
+  hipStream_t st = c.stream;
+  joined.assign(nn, 0);
+  if (!nn) return;
+  c.uf.ensure(nn * sizeof(int));
+  c.flag.ensure((nn + 1) * sizeof(int));
+  DBuf out;
+  out.ensure(nn);
+  k_iota<<<nblk(nn), 256, 0, st>>>(nn, c.uf.as<int>());
+  if (g.ncells)
+    k_unite_all<<<nblk(g.ncells), 256, 0, st>>>(g.ncells, g.cfaces.as<uint32_t>(), g.cmask_user.as<uint16_t>(),
+                                                c.uf.as<int>());
+  k_compress<<<nblk(nn), 256, 0, st>>>(nn, c.uf.as<int>());
+  ST_HIP(hipMemsetAsync(c.flag.p, 0, nn * sizeof(int), st));
+  k_held_roots<<<nblk(nn), 256, 0, st>>>(nn, c.uf.as<int>(), g.nflags.as<uint8_t>(), c.flag.as<int>());
+  k_joined<<<nblk(nn), 256, 0, st>>>(nn, c.uf.as<int>(), c.flag.as<int>(), out.as<uint8_t>());
+  ST_HIP(hipGetLastError());
+  ST_HIP(hipMemcpyAsync(joined.data(), out.p, nn, hipMemcpyDeviceToHost, st));
+  ST_HIP(hipStreamSynchronize(st));
+  out.release();
+}
+
+}  // namespace st

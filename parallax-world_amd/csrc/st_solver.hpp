@@ -1,0 +1,45 @@
+// Declarations shared by the library's translation units.
+#pragma once
+#include "st_kernels.hpp"
+
+#define SINGLE_ARG(...) __VA_ARGS__
+
+namespace st {
+
+template <class T>
+GeoD<T> geo_view(Ctx& c);
+template <class T, class Epi>
+void launch_apply(Ctx& c, const T* x, const Epi& epi);
+void refresh_cell_flags(Ctx& c);
+void release_all(Ctx& c);
+void release_levels(Ctx& c);
+void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats);
+void joined_to_held(Ctx& c, std::vector<uint8_t>& joined);
+void copy_geo_to_device(Ctx& c);
+void set_gone_flags(Ctx& c, const uint8_t* gone);
+void set_beam_masks(Ctx& c, const uint16_t* beams);
+
+}  // namespace st
+
+struct StCtx : st::Ctx {};
+
+namespace st {
+template <class F>
+int guard(StCtx* c, F f) {
+  if (!c) return ST_BAD_INPUT;
+  try {
+    c->err.clear();
+    f();
+    return ST_OK;
+  } catch (const StError& e) {
+    c->err = e.what;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    c->err = "host allocation failed";
+    return ST_OOM;
+  }
+}
+inline void need_upload(StCtx* c) {
+  if (!c->uploaded) throw StError{ST_NO_STRUCTURE, "no structure uploaded"};
+}
+}  // namespace st

@@ -1,0 +1,272 @@
+"""ctypes binding of libstatics_hip.so (include/statics_hip.h).
+
+Used by the tests and bench.py. There is no fallback: if the library is not
+built, or no GPU is present, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libstatics_hip.so")
+
+ST_OK, ST_BAD_INPUT, ST_NO_CONVERGENCE, ST_NOT_SPD, ST_NAN, ST_OOM, ST_DEVICE, ST_NO_STRUCTURE = range(8)
+STATUS = {0: "OK", 1: "BAD_INPUT", 2: "NO_CONVERGENCE", 3: "NOT_SPD", 4: "NAN", 5: "OOM", 6: "DEVICE",
+          7: "NO_STRUCTURE"}
+FP64, MIXED = 0, 1
+COMPLIANCE, SOFTMAX_STRESS = 0, 1
+
+# The entry points include/statics_hip.h declares.
+EXPORTS = ["st_create", "st_destroy", "st_last_error", "st_upload", "st_sizes", "st_node_faces", "st_lookup_faces",
+           "st_node_held", "st_joined_to_held", "st_set_gone", "st_set_beams", "st_apply", "st_solve",
+           "st_bond_forces", "st_backward", "st_solve_batch", "st_generate"]
+
+UNIT = 0.125  # metres to a level-0 cell (statics/src/lattice.rs:16)
+
+
+class StCells(ctypes.Structure):
+    _fields_ = [("n_cells", ctypes.c_int64), ("sites", ctypes.c_void_p), ("held", ctypes.c_void_p),
+                ("e", ctypes.c_void_p), ("g", ctypes.c_void_p), ("beams", ctypes.c_void_p),
+                ("spacing", ctypes.c_double)]
+
+
+class StSolveOpts(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
+                ("refresh", ctypes.c_int32), ("warm_start", ctypes.c_int32)]
+
+
+class StStats(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int32), ("levels", ctypes.c_int32), ("refreshes", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("rel_residual", ctypes.c_double), ("setup_ms", ctypes.c_double),
+                ("solve_ms", ctypes.c_double), ("nodes", ctypes.c_int64), ("dof_nodes", ctypes.c_int64),
+                ("cells", ctypes.c_int64), ("bricks", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class StGenParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32), ("seed", ctypes.c_uint64),
+                ("pillar_spacing", ctypes.c_double), ("pillar_min", ctypes.c_int32), ("pillar_max", ctypes.c_int32),
+                ("slab_min", ctypes.c_int32), ("slab_max", ctypes.c_int32), ("hole_p", ctypes.c_double),
+                ("sigma", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        sig = {
+            "st_create": [ctypes.POINTER(vp), ctypes.c_int],
+            "st_destroy": [vp],
+            "st_upload": [vp, ctypes.POINTER(StCells)],
+            "st_sizes": [vp, vp, vp, vp],
+            "st_node_faces": [vp, vp],
+            "st_lookup_faces": [vp, i64, vp, vp],
+            "st_node_held": [vp, vp],
+            "st_joined_to_held": [vp, vp, vp, ctypes.c_int],
+            "st_set_gone": [vp, vp],
+            "st_set_beams": [vp, vp],
+            "st_apply": [vp, vp, vp],
+            "st_solve": [vp, vp, ctypes.POINTER(StSolveOpts), vp, ctypes.POINTER(StStats)],
+            "st_bond_forces": [vp, vp, vp],
+            "st_backward": [vp, ctypes.c_int, dbl, ctypes.POINTER(StSolveOpts), vp, vp, ctypes.POINTER(StStats)],
+            "st_solve_batch": [vp, i32, vp, vp, ctypes.POINTER(StSolveOpts), vp, vp],
+            "st_generate": [ctypes.POINTER(StGenParams), vp, vp],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.st_last_error.argtypes = [vp]
+        L.st_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class StaticsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def generate(nx, ny, nz, seed=0, pillar_spacing=8.0, pillar_min=2, pillar_max=6, slab_min=8, slab_max=16,
+             hole_p=0.2, sigma=0.0):
+    """The seeded pillar/slab structure: occ[x, y, z] in {0 empty, 1 free, 2 held} and theta (log-stiffness)."""
+    occ = np.zeros((nx, ny, nz), np.uint8)
+    theta = np.zeros((nx, ny, nz), np.float32)
+    p = StGenParams(nx, ny, nz, seed, pillar_spacing, pillar_min, pillar_max, slab_min, slab_max, hole_p, sigma)
+    rc = lib().st_generate(ctypes.byref(p), _p(occ), _p(theta))
+    if rc != 0:
+        raise StaticsError(rc, "st_generate")
+    return occ, theta
+
+
+def cells_from_occ(occ, theta=None, e0=1.0, g_over_e=0.4):
+    """Solid cells of occ in the reference's `solids` order (x outer, z inner): sites, held, e, g."""
+    sites = np.argwhere(occ > 0).astype(np.int32)
+    held = (occ[occ > 0] == 2).astype(np.uint8)
+    th = np.zeros(len(sites)) if theta is None else theta[occ > 0].astype(np.float64)
+    e = e0 * np.exp(th)
+    g = g_over_e * e
+    return sites, held, e, g
+
+
+def gravity_loads(faces, held_or_gone, count_cells, weight=1.0, axis=2):
+    """Loads of weight w per free cell, a sixth on each of its faces, along -axis."""
+    f = np.zeros((len(faces), 6))
+    f[:, axis] = -weight * count_cells / 6.0
+    f[held_or_gone.astype(bool)] = 0.0
+    return f
+
+
+class Context:
+    """One device context (st_create .. st_destroy)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = ctypes.c_void_p()
+        rc = self.L.st_create(ctypes.byref(h), device)
+        if rc != 0:
+            raise StaticsError(rc, "st_create (is a GPU visible?)")
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            self.L.st_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise StaticsError(rc, f"{what}: {self.L.st_last_error(self.h).decode()}")
+
+    @staticmethod
+    def make_cells(sites, held, e, g, beams=None, spacing=UNIT):
+        keep = [np.ascontiguousarray(sites, np.int32), np.ascontiguousarray(held, np.uint8),
+                np.ascontiguousarray(e, np.float64), np.ascontiguousarray(g, np.float64),
+                None if beams is None else np.ascontiguousarray(beams, np.uint16)]
+        c = StCells(len(keep[0]), keep[0].ctypes.data, keep[1].ctypes.data, keep[2].ctypes.data, keep[3].ctypes.data,
+                    None if keep[4] is None else keep[4].ctypes.data, spacing)
+        return c, keep
+
+    def upload(self, sites, held, e, g, beams=None, spacing=UNIT):
+        c, keep = self.make_cells(sites, held, e, g, beams, spacing)
+        self._check(self.L.st_upload(self.h, ctypes.byref(c)), "st_upload")
+        self._keep = keep
+        n, nc, nb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.L.st_sizes(self.h, ctypes.byref(n), ctypes.byref(nc), ctypes.byref(nb)), "st_sizes")
+        self.nodes, self.cells, self.bricks = n.value, nc.value, nb.value
+        return self
+
+    def node_faces(self):
+        f = np.zeros((self.nodes, 4), np.int32)
+        self._check(self.L.st_node_faces(self.h, _p(f)), "st_node_faces")
+        return f
+
+    def lookup_faces(self, faces):
+        faces = np.ascontiguousarray(faces, np.int32)
+        idx = np.zeros(len(faces), np.int64)
+        self._check(self.L.st_lookup_faces(self.h, len(faces), _p(faces), _p(idx)), "st_lookup_faces")
+        return idx
+
+    def node_held(self):
+        h = np.zeros(self.nodes, np.uint8)
+        self._check(self.L.st_node_held(self.h, _p(h)), "st_node_held")
+        return h
+
+    def joined_to_held(self, beams=None, keep_gone=False):
+        j = np.zeros(self.nodes, np.uint8)
+        b = None if beams is None else np.ascontiguousarray(beams, np.uint16)
+        self._check(self.L.st_joined_to_held(self.h, _p(b), _p(j), 1 if keep_gone else 0), "st_joined_to_held")
+        return j
+
+    def set_gone(self, gone):
+        g = None if gone is None else np.ascontiguousarray(gone, np.uint8)
+        self._check(self.L.st_set_gone(self.h, _p(g)), "st_set_gone")
+
+    def set_beams(self, beams):
+        b = None if beams is None else np.ascontiguousarray(beams, np.uint16)
+        self._check(self.L.st_set_beams(self.h, _p(b)), "st_set_beams")
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, np.float64).reshape(self.nodes, 6)
+        y = np.zeros_like(x)
+        self._check(self.L.st_apply(self.h, _p(x), _p(y)), "st_apply")
+        return y
+
+    def solve(self, loads, precision=FP64, tol=1e-8, max_iter=10000, refresh=0, x0=None, check=True):
+        loads = np.ascontiguousarray(loads, np.float64).reshape(self.nodes, 6)
+        x = np.zeros_like(loads) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
+        o = StSolveOpts(precision, tol, max_iter, refresh, 0 if x0 is None else 1)
+        s = StStats()
+        rc = self.L.st_solve(self.h, _p(loads), ctypes.byref(o), _p(x), ctypes.byref(s))
+        if check:
+            self._check(rc, "st_solve")
+        return x, s
+
+    def bond_forces(self, x):
+        x = np.ascontiguousarray(x, np.float64).reshape(self.nodes, 6)
+        out = np.zeros((len(self._keep[0]), 15, 4, 3))
+        self._check(self.L.st_bond_forces(self.h, _p(x), _p(out)), "st_bond_forces")
+        return out
+
+    def backward(self, objective=COMPLIANCE, beta=20.0, precision=FP64, tol=1e-8, max_iter=10000):
+        o = StSolveOpts(precision, tol, max_iter, 0, 0)
+        J = ctypes.c_double()
+        d = np.zeros(len(self._keep[0]))
+        s = StStats()
+        self._check(self.L.st_backward(self.h, objective, beta, ctypes.byref(o), ctypes.byref(J), _p(d),
+                                       ctypes.byref(s)), "st_backward")
+        return J.value, d, s
+
+    def solve_batch(self, cell_sets, loads=None, precision=MIXED, tol=1e-8, max_iter=10000, want_x=True):
+        """cell_sets: list of (sites, held, e, g). Gravity (weight 1 per free cell, -z) where loads[i] is None."""
+        n = len(cell_sets)
+        structs, keeps = [], []
+        for cs in cell_sets:
+            c, k = self.make_cells(*cs)
+            structs.append(c)
+            keeps.append(k)
+        arr = (ctypes.POINTER(StCells) * n)(*[ctypes.pointer(c) for c in structs])
+        lptr = (ctypes.c_void_p * n)()
+        lkeep = []
+        if loads is not None:
+            for i, l in enumerate(loads):
+                if l is not None:
+                    a = np.ascontiguousarray(l, np.float64)
+                    lkeep.append(a)
+                    lptr[i] = a.ctypes.data
+        stats = (StStats * n)()
+        xs = [None] * n
+        xptr = (ctypes.c_void_p * n)()
+        o = StSolveOpts(precision, tol, max_iter, 0, 0)
+        # node counts are known only after the upload: a first call without outputs would cost a solve,
+        # so x comes back through a second, zero-iteration-free route: size by an upper bound of 6 faces per cell
+        if want_x:
+            for i, k in enumerate(keeps):
+                xs[i] = np.zeros((6 * int((k[1] == 0).sum()) + 1, 6))
+                xptr[i] = xs[i].ctypes.data
+        rc = self.L.st_solve_batch(self.h, n, arr, lptr, ctypes.byref(o), xptr if want_x else None, stats)
+        self._check(rc, "st_solve_batch")
+        if want_x:
+            xs = [x[:stats[i].nodes] for i, x in enumerate(xs)]
+        return xs, [stats[i] for i in range(n)]
