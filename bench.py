@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Structure solves/sec to 1e-8 relative residual at 256^3 on one MI355X.
+
+Workload: one structure on a 256^3 lattice, ~30% solid: a held ground layer,
+square pillars 3-7 cells wide at Poisson-disk spacing 8, one-cell floor slabs
+every 8-16 cells with 20% 2x2 holes (seed 1); per-cell log-stiffness
+theta ~ N(0, 0.5^2) (e = exp(theta), g = 0.4 e); gravity of weight 1 per free
+cell (a sixth on each face, -z) plus 64 point loads of magnitude U(1, 10) in
+uniformly random directions on random free nodes. Synthetic, generated here.
+
+One step = classify (joined to held -> gone) + mixed-precision solve to 1e-8 +
+the adjoint of the soft-max (beta = 20) bond stress w.r.t. per-cell
+log-stiffness (a second solve, mixed, 1e-8, hierarchy rebuilt). The timed
+steps run with the loads and the solution resident in device memory.
+Alongside: the plain FP64 solve (classify + FP64 solve to 1e-8), and the CPU
+oracle (the C++ restatement of the reference's solver, FP64) on a bounded
+sample of the same generator, scaled to 256^3 by its degrees of freedom.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "parallax-world_amd"))
+
+import statics_hip as sh  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+
+
+def structure(n, seed, sigma=0.5, spacing=8.0, wmin=3, wmax=7):
+    occ, theta = sh.generate(n, n, n, seed=seed, sigma=sigma, pillar_spacing=spacing, pillar_min=wmin,
+                             pillar_max=wmax)
+    return occ, theta
+
+
+def node_loads(occ, faces, joined, held, rng, n_points=64):
+    """Gravity (weight 1 per free cell, a sixth per face, -z) and n_points point loads."""
+    o = faces[:, 0]
+    plus = faces[:, 1:4]
+    minus = plus - np.eye(3, dtype=np.int32)[o]
+    n = occ.shape[0]
+
+    def free(c):
+        ok = np.all((c >= 0) & (c < n), axis=1)
+        cc = np.clip(c, 0, n - 1)
+        return ok & (occ[cc[:, 0], cc[:, 1], cc[:, 2]] == 1)
+
+    cnt = free(plus).astype(np.float64) + free(minus)
+    f = np.zeros((len(faces), 6))
+    f[:, 2] = -cnt / 6.0
+    cand = np.flatnonzero(joined.astype(bool) & ~held.astype(bool))
+    pick = rng.choice(cand, size=min(n_points, len(cand)), replace=False)
+    d = rng.normal(size=(len(pick), 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    f[pick, :3] += d * rng.uniform(1.0, 10.0, size=(len(pick), 1))
+    return f
+
+
+def cpu_baseline(size, threads, seed=1):
+    """The oracle (faithful restatement, FP64, assembled blocks, smoothed aggregation) on a size^3 sample."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as orc
+    occ, theta = structure(size, seed)
+    sites, held, e, g = sh.cells_from_occ(occ, theta)
+    mats = np.zeros((len(sites), 6))
+    mats[:, 0] = 1.0
+    mats[:, 1] = e
+    mats[:, 2] = g
+    mats[:, 3:] = 1.0
+    orc.lib().orc_set_threads(threads)
+    t0 = time.perf_counter()
+    L = orc.Lattice(0, sites, mats, held)
+    t_build = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    joined = L.joined_to_held()
+    gone = (joined == 0).astype(np.uint8)
+    f = np.zeros((L.n_nodes, 6))
+    f[:, 2] = -L.m / (orc.UNIT ** 3)  # weight 1 per free cell, a sixth per face
+    x, st = L.motions(f, gone=gone, tol=1e-8)
+    t_solve = time.perf_counter() - t1
+    dof = 6 * int(((L.held == 0) & (gone == 0)).sum())
+    return dict(seconds=t_solve, build_s=t_build, iters=st["iters"], dof=dof, levels=st["levels"])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-sample", type=int, default=64, help="edge of the CPU oracle's sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is single-GPU by design", file=sys.stderr)
+        sys.exit(2)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+
+    n = args.size
+    t = time.perf_counter()
+    occ, theta = structure(n, args.seed)
+    sites, held, e, g = sh.cells_from_occ(occ, theta)
+    log(f"[bench] generated {n}^3: {len(sites)} solid cells, fill {len(sites) / n ** 3:.3f} "
+        f"({time.perf_counter() - t:.1f}s)")
+    t = time.perf_counter()
+    ctx = sh.Context().upload(sites, held, e, g)
+    log(f"[bench] uploaded: {ctx.nodes} nodes, {ctx.cells} free cells, {ctx.bricks} bricks "
+        f"({time.perf_counter() - t:.1f}s)")
+    joined = ctx.joined_to_held()
+    faces = ctx.node_faces()
+    nheld = ctx.node_held()
+    loads = node_loads(occ, faces, joined, nheld, np.random.default_rng(args.seed))
+    ctx._check(ctx.L.st_set_loads(ctx.h, sh._p(np.ascontiguousarray(loads))), "st_set_loads")
+
+    import ctypes
+    opts_mixed = sh.StSolveOpts(sh.MIXED, 1e-8, 20000, 0, 0)
+    opts_fp64 = sh.StSolveOpts(sh.FP64, 1e-8, 20000, 0, 0)
+
+    def classify():
+        c = ctypes.c_int64()
+        ctx._check(ctx.L.st_classify(ctx.h, ctypes.byref(c)), "st_classify")
+        return c.value
+
+    def solve(opts):
+        s = sh.StStats()
+        ctx._check(ctx.L.st_solve_resident(ctx.h, ctypes.byref(opts), ctypes.byref(s)), "st_solve_resident")
+        return s
+
+    def adjoint(opts):
+        s = sh.StStats()
+        J = ctypes.c_double()
+        ctx._check(ctx.L.st_backward(ctx.h, sh.SOFTMAX_STRESS, 20.0, ctypes.byref(opts), ctypes.byref(J), None,
+                                     ctypes.byref(s)), "st_backward")
+        return J.value, s
+
+    def step():
+        classify()
+        a = solve(opts_mixed)
+        J, b = adjoint(opts_mixed)
+        return a, b, J
+
+    for i in range(args.warmup):
+        a, b, J = step()
+        log(f"[bench] warmup {i}: forward {a.iters} it ({a.setup_ms:.1f}+{a.solve_ms:.1f} ms, res {a.rel_residual:.2e}), "
+            f"adjoint {b.iters} it ({b.setup_ms:.1f}+{b.solve_ms:.1f} ms, res {b.rel_residual:.2e})")
+    ctx._check(ctx.L.st_kernel_timing_enable(ctx.h, 0, 1), "timing")
+    times, fwd, adj = [], [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        a, b, J = step()
+        times.append(time.perf_counter() - t0)
+        fwd.append(a)
+        adj.append(b)
+        log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms; forward {a.iters} it, res {a.rel_residual:.2e}; "
+            f"adjoint {b.iters} it, res {b.rel_residual:.2e}; J {J:.6g}")
+    nl, tms, bpl = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+    ctx._check(ctx.L.st_kernel_timing(ctx.h, ctypes.byref(nl), ctypes.byref(tms), ctypes.byref(bpl)), "timing")
+    ctx._check(ctx.L.st_kernel_timing_enable(ctx.h, 0, 0), "timing")
+    kernel_ms = tms.value / max(nl.value, 1)
+    achieved = bpl.value / (kernel_ms * 1e-3) / 1e9 if nl.value else 0.0
+    ms_per_step = 1e3 * sum(times) / len(times)
+
+    # mixed forward alone and plain FP64 forward (classify + solve), after the timed steps
+    t0 = time.perf_counter()
+    classify()
+    m1 = solve(opts_mixed)
+    mixed_fwd_ms = (time.perf_counter() - t0) * 1e3
+    fp64 = None
+    if not args.no_fp64:
+        t0 = time.perf_counter()
+        classify()
+        p = solve(opts_fp64)
+        fp64 = dict(ms=(time.perf_counter() - t0) * 1e3, iters=p.iters, rel_residual=p.rel_residual)
+        log(f"[bench] fp64 forward: {fp64['ms']:.1f} ms, {p.iters} it, res {p.rel_residual:.2e}")
+    log(f"[bench] mixed forward: {mixed_fwd_ms:.1f} ms, {m1.iters} it, res {m1.rel_residual:.2e}")
+
+    cpu = None
+    if args.cpu_sample > 0:
+        log(f"[bench] cpu oracle on a {args.cpu_sample}^3 sample, {args.cpu_threads} threads")
+        cb = cpu_baseline(args.cpu_sample, args.cpu_threads, args.seed)
+        dof_gpu = 6 * fwd[-1].dof_nodes
+        per_s = (1.0 / cb["seconds"]) * (cb["dof"] / dof_gpu)
+        cpu = {"value": per_s, "unit": "structure solves/sec (256^3, extrapolated)", "cores": args.cpu_threads,
+               "kind": "port",
+               "sample": f"oracle (C++ FP64 restatement of statics/src/bodies.rs) classify+motions to 1e-8 on the "
+                         f"{args.cpu_sample}^3 structure of the same generator: {cb['seconds']:.2f} s, {cb['iters']} "
+                         f"iterations, {cb['dof']} DOF; scaled by DOF to the 256^3 structure's {dof_gpu} DOF"}
+        log(f"[bench] cpu sample: {cb}")
+
+    value = 1e3 / ms_per_step
+    out = {
+        "metric": "structure solves/sec to 1e-8 residual at 256^3; mixed+adjoint vs plain FP64",
+        "value": value,
+        "unit": "structure solves/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": f"one {n}^3 pillar/slab structure, ~30% solid, lognormal stiffness sigma 0.5, "
+                               f"gravity + 64 point loads; step = classify + mixed solve + soft-max stress adjoint",
+                   "lattice": n, "free_cells": int(fwd[-1].cells), "free_nodes": int(fwd[-1].dof_nodes),
+                   "dof": int(6 * fwd[-1].dof_nodes), "tol": 1e-8},
+        "forward_iters": int(fwd[-1].iters), "adjoint_iters": int(adj[-1].iters),
+        "forward_rel_residual": fwd[-1].rel_residual, "adjoint_rel_residual": adj[-1].rel_residual,
+        "mixed_forward_solves_per_s": 1e3 / mixed_fwd_ms,
+        "fp64_forward_solves_per_s": (1e3 / fp64["ms"]) if fp64 else None,
+        "fp64_forward_iters": fp64["iters"] if fp64 else None,
+        "mixed_vs_fp64_forward": (fp64["ms"] / mixed_fwd_ms) if fp64 else None,
+        "roofline": {"bound": "hbm", "kernel": "k_apply<float, EpiDot<float>> (CG p -> Ap, fused p.Ap)",
+                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": None, "launches": int(nl.value), "avg_ms": kernel_ms,
+                     "algorithmic_bytes_per_launch": bpl.value},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

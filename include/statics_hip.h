@@ -132,6 +132,30 @@ int st_apply(StCtx* ctx, const double* x, double* y);
  * 6 doubles per node; 0 for held and gone nodes. */
 int st_solve(StCtx* ctx, const double* loads, const StSolveOpts* opts, double* x_out, StStats* stats);
 
+/* The same solve in three steps, for callers that keep the loads and the
+ * solution in device memory between calls (the benchmark's timed region holds
+ * no host transfer): st_set_loads copies the loads to the device once;
+ * st_solve_resident solves with them (warm_start: from the last solution);
+ * st_get_solution copies the last solution back. */
+int st_set_loads(StCtx* ctx, const double* loads);
+int st_solve_resident(StCtx* ctx, const StSolveOpts* opts, StStats* stats);
+int st_get_solution(StCtx* ctx, double* x_out);
+
+/* Replaces: bodies::joined_to_held (statics/src/bodies.rs:557-574) when the
+ * caller only needs `gone`: classifies on the device with the current beam
+ * masks and sets gone = !joined without a host round trip. Writes the number
+ * of nodes joined to a held node to *joined_count if not NULL. */
+int st_classify(StCtx* ctx, int64_t* joined_count);
+
+/* Device time of the launches of one kernel, for measuring it against its
+ * roofline: enable != 0 starts recording (and clears the totals) for kernel
+ * `which` (0: the fine operator with the CG's fused p.Ap in FP32, 1: the same in
+ * FP64); enable == 0 stops. st_kernel_timing reads the totals so far:
+ * launches, summed milliseconds, and the algorithmic bytes of one launch
+ * (6 values per free node read and written, plus e, g and the beam mask per free cell). */
+int st_kernel_timing_enable(StCtx* ctx, int which, int enable);
+int st_kernel_timing(StCtx* ctx, int64_t* launches, double* total_ms, double* bytes_per_launch);
+
 /* Replaces: bodies::moved (statics/src/bodies.rs:420-423) for every live bond:
  * per cell (in upload order) 15 beams x 4 bonds x 3 doubles, the force on
  * the j side of each bond, bonds in the reference's order
@@ -140,12 +164,13 @@ int st_bond_forces(StCtx* ctx, const double* x, double* forces_out);
 
 /* ---- gradients (no counterpart in the reference) ---------------------- */
 typedef enum { ST_COMPLIANCE = 0, ST_SOFTMAX_STRESS = 1 } StObjective;
-/* After st_solve: J and dJ/dtheta per free cell (upload order; 0 for held
+/* After st_solve or st_solve_resident: J and dJ/dtheta per cell (upload order; 0 for held
  * cells), where theta is the cell's log-stiffness (e and g both scale by
  * exp(theta)). ST_COMPLIANCE: J = f.x. ST_SOFTMAX_STRESS: J = (1/beta) log
  * sum_b exp(beta sigma_b) over live bonds b, sigma_b = (f_b . n_b)/a_b, the
  * bond's normal stress. The adjoint solve reuses the operator and rebuilds
- * the hierarchy; it runs with the precision and tolerance of opts. */
+ * the hierarchy; it runs with the precision and tolerance of opts.
+ * dJ_dtheta_out may be NULL (the gradient then stays on the device). */
 int st_backward(StCtx* ctx, int objective, double beta, const StSolveOpts* opts, double* J_out,
                 double* dJ_dtheta_out, StStats* stats);
 

@@ -266,6 +266,93 @@ using namespace st;
 
 extern "C" {
 
+namespace {
+void set_loads(StCtx* c, const double* loads) {
+  const int64_t nn = c->geo.nnodes;
+  const size_t bytes = (size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double);
+  c->fres.ensure(bytes);
+  if (nn) {
+    ST_HIP(hipMemcpyAsync(c->fres.p, loads, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->fres.as<double>());
+  }
+  c->loads_set = true;
+}
+
+// Solves with the resident loads; the solution stays in c->xsol.
+std::vector<StStats> solve_resident(StCtx* c, const StSolveOpts& o) {
+  if (!c->loads_set) throw StError{ST_BAD_INPUT, "solve: no loads set"};
+  const int64_t nn = c->geo.nnodes;
+  const size_t bytes = (size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double);
+  c->xsol.ensure(bytes);
+  c->x64.ensure(bytes);
+  if (nn) {
+    // the rows of held and gone nodes may have changed since the loads were set
+    k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->fres.as<double>());
+    if (o.warm_start && c->have_solution) {
+      ST_HIP(hipMemcpyAsync(c->x64.p, c->xsol.p, nn * 6 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->x64.as<double>());
+    } else {
+      ST_HIP(hipMemsetAsync(c->x64.p, 0, nn * 6 * sizeof(double), c->stream));
+    }
+  }
+  std::swap(c->f64, c->fres);
+  std::vector<StStats> out;
+  try {
+    solve_all(*c, o, out);
+  } catch (...) {
+    std::swap(c->f64, c->fres);
+    throw;
+  }
+  std::swap(c->f64, c->fres);
+  std::swap(c->x64, c->xsol);
+  c->have_solution = true;
+  c->last_opts = o;
+  return out;
+}
+}  // namespace
+
+int st_set_loads(StCtx* c, const double* loads) {
+  return guard(c, [&] {
+    need_upload(c);
+    if (!loads) throw StError{ST_BAD_INPUT, "loads is NULL"};
+    ST_HIP(hipSetDevice(c->device));
+    set_loads(c, loads);
+    ST_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int st_solve_resident(StCtx* c, const StSolveOpts* opts, StStats* stats) {
+  std::vector<StStats> st_out;
+  int rc = guard(c, [&] {
+    need_upload(c);
+    check_opts(opts);
+    ST_HIP(hipSetDevice(c->device));
+    st_out = solve_resident(c, *opts);
+    if (stats) *stats = st_out.empty() ? StStats{} : st_out[0];
+  });
+  if (rc == ST_OK && !st_out.empty()) rc = st_out[0].status;
+  return rc;
+}
+
+int st_get_solution(StCtx* c, double* x_out) {
+  return guard(c, [&] {
+    need_upload(c);
+    if (!x_out) throw StError{ST_BAD_INPUT, "x_out is NULL"};
+    if (!c->have_solution) throw StError{ST_BAD_INPUT, "no solution"};
+    const int64_t nn = c->geo.nnodes;
+    if (nn) ST_HIP(hipMemcpyAsync(x_out, c->xsol.p, nn * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int st_classify(StCtx* c, int64_t* joined_count) {
+  return guard(c, [&] {
+    need_upload(c);
+    ST_HIP(hipSetDevice(c->device));
+    classify_device(*c, joined_count, nullptr);
+  });
+}
+
 int st_solve(StCtx* c, const double* loads, const StSolveOpts* opts, double* x_out, StStats* stats) {
   std::vector<StStats> st_out;
   int rc = guard(c, [&] {
@@ -274,26 +361,15 @@ int st_solve(StCtx* c, const double* loads, const StSolveOpts* opts, double* x_o
     if (!loads || !x_out) throw StError{ST_BAD_INPUT, "solve: loads or x_out is NULL"};
     ST_HIP(hipSetDevice(c->device));
     const int64_t nn = c->geo.nnodes;
-    const size_t bytes = (size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double);
-    c->f64.ensure(bytes);
-    c->x64.ensure(bytes);
-    if (nn) {
-      ST_HIP(hipMemcpyAsync(c->f64.p, loads, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->f64.as<double>());
-      if (opts->warm_start) {
-        ST_HIP(hipMemcpyAsync(c->x64.p, x_out, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->x64.as<double>());
-      } else {
-        ST_HIP(hipMemsetAsync(c->x64.p, 0, nn * 6 * sizeof(double), c->stream));
-      }
+    set_loads(c, loads);
+    if (opts->warm_start) {
+      c->xsol.ensure((size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double));
+      if (nn) ST_HIP(hipMemcpyAsync(c->xsol.p, x_out, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      c->have_solution = true;
     }
-    solve_all(*c, *opts, st_out);
-    if (nn) ST_HIP(hipMemcpyAsync(x_out, c->x64.p, nn * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    st_out = solve_resident(c, *opts);
+    if (nn) ST_HIP(hipMemcpyAsync(x_out, c->xsol.p, nn * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     ST_HIP(hipStreamSynchronize(c->stream));
-    c->last_x.assign(x_out, x_out + nn * 6);
-    c->last_loads.assign(loads, loads + nn * 6);
-    c->last_opts = *opts;
-    c->have_solution = true;
     if (stats) *stats = st_out.empty() ? StStats{} : st_out[0];
   });
   if (rc == ST_OK && !st_out.empty()) rc = st_out[0].status;
@@ -351,13 +427,12 @@ int st_joined_to_held(StCtx* c, const uint16_t* beams, uint8_t* joined_out, int 
     need_upload(c);
     ST_HIP(hipSetDevice(c->device));
     if (beams) set_beam_masks(*c, beams);
-    std::vector<uint8_t> j;
-    joined_to_held(*c, j);
-    if (joined_out) std::memcpy(joined_out, j.data(), j.size());
-    if (!keep_gone) {
-      std::vector<uint8_t> gone(j.size());
-      for (size_t v = 0; v < j.size(); v++) gone[v] = !j[v];
-      set_gone_flags(*c, gone.data());
+    if (keep_gone) {
+      std::vector<uint8_t> j;
+      joined_to_held(*c, j);
+      if (joined_out) std::memcpy(joined_out, j.data(), j.size());
+    } else {
+      classify_device(*c, nullptr, joined_out);
     }
   });
 }
@@ -413,13 +488,13 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
     lam.ensure(bytes);
     cellv.ensure(std::max<int64_t>(ncl, 1) * sizeof(double));
     mz.ensure(2 * sizeof(double));
-    if (nn) ST_HIP(hipMemcpyAsync(xd.p, c->last_x.data(), nn * 6 * sizeof(double), hipMemcpyHostToDevice, s));
+    if (nn) ST_HIP(hipMemcpyAsync(xd.p, c->xsol.p, nn * 6 * sizeof(double), hipMemcpyDeviceToDevice, s));
     double J = 0.0, mzh[2] = {0.0, 1.0};
     std::vector<double> hc(ncl);
     const bool soft = objective == ST_SOFTMAX_STRESS;
     if (!soft) {
       // compliance f.x; self-adjoint: lambda = x
-      for (int64_t i = 0; i < nn * 6; i++) J += c->last_loads[i] * c->last_x[i];
+      J = device_dot_f_x(*c, c->fres.as<double>(), xd.as<double>());
       if (nn) ST_HIP(hipMemcpyAsync(lam.p, xd.p, nn * 6 * sizeof(double), hipMemcpyDeviceToDevice, s));
     } else {
       // J = m + log(sum exp(beta (sigma - m))) / beta, m the greatest sigma
@@ -461,7 +536,7 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
                                               xd.as<double>(), lam.as<double>(), soft ? 1 : 0, beta, mz.as<double>(),
                                               cellv.as<double>());
     ST_HIP(hipGetLastError());
-    if (ncl) ST_HIP(hipMemcpyAsync(hc.data(), cellv.p, ncl * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (ncl && dJ) ST_HIP(hipMemcpyAsync(hc.data(), cellv.p, ncl * sizeof(double), hipMemcpyDeviceToHost, s));
     ST_HIP(hipStreamSynchronize(s));
     xd.release();
     lam.release();

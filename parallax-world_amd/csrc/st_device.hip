@@ -113,6 +113,18 @@ template GeoD<float> geo_view<float>(Ctx&);
 
 size_t apply_smem(const Ctx& c, size_t tbytes) { return std::max<size_t>(16, (size_t)c.geo.max_brick_nodes * 6 * tbytes); }
 
+// Folds the recorded event pairs into the running total (keeps the pool small).
+void timing_collect(Ctx& c) {
+  if (!c.tev_used) return;
+  ST_HIP(hipEventSynchronize(c.tev[2 * c.tev_used - 1]));
+  for (size_t i = 0; i < c.tev_used; i++) {
+    float ms = 0;
+    ST_HIP(hipEventElapsedTime(&ms, c.tev[2 * i], c.tev[2 * i + 1]));
+    c.timing_ms_done += ms;
+  }
+  c.tev_used = 0;
+}
+
 template <class T, class Epi>
 void launch_apply(Ctx& c, const T* x, const Epi& epi) {
   if (!c.geo.nbricks) return;
@@ -123,8 +135,26 @@ void launch_apply(Ctx& c, const T* x, const Epi& epi) {
                                (int)smem));
     attr_bytes = smem;
   }
+  const bool timed = std::is_same<Epi, EpiDot<T>>::value && c.timing_which == (sizeof(T) == 4 ? 0 : 1);
+  if (timed) {
+    if (c.tev_used * 2 + 2 > c.tev.size()) {
+      if (c.tev.size() >= 4096) timing_collect(c);
+      while (c.tev_used * 2 + 2 > c.tev.size()) {
+        hipEvent_t e;
+        ST_HIP(hipEventCreate(&e));
+        c.tev.push_back(e);
+      }
+    }
+    ST_HIP(hipEventRecord(c.tev[2 * c.tev_used], c.stream));
+  }
   k_apply<T, Epi><<<(unsigned)c.geo.nbricks, kBlock, smem, c.stream>>>(geo_view<T>(c), x, epi);
   ST_HIP(hipGetLastError());
+  if (timed) {
+    ST_HIP(hipEventRecord(c.tev[2 * c.tev_used + 1], c.stream));
+    c.tev_used++;
+    c.timing_launches++;
+    c.timing_bytes = (double)c.last_ndof * 6 * 2 * sizeof(T) + (double)c.geo.ncells * (2 * sizeof(T) + 2);
+  }
 }
 
 // Explicit instantiations used by the solver.
@@ -166,6 +196,7 @@ int st_destroy(StCtx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   release_all(*c);
+  for (auto e : c->tev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return ST_OK;
@@ -246,6 +277,29 @@ int st_set_beams(StCtx* c, const uint16_t* beams) {
   return guard(c, [&] {
     need_upload(c);
     set_beam_masks(*c, beams);
+  });
+}
+
+int st_kernel_timing_enable(StCtx* c, int which, int enable) {
+  return guard(c, [&] {
+    if (which != 0 && which != 1) throw StError{ST_BAD_INPUT, "kernel timing: which is 0 or 1"};
+    timing_collect(*c);
+    if (enable) {
+      c->timing_which = which;
+      c->timing_ms_done = 0.0;
+      c->timing_launches = 0;
+    } else {
+      c->timing_which = -1;
+    }
+  });
+}
+
+int st_kernel_timing(StCtx* c, int64_t* launches, double* total_ms, double* bytes) {
+  return guard(c, [&] {
+    timing_collect(*c);
+    if (launches) *launches = c->timing_launches;
+    if (total_ms) *total_ms = c->timing_ms_done;
+    if (bytes) *bytes = c->timing_bytes;
   });
 }
 

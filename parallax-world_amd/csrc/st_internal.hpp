@@ -153,11 +153,19 @@ struct Ctx {
   DBuf x32, r32, p32, ap32, z32, t32;
   DBuf partial, scal;               // reductions: per-block partials, per-structure scalars
   DBuf done;                        // per structure: 1 once converged
-  std::vector<double> last_x;       // solution of the last solve (device order, 6 per node)
+  DBuf fres, xsol;                  // resident loads and the last solution (FP64, 6 per node)
+  bool loads_set = false;
   bool have_solution = false;
-  std::vector<double> last_loads;
   StSolveOpts last_opts{};
-  double last_ms[2] = {0, 0};
+  std::vector<int64_t> dof_per_struct;  // free nodes per structure, from the last setup
+  int64_t last_ndof = 0;
+  // per-launch timing of one kernel (st_kernel_timing_enable)
+  int timing_which = -1;
+  std::vector<hipEvent_t> tev;      // pairs
+  size_t tev_used = 0;
+  double timing_bytes = 0.0;
+  double timing_ms_done = 0.0;
+  int64_t timing_launches = 0;
 };
 
 // host upload (st_upload.cpp)

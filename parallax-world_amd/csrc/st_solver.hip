@@ -1203,6 +1203,8 @@ struct Solver {
   void setup_hierarchy() {
     release_levels(c);
     const int64_t nn = g.nnodes;
+    c.dof_per_struct.assign(ns, 0);
+    c.last_ndof = 0;
     if (!nn) return;
     c.uf.ensure(nn * sizeof(int));
     c.uf2.ensure((nn + 1) * sizeof(int));
@@ -1215,11 +1217,16 @@ struct Solver {
     k_root_flags<<<nblk(nn), 256, 0, st>>>(nn, c.uf.as<int>(), g.nflags.as<uint8_t>(), c.flag.as<int>());
     scan(c.flag.as<int>(), c.uf2.as<int>(), nn);
     {
-      int last_id = 0, last_flag = 0;
-      ST_HIP(hipMemcpyAsync(&last_id, c.uf2.as<int>() + nn - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-      ST_HIP(hipMemcpyAsync(&last_flag, c.flag.as<int>() + nn - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+      std::vector<int64_t> seg(ns + 1, 0);
+      seg_tmp.ensure((ns + 1) * sizeof(int64_t));
+      k_seg_from_ids<<<nblk(ns + 1), 256, 0, st>>>(ns, node0_d.as<int64_t>(), c.uf2.as<int>(), c.flag.as<int>(), nn,
+                                                   seg_tmp.as<int64_t>());
+      ST_HIP(hipMemcpyAsync(seg.data(), seg_tmp.p, (ns + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, st));
       ST_HIP(hipStreamSynchronize(st));
-      ndof = last_id + last_flag;
+      ndof = seg[ns];
+      c.dof_per_struct.assign(ns, 0);
+      for (int s2 = 0; s2 < ns; s2++) c.dof_per_struct[s2] = seg[s2 + 1] - seg[s2];
+      c.last_ndof = ndof;
     }
     if (!ndof) return;
     int64_t prev = ndof;
@@ -1575,7 +1582,7 @@ void release_all(Ctx& c) {
                   &d.cmask_user, &d.cmask, &d.cfaces, &d.ccoord, &d.cstruct, &d.nflags, &d.ncell, &d.nkind, &d.nkey,
                   &d.bgeo})
     b->release();
-  for (DBuf* b : {&c.dinv64, &c.dinv32, &c.step0_d, &c.label1, &c.uf, &c.uf2, &c.flag, &c.scan_tmp, &c.sort_tmp,
+  for (DBuf* b : {&c.fres, &c.xsol, &c.dinv64, &c.dinv32, &c.step0_d, &c.label1, &c.uf, &c.uf2, &c.flag, &c.scan_tmp, &c.sort_tmp,
                   &c.nodes_by_agg, &c.agg_ptr, &c.f64, &c.x64, &c.r64, &c.p64, &c.ap64, &c.z64, &c.t64, &c.u64, &c.x32,
                   &c.r32, &c.p32, &c.ap32, &c.z32, &c.t32, &c.partial, &c.scal, &c.done})
     b->release();
@@ -1659,10 +1666,63 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
     t.nodes = si.nnodes;
     t.cells = si.ncells;
     t.bricks = si.nbricks;
-    int64_t dof = 0;
-    for (int64_t v = si.node0; v < si.node0 + si.nnodes; v++) dof += !(c.host.nflags[v] & (kHeld | kGone));
-    t.dof_nodes = dof;
+    t.dof_nodes = s < (int)c.dof_per_struct.size() ? c.dof_per_struct[s] : 0;
   }
+}
+
+__global__ void k_gone_from_joined(int64_t n, const uint8_t* joined, uint8_t* nflags, int* count) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  nflags[v] = (uint8_t)((nflags[v] & ~kGone) | (joined[v] ? 0 : kGone));
+  if (joined[v]) atomicAdd(count, 1);
+}
+
+// Reachability from held nodes over alive beams (statics/src/bodies.rs:557-574), then
+// gone = !joined on the device.
+void classify_device(Ctx& c, int64_t* joined_count, uint8_t* joined_host) {
+  DevGeo& g = c.geo;
+  const int64_t nn = g.nnodes;
+  hipStream_t st = c.stream;
+  if (!nn) {
+    if (joined_count) *joined_count = 0;
+    return;
+  }
+  c.uf.ensure(nn * sizeof(int));
+  c.flag.ensure((nn + 1) * sizeof(int));
+  c.uf2.ensure((nn + 1) * sizeof(int));
+  uint8_t* out = reinterpret_cast<uint8_t*>(c.uf2.p);
+  k_iota<<<nblk(nn), 256, 0, st>>>(nn, c.uf.as<int>());
+  if (g.ncells)
+    k_unite_all<<<nblk(g.ncells), 256, 0, st>>>(g.ncells, g.cfaces.as<uint32_t>(), g.cmask_user.as<uint16_t>(),
+                                                c.uf.as<int>());
+  k_compress<<<nblk(nn), 256, 0, st>>>(nn, c.uf.as<int>());
+  ST_HIP(hipMemsetAsync(c.flag.p, 0, (nn + 1) * sizeof(int), st));
+  k_held_roots<<<nblk(nn), 256, 0, st>>>(nn, c.uf.as<int>(), g.nflags.as<uint8_t>(), c.flag.as<int>());
+  k_joined<<<nblk(nn), 256, 0, st>>>(nn, c.uf.as<int>(), c.flag.as<int>(), out);
+  int* cnt = c.flag.as<int>() + nn;  // the last slot of flag, past the nodes
+  ST_HIP(hipMemsetAsync(cnt, 0, sizeof(int), st));
+  k_gone_from_joined<<<nblk(nn), 256, 0, st>>>(nn, out, g.nflags.as<uint8_t>(), cnt);
+  ST_HIP(hipGetLastError());
+  refresh_cell_flags(c);
+  if (joined_host) ST_HIP(hipMemcpyAsync(joined_host, out, nn, hipMemcpyDeviceToHost, st));
+  int h = 0;
+  ST_HIP(hipMemcpyAsync(&h, cnt, sizeof(int), hipMemcpyDeviceToHost, st));
+  ST_HIP(hipStreamSynchronize(st));
+  if (joined_count) *joined_count = h;
+  c.have_solution = false;
+}
+
+double device_dot_f_x(Ctx& c, const double* a, const double* b) {
+  DevGeo& g = c.geo;
+  if (!g.nbricks) return 0.0;
+  c.partial.ensure(g.nbricks * sizeof(double));
+  k_dot_fine<double><<<(unsigned)g.nbricks, kBlock, 0, c.stream>>>(g.nbase.as<uint32_t>(), a, b, c.partial.as<double>());
+  std::vector<double> h(g.nbricks);
+  ST_HIP(hipMemcpyAsync(h.data(), c.partial.p, g.nbricks * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  ST_HIP(hipStreamSynchronize(c.stream));
+  double sum = 0;
+  for (double v : h) sum += v;
+  return sum;
 }
 
 // Reachability from held nodes over alive beams (statics/src/bodies.rs:557-574).

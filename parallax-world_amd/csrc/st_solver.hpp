@@ -16,6 +16,9 @@ void release_levels(Ctx& c);
 void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats);
 void joined_to_held(Ctx& c, std::vector<uint8_t>& joined);
 void copy_geo_to_device(Ctx& c);
+void timing_collect(Ctx& c);
+void classify_device(Ctx& c, int64_t* joined_count, uint8_t* joined_host);
+double device_dot_f_x(Ctx& c, const double* a, const double* b);
 void set_gone_flags(Ctx& c, const uint8_t* gone);
 void set_beam_masks(Ctx& c, const uint16_t* beams);
 
