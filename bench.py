@@ -117,7 +117,8 @@ def main():
     faces = ctx.node_faces()
     nheld = ctx.node_held()
     loads = node_loads(occ, faces, joined, nheld, np.random.default_rng(args.seed))
-    ctx._check(ctx.L.st_set_loads(ctx.h, sh._p(np.ascontiguousarray(loads))), "st_set_loads")
+    loads = np.ascontiguousarray(loads)
+    ctx._check(ctx.L.st_set_loads(ctx.h, sh._p(loads)), "st_set_loads")
 
     import ctypes
     opts_mixed = sh.StSolveOpts(sh.MIXED, 1e-8, 20000, 0, 0)

@@ -177,7 +177,8 @@ int st_backward(StCtx* ctx, int objective, double beta, const StSolveOpts* opts,
 /* ---- batches ---------------------------------------------------------- */
 /* Many independent structures in one set of launches: every kernel covers
  * all structures' bricks; CG scalars are per structure and a structure that
- * has converged stops updating. cells[s] and loads[s] as for st_upload and
+ * has converged stops updating. Each structure is classified first (what is
+ * not joined to a held node is gone), as forces does before its solve. cells[s] and loads[s] as for st_upload and
  * st_solve (loads may be NULL: gravity along -z of weight 1 per free cell,
  * a sixth on each face). x_out[s] may be NULL. stats: one per structure. */
 int st_solve_batch(StCtx* ctx, int32_t n, const StCells* const* cells, const double* const* loads,

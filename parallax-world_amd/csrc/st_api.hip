@@ -125,6 +125,7 @@ __global__ void k_softmax_stats(int pass, int64_t ncells, const uint32_t* cfaces
   double acc = pass == 0 ? -INFINITY : 0.0;
   for (int q = 0; q < 15; q++) {
     if (!(m & (1u << q))) continue;
+    if (cfaces[6 * c + kPairA[q]] & cfaces[6 * c + kPairB[q]] & kNotFree) continue;  // held-held: no such beam
     BeamState b;
     beam_state(G, q, b);
     double d[3], dw[3];
@@ -157,6 +158,7 @@ __global__ void k_softmax_rhs(int64_t ncells, const uint32_t* cfaces, const uint
   const double mm = mz[0], Z = mz[1];
   for (int q = 0; q < 15; q++) {
     if (!(m & (1u << q))) continue;
+    if (cfaces[6 * c + kPairA[q]] & cfaces[6 * c + kPairB[q]] & kNotFree) continue;  // held-held: no such beam
     BeamState b;
     beam_state(G, q, b);
     double d[3], dw[3];
@@ -201,6 +203,7 @@ __global__ void k_sensitivity(int64_t ncells, const uint32_t* cfaces, const uint
   double acc = 0.0;
   for (int q = 0; q < 15; q++) {
     if (!(m & (1u << q))) continue;
+    if (cfaces[6 * c + kPairA[q]] & cfaces[6 * c + kPairB[q]] & kNotFree) continue;  // held-held: no such beam
     BeamState b;
     beam_state(G, q, b);
     double d[3], dw[3], dl[3], dwl[3];
@@ -390,6 +393,9 @@ int st_solve_batch(StCtx* c, int32_t n, const StCells* const* cells, const doubl
     c->gone_host.assign(c->geo.nnodes, 0);
     refresh_cell_flags(*c);
     c->uploaded = true;
+    // as forces does (statics/src/bodies.rs:407-409 under settle's 519-522): what is not
+    // joined to a held node is gone before the solve
+    classify_device(*c, nullptr, nullptr);
     const int64_t nn = c->geo.nnodes;
     const size_t bytes = (size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double);
     c->f64.ensure(bytes);

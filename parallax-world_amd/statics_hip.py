@@ -20,7 +20,8 @@ COMPLIANCE, SOFTMAX_STRESS = 0, 1
 # The entry points include/statics_hip.h declares.
 EXPORTS = ["st_create", "st_destroy", "st_last_error", "st_upload", "st_sizes", "st_node_faces", "st_lookup_faces",
            "st_node_held", "st_joined_to_held", "st_set_gone", "st_set_beams", "st_apply", "st_solve",
-           "st_bond_forces", "st_backward", "st_solve_batch", "st_generate"]
+           "st_set_loads", "st_solve_resident", "st_get_solution", "st_classify", "st_kernel_timing_enable",
+           "st_kernel_timing", "st_bond_forces", "st_backward", "st_solve_batch", "st_generate"]
 
 UNIT = 0.125  # metres to a level-0 cell (statics/src/lattice.rs:16)
 
@@ -81,6 +82,12 @@ def lib():
             "st_backward": [vp, ctypes.c_int, dbl, ctypes.POINTER(StSolveOpts), vp, vp, ctypes.POINTER(StStats)],
             "st_solve_batch": [vp, i32, vp, vp, ctypes.POINTER(StSolveOpts), vp, vp],
             "st_generate": [ctypes.POINTER(StGenParams), vp, vp],
+            "st_set_loads": [vp, vp],
+            "st_solve_resident": [vp, ctypes.POINTER(StSolveOpts), ctypes.POINTER(StStats)],
+            "st_get_solution": [vp, vp],
+            "st_classify": [vp, vp],
+            "st_kernel_timing_enable": [vp, ctypes.c_int, ctypes.c_int],
+            "st_kernel_timing": [vp, vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -221,6 +228,8 @@ class Context:
         rc = self.L.st_solve(self.h, _p(loads), ctypes.byref(o), _p(x), ctypes.byref(s))
         if check:
             self._check(rc, "st_solve")
+        if rc != 0 and s.status == 0:
+            s.status = rc
         return x, s
 
     def bond_forces(self, x):

@@ -1,0 +1,209 @@
+"""Device path against the oracle, through the C ABI.
+
+Bars: classification (joined_to_held) bit-exact; the operator and bond forces to FP64
+rounding; displacements within 1e-6 relative of the oracle's tightly converged solve
+(the north star's bar; the tests ask for 1e-8 at tol 1e-10); gradients within 1e-4 of
+central differences (eps = 1e-4) of the oracle's solve.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+import scenes
+import statics_hip as sh
+from harness import Pair, generated, mats_from
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def classify_both(P):
+    jo = P.L.joined_to_held()
+    jg = P.from_dev(P.ctx.joined_to_held())  # also sets gone on the device
+    assert (jo == jg).all()
+    return (jo == 0).astype(np.uint8)
+
+
+def gravity_z(L):
+    f = np.zeros((L.n_nodes, 6))
+    f[:, 2] = -L.m / orc.UNIT ** 3
+    return f
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_apply_parity_generated(ctx, seed):
+    P = generated(ctx, 24, seed)
+    gone = classify_both(P)
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(P.L.n_nodes, 6))
+    y_or = P.L.apply(x, gone=gone)
+    y_dev = P.from_dev(ctx.apply(P.to_dev(x)))
+    assert rel(y_dev, y_or) < 1e-13
+
+
+def test_apply_parity_dead_beams_and_gone(ctx):
+    P = generated(ctx, 16, 7)
+    rng = np.random.default_rng(7)
+    beam_alive = rng.random(P.L.n_bonds // 4) > 0.2
+    alive = np.repeat(beam_alive, 4).astype(np.uint8)
+    jo = P.L.joined_to_held(alive)
+    jg = P.from_dev(ctx.joined_to_held(P.beam_masks(alive.astype(bool))))
+    assert (jo == jg).all()
+    gone = (jo == 0).astype(np.uint8)
+    x = rng.normal(size=(P.L.n_nodes, 6))
+    assert rel(P.from_dev(ctx.apply(P.to_dev(x))), P.L.apply(x, alive=alive, gone=gone)) < 1e-13
+    # an arbitrary gone set: beams touching it are not live
+    gone2 = (rng.random(P.L.n_nodes) < 0.05).astype(np.uint8)
+    ctx.set_beams(None)
+    ctx.set_gone(P.to_dev(gone2.astype(float)).astype(np.uint8))
+    assert rel(P.from_dev(ctx.apply(P.to_dev(x))), P.L.apply(x, gone=gone2)) < 1e-13
+
+
+@pytest.mark.parametrize("precision", [sh.FP64, sh.MIXED])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_solve_parity_generated(ctx, precision, seed):
+    P = generated(ctx, 32, seed)
+    gone = classify_both(P)
+    f = gravity_z(P.L)
+    x_or, st_or = P.L.motions(f, gone=gone, tol=1e-13, max_iter=5000)
+    x_dev, st = ctx.solve(P.to_dev(f), precision=precision, tol=1e-10)
+    assert st.status == 0 and st.rel_residual <= 1e-10
+    assert rel(P.from_dev(x_dev), x_or) < 1e-8
+
+
+@pytest.mark.parametrize("scene", [scenes.cantilever(orc.STONE, 0.5, 3.0), scenes.portico(1, 0.5)])
+def test_solve_parity_reference_scenes(ctx, scene):
+    sites, mats, held = scenes.solids(0, *scene)
+    P = Pair(ctx, sites, mats, held)
+    gone = classify_both(P)
+    f = P.L.gravity()  # -y, as forces (statics/src/bodies.rs:408)
+    x_or, _ = P.L.motions(f, gone=gone, tol=1e-13, max_iter=5000)
+    for prec in (sh.FP64, sh.MIXED):
+        x_dev, st = ctx.solve(P.to_dev(f), precision=prec, tol=1e-10)
+        assert st.status == 0
+        assert rel(P.from_dev(x_dev), x_or) < 1e-8
+    # bond forces and used, as forces/used_standing do
+    fb_or = P.L.moved(x_or)
+    fb_dev = P.bond_forces_dev(P.from_dev(x_dev))
+    assert rel(fb_dev, fb_or) < 1e-7
+    u_or = P.L.used(fb_or)
+    u_dev = P.L.used(fb_dev)
+    assert abs(u_dev.max() - u_or.max()) <= 1e-6 * u_or.max()
+
+
+def test_bond_forces_parity(ctx):
+    P = generated(ctx, 16, 3)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(P.L.n_nodes, 6))
+    x[P.L.held.astype(bool)] = 0
+    assert rel(P.bond_forces_dev(x), P.L.moved(x)) < 1e-12
+
+
+def softmax_J(L, x, beta, live):
+    f = L.moved(x)
+    sigma = ((f * L.n).sum(axis=1) / L.a)[live]
+    m = sigma.max()
+    return m + np.log(np.exp(beta * (sigma - m)).sum()) / beta
+
+
+@pytest.mark.parametrize("objective", [sh.COMPLIANCE, sh.SOFTMAX_STRESS])
+def test_gradient_central_differences(ctx, objective):
+    """dJ/dtheta against central differences of the oracle's solve, eps = 1e-4, solves to 1e-13."""
+    P = generated(ctx, 12, 11, slab_min=4, slab_max=6)
+    gone = classify_both(P)
+    f = gravity_z(P.L)
+    beta = 20.0
+    x_dev, st = ctx.solve(P.to_dev(f), precision=sh.FP64, tol=1e-12)
+    J, dJ, _ = ctx.backward(objective, beta=beta, precision=sh.FP64, tol=1e-12)
+    rng = np.random.default_rng(11)
+    free_cells = np.flatnonzero(P.held_cells == 0)
+    eps = 1e-4
+
+    live = ~(gone[P.L.ij[:, 0]].astype(bool) | gone[P.L.ij[:, 1]].astype(bool))
+
+    def J_of(L):
+        x, _ = L.motions(f, gone=gone, tol=1e-13, max_iter=5000)
+        return (f * x).sum() if objective == sh.COMPLIANCE else softmax_J(L, x, beta, live)
+
+    assert abs(J - J_of(P.L)) <= 1e-9 * abs(J)
+    for c in rng.choice(free_cells, size=6, replace=False):
+        Lp = orc.Lattice(0, P.sites, P.mats, P.held_cells)
+        Lp.scale_cell(c, np.exp(eps))
+        Lm = orc.Lattice(0, P.sites, P.mats, P.held_cells)
+        Lm.scale_cell(c, np.exp(-eps))
+        fd = (J_of(Lp) - J_of(Lm)) / (2 * eps)
+        assert abs(dJ[c] - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(dJ).max()), (c, dJ[c], fd)
+
+
+def test_gradient_scaling_identities(ctx):
+    """theta -> theta + t scales x by exp(-t): sum dC/dtheta = -C and sum dJ_softmax/dtheta = 0."""
+    P = generated(ctx, 48, 5)
+    ctx.joined_to_held()
+    f = gravity_z(P.L)
+    ctx.solve(P.to_dev(f), precision=sh.MIXED, tol=1e-10)
+    C, dC, _ = ctx.backward(sh.COMPLIANCE)
+    assert abs(dC.sum() + C) <= 1e-8 * abs(C)
+    J, dJ, _ = ctx.backward(sh.SOFTMAX_STRESS, beta=20.0, precision=sh.MIXED, tol=1e-10)
+    assert abs(dJ.sum()) <= 1e-6 * np.abs(dJ).sum()
+
+
+def test_batch_matches_single(ctx):
+    sets = []
+    for n, seed in ((16, 1), (24, 2), (20, 3)):
+        occ, theta = sh.generate(n, n, n, seed=seed, sigma=0.5)
+        sets.append(sh.cells_from_occ(occ, theta))
+    # the batch classifies each structure and loads it with gravity (weight 1 per free cell, -z)
+    xs_b, st_b = ctx.solve_batch([(s, h, e, g) for s, h, e, g in sets], precision=sh.FP64, tol=1e-10, max_iter=20000)
+    for (s, h, e, g), xb, sb in zip(sets, xs_b, st_b):
+        ctx.upload(s, h, e, g)
+        ctx.joined_to_held()
+        faces = ctx.node_faces()
+        held = ctx.node_held()
+        cnt = np.zeros(len(faces))
+        occ_cells = {tuple(c) for c, hh in zip(s.tolist(), h) if not hh}
+        for v, (o, x, y, z) in enumerate(faces):
+            cnt[v] = ((x, y, z) in occ_cells) + ((x - (o == 0), y - (o == 1), z - (o == 2)) in occ_cells)
+        f = np.zeros((len(faces), 6))
+        f[:, 2] = -cnt / 6
+        f[held.astype(bool)] = 0
+        x1, s1 = ctx.solve(f, precision=sh.FP64, tol=1e-10, max_iter=20000, check=False)
+        assert s1.status == sb.status
+        if s1.status == 0:
+            assert rel(xb, x1) < 1e-8
+
+
+def test_edge_cases(ctx):
+    # one free cell on held ground
+    sites = np.array([[0, 0, 0], [0, 0, 1]])
+    held = np.array([1, 0], np.uint8)
+    P = Pair(ctx, sites, mats_from(np.ones(2), 0.4 * np.ones(2)), held)
+    f = gravity_z(P.L)
+    x_or, _ = P.L.motions(f, tol=1e-13)
+    x_dev, st = ctx.solve(P.to_dev(f), tol=1e-12)
+    assert st.status == 0 and rel(P.from_dev(x_dev), x_or) < 1e-10
+    # zero loads: no iteration, x = 0 (statics/src/bodies.rs:350-353)
+    x0, st0 = ctx.solve(np.zeros((ctx.nodes, 6)), tol=1e-8)
+    assert st0.status == 0 and st0.iters == 0 and not x0.any()
+    # an iteration cap of 1 is reported, not hidden
+    P = generated(ctx, 16, 0)
+    ctx.joined_to_held()
+    f = gravity_z(P.L)
+    _, st1 = ctx.solve(P.to_dev(f), tol=1e-12, max_iter=1, check=False)
+    assert st1.status == sh.ST_NO_CONVERGENCE
+    # NaN loads are reported
+    fn = P.to_dev(f)
+    fn[np.flatnonzero(np.abs(fn).sum(axis=1) > 0)[0], 2] = np.nan
+    _, st2 = ctx.solve(fn, tol=1e-8, check=False)
+    assert st2.status == sh.ST_NAN
+    # a floating piece (gone not set) is not SPD: the solve does not claim convergence
+    ctx.set_gone(None)
+    _, st3 = ctx.solve(P.to_dev(f), tol=1e-8, max_iter=300, check=False)
+    assert st3.status != 0
+    # bad input is refused
+    with pytest.raises(sh.StaticsError):
+        ctx.upload(np.array([[0, 0, 0], [0, 0, 0]]), np.zeros(2, np.uint8), np.ones(2), np.ones(2))
+    with pytest.raises(sh.StaticsError):
+        ctx.upload(np.array([[0, 0, 0]]), np.zeros(1, np.uint8), -np.ones(1), np.ones(1))

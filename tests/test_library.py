@@ -1,0 +1,54 @@
+"""The C ABI without a GPU: the library loads, exports what include/statics_hip.h
+declares, and its host-only parts (the generator) work."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import statics_hip as sh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    with open(os.path.join(ROOT, "include", "statics_hip.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(st_\w+)\s*\(", text, re.M)))
+
+
+def test_exports_every_declared_symbol():
+    lib = ctypes.CDLL(sh.LIB_PATH)
+    names = declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(sh.EXPORTS)
+
+
+def test_no_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(sh.StaticsError):
+        sh.Context()
+
+
+def test_generator_is_seeded_and_shaped():
+    a, ta = sh.generate(32, 32, 32, seed=5, sigma=0.5)
+    b, tb = sh.generate(32, 32, 32, seed=5, sigma=0.5)
+    c, _ = sh.generate(32, 32, 32, seed=6, sigma=0.5)
+    assert (a == b).all() and (ta == tb).all() and not (a == c).all()
+    assert (a[:, :, 0] == 2).all() and not (a[:, :, 1:] == 2).any()  # held ground layer only
+    fill = (a > 0).mean()
+    assert 0.15 < fill < 0.45
+    # pillars run from z = 1 to the top: some column is full above the ground
+    assert (a[:, :, 1:] == 1).all(axis=2).any()
+    assert abs(ta.std() - 0.5) < 0.02 and abs(ta.mean()) < 0.02
+
+
+def test_generator_rejects_bad_params():
+    p = sh.StGenParams(0, 4, 4, 0, 8.0, 2, 6, 8, 16, 0.2, 0.0)
+    occ = np.zeros(1, np.uint8)
+    assert sh.lib().st_generate(ctypes.byref(p), sh._p(occ), None) == sh.ST_BAD_INPUT
