@@ -454,7 +454,7 @@ int st_bond_forces(StCtx* c, const double* x, double* out) {
     od.ensure(std::max<int64_t>(ncl, 1) * 180 * sizeof(double));
     if (nn) ST_HIP(hipMemcpyAsync(xd.p, x, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (ncl)
-      k_bond_forces<<<nblk(ncl), 128, 0, c->stream>>>(ncl, c->geo.cfaces.as<uint32_t>(), c->geo.cmask.as<uint16_t>(),
+      k_bond_forces<<<nblk(ncl, 128), 128, 0, c->stream>>>(ncl, c->geo.cfaces.as<uint32_t>(), c->geo.cmask.as<uint16_t>(),
                                                       c->geo.ce64.as<double>(), c->geo.cg64.as<double>(),
                                                       c->geo.cstruct.as<uint32_t>(), c->geo.bgeo.as<BeamGeo>(),
                                                       xd.as<double>(), od.as<double>());
@@ -505,7 +505,7 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
     } else {
       // J = m + log(sum exp(beta (sigma - m))) / beta, m the greatest sigma
       if (ncl) {
-        k_softmax_stats<<<nblk(ncl), 128, 0, s>>>(0, ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(),
+        k_softmax_stats<<<nblk(ncl, 128), 128, 0, s>>>(0, ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(),
                                                   g.ce64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
                                                   xd.as<double>(), beta, nullptr, cellv.as<double>());
         ST_HIP(hipMemcpyAsync(hc.data(), cellv.p, ncl * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -516,7 +516,7 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
       if (!std::isfinite(m)) throw StError{ST_BAD_INPUT, "backward: no live bond"};
       mzh[0] = m;
       ST_HIP(hipMemcpyAsync(mz.p, mzh, 2 * sizeof(double), hipMemcpyHostToDevice, s));
-      k_softmax_stats<<<nblk(ncl), 128, 0, s>>>(1, ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(),
+      k_softmax_stats<<<nblk(ncl, 128), 128, 0, s>>>(1, ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(),
                                                 g.ce64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
                                                 xd.as<double>(), beta, mz.as<double>(), cellv.as<double>());
       ST_HIP(hipMemcpyAsync(hc.data(), cellv.p, ncl * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -528,7 +528,7 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
       c->f64.ensure(bytes);
       c->x64.ensure(bytes);
       ST_HIP(hipMemsetAsync(c->f64.p, 0, bytes, s));
-      k_softmax_rhs<<<nblk(ncl), 128, 0, s>>>(ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(), g.ce64.as<double>(),
+      k_softmax_rhs<<<nblk(ncl, 128), 128, 0, s>>>(ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(), g.ce64.as<double>(),
                                               g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), xd.as<double>(), beta,
                                               mz.as<double>(), c->f64.as<double>());
       k_mask_rows2<<<nblk(nn), 256, 0, s>>>(nn, g.nflags.as<uint8_t>(), c->f64.as<double>());
@@ -537,7 +537,7 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
       ST_HIP(hipMemcpyAsync(lam.p, c->x64.p, bytes, hipMemcpyDeviceToDevice, s));
     }
     if (ncl)
-      k_sensitivity<<<nblk(ncl), 128, 0, s>>>(ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(), g.ce64.as<double>(),
+      k_sensitivity<<<nblk(ncl, 128), 128, 0, s>>>(ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(), g.ce64.as<double>(),
                                               g.cg64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
                                               xd.as<double>(), lam.as<double>(), soft ? 1 : 0, beta, mz.as<double>(),
                                               cellv.as<double>());

@@ -43,7 +43,7 @@ namespace st {
 namespace {
 
 constexpr int kScal = 16;  // doubles of CG state per structure
-enum { S_RZ = 0, S_RR, S_PAP, S_ALPHA, S_BETA, S_GOAL2, S_FF, S_RZNEW, S_RRTRUE, S_RRLAST, S_LAM_A, S_LAM_B };
+enum { S_RZ = 0, S_RR, S_PAP, S_ALPHA, S_BETA, S_GOAL2, S_FF, S_RZNEW, S_RRTRUE, S_RRLAST, S_LAM_A, S_LAM_B, S_BEST, S_STALL };
 constexpr int kMaxJ = 96;  // distinct neighbours of one aggregate
 
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)std::max<int64_t>(1, (n + b - 1) / b); }
@@ -464,6 +464,8 @@ __global__ void k_cg_start(int ns, double* scal, uint8_t* done, int* status, int
   double* S = scal + kScal * s;
   S[S_GOAL2] = tol * tol * S[S_FF];
   S[S_RRLAST] = S[S_RR];
+  S[S_BEST] = S[S_RR];
+  S[S_STALL] = 0.0;
   iters[s] = 0;
   status[s] = 0;
   want[s] = 0;
@@ -518,20 +520,35 @@ __global__ void k_cg_beta(int ns, double* scal, const uint8_t* done) {
   S[S_BETA] = S[S_RZ] != 0.0 ? rzn / S[S_RZ] : 0.0;
   S[S_RZ] = rzn;
 }
-// after a true residual: it is the residual from now on
+// After a true residual: it is the residual from now on. A structure whose true
+// residual has not fallen by a tenth over kStall replacements stops (the
+// precision's attainable accuracy is reached) and is reported as not converged.
+constexpr int kStall = 8;
 __global__ void k_cg_true(int ns, double* scal, uint8_t* done, int* status, uint8_t* want, int* refreshes) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns || done[s]) return;
   double* S = scal + kScal * s;
   refreshes[s] += 1;
-  S[S_RR] = S[S_RRTRUE];
-  S[S_RRLAST] = S[S_RRTRUE];
+  const double rt = S[S_RRTRUE];
+  S[S_RR] = rt;
+  S[S_RRLAST] = rt;
   want[s] = 0;
-  if (!isfinite(S[S_RRTRUE])) {
+  if (!isfinite(rt)) {
     status[s] = ST_NAN;
     done[s] = 1;
-  } else if (S[S_RRTRUE] <= S[S_GOAL2]) {
+  } else if (rt <= S[S_GOAL2]) {
     done[s] = 1;
+  } else {
+    if (rt < 0.81 * S[S_BEST]) {
+      S[S_BEST] = rt;
+      S[S_STALL] = 0.0;
+    } else {
+      S[S_STALL] += 1.0;
+      if (S[S_STALL] >= kStall) {
+        status[s] = ST_NO_CONVERGENCE;
+        done[s] = 1;
+      }
+    }
   }
 }
 
@@ -1467,7 +1484,7 @@ struct Solver {
           k_cg_alpha<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>());
           k_update_xr<double><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, x, r, p, ap, partial);
           reduce_bricks(c.partial, S_RR);
-          k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), 0,
+          k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), 1,
                                                swant.as<uint8_t>(), 0.0);
           vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>());
           reduce_bricks(c.partial, S_RZNEW);
@@ -1476,6 +1493,21 @@ struct Solver {
         }
         ST_HIP(hipGetLastError());
         sync_flags(dh, wh);
+        bool any_want = false;
+        for (int s = 0; s < ns; s++) any_want |= !dh[s] && wh[s];
+        if (any_want || it >= max_iter) {
+          // the recursive residual says done: replace it by the true one, f - K x, and go on if short
+          EpiResid<double, double> er;
+          er.done = done;
+          er.partial = partial;
+          er.b = c.f64.as<double>();
+          er.r = r;
+          launch_apply<double>(c, x, er);
+          reduce_bricks(c.partial, S_RRTRUE);
+          k_cg_true<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), swant.as<uint8_t>(),
+                                              srefresh.as<int>());
+          sync_flags(dh, wh);
+        }
         if (std::all_of(dh.begin(), dh.end(), [](uint8_t d) { return d != 0; })) break;
       }
     } else {
