@@ -168,6 +168,10 @@ ST_INST_APPLY(float, SINGLE_ARG(EpiResid<float, float>))
 ST_INST_APPLY(float, SINGLE_ARG(EpiResid<float, double>))
 ST_INST_APPLY(double, EpiSmooth<double>)
 ST_INST_APPLY(float, EpiSmooth<float>)
+ST_INST_APPLY(double, EpiResidJac<double>)
+ST_INST_APPLY(float, EpiResidJac<float>)
+ST_INST_APPLY(double, EpiProlong<double>)
+ST_INST_APPLY(float, EpiProlong<float>)
 
 }  // namespace st
 

@@ -125,6 +125,10 @@ struct Level {
   DBuf rep;             // per aggregate: one fine node of it (int32)
   DBuf key;             // per aggregate: cube coordinates packed (uint64: struct, x, y, z)
   DBuf kid_ptr, kids;   // children (aggregates of the level below) grouped by this level's aggregate
+  // smoothed prolongator from the next level to this one (rows: this level's aggregates),
+  // block-sparse by rows, and the same blocks grouped by column (the next level's aggregates)
+  int64_t pblocks = 0;
+  DBuf prow, pcol, pval64, pval32, pt_ptr, pt_blk, pblk_row;
   DBuf agg_struct;      // per aggregate: structure
   std::vector<double> step;  // smoother step per structure
   DBuf step_d;               // per structure (double)
@@ -147,10 +151,15 @@ struct Ctx {
   // fine-node aggregation (level 1) and the union-find scratch
   DBuf label1, uf, uf2, flag, scan_tmp, sort_tmp;
   DBuf nodes_by_agg, agg_ptr;
+  // smoothed prolongator from level 1 to the fine nodes (rows: nodes), used to build level 1
+  DBuf p0row, p0col, p0val, p0t_ptr, p0t_blk, p0blk_row;
+  int64_t p0blocks = 0;
+  bool smoothed = true;  // smoothed aggregation (false: rigid aggregates only)
+  int sa_levels = 2;     // transfers smoothed, from the finest up
   std::vector<Level> levels;        // coarse levels 1..
   // solver vectors
-  DBuf f64, x64, r64, p64, ap64, z64, t64, u64;
-  DBuf x32, r32, p32, ap32, z32, t32;
+  DBuf f64, x64, r64, p64, ap64, z64, t64, u64, w64;
+  DBuf x32, r32, p32, ap32, z32, t32, w32;
   DBuf partial, scal;               // reductions: per-block partials, per-structure scalars
   DBuf done;                        // per structure: 1 once converged
   DBuf fres, xsol;                  // resident loads and the last solution (FP64, 6 per node)

@@ -239,4 +239,49 @@ struct EpiSmooth : EpiBase {
   }
 };
 
+// r1 = b - K x and s = step D^-1 r1 (the restriction's smoothing step, fused)
+template <class T>
+struct EpiResidJac : EpiBase {
+  static constexpr bool kReduce = false;
+  const T* b;
+  const T* dinv;
+  const double* step;
+  T* r1;
+  T* sout;
+  __device__ double node(uint32_t n, const T* a, uint32_t s) const {
+    T bv[6], t[6], d[6];
+    load6<T>(b + 6 * (size_t)n, bv);
+#pragma unroll
+    for (int k = 0; k < 6; k++) t[k] = bv[k] - a[k];
+    store6<T>(r1 + 6 * (size_t)n, t);
+    mat6v<T>(dinv + 36 * (size_t)n, t, d);
+    const T w = (T)step[s];
+#pragma unroll
+    for (int k = 0; k < 6; k++) d[k] *= w;
+    store6<T>(sout + 6 * (size_t)n, d);
+    return 0.0;
+  }
+};
+
+// x += u - step D^-1 K u, u the rigid prolongation (the smoothed prolongator, matrix-free)
+template <class T>
+struct EpiProlong : EpiBase {
+  static constexpr bool kReduce = false;
+  const T* u;
+  const T* dinv;
+  const double* step;
+  T* x;
+  __device__ double node(uint32_t n, const T* a, uint32_t s) const {
+    T uv[6], xv[6], d[6];
+    load6<T>(u + 6 * (size_t)n, uv);
+    load6<T>(x + 6 * (size_t)n, xv);
+    mat6v<T>(dinv + 36 * (size_t)n, a, d);
+    const T w = (T)step[s];
+#pragma unroll
+    for (int k = 0; k < 6; k++) xv[k] += uv[k] - w * d[k];
+    store6<T>(x + 6 * (size_t)n, xv);
+    return 0.0;
+  }
+};
+
 }  // namespace st

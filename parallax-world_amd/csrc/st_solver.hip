@@ -35,6 +35,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 
 #include "st_solver.hpp"
 
@@ -961,6 +962,624 @@ __global__ void k_seg_from_ids(int ns, const int64_t* node0, const int* ids, con
     out[s] = nnodes ? ids[nnodes - 1] + flag[nnodes - 1] : 0;
 }
 
+// ------------------------------------------------------- smoothed aggregation
+// The prolongator P = (I - w D^-1 A) P0 (statics/src/bodies.rs:238-247), built
+// explicitly row by row, and the Galerkin product P^T A P (:255-269) gathered
+// row by row of the coarse operator over the columns of P: every output row is
+// owned by one thread, so there are no atomics and the sums run in a fixed order.
+constexpr int kMaxP = 16;   // blocks in one row of the fine P0 (a node has at most 10 beams)
+constexpr int kMaxPc = 96;  // blocks in one row of a coarse P
+constexpr int kMaxR = 192;  // blocks in one row of a smoothed coarse operator
+
+__device__ __forceinline__ void mat6_mul_add(const double* a, const double* b, double* out) {  // out += a b
+  for (int r = 0; r < 6; r++)
+    for (int c2 = 0; c2 < 6; c2++) {
+      double s = 0;
+      for (int k = 0; k < 6; k++) s += a[6 * r + k] * b[6 * k + c2];
+      out[6 * r + c2] += s;
+    }
+}
+__device__ __forceinline__ void mat6_tmul(const double* a, const double* b, double* out) {  // out = a^T b
+  for (int r = 0; r < 6; r++)
+    for (int c2 = 0; c2 < 6; c2++) {
+      double s = 0;
+      for (int k = 0; k < 6; k++) s += a[6 * k + r] * b[6 * k + c2];
+      out[6 * r + c2] = s;
+    }
+}
+// Sh(r) as a 6x6 matrix: [[I, -[r]x], [0, I]]
+__device__ __forceinline__ void shift_mat(const double* r, double* S) {
+  for (int i = 0; i < 36; i++) S[i] = (i % 7 == 0) ? 1.0 : 0.0;
+  S[0 * 6 + 4] = r[2];
+  S[0 * 6 + 5] = -r[1];
+  S[1 * 6 + 3] = -r[2];
+  S[1 * 6 + 5] = r[0];
+  S[2 * 6 + 3] = r[1];
+  S[2 * 6 + 4] = -r[0];
+}
+// out = M Sh(r) for a 6x6 M
+__device__ __forceinline__ void mat6_shift(const double* M, const double* r, double* out) {
+  double S[36];
+  shift_mat(r, S);
+  for (int i = 0; i < 36; i++) out[i] = 0.0;
+  mat6_mul_add(M, S, out);
+}
+__device__ __forceinline__ int slot_of(const int* cols, int len, int J) {
+  for (int t = 0; t < len; t++)
+    if (cols[t] == J) return t;
+  return -1;
+}
+
+// One live beam at fine node v, seen from v: the beam's pair q, v's sign, the other end.
+struct BeamAt {
+  int q, sv, c;
+  uint32_t wf;  // other end's face entry (kNotFree if held)
+};
+// Enumerates the live beams at v (over its two cells) into list; returns the count (<= 10).
+__device__ int beams_at(uint32_t v, const uint8_t* nkind, const int32_t* ncell, const uint32_t* cfaces,
+                        const uint16_t* cmask, BeamAt* list) {
+  const int o = nkind[v] >> 1;
+  int nb = 0;
+  for (int side = 0; side < 2; side++) {
+    const int c = ncell[2 * v + side];
+    if (c < 0) continue;
+    const int f = side == 0 ? 2 * o + 1 : 2 * o;
+    const uint32_t m = (uint32_t)cmask[c] & face_beams(f);
+    for (int q = 0; q < 15; q++) {
+      if (!(m & (1u << q))) continue;
+      const int f2 = kPairA[q] == f ? kPairB[q] : kPairA[q];
+      list[nb].q = q;
+      list[nb].sv = kPairA[q] == f ? -1 : 1;
+      list[nb].c = c;
+      list[nb].wf = cfaces[6 * c + f2];
+      nb++;
+    }
+  }
+  return nb;
+}
+
+// Rows of P0 = (I - w D^-1 K) P_rigid at the fine nodes. pass 0: count blocks; pass 1: fill.
+// (K chi_J)(v), the load at v when aggregate J moves rigidly: for v in J only the beams
+// leaving J count (a beam inside J moves rigidly), K_b,vv Sh(r_v); for v not in J the beams
+// reaching into J, K_b,vw Sh(r_w).
+__global__ void k_p0(int pass, int64_t nn, const uint8_t* nflags, const int* label, const double* centre,
+                     const uint32_t* nkey, const uint8_t* nkind, const int32_t* ncell, const uint32_t* cfaces,
+                     const double* ce, const double* cg, const uint16_t* cmask, const uint32_t* cstruct,
+                     const BeamGeo* bgeo, const double* dinv, const double* step, int* cnt, const int* prow, int* pcol,
+                     double* pval, int* overflow) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nn) return;
+  if (nflags[v] & (kHeld | kGone)) {
+    if (pass == 0) cnt[v] = 0;
+    return;
+  }
+  BeamAt bl[10];
+  const int nbm = beams_at((uint32_t)v, nkind, ncell, cfaces, cmask, bl);
+  const int I0 = label[v];
+  int js[kMaxP];
+  int nj = 1;
+  js[0] = I0;
+  for (int t = 0; t < nbm; t++) {
+    if (bl[t].wf & kNotFree) continue;
+    const int J = label[bl[t].wf];
+    if (J != I0 && slot_of(js, nj, J) < 0) {
+      if (nj < kMaxP)
+        js[nj++] = J;
+      else
+        atomicExch(overflow, 1);
+    }
+  }
+  if (pass == 0) {
+    cnt[v] = nj;
+    return;
+  }
+  const int base = prow[v];
+  double* out = pval + 36 * (int64_t)base;
+  for (int t = 0; t < nj; t++) pcol[base + t] = js[t];
+  for (int i = 0; i < 36 * nj; i++) out[i] = 0.0;
+  const uint32_t sidx = cstruct[ncell[2 * v] >= 0 ? ncell[2 * v] : ncell[2 * v + 1]];
+  const BeamGeo G = bgeo[sidx];
+  const double sp = 2.0 * G.hd[0];
+  double pv[3], rv[3];
+  node_pos((uint32_t)v, nkey, nkind, sp, pv);
+  for (int k = 0; k < 3; k++) rv[k] = pv[k] - centre[3 * (int64_t)I0 + k];
+  const double zero[3] = {0, 0, 0};
+  for (int t = 0; t < nbm; t++) {
+    const BeamAt& b = bl[t];
+    const bool wfree = !(b.wf & kNotFree);
+    const int Jw = wfree ? label[b.wf] : -1;
+    if (Jw == I0) continue;  // inside the aggregate: rigid, no load
+    double n[3];
+    int cls;
+    pair_n(b.q, G, n, cls);
+    const double e = ce[b.c], g = cg[b.c];
+    double Bv[18], Gv[18];
+    gmat(b.sv, n, G.hd[cls], zero, Bv);
+    gmat(b.sv, n, G.hd[cls], rv, Gv);
+    beam_coarse_add(Bv, Gv, b.sv, b.sv, n, G.K4[cls], G.R2[cls], e, g, out);  // K_b,vv Sh(r_v)
+    if (Jw >= 0) {
+      double pw[3], rw[3], Gw[18];
+      node_pos(b.wf, nkey, nkind, sp, pw);
+      for (int k = 0; k < 3; k++) rw[k] = pw[k] - centre[3 * (int64_t)Jw + k];
+      gmat(-b.sv, n, G.hd[cls], rw, Gw);
+      beam_coarse_add(Bv, Gw, b.sv, -b.sv, n, G.K4[cls], G.R2[cls], e, g, out + 36 * slot_of(js, nj, Jw));
+    }
+  }
+  // P[v, J] = [J = I0] Sh(r_v) - w D^-1 (K chi_J)(v)
+  const double w = step[sidx];
+  const double* D = dinv + 36 * v;
+  for (int t = 0; t < nj; t++) {
+    double* blk = out + 36 * t;
+    double tmp[36];
+    for (int i = 0; i < 36; i++) tmp[i] = 0.0;
+    mat6_mul_add(D, blk, tmp);
+    for (int i = 0; i < 36; i++) blk[i] = -w * tmp[i];
+  }
+  double S[36];
+  shift_mat(rv, S);
+  for (int i = 0; i < 36; i++) out[i] += S[i];
+}
+
+// Level 1 = P0^T K P0, row I gathered over the fine nodes v in column I of P0:
+// row I += P0[v,I]^T sum over the beams b at v of (K_b,vv P0[v,:] + K_b,vw P0[w,:]).
+__global__ void k_sa_a1(int pass, int64_t n1, const int* pt_ptr, const int* pt_blk, const int* pblk_row,
+                        const int* prow, const int* pcol, const double* pval, const uint8_t* nkind,
+                        const int32_t* ncell, const uint32_t* cfaces, const double* ce, const double* cg,
+                        const uint16_t* cmask, const uint32_t* cstruct, const BeamGeo* bgeo, int* rowcnt,
+                        const int* rowptr, int* col, double* A, int* overflow) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n1) return;
+  int js[kMaxR];
+  int nj = 1;
+  js[0] = (int)I;
+  int base = 0, len = 0;
+  if (pass == 1) {
+    base = rowptr[I];
+    len = rowptr[I + 1] - base;
+    for (int i = 0; i < 36 * len; i++) A[36 * (int64_t)base + i] = 0.0;
+    col[base] = (int)I;
+  }
+  const double zero[3] = {0, 0, 0};
+  for (int kk = pt_ptr[I]; kk < pt_ptr[I + 1]; kk++) {
+    const int blk = pt_blk[kk];
+    const int v = pblk_row[blk];
+    const double* PvI = pval + 36 * (int64_t)blk;
+    BeamAt bl[10];
+    const int nbm = beams_at((uint32_t)v, nkind, ncell, cfaces, cmask, bl);
+    for (int t = 0; t < nbm; t++) {
+      const BeamAt& b = bl[t];
+      const bool wfree = !(b.wf & kNotFree);
+      // columns this beam reaches: P0 row v and, if w is free, P0 row w
+      for (int side = 0; side < (wfree ? 2 : 1); side++) {
+        const int row = side == 0 ? v : (int)b.wf;
+        for (int k2 = prow[row]; k2 < prow[row + 1]; k2++) {
+          const int J = pcol[k2];
+          int sl = slot_of(js, nj, J);
+          if (sl < 0) {
+            if (nj < kMaxR) {
+              js[nj] = J;
+              sl = nj++;
+              if (pass == 1) col[base + sl] = J;
+            } else {
+              atomicExch(overflow, 1);
+              continue;
+            }
+          }
+        }
+      }
+      if (pass == 0) continue;
+      const BeamGeo G = bgeo[cstruct[b.c]];
+      double n[3];
+      int cls;
+      pair_n(b.q, G, n, cls);
+      const double e = ce[b.c], g = cg[b.c];
+      double Bv[18], Bw[18], K[36], Q[36];
+      gmat(b.sv, n, G.hd[cls], zero, Bv);
+      for (int side = 0; side < (wfree ? 2 : 1); side++) {
+        for (int i = 0; i < 36; i++) K[i] = 0.0;
+        if (side == 0) {
+          beam_coarse_add(Bv, Bv, b.sv, b.sv, n, G.K4[cls], G.R2[cls], e, g, K);
+        } else {
+          gmat(-b.sv, n, G.hd[cls], zero, Bw);
+          beam_coarse_add(Bv, Bw, b.sv, -b.sv, n, G.K4[cls], G.R2[cls], e, g, K);
+        }
+        mat6_tmul(PvI, K, Q);  // Q = P0[v,I]^T K_b
+        const int row = side == 0 ? v : (int)b.wf;
+        for (int k2 = prow[row]; k2 < prow[row + 1]; k2++) {
+          const int sl = slot_of(js, nj, pcol[k2]);
+          if (sl >= 0 && sl < len) mat6_mul_add(Q, pval + 36 * (int64_t)k2, A + 36 * ((int64_t)base + sl));
+        }
+      }
+    }
+  }
+  if (pass == 0) rowcnt[I] = nj;
+}
+
+// Rows of P = (I - w D^-1 A) P_rigid at an explicit level (rows i, columns the next level's J).
+__global__ void k_pc(int pass, int smooth, int64_t n, const int* rowptr, const int* colA, const double* A,
+                     const double* dinv, const double* step, const uint32_t* ast, const int* parent,
+                     const double* centre, const double* centre_up, int* cnt, const int* prow, int* pcol, double* pval,
+                     int* overflow) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int J0 = parent[i];
+  int js[kMaxPc];
+  int nj = 1;
+  js[0] = J0;
+  for (int b = smooth ? rowptr[i] : rowptr[i + 1]; b < rowptr[i + 1]; b++) {
+    const int J = parent[colA[b]];
+    if (slot_of(js, nj, J) < 0) {
+      if (nj < kMaxPc)
+        js[nj++] = J;
+      else
+        atomicExch(overflow, 1);
+    }
+  }
+  if (pass == 0) {
+    cnt[i] = nj;
+    return;
+  }
+  const int base = prow[i];
+  double* out = pval + 36 * (int64_t)base;
+  for (int t = 0; t < nj; t++) pcol[base + t] = js[t];
+  for (int k = 0; k < 36 * nj; k++) out[k] = 0.0;
+  for (int b = smooth ? rowptr[i] : rowptr[i + 1]; b < rowptr[i + 1]; b++) {
+    const int j = colA[b];
+    const int J = parent[j];
+    double r[3], M[36];
+    for (int k = 0; k < 3; k++) r[k] = centre[3 * (int64_t)j + k] - centre_up[3 * (int64_t)J + k];
+    mat6_shift(A + 36 * (int64_t)b, r, M);
+    double* blk = out + 36 * slot_of(js, nj, J);
+    for (int k = 0; k < 36; k++) blk[k] += M[k];
+  }
+  const double w = step[ast[i]];
+  const double* D = dinv + 36 * i;
+  for (int t = 0; t < nj; t++) {
+    double* blk = out + 36 * t;
+    double tmp[36];
+    for (int k = 0; k < 36; k++) tmp[k] = 0.0;
+    mat6_mul_add(D, blk, tmp);
+    for (int k = 0; k < 36; k++) blk[k] = -w * tmp[k];
+  }
+  double r0[3], S[36];
+  for (int k = 0; k < 3; k++) r0[k] = centre[3 * i + k] - centre_up[3 * (int64_t)J0 + k];
+  shift_mat(r0, S);
+  for (int k = 0; k < 36; k++) out[k] += S[k];
+}
+
+// Next level = P^T A P, row I gathered over the rows i in column I of P:
+// row I += P[i,I]^T sum over blocks (i, j) of A of A[i,j] P[j,:].
+__global__ void k_sa_gal(int pass, int64_t nI, const int* pt_ptr, const int* pt_blk, const int* pblk_row,
+                         const int* prow, const int* pcol, const double* pval, const int* rowptrA, const int* colA,
+                         const double* Aa, int* rowcnt, const int* rowptr, int* col, double* A, int* overflow) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nI) return;
+  int js[kMaxR];
+  int nj = 1;
+  js[0] = (int)I;
+  int base = 0, len = 0;
+  if (pass == 1) {
+    base = rowptr[I];
+    len = rowptr[I + 1] - base;
+    for (int k = 0; k < 36 * len; k++) A[36 * (int64_t)base + k] = 0.0;
+    col[base] = (int)I;
+  }
+  for (int kk = pt_ptr[I]; kk < pt_ptr[I + 1]; kk++) {
+    const int blk = pt_blk[kk];
+    const int i = pblk_row[blk];
+    const double* PiI = pval + 36 * (int64_t)blk;
+    for (int b = rowptrA[i]; b < rowptrA[i + 1]; b++) {
+      const int j = colA[b];
+      for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) {
+        const int J = pcol[k2];
+        if (slot_of(js, nj, J) < 0) {
+          if (nj < kMaxR) {
+            if (pass == 1) col[base + nj] = J;
+            js[nj++] = J;
+          } else {
+            atomicExch(overflow, 1);
+          }
+        }
+      }
+      if (pass == 0) continue;
+      double Q[36];
+      mat6_tmul(PiI, Aa + 36 * (int64_t)b, Q);
+      for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) {
+        const int sl = slot_of(js, nj, pcol[k2]);
+        if (sl >= 0 && sl < len) mat6_mul_add(Q, pval + 36 * (int64_t)k2, A + 36 * ((int64_t)base + sl));
+      }
+    }
+  }
+  if (pass == 0) rowcnt[I] = nj;
+}
+
+// ---- Galerkin rows, one wavefront per output row. The row's columns go into an LDS
+// hash set (open addressing), are sorted (diagonal first, then ascending) and numbered,
+// and the 6x6 products accumulate into LDS with native ds_add_f64. pass 0 counts the
+// row's columns; pass 1 fills col and A. kSlots bounds the columns of one row.
+constexpr int kHash = 2048;
+
+__device__ __forceinline__ unsigned hslot(int J) { return ((unsigned)J * 2654435761u) & (kHash - 1); }
+__device__ __forceinline__ void hinsert(int* keys, int J) {
+  unsigned h = hslot(J);
+  for (;;) {
+    const int old = atomicCAS(&keys[h], -1, J);
+    if (old == -1 || old == J) return;
+    h = (h + 1) & (kHash - 1);
+  }
+}
+__device__ __forceinline__ int hfind(const int* keys, const int* slot, int J) {
+  unsigned h = hslot(J);
+  for (;;) {
+    const int k = keys[h];
+    if (k == J) return slot[h];
+    if (k == -1) return -1;
+    h = (h + 1) & (kHash - 1);
+  }
+}
+
+// Shared part of both row kernels once the hash holds the row's columns: count them
+// (pass 0) or sort and number them (pass 1, when they fit in cap). Returns the count,
+// the diagonal included. Needs blockDim 64.
+__device__ int row_columns(int pass, int I, int* keys, int* slot, int* sorted, int cap, int* ncount) {
+  __syncthreads();
+  if (threadIdx.x == 0) *ncount = 0;
+  __syncthreads();
+  for (int h = threadIdx.x; h < kHash; h += blockDim.x)
+    if (keys[h] != -1 && keys[h] != I) {
+      const int p = atomicAdd(ncount, 1);
+      if (pass == 1 && p < cap) sorted[p] = keys[h];
+    }
+  __syncthreads();
+  const int m = *ncount;
+  if (pass == 1 && m + 1 <= cap) {
+    // rank sort (m is small): an entry's place is the number of smaller keys
+    for (int a = threadIdx.x; a < m; a += blockDim.x) {
+      const int key = sorted[a];
+      int rank = 0;
+      for (int b = 0; b < m; b++) rank += sorted[b] < key;
+      unsigned h = hslot(key);
+      while (keys[h] != key) h = (h + 1) & (kHash - 1);
+      slot[h] = 1 + rank;
+    }
+    if (threadIdx.x == 0) {
+      unsigned h = hslot(I);
+      while (keys[h] != I) h = (h + 1) & (kHash - 1);
+      slot[h] = 0;
+    }
+  }
+  __syncthreads();
+  return m + 1;
+}
+
+template <int kSlots>
+__global__ __launch_bounds__(64) void k_sa_gal_w(int pass, int64_t nI, const int* pt_ptr, const int* pt_blk,
+                                                 const int* pblk_row, const int* prow, const int* pcol,
+                                                 const double* pval, const int* rowptrA, const int* colA,
+                                                 const double* Aa, int* rowcnt, const int* rowptr, int* col, double* A,
+                                                 int* overflow) {
+  __shared__ int keys[kHash];
+  __shared__ int slot[kHash];
+  __shared__ int sorted[kSlots];
+  __shared__ int ncount;
+  __shared__ double acc[36 * kSlots];
+  const int I = blockIdx.x;
+  if (I >= nI) return;
+  for (int h = threadIdx.x; h < kHash; h += blockDim.x) keys[h] = -1;
+  __syncthreads();
+  if (threadIdx.x == 0) hinsert(keys, I);
+  const int k0 = pt_ptr[I], k1 = pt_ptr[I + 1];
+  // work items: (column entry kk, block b of A's row i)
+  for (int kk = k0; kk < k1; kk++) {
+    const int i = pblk_row[pt_blk[kk]];
+    for (int b = rowptrA[i] + threadIdx.x; b < rowptrA[i + 1]; b += blockDim.x) {
+      const int j = colA[b];
+      for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) hinsert(keys, pcol[k2]);
+    }
+  }
+  const int m = row_columns(pass, I, keys, slot, sorted, kSlots, &ncount);
+  if (pass == 0) {
+    if (threadIdx.x == 0) rowcnt[I] = m;
+    return;
+  }
+  if (m > kSlots) {
+    if (threadIdx.x == 0) atomicMax(overflow, m);
+    return;
+  }
+  for (int t = threadIdx.x; t < 36 * m; t += blockDim.x) acc[t] = 0.0;
+  __syncthreads();
+  for (int kk = k0; kk < k1; kk++) {
+    const int blk = pt_blk[kk];
+    const int i = pblk_row[blk];
+    const double* PiI = pval + 36 * (int64_t)blk;
+    for (int b = rowptrA[i] + threadIdx.x; b < rowptrA[i + 1]; b += blockDim.x) {
+      const int j = colA[b];
+      double Q[36];
+      mat6_tmul(PiI, Aa + 36 * (int64_t)b, Q);
+      for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) {
+        const int sl = hfind(keys, slot, pcol[k2]);
+        const double* Pj = pval + 36 * (int64_t)k2;
+        double* out = acc + 36 * sl;
+        for (int r = 0; r < 6; r++)
+          for (int cc = 0; cc < 6; cc++) {
+            double s = 0;
+            for (int k = 0; k < 6; k++) s += Q[6 * r + k] * Pj[6 * k + cc];
+            atomicAdd(&out[6 * r + cc], s);
+          }
+      }
+    }
+  }
+  __syncthreads();
+  const int base = rowptr[I];
+  if (threadIdx.x == 0) col[base] = I;
+  for (int h = threadIdx.x; h < kHash; h += blockDim.x)
+    if (keys[h] != -1 && keys[h] != I) col[base + slot[h]] = keys[h];
+  for (int t = threadIdx.x; t < 36 * m; t += blockDim.x) A[36 * (int64_t)base + t] = acc[t];
+}
+
+template <int kSlots>
+__global__ __launch_bounds__(64) void k_sa_a1_w(int pass, int64_t n1, const int* pt_ptr, const int* pt_blk,
+                                                const int* pblk_row, const int* prow, const int* pcol,
+                                                const double* pval, const uint8_t* nkind, const int32_t* ncell,
+                                                const uint32_t* cfaces, const double* ce, const double* cg,
+                                                const uint16_t* cmask, const uint32_t* cstruct, const BeamGeo* bgeo,
+                                                int* rowcnt, const int* rowptr, int* col, double* A, int* overflow) {
+  __shared__ int keys[kHash];
+  __shared__ int slot[kHash];
+  __shared__ int sorted[kSlots];
+  __shared__ int ncount;
+  __shared__ double acc[36 * kSlots];
+  const int I = blockIdx.x;
+  if (I >= n1) return;
+  for (int h = threadIdx.x; h < kHash; h += blockDim.x) keys[h] = -1;
+  __syncthreads();
+  if (threadIdx.x == 0) hinsert(keys, I);
+  const int k0 = pt_ptr[I], k1 = pt_ptr[I + 1];
+  const double zero[3] = {0, 0, 0};
+  // work items: (column entry kk, beam t at its node): 10 beams at most per node
+  const int items = (k1 - k0) * 10;
+  for (int it = threadIdx.x; it < items; it += blockDim.x) {
+    const int v = pblk_row[pt_blk[k0 + it / 10]];
+    BeamAt bl[10];
+    const int nbm = beams_at((uint32_t)v, nkind, ncell, cfaces, cmask, bl);
+    const int t = it % 10;
+    if (t >= nbm) continue;
+    for (int k2 = prow[v]; k2 < prow[v + 1]; k2++) hinsert(keys, pcol[k2]);
+    if (!(bl[t].wf & kNotFree))
+      for (int k2 = prow[bl[t].wf]; k2 < prow[bl[t].wf + 1]; k2++) hinsert(keys, pcol[k2]);
+  }
+  const int m = row_columns(pass, I, keys, slot, sorted, kSlots, &ncount);
+  if (pass == 0) {
+    if (threadIdx.x == 0) rowcnt[I] = m;
+    return;
+  }
+  if (m > kSlots) {
+    if (threadIdx.x == 0) atomicMax(overflow, m);
+    return;
+  }
+  for (int x = threadIdx.x; x < 36 * m; x += blockDim.x) acc[x] = 0.0;
+  __syncthreads();
+  for (int it = threadIdx.x; it < items; it += blockDim.x) {
+    const int blk = pt_blk[k0 + it / 10];
+    const int v = pblk_row[blk];
+    BeamAt bl[10];
+    const int nbm = beams_at((uint32_t)v, nkind, ncell, cfaces, cmask, bl);
+    const int t = it % 10;
+    if (t >= nbm) continue;
+    const BeamAt& b = bl[t];
+    const bool wfree = !(b.wf & kNotFree);
+    const double* PvI = pval + 36 * (int64_t)blk;
+    const BeamGeo G = bgeo[cstruct[b.c]];
+    double n[3];
+    int cls;
+    pair_n(b.q, G, n, cls);
+    const double e = ce[b.c], g = cg[b.c];
+    double Bv[18], Bw[18], K[36], Q[36];
+    gmat(b.sv, n, G.hd[cls], zero, Bv);
+    for (int side = 0; side < (wfree ? 2 : 1); side++) {
+      for (int x = 0; x < 36; x++) K[x] = 0.0;
+      if (side == 0) {
+        beam_coarse_add(Bv, Bv, b.sv, b.sv, n, G.K4[cls], G.R2[cls], e, g, K);
+      } else {
+        gmat(-b.sv, n, G.hd[cls], zero, Bw);
+        beam_coarse_add(Bv, Bw, b.sv, -b.sv, n, G.K4[cls], G.R2[cls], e, g, K);
+      }
+      mat6_tmul(PvI, K, Q);
+      const int row = side == 0 ? v : (int)b.wf;
+      for (int k2 = prow[row]; k2 < prow[row + 1]; k2++) {
+        const int sl = hfind(keys, slot, pcol[k2]);
+        const double* Pj = pval + 36 * (int64_t)k2;
+        double* out = acc + 36 * sl;
+        for (int r = 0; r < 6; r++)
+          for (int cc = 0; cc < 6; cc++) {
+            double s = 0;
+            for (int k = 0; k < 6; k++) s += Q[6 * r + k] * Pj[6 * k + cc];
+            atomicAdd(&out[6 * r + cc], s);
+          }
+      }
+    }
+  }
+  __syncthreads();
+  const int base = rowptr[I];
+  if (threadIdx.x == 0) col[base] = I;
+  for (int h = threadIdx.x; h < kHash; h += blockDim.x)
+    if (keys[h] != -1 && keys[h] != I) col[base + slot[h]] = keys[h];
+  for (int x = threadIdx.x; x < 36 * m; x += blockDim.x) A[36 * (int64_t)base + x] = acc[x];
+}
+
+__global__ void k_blk_rows(int64_t n, const int* prow, int* blk_row) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = prow[i]; k < prow[i + 1]; k++) blk_row[k] = (int)i;
+}
+
+// V-cycle transfers with an explicit P. restrict: c[J] = sum over column J of P^T t.
+template <class T>
+__global__ void k_restrict_p(int64_t nJ, const uint32_t* ast, const uint8_t* done, const int* pt_ptr, const int* pt_blk,
+                             const int* pblk_row, const T* pval, const T* t, T* c) {
+  const int64_t J = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= nJ) return;
+  if (done && done[ast[J]]) return;
+  T acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int kk = pt_ptr[J]; kk < pt_ptr[J + 1]; kk++) {
+    const int blk = pt_blk[kk];
+    const T* M = pval + 36 * (int64_t)blk;
+    T tv[6];
+    load6<T>(t + 6 * (int64_t)pblk_row[blk], tv);
+    for (int a = 0; a < 6; a++) {
+      T s = 0;
+      for (int b = 0; b < 6; b++) s += M[6 * b + a] * tv[b];
+      acc[a] += s;
+    }
+  }
+  store6<T>(c + 6 * J, acc);
+}
+// prolong: x[i] += sum over row i of P y
+template <class T>
+__global__ void k_prolong_p(int64_t n, const uint32_t* ast, const uint8_t* done, const int* prow, const int* pcol,
+                            const T* pval, const T* y, T* x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (done && done[ast[i]]) return;
+  T xv[6];
+  load6<T>(x + 6 * i, xv);
+  for (int k = prow[i]; k < prow[i + 1]; k++) {
+    const T* M = pval + 36 * (int64_t)k;
+    T yv[6];
+    load6<T>(y + 6 * (int64_t)pcol[k], yv);
+    for (int a = 0; a < 6; a++) {
+      T s = 0;
+      for (int b = 0; b < 6; b++) s += M[6 * a + b] * yv[b];
+      xv[a] += s;
+    }
+  }
+  store6<T>(x + 6 * i, xv);
+}
+
+// rigid prolongation level 1 -> fine, overwriting: x[v] = Sh(r_v) y[label[v]] (0 off aggregates)
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prolong1_set(const uint32_t* nbase, const uint32_t* bstruct,
+                                                         const uint8_t* done, const int* label, const double* centre,
+                                                         const uint32_t* nkey, const uint8_t* nkind, const BeamGeo* bgeo,
+                                                         const T* y, T* x) {
+  const uint32_t b = blockIdx.x, s = bstruct[b];
+  if (done && done[s]) return;
+  const double sp = 2.0 * bgeo[s].hd[0];
+  for (uint32_t v = nbase[b] + threadIdx.x; v < nbase[b + 1]; v += blockDim.x) {
+    const int I = label[v];
+    T xv[6] = {0, 0, 0, 0, 0, 0};
+    if (I >= 0) {
+      double p[3];
+      node_pos(v, nkey, nkind, sp, p);
+      for (int k = 0; k < 3; k++) p[k] -= centre[3 * (int64_t)I + k];
+      T yv[6];
+      load6<T>(y + 6 * (int64_t)I, yv);
+      shift_add<T>(yv, p, xv);
+    }
+    store6<T>(x + 6 * (size_t)v, xv);
+  }
+}
+
 template <class T>
 T* ptr(DBuf& b) {
   return b.as<T>();
@@ -1166,6 +1785,150 @@ struct Solver {
     if (read_int(sbad.as<int>())) throw StError{ST_NOT_SPD, "a coarse diagonal block is singular"};
   }
 
+  // Two-pass construction of a block-sparse prolongator (rows nrows, columns ncols) and of
+  // its blocks grouped by column (stable sort by column keeps rows ascending within a column).
+  template <class F>
+  int64_t build_p(int64_t nrows, int64_t ncols, DBuf& prow, DBuf& pcol, DBuf& pval, DBuf& pt_ptr, DBuf& pt_blk,
+                  DBuf& pblk_row, F launch) {
+    DBuf cnt;
+    cnt.ensure((nrows + 1) * sizeof(int));
+    prow.ensure((nrows + 1) * sizeof(int));
+    ST_HIP(hipMemsetAsync(sbad.p, 0, sizeof(int), st));
+    ST_HIP(hipMemsetAsync(cnt.p, 0, (nrows + 1) * sizeof(int), st));
+    launch(0, cnt.as<int>(), prow.as<int>(), (int*)nullptr, (double*)nullptr);
+    if (read_int(sbad.as<int>())) throw StError{ST_DEVICE, "a prolongator row has more blocks than its capacity"};
+    scan(cnt.as<int>(), prow.as<int>(), nrows + 1);
+    int nb = 0;
+    ST_HIP(hipMemcpyAsync(&nb, prow.as<int>() + nrows, sizeof(int), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipStreamSynchronize(st));
+    pcol.ensure(std::max(nb, 1) * sizeof(int));
+    pval.ensure((size_t)std::max(nb, 1) * 36 * sizeof(double));
+    launch(1, cnt.as<int>(), prow.as<int>(), pcol.as<int>(), pval.as<double>());
+    ST_HIP(hipGetLastError());
+    cnt.release();
+    // blocks by column
+    pblk_row.ensure(std::max(nb, 1) * sizeof(int));
+    k_blk_rows<<<nblk(nrows), 256, 0, st>>>(nrows, prow.as<int>(), pblk_row.as<int>());
+    DBuf vals, skeys;
+    vals.ensure(std::max(nb, 1) * sizeof(int));
+    skeys.ensure(std::max(nb, 1) * sizeof(int));
+    pt_blk.ensure(std::max(nb, 1) * sizeof(int));
+    pt_ptr.ensure((ncols + 1) * sizeof(int));
+    k_iota<<<nblk(nb), 256, 0, st>>>(nb, vals.as<int>());
+    sort_pairs(pcol.as<int>(), skeys.as<int>(), vals.as<int>(), pt_blk.as<int>(), nb);
+    k_fill_int<<<nblk(ncols + 1), 256, 0, st>>>(ncols + 1, pt_ptr.as<int>(), 0);
+    k_seg_ptr<<<nblk(nb), 256, 0, st>>>(nb, skeys.as<int>(), pt_ptr.as<int>(), (int)ncols);
+    ST_HIP(hipGetLastError());
+    vals.release();
+    skeys.release();
+    return nb;
+  }
+
+  int max_int(const int* d, int64_t n) {
+    DBuf out;
+    out.ensure(sizeof(int));
+    size_t bytes = 0;
+    ST_HIP(hipcub::DeviceReduce::Max(nullptr, bytes, d, out.as<int>(), (int)n, st));
+    c.scan_tmp.ensure(bytes + 16);
+    ST_HIP(hipcub::DeviceReduce::Max(c.scan_tmp.p, bytes, d, out.as<int>(), (int)n, st));
+    const int h = read_int(out.as<int>());
+    out.release();
+    return h;
+  }
+
+  // Rows of a Galerkin product with one wavefront per row: launch(pass, slots) runs the
+  // kernel instance holding `slots` columns in LDS (pass 0 needs none).
+  template <class F>
+  void wave_rows(Level& lv, F launch) {
+    const int64_t n = lv.n;
+    DBuf cnt;
+    cnt.ensure((n + 1) * sizeof(int));
+    lv.rowptr.ensure((n + 1) * sizeof(int));
+    ST_HIP(hipMemsetAsync(sbad.p, 0, sizeof(int), st));
+    ST_HIP(hipMemsetAsync(cnt.p, 0, (n + 1) * sizeof(int), st));
+    launch(0, 64, cnt.as<int>());
+    const int widest = max_int(cnt.as<int>(), n);
+    int slots = 64;
+    while (slots < widest && slots < 448) slots = slots == 256 ? 448 : slots * 2;
+    if (widest > slots) throw StError{ST_DEVICE, "a coarse operator row has more than 448 blocks"};
+    finish_rows(lv, cnt, [&](int pass) {
+      if (pass == 1) launch(1, slots, cnt.as<int>());
+    });
+    cnt.release();
+  }
+
+  // Level 1 of smoothed aggregation: P0 at the fine nodes, then A1 = P0^T K P0.
+  void build_sa_level1(Level& lv) {
+    const int64_t nn = g.nnodes, n = lv.n;
+    c.p0blocks = build_p(nn, n, c.p0row, c.p0col, c.p0val, c.p0t_ptr, c.p0t_blk, c.p0blk_row,
+                         [&](int pass, int* cnt, int* prow, int* pcol, double* pval) {
+                           k_p0<<<nblk(nn), 256, 0, st>>>(pass, nn, g.nflags.as<uint8_t>(), c.label1.as<int>(),
+                                                          lv.centre.as<double>(), g.nkey.as<uint32_t>(),
+                                                          g.nkind.as<uint8_t>(), g.ncell.as<int32_t>(),
+                                                          g.cfaces.as<uint32_t>(), g.ce64.as<double>(),
+                                                          g.cg64.as<double>(), g.cmask.as<uint16_t>(),
+                                                          g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
+                                                          c.dinv64.as<double>(), c.step0_d.as<double>(), cnt, prow,
+                                                          pcol, pval, sbad.as<int>());
+                         });
+    wave_rows(lv, [&](int pass, int slots, int* cnt) {
+#define ST_A1(S)                                                                                                   \
+  k_sa_a1_w<S><<<(unsigned)n, 64, 0, st>>>(pass, n, c.p0t_ptr.as<int>(), c.p0t_blk.as<int>(), c.p0blk_row.as<int>(), \
+                                          c.p0row.as<int>(), c.p0col.as<int>(), c.p0val.as<double>(),              \
+                                          g.nkind.as<uint8_t>(), g.ncell.as<int32_t>(), g.cfaces.as<uint32_t>(),   \
+                                          g.ce64.as<double>(), g.cg64.as<double>(), g.cmask.as<uint16_t>(),        \
+                                          g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), cnt, lv.rowptr.as<int>(), \
+                                          lv.col.as<int>(), lv.a64.as<double>(), sbad.as<int>())
+      if (slots <= 64)
+        ST_A1(64);
+      else if (slots <= 128)
+        ST_A1(128);
+      else if (slots <= 256)
+        ST_A1(256);
+      else
+        ST_A1(448);
+#undef ST_A1
+      ST_HIP(hipGetLastError());
+    });
+  }
+
+  // Above level 1: P of the level below (smoothed, or rigid when smooth is false), then this level = P^T A P.
+  void build_sa_coarse(Level& below, Level& lv, bool smooth) {
+    const int64_t nb_rows = below.n, n = lv.n;
+    below.pblocks = build_p(nb_rows, n, below.prow, below.pcol, below.pval64, below.pt_ptr, below.pt_blk,
+                            below.pblk_row, [&](int pass, int* cnt, int* prow, int* pcol, double* pval) {
+                              k_pc<<<nblk(nb_rows), 256, 0, st>>>(pass, smooth ? 1 : 0, nb_rows,
+                                                                  below.rowptr.as<int>(),
+                                                                  below.col.as<int>(), below.a64.as<double>(),
+                                                                  below.dinv64.as<double>(), below.step_d.as<double>(),
+                                                                  below.agg_struct.as<uint32_t>(),
+                                                                  below.parent.as<int>(), below.centre.as<double>(),
+                                                                  lv.centre.as<double>(), cnt, prow, pcol, pval,
+                                                                  sbad.as<int>());
+                            });
+    below.pval32.ensure((size_t)std::max<int64_t>(below.pblocks, 1) * 36 * sizeof(float));
+    k_to_f32<<<nblk(below.pblocks * 36), 256, 0, st>>>(below.pblocks * 36, below.pval64.as<double>(),
+                                                       below.pval32.as<float>());
+    wave_rows(lv, [&](int pass, int slots, int* cnt) {
+#define ST_GAL(S)                                                                                                   \
+  k_sa_gal_w<S><<<(unsigned)n, 64, 0, st>>>(pass, n, below.pt_ptr.as<int>(), below.pt_blk.as<int>(),                 \
+                                           below.pblk_row.as<int>(), below.prow.as<int>(), below.pcol.as<int>(),     \
+                                           below.pval64.as<double>(), below.rowptr.as<int>(), below.col.as<int>(),   \
+                                           below.a64.as<double>(), cnt, lv.rowptr.as<int>(), lv.col.as<int>(),       \
+                                           lv.a64.as<double>(), sbad.as<int>())
+      if (slots <= 64)
+        ST_GAL(64);
+      else if (slots <= 128)
+        ST_GAL(128);
+      else if (slots <= 256)
+        ST_GAL(256);
+      else
+        ST_GAL(448);
+#undef ST_GAL
+      ST_HIP(hipGetLastError());
+    });
+  }
+
   void coarse_step(Level& lv, const std::vector<int64_t>& seg) {
     const int64_t n = lv.n;
     DBuf v, w, sq;
@@ -1287,7 +2050,10 @@ struct Solver {
         keys.release();
         vals.release();
         skeys.release();
-        build_rows_a1(lv);
+        if (c.smoothed)
+          build_sa_level1(lv);
+        else
+          build_rows_a1(lv);
       } else {
         Level& below = c.levels.back();
         // parent of each aggregate below
@@ -1304,6 +2070,15 @@ struct Solver {
                                               lv.centre.as<double>());
         vals.release();
         skeys.release();
+        if (c.smoothed) {
+          build_sa_coarse(below, lv, (int)c.levels.size() < c.sa_levels);
+          coarse_step(lv, seg);
+          alloc_level_vectors(lv);
+          c.levels.push_back(std::move(lv));
+          prev = cnt;
+          if (c.levels.back().nblocks == c.levels.back().n) break;  // block diagonal: exact top
+          continue;
+        }
         DBuf cntb;
         cntb.ensure((cnt + 1) * sizeof(int));
         lv.rowptr.ensure((cnt + 1) * sizeof(int));
@@ -1375,12 +2150,21 @@ struct Solver {
     k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), r, x);
     k_c_resid<T><<<nblk(n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv), nullptr,
                                           nullptr, r, x, t);
-    k_restrict_c<T><<<nblk(up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, up.kid_ptr.as<int>(),
-                                                up.kids.as<int>(), up.centre.as<double>(), lv.centre.as<double>(), t,
-                                                lvbuf<T>(up, 0));
+    const T* pv = std::is_same<T, double>::value ? (const T*)lv.pval64.p : (const T*)lv.pval32.p;
+    if (lv.pblocks)
+      k_restrict_p<T><<<nblk(up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(),
+                                                  lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv, t, lvbuf<T>(up, 0));
+    else
+      k_restrict_c<T><<<nblk(up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, up.kid_ptr.as<int>(),
+                                                  up.kids.as<int>(), up.centre.as<double>(), lv.centre.as<double>(), t,
+                                                  lvbuf<T>(up, 0));
     coarse_cycle<T>(li + 1);
-    k_prolong_c<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lv.parent.as<int>(), lv.centre.as<double>(),
-                                            up.centre.as<double>(), lvbuf<T>(up, 1), x);
+    if (lv.pblocks)
+      k_prolong_p<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.pcol.as<int>(), pv,
+                                              lvbuf<T>(up, 1), x);
+    else
+      k_prolong_c<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lv.parent.as<int>(), lv.centre.as<double>(),
+                                              up.centre.as<double>(), lvbuf<T>(up, 1), x);
     k_c_resid<T><<<nblk(n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv), lvD<T>(lv),
                                           lv.step_d.as<double>(), r, x, t);
     ST_HIP(hipMemcpyAsync(x, t, n * 6 * sizeof(T), hipMemcpyDeviceToDevice, st));
@@ -1388,13 +2172,45 @@ struct Solver {
 
   // z = M^-1 r at the fine level; partial r.z per brick into c.partial
   template <class T>
-  void vcycle(const T* r, T* z, T* x1, T* r1) {
+  void vcycle(const T* r, T* z, T* x1, T* r1, T* s1) {
     const uint8_t* done = c.done.as<uint8_t>();
     const unsigned nb = (unsigned)g.nbricks;
     const T* dinv = std::is_same<T, double>::value ? (const T*)c.dinv64.p : (const T*)c.dinv32.p;
     k_jacobi_fine<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done, dinv,
                                             c.step0_d.as<double>(), r, x1);
-    if (!c.levels.empty()) {
+    if (!c.levels.empty() && c.smoothed) {
+      // r1 = r - K x1, s1 = w D^-1 r1; u = r1 - K s1 (= the smoothed restriction's argument);
+      // c = P_rigid^T u; coarse; s1 = P_rigid y; x1 += s1 - w D^-1 K s1; z = x1 + w D^-1 (r - K x1)
+      EpiResidJac<T> ej;
+      ej.done = done;
+      ej.b = r;
+      ej.dinv = dinv;
+      ej.step = c.step0_d.as<double>();
+      ej.r1 = r1;
+      ej.sout = s1;
+      launch_apply<T>(c, x1, ej);
+      EpiResid<T, T> er;
+      er.done = done;
+      er.partial = c.partial.as<double>();
+      er.b = r1;
+      er.r = r1;
+      launch_apply<T>(c, s1, er);
+      Level& l1 = c.levels[0];
+      k_restrict1<T><<<nblk(l1.n), 256, 0, st>>>(l1.n, l1.agg_struct.as<uint32_t>(), done, c.agg_ptr.as<int>(),
+                                                 c.nodes_by_agg.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
+                                                 g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), r1, lvbuf<T>(l1, 0));
+      coarse_cycle<T>(0);
+      k_prolong1_set<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done,
+                                               c.label1.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
+                                               g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), lvbuf<T>(l1, 1), s1);
+      EpiProlong<T> ep;
+      ep.done = done;
+      ep.u = s1;
+      ep.dinv = dinv;
+      ep.step = c.step0_d.as<double>();
+      ep.x = x1;
+      launch_apply<T>(c, s1, ep);
+    } else if (!c.levels.empty()) {
       EpiResid<T, T> er;
       er.done = done;
       er.partial = c.partial.as<double>();
@@ -1463,9 +2279,10 @@ struct Solver {
       c.z64.ensure(v64);
       c.t64.ensure(v64);
       c.u64.ensure(v64);
+      c.w64.ensure(v64);
       double *x = c.x64.as<double>(), *r = c.r64.as<double>(), *p = c.p64.as<double>(), *ap = c.ap64.as<double>(),
              *z = c.z64.as<double>();
-      vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>());
+      vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>(), c.w64.as<double>());
       reduce_bricks(c.partial, S_RZ);
       k_copy_fine<double, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, z, p);
       std::vector<uint8_t> dh, wh;
@@ -1486,7 +2303,7 @@ struct Solver {
           reduce_bricks(c.partial, S_RR);
           k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), 1,
                                                swant.as<uint8_t>(), 0.0);
-          vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>());
+          vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>(), c.w64.as<double>());
           reduce_bricks(c.partial, S_RZNEW);
           k_cg_beta<<<nblk(ns), 256, 0, st>>>(ns, scal, done);
           k_update_p<double><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z, p);
@@ -1518,13 +2335,14 @@ struct Solver {
       c.z32.ensure(v32);
       c.t32.ensure(v32);
       c.u64.ensure(v32);  // reused as the second fine FP32 scratch
+      c.w32.ensure(v32);
       float *x = c.x32.as<float>(), *r = c.r32.as<float>(), *p = c.p32.as<float>(), *ap = c.ap32.as<float>(),
             *z = c.z32.as<float>();
       float* x1 = c.t32.as<float>();
       float* r1 = c.u64.as<float>();
       ST_HIP(hipMemsetAsync(x, 0, v32, st));
       k_copy_fine<float, double><<<nb, kBlock, 0, st>>>(nbase, bst, nullptr, c.r64.as<double>(), r);
-      vcycle<float>(r, z, x1, r1);
+      vcycle<float>(r, z, x1, r1, c.w32.as<float>());
       reduce_bricks(c.partial, S_RZ);
       k_copy_fine<float, float><<<nb, kBlock, 0, st>>>(nbase, bst, done, z, p);
       std::vector<uint8_t> dh, wh;
@@ -1546,7 +2364,7 @@ struct Solver {
           reduce_bricks(c.partial, S_RR);
           k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), 1,
                                                swant.as<uint8_t>(), 1e-2);
-          vcycle<float>(r, z, x1, r1);
+          vcycle<float>(r, z, x1, r1, c.w32.as<float>());
           reduce_bricks(c.partial, S_RZNEW);
           k_cg_beta<<<nblk(ns), 256, 0, st>>>(ns, scal, done);
           k_update_p<float><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z, p);
@@ -1603,7 +2421,8 @@ void release_levels(Ctx& c) {
   for (auto& lv : c.levels)
     for (DBuf* b : {&lv.rowptr, &lv.col, &lv.a64, &lv.a32, &lv.dinv64, &lv.dinv32, &lv.centre, &lv.parent, &lv.rep,
                     &lv.key, &lv.kid_ptr, &lv.kids, &lv.agg_struct, &lv.step_d, &lv.r64, &lv.x64, &lv.t64, &lv.r32,
-                    &lv.x32, &lv.t32})
+                    &lv.x32, &lv.t32, &lv.prow, &lv.pcol, &lv.pval64, &lv.pval32, &lv.pt_ptr, &lv.pt_blk,
+                    &lv.pblk_row})
       b->release();
   c.levels.clear();
 }
@@ -1616,7 +2435,8 @@ void release_all(Ctx& c) {
     b->release();
   for (DBuf* b : {&c.fres, &c.xsol, &c.dinv64, &c.dinv32, &c.step0_d, &c.label1, &c.uf, &c.uf2, &c.flag, &c.scan_tmp, &c.sort_tmp,
                   &c.nodes_by_agg, &c.agg_ptr, &c.f64, &c.x64, &c.r64, &c.p64, &c.ap64, &c.z64, &c.t64, &c.u64, &c.x32,
-                  &c.r32, &c.p32, &c.ap32, &c.z32, &c.t32, &c.partial, &c.scal, &c.done})
+                  &c.r32, &c.p32, &c.ap32, &c.z32, &c.t32, &c.partial, &c.scal, &c.done, &c.w64, &c.w32, &c.p0row,
+                  &c.p0col, &c.p0val, &c.p0t_ptr, &c.p0t_blk, &c.p0blk_row})
     b->release();
   release_levels(c);
 }
@@ -1659,6 +2479,12 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   ST_HIP(hipEventCreate(&e1));
   ST_HIP(hipEventCreate(&e2));
   ST_HIP(hipEventRecord(e0, c.stream));
+  // smoothed aggregation unless STATICS_HIP_RIGID is set (rigid aggregates only, for comparison)
+  c.smoothed = std::getenv("STATICS_HIP_RIGID") == nullptr;
+  // how many transfers are smoothed, from the finest up (the others are rigid): smoothing
+  // every level widens the coarse stencils about 1.5x a level (STATICS_HIP_SA_LEVELS)
+  c.sa_levels = 2;
+  if (const char* e = std::getenv("STATICS_HIP_SA_LEVELS")) c.sa_levels = std::max(1, std::atoi(e));
   S.init_struct_arrays();
   S.setup_fine();
   S.setup_hierarchy();
