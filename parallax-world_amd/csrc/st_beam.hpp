@@ -108,6 +108,37 @@ ST_HD void beam_apply(const CellCoef<T>& c, const T* xi, const T* xj, T* yi, T* 
   yi[3] += L0 - M0; yi[4] += L1 - M1; yi[5] += L2 - M2;
 }
 
+// The same beam's load at one end only: end j (J) or end i, added into y.
+template <class T, int NX, int NY, int NZ, bool J>
+ST_HD void beam_end(const CellCoef<T>& c, const T* xi, const T* xj, T* y) {
+  const T ns = c.ns;
+  const T n0 = (T)NX * ns, n1 = (T)NY * ns, n2 = (T)NZ * ns;
+  const T w0 = xi[3] + xj[3], w1 = xi[4] + xj[4], w2 = xi[5] + xj[5];
+  const T cx0 = n1 * w2 - n2 * w1, cx1 = n2 * w0 - n0 * w2, cx2 = n0 * w1 - n1 * w0;
+  const T d0 = (xj[0] - xi[0]) + c.hd * cx0;
+  const T d1 = (xj[1] - xi[1]) + c.hd * cx1;
+  const T d2 = (xj[2] - xi[2]) + c.hd * cx2;
+  const T nd = n0 * d0 + n1 * d1 + n2 * d2;
+  const T F0 = c.kg * d0 + c.ke * nd * n0;
+  const T F1 = c.kg * d1 + c.ke * nd * n1;
+  const T F2 = c.kg * d2 + c.ke * nd * n2;
+  const T q0 = xj[3] - xi[3], q1 = xj[4] - xi[4], q2 = xj[5] - xi[5];
+  const T nq = n0 * q0 + n1 * q1 + n2 * q2;
+  const T M0 = c.re * q0 + c.rx * nq * n0;
+  const T M1 = c.re * q1 + c.rx * nq * n1;
+  const T M2 = c.re * q2 + c.rx * nq * n2;
+  const T L0 = -c.hk * (n1 * d2 - n2 * d1);
+  const T L1 = -c.hk * (n2 * d0 - n0 * d2);
+  const T L2 = -c.hk * (n0 * d1 - n1 * d0);
+  if (J) {
+    y[0] += F0; y[1] += F1; y[2] += F2;
+    y[3] += L0 + M0; y[4] += L1 + M1; y[5] += L2 + M2;
+  } else {
+    y[0] -= F0; y[1] -= F1; y[2] -= F2;
+    y[3] += L0 - M0; y[4] += L1 - M1; y[5] += L2 - M2;
+  }
+}
+
 // All live beams of one cell: x[f][6] the motions of its faces (f in VOXEL_FACES
 // order), y[f][6] accumulates. mask bit q = beam q alive. only_face >= 0 restricts
 // to the beams touching that face (the others' loads are still added, but the

@@ -49,15 +49,10 @@ void copy_geo_to_device(Ctx& c) {
   d.nbricks = (int64_t)h.cbase.size() - 1;
   d.ncells = (int64_t)h.ce.size();
   d.nnodes = (int64_t)h.nflags.size();
-  d.nhalo = (int64_t)h.hcell.size();
   d.nstruct = (int32_t)h.structs.size();
-  d.max_brick_nodes = h.max_brick_nodes;
   up(d.cbase, h.cbase, s);
   up(d.nbase, h.nbase, s);
-  up(d.hbase, h.hbase, s);
   up(d.bstruct, h.bstruct, s);
-  up(d.hcell, h.hcell, s);
-  up(d.ho, h.ho, s);
   up(d.ce64, h.ce, s);
   up(d.cg64, h.cg, s);
   up(d.cmask_user, h.cmask, s);
@@ -97,11 +92,11 @@ GeoD<T> geo_view(Ctx& c) {
   GeoD<T> g;
   g.cbase = d.cbase.as<uint32_t>();
   g.nbase = d.nbase.as<uint32_t>();
-  g.hbase = d.hbase.as<uint32_t>();
-  g.hcell = d.hcell.as<uint32_t>();
   g.bstruct = d.bstruct.as<uint32_t>();
   g.cfaces = d.cfaces.as<uint32_t>();
-  g.ho = d.ho.as<uint8_t>();
+  g.nflags = d.nflags.as<uint8_t>();
+  g.nkind = d.nkind.as<uint8_t>();
+  g.ncell = d.ncell.as<int32_t>();
   g.ce = std::is_same<T, double>::value ? (const T*)d.ce64.p : (const T*)d.ce32.p;
   g.cg = std::is_same<T, double>::value ? (const T*)d.cg64.p : (const T*)d.cg32.p;
   g.cmask = d.cmask.as<uint16_t>();
@@ -110,8 +105,6 @@ GeoD<T> geo_view(Ctx& c) {
 }
 template GeoD<double> geo_view<double>(Ctx&);
 template GeoD<float> geo_view<float>(Ctx&);
-
-size_t apply_smem(const Ctx& c, size_t tbytes) { return std::max<size_t>(16, (size_t)c.geo.max_brick_nodes * 6 * tbytes); }
 
 // Folds the recorded event pairs into the running total (keeps the pool small).
 void timing_collect(Ctx& c) {
@@ -128,13 +121,6 @@ void timing_collect(Ctx& c) {
 template <class T, class Epi>
 void launch_apply(Ctx& c, const T* x, const Epi& epi) {
   if (!c.geo.nbricks) return;
-  const size_t smem = apply_smem(c, sizeof(T));
-  static size_t attr_bytes = 0;  // one per instantiation
-  if (smem > attr_bytes) {
-    ST_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply<T, Epi>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)smem));
-    attr_bytes = smem;
-  }
   const bool timed = std::is_same<Epi, EpiDot<T>>::value && c.timing_which == (sizeof(T) == 4 ? 0 : 1);
   if (timed) {
     if (c.tev_used * 2 + 2 > c.tev.size()) {
@@ -147,7 +133,7 @@ void launch_apply(Ctx& c, const T* x, const Epi& epi) {
     }
     ST_HIP(hipEventRecord(c.tev[2 * c.tev_used], c.stream));
   }
-  k_apply<T, Epi><<<(unsigned)c.geo.nbricks, kBlock, smem, c.stream>>>(geo_view<T>(c), x, epi);
+  k_apply<T, Epi><<<(unsigned)c.geo.nbricks, kBlock, 0, c.stream>>>(geo_view<T>(c), x, epi);
   ST_HIP(hipGetLastError());
   if (timed) {
     ST_HIP(hipEventRecord(c.tev[2 * c.tev_used + 1], c.stream));

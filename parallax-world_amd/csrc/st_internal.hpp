@@ -4,8 +4,7 @@
 // concatenates them, structure after structure):
 //   bricks   8^3 cells each, numbered in Morton order within a structure; only
 //            bricks holding a free cell exist. Per brick: the range of its free
-//            cells [cbase[b], cbase[b+1]), of its nodes [nbase[b], nbase[b+1]),
-//            and of the halo cells it recomputes [hbase[b], hbase[b+1]).
+//            cells [cbase[b], cbase[b+1]) and of its nodes [nbase[b], nbase[b+1]).
 //   cells    the free cells, brick-major, slot order within a brick (slot
 //            ((z&7)*8 + (y&7))*8 + (x&7), lattice/src/lib.rs:310-314). Per cell:
 //            e, g (FP64 and FP32), the live-beam mask, its six face nodes
@@ -80,10 +79,8 @@ struct StructInfo {
 
 // What the host builds from cells (st_upload); copied to the device as is.
 struct HostGeo {
-  std::vector<uint32_t> cbase, nbase, hbase;  // per brick + 1
+  std::vector<uint32_t> cbase, nbase;         // per brick + 1
   std::vector<uint32_t> bstruct;              // per brick: structure
-  std::vector<uint32_t> hcell;                // halo cell (compact id)
-  std::vector<uint8_t> ho;                    // halo orientation
   std::vector<double> ce, cg;                 // per free cell
   std::vector<uint16_t> cmask;                // per free cell, alive beams
   std::vector<uint32_t> cfaces;               // 6 per free cell
@@ -96,16 +93,14 @@ struct HostGeo {
   std::vector<uint32_t> nkey;                 // per node: its key cell (the cell keeping it), packed as ccoord
   std::vector<int32_t> nface;                 // 4 per node: o, x, y, z (+ side cell, lattice units)
   std::vector<StructInfo> structs;
-  uint32_t max_brick_nodes = 0;
   void clear() { *this = HostGeo(); }
 };
 
 // Device copies of the geometry.
 struct DevGeo {
-  int64_t nbricks = 0, ncells = 0, nnodes = 0, nhalo = 0;
+  int64_t nbricks = 0, ncells = 0, nnodes = 0;
   int32_t nstruct = 0;
-  uint32_t max_brick_nodes = 0;
-  DBuf cbase, nbase, hbase, bstruct, hcell, ho;
+  DBuf cbase, nbase, bstruct;
   DBuf ce64, cg64, ce32, cg32, cmask_user, cmask, cfaces, ccoord, cstruct;
   DBuf nflags, ncell, nkind, nkey;
   DBuf bgeo;  // BeamGeo per structure
@@ -145,7 +140,7 @@ struct Ctx {
   bool uploaded = false;
   std::vector<uint8_t> gone_host;  // per node, last set
   // fine-level smoother
-  DBuf dinv64, dinv32;              // 36 per node (row-major), zero for non-free nodes
+  DBuf dinv64, dinv32;              // 21 per node (upper triangle), zero for non-free nodes
   std::vector<double> step0;        // per structure
   DBuf step0_d;
   // fine-node aggregation (level 1) and the union-find scratch
@@ -156,6 +151,11 @@ struct Ctx {
   int64_t p0blocks = 0;
   bool smoothed = true;  // smoothed aggregation (false: rigid aggregates only)
   int sa_levels = 2;     // transfers smoothed, from the finest up
+  int nu = 1;            // fine smoothing steps before and after the coarse correction
+  int nu1 = 1;           // the same at level 1
+  int nu_c = 1;          // the same above level 1
+  int gamma = 1;         // coarse corrections per visit above level 1 (1: V-cycle, 2: W-cycle)
+  int gamma_from = 1;    // first level (0: level 1) that repeats its coarse correction
   std::vector<Level> levels;        // coarse levels 1..
   // solver vectors
   DBuf f64, x64, r64, p64, ap64, z64, t64, u64, w64;

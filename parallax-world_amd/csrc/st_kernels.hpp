@@ -3,13 +3,12 @@
 //
 // K x (the reference's Level::apply at level 0, statics/src/bodies.rs:193-206,
 // with the level's blocks assembled per bond in Level::new, :126-164) is
-// computed cell by cell from the closed-form beams of st_beam.hpp. One
-// workgroup owns one brick: it applies the brick's free cells and, for the
-// nodes on the brick's lower faces, the free cells across them (the "halo"),
-// accumulating into LDS with native ds_add; then every node of the brick has
-// its complete row and an epilogue consumes it (store, residual, smoothing,
-// dot product). No global atomics: every node is written by exactly one
-// workgroup, and the sum into each node runs in a fixed set of terms.
+// computed node by node from the closed-form beams of st_beam.hpp. One
+// workgroup owns one brick and one thread one node: the thread gathers the
+// motions of the faces of the node's (at most two) cells and sums the loads of
+// the (at most ten) beams ending on the node, then an epilogue consumes the
+// row in registers (store, residual, smoothing, dot product). No LDS staging
+// and no atomics: every row is summed by one thread in a fixed order.
 #pragma once
 #include "st_internal.hpp"
 
@@ -19,8 +18,9 @@ constexpr int kBlock = 256;
 
 template <class T>
 struct GeoD {
-  const uint32_t *cbase, *nbase, *hbase, *hcell, *bstruct, *cfaces;
-  const uint8_t* ho;
+  const uint32_t *cbase, *nbase, *bstruct, *cfaces;
+  const uint8_t *nflags, *nkind;
+  const int32_t* ncell;
   const T *ce, *cg;
   const uint16_t* cmask;
   const BeamGeo* bgeo;
@@ -63,39 +63,49 @@ __device__ __forceinline__ void store6<float>(float* __restrict__ p, const float
   q[2] = make_float2(v[4], v[5]);
 }
 
-// One free cell's beams applied to x; the loads of its faces kept in this brick
-// (node index in [n0, n0 + nn), free) are added into acc. keep: beams to apply;
-// only: if >= 0, only that face's loads are kept (halo cells).
-template <class T>
-__device__ __forceinline__ void cell_work(const GeoD<T>& g, const BeamGeo& G, const T* __restrict__ x, uint32_t c,
-                                          uint32_t keep, int only, T* acc, uint32_t n0, uint32_t nn) {
-  T xv[6][6], yv[6][6];
-  uint32_t fi[6];
+// The load at face F of one cell from the cell's live beams that end there (the five
+// beams (F, f2)), given the motions xv of its six faces. F is known at compile time,
+// so each beam's direction, class and end are constants.
+template <class T, int F, int Q>
+__device__ __forceinline__ void face_beam(const CellCoef<T>& c0, const CellCoef<T>& c1, uint32_t mask,
+                                          const T (*xv)[6], T* y) {
+  if constexpr (kPairA[Q] == F || kPairB[Q] == F) {
+    if (mask & (1u << Q))
+      beam_end<T, pair_dir(Q, 0), pair_dir(Q, 1), pair_dir(Q, 2), kPairB[Q] == F>(pair_opposite(Q) ? c0 : c1,
+                                                                                 xv[kPairA[Q]], xv[kPairB[Q]], y);
+  }
+  if constexpr (Q < 14) face_beam<T, F, Q + 1>(c0, c1, mask, xv, y);
+}
+
+template <class T, int F>
+__device__ __forceinline__ void cell_face_row(const GeoD<T>& g, const BeamGeo& G, const T* __restrict__ x, uint32_t c,
+                                              T* y) {
+  const uint32_t mask = (uint32_t)g.cmask[c] & face_beams(F);
+  if (!mask) return;
+  T xv[6][6];
+  const uint2* fp = reinterpret_cast<const uint2*>(g.cfaces + 6 * (size_t)c);
+  const uint2 f01 = fp[0], f23 = fp[1], f45 = fp[2];
+  const uint32_t fi[6] = {f01.x, f01.y, f23.x, f23.y, f45.x, f45.y};
 #pragma unroll
   for (int f = 0; f < 6; f++) {
-    fi[f] = g.cfaces[6 * c + f];
     if (fi[f] & kNotFree) {
 #pragma unroll
       for (int k = 0; k < 6; k++) xv[f][k] = T(0);
     } else {
       load6<T>(x + 6 * (size_t)fi[f], xv[f]);
     }
-#pragma unroll
-    for (int k = 0; k < 6; k++) yv[f][k] = T(0);
   }
   const T e = g.ce[c], gg = g.cg[c];
-  const uint32_t mask = (uint32_t)g.cmask[c] & keep;
-  const CellCoef<T> c0 = cell_coef<T>(G, 0, e, gg), c1 = cell_coef<T>(G, 1, e, gg);
-  cell_apply<T>(c0, c1, mask, xv, yv);
-#pragma unroll
-  for (int f = 0; f < 6; f++) {
-    if (only >= 0 && f != only) continue;
-    if (fi[f] & kNotFree) continue;
-    const uint32_t local = fi[f] - n0;
-    if (local >= nn) continue;
-#pragma unroll
-    for (int k = 0; k < 6; k++) atomicAdd(&acc[6 * local + k], yv[f][k]);
-  }
+  face_beam<T, F, 0>(cell_coef<T>(G, 0, e, gg), cell_coef<T>(G, 1, e, gg), mask, xv, y);
+}
+
+// Row of K x at a free node of orientation O: face 2O+1 of the cell below (c-) and
+// face 2O of the cell above (c+), where those cells are free.
+template <class T, int O>
+__device__ __forceinline__ void node_row(const GeoD<T>& g, const BeamGeo& G, const T* __restrict__ x, int2 cells,
+                                         T* y) {
+  if (cells.x >= 0) cell_face_row<T, 2 * O + 1>(g, G, x, (uint32_t)cells.x, y);
+  if (cells.y >= 0) cell_face_row<T, 2 * O>(g, G, x, (uint32_t)cells.y, y);
 }
 
 // Block-wide sum of one double per thread; the result is valid in thread 0.
@@ -111,12 +121,12 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
-// y = K x over bricks, then epi.node(n, row, s) for each node of the brick.
+// y = K x, one workgroup per brick and one thread per node: every node gathers the
+// beams of its two cells that end on it, so each row is summed by one thread in a
+// fixed order, with no atomics, and epi.node(n, row, s) consumes it in registers.
 // Epi::kReduce: the per-node values returned are summed per brick into epi.partial[b].
 template <class T, class Epi>
 __global__ __launch_bounds__(kBlock) void k_apply(GeoD<T> g, const T* __restrict__ x, Epi epi) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* acc = reinterpret_cast<T*>(smem_raw);
   __shared__ double red[kBlock / 64];
   const uint32_t b = blockIdx.x;
   const uint32_t s = g.bstruct[b];
@@ -124,22 +134,25 @@ __global__ __launch_bounds__(kBlock) void k_apply(GeoD<T> g, const T* __restrict
     if (Epi::kReduce && threadIdx.x == 0) epi.partial[b] = 0.0;
     return;
   }
-  const uint32_t n0 = g.nbase[b], nn = g.nbase[b + 1] - n0;
-  for (uint32_t i = threadIdx.x; i < nn * 6; i += blockDim.x) acc[i] = T(0);
   const BeamGeo G = g.bgeo[s];
-  __syncthreads();
-  const uint32_t c_end = g.cbase[b + 1];
-  for (uint32_t c = g.cbase[b] + threadIdx.x; c < c_end; c += blockDim.x) cell_work<T>(g, G, x, c, 0x7fffu, -1, acc, n0, nn);
-  const uint32_t h_end = g.hbase[b + 1];
-  for (uint32_t h = g.hbase[b] + threadIdx.x; h < h_end; h += blockDim.x) {
-    const int f = 2 * (int)g.ho[h] + 1;
-    cell_work<T>(g, G, x, g.hcell[h], face_beams(f), f, acc, n0, nn);
-  }
-  __syncthreads();
+  const uint32_t n_end = g.nbase[b + 1];
   double part = 0.0;
-  for (uint32_t i = threadIdx.x; i < nn; i += blockDim.x) part += epi.node(n0 + i, acc + 6 * i, s);
+  for (uint32_t v = g.nbase[b] + threadIdx.x; v < n_end; v += blockDim.x) {
+    T y[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (!(g.nflags[v] & (kHeld | kGone))) {
+      const int o = g.nkind[v] >> 1;
+      const int2 cells = reinterpret_cast<const int2*>(g.ncell)[v];
+      if (o == 0)
+        node_row<T, 0>(g, G, x, cells, y);
+      else if (o == 1)
+        node_row<T, 1>(g, G, x, cells, y);
+      else
+        node_row<T, 2>(g, G, x, cells, y);
+    }
+    part += epi.node(v, y, s);
+  }
   if (Epi::kReduce) {
-    double t = block_sum(part, red);
+    const double t = block_sum(part, red);
     if (threadIdx.x == 0) epi.partial[b] = t;
   }
 }
@@ -199,6 +212,27 @@ struct EpiResid : EpiBase {
   }
 };
 
+// Symmetric 6x6 blocks (the inverse diagonal blocks D^-1) are stored as their upper
+// triangle row by row: 21 values, (i, j) with i <= j at 6 i - i (i - 1) / 2 + j - i.
+__host__ __device__ constexpr int sym_idx(int i, int j) {
+  return i <= j ? 6 * i - i * (i - 1) / 2 + (j - i) : 6 * j - j * (j - 1) / 2 + (i - j);
+}
+constexpr int kSym = 21;
+
+template <class T>
+__device__ __forceinline__ void sym6v(const T* __restrict__ m, const T* v, T* out) {
+  T a[kSym];
+#pragma unroll
+  for (int k = 0; k < kSym; k++) a[k] = m[k];
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    T s = 0;
+#pragma unroll
+    for (int j = 0; j < 6; j++) s += a[sym_idx(i, j)] * v[j];
+    out[i] = s;
+  }
+}
+
 // 6x6 row-major times vector
 template <class T>
 __device__ __forceinline__ void mat6v(const T* __restrict__ m, const T* v, T* out) {
@@ -226,7 +260,7 @@ struct EpiSmooth : EpiBase {
     load6<T>(xin + 6 * (size_t)n, xv);
 #pragma unroll
     for (int k = 0; k < 6; k++) t[k] = bv[k] - a[k];
-    mat6v<T>(dinv + 36 * (size_t)n, t, d);
+    sym6v<T>(dinv + kSym * (size_t)n, t, d);
     const T w = (T)step[s];
     double acc = 0;
 #pragma unroll
@@ -254,7 +288,7 @@ struct EpiResidJac : EpiBase {
 #pragma unroll
     for (int k = 0; k < 6; k++) t[k] = bv[k] - a[k];
     store6<T>(r1 + 6 * (size_t)n, t);
-    mat6v<T>(dinv + 36 * (size_t)n, t, d);
+    sym6v<T>(dinv + kSym * (size_t)n, t, d);
     const T w = (T)step[s];
 #pragma unroll
     for (int k = 0; k < 6; k++) d[k] *= w;
@@ -275,7 +309,7 @@ struct EpiProlong : EpiBase {
     T uv[6], xv[6], d[6];
     load6<T>(u + 6 * (size_t)n, uv);
     load6<T>(x + 6 * (size_t)n, xv);
-    mat6v<T>(dinv + 36 * (size_t)n, a, d);
+    sym6v<T>(dinv + kSym * (size_t)n, a, d);
     const T w = (T)step[s];
 #pragma unroll
     for (int k = 0; k < 6; k++) xv[k] += uv[k] - w * d[k];

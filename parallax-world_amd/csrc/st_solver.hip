@@ -35,6 +35,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
 #include <cstdlib>
 
 #include "st_solver.hpp"
@@ -187,6 +188,26 @@ __device__ void beam_coarse_add(const double* Gv, const double* Gw, int sv, int 
   const double ss = (double)(sv * sw);
   for (int r = 0; r < 3; r++)
     for (int c = 0; c < 3; c++) out[6 * (3 + r) + 3 + c] += ss * Rm[3 * r + c];
+}
+
+// Row vector pc^T K_b for the beam block K_b = Gv^T Sf Gw + sv sw diag(0, Rm) of
+// beam_coarse_add at zero offsets (Gv = [sv I, hd [n]x], Gw = [sw I, hd [n]x]):
+// a = Gv pc, b = Sf a, then pc^T K_b = [sw b, -hd n x b + sv sw Rm pc_w].
+__device__ __forceinline__ void beam_row(const double* pc, int sv, int sw, const double* n, double hd, double K4,
+                                         double R2, double e, double g, double* qr) {
+  double a[3];
+  a[0] = sv * pc[0] + hd * (n[1] * pc[5] - n[2] * pc[4]);
+  a[1] = sv * pc[1] + hd * (n[2] * pc[3] - n[0] * pc[5]);
+  a[2] = sv * pc[2] + hd * (n[0] * pc[4] - n[1] * pc[3]);
+  const double na = n[0] * a[0] + n[1] * a[1] + n[2] * a[2];
+  double b[3];
+  for (int k = 0; k < 3; k++) b[k] = K4 * (g * a[k] + (e - g) * na * n[k]);
+  const double nw = n[0] * pc[3] + n[1] * pc[4] + n[2] * pc[5];
+  const double ss = (double)(sv * sw);
+  for (int k = 0; k < 3; k++) qr[k] = sw * b[k];
+  qr[3] = -hd * (n[1] * b[2] - n[2] * b[1]) + ss * R2 * (e * pc[3] + (g + g - e) * nw * n[0]);
+  qr[4] = -hd * (n[2] * b[0] - n[0] * b[2]) + ss * R2 * (e * pc[4] + (g + g - e) * nw * n[1]);
+  qr[5] = -hd * (n[0] * b[1] - n[1] * b[0]) + ss * R2 * (e * pc[5] + (g + g - e) * nw * n[2]);
 }
 
 // ------------------------------------------------------------- union-find
@@ -347,10 +368,13 @@ __global__ void k_dinv(int64_t nnodes, const uint8_t* nflags, const int32_t* nce
   } else {
     for (int i = 0; i < 36; i++) inv[i] = 0.0;
   }
-  for (int i = 0; i < 36; i++) {
-    dinv64[36 * v + i] = inv[i];
-    dinv32[36 * v + i] = (float)inv[i];
-  }
+  for (int i = 0; i < 6; i++)
+    for (int j = i; j < 6; j++) {
+      // the inverse of a symmetric block, symmetrised against rounding
+      const double a = 0.5 * (inv[6 * i + j] + inv[6 * j + i]);
+      dinv64[kSym * v + sym_idx(i, j)] = a;
+      dinv32[kSym * v + sym_idx(i, j)] = (float)a;
+    }
 }
 
 // ----------------------------------------------------------- brick vector ops
@@ -366,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_fine(const uint32_t* nbase, c
   for (uint32_t n = nbase[b] + threadIdx.x; n < nbase[b + 1]; n += blockDim.x) {
     T rv[6], d[6];
     load6<T>(r + 6 * (size_t)n, rv);
-    mat6v<T>(dinv + 36 * (size_t)n, rv, d);
+    sym6v<T>(dinv + kSym * (size_t)n, rv, d);
     for (int k = 0; k < 6; k++) d[k] *= w;
     store6<T>(x + 6 * (size_t)n, d);
   }
@@ -806,34 +830,51 @@ __global__ void k_c_jacobi(int64_t n, const uint32_t* ast, const uint8_t* done, 
 }
 
 // t = r - A x  (mode 0)   or   z = x + step D^-1 (r - A x)  (mode 1)
+// Coarse block rows are worked by groups of 8 lanes: lane i < 6 of a group computes
+// component i of its row, reading row i of every 6x6 block (the group reads each
+// block as one contiguous 144-byte run); lanes 6 and 7 idle.
+template <class T>
+__device__ __forceinline__ T row6_dot(const T* __restrict__ m, const T* v) {
+  T s = 0;
+#pragma unroll
+  for (int j = 0; j < 6; j++) s += m[j] * v[j];
+  return s;
+}
+// The six components of the group's vector, gathered from its lanes 0..5.
+template <class T>
+__device__ __forceinline__ void group_gather6(T a, T* all) {
+  const int base = (threadIdx.x & 63) & ~7;
+#pragma unroll
+  for (int j = 0; j < 6; j++) all[j] = __shfl(a, base + j, 64);
+}
+
+// mode 0: out = r - A x; mode 1: out = x + w D^-1 (r - A x)
 template <class T>
 __global__ void k_c_resid(int mode, int64_t n, const uint32_t* ast, const uint8_t* done, const int* rowptr, const int* col,
                           const T* A, const T* dinv, const double* step, const T* r, const T* x, T* out) {
-  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= n) return;
-  const uint32_t s = ast[I];
-  if (done && done[s]) return;
-  T acc[6];
-  load6<T>(r + 6 * I, acc);
-  for (int b = rowptr[I]; b < rowptr[I + 1]; b++) {
-    const T* M = A + 36 * (int64_t)b;
-    T xv[6];
-    load6<T>(x + 6 * (int64_t)col[b], xv);
-    for (int i = 0; i < 6; i++) {
-      T t = 0;
-      for (int j = 0; j < 6; j++) t += M[6 * i + j] * xv[j];
-      acc[i] -= t;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t I = t >> 3;
+  const int i = (int)(t & 7);
+  const bool live = I < n && !(done && done[ast[I]]);
+  T acc = 0;
+  if (live && i < 6) {
+    acc = r[6 * I + i];
+    const int b1 = rowptr[I + 1];
+    for (int b = rowptr[I]; b < b1; b++) {
+      T xv[6];
+      load6<T>(x + 6 * (int64_t)col[b], xv);
+      acc -= row6_dot<T>(A + 36 * (int64_t)b + 6 * i, xv);
     }
   }
   if (mode == 0) {
-    store6<T>(out + 6 * I, acc);
-  } else {
-    T d[6], xv[6];
-    mat6v<T>(dinv + 36 * I, acc, d);
-    load6<T>(x + 6 * I, xv);
-    const T w = (T)step[s];
-    for (int k = 0; k < 6; k++) xv[k] += w * d[k];
-    store6<T>(out + 6 * I, xv);
+    if (live && i < 6) out[6 * I + i] = acc;
+    return;
+  }
+  T all[6];
+  group_gather6<T>(acc, all);
+  if (live && i < 6) {
+    const T d = row6_dot<T>(dinv + 36 * I + 6 * i, all);
+    out[6 * I + i] = x[6 * I + i] + (T)step[ast[I]] * d;
   }
 }
 
@@ -969,7 +1010,6 @@ __global__ void k_seg_from_ids(int ns, const int64_t* node0, const int* ids, con
 // owned by one thread, so there are no atomics and the sums run in a fixed order.
 constexpr int kMaxP = 16;   // blocks in one row of the fine P0 (a node has at most 10 beams)
 constexpr int kMaxPc = 96;  // blocks in one row of a coarse P
-constexpr int kMaxR = 192;  // blocks in one row of a smoothed coarse operator
 
 __device__ __forceinline__ void mat6_mul_add(const double* a, const double* b, double* out) {  // out += a b
   for (int r = 0; r < 6; r++)
@@ -1107,7 +1147,9 @@ __global__ void k_p0(int pass, int64_t nn, const uint8_t* nflags, const int* lab
   }
   // P[v, J] = [J = I0] Sh(r_v) - w D^-1 (K chi_J)(v)
   const double w = step[sidx];
-  const double* D = dinv + 36 * v;
+  double D[36];
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) D[6 * i + j] = dinv[kSym * v + sym_idx(i, j)];
   for (int t = 0; t < nj; t++) {
     double* blk = out + 36 * t;
     double tmp[36];
@@ -1120,80 +1162,6 @@ __global__ void k_p0(int pass, int64_t nn, const uint8_t* nflags, const int* lab
   for (int i = 0; i < 36; i++) out[i] += S[i];
 }
 
-// Level 1 = P0^T K P0, row I gathered over the fine nodes v in column I of P0:
-// row I += P0[v,I]^T sum over the beams b at v of (K_b,vv P0[v,:] + K_b,vw P0[w,:]).
-__global__ void k_sa_a1(int pass, int64_t n1, const int* pt_ptr, const int* pt_blk, const int* pblk_row,
-                        const int* prow, const int* pcol, const double* pval, const uint8_t* nkind,
-                        const int32_t* ncell, const uint32_t* cfaces, const double* ce, const double* cg,
-                        const uint16_t* cmask, const uint32_t* cstruct, const BeamGeo* bgeo, int* rowcnt,
-                        const int* rowptr, int* col, double* A, int* overflow) {
-  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= n1) return;
-  int js[kMaxR];
-  int nj = 1;
-  js[0] = (int)I;
-  int base = 0, len = 0;
-  if (pass == 1) {
-    base = rowptr[I];
-    len = rowptr[I + 1] - base;
-    for (int i = 0; i < 36 * len; i++) A[36 * (int64_t)base + i] = 0.0;
-    col[base] = (int)I;
-  }
-  const double zero[3] = {0, 0, 0};
-  for (int kk = pt_ptr[I]; kk < pt_ptr[I + 1]; kk++) {
-    const int blk = pt_blk[kk];
-    const int v = pblk_row[blk];
-    const double* PvI = pval + 36 * (int64_t)blk;
-    BeamAt bl[10];
-    const int nbm = beams_at((uint32_t)v, nkind, ncell, cfaces, cmask, bl);
-    for (int t = 0; t < nbm; t++) {
-      const BeamAt& b = bl[t];
-      const bool wfree = !(b.wf & kNotFree);
-      // columns this beam reaches: P0 row v and, if w is free, P0 row w
-      for (int side = 0; side < (wfree ? 2 : 1); side++) {
-        const int row = side == 0 ? v : (int)b.wf;
-        for (int k2 = prow[row]; k2 < prow[row + 1]; k2++) {
-          const int J = pcol[k2];
-          int sl = slot_of(js, nj, J);
-          if (sl < 0) {
-            if (nj < kMaxR) {
-              js[nj] = J;
-              sl = nj++;
-              if (pass == 1) col[base + sl] = J;
-            } else {
-              atomicExch(overflow, 1);
-              continue;
-            }
-          }
-        }
-      }
-      if (pass == 0) continue;
-      const BeamGeo G = bgeo[cstruct[b.c]];
-      double n[3];
-      int cls;
-      pair_n(b.q, G, n, cls);
-      const double e = ce[b.c], g = cg[b.c];
-      double Bv[18], Bw[18], K[36], Q[36];
-      gmat(b.sv, n, G.hd[cls], zero, Bv);
-      for (int side = 0; side < (wfree ? 2 : 1); side++) {
-        for (int i = 0; i < 36; i++) K[i] = 0.0;
-        if (side == 0) {
-          beam_coarse_add(Bv, Bv, b.sv, b.sv, n, G.K4[cls], G.R2[cls], e, g, K);
-        } else {
-          gmat(-b.sv, n, G.hd[cls], zero, Bw);
-          beam_coarse_add(Bv, Bw, b.sv, -b.sv, n, G.K4[cls], G.R2[cls], e, g, K);
-        }
-        mat6_tmul(PvI, K, Q);  // Q = P0[v,I]^T K_b
-        const int row = side == 0 ? v : (int)b.wf;
-        for (int k2 = prow[row]; k2 < prow[row + 1]; k2++) {
-          const int sl = slot_of(js, nj, pcol[k2]);
-          if (sl >= 0 && sl < len) mat6_mul_add(Q, pval + 36 * (int64_t)k2, A + 36 * ((int64_t)base + sl));
-        }
-      }
-    }
-  }
-  if (pass == 0) rowcnt[I] = nj;
-}
 
 // Rows of P = (I - w D^-1 A) P_rigid at an explicit level (rows i, columns the next level's J).
 __global__ void k_pc(int pass, int smooth, int64_t n, const int* rowptr, const int* colA, const double* A,
@@ -1247,51 +1215,6 @@ __global__ void k_pc(int pass, int smooth, int64_t n, const int* rowptr, const i
   for (int k = 0; k < 36; k++) out[k] += S[k];
 }
 
-// Next level = P^T A P, row I gathered over the rows i in column I of P:
-// row I += P[i,I]^T sum over blocks (i, j) of A of A[i,j] P[j,:].
-__global__ void k_sa_gal(int pass, int64_t nI, const int* pt_ptr, const int* pt_blk, const int* pblk_row,
-                         const int* prow, const int* pcol, const double* pval, const int* rowptrA, const int* colA,
-                         const double* Aa, int* rowcnt, const int* rowptr, int* col, double* A, int* overflow) {
-  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= nI) return;
-  int js[kMaxR];
-  int nj = 1;
-  js[0] = (int)I;
-  int base = 0, len = 0;
-  if (pass == 1) {
-    base = rowptr[I];
-    len = rowptr[I + 1] - base;
-    for (int k = 0; k < 36 * len; k++) A[36 * (int64_t)base + k] = 0.0;
-    col[base] = (int)I;
-  }
-  for (int kk = pt_ptr[I]; kk < pt_ptr[I + 1]; kk++) {
-    const int blk = pt_blk[kk];
-    const int i = pblk_row[blk];
-    const double* PiI = pval + 36 * (int64_t)blk;
-    for (int b = rowptrA[i]; b < rowptrA[i + 1]; b++) {
-      const int j = colA[b];
-      for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) {
-        const int J = pcol[k2];
-        if (slot_of(js, nj, J) < 0) {
-          if (nj < kMaxR) {
-            if (pass == 1) col[base + nj] = J;
-            js[nj++] = J;
-          } else {
-            atomicExch(overflow, 1);
-          }
-        }
-      }
-      if (pass == 0) continue;
-      double Q[36];
-      mat6_tmul(PiI, Aa + 36 * (int64_t)b, Q);
-      for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) {
-        const int sl = slot_of(js, nj, pcol[k2]);
-        if (sl >= 0 && sl < len) mat6_mul_add(Q, pval + 36 * (int64_t)k2, A + 36 * ((int64_t)base + sl));
-      }
-    }
-  }
-  if (pass == 0) rowcnt[I] = nj;
-}
 
 // ---- Galerkin rows, one wavefront per output row. The row's columns go into an LDS
 // hash set (open addressing), are sorted (diagonal first, then ascending) and numbered,
@@ -1299,33 +1222,39 @@ __global__ void k_sa_gal(int pass, int64_t nI, const int* pt_ptr, const int* pt_
 // row's columns; pass 1 fills col and A. kSlots bounds the columns of one row.
 constexpr int kHash = 2048;
 
-__device__ __forceinline__ unsigned hslot(int J) { return ((unsigned)J * 2654435761u) & (kHash - 1); }
+template <int kH>
+__device__ __forceinline__ unsigned hslot(int J) {
+  return ((unsigned)J * 2654435761u) & (kH - 1);
+}
+template <int kH>
 __device__ __forceinline__ void hinsert(int* keys, int J) {
-  unsigned h = hslot(J);
+  unsigned h = hslot<kH>(J);
   for (;;) {
     const int old = atomicCAS(&keys[h], -1, J);
     if (old == -1 || old == J) return;
-    h = (h + 1) & (kHash - 1);
+    h = (h + 1) & (kH - 1);
   }
 }
+template <int kH>
 __device__ __forceinline__ int hfind(const int* keys, const int* slot, int J) {
-  unsigned h = hslot(J);
+  unsigned h = hslot<kH>(J);
   for (;;) {
     const int k = keys[h];
     if (k == J) return slot[h];
     if (k == -1) return -1;
-    h = (h + 1) & (kHash - 1);
+    h = (h + 1) & (kH - 1);
   }
 }
 
 // Shared part of both row kernels once the hash holds the row's columns: count them
 // (pass 0) or sort and number them (pass 1, when they fit in cap). Returns the count,
-// the diagonal included. Needs blockDim 64.
+// the diagonal included.
+template <int kH>
 __device__ int row_columns(int pass, int I, int* keys, int* slot, int* sorted, int cap, int* ncount) {
   __syncthreads();
   if (threadIdx.x == 0) *ncount = 0;
   __syncthreads();
-  for (int h = threadIdx.x; h < kHash; h += blockDim.x)
+  for (int h = threadIdx.x; h < kH; h += blockDim.x)
     if (keys[h] != -1 && keys[h] != I) {
       const int p = atomicAdd(ncount, 1);
       if (pass == 1 && p < cap) sorted[p] = keys[h];
@@ -1338,13 +1267,13 @@ __device__ int row_columns(int pass, int I, int* keys, int* slot, int* sorted, i
       const int key = sorted[a];
       int rank = 0;
       for (int b = 0; b < m; b++) rank += sorted[b] < key;
-      unsigned h = hslot(key);
-      while (keys[h] != key) h = (h + 1) & (kHash - 1);
+      unsigned h = hslot<kH>(key);
+      while (keys[h] != key) h = (h + 1) & (kH - 1);
       slot[h] = 1 + rank;
     }
     if (threadIdx.x == 0) {
-      unsigned h = hslot(I);
-      while (keys[h] != I) h = (h + 1) & (kHash - 1);
+      unsigned h = hslot<kH>(I);
+      while (keys[h] != I) h = (h + 1) & (kH - 1);
       slot[h] = 0;
     }
   }
@@ -1352,103 +1281,186 @@ __device__ int row_columns(int pass, int I, int* keys, int* slot, int* sorted, i
   return m + 1;
 }
 
-template <int kSlots>
-__global__ __launch_bounds__(64) void k_sa_gal_w(int pass, int64_t nI, const int* pt_ptr, const int* pt_blk,
-                                                 const int* pblk_row, const int* prow, const int* pcol,
-                                                 const double* pval, const int* rowptrA, const int* colA,
-                                                 const double* Aa, int* rowcnt, const int* rowptr, int* col, double* A,
-                                                 int* overflow) {
-  __shared__ int keys[kHash];
-  __shared__ int slot[kHash];
+// Inclusive sum over a 256-thread block (wsum: 4 ints of LDS).
+__device__ __forceinline__ int block_scan256(int v, int* wsum) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  if (lane == 63) wsum[w] = v;
+  __syncthreads();
+  int add = 0;
+  for (int k = 0; k < w; k++) add += wsum[k];
+  __syncthreads();
+  return v + add;
+}
+
+// Next level = P^T A P by rows, one workgroup per row I (rows[blockIdx.x], or
+// blockIdx.x): row I += P[i,I]^T sum over blocks (i, j) of A of A[i,j] P[j,:], the
+// (column entry, block of A's row) pairs spread over the threads 256 at a time.
+template <int pass, int kSlots, int kH>
+__global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, const int* pt_ptr,
+                                                  const int* pt_blk, const int* pblk_row, const int* prow,
+                                                  const int* pcol, const double* pval, const int* rowptrA,
+                                                  const int* colA, const double* Aa, int* rowcnt, const int* rowptr,
+                                                  int* col, double* A, int* overflow) {
+  __shared__ int keys[kH];
+  __shared__ int slot[kH];
   __shared__ int sorted[kSlots];
   __shared__ int ncount;
+  __shared__ int wsum[4];
+  __shared__ int pre[257];
+  __shared__ int rowi[256], blki[256];
   __shared__ double acc[36 * kSlots];
-  const int I = blockIdx.x;
-  if (I >= nI) return;
-  for (int h = threadIdx.x; h < kHash; h += blockDim.x) keys[h] = -1;
+  if ((int64_t)blockIdx.x >= nI) return;
+  const int I = rows ? rows[blockIdx.x] : (int)blockIdx.x;
+  for (int h = threadIdx.x; h < kH; h += blockDim.x) keys[h] = -1;
   __syncthreads();
-  if (threadIdx.x == 0) hinsert(keys, I);
+  if (threadIdx.x == 0) hinsert<kH>(keys, I);
   const int k0 = pt_ptr[I], k1 = pt_ptr[I + 1];
-  // work items: (column entry kk, block b of A's row i)
-  for (int kk = k0; kk < k1; kk++) {
-    const int i = pblk_row[pt_blk[kk]];
-    for (int b = rowptrA[i] + threadIdx.x; b < rowptrA[i + 1]; b += blockDim.x) {
-      const int j = colA[b];
-      for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) hinsert(keys, pcol[k2]);
-    }
-  }
-  const int m = row_columns(pass, I, keys, slot, sorted, kSlots, &ncount);
-  if (pass == 0) {
-    if (threadIdx.x == 0) rowcnt[I] = m;
-    return;
-  }
-  if (m > kSlots) {
-    if (threadIdx.x == 0) atomicMax(overflow, m);
-    return;
-  }
-  for (int t = threadIdx.x; t < 36 * m; t += blockDim.x) acc[t] = 0.0;
-  __syncthreads();
-  for (int kk = k0; kk < k1; kk++) {
-    const int blk = pt_blk[kk];
-    const int i = pblk_row[blk];
-    const double* PiI = pval + 36 * (int64_t)blk;
-    for (int b = rowptrA[i] + threadIdx.x; b < rowptrA[i + 1]; b += blockDim.x) {
-      const int j = colA[b];
-      double Q[36];
-      mat6_tmul(PiI, Aa + 36 * (int64_t)b, Q);
-      for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) {
-        const int sl = hfind(keys, slot, pcol[k2]);
-        const double* Pj = pval + 36 * (int64_t)k2;
-        double* out = acc + 36 * sl;
-        for (int r = 0; r < 6; r++)
-          for (int cc = 0; cc < 6; cc++) {
-            double s = 0;
-            for (int k = 0; k < 6; k++) s += Q[6 * r + k] * Pj[6 * k + cc];
-            atomicAdd(&out[6 * r + cc], s);
-          }
+  int m = 0;
+  for (int run = 0; run < 2; run++) {  // run 0: columns; run 1 (pass 1): products
+    if (run == 1) {
+      m = row_columns<kH>(pass, I, keys, slot, sorted, kSlots, &ncount);
+      if (pass == 0) {
+        if (threadIdx.x == 0) rowcnt[I] = m;
+        return;
       }
+      if (m > kSlots) {
+        if (threadIdx.x == 0) atomicMax(overflow, m);
+        return;
+      }
+      for (int t = threadIdx.x; t < 36 * m; t += blockDim.x) acc[t] = 0.0;
+      __syncthreads();
+    }
+    for (int c0 = k0; c0 < k1; c0 += 256) {
+      const int nc = min(256, k1 - c0);
+      int len = 0;
+      if ((int)threadIdx.x < nc) {
+        const int blk = pt_blk[c0 + threadIdx.x];
+        const int i = pblk_row[blk];
+        blki[threadIdx.x] = blk;
+        rowi[threadIdx.x] = i;
+        len = rowptrA[i + 1] - rowptrA[i];
+      }
+      const int incl = block_scan256(len, wsum);
+      pre[threadIdx.x + 1] = incl;
+      if (threadIdx.x == 0) pre[0] = 0;
+      __syncthreads();
+      const int total = pre[nc];
+      const int split = run == 0 ? 1 : 6;  // run 1: one thread per output row r
+      for (int it2 = threadIdx.x; it2 < total * split; it2 += blockDim.x) {
+        const int it = it2 / split, r = it2 % split;
+        int lo = 0, hi = nc - 1;  // the last t with pre[t] <= it
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (pre[mid] <= it)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        const int i = rowi[lo];
+        const int b = rowptrA[i] + (it - pre[lo]);
+        const int j = colA[b];
+        if (run == 0) {
+          for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) hinsert<kH>(keys, pcol[k2]);
+          continue;
+        }
+        // row r of P[i,I]^T A[i,j], then row r of its products with P[j,:]
+        const double* PiI = pval + 36 * (int64_t)blki[lo];
+        const double* Ab = Aa + 36 * (int64_t)b;
+        double qr[6];
+        for (int k = 0; k < 6; k++) {
+          double t = 0;
+          for (int i2 = 0; i2 < 6; i2++) t += PiI[6 * i2 + r] * Ab[6 * i2 + k];
+          qr[k] = t;
+        }
+        for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) {
+          const int sl = hfind<kH>(keys, slot, pcol[k2]);
+          const double* Pj = pval + 36 * (int64_t)k2;
+          double* out = acc + 36 * sl + 6 * r;
+          double o6[6] = {0, 0, 0, 0, 0, 0};
+          for (int k = 0; k < 6; k++)
+            for (int cc = 0; cc < 6; cc++) o6[cc] += qr[k] * Pj[6 * k + cc];
+          for (int cc = 0; cc < 6; cc++) atomicAdd(&out[cc], o6[cc]);
+        }
+      }
+      __syncthreads();
     }
   }
-  __syncthreads();
   const int base = rowptr[I];
   if (threadIdx.x == 0) col[base] = I;
-  for (int h = threadIdx.x; h < kHash; h += blockDim.x)
+  for (int h = threadIdx.x; h < kH; h += blockDim.x)
     if (keys[h] != -1 && keys[h] != I) col[base + slot[h]] = keys[h];
   for (int t = threadIdx.x; t < 36 * m; t += blockDim.x) A[36 * (int64_t)base + t] = acc[t];
 }
 
-template <int kSlots>
-__global__ __launch_bounds__(64) void k_sa_a1_w(int pass, int64_t n1, const int* pt_ptr, const int* pt_blk,
-                                                const int* pblk_row, const int* prow, const int* pcol,
-                                                const double* pval, const uint8_t* nkind, const int32_t* ncell,
-                                                const uint32_t* cfaces, const double* ce, const double* cg,
-                                                const uint16_t* cmask, const uint32_t* cstruct, const BeamGeo* bgeo,
-                                                int* rowcnt, const int* rowptr, int* col, double* A, int* overflow) {
-  __shared__ int keys[kHash];
-  __shared__ int slot[kHash];
+// Width classes of Galerkin rows (columns held in LDS by each kernel instance).
+constexpr int kRowClass[4] = {64, 128, 256, 448};
+
+// Rows by width class: lists[k * n + i] for class k (order within a class is free:
+// every row is computed on its own).
+__global__ void k_row_classes(int64_t n, const int* cnt, int* lists, int* ctr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int w = cnt[i];
+  const int k = w <= kRowClass[0] ? 0 : (w <= kRowClass[1] ? 1 : (w <= kRowClass[2] ? 2 : 3));
+  lists[k * n + atomicAdd(&ctr[k], 1)] = (int)i;
+}
+
+// The five beams touching face f (pair indices, ascending).
+__device__ __forceinline__ int face_beam_q(int f, int t) {
+  int k = 0;
+  for (int q = 0; q < 15; q++)
+    if (kPairA[q] == f || kPairB[q] == f) {
+      if (k == t) return q;
+      k++;
+    }
+  return 0;
+}
+
+// Level 1 = P0^T K P0 by rows, one workgroup per row I: work items are (node v in
+// column I of P0, one of the ten beams ending on v), adjacent threads on different
+// nodes. kH: hash slots (4x kSlots; pass 0 sizes with the largest instance).
+template <int pass, int kSlots, int kH>
+__global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, const int* pt_ptr,
+                                                 const int* pt_blk,
+                                                 const int* pblk_row, const int* prow, const int* pcol,
+                                                 const double* pval, const uint8_t* nkind, const int32_t* ncell,
+                                                 const uint32_t* cfaces, const double* ce, const double* cg,
+                                                 const uint16_t* cmask, const uint32_t* cstruct, const BeamGeo* bgeo,
+                                                 int* rowcnt, const int* rowptr, int* col, double* A, int* overflow) {
+  __shared__ int keys[kH];
+  __shared__ int slot[kH];
   __shared__ int sorted[kSlots];
   __shared__ int ncount;
   __shared__ double acc[36 * kSlots];
-  const int I = blockIdx.x;
-  if (I >= n1) return;
-  for (int h = threadIdx.x; h < kHash; h += blockDim.x) keys[h] = -1;
+  if ((int64_t)blockIdx.x >= n1) return;
+  const int I = rows ? rows[blockIdx.x] : (int)blockIdx.x;
+  for (int h = threadIdx.x; h < kH; h += blockDim.x) keys[h] = -1;
   __syncthreads();
-  if (threadIdx.x == 0) hinsert(keys, I);
-  const int k0 = pt_ptr[I], k1 = pt_ptr[I + 1];
-  const double zero[3] = {0, 0, 0};
-  // work items: (column entry kk, beam t at its node): 10 beams at most per node
-  const int items = (k1 - k0) * 10;
-  for (int it = threadIdx.x; it < items; it += blockDim.x) {
-    const int v = pblk_row[pt_blk[k0 + it / 10]];
-    BeamAt bl[10];
-    const int nbm = beams_at((uint32_t)v, nkind, ncell, cfaces, cmask, bl);
-    const int t = it % 10;
-    if (t >= nbm) continue;
-    for (int k2 = prow[v]; k2 < prow[v + 1]; k2++) hinsert(keys, pcol[k2]);
-    if (!(bl[t].wf & kNotFree))
-      for (int k2 = prow[bl[t].wf]; k2 < prow[bl[t].wf + 1]; k2++) hinsert(keys, pcol[k2]);
+  if (threadIdx.x == 0) hinsert<kH>(keys, I);
+  const int k0 = pt_ptr[I], nv = pt_ptr[I + 1] - k0;
+  const int items = nv * 10;
+  {
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
+      const int vi = it % nv, j = it / nv;
+      const int v = pblk_row[pt_blk[k0 + vi]];
+      if (j == 0)
+        for (int k2 = prow[v]; k2 < prow[v + 1]; k2++) hinsert<kH>(keys, pcol[k2]);
+      const int side = j / 5, o = nkind[v] >> 1;
+      const int c = ncell[2 * v + side];
+      if (c < 0) continue;
+      const int f = side == 0 ? 2 * o + 1 : 2 * o;
+      const int q = face_beam_q(f, j % 5);
+      if (!(cmask[c] & (1u << q))) continue;
+      const uint32_t wf = cfaces[6 * c + (kPairA[q] == f ? kPairB[q] : kPairA[q])];
+      if (!(wf & kNotFree))
+        for (int k2 = prow[wf]; k2 < prow[wf + 1]; k2++) hinsert<kH>(keys, pcol[k2]);
+    }
   }
-  const int m = row_columns(pass, I, keys, slot, sorted, kSlots, &ncount);
+  const int m = row_columns<kH>(pass, I, keys, slot, sorted, kSlots, &ncount);
   if (pass == 0) {
     if (threadIdx.x == 0) rowcnt[I] = m;
     return;
@@ -1459,50 +1471,48 @@ __global__ __launch_bounds__(64) void k_sa_a1_w(int pass, int64_t n1, const int*
   }
   for (int x = threadIdx.x; x < 36 * m; x += blockDim.x) acc[x] = 0.0;
   __syncthreads();
-  for (int it = threadIdx.x; it < items; it += blockDim.x) {
-    const int blk = pt_blk[k0 + it / 10];
+  // products split by output row r: thread task (row r, node, beam)
+  for (int it = threadIdx.x; it < items * 6; it += blockDim.x) {
+    const int r = it % 6, rest = it / 6;
+    const int vi = rest % nv, j = rest / nv;
+    const int blk = pt_blk[k0 + vi];
     const int v = pblk_row[blk];
-    BeamAt bl[10];
-    const int nbm = beams_at((uint32_t)v, nkind, ncell, cfaces, cmask, bl);
-    const int t = it % 10;
-    if (t >= nbm) continue;
-    const BeamAt& b = bl[t];
-    const bool wfree = !(b.wf & kNotFree);
+    const int side = j / 5, o = nkind[v] >> 1;
+    const int c = ncell[2 * v + side];
+    if (c < 0) continue;
+    const int f = side == 0 ? 2 * o + 1 : 2 * o;
+    const int q = face_beam_q(f, j % 5);
+    if (!(cmask[c] & (1u << q))) continue;
+    const int sv = kPairA[q] == f ? -1 : 1;
+    const uint32_t wf = cfaces[6 * c + (kPairA[q] == f ? kPairB[q] : kPairA[q])];
+    const bool wfree = !(wf & kNotFree);
     const double* PvI = pval + 36 * (int64_t)blk;
-    const BeamGeo G = bgeo[cstruct[b.c]];
+    double pc[6];  // column r of P0[v, I]
+    for (int i = 0; i < 6; i++) pc[i] = PvI[6 * i + r];
+    const BeamGeo G = bgeo[cstruct[c]];
     double n[3];
     int cls;
-    pair_n(b.q, G, n, cls);
-    const double e = ce[b.c], g = cg[b.c];
-    double Bv[18], Bw[18], K[36], Q[36];
-    gmat(b.sv, n, G.hd[cls], zero, Bv);
-    for (int side = 0; side < (wfree ? 2 : 1); side++) {
-      for (int x = 0; x < 36; x++) K[x] = 0.0;
-      if (side == 0) {
-        beam_coarse_add(Bv, Bv, b.sv, b.sv, n, G.K4[cls], G.R2[cls], e, g, K);
-      } else {
-        gmat(-b.sv, n, G.hd[cls], zero, Bw);
-        beam_coarse_add(Bv, Bw, b.sv, -b.sv, n, G.K4[cls], G.R2[cls], e, g, K);
-      }
-      mat6_tmul(PvI, K, Q);
-      const int row = side == 0 ? v : (int)b.wf;
+    pair_n(q, G, n, cls);
+    const double e = ce[c], g = cg[c];
+    for (int sd = 0; sd < (wfree ? 2 : 1); sd++) {
+      double qr[6];  // row r of P0[v,I]^T K_b
+      beam_row(pc, sv, sd == 0 ? sv : -sv, n, G.hd[cls], G.K4[cls], G.R2[cls], e, g, qr);
+      const int row = sd == 0 ? v : (int)wf;
       for (int k2 = prow[row]; k2 < prow[row + 1]; k2++) {
-        const int sl = hfind(keys, slot, pcol[k2]);
+        const int sl = hfind<kH>(keys, slot, pcol[k2]);
         const double* Pj = pval + 36 * (int64_t)k2;
-        double* out = acc + 36 * sl;
-        for (int r = 0; r < 6; r++)
-          for (int cc = 0; cc < 6; cc++) {
-            double s = 0;
-            for (int k = 0; k < 6; k++) s += Q[6 * r + k] * Pj[6 * k + cc];
-            atomicAdd(&out[6 * r + cc], s);
-          }
+        double* out = acc + 36 * sl + 6 * r;
+        double o6[6] = {0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < 6; k++)
+          for (int cc = 0; cc < 6; cc++) o6[cc] += qr[k] * Pj[6 * k + cc];
+        for (int cc = 0; cc < 6; cc++) atomicAdd(&out[cc], o6[cc]);
       }
     }
   }
   __syncthreads();
   const int base = rowptr[I];
   if (threadIdx.x == 0) col[base] = I;
-  for (int h = threadIdx.x; h < kHash; h += blockDim.x)
+  for (int h = threadIdx.x; h < kH; h += blockDim.x)
     if (keys[h] != -1 && keys[h] != I) col[base + slot[h]] = keys[h];
   for (int x = threadIdx.x; x < 36 * m; x += blockDim.x) A[36 * (int64_t)base + x] = acc[x];
 }
@@ -1517,43 +1527,38 @@ __global__ void k_blk_rows(int64_t n, const int* prow, int* blk_row) {
 template <class T>
 __global__ void k_restrict_p(int64_t nJ, const uint32_t* ast, const uint8_t* done, const int* pt_ptr, const int* pt_blk,
                              const int* pblk_row, const T* pval, const T* t, T* c) {
-  const int64_t J = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (J >= nJ) return;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t J = g >> 3;
+  const int a = (int)(g & 7);
+  if (J >= nJ || a >= 6) return;
   if (done && done[ast[J]]) return;
-  T acc[6] = {0, 0, 0, 0, 0, 0};
+  T acc = 0;
   for (int kk = pt_ptr[J]; kk < pt_ptr[J + 1]; kk++) {
     const int blk = pt_blk[kk];
-    const T* M = pval + 36 * (int64_t)blk;
+    const T* M = pval + 36 * (int64_t)blk + a;  // column a of the block
     T tv[6];
     load6<T>(t + 6 * (int64_t)pblk_row[blk], tv);
-    for (int a = 0; a < 6; a++) {
-      T s = 0;
-      for (int b = 0; b < 6; b++) s += M[6 * b + a] * tv[b];
-      acc[a] += s;
-    }
+#pragma unroll
+    for (int b = 0; b < 6; b++) acc += M[6 * b] * tv[b];
   }
-  store6<T>(c + 6 * J, acc);
+  c[6 * J + a] = acc;
 }
 // prolong: x[i] += sum over row i of P y
 template <class T>
 __global__ void k_prolong_p(int64_t n, const uint32_t* ast, const uint8_t* done, const int* prow, const int* pcol,
                             const T* pval, const T* y, T* x) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = g >> 3;
+  const int a = (int)(g & 7);
+  if (i >= n || a >= 6) return;
   if (done && done[ast[i]]) return;
-  T xv[6];
-  load6<T>(x + 6 * i, xv);
+  T acc = x[6 * i + a];
   for (int k = prow[i]; k < prow[i + 1]; k++) {
-    const T* M = pval + 36 * (int64_t)k;
     T yv[6];
     load6<T>(y + 6 * (int64_t)pcol[k], yv);
-    for (int a = 0; a < 6; a++) {
-      T s = 0;
-      for (int b = 0; b < 6; b++) s += M[6 * a + b] * yv[b];
-      xv[a] += s;
-    }
+    acc += row6_dot<T>(pval + 36 * (int64_t)k + 6 * a, yv);
   }
-  store6<T>(x + 6 * i, xv);
+  x[6 * i + a] = acc;
 }
 
 // rigid prolongation level 1 -> fine, overwriting: x[v] = Sh(r_v) y[label[v]] (0 off aggregates)
@@ -1664,8 +1669,8 @@ struct Solver {
   // ---------------------------------------------------------------- setup
   void setup_fine() {
     const int64_t nn = g.nnodes;
-    c.dinv64.ensure(std::max<int64_t>(nn, 1) * 36 * sizeof(double));
-    c.dinv32.ensure(std::max<int64_t>(nn, 1) * 36 * sizeof(float));
+    c.dinv64.ensure(std::max<int64_t>(nn, 1) * kSym * sizeof(double));
+    c.dinv32.ensure(std::max<int64_t>(nn, 1) * kSym * sizeof(float));
     ST_HIP(hipMemsetAsync(sbad.p, 0, sizeof(int), st));
     if (nn)
       k_dinv<<<nblk(nn), 256, 0, st>>>(nn, g.nflags.as<uint8_t>(), g.ncell.as<int32_t>(), g.nkind.as<uint8_t>(),
@@ -1836,25 +1841,35 @@ struct Solver {
     return h;
   }
 
-  // Rows of a Galerkin product with one wavefront per row: launch(pass, slots) runs the
-  // kernel instance holding `slots` columns in LDS (pass 0 needs none).
+  // Rows of a Galerkin product, one workgroup per row: launch(pass, slots, cnt, rows,
+  // nrows). Pass 0 counts every row's columns; pass 1 runs, for each width class, the
+  // kernel instance holding that many columns in LDS over the class's rows.
   template <class F>
   void wave_rows(Level& lv, F launch) {
     const int64_t n = lv.n;
-    DBuf cnt;
+    DBuf cnt, lists, ctr;
     cnt.ensure((n + 1) * sizeof(int));
     lv.rowptr.ensure((n + 1) * sizeof(int));
     ST_HIP(hipMemsetAsync(sbad.p, 0, sizeof(int), st));
     ST_HIP(hipMemsetAsync(cnt.p, 0, (n + 1) * sizeof(int), st));
-    launch(0, 64, cnt.as<int>());
+    launch(0, 0, cnt.as<int>(), (const int*)nullptr, n);
     const int widest = max_int(cnt.as<int>(), n);
-    int slots = 64;
-    while (slots < widest && slots < 448) slots = slots == 256 ? 448 : slots * 2;
-    if (widest > slots) throw StError{ST_DEVICE, "a coarse operator row has more than 448 blocks"};
+    if (widest > kRowClass[3]) throw StError{ST_DEVICE, "a coarse operator row has more than 448 blocks"};
+    lists.ensure(4 * n * sizeof(int));
+    ctr.ensure(4 * sizeof(int));
+    ST_HIP(hipMemsetAsync(ctr.p, 0, 4 * sizeof(int), st));
+    k_row_classes<<<nblk(n), 256, 0, st>>>(n, cnt.as<int>(), lists.as<int>(), ctr.as<int>());
+    int count[4];
+    ST_HIP(hipMemcpyAsync(count, ctr.p, sizeof(count), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipStreamSynchronize(st));
     finish_rows(lv, cnt, [&](int pass) {
-      if (pass == 1) launch(1, slots, cnt.as<int>());
+      if (pass == 1)
+        for (int k = 0; k < 4; k++)
+          if (count[k]) launch(1, kRowClass[k], cnt.as<int>(), lists.as<int>() + k * n, (int64_t)count[k]);
     });
     cnt.release();
+    lists.release();
+    ctr.release();
   }
 
   // Level 1 of smoothed aggregation: P0 at the fine nodes, then A1 = P0^T K P0.
@@ -1871,22 +1886,25 @@ struct Solver {
                                                           c.dinv64.as<double>(), c.step0_d.as<double>(), cnt, prow,
                                                           pcol, pval, sbad.as<int>());
                          });
-    wave_rows(lv, [&](int pass, int slots, int* cnt) {
-#define ST_A1(S)                                                                                                   \
-  k_sa_a1_w<S><<<(unsigned)n, 64, 0, st>>>(pass, n, c.p0t_ptr.as<int>(), c.p0t_blk.as<int>(), c.p0blk_row.as<int>(), \
-                                          c.p0row.as<int>(), c.p0col.as<int>(), c.p0val.as<double>(),              \
-                                          g.nkind.as<uint8_t>(), g.ncell.as<int32_t>(), g.cfaces.as<uint32_t>(),   \
-                                          g.ce64.as<double>(), g.cg64.as<double>(), g.cmask.as<uint16_t>(),        \
-                                          g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), cnt, lv.rowptr.as<int>(), \
-                                          lv.col.as<int>(), lv.a64.as<double>(), sbad.as<int>())
-      if (slots <= 64)
-        ST_A1(64);
+    wave_rows(lv, [&](int pass, int slots, int* cnt, const int* rows, int64_t nr) {
+#define ST_A1(P, S, H)                                                                                                   \
+  k_sa_a1_w<P, S, H><<<(unsigned)nr, 256, 0, st>>>(rows, nr, c.p0t_ptr.as<int>(), c.p0t_blk.as<int>(),          \
+                                               c.p0blk_row.as<int>(), c.p0row.as<int>(), c.p0col.as<int>(),         \
+                                               c.p0val.as<double>(), g.nkind.as<uint8_t>(), g.ncell.as<int32_t>(),  \
+                                               g.cfaces.as<uint32_t>(), g.ce64.as<double>(), g.cg64.as<double>(),   \
+                                               g.cmask.as<uint16_t>(), g.cstruct.as<uint32_t>(),                    \
+                                               g.bgeo.as<BeamGeo>(), cnt, lv.rowptr.as<int>(), lv.col.as<int>(),    \
+                                               lv.a64.as<double>(), sbad.as<int>())
+      if (pass == 0)
+        ST_A1(0, 1, 2048);
+      else if (slots <= 64)
+        ST_A1(1, 64, 256);
       else if (slots <= 128)
-        ST_A1(128);
+        ST_A1(1, 128, 512);
       else if (slots <= 256)
-        ST_A1(256);
+        ST_A1(1, 256, 1024);
       else
-        ST_A1(448);
+        ST_A1(1, 448, 2048);
 #undef ST_A1
       ST_HIP(hipGetLastError());
     });
@@ -1909,21 +1927,24 @@ struct Solver {
     below.pval32.ensure((size_t)std::max<int64_t>(below.pblocks, 1) * 36 * sizeof(float));
     k_to_f32<<<nblk(below.pblocks * 36), 256, 0, st>>>(below.pblocks * 36, below.pval64.as<double>(),
                                                        below.pval32.as<float>());
-    wave_rows(lv, [&](int pass, int slots, int* cnt) {
-#define ST_GAL(S)                                                                                                   \
-  k_sa_gal_w<S><<<(unsigned)n, 64, 0, st>>>(pass, n, below.pt_ptr.as<int>(), below.pt_blk.as<int>(),                 \
-                                           below.pblk_row.as<int>(), below.prow.as<int>(), below.pcol.as<int>(),     \
-                                           below.pval64.as<double>(), below.rowptr.as<int>(), below.col.as<int>(),   \
-                                           below.a64.as<double>(), cnt, lv.rowptr.as<int>(), lv.col.as<int>(),       \
-                                           lv.a64.as<double>(), sbad.as<int>())
-      if (slots <= 64)
-        ST_GAL(64);
+    wave_rows(lv, [&](int pass, int slots, int* cnt, const int* rows, int64_t nr) {
+#define ST_GAL(P, S, H)                                                                                                  \
+  k_sa_gal_w<P, S, H><<<(unsigned)nr, 256, 0, st>>>(rows, nr, below.pt_ptr.as<int>(), below.pt_blk.as<int>(),    \
+                                                below.pblk_row.as<int>(), below.prow.as<int>(),                     \
+                                                below.pcol.as<int>(), below.pval64.as<double>(),                    \
+                                                below.rowptr.as<int>(), below.col.as<int>(), below.a64.as<double>(),\
+                                                cnt, lv.rowptr.as<int>(), lv.col.as<int>(), lv.a64.as<double>(),    \
+                                                sbad.as<int>())
+      if (pass == 0)
+        ST_GAL(0, 1, 2048);
+      else if (slots <= 64)
+        ST_GAL(1, 64, 256);
       else if (slots <= 128)
-        ST_GAL(128);
+        ST_GAL(1, 128, 512);
       else if (slots <= 256)
-        ST_GAL(256);
+        ST_GAL(1, 256, 1024);
       else
-        ST_GAL(448);
+        ST_GAL(1, 448, 2048);
 #undef ST_GAL
       ST_HIP(hipGetLastError());
     });
@@ -2137,7 +2158,7 @@ struct Solver {
       } else {  // not block diagonal (should not happen): a few smoothing sweeps
         k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), r, x);
         for (int k = 0; k < 8; k++) {
-          k_c_resid<T><<<nblk(n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
+          k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
                                                 lvD<T>(lv), lv.step_d.as<double>(), r, x, t);
           std::swap(x, t);
         }
@@ -2147,27 +2168,38 @@ struct Solver {
       return;
     }
     Level& up = c.levels[li + 1];
+    const int nu = li == 0 ? c.nu1 : c.nu_c;
+    const int gamma = (int)li < c.gamma_from ? 1 : c.gamma;
+    // x <- x + w D^-1 (r - A x), k times (t as the other buffer; the result ends in x)
+    auto smooth = [&](int k) {
+      for (int i = 0; i < k; i++) {
+        k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
+                                              lvD<T>(lv), lv.step_d.as<double>(), r, x, t);
+        ST_HIP(hipMemcpyAsync(x, t, n * 6 * sizeof(T), hipMemcpyDeviceToDevice, st));
+      }
+    };
     k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), r, x);
-    k_c_resid<T><<<nblk(n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv), nullptr,
-                                          nullptr, r, x, t);
+    smooth(nu - 1);
     const T* pv = std::is_same<T, double>::value ? (const T*)lv.pval64.p : (const T*)lv.pval32.p;
-    if (lv.pblocks)
-      k_restrict_p<T><<<nblk(up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(),
-                                                  lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv, t, lvbuf<T>(up, 0));
-    else
-      k_restrict_c<T><<<nblk(up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, up.kid_ptr.as<int>(),
-                                                  up.kids.as<int>(), up.centre.as<double>(), lv.centre.as<double>(), t,
-                                                  lvbuf<T>(up, 0));
-    coarse_cycle<T>(li + 1);
-    if (lv.pblocks)
-      k_prolong_p<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.pcol.as<int>(), pv,
-                                              lvbuf<T>(up, 1), x);
-    else
-      k_prolong_c<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lv.parent.as<int>(), lv.centre.as<double>(),
-                                              up.centre.as<double>(), lvbuf<T>(up, 1), x);
-    k_c_resid<T><<<nblk(n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv), lvD<T>(lv),
-                                          lv.step_d.as<double>(), r, x, t);
-    ST_HIP(hipMemcpyAsync(x, t, n * 6 * sizeof(T), hipMemcpyDeviceToDevice, st));
+    for (int g2 = 0; g2 < gamma; g2++) {
+      k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv), nullptr,
+                                            nullptr, r, x, t);
+      if (lv.pblocks)
+        k_restrict_p<T><<<nblk(8 * up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(),
+                                                    lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv, t, lvbuf<T>(up, 0));
+      else
+        k_restrict_c<T><<<nblk(up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, up.kid_ptr.as<int>(),
+                                                    up.kids.as<int>(), up.centre.as<double>(), lv.centre.as<double>(), t,
+                                                    lvbuf<T>(up, 0));
+      coarse_cycle<T>(li + 1);
+      if (lv.pblocks)
+        k_prolong_p<T><<<nblk(8 * n), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.pcol.as<int>(), pv,
+                                                lvbuf<T>(up, 1), x);
+      else
+        k_prolong_c<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lv.parent.as<int>(), lv.centre.as<double>(),
+                                                up.centre.as<double>(), lvbuf<T>(up, 1), x);
+      smooth(nu);
+    }
   }
 
   // z = M^-1 r at the fine level; partial r.z per brick into c.partial
@@ -2178,6 +2210,22 @@ struct Solver {
     const T* dinv = std::is_same<T, double>::value ? (const T*)c.dinv64.p : (const T*)c.dinv32.p;
     k_jacobi_fine<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done, dinv,
                                             c.step0_d.as<double>(), r, x1);
+    // further damped Jacobi steps x1 += w D^-1 (r - K x1), ping-ponging through s1
+    auto smooth_steps = [&](int k) {
+      for (int i = 0; i < k; i++) {
+        EpiSmooth<T> es;
+        es.done = done;
+        es.partial = c.partial.as<double>();
+        es.b = r;
+        es.xin = x1;
+        es.dinv = dinv;
+        es.step = c.step0_d.as<double>();
+        es.z = s1;
+        launch_apply<T>(c, x1, es);
+        k_copy_fine<T, T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done, s1, x1);
+      }
+    };
+    smooth_steps(c.nu - 1);
     if (!c.levels.empty() && c.smoothed) {
       // r1 = r - K x1, s1 = w D^-1 r1; u = r1 - K s1 (= the smoothed restriction's argument);
       // c = P_rigid^T u; coarse; s1 = P_rigid y; x1 += s1 - w D^-1 K s1; z = x1 + w D^-1 (r - K x1)
@@ -2226,6 +2274,7 @@ struct Solver {
                                            l1.centre.as<double>(), g.nkey.as<uint32_t>(), g.nkind.as<uint8_t>(),
                                            g.bgeo.as<BeamGeo>(), lvbuf<T>(l1, 1), x1);
     }
+    smooth_steps(c.nu - 1);
     EpiSmooth<T> es;
     es.done = done;
     es.partial = c.partial.as<double>();
@@ -2429,7 +2478,7 @@ void release_levels(Ctx& c) {
 
 void release_all(Ctx& c) {
   DevGeo& d = c.geo;
-  for (DBuf* b : {&d.cbase, &d.nbase, &d.hbase, &d.bstruct, &d.hcell, &d.ho, &d.ce64, &d.cg64, &d.ce32, &d.cg32,
+  for (DBuf* b : {&d.cbase, &d.nbase, &d.bstruct, &d.ce64, &d.cg64, &d.ce32, &d.cg32,
                   &d.cmask_user, &d.cmask, &d.cfaces, &d.ccoord, &d.cstruct, &d.nflags, &d.ncell, &d.nkind, &d.nkey,
                   &d.bgeo})
     b->release();
@@ -2485,10 +2534,30 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   // every level widens the coarse stencils about 1.5x a level (STATICS_HIP_SA_LEVELS)
   c.sa_levels = 2;
   if (const char* e = std::getenv("STATICS_HIP_SA_LEVELS")) c.sa_levels = std::max(1, std::atoi(e));
+  c.nu = 1;
+  if (const char* e = std::getenv("STATICS_HIP_NU")) c.nu = std::max(1, std::atoi(e));
+  c.nu1 = 1;
+  if (const char* e = std::getenv("STATICS_HIP_NU1")) c.nu1 = std::max(1, std::atoi(e));
+  c.nu_c = 1;
+  if (const char* e = std::getenv("STATICS_HIP_NUC")) c.nu_c = std::max(1, std::atoi(e));
+  // W-cycle from level 3 up: the coarse levels are built from rigid transfers, and
+  // visiting them twice per cycle halves the iteration count at little cost
+  c.gamma = 2;
+  if (const char* e = std::getenv("STATICS_HIP_GAMMA")) c.gamma = std::max(1, std::atoi(e));
+  c.gamma_from = 2;
+  if (const char* e = std::getenv("STATICS_HIP_GFROM")) c.gamma_from = std::max(0, std::atoi(e));
   S.init_struct_arrays();
   S.setup_fine();
   S.setup_hierarchy();
   ST_HIP(hipEventRecord(e1, c.stream));
+  if (std::getenv("STATICS_HIP_VERBOSE")) {  // the hierarchy's shape, for tuning
+    std::fprintf(stderr, "statics_hip: fine %lld nodes (%lld free)\n", (long long)c.geo.nnodes, (long long)S.ndof);
+    for (size_t l = 0; l < c.levels.size(); l++)
+      std::fprintf(stderr, "statics_hip: level %zu cube %d: %lld aggregates, %lld blocks (%.1f/row), P below %lld\n",
+                   l + 1, c.levels[l].cube, (long long)c.levels[l].n, (long long)c.levels[l].nblocks,
+                   c.levels[l].n ? (double)c.levels[l].nblocks / c.levels[l].n : 0.0,
+                   (long long)(l ? c.levels[l - 1].pblocks : c.p0blocks));
+  }
   if (S.ndof > 0) S.pcg(o.precision, o.tol, o.max_iter, o.refresh);
   S.true_residual();
   ST_HIP(hipEventRecord(e2, c.stream));

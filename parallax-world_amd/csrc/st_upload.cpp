@@ -69,7 +69,6 @@ void build_host_geo(HostGeo& hg, const StCells* const* cells, int32_t n) {
   hg.clear();
   hg.cbase.push_back(0);
   hg.nbase.push_back(0);
-  hg.hbase.push_back(0);
   for (int32_t si = 0; si < n; si++) {
     const StCells& C = *cells[si];
     if (C.n_cells < 0 || (C.n_cells > 0 && (!C.sites || !C.held || !C.e || !C.g)) || !(C.spacing > 0))
@@ -189,7 +188,6 @@ void build_host_geo(HostGeo& hg, const StCells* const* cells, int32_t n) {
         }
       }
       node_base_local[a + 1] = node_base_local[a] + cnt;
-      hg.max_brick_nodes = std::max(hg.max_brick_nodes, cnt);
     }
     const int64_t n0 = info.node0;
     auto node_id = [&](int32_t x, int32_t y, int32_t z, int k) -> int64_t {
@@ -227,26 +225,6 @@ void build_host_geo(HostGeo& hg, const StCells* const* cells, int32_t n) {
     for (size_t a = 0; a < act.size(); a++) {
       const Occ& o = B.occ[act[a].occ];
       hg.bstruct.push_back((uint32_t)si);
-      // halo cells: free cells across the brick's lower faces next to its own free cells
-      for (int ori = 0; ori < 3; ori++)
-        for (int w = 0; w < 8; w++) {
-          uint64_t fm = o.freem[w];
-          while (fm) {
-            int bit = __builtin_ctzll(fm);
-            fm &= fm - 1;
-            int s = w * 64 + bit;
-            int lx = s & 7, ly = (s >> 3) & 7, lz = s >> 6;
-            int lc = ori == 0 ? lx : (ori == 1 ? ly : lz);
-            if (lc != 0) continue;
-            int x = act[a].bx * 8 + lx - (ori == 0), y = act[a].by * 8 + ly - (ori == 1),
-                z = act[a].bz * 8 + lz - (ori == 2);
-            if (B.is_free(x, y, z)) {
-              hg.hcell.push_back((uint32_t)cell_id(x, y, z));
-              hg.ho.push_back((uint8_t)ori);
-            }
-          }
-        }
-      hg.hbase.push_back((uint32_t)hg.hcell.size());
       // cells
       for (int w = 0; w < 8; w++) {
         uint64_t fm = o.freem[w];
