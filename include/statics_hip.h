@@ -152,7 +152,8 @@ int st_classify(StCtx* ctx, int64_t* joined_count);
  * `which` (0: the fine operator with the CG's fused p.Ap in FP32, 1: the same in
  * FP64); enable == 0 stops. st_kernel_timing reads the totals so far:
  * launches, summed milliseconds, and the algorithmic bytes of one launch
- * (6 values per free node read and written, plus e, g and the beam mask per free cell). */
+ * (6 values per free node read and written, plus e, g, the beam mask and the
+ * six face indices per free cell). */
 int st_kernel_timing_enable(StCtx* ctx, int which, int enable);
 int st_kernel_timing(StCtx* ctx, int64_t* launches, double* total_ms, double* bytes_per_launch);
 

@@ -139,7 +139,9 @@ void launch_apply(Ctx& c, const T* x, const Epi& epi) {
     ST_HIP(hipEventRecord(c.tev[2 * c.tev_used + 1], c.stream));
     c.tev_used++;
     c.timing_launches++;
-    c.timing_bytes = (double)c.last_ndof * 6 * 2 * sizeof(T) + (double)c.geo.ncells * (2 * sizeof(T) + 2);
+    // algorithmic bytes: x read and y written at every free node; e, g, the beam mask and
+    // the six face indices of every free cell
+    c.timing_bytes = (double)c.last_ndof * 6 * 2 * sizeof(T) + (double)c.geo.ncells * (2 * sizeof(T) + 2 + 6 * 4);
   }
 }
 

@@ -128,7 +128,7 @@ struct Level {
   std::vector<double> step;  // smoother step per structure
   DBuf step_d;               // per structure (double)
   // V-cycle vectors at this level
-  DBuf r64, x64, t64, r32, x32, t32;
+  DBuf r64, x64, t64, u64, v64, r32, x32, t32, u32, v32;
 };
 
 struct Ctx {
@@ -156,6 +156,7 @@ struct Ctx {
   int nu_c = 1;          // the same above level 1
   int gamma = 1;         // coarse corrections per visit above level 1 (1: V-cycle, 2: W-cycle)
   int gamma_from = 1;    // first level (0: level 1) that repeats its coarse correction
+  int f64_from = 2;      // first level (0: level 1) kept in FP64 by the mixed V-cycle
   std::vector<Level> levels;        // coarse levels 1..
   // solver vectors
   DBuf f64, x64, r64, p64, ap64, z64, t64, u64, w64;

@@ -4,11 +4,12 @@
 // K x (the reference's Level::apply at level 0, statics/src/bodies.rs:193-206,
 // with the level's blocks assembled per bond in Level::new, :126-164) is
 // computed node by node from the closed-form beams of st_beam.hpp. One
-// workgroup owns one brick and one thread one node: the thread gathers the
-// motions of the faces of the node's (at most two) cells and sums the loads of
-// the (at most ten) beams ending on the node, then an epilogue consumes the
-// row in registers (store, residual, smoothing, dot product). No LDS staging
-// and no atomics: every row is summed by one thread in a fixed order.
+// workgroup owns one brick: it stages the brick's node motions and cell table in
+// LDS with coalesced reads, then one thread per node gathers the motions of the
+// faces of the node's (at most two) cells and sums the loads of the (at most
+// ten) beams ending on the node, and an epilogue consumes the row in registers
+// (store, residual, smoothing, dot product). No atomics: every row is summed by
+// one thread in a fixed order.
 #pragma once
 #include "st_internal.hpp"
 
@@ -77,35 +78,61 @@ __device__ __forceinline__ void face_beam(const CellCoef<T>& c0, const CellCoef<
   if constexpr (Q < 14) face_beam<T, F, Q + 1>(c0, c1, mask, xv, y);
 }
 
+// A brick's share of the operator's inputs staged in LDS: the motions of its first
+// ns nodes and the face tables, e, g and beam masks of its first ncs cells (both
+// read coalesced). Faces and cells outside the staged ranges (the neighbours across
+// the brick's lower faces, and the rest of very full bricks) are read from memory.
+template <class T>
+struct Tile {
+  static constexpr int kNodes = sizeof(T) == 4 ? 768 : 512;
+  static constexpr int kCells = 256;
+  const T* xs;
+  const uint32_t* cf;
+  const T *ce, *cg;
+  const uint16_t* cm;
+  uint32_t n0, ns, c0, ncs;
+};
+
 template <class T, int F>
-__device__ __forceinline__ void cell_face_row(const GeoD<T>& g, const BeamGeo& G, const T* __restrict__ x, uint32_t c,
-                                              T* y) {
-  const uint32_t mask = (uint32_t)g.cmask[c] & face_beams(F);
+__device__ __forceinline__ void cell_face_row(const GeoD<T>& g, const Tile<T>& t, const BeamGeo& G,
+                                              const T* __restrict__ x, uint32_t c, T* y) {
+  const uint32_t lc = c - t.c0;
+  const bool staged = lc < t.ncs;
+  const uint32_t mask = (uint32_t)(staged ? t.cm[lc] : g.cmask[c]) & face_beams(F);
   if (!mask) return;
+  uint32_t fi[6];
+  if (staged) {
+#pragma unroll
+    for (int f = 0; f < 6; f++) fi[f] = t.cf[6 * lc + f];
+  } else {
+    const uint2* fp = reinterpret_cast<const uint2*>(g.cfaces + 6 * (size_t)c);
+    const uint2 f01 = fp[0], f23 = fp[1], f45 = fp[2];
+    fi[0] = f01.x; fi[1] = f01.y; fi[2] = f23.x; fi[3] = f23.y; fi[4] = f45.x; fi[5] = f45.y;
+  }
   T xv[6][6];
-  const uint2* fp = reinterpret_cast<const uint2*>(g.cfaces + 6 * (size_t)c);
-  const uint2 f01 = fp[0], f23 = fp[1], f45 = fp[2];
-  const uint32_t fi[6] = {f01.x, f01.y, f23.x, f23.y, f45.x, f45.y};
 #pragma unroll
   for (int f = 0; f < 6; f++) {
+    const uint32_t ln = fi[f] - t.n0;
     if (fi[f] & kNotFree) {
 #pragma unroll
       for (int k = 0; k < 6; k++) xv[f][k] = T(0);
+    } else if (ln < t.ns) {
+      load6<T>(t.xs + 6 * ln, xv[f]);
     } else {
       load6<T>(x + 6 * (size_t)fi[f], xv[f]);
     }
   }
-  const T e = g.ce[c], gg = g.cg[c];
+  const T e = staged ? t.ce[lc] : g.ce[c], gg = staged ? t.cg[lc] : g.cg[c];
   face_beam<T, F, 0>(cell_coef<T>(G, 0, e, gg), cell_coef<T>(G, 1, e, gg), mask, xv, y);
 }
 
 // Row of K x at a free node of orientation O: face 2O+1 of the cell below (c-) and
 // face 2O of the cell above (c+), where those cells are free.
 template <class T, int O>
-__device__ __forceinline__ void node_row(const GeoD<T>& g, const BeamGeo& G, const T* __restrict__ x, int2 cells,
-                                         T* y) {
-  if (cells.x >= 0) cell_face_row<T, 2 * O + 1>(g, G, x, (uint32_t)cells.x, y);
-  if (cells.y >= 0) cell_face_row<T, 2 * O>(g, G, x, (uint32_t)cells.y, y);
+__device__ __forceinline__ void node_row(const GeoD<T>& g, const Tile<T>& t, const BeamGeo& G,
+                                         const T* __restrict__ x, int2 cells, T* y) {
+  if (cells.x >= 0) cell_face_row<T, 2 * O + 1>(g, t, G, x, (uint32_t)cells.x, y);
+  if (cells.y >= 0) cell_face_row<T, 2 * O>(g, t, G, x, (uint32_t)cells.y, y);
 }
 
 // Block-wide sum of one double per thread; the result is valid in thread 0.
@@ -125,8 +152,24 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // beams of its two cells that end on it, so each row is summed by one thread in a
 // fixed order, with no atomics, and epi.node(n, row, s) consumes it in registers.
 // Epi::kReduce: the per-node values returned are summed per brick into epi.partial[b].
+template <class T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+
 template <class T, class Epi>
 __global__ __launch_bounds__(kBlock) void k_apply(GeoD<T> g, const T* __restrict__ x, Epi epi) {
+  using V2 = typename Vec2<T>::type;
+  __shared__ V2 xs2[Tile<T>::kNodes * 3];
+  __shared__ uint2 cf2[Tile<T>::kCells * 3];
+  __shared__ T ces[Tile<T>::kCells], cgs[Tile<T>::kCells];
+  __shared__ uint16_t cms[Tile<T>::kCells];
   __shared__ double red[kBlock / 64];
   const uint32_t b = blockIdx.x;
   const uint32_t s = g.bstruct[b];
@@ -134,26 +177,48 @@ __global__ __launch_bounds__(kBlock) void k_apply(GeoD<T> g, const T* __restrict
     if (Epi::kReduce && threadIdx.x == 0) epi.partial[b] = 0.0;
     return;
   }
-  const BeamGeo G = g.bgeo[s];
+  Tile<T> t;
+  t.n0 = g.nbase[b];
   const uint32_t n_end = g.nbase[b + 1];
+  t.ns = min(n_end - t.n0, (uint32_t)Tile<T>::kNodes);
+  t.c0 = g.cbase[b];
+  t.ncs = min(g.cbase[b + 1] - t.c0, (uint32_t)Tile<T>::kCells);
+  {
+    const V2* xg = reinterpret_cast<const V2*>(x + 6 * (size_t)t.n0);
+    for (uint32_t i = threadIdx.x; i < 3 * t.ns; i += blockDim.x) xs2[i] = xg[i];
+    const uint2* cg2 = reinterpret_cast<const uint2*>(g.cfaces + 6 * (size_t)t.c0);
+    for (uint32_t i = threadIdx.x; i < 3 * t.ncs; i += blockDim.x) cf2[i] = cg2[i];
+    for (uint32_t i = threadIdx.x; i < t.ncs; i += blockDim.x) {
+      ces[i] = g.ce[t.c0 + i];
+      cgs[i] = g.cg[t.c0 + i];
+      cms[i] = g.cmask[t.c0 + i];
+    }
+  }
+  t.xs = reinterpret_cast<const T*>(xs2);
+  t.cf = reinterpret_cast<const uint32_t*>(cf2);
+  t.ce = ces;
+  t.cg = cgs;
+  t.cm = cms;
+  const BeamGeo G = g.bgeo[s];
+  __syncthreads();
   double part = 0.0;
-  for (uint32_t v = g.nbase[b] + threadIdx.x; v < n_end; v += blockDim.x) {
+  for (uint32_t v = t.n0 + threadIdx.x; v < n_end; v += blockDim.x) {
     T y[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
     if (!(g.nflags[v] & (kHeld | kGone))) {
       const int o = g.nkind[v] >> 1;
       const int2 cells = reinterpret_cast<const int2*>(g.ncell)[v];
       if (o == 0)
-        node_row<T, 0>(g, G, x, cells, y);
+        node_row<T, 0>(g, t, G, x, cells, y);
       else if (o == 1)
-        node_row<T, 1>(g, G, x, cells, y);
+        node_row<T, 1>(g, t, G, x, cells, y);
       else
-        node_row<T, 2>(g, G, x, cells, y);
+        node_row<T, 2>(g, t, G, x, cells, y);
     }
     part += epi.node(v, y, s);
   }
   if (Epi::kReduce) {
-    const double t = block_sum(part, red);
-    if (threadIdx.x == 0) epi.partial[b] = t;
+    const double tot = block_sum(part, red);
+    if (threadIdx.x == 0) epi.partial[b] = tot;
   }
 }
 

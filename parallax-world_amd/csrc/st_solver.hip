@@ -430,6 +430,25 @@ __global__ __launch_bounds__(kBlock) void k_update_p(const uint32_t* nbase, cons
     p[i] = (T)((double)z[i] + be * (double)p[i]);
 }
 
+// z64 = z32 (widened) and the partial r.z64 per brick (0 for finished structures)
+__global__ __launch_bounds__(kBlock) void k_widen_dot(const uint32_t* nbase, const uint32_t* bstruct, const uint8_t* done,
+                                                      const float* z32, double* z64, const double* r, double* partial) {
+  __shared__ double red[kBlock / 64];
+  const uint32_t b = blockIdx.x;
+  if (done[bstruct[b]]) {
+    if (threadIdx.x == 0) partial[b] = 0.0;
+    return;
+  }
+  double acc = 0.0;
+  for (uint32_t i = 6 * nbase[b] + threadIdx.x; i < 6 * nbase[b + 1]; i += blockDim.x) {
+    const double zd = (double)z32[i];
+    z64[i] = zd;
+    acc += r[i] * zd;
+  }
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) partial[b] = t;
+}
+
 // partial a.b per brick
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_dot_fine(const uint32_t* nbase, const T* a, const T* bb, double* partial) {
@@ -451,16 +470,6 @@ __global__ __launch_bounds__(kBlock) void k_copy_fine(const uint32_t* nbase, con
   for (uint32_t i = 6 * nbase[b] + threadIdx.x; i < 6 * nbase[b + 1]; i += blockDim.x) dst[i] = (TD)src[i];
 }
 
-// x64 += x32; x32 = 0
-__global__ __launch_bounds__(kBlock) void k_flush(const uint32_t* nbase, const uint32_t* bstruct, const uint8_t* done,
-                                                  double* x64, float* x32) {
-  const uint32_t b = blockIdx.x, s = bstruct[b];
-  if (done[s]) return;
-  for (uint32_t i = 6 * nbase[b] + threadIdx.x; i < 6 * nbase[b + 1]; i += blockDim.x) {
-    x64[i] += (double)x32[i];
-    x32[i] = 0.0f;
-  }
-}
 
 // zero rows of held and gone nodes (a load on them is not read)
 __global__ void k_mask_rows(int64_t n, const uint8_t* nflags, double* v) {
@@ -830,6 +839,19 @@ __global__ void k_c_jacobi(int64_t n, const uint32_t* ast, const uint8_t* done, 
 }
 
 // t = r - A x  (mode 0)   or   z = x + step D^-1 (r - A x)  (mode 1)
+template <class A, class B>
+__global__ void k_convert(int64_t n, const A* a, B* b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (B)a[i];
+}
+
+// y = a y + x  (a = 0: copy)
+template <class T>
+__global__ void k_axpy(int64_t n, T a, const T* x, T* y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a == T(0) ? x[i] : a * y[i] + x[i];
+}
+
 // Coarse block rows are worked by groups of 8 lanes: lane i < 6 of a group computes
 // component i of its row, reading row i of every 6x6 block (the group reads each
 // block as one contiguous 144-byte run); lanes 6 and 7 idle.
@@ -1996,9 +2018,13 @@ struct Solver {
     lv.r64.ensure(n * 6 * 8);
     lv.x64.ensure(n * 6 * 8);
     lv.t64.ensure(n * 6 * 8);
+    lv.u64.ensure(n * 6 * 8);
+    lv.v64.ensure(n * 6 * 8);
     lv.r32.ensure(n * 6 * 4);
     lv.x32.ensure(n * 6 * 4);
     lv.t32.ensure(n * 6 * 4);
+    lv.u32.ensure(n * 6 * 4);
+    lv.v32.ensure(n * 6 * 4);
   }
 
   void setup_hierarchy() {
@@ -2129,8 +2155,8 @@ struct Solver {
   // --------------------------------------------------------------- V-cycle
   template <class T>
   T* lvbuf(Level& lv, int which) {
-    DBuf* b64[3] = {&lv.r64, &lv.x64, &lv.t64};
-    DBuf* b32[3] = {&lv.r32, &lv.x32, &lv.t32};
+    DBuf* b64[5] = {&lv.r64, &lv.x64, &lv.t64, &lv.u64, &lv.v64};
+    DBuf* b32[5] = {&lv.r32, &lv.x32, &lv.t32, &lv.u32, &lv.v32};
     return std::is_same<T, double>::value ? (T*)b64[which]->p : (T*)b32[which]->p;
   }
   template <class T>
@@ -2142,64 +2168,100 @@ struct Solver {
     return std::is_same<T, double>::value ? (const T*)lv.dinv64.p : (const T*)lv.dinv32.p;
   }
 
-  // Solves approximately A_li z = r where r is in lv.r; result in lv.x.
+  // The coarse cycle of level li for a V-cycle in precision T: levels from
+  // c.f64_from up run in FP64 whatever T is (their operators are the most
+  // ill-conditioned and the smallest), the right-hand side and the result
+  // converted at the switch.
+  template <class T>
+  void descend(size_t li) {
+    if (std::is_same<T, float>::value && (int)li >= c.f64_from) {
+      Level& lv = c.levels[li];
+      const int64_t m = lv.n * 6;
+      k_convert<float, double><<<nblk(m), 256, 0, st>>>(m, lv.r32.as<float>(), lv.r64.as<double>());
+      coarse_cycle<double>(li);
+      k_convert<double, float><<<nblk(m), 256, 0, st>>>(m, lv.x64.as<double>(), lv.x32.as<float>());
+    } else {
+      coarse_cycle<T>(li);
+    }
+  }
+
+  // Solves A_li x = r approximately, r in lv.r, x into lv.x: one cycle, or (levels
+  // above c.gamma_from + 1) gamma of them, the later ones on the remaining residual
+  // (x += cycle(r - A x)), which keeps the preconditioner symmetric.
   template <class T>
   void coarse_cycle(size_t li) {
     Level& lv = c.levels[li];
     const uint8_t* done = c.done.as<uint8_t>();
     T* r = lvbuf<T>(lv, 0);
     T* x = lvbuf<T>(lv, 1);
-    T* t = lvbuf<T>(lv, 2);
     const int64_t n = lv.n;
     const uint32_t* ast = lv.agg_struct.as<uint32_t>();
-    if (li + 1 == c.levels.size()) {
-      if (lv.nblocks == lv.n) {
-        k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), nullptr, r, x);
-      } else {  // not block diagonal (should not happen): a few smoothing sweeps
-        k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), r, x);
-        for (int k = 0; k < 8; k++) {
-          k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
-                                                lvD<T>(lv), lv.step_d.as<double>(), r, x, t);
-          std::swap(x, t);
-        }
-        if (x != lvbuf<T>(lv, 1))
-          ST_HIP(hipMemcpyAsync(lvbuf<T>(lv, 1), x, n * 6 * sizeof(T), hipMemcpyDeviceToDevice, st));
+    const bool top = li + 1 == c.levels.size();
+    if (top && lv.nblocks == lv.n) {  // block diagonal: exact
+      k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), nullptr, r, x);
+      return;
+    }
+    one_cycle<T>(li, r, x);
+    const int reps = (int)li > c.gamma_from ? c.gamma : 1;
+    T* t = lvbuf<T>(lv, 2);
+    T* v = lvbuf<T>(lv, 4);
+    for (int k = 1; k < reps; k++) {
+      k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
+                                               nullptr, nullptr, r, x, t);
+      k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(0), x, v);  // v = x
+      one_cycle<T>(li, t, x);
+      k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(1), v, x);  // x += v
+    }
+  }
+
+  // One cycle at level li: x = cycle(rhs); uses lv.u as scratch (and lv.t for smoothing).
+  template <class T>
+  void one_cycle(size_t li, const T* rhs, T* x) {
+    Level& lv = c.levels[li];
+    const uint8_t* done = c.done.as<uint8_t>();
+    const int64_t n = lv.n;
+    const uint32_t* ast = lv.agg_struct.as<uint32_t>();
+    T* u = lvbuf<T>(lv, 3);
+    if (li + 1 == c.levels.size()) {  // not block diagonal (should not happen): smoothing sweeps
+      k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), rhs, x);
+      for (int k = 0; k < 8; k++) {
+        k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
+                                                 lvD<T>(lv), lv.step_d.as<double>(), rhs, x, u);
+        k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(0), u, x);
       }
       return;
     }
     Level& up = c.levels[li + 1];
     const int nu = li == 0 ? c.nu1 : c.nu_c;
-    const int gamma = (int)li < c.gamma_from ? 1 : c.gamma;
-    // x <- x + w D^-1 (r - A x), k times (t as the other buffer; the result ends in x)
+    // x <- x + w D^-1 (rhs - A x), k times
     auto smooth = [&](int k) {
       for (int i = 0; i < k; i++) {
         k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
-                                              lvD<T>(lv), lv.step_d.as<double>(), r, x, t);
-        ST_HIP(hipMemcpyAsync(x, t, n * 6 * sizeof(T), hipMemcpyDeviceToDevice, st));
+                                                 lvD<T>(lv), lv.step_d.as<double>(), rhs, x, u);
+        k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(0), u, x);
       }
     };
-    k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), r, x);
+    k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), rhs, x);
     smooth(nu - 1);
     const T* pv = std::is_same<T, double>::value ? (const T*)lv.pval64.p : (const T*)lv.pval32.p;
-    for (int g2 = 0; g2 < gamma; g2++) {
-      k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv), nullptr,
-                                            nullptr, r, x, t);
-      if (lv.pblocks)
-        k_restrict_p<T><<<nblk(8 * up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(),
-                                                    lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv, t, lvbuf<T>(up, 0));
-      else
-        k_restrict_c<T><<<nblk(up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, up.kid_ptr.as<int>(),
-                                                    up.kids.as<int>(), up.centre.as<double>(), lv.centre.as<double>(), t,
-                                                    lvbuf<T>(up, 0));
-      coarse_cycle<T>(li + 1);
-      if (lv.pblocks)
-        k_prolong_p<T><<<nblk(8 * n), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.pcol.as<int>(), pv,
-                                                lvbuf<T>(up, 1), x);
-      else
-        k_prolong_c<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lv.parent.as<int>(), lv.centre.as<double>(),
-                                                up.centre.as<double>(), lvbuf<T>(up, 1), x);
-      smooth(nu);
-    }
+    k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
+                                             nullptr, nullptr, rhs, x, u);
+    if (lv.pblocks)
+      k_restrict_p<T><<<nblk(8 * up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(),
+                                                     lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv, u,
+                                                     lvbuf<T>(up, 0));
+    else
+      k_restrict_c<T><<<nblk(up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, up.kid_ptr.as<int>(),
+                                                  up.kids.as<int>(), up.centre.as<double>(), lv.centre.as<double>(), u,
+                                                  lvbuf<T>(up, 0));
+    descend<T>(li + 1);
+    if (lv.pblocks)
+      k_prolong_p<T><<<nblk(8 * n), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.pcol.as<int>(), pv,
+                                                 lvbuf<T>(up, 1), x);
+    else
+      k_prolong_c<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lv.parent.as<int>(), lv.centre.as<double>(),
+                                              up.centre.as<double>(), lvbuf<T>(up, 1), x);
+    smooth(nu);
   }
 
   // z = M^-1 r at the fine level; partial r.z per brick into c.partial
@@ -2247,7 +2309,7 @@ struct Solver {
       k_restrict1<T><<<nblk(l1.n), 256, 0, st>>>(l1.n, l1.agg_struct.as<uint32_t>(), done, c.agg_ptr.as<int>(),
                                                  c.nodes_by_agg.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
                                                  g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), r1, lvbuf<T>(l1, 0));
-      coarse_cycle<T>(0);
+      descend<T>(0);
       k_prolong1_set<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done,
                                                c.label1.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
                                                g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), lvbuf<T>(l1, 1), s1);
@@ -2269,7 +2331,7 @@ struct Solver {
       k_restrict1<T><<<nblk(l1.n), 256, 0, st>>>(l1.n, l1.agg_struct.as<uint32_t>(), done, c.agg_ptr.as<int>(),
                                                  c.nodes_by_agg.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
                                                  g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), r1, lvbuf<T>(l1, 0));
-      coarse_cycle<T>(0);
+      descend<T>(0);
       k_prolong1<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done, c.label1.as<int>(),
                                            l1.centre.as<double>(), g.nkey.as<uint32_t>(), g.nkind.as<uint8_t>(),
                                            g.bgeo.as<BeamGeo>(), lvbuf<T>(l1, 1), x1);
@@ -2322,7 +2384,7 @@ struct Solver {
     }
     k_cg_start<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), tol, mixed ? 1 : 0,
                                          swant.as<uint8_t>());
-    if (!mixed) {
+    {
       c.p64.ensure(v64);
       c.ap64.ensure(v64);
       c.z64.ensure(v64);
@@ -2331,8 +2393,28 @@ struct Solver {
       c.w64.ensure(v64);
       double *x = c.x64.as<double>(), *r = c.r64.as<double>(), *p = c.p64.as<double>(), *ap = c.ap64.as<double>(),
              *z = c.z64.as<double>();
-      vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>(), c.w64.as<double>());
-      reduce_bricks(c.partial, S_RZ);
+      // z = M^-1 r and the partial r.z: the V-cycle in FP64, or (mixed) in FP32 on the
+      // rounded residual with z widened back to FP64 and r.z summed in FP64
+      if (mixed) {
+        c.r32.ensure(v32);
+        c.z32.ensure(v32);
+        c.t32.ensure(v32);
+        c.p32.ensure(v32);
+        c.w32.ensure(v32);
+      }
+      auto precond = [&](int slot) {
+        if (!mixed) {
+          vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>(), c.w64.as<double>());
+        } else {
+          float* r32 = c.r32.as<float>();
+          float* z32 = c.z32.as<float>();
+          k_copy_fine<float, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, r, r32);
+          vcycle<float>(r32, z32, c.t32.as<float>(), c.p32.as<float>(), c.w32.as<float>());
+          k_widen_dot<<<nb, kBlock, 0, st>>>(nbase, bst, done, z32, z, r, partial);
+        }
+        reduce_bricks(c.partial, slot);
+      };
+      precond(S_RZ);
       k_copy_fine<double, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, z, p);
       std::vector<uint8_t> dh, wh;
       int it = 0;
@@ -2352,8 +2434,7 @@ struct Solver {
           reduce_bricks(c.partial, S_RR);
           k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), 1,
                                                swant.as<uint8_t>(), 0.0);
-          vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>(), c.w64.as<double>());
-          reduce_bricks(c.partial, S_RZNEW);
+          precond(S_RZNEW);
           k_cg_beta<<<nblk(ns), 256, 0, st>>>(ns, scal, done);
           k_update_p<double><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z, p);
         }
@@ -2376,73 +2457,6 @@ struct Solver {
         }
         if (std::all_of(dh.begin(), dh.end(), [](uint8_t d) { return d != 0; })) break;
       }
-    } else {
-      c.x32.ensure(v32);
-      c.r32.ensure(v32);
-      c.p32.ensure(v32);
-      c.ap32.ensure(v32);
-      c.z32.ensure(v32);
-      c.t32.ensure(v32);
-      c.u64.ensure(v32);  // reused as the second fine FP32 scratch
-      c.w32.ensure(v32);
-      float *x = c.x32.as<float>(), *r = c.r32.as<float>(), *p = c.p32.as<float>(), *ap = c.ap32.as<float>(),
-            *z = c.z32.as<float>();
-      float* x1 = c.t32.as<float>();
-      float* r1 = c.u64.as<float>();
-      ST_HIP(hipMemsetAsync(x, 0, v32, st));
-      k_copy_fine<float, double><<<nb, kBlock, 0, st>>>(nbase, bst, nullptr, c.r64.as<double>(), r);
-      vcycle<float>(r, z, x1, r1, c.w32.as<float>());
-      reduce_bricks(c.partial, S_RZ);
-      k_copy_fine<float, float><<<nb, kBlock, 0, st>>>(nbase, bst, done, z, p);
-      std::vector<uint8_t> dh, wh;
-      int it = 0, since = 0;
-      const int chunk = 4;
-      const int period = refresh > 0 ? refresh : 50;
-      while (it < max_iter) {
-        const int k_end = std::min(max_iter, it + chunk);
-        for (; it < k_end; it++, since++) {
-          EpiDot<float> ed;
-          ed.done = done;
-          ed.partial = partial;
-          ed.y = ap;
-          ed.p = p;
-          launch_apply<float>(c, p, ed);
-          reduce_bricks(c.partial, S_PAP);
-          k_cg_alpha<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>());
-          k_update_xr<float><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, x, r, p, ap, partial);
-          reduce_bricks(c.partial, S_RR);
-          k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), 1,
-                                               swant.as<uint8_t>(), 1e-2);
-          vcycle<float>(r, z, x1, r1, c.w32.as<float>());
-          reduce_bricks(c.partial, S_RZNEW);
-          k_cg_beta<<<nblk(ns), 256, 0, st>>>(ns, scal, done);
-          k_update_p<float><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z, p);
-        }
-        ST_HIP(hipGetLastError());
-        sync_flags(dh, wh);
-        bool any_want = false;
-        for (int s = 0; s < ns; s++) any_want |= !dh[s] && wh[s];
-        if (any_want || since >= period || it >= max_iter) {
-          // reliable update: x64 += x32; r = f - K x64 in FP64; the CG goes on from it
-          k_flush<<<nb, kBlock, 0, st>>>(nbase, bst, done, c.x64.as<double>(), x);
-          EpiResid<double, double> er;
-          er.done = done;
-          er.partial = partial;
-          er.b = c.f64.as<double>();
-          er.r = c.r64.as<double>();
-          launch_apply<double>(c, c.x64.as<double>(), er);
-          reduce_bricks(c.partial, S_RRTRUE);
-          k_copy_fine<float, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, c.r64.as<double>(), r);
-          k_cg_true<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), swant.as<uint8_t>(),
-                                              srefresh.as<int>());
-          since = 0;
-          sync_flags(dh, wh);
-        }
-        if (std::all_of(dh.begin(), dh.end(), [](uint8_t d) { return d != 0; })) break;
-      }
-      // anything left in the FP32 correction
-      ST_HIP(hipMemsetAsync(done, 0, ns, st));
-      k_flush<<<nb, kBlock, 0, st>>>(nbase, bst, done, c.x64.as<double>(), x);
     }
     ST_HIP(hipGetLastError());
   }
@@ -2469,8 +2483,8 @@ struct Solver {
 void release_levels(Ctx& c) {
   for (auto& lv : c.levels)
     for (DBuf* b : {&lv.rowptr, &lv.col, &lv.a64, &lv.a32, &lv.dinv64, &lv.dinv32, &lv.centre, &lv.parent, &lv.rep,
-                    &lv.key, &lv.kid_ptr, &lv.kids, &lv.agg_struct, &lv.step_d, &lv.r64, &lv.x64, &lv.t64, &lv.r32,
-                    &lv.x32, &lv.t32, &lv.prow, &lv.pcol, &lv.pval64, &lv.pval32, &lv.pt_ptr, &lv.pt_blk,
+                    &lv.key, &lv.kid_ptr, &lv.kids, &lv.agg_struct, &lv.step_d, &lv.r64, &lv.x64, &lv.t64, &lv.u64,
+                    &lv.v64, &lv.r32, &lv.x32, &lv.t32, &lv.u32, &lv.v32, &lv.prow, &lv.pcol, &lv.pval64, &lv.pval32, &lv.pt_ptr, &lv.pt_blk,
                     &lv.pblk_row})
       b->release();
   c.levels.clear();
@@ -2546,6 +2560,9 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   if (const char* e = std::getenv("STATICS_HIP_GAMMA")) c.gamma = std::max(1, std::atoi(e));
   c.gamma_from = 2;
   if (const char* e = std::getenv("STATICS_HIP_GFROM")) c.gamma_from = std::max(0, std::atoi(e));
+  // levels from 3 up in FP64 in the mixed V-cycle
+  c.f64_from = 2;
+  if (const char* e = std::getenv("STATICS_HIP_F64FROM")) c.f64_from = std::max(0, std::atoi(e));
   S.init_struct_arrays();
   S.setup_fine();
   S.setup_hierarchy();
