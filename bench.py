@@ -30,6 +30,89 @@ sys.path.insert(0, os.path.join(HERE, "parallax-world_amd"))
 import statics_hip as sh  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
+ROOFLINE_KERNEL = "k_apply<double, st::EpiDot<double> >"
+COPY_BYTES = 1 << 30
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "k_apply_epidot_f64_256_traffic.txt")
+
+
+def _pmc_per_dispatch(csv_path, counter):
+    """Mean counter value per dispatch for the roofline kernel and for the three calibration
+    copies (the last three copy dispatches of the run)."""
+    import csv
+    apply, copies = [], []
+    with open(csv_path) as f:
+        for row in csv.DictReader(f):
+            if row.get("Counter_Name") != counter:
+                continue
+            name = row["Kernel_Name"]
+            v = float(row["Counter_Value"])
+            if "EpiDot<double>" in name:
+                apply.append(v)
+            elif "copyBuffer" in name:
+                copies.append((int(row["Dispatch_Id"]), v))
+    copies = [v for _, v in sorted(copies)[-3:]]
+    vals = {"apply": apply, "copy": copies}
+    return {k: (sum(v) / len(v) if v else None) for k, v in vals.items()}, {k: len(v) for k, v in vals.items()}
+
+
+def copy_calibration():
+    """Three device-to-device copies of COPY_BYTES through the HIP runtime (its copy kernel
+    reads and writes exactly that many bytes): the reference for the traffic counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for buf in (a, b):
+        if hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(COPY_BYTES)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(COPY_BYTES))
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(COPY_BYTES), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def traffic_summary(fetch_csv, write_csv):
+    """FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the roofline kernel, each scaled by the
+    ratio of a 1 GiB device copy's known bytes to what the same counter reads for it."""
+    fe, nf = _pmc_per_dispatch(fetch_csv, "FETCH_SIZE")
+    wr, nw = _pmc_per_dispatch(write_csv, "WRITE_SIZE")
+    kf = COPY_BYTES / (fe["copy"] * 1024.0)
+    kw = COPY_BYTES / (wr["copy"] * 1024.0)
+    fetch_b = fe["apply"] * 1024.0 * kf
+    write_b = wr["apply"] * 1024.0 * kw
+    lines = [
+        f"# HBM traffic per launch of {ROOFLINE_KERNEL}, 256^3 bench workload",
+        "# commands (GPU, from the repository root, cd /tmp && export TMPDIR=/tmp first):",
+        "#   rocprofv3 --pmc FETCH_SIZE --kernel-include-regex 'EpiDot<double>|copyBuffer' -d <dir> -o fetch "
+        "--output-format csv -- python3 bench.py --steps 1 --warmup 0 --cpu-sample 0 --no-fp64 --copy-calib",
+        "#   rocprofv3 --pmc WRITE_SIZE (same otherwise, -o write)",
+        f"#   python3 bench.py --traffic <fetch_counter_collection.csv> <write_counter_collection.csv>",
+        f"fetch_kib_per_dispatch_raw: {fe['apply']:.1f} (dispatches {nf['apply']})",
+        f"write_kib_per_dispatch_raw: {wr['apply']:.1f} (dispatches {nw['apply']})",
+        f"copy_1GiB_fetch_kib_raw: {fe['copy']:.1f} (dispatches {nf['copy']}) -> fetch scale {kf:.4f}",
+        f"copy_1GiB_write_kib_raw: {wr['copy']:.1f} (dispatches {nw['copy']}) -> write scale {kw:.4f}",
+        f"fetch_bytes_per_launch: {fetch_b:.6e}",
+        f"write_bytes_per_launch: {write_b:.6e}",
+        f"traffic_bytes_per_launch: {fetch_b + write_b:.6e}",
+    ]
+    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
+    with open(TRAFFIC_FILE, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def measured_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed counter summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        return None
+    return None
 
 
 def structure(n, seed, sigma=0.5, spacing=8.0, wmin=3, wmax=7):
@@ -97,7 +180,16 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64, help="edge of the CPU oracle's sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="end with three device copies of 1 GiB (the calibration kernel of the traffic counters)")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="offline: HBM bytes per launch of the roofline kernel from two rocprofv3 --pmc "
+                         "counter_collection.csv files (FETCH_SIZE, WRITE_SIZE) of a --copy-calib run; "
+                         "writes " + TRAFFIC_FILE)
     args = ap.parse_args()
+    if args.traffic:
+        traffic_summary(*args.traffic)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is single-GPU by design", file=sys.stderr)
         sys.exit(2)
@@ -207,7 +299,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64",
+        "dtype": "f64 CG + f32 V-cycle",
         "data": "synthetic",
         "config": {"workload": f"one {n}^3 pillar/slab structure, ~30% solid, lognormal stiffness sigma 0.5, "
                                f"gravity + 64 point loads; step = classify + mixed solve + soft-max stress adjoint",
@@ -219,13 +311,15 @@ def main():
         "fp64_forward_solves_per_s": (1e3 / fp64["ms"]) if fp64 else None,
         "fp64_forward_iters": fp64["iters"] if fp64 else None,
         "mixed_vs_fp64_forward": (fp64["ms"] / mixed_fwd_ms) if fp64 else None,
-        "roofline": {"bound": "hbm", "kernel": "k_apply<float, EpiDot<float>> (CG p -> Ap, fused p.Ap)",
+        "roofline": {"bound": "hbm", "kernel": ROOFLINE_KERNEL + " (the CG's K p with the fused p.Ap, FP64)",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": None, "launches": int(nl.value), "avg_ms": kernel_ms,
+                     "traffic": measured_traffic(), "launches": int(nl.value), "avg_ms": kernel_ms,
                      "algorithmic_bytes_per_launch": bpl.value},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_calib:
+        copy_calibration()
 
 
 if __name__ == "__main__":

@@ -149,8 +149,8 @@ int st_classify(StCtx* ctx, int64_t* joined_count);
 
 /* Device time of the launches of one kernel, for measuring it against its
  * roofline: enable != 0 starts recording (and clears the totals) for kernel
- * `which` (0: the fine operator with the CG's fused p.Ap in FP32, 1: the same in
- * FP64); enable == 0 stops. st_kernel_timing reads the totals so far:
+ * `which` (0: the fine operator K p of the CG with its fused p.Ap, FP64 in both
+ * precision modes); enable == 0 stops. st_kernel_timing reads the totals so far:
  * launches, summed milliseconds, and the algorithmic bytes of one launch
  * (6 values per free node read and written, plus e, g, the beam mask and the
  * six face indices per free cell). */

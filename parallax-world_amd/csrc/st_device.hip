@@ -121,7 +121,7 @@ void timing_collect(Ctx& c) {
 template <class T, class Epi>
 void launch_apply(Ctx& c, const T* x, const Epi& epi) {
   if (!c.geo.nbricks) return;
-  const bool timed = std::is_same<Epi, EpiDot<T>>::value && c.timing_which == (sizeof(T) == 4 ? 0 : 1);
+  const bool timed = std::is_same<Epi, EpiDot<T>>::value && sizeof(T) == 8 && c.timing_which == 0;
   if (timed) {
     if (c.tev_used * 2 + 2 > c.tev.size()) {
       if (c.tev.size() >= 4096) timing_collect(c);

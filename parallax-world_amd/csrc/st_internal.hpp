@@ -155,8 +155,9 @@ struct Ctx {
   int nu1 = 1;           // the same at level 1
   int nu_c = 1;          // the same above level 1
   int gamma = 1;         // coarse corrections per visit above level 1 (1: V-cycle, 2: W-cycle)
-  int gamma_from = 1;    // first level (0: level 1) that repeats its coarse correction
+  int gamma_from = 1;    // levels above this index (0: level 1) are solved by gamma cycles
   int f64_from = 2;      // first level (0: level 1) kept in FP64 by the mixed V-cycle
+  bool graphs = false;   // replay each CG iteration's tail as a captured graph
   std::vector<Level> levels;        // coarse levels 1..
   // solver vectors
   DBuf f64, x64, r64, p64, ap64, z64, t64, u64, w64;

@@ -84,7 +84,7 @@ __device__ __forceinline__ void face_beam(const CellCoef<T>& c0, const CellCoef<
 // the brick's lower faces, and the rest of very full bricks) are read from memory.
 template <class T>
 struct Tile {
-  static constexpr int kNodes = sizeof(T) == 4 ? 768 : 512;
+  static constexpr int kNodes = 768;
   static constexpr int kCells = 256;
   const T* xs;
   const uint32_t* cf;
@@ -150,7 +150,8 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 
 // y = K x, one workgroup per brick and one thread per node: every node gathers the
 // beams of its two cells that end on it, so each row is summed by one thread in a
-// fixed order, with no atomics, and epi.node(n, row, s) consumes it in registers.
+// fixed order, with no atomics, and epi.node(n, row, s, xn) consumes it in
+// registers (xn: the node's own entries of x, from the tile where staged).
 // Epi::kReduce: the per-node values returned are summed per brick into epi.partial[b].
 template <class T>
 struct Vec2;
@@ -163,6 +164,16 @@ struct Vec2<double> {
   using type = double2;
 };
 
+// Workgroups are dealt to the 8 XCDs in turn (workgroup i runs on XCD i mod 8); this
+// maps each XCD's workgroups to one contiguous run of bricks, so bricks that are
+// neighbours in Morton order, and share halo nodes and cells, meet in the same L2.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_brick(uint32_t w, uint32_t nb) {
+  const uint32_t q = nb / kXcds, r = nb % kXcds;  // XCD k takes q bricks, plus one if k < r
+  const uint32_t k = w % kXcds, i = w / kXcds;
+  return k * q + min(k, r) + i;
+}
+
 template <class T, class Epi>
 __global__ __launch_bounds__(kBlock) void k_apply(GeoD<T> g, const T* __restrict__ x, Epi epi) {
   using V2 = typename Vec2<T>::type;
@@ -171,7 +182,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(GeoD<T> g, const T* __restrict
   __shared__ T ces[Tile<T>::kCells], cgs[Tile<T>::kCells];
   __shared__ uint16_t cms[Tile<T>::kCells];
   __shared__ double red[kBlock / 64];
-  const uint32_t b = blockIdx.x;
+  const uint32_t b = xcd_brick(blockIdx.x, gridDim.x);
   const uint32_t s = g.bstruct[b];
   if (epi.skip(s)) {
     if (Epi::kReduce && threadIdx.x == 0) epi.partial[b] = 0.0;
@@ -214,7 +225,8 @@ __global__ __launch_bounds__(kBlock) void k_apply(GeoD<T> g, const T* __restrict
       else
         node_row<T, 2>(g, t, G, x, cells, y);
     }
-    part += epi.node(v, y, s);
+    const uint32_t lv = v - t.n0;
+    part += epi.node(v, y, s, lv < t.ns ? t.xs + 6 * lv : x + 6 * (size_t)v);
   }
   if (Epi::kReduce) {
     const double tot = block_sum(part, red);
@@ -233,21 +245,20 @@ template <class T>
 struct EpiStore : EpiBase {
   static constexpr bool kReduce = false;
   T* y;
-  __device__ double node(uint32_t n, const T* a, uint32_t) const {
+  __device__ double node(uint32_t n, const T* a, uint32_t, const T*) const {
     store6<T>(y + 6 * (size_t)n, a);
     return 0.0;
   }
 };
 
-// y = K p and the partial p . y
+// y = K p and the partial p . y (p is the operator's argument)
 template <class T>
 struct EpiDot : EpiBase {
   static constexpr bool kReduce = true;
   T* y;
-  const T* p;
-  __device__ double node(uint32_t n, const T* a, uint32_t) const {
+  __device__ double node(uint32_t n, const T* a, uint32_t, const T* pn) const {
     T pv[6];
-    load6<T>(p + 6 * (size_t)n, pv);
+    load6<T>(pn, pv);
     store6<T>(y + 6 * (size_t)n, a);
     double d = 0;
 #pragma unroll
@@ -262,7 +273,7 @@ struct EpiResid : EpiBase {
   static constexpr bool kReduce = true;
   const TB* b;
   T* r;
-  __device__ double node(uint32_t n, const T* a, uint32_t) const {
+  __device__ double node(uint32_t n, const T* a, uint32_t, const T*) const {
     TB bv[6];
     load6<TB>(b + 6 * (size_t)n, bv);
     T rv[6];
@@ -310,28 +321,35 @@ __device__ __forceinline__ void mat6v(const T* __restrict__ m, const T* v, T* ou
   }
 }
 
-// z = xin + step D^-1 (b - K xin); partial b . z
+// z = x + step D^-1 (b - K x) (x the operator's argument); partial b . z, or with
+// bdot set, the FP64 bdot . z
 template <class T>
 struct EpiSmooth : EpiBase {
   static constexpr bool kReduce = true;
   const T* b;
-  const T* xin;
   const T* dinv;
   const double* step;  // per structure
+  const double* bdot = nullptr;
   T* z;
-  __device__ double node(uint32_t n, const T* a, uint32_t s) const {
+  __device__ double node(uint32_t n, const T* a, uint32_t s, const T* xn) const {
     T bv[6], xv[6], t[6], d[6];
     load6<T>(b + 6 * (size_t)n, bv);
-    load6<T>(xin + 6 * (size_t)n, xv);
+    load6<T>(xn, xv);
 #pragma unroll
     for (int k = 0; k < 6; k++) t[k] = bv[k] - a[k];
     sym6v<T>(dinv + kSym * (size_t)n, t, d);
     const T w = (T)step[s];
-    double acc = 0;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-      xv[k] += w * d[k];
-      acc += (double)bv[k] * (double)xv[k];
+    for (int k = 0; k < 6; k++) xv[k] += w * d[k];
+    double acc = 0;
+    if (bdot) {
+      double bd[6];
+      load6<double>(bdot + 6 * (size_t)n, bd);
+#pragma unroll
+      for (int k = 0; k < 6; k++) acc += bd[k] * (double)xv[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; k++) acc += (double)bv[k] * (double)xv[k];
     }
     store6<T>(z + 6 * (size_t)n, xv);
     return acc;
@@ -347,7 +365,7 @@ struct EpiResidJac : EpiBase {
   const double* step;
   T* r1;
   T* sout;
-  __device__ double node(uint32_t n, const T* a, uint32_t s) const {
+  __device__ double node(uint32_t n, const T* a, uint32_t s, const T*) const {
     T bv[6], t[6], d[6];
     load6<T>(b + 6 * (size_t)n, bv);
 #pragma unroll
@@ -362,17 +380,17 @@ struct EpiResidJac : EpiBase {
   }
 };
 
-// x += u - step D^-1 K u, u the rigid prolongation (the smoothed prolongator, matrix-free)
+// x += u - step D^-1 K u, u the operator's argument (the rigid prolongation of the
+// coarse correction): the smoothed prolongator, matrix-free
 template <class T>
 struct EpiProlong : EpiBase {
   static constexpr bool kReduce = false;
-  const T* u;
   const T* dinv;
   const double* step;
   T* x;
-  __device__ double node(uint32_t n, const T* a, uint32_t s) const {
+  __device__ double node(uint32_t n, const T* a, uint32_t s, const T* un) const {
     T uv[6], xv[6], d[6];
-    load6<T>(u + 6 * (size_t)n, uv);
+    load6<T>(un, uv);
     load6<T>(x + 6 * (size_t)n, xv);
     sym6v<T>(dinv + kSym * (size_t)n, a, d);
     const T w = (T)step[s];

@@ -400,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_fine(const uint32_t* nbase, c
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_update_xr(const uint32_t* nbase, const uint32_t* bstruct, const uint8_t* done,
                                                       const double* scal, T* x, T* r, const T* p, const T* ap,
-                                                      double* partial) {
+                                                      double* partial, float* r32 = nullptr) {
   __shared__ double red[kBlock / 64];
   const uint32_t b = blockIdx.x, s = bstruct[b];
   if (done[s]) {
@@ -413,6 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(const uint32_t* nbase, con
     x[i] = (T)((double)x[i] + a * (double)p[i]);
     const T rn = (T)((double)r[i] - a * (double)ap[i]);
     r[i] = rn;
+    if (r32) r32[i] = (float)rn;
     acc += (double)rn * (double)rn;
   }
   const double t = block_sum(acc, red);
@@ -420,9 +421,9 @@ __global__ __launch_bounds__(kBlock) void k_update_xr(const uint32_t* nbase, con
 }
 
 // p = z + beta p
-template <class T>
+template <class T, class TZ>
 __global__ __launch_bounds__(kBlock) void k_update_p(const uint32_t* nbase, const uint32_t* bstruct, const uint8_t* done,
-                                                     const double* scal, const T* z, T* p) {
+                                                     const double* scal, const TZ* z, T* p) {
   const uint32_t b = blockIdx.x, s = bstruct[b];
   if (done[s]) return;
   const double be = scal[kScal * s + S_BETA];
@@ -430,24 +431,6 @@ __global__ __launch_bounds__(kBlock) void k_update_p(const uint32_t* nbase, cons
     p[i] = (T)((double)z[i] + be * (double)p[i]);
 }
 
-// z64 = z32 (widened) and the partial r.z64 per brick (0 for finished structures)
-__global__ __launch_bounds__(kBlock) void k_widen_dot(const uint32_t* nbase, const uint32_t* bstruct, const uint8_t* done,
-                                                      const float* z32, double* z64, const double* r, double* partial) {
-  __shared__ double red[kBlock / 64];
-  const uint32_t b = blockIdx.x;
-  if (done[bstruct[b]]) {
-    if (threadIdx.x == 0) partial[b] = 0.0;
-    return;
-  }
-  double acc = 0.0;
-  for (uint32_t i = 6 * nbase[b] + threadIdx.x; i < 6 * nbase[b + 1]; i += blockDim.x) {
-    const double zd = (double)z32[i];
-    z64[i] = zd;
-    acc += r[i] * zd;
-  }
-  const double t = block_sum(acc, red);
-  if (threadIdx.x == 0) partial[b] = t;
-}
 
 // partial a.b per brick
 template <class T>
@@ -1616,6 +1599,35 @@ T* ptr(DBuf& b) {
 
 // ============================================================== host side
 
+// A captured launch sequence, replayed: run(stream, f) captures f's launches on the
+// first call (f must only enqueue work on the stream) and launches the graph.
+struct GraphExec {
+  hipGraphExec_t exec = nullptr;
+  template <class F>
+  void run(hipStream_t st, F f) {
+    if (!exec) {
+      ST_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        f();
+      } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(st, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        throw;
+      }
+      hipGraph_t gr = nullptr;
+      ST_HIP(hipStreamEndCapture(st, &gr));
+      const hipError_t e = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      ST_HIP(e);
+    }
+    ST_HIP(hipGraphLaunch(exec, st));
+  }
+  ~GraphExec() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
 struct Solver {
   Ctx& c;
   hipStream_t st;
@@ -2265,8 +2277,9 @@ struct Solver {
   }
 
   // z = M^-1 r at the fine level; partial r.z per brick into c.partial
+  // rdot: if set, the partial is rdot . z in FP64 instead of r . z
   template <class T>
-  void vcycle(const T* r, T* z, T* x1, T* r1, T* s1) {
+  void vcycle(const T* r, T* z, T* x1, T* r1, T* s1, const double* rdot = nullptr) {
     const uint8_t* done = c.done.as<uint8_t>();
     const unsigned nb = (unsigned)g.nbricks;
     const T* dinv = std::is_same<T, double>::value ? (const T*)c.dinv64.p : (const T*)c.dinv32.p;
@@ -2279,7 +2292,6 @@ struct Solver {
         es.done = done;
         es.partial = c.partial.as<double>();
         es.b = r;
-        es.xin = x1;
         es.dinv = dinv;
         es.step = c.step0_d.as<double>();
         es.z = s1;
@@ -2315,7 +2327,6 @@ struct Solver {
                                                g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), lvbuf<T>(l1, 1), s1);
       EpiProlong<T> ep;
       ep.done = done;
-      ep.u = s1;
       ep.dinv = dinv;
       ep.step = c.step0_d.as<double>();
       ep.x = x1;
@@ -2341,9 +2352,9 @@ struct Solver {
     es.done = done;
     es.partial = c.partial.as<double>();
     es.b = r;
-    es.xin = x1;
     es.dinv = dinv;
     es.step = c.step0_d.as<double>();
+    es.bdot = rdot;
     es.z = z;
     launch_apply<T>(c, x1, es);
     ST_HIP(hipGetLastError());
@@ -2402,23 +2413,43 @@ struct Solver {
         c.p32.ensure(v32);
         c.w32.ensure(v32);
       }
+      // (mixed: r32 is kept equal to r by the residual update; z stays FP32 in z32)
+      float* r32 = mixed ? c.r32.as<float>() : nullptr;
+      float* z32 = mixed ? c.z32.as<float>() : nullptr;
       auto precond = [&](int slot) {
-        if (!mixed) {
+        if (!mixed)
           vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>(), c.w64.as<double>());
-        } else {
-          float* r32 = c.r32.as<float>();
-          float* z32 = c.z32.as<float>();
-          k_copy_fine<float, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, r, r32);
-          vcycle<float>(r32, z32, c.t32.as<float>(), c.p32.as<float>(), c.w32.as<float>());
-          k_widen_dot<<<nb, kBlock, 0, st>>>(nbase, bst, done, z32, z, r, partial);
-        }
+        else
+          vcycle<float>(r32, z32, c.t32.as<float>(), c.p32.as<float>(), c.w32.as<float>(), r);
         reduce_bricks(c.partial, slot);
       };
+      if (mixed) k_copy_fine<float, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, r, r32);
       precond(S_RZ);
-      k_copy_fine<double, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, z, p);
+      if (mixed)
+        k_copy_fine<double, float><<<nb, kBlock, 0, st>>>(nbase, bst, done, z32, p);
+      else
+        k_copy_fine<double, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, z, p);
       std::vector<uint8_t> dh, wh;
       int it = 0;
       const int chunk = 4;
+      // Everything of an iteration after K p (step length, updates, the preconditioner
+      // with its hundreds of coarse-level launches, the next direction) has fixed
+      // arguments: it is captured once into a graph and replayed.
+      auto tail = [&]() {
+        reduce_bricks(c.partial, S_PAP);
+        k_cg_alpha<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>());
+        k_update_xr<double><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, x, r, p, ap, partial, r32);
+        reduce_bricks(c.partial, S_RR);
+        k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), 1,
+                                             swant.as<uint8_t>(), 0.0);
+        precond(S_RZNEW);
+        k_cg_beta<<<nblk(ns), 256, 0, st>>>(ns, scal, done);
+        if (mixed)
+          k_update_p<double, float><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z32, p);
+        else
+          k_update_p<double, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z, p);
+      };
+      GraphExec tail_graph;
       while (it < max_iter) {
         const int k_end = std::min(max_iter, it + chunk);
         for (; it < k_end; it++) {
@@ -2426,17 +2457,11 @@ struct Solver {
           ed.done = done;
           ed.partial = partial;
           ed.y = ap;
-          ed.p = p;
           launch_apply<double>(c, p, ed);
-          reduce_bricks(c.partial, S_PAP);
-          k_cg_alpha<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>());
-          k_update_xr<double><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, x, r, p, ap, partial);
-          reduce_bricks(c.partial, S_RR);
-          k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), 1,
-                                               swant.as<uint8_t>(), 0.0);
-          precond(S_RZNEW);
-          k_cg_beta<<<nblk(ns), 256, 0, st>>>(ns, scal, done);
-          k_update_p<double><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z, p);
+          if (c.graphs)
+            tail_graph.run(st, tail);
+          else
+            tail();
         }
         ST_HIP(hipGetLastError());
         sync_flags(dh, wh);
@@ -2550,15 +2575,19 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   if (const char* e = std::getenv("STATICS_HIP_SA_LEVELS")) c.sa_levels = std::max(1, std::atoi(e));
   c.nu = 1;
   if (const char* e = std::getenv("STATICS_HIP_NU")) c.nu = std::max(1, std::atoi(e));
+  // graph replay of the iteration tail: opt-in (STATICS_HIP_GRAPHS=1); at 256^3 the launch
+  // queue stays full without it, and rocprofv3's kernel tracing does not follow captures
+  c.graphs = std::getenv("STATICS_HIP_GRAPHS") != nullptr;
   c.nu1 = 1;
   if (const char* e = std::getenv("STATICS_HIP_NU1")) c.nu1 = std::max(1, std::atoi(e));
   c.nu_c = 1;
   if (const char* e = std::getenv("STATICS_HIP_NUC")) c.nu_c = std::max(1, std::atoi(e));
-  // W-cycle from level 3 up: the coarse levels are built from rigid transfers, and
-  // visiting them twice per cycle halves the iteration count at little cost
+  // W-cycle: levels from 3 up are solved by two cycles each (the second on the remaining
+  // residual); their operators come from rigid transfers, and the second visit
+  // halves the iteration count at a fraction of an iteration's cost
   c.gamma = 2;
   if (const char* e = std::getenv("STATICS_HIP_GAMMA")) c.gamma = std::max(1, std::atoi(e));
-  c.gamma_from = 2;
+  c.gamma_from = 1;
   if (const char* e = std::getenv("STATICS_HIP_GFROM")) c.gamma_from = std::max(0, std::atoi(e));
   // levels from 3 up in FP64 in the mixed V-cycle
   c.f64_from = 2;
