@@ -62,20 +62,25 @@ typedef struct {
   double spacing;       /* metres per cell: UNIT * 2^level (statics/src/lattice.rs:34-36) */
 } StCells;
 
+/* ST_FP64: everything in FP64. ST_MIXED: the CG (vectors, K p, inner products,
+ * step lengths) stays in FP64 and the V-cycle preconditioner runs in FP32 on the
+ * fine level and the first two coarse levels (their operators, transfers and
+ * smoothers), FP64 above. Both recompute the FP64 true residual when the
+ * recursive one meets the tolerance, and go on from it if it does not. */
 typedef enum { ST_FP64 = 0, ST_MIXED = 1 } StPrecision;
 
 typedef struct {
   int32_t precision;   /* StPrecision */
   double tol;          /* stop when ||f - K x||_2 <= tol * ||f||_2 (statics/src/bodies.rs:350-352) */
   int32_t max_iter;    /* cap on CG iterations (the reference: 100,000) */
-  int32_t refresh;     /* ST_MIXED: recompute the FP64 true residual every `refresh` iterations (0: 50) */
+  int32_t refresh;     /* reserved, ignored (0) */
   int32_t warm_start;  /* nonzero: start from x_out's contents instead of 0 */
 } StSolveOpts;
 
 typedef struct {
   int32_t iters;          /* CG iterations taken */
   int32_t levels;         /* multigrid levels, the finest included */
-  int32_t refreshes;      /* FP64 true-residual recomputations (ST_MIXED) */
+  int32_t refreshes;      /* FP64 true-residual recomputations */
   int32_t status;         /* StStatus of the solve */
   double rel_residual;    /* final ||f - K x|| / ||f||, recomputed in FP64 */
   double setup_ms;        /* classification, numbering, hierarchy */

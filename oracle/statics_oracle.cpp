@@ -24,9 +24,10 @@
 // converged solution.
 //
 // Additions that the reference does not have (oracle for new functionality):
-//   orc_grad_*     compliance and soft-max stress gradients w.r.t. per-cell
-//                  log-stiffness (adjoint), pinned by central differences and
-//                  the scaling invariants.
+//   orc_scale_cell scales one cell's stiffness (its bonds' E and G), so tests take
+//                  central differences of compliance and soft-max stress w.r.t.
+//                  per-cell log-stiffness from oracle solves; the device adjoint
+//                  is checked against those and the scaling invariants.
 #include <algorithm>
 #include <array>
 #include <chrono>
