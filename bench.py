@@ -170,6 +170,73 @@ def cpu_baseline(size, threads, seed=1):
     return dict(seconds=t_solve, build_s=t_build, iters=st["iters"], dof=dof, levels=st["levels"])
 
 
+def batch_structures(count, seed):
+    """The batch workload: side L = 16 * 8^U (16..128), solid fraction about U(0.15, 0.40) reached
+    through the pillar spacing (widths 3-7; fill 0.32 at spacing 8, ~ 1/spacing^2), lognormal stiffness sigma 0.5."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        side = int(16 * 8 ** rng.uniform(0.0, 1.0))
+        phi = rng.uniform(0.15, 0.40)
+        spacing = 8.0 * np.sqrt(0.32 / phi)
+        occ, theta = sh.generate(side, side, side, seed=seed * 100003 + i, sigma=0.5, pillar_spacing=spacing,
+                                 pillar_min=3, pillar_max=7)
+        out.append(sh.cells_from_occ(occ, theta))
+    return out
+
+
+def batch_bench(args):
+    """Solves the batch in st_solve_batch calls of about --batch-cells free cells each (mixed, 1e-8,
+    gravity), after one untimed call on the first chunk; reports structures solved per second."""
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    t = time.perf_counter()
+    sets = batch_structures(args.batch, args.seed)
+    cells = [int((h == 0).sum()) for _, h, _, _ in sets]
+    sides = [int(s[0].max()) + 1 for s in sets]
+    log(f"[bench] batch of {len(sets)}: {sum(cells)} free cells, sides {min(sides)}..{max(sides)} "
+        f"({time.perf_counter() - t:.1f}s)")
+    chunks, cur, acc = [], [], 0
+    for i, c in enumerate(cells):
+        if cur and acc + c > args.batch_cells:
+            chunks.append(cur)
+            cur, acc = [], 0
+        cur.append(i)
+        acc += c
+    if cur:
+        chunks.append(cur)
+    ctx = sh.Context()
+    ctx.solve_batch([sets[i] for i in chunks[0]], precision=sh.MIXED, tol=1e-8, max_iter=20000, want_x=False)
+    ctx._check(ctx.L.st_kernel_timing_enable(ctx.h, 0, 1), "timing")
+    t0 = time.perf_counter()
+    stats = []
+    for k, ch in enumerate(chunks):
+        tc = time.perf_counter()
+        _, st = ctx.solve_batch([sets[i] for i in ch], precision=sh.MIXED, tol=1e-8, max_iter=20000, want_x=False)
+        stats += st
+        log(f"[bench] chunk {k}: {len(ch)} structures, {sum(cells[i] for i in ch)} cells, "
+            f"{(time.perf_counter() - tc) * 1e3:.0f} ms, iterations max {max(s.iters for s in st)}, "
+            f"setup {st[0].setup_ms:.0f} ms, solve {st[0].solve_ms:.0f} ms")
+    elapsed = time.perf_counter() - t0
+    import ctypes
+    nl, tms, bpl = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+    ctx._check(ctx.L.st_kernel_timing(ctx.h, ctypes.byref(nl), ctypes.byref(tms), ctypes.byref(bpl)), "timing")
+    bad = [i for i, s in enumerate(stats) if s.status != 0]
+    out = {
+        "metric": "structures solved/sec to 1e-8 residual, batch of uneven structures (mixed precision)",
+        "value": len(sets) / elapsed, "unit": "structures/sec", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": elapsed * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 CG + f32 V-cycle", "data": "synthetic",
+        "config": {"workload": f"batch of {len(sets)} pillar/slab structures, side 16*8^U, fill ~U(0.15,0.40), "
+                               f"lognormal sigma 0.5, gravity; {len(chunks)} st_solve_batch calls of <= "
+                               f"{args.batch_cells:.0f} free cells", "structures": len(sets),
+                   "free_cells": sum(cells), "chunks": len(chunks)},
+        "iterations_max": max(s.iters for s in stats), "iterations_mean": float(np.mean([s.iters for s in stats])),
+        "failed": bad,
+        "roofline_kernel_avg_ms": tms.value / max(nl.value, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,6 +247,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=64, help="edge of the CPU oracle's sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="instead: the batch workload, this many structures (512: the batch config)")
+    ap.add_argument("--batch-cells", type=float, default=14e6,
+                    help="batch: free cells per st_solve_batch call (the structures are packed in order)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="end with three device copies of 1 GiB (the calibration kernel of the traffic counters)")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -193,6 +264,9 @@ def main():
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is single-GPU by design", file=sys.stderr)
         sys.exit(2)
+    if args.batch:
+        batch_bench(args)
+        return
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
 
     n = args.size

@@ -2,6 +2,8 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <unordered_map>
 
@@ -50,6 +52,7 @@ void copy_geo_to_device(Ctx& c) {
   d.ncells = (int64_t)h.ce.size();
   d.nnodes = (int64_t)h.nflags.size();
   d.nstruct = (int32_t)h.structs.size();
+  c.tuned = false;
   up(d.cbase, h.cbase, s);
   up(d.nbase, h.nbase, s);
   up(d.bstruct, h.bstruct, s);
@@ -133,7 +136,24 @@ void launch_apply(Ctx& c, const T* x, const Epi& epi) {
     }
     ST_HIP(hipEventRecord(c.tev[2 * c.tev_used], c.stream));
   }
-  k_apply<T, Epi><<<(unsigned)c.geo.nbricks, kBlock, 0, c.stream>>>(geo_view<T>(c), x, epi);
+  GeoD<T> g = geo_view<T>(c);
+  const int sched = c.apply_sched[sizeof(T) == 8 ? 1 : 0];
+  g.xcd = kSchedules[sched].xcd;
+  const unsigned nb = (unsigned)c.geo.nbricks;
+  switch (kSchedules[sched].tile) {
+    case 0:
+      k_apply<T, Epi, 768, 256><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
+      break;
+    case 1:
+      k_apply<T, Epi, 512, 192><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
+      break;
+    case 2:
+      k_apply<T, Epi, 384, 128><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
+      break;
+    default:
+      k_apply<T, Epi, 0, 0><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
+      break;
+  }
   ST_HIP(hipGetLastError());
   if (timed) {
     ST_HIP(hipEventRecord(c.tev[2 * c.tev_used + 1], c.stream));
@@ -160,6 +180,65 @@ ST_INST_APPLY(double, EpiResidJac<double>)
 ST_INST_APPLY(float, EpiResidJac<float>)
 ST_INST_APPLY(double, EpiProlong<double>)
 ST_INST_APPLY(float, EpiProlong<float>)
+
+namespace {
+template <class T>
+int tune_one(Ctx& c, T* x, T* y) {
+  hipEvent_t e0, e1;
+  ST_HIP(hipEventCreate(&e0));
+  ST_HIP(hipEventCreate(&e1));
+  EpiStore<T> es;
+  es.y = y;
+  int best = 0;
+  float best_ms = 1e30f;
+  const int saved = c.apply_sched[sizeof(T) == 8 ? 1 : 0];
+  for (int k = 0; k < kNumSchedules; k++) {
+    c.apply_sched[sizeof(T) == 8 ? 1 : 0] = k;
+    launch_apply<T>(c, x, es);  // warm
+    ST_HIP(hipEventRecord(e0, c.stream));
+    for (int r = 0; r < 3; r++) launch_apply<T>(c, x, es);
+    ST_HIP(hipEventRecord(e1, c.stream));
+    ST_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    ST_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (std::getenv("STATICS_HIP_VERBOSE"))
+      std::fprintf(stderr, "statics_hip: schedule %d (tile %d, xcd %d) %s: %.3f ms\n", k, kSchedules[k].tile,
+                   kSchedules[k].xcd, sizeof(T) == 8 ? "fp64" : "fp32", ms / 3);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = k;
+    }
+  }
+  c.apply_sched[sizeof(T) == 8 ? 1 : 0] = saved;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+}  // namespace
+
+// Picks the operator's schedule (LDS tile capacity, brick-to-XCD mapping) for each
+// precision by timing every one on the uploaded structures; STATICS_HIP_SCHEDULE
+// forces one. Runs once per upload.
+void autotune_apply(Ctx& c) {
+  if (c.tuned || !c.geo.nbricks) return;
+  if (const char* e = std::getenv("STATICS_HIP_SCHEDULE")) {
+    const int k = std::max(0, std::min(kNumSchedules - 1, std::atoi(e)));
+    c.apply_sched[0] = c.apply_sched[1] = k;
+    c.tuned = true;
+    return;
+  }
+  const size_t n = (size_t)c.geo.nnodes * 6;
+  DBuf a, b;
+  a.ensure(n * sizeof(double));
+  b.ensure(n * sizeof(double));
+  ST_HIP(hipMemsetAsync(a.p, 0, n * sizeof(double), c.stream));
+  c.apply_sched[0] = tune_one<float>(c, a.as<float>(), b.as<float>());
+  c.apply_sched[1] = tune_one<double>(c, a.as<double>(), b.as<double>());
+  ST_HIP(hipStreamSynchronize(c.stream));
+  a.release();
+  b.release();
+  c.tuned = true;
+}
 
 }  // namespace st
 

@@ -106,6 +106,15 @@ struct DevGeo {
   DBuf bgeo;  // BeamGeo per structure
 };
 
+// Schedules of the fine operator: LDS tile capacity (0: 768 nodes / 256 cells,
+// 1: 512 / 192, 2: 384 / 128, 3: none) and whether bricks are dealt to XCDs in
+// contiguous runs.
+struct ApplySchedule {
+  int tile, xcd;
+};
+constexpr ApplySchedule kSchedules[] = {{0, 1}, {1, 1}, {2, 1}, {3, 1}, {0, 0}, {1, 0}, {2, 0}, {3, 0}};
+constexpr int kNumSchedules = (int)(sizeof(kSchedules) / sizeof(kSchedules[0]));
+
 // One coarse level (aggregates of the level below, 6 DOF each), block-sparse.
 struct Level {
   int64_t n = 0;        // aggregates
@@ -156,8 +165,11 @@ struct Ctx {
   int nu_c = 1;          // the same above level 1
   int gamma = 1;         // coarse corrections per visit above level 1 (1: V-cycle, 2: W-cycle)
   int gamma_from = 1;    // levels above this index (0: level 1) are solved by gamma cycles
+  int gamma_to = 1000;   // ... up to this index
   int f64_from = 2;      // first level (0: level 1) kept in FP64 by the mixed V-cycle
   bool graphs = false;   // replay each CG iteration's tail as a captured graph
+  int apply_sched[2] = {0, 0};  // operator schedule per precision (FP32, FP64), see kSchedules
+  bool tuned = false;           // apply_sched chosen for the uploaded structures
   std::vector<Level> levels;        // coarse levels 1..
   // solver vectors
   DBuf f64, x64, r64, p64, ap64, z64, t64, u64, w64;

@@ -22,6 +22,7 @@ struct GeoD {
   const uint32_t *cbase, *nbase, *bstruct, *cfaces;
   const uint8_t *nflags, *nkind;
   const int32_t* ncell;
+  int xcd;  // deal bricks to XCDs in contiguous runs
   const T *ce, *cg;
   const uint16_t* cmask;
   const BeamGeo* bgeo;
@@ -82,16 +83,28 @@ __device__ __forceinline__ void face_beam(const CellCoef<T>& c0, const CellCoef<
 // ns nodes and the face tables, e, g and beam masks of its first ncs cells (both
 // read coalesced). Faces and cells outside the staged ranges (the neighbours across
 // the brick's lower faces, and the rest of very full bricks) are read from memory.
+// How much to stage is a schedule choice (LDS per workgroup against waves per CU):
+// the kernel is instantiated for several (nodes, cells) capacities and the fastest
+// is picked by timing on the uploaded structure.
+// Pointers into LDS, so that loads through them are ds_read (not flat) loads.
+template <class T>
+using lds_cptr = const T __attribute__((address_space(3)))*;
+
 template <class T>
 struct Tile {
-  static constexpr int kNodes = 768;
-  static constexpr int kCells = 256;
-  const T* xs;
-  const uint32_t* cf;
-  const T *ce, *cg;
-  const uint16_t* cm;
+  lds_cptr<T> xs;
+  lds_cptr<uint32_t> cf;
+  lds_cptr<T> ce;
+  lds_cptr<T> cg;
+  lds_cptr<uint16_t> cm;
   uint32_t n0, ns, c0, ncs;
 };
+
+template <class T>
+__device__ __forceinline__ void lds_load6(lds_cptr<T> p, T* v) {
+#pragma unroll
+  for (int k = 0; k < 6; k++) v[k] = p[k];
+}
 
 template <class T, int F>
 __device__ __forceinline__ void cell_face_row(const GeoD<T>& g, const Tile<T>& t, const BeamGeo& G,
@@ -117,7 +130,7 @@ __device__ __forceinline__ void cell_face_row(const GeoD<T>& g, const Tile<T>& t
 #pragma unroll
       for (int k = 0; k < 6; k++) xv[f][k] = T(0);
     } else if (ln < t.ns) {
-      load6<T>(t.xs + 6 * ln, xv[f]);
+      lds_load6<T>(t.xs + 6 * ln, xv[f]);
     } else {
       load6<T>(x + 6 * (size_t)fi[f], xv[f]);
     }
@@ -174,15 +187,21 @@ __device__ __forceinline__ uint32_t xcd_brick(uint32_t w, uint32_t nb) {
   return k * q + min(k, r) + i;
 }
 
-template <class T, class Epi>
-__global__ __launch_bounds__(kBlock) void k_apply(GeoD<T> g, const T* __restrict__ x, Epi epi) {
+// Waves per SIMD the operator's register allocation aims at (it is latency-bound:
+// more waves in flight hide the gathers).
+#ifndef ST_APPLY_WAVES
+#define ST_APPLY_WAVES 1
+#endif
+
+template <class T, class Epi, int kNodes, int kCells>
+__global__ __launch_bounds__(kBlock, ST_APPLY_WAVES) void k_apply(GeoD<T> g, const T* __restrict__ x, Epi epi) {
   using V2 = typename Vec2<T>::type;
-  __shared__ V2 xs2[Tile<T>::kNodes * 3];
-  __shared__ uint2 cf2[Tile<T>::kCells * 3];
-  __shared__ T ces[Tile<T>::kCells], cgs[Tile<T>::kCells];
-  __shared__ uint16_t cms[Tile<T>::kCells];
+  __shared__ V2 xs2[kNodes > 0 ? kNodes * 3 : 1];
+  __shared__ uint2 cf2[kCells > 0 ? kCells * 3 : 1];
+  __shared__ T ces[kCells > 0 ? kCells : 1], cgs[kCells > 0 ? kCells : 1];
+  __shared__ uint16_t cms[kCells > 0 ? kCells : 1];
   __shared__ double red[kBlock / 64];
-  const uint32_t b = xcd_brick(blockIdx.x, gridDim.x);
+  const uint32_t b = g.xcd ? xcd_brick(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint32_t s = g.bstruct[b];
   if (epi.skip(s)) {
     if (Epi::kReduce && threadIdx.x == 0) epi.partial[b] = 0.0;
@@ -191,9 +210,9 @@ __global__ __launch_bounds__(kBlock) void k_apply(GeoD<T> g, const T* __restrict
   Tile<T> t;
   t.n0 = g.nbase[b];
   const uint32_t n_end = g.nbase[b + 1];
-  t.ns = min(n_end - t.n0, (uint32_t)Tile<T>::kNodes);
+  t.ns = min(n_end - t.n0, (uint32_t)kNodes);
   t.c0 = g.cbase[b];
-  t.ncs = min(g.cbase[b + 1] - t.c0, (uint32_t)Tile<T>::kCells);
+  t.ncs = min(g.cbase[b + 1] - t.c0, (uint32_t)kCells);
   {
     const V2* xg = reinterpret_cast<const V2*>(x + 6 * (size_t)t.n0);
     for (uint32_t i = threadIdx.x; i < 3 * t.ns; i += blockDim.x) xs2[i] = xg[i];
@@ -205,11 +224,11 @@ __global__ __launch_bounds__(kBlock) void k_apply(GeoD<T> g, const T* __restrict
       cms[i] = g.cmask[t.c0 + i];
     }
   }
-  t.xs = reinterpret_cast<const T*>(xs2);
-  t.cf = reinterpret_cast<const uint32_t*>(cf2);
-  t.ce = ces;
-  t.cg = cgs;
-  t.cm = cms;
+  t.xs = (lds_cptr<T>)(reinterpret_cast<const T*>(xs2));
+  t.cf = (lds_cptr<uint32_t>)(reinterpret_cast<const uint32_t*>(cf2));
+  t.ce = (lds_cptr<T>)ces;
+  t.cg = (lds_cptr<T>)cgs;
+  t.cm = (lds_cptr<uint16_t>)cms;
   const BeamGeo G = g.bgeo[s];
   __syncthreads();
   double part = 0.0;
@@ -226,7 +245,12 @@ __global__ __launch_bounds__(kBlock) void k_apply(GeoD<T> g, const T* __restrict
         node_row<T, 2>(g, t, G, x, cells, y);
     }
     const uint32_t lv = v - t.n0;
-    part += epi.node(v, y, s, lv < t.ns ? t.xs + 6 * lv : x + 6 * (size_t)v);
+    T xo[6];  // the node's own entries of x, for epilogues that use them
+    if (lv < t.ns)
+      lds_load6<T>(t.xs + 6 * lv, xo);
+    else
+      load6<T>(x + 6 * (size_t)v, xo);
+    part += epi.node(v, y, s, xo);
   }
   if (Epi::kReduce) {
     const double tot = block_sum(part, red);

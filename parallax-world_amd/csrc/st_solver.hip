@@ -886,19 +886,21 @@ __global__ void k_c_resid(int mode, int64_t n, const uint32_t* ast, const uint8_
 // y = A x (for power iterations), then partial sums of x.x per aggregate are not needed: w = D^-1 A x
 template <class T>
 __global__ void k_c_dinvA(int64_t n, const int* rowptr, const int* col, const T* A, const T* dinv, const T* x, T* w) {
-  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (I >= n) return;
-  T acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int b = rowptr[I]; b < rowptr[I + 1]; b++) {
-    const T* M = A + 36 * (int64_t)b;
-    T xv[6];
-    load6<T>(x + 6 * (int64_t)col[b], xv);
-    for (int i = 0; i < 6; i++)
-      for (int j = 0; j < 6; j++) acc[i] += M[6 * i + j] * xv[j];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t I = t >> 3;
+  const int i = (int)(t & 7);
+  T acc = 0;
+  if (I < n && i < 6) {
+    const int b1 = rowptr[I + 1];
+    for (int b = rowptr[I]; b < b1; b++) {
+      T xv[6];
+      load6<T>(x + 6 * (int64_t)col[b], xv);
+      acc += row6_dot<T>(A + 36 * (int64_t)b + 6 * i, xv);
+    }
   }
-  T d[6];
-  mat6v<T>(dinv + 36 * I, acc, d);
-  store6<T>(w + 6 * I, d);
+  T all[6];
+  group_gather6<T>(acc, all);
+  if (I < n && i < 6) w[6 * I + i] = row6_dot<T>(dinv + 36 * I + 6 * i, all);
 }
 
 template <class T>
@@ -1381,7 +1383,9 @@ __global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, c
           for (int i2 = 0; i2 < 6; i2++) t += PiI[6 * i2 + r] * Ab[6 * i2 + k];
           qr[k] = t;
         }
-        for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) {
+        const int k2a = prow[j], len = prow[j + 1] - k2a;
+        for (int kk = 0; kk < len; kk++) {
+          const int k2 = k2a + (kk + it) % len;
           const int sl = hfind<kH>(keys, slot, pcol[k2]);
           const double* Pj = pval + 36 * (int64_t)k2;
           double* out = acc + 36 * sl + 6 * r;
@@ -1407,11 +1411,20 @@ constexpr int kRowClass[4] = {64, 128, 256, 448};
 // Rows by width class: lists[k * n + i] for class k (order within a class is free:
 // every row is computed on its own).
 __global__ void k_row_classes(int64_t n, const int* cnt, int* lists, int* ctr) {
+  __shared__ int local[4], base[4];
+  if (threadIdx.x < 4) local[threadIdx.x] = 0;
+  __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int w = cnt[i];
-  const int k = w <= kRowClass[0] ? 0 : (w <= kRowClass[1] ? 1 : (w <= kRowClass[2] ? 2 : 3));
-  lists[k * n + atomicAdd(&ctr[k], 1)] = (int)i;
+  int k = -1, pos = 0;
+  if (i < n) {
+    const int w = cnt[i];
+    k = w <= kRowClass[0] ? 0 : (w <= kRowClass[1] ? 1 : (w <= kRowClass[2] ? 2 : 3));
+    pos = atomicAdd(&local[k], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) base[threadIdx.x] = local[threadIdx.x] ? atomicAdd(&ctr[threadIdx.x], local[threadIdx.x]) : 0;
+  __syncthreads();
+  if (k >= 0) lists[k * n + base[k] + pos] = (int)i;
 }
 
 // The five beams touching face f (pair indices, ascending).
@@ -1503,7 +1516,11 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
       double qr[6];  // row r of P0[v,I]^T K_b
       beam_row(pc, sv, sd == 0 ? sv : -sv, n, G.hd[cls], G.K4[cls], G.R2[cls], e, g, qr);
       const int row = sd == 0 ? v : (int)wf;
-      for (int k2 = prow[row]; k2 < prow[row + 1]; k2++) {
+      // start each task's column loop at a different column, so that the tasks of a
+      // wave (mostly nodes of one aggregate) do not add into the same block at once
+      const int k2a = prow[row], len = prow[row + 1] - k2a;
+      for (int kk = 0; kk < len; kk++) {
+        const int k2 = k2a + (kk + rest) % len;
         const int sl = hfind<kH>(keys, slot, pcol[k2]);
         const double* Pj = pval + 36 * (int64_t)k2;
         double* out = acc + 36 * sl + 6 * r;
@@ -1998,7 +2015,7 @@ struct Solver {
     k_c_init<<<nblk(n), 256, 0, st>>>(n, a);
     const int iters = 12;
     for (int it = 0; it < iters; it++) {
-      k_c_dinvA<double><<<nblk(n), 256, 0, st>>>(n, lv.rowptr.as<int>(), lv.col.as<int>(), lv.a64.as<double>(),
+      k_c_dinvA<double><<<nblk(8 * n), 256, 0, st>>>(n, lv.rowptr.as<int>(), lv.col.as<int>(), lv.a64.as<double>(),
                                                  lv.dinv64.as<double>(), a, b);
       if (it >= iters - 2) {
         k_c_sq<double><<<nblk(n), 256, 0, st>>>(n, b, sq.as<double>());
@@ -2214,7 +2231,7 @@ struct Solver {
       return;
     }
     one_cycle<T>(li, r, x);
-    const int reps = (int)li > c.gamma_from ? c.gamma : 1;
+    const int reps = ((int)li > c.gamma_from && (int)li <= c.gamma_to) ? c.gamma : 1;
     T* t = lvbuf<T>(lv, 2);
     T* v = lvbuf<T>(lv, 4);
     for (int k = 1; k < reps; k++) {
@@ -2589,15 +2606,21 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   if (const char* e = std::getenv("STATICS_HIP_GAMMA")) c.gamma = std::max(1, std::atoi(e));
   c.gamma_from = 1;
   if (const char* e = std::getenv("STATICS_HIP_GFROM")) c.gamma_from = std::max(0, std::atoi(e));
+  c.gamma_to = 5;  // levels 3-6 (above, the levels are tiny and nearly exact)
+  if (const char* e = std::getenv("STATICS_HIP_GTO")) c.gamma_to = std::max(0, std::atoi(e));
   // levels from 3 up in FP64 in the mixed V-cycle
   c.f64_from = 2;
   if (const char* e = std::getenv("STATICS_HIP_F64FROM")) c.f64_from = std::max(0, std::atoi(e));
+  autotune_apply(c);
   S.init_struct_arrays();
   S.setup_fine();
   S.setup_hierarchy();
   ST_HIP(hipEventRecord(e1, c.stream));
   if (std::getenv("STATICS_HIP_VERBOSE")) {  // the hierarchy's shape, for tuning
-    std::fprintf(stderr, "statics_hip: fine %lld nodes (%lld free)\n", (long long)c.geo.nnodes, (long long)S.ndof);
+    size_t mfree = 0, mtotal = 0;
+    (void)hipMemGetInfo(&mfree, &mtotal);
+    std::fprintf(stderr, "statics_hip: fine %lld nodes (%lld free), device memory in use %.1f GB\n",
+                 (long long)c.geo.nnodes, (long long)S.ndof, (double)(mtotal - mfree) / 1e9);
     for (size_t l = 0; l < c.levels.size(); l++)
       std::fprintf(stderr, "statics_hip: level %zu cube %d: %lld aggregates, %lld blocks (%.1f/row), P below %lld\n",
                    l + 1, c.levels[l].cube, (long long)c.levels[l].n, (long long)c.levels[l].nblocks,

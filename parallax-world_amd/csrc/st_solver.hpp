@@ -21,6 +21,7 @@ void classify_device(Ctx& c, int64_t* joined_count, uint8_t* joined_host);
 double device_dot_f_x(Ctx& c, const double* a, const double* b);
 void set_gone_flags(Ctx& c, const uint8_t* gone);
 void set_beam_masks(Ctx& c, const uint16_t* beams);
+void autotune_apply(Ctx& c);
 
 }  // namespace st
 
