@@ -237,6 +237,79 @@ def batch_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def fd_check(args, eps=1e-4, beta=20.0):
+    import ctypes
+    """Adjoint gradients against central differences of FP64 solves (tol 1e-10: the FP64 floor of
+    these systems lies near 1e-11; the achieved residuals are reported) at random free cells."""
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    n = args.size
+    occ, theta = structure(n, args.seed)
+    sites, held, e, g = sh.cells_from_occ(occ, theta)
+    ctx = sh.Context().upload(sites, held, e, g)
+    joined = ctx.joined_to_held()
+    nheld = ctx.node_held()
+    faces = ctx.node_faces()
+    f = node_loads(occ, faces, joined, nheld, np.random.default_rng(args.seed), n_points=0)
+    t0 = time.perf_counter()
+    _, st0 = ctx.solve(f, precision=sh.FP64, tol=1e-10, max_iter=20000, check=False)
+    worst = st0.rel_residual
+    grads = {}
+    for name, obj in (("compliance", sh.COMPLIANCE), ("softmax_stress", sh.SOFTMAX_STRESS)):
+        o = sh.StSolveOpts(sh.FP64, 1e-10, 20000, 0, 0)
+        Jc, d, sa = ctypes.c_double(), np.zeros(len(sites)), sh.StStats()
+        ctx.L.st_backward(ctx.h, obj, beta, ctypes.byref(o), ctypes.byref(Jc), sh._p(d), ctypes.byref(sa))
+        worst = max(worst, sa.rel_residual)
+        grads[name] = (Jc.value, d)
+    t_adj = time.perf_counter() - t0
+    log(f"[bench] adjoints done ({t_adj:.1f} s), worst residual {worst:.1e}")
+    rng = np.random.default_rng(args.seed + 1)
+    free = np.flatnonzero(held == 0)
+    pick = rng.choice(free, size=min(args.fd_check, len(free)), replace=False)
+    errs = {k: [] for k in grads}
+    absd = {k: [] for k in grads}
+    t1 = time.perf_counter()
+    for i, c in enumerate(pick):
+        Jpm = {k: [] for k in grads}
+        for sgn in (1.0, -1.0):
+            e2, g2 = e.copy(), g.copy()
+            e2[c] *= np.exp(sgn * eps)
+            g2[c] *= np.exp(sgn * eps)
+            ctx.set_stiffness(e2, g2)
+            xp, stp = ctx.solve(f, precision=sh.FP64, tol=1e-10, max_iter=20000, check=False)
+            worst = max(worst, stp.rel_residual)
+            # compliance in its stationary form 2 f.x - x.K x: its error is second order in the
+            # solve's error (f.x alone would carry the 1e-10 residual into a difference of 1e-4)
+            Jpm["compliance"].append(2.0 * float((f * xp).sum()) - float((xp * ctx.apply(xp)).sum()))
+            Jpm["softmax_stress"].append(ctx.objective(sh.SOFTMAX_STRESS, beta))
+        for k in grads:
+            fd = (Jpm[k][0] - Jpm[k][1]) / (2 * eps)
+            dJ = grads[k][1]
+            scale = max(abs(fd), 1e-3 * np.abs(dJ).max())
+            errs[k].append(abs(dJ[c] - fd) / scale)
+            absd[k].append((abs(dJ[c] - fd), abs(fd)))
+        if True:
+            log(f"[bench] fd cell {i + 1}/{len(pick)}: max rel err compliance {max(errs['compliance']):.2e}, "
+                f"soft-max {max(errs['softmax_stress']):.2e}")
+    ctx.set_stiffness(e, g)
+    out = {
+        "check": f"adjoint gradients vs central differences, {n}^3 structure, {len(pick)} random free cells, "
+                 f"eps {eps}, FP64 solves to 1e-10, beta {beta}",
+        "worst_rel_residual": worst,
+        "free_cells": int(len(free)),
+        "max_rel_err_compliance": float(max(errs["compliance"])),
+        "max_rel_err_softmax_stress": float(max(errs["softmax_stress"])),
+        "per_cell_rel_err": "|dJ_c - fd_c| / max(|fd_c|, 1e-3 max_c |dJ_c|)",
+        "norm_rel_err_compliance": max(a for a, _ in absd["compliance"]) / max(b for _, b in absd["compliance"]),
+        "norm_rel_err_softmax_stress": max(a for a, _ in absd["softmax_stress"]) / max(b for _, b in absd["softmax_stress"]),
+        "norm_rel_err": "max_c |dJ_c - fd_c| / max_c |fd_c| over the sampled cells",
+        "bar": 1e-4,
+        "invariant_sum_dC_plus_C": float(grads["compliance"][1].sum() + grads["compliance"][0]) / grads["compliance"][0],
+        "invariant_sum_dJ_softmax": float(grads["softmax_stress"][1].sum() / np.abs(grads["softmax_stress"][1]).sum()),
+        "adjoint_s": t_adj, "fd_s": time.perf_counter() - t1,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,6 +317,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--spacing", type=float, default=8.0,
+                    help="pillar Poisson-disk spacing (8: ~32%% solid; 11.7: ~15%%, the 512^3 sparse world)")
+    ap.add_argument("--points", type=int, default=64, help="point loads on top of gravity (1024: the 512^3 world)")
     ap.add_argument("--cpu-sample", type=int, default=64, help="edge of the CPU oracle's sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-fp64", action="store_true")
@@ -251,6 +327,10 @@ def main():
                     help="instead: the batch workload, this many structures (512: the batch config)")
     ap.add_argument("--batch-cells", type=float, default=14e6,
                     help="batch: free cells per st_solve_batch call (the structures are packed in order)")
+    ap.add_argument("--fd-check", type=int, default=0, metavar="CELLS",
+                    help="instead: the gradient check, d(compliance) and d(soft-max stress)/d(log-stiffness) "
+                         "against central differences (eps 1e-4, FP64 solves to 1e-10) at this many cells "
+                         "of a --size structure (64 at 128: the adjoint config)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="end with three device copies of 1 GiB (the calibration kernel of the traffic counters)")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -267,11 +347,14 @@ def main():
     if args.batch:
         batch_bench(args)
         return
+    if args.fd_check:
+        fd_check(args)
+        return
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
 
     n = args.size
     t = time.perf_counter()
-    occ, theta = structure(n, args.seed)
+    occ, theta = structure(n, args.seed, spacing=args.spacing)
     sites, held, e, g = sh.cells_from_occ(occ, theta)
     log(f"[bench] generated {n}^3: {len(sites)} solid cells, fill {len(sites) / n ** 3:.3f} "
         f"({time.perf_counter() - t:.1f}s)")
@@ -282,7 +365,7 @@ def main():
     joined = ctx.joined_to_held()
     faces = ctx.node_faces()
     nheld = ctx.node_held()
-    loads = node_loads(occ, faces, joined, nheld, np.random.default_rng(args.seed))
+    loads = node_loads(occ, faces, joined, nheld, np.random.default_rng(args.seed), n_points=args.points)
     loads = np.ascontiguousarray(loads)
     ctx._check(ctx.L.st_set_loads(ctx.h, sh._p(loads)), "st_set_loads")
 
@@ -375,8 +458,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 CG + f32 V-cycle",
         "data": "synthetic",
-        "config": {"workload": f"one {n}^3 pillar/slab structure, ~30% solid, lognormal stiffness sigma 0.5, "
-                               f"gravity + 64 point loads; step = classify + mixed solve + soft-max stress adjoint",
+        "config": {"workload": f"one {n}^3 pillar/slab structure, {len(sites) / n ** 3:.0%} solid (pillar spacing "
+                               f"{args.spacing:g}), lognormal stiffness sigma 0.5, gravity + {args.points} point "
+                               f"loads; step = classify + mixed solve + soft-max stress adjoint",
                    "lattice": n, "free_cells": int(fwd[-1].cells), "free_nodes": int(fwd[-1].dof_nodes),
                    "dof": int(6 * fwd[-1].dof_nodes), "tol": 1e-8},
         "forward_iters": int(fwd[-1].iters), "adjoint_iters": int(adj[-1].iters),

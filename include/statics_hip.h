@@ -125,6 +125,12 @@ int st_set_gone(StCtx* ctx, const uint8_t* gone);
 /* Replaces the per-cell alive masks (settle's broken beams, statics/src/bodies.rs:525-533). */
 int st_set_beams(StCtx* ctx, const uint16_t* beams_alive);
 
+/* New stiffnesses e, g for the uploaded cells (upload order; held cells ignored),
+ * keeping the lattice: what an optimisation loop over per-cell stiffness changes
+ * between solves (the reference rebuilds its Lattice, statics/src/lattice.rs:124-163).
+ * The next solve rebuilds the hierarchy. */
+int st_set_stiffness(StCtx* ctx, const double* e, const double* g);
+
 /* ---- operator --------------------------------------------------------- */
 /* Replaces: Level::apply at level 0 (statics/src/bodies.rs:193-206): y = K x
  * over the live beams, 6 doubles per node in device order; rows of held and
@@ -169,14 +175,15 @@ int st_kernel_timing(StCtx* ctx, int64_t* launches, double* total_ms, double* by
 int st_bond_forces(StCtx* ctx, const double* x, double* forces_out);
 
 /* ---- gradients (no counterpart in the reference) ---------------------- */
-typedef enum { ST_COMPLIANCE = 0, ST_SOFTMAX_STRESS = 1 } StObjective;
+typedef enum { ST_COMPLIANCE = 0, ST_SOFTMAX_STRESS = 1, ST_VALUE_ONLY = 256 } StObjective;
 /* After st_solve or st_solve_resident: J and dJ/dtheta per cell (upload order; 0 for held
  * cells), where theta is the cell's log-stiffness (e and g both scale by
  * exp(theta)). ST_COMPLIANCE: J = f.x. ST_SOFTMAX_STRESS: J = (1/beta) log
  * sum_b exp(beta sigma_b) over live bonds b, sigma_b = (f_b . n_b)/a_b, the
  * bond's normal stress. The adjoint solve reuses the operator and rebuilds
  * the hierarchy; it runs with the precision and tolerance of opts.
- * dJ_dtheta_out may be NULL (the gradient then stays on the device). */
+ * dJ_dtheta_out may be NULL (the gradient then stays on the device).
+ * objective | ST_VALUE_ONLY: J alone, no adjoint solve, no gradient. */
 int st_backward(StCtx* ctx, int objective, double beta, const StSolveOpts* opts, double* J_out,
                 double* dJ_dtheta_out, StStats* stats);
 

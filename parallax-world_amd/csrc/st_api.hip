@@ -478,6 +478,8 @@ int st_bond_forces(StCtx* c, const double* x, double* out) {
 int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, double* J_out, double* dJ,
                 StStats* stats) {
   std::vector<StStats> st_out;
+  const bool value_only = (objective & ST_VALUE_ONLY) != 0;
+  objective &= ~ST_VALUE_ONLY;
   int rc = guard(c, [&] {
     need_upload(c);
     check_opts(opts);
@@ -523,6 +525,14 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
       ST_HIP(hipStreamSynchronize(s));
       mzh[1] = host_sum(hc);
       J = m + std::log(mzh[1]) / beta;
+      if (value_only) {  // the objective only: no adjoint solve
+        xd.release();
+        lam.release();
+        cellv.release();
+        mz.release();
+        if (J_out) *J_out = J;
+        return;
+      }
       ST_HIP(hipMemcpyAsync(mz.p, mzh, 2 * sizeof(double), hipMemcpyHostToDevice, s));
       // adjoint: K lambda = dJ/dx, solved with the same operator and a rebuilt hierarchy
       c->f64.ensure(bytes);
@@ -536,20 +546,21 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
       solve_all(*c, *opts, st_out);
       ST_HIP(hipMemcpyAsync(lam.p, c->x64.p, bytes, hipMemcpyDeviceToDevice, s));
     }
-    if (ncl)
+    if (ncl && !value_only)
       k_sensitivity<<<nblk(ncl, 128), 128, 0, s>>>(ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(), g.ce64.as<double>(),
                                               g.cg64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
                                               xd.as<double>(), lam.as<double>(), soft ? 1 : 0, beta, mz.as<double>(),
                                               cellv.as<double>());
     ST_HIP(hipGetLastError());
-    if (ncl && dJ) ST_HIP(hipMemcpyAsync(hc.data(), cellv.p, ncl * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (ncl && dJ && !value_only)
+      ST_HIP(hipMemcpyAsync(hc.data(), cellv.p, ncl * sizeof(double), hipMemcpyDeviceToHost, s));
     ST_HIP(hipStreamSynchronize(s));
     xd.release();
     lam.release();
     cellv.release();
     mz.release();
     if (J_out) *J_out = J;
-    if (dJ) {
+    if (dJ && !value_only) {
       const auto& coi = c->host.cell_of_input;
       for (size_t i = 0; i < coi.size(); i++) dJ[i] = coi[i] < 0 ? 0.0 : hc[coi[i]];
     }

@@ -351,6 +351,35 @@ int st_set_beams(StCtx* c, const uint16_t* beams) {
   });
 }
 
+int st_set_stiffness(StCtx* c, const double* e, const double* g) {
+  return guard(c, [&] {
+    need_upload(c);
+    if (!e || !g) throw StError{ST_BAD_INPUT, "set_stiffness: e and g are required"};
+    ST_HIP(hipSetDevice(c->device));
+    HostGeo& h = c->host;
+    const auto& coi = h.cell_of_input;
+    for (size_t i = 0; i < coi.size(); i++) {
+      const int64_t k = coi[i];
+      if (k < 0) continue;  // held
+      if (!(e[i] > 0) || !(g[i] > 0) || !std::isfinite(e[i]) || !std::isfinite(g[i]))
+        throw StError{ST_BAD_INPUT, "set_stiffness: a free cell has a non-positive or non-finite stiffness"};
+      h.ce[k] = e[i];
+      h.cg[k] = g[i];
+    }
+    DevGeo& d = c->geo;
+    up(d.ce64, h.ce, c->stream);
+    up(d.cg64, h.cg, c->stream);
+    if (d.ncells) {
+      const int nb = (int)((d.ncells + 255) / 256);
+      k_to_float<<<nb, 256, 0, c->stream>>>(d.ncells, d.ce64.as<double>(), d.ce32.as<float>());
+      k_to_float<<<nb, 256, 0, c->stream>>>(d.ncells, d.cg64.as<double>(), d.cg32.as<float>());
+      ST_HIP(hipGetLastError());
+    }
+    ST_HIP(hipStreamSynchronize(c->stream));
+    c->have_solution = false;
+  });
+}
+
 int st_kernel_timing_enable(StCtx* c, int which, int enable) {
   return guard(c, [&] {
     if (which != 0 && which != 1) throw StError{ST_BAD_INPUT, "kernel timing: which is 0 or 1"};

@@ -1657,6 +1657,7 @@ struct Solver {
   std::vector<DBuf*> lseg_d;
   DBuf seg_tmp;
   int64_t ndof = 0;
+  bool mixed = false;  // the solve's precision: the FP32 levels need no FP64 copies afterwards
   explicit Solver(Ctx& cc) : c(cc), st(cc.stream), ns(cc.geo.nstruct), g(cc.geo) {}
   ~Solver() {
     sbrick_d.release();
@@ -1959,6 +1960,8 @@ struct Solver {
 #undef ST_A1
       ST_HIP(hipGetLastError());
     });
+    // the explicit fine prolongator only serves this product (the V-cycle applies it matrix-free)
+    for (DBuf* b : {&c.p0row, &c.p0col, &c.p0val, &c.p0t_ptr, &c.p0t_blk, &c.p0blk_row}) b->release();
   }
 
   // Above level 1: P of the level below (smoothed, or rigid when smooth is false), then this level = P^T A P.
@@ -2179,6 +2182,9 @@ struct Solver {
       if (c.levels.back().nblocks == c.levels.back().n) break;  // block diagonal: exact top
     }
     label_tmp.release();
+    if (mixed)  // the FP32 levels' FP64 copies served the Galerkin products and the step estimates only
+      for (size_t li = 0; li < c.levels.size() && (int)li < c.f64_from; li++)
+        for (DBuf* b : {&c.levels[li].a64, &c.levels[li].dinv64, &c.levels[li].pval64}) b->release();
   }
 
   // --------------------------------------------------------------- V-cycle
@@ -2612,6 +2618,7 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   c.f64_from = 2;
   if (const char* e = std::getenv("STATICS_HIP_F64FROM")) c.f64_from = std::max(0, std::atoi(e));
   autotune_apply(c);
+  S.mixed = o.precision == ST_MIXED;
   S.init_struct_arrays();
   S.setup_fine();
   S.setup_hierarchy();

@@ -16,12 +16,13 @@ STATUS = {0: "OK", 1: "BAD_INPUT", 2: "NO_CONVERGENCE", 3: "NOT_SPD", 4: "NAN", 
           7: "NO_STRUCTURE"}
 FP64, MIXED = 0, 1
 COMPLIANCE, SOFTMAX_STRESS = 0, 1
+VALUE_ONLY = 256  # or-ed into the objective: J alone
 
 # The entry points include/statics_hip.h declares.
 EXPORTS = ["st_create", "st_destroy", "st_last_error", "st_upload", "st_sizes", "st_node_faces", "st_lookup_faces",
            "st_node_held", "st_joined_to_held", "st_set_gone", "st_set_beams", "st_apply", "st_solve",
            "st_set_loads", "st_solve_resident", "st_get_solution", "st_classify", "st_kernel_timing_enable",
-           "st_kernel_timing", "st_bond_forces", "st_backward", "st_solve_batch", "st_generate"]
+           "st_kernel_timing", "st_bond_forces", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness"]
 
 UNIT = 0.125  # metres to a level-0 cell (statics/src/lattice.rs:16)
 
@@ -76,6 +77,7 @@ def lib():
             "st_joined_to_held": [vp, vp, vp, ctypes.c_int],
             "st_set_gone": [vp, vp],
             "st_set_beams": [vp, vp],
+            "st_set_stiffness": [vp, vp, vp],
             "st_apply": [vp, vp, vp],
             "st_solve": [vp, vp, ctypes.POINTER(StSolveOpts), vp, ctypes.POINTER(StStats)],
             "st_bond_forces": [vp, vp, vp],
@@ -246,6 +248,20 @@ class Context:
         self._check(self.L.st_backward(self.h, objective, beta, ctypes.byref(o), ctypes.byref(J), _p(d),
                                        ctypes.byref(s)), "st_backward")
         return J.value, d, s
+
+    def objective(self, objective=COMPLIANCE, beta=20.0):
+        """J of the last solve alone (no adjoint solve)."""
+        o = StSolveOpts(FP64, 1e-8, 1, 0, 0)
+        J = ctypes.c_double()
+        self._check(self.L.st_backward(self.h, objective | VALUE_ONLY, beta, ctypes.byref(o), ctypes.byref(J),
+                                       None, None), "st_backward")
+        return J.value
+
+    def set_stiffness(self, e, g):
+        """New e, g per uploaded cell (upload order), keeping the lattice."""
+        e = np.ascontiguousarray(e, np.float64)
+        g = np.ascontiguousarray(g, np.float64)
+        self._check(self.L.st_set_stiffness(self.h, _p(e), _p(g)), "st_set_stiffness")
 
     def solve_batch(self, cell_sets, loads=None, precision=MIXED, tol=1e-8, max_iter=10000, want_x=True):
         """cell_sets: list of (sites, held, e, g). Gravity (weight 1 per free cell, -z) where loads[i] is None."""

@@ -140,6 +140,40 @@ def test_gradient_central_differences(ctx, objective):
         assert abs(dJ[c] - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(dJ).max()), (c, dJ[c], fd)
 
 
+def test_gradient_device_differences(ctx):
+    """dJ/dtheta against central differences of the device's own FP64 solves (tol 1e-11, near the
+    FP64 floor of these systems) on a larger structure than the oracle's check: stiffness changed
+    in place by st_set_stiffness."""
+    occ, theta = sh.generate(40, 40, 40, seed=9, sigma=0.5)
+    sites, held, e, g = sh.cells_from_occ(occ, theta)
+    ctx.upload(sites, held, e, g)
+    ctx.joined_to_held()
+    nh = ctx.node_held()
+    f = np.zeros((ctx.nodes, 6))
+    f[:, 2] = -1.0
+    f[nh.astype(bool)] = 0
+    _, st = ctx.solve(f, precision=sh.FP64, tol=1e-11, max_iter=20000, check=False)
+    assert st.rel_residual <= 1e-10
+    J, dC, _ = ctx.backward(sh.COMPLIANCE, precision=sh.FP64, tol=1e-11, max_iter=20000)
+    S, dS, _ = ctx.backward(sh.SOFTMAX_STRESS, beta=20.0, precision=sh.FP64, tol=1e-11, max_iter=20000)
+    assert abs(ctx.objective(sh.COMPLIANCE) - J) <= 1e-12 * abs(J)
+    assert abs(ctx.objective(sh.SOFTMAX_STRESS, 20.0) - S) <= 1e-12 * abs(S)
+    rng = np.random.default_rng(9)
+    eps = 1e-4
+    for c in rng.choice(np.flatnonzero(held == 0), size=6, replace=False):
+        Jp = []
+        for sgn in (1.0, -1.0):
+            ctx.set_stiffness(e * np.where(np.arange(len(e)) == c, np.exp(sgn * eps), 1.0),
+                              g * np.where(np.arange(len(g)) == c, np.exp(sgn * eps), 1.0))
+            _, st = ctx.solve(f, precision=sh.FP64, tol=1e-11, max_iter=20000, check=False)
+            assert st.rel_residual <= 1e-10
+            Jp.append((ctx.objective(sh.COMPLIANCE), ctx.objective(sh.SOFTMAX_STRESS, 20.0)))
+        for k, d in ((0, dC), (1, dS)):
+            fd = (Jp[0][k] - Jp[1][k]) / (2 * eps)
+            assert abs(d[c] - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(d).max()), (k, c, d[c], fd)
+    ctx.set_stiffness(e, g)
+
+
 def test_gradient_scaling_identities(ctx):
     """theta -> theta + t scales x by exp(-t): sum dC/dtheta = -C and sum dJ_softmax/dtheta = 0."""
     P = generated(ctx, 48, 5)
