@@ -53,6 +53,22 @@ void copy_geo_to_device(Ctx& c) {
   d.nnodes = (int64_t)h.nflags.size();
   d.nstruct = (int32_t)h.structs.size();
   c.tuned = false;
+  {
+    // bricks in two classes by node count (Morton order kept inside each)
+    constexpr uint32_t kSmall = 640;
+    std::vector<uint32_t> small, big;
+    d.class_nodes[0] = d.class_nodes[1] = 0;
+    for (size_t b = 0; b + 1 < h.nbase.size(); b++) {
+      const uint32_t nn = h.nbase[b + 1] - h.nbase[b];
+      const int k = nn <= kSmall ? 0 : 1;
+      (k ? big : small).push_back((uint32_t)b);
+      d.class_nodes[k] = std::max(d.class_nodes[k], nn);
+    }
+    d.nclass[0] = (int64_t)small.size();
+    d.nclass[1] = (int64_t)big.size();
+    small.insert(small.end(), big.begin(), big.end());
+    up(d.blist, small, s);
+  }
   up(d.cbase, h.cbase, s);
   up(d.nbase, h.nbase, s);
   up(d.bstruct, h.bstruct, s);
@@ -145,14 +161,26 @@ void launch_apply(Ctx& c, const T* x, const Epi& epi) {
       k_apply<T, Epi, 768, 256><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
       break;
     case 1:
-      k_apply<T, Epi, 512, 192><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
-      break;
-    case 2:
-      k_apply<T, Epi, 384, 128><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
-      break;
-    default:
       k_apply<T, Epi, 0, 0><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
       break;
+    default: {
+      static size_t attr_bytes = 0;  // one per instantiation
+      const DevGeo& d = c.geo;
+      const size_t need = (size_t)std::max(d.class_nodes[0], d.class_nodes[1]) * 6 * sizeof(T);
+      if (need > attr_bytes) {
+        ST_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_cells<T, Epi>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+        attr_bytes = need;
+      }
+      const uint32_t* bl = d.blist.as<uint32_t>();
+      for (int k = 0; k < 2; k++) {
+        if (d.nclass[k])
+          k_apply_cells<T, Epi><<<(unsigned)d.nclass[k], kBlock, (size_t)d.class_nodes[k] * 6 * sizeof(T), c.stream>>>(
+              g, bl, x, epi);
+        bl += d.nclass[k];
+      }
+      break;
+    }
   }
   ST_HIP(hipGetLastError());
   if (timed) {
@@ -202,7 +230,7 @@ int tune_one(Ctx& c, T* x, T* y) {
     float ms = 0;
     ST_HIP(hipEventElapsedTime(&ms, e0, e1));
     if (std::getenv("STATICS_HIP_VERBOSE"))
-      std::fprintf(stderr, "statics_hip: schedule %d (tile %d, xcd %d) %s: %.3f ms\n", k, kSchedules[k].tile,
+      std::fprintf(stderr, "statics_hip: schedule %d (kind %d, xcd %d) %s: %.3f ms\n", k, kSchedules[k].tile,
                    kSchedules[k].xcd, sizeof(T) == 8 ? "fp64" : "fp32", ms / 3);
     if (ms < best_ms) {
       best_ms = ms;
@@ -409,6 +437,7 @@ int st_apply(StCtx* c, const double* x, double* y) {
     if (!x || !y) throw StError{ST_BAD_INPUT, "apply: x or y is NULL"};
     ST_HIP(hipSetDevice(c->device));
     const size_t bytes = (size_t)c->geo.nnodes * 6 * sizeof(double);
+    autotune_apply(*c);
     c->t64.ensure(bytes);
     c->u64.ensure(bytes);
     ST_HIP(hipMemcpyAsync(c->t64.p, x, bytes, hipMemcpyHostToDevice, c->stream));

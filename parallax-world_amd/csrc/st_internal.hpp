@@ -101,18 +101,21 @@ struct DevGeo {
   int64_t nbricks = 0, ncells = 0, nnodes = 0;
   int32_t nstruct = 0;
   DBuf cbase, nbase, bstruct;
+  DBuf blist;                 // bricks by node-count class (the cell-by-cell operator's launches)
+  int64_t nclass[2] = {0, 0};  // bricks per class
+  uint32_t class_nodes[2] = {0, 0};  // largest node count in each class
   DBuf ce64, cg64, ce32, cg32, cmask_user, cmask, cfaces, ccoord, cstruct;
   DBuf nflags, ncell, nkind, nkey;
   DBuf bgeo;  // BeamGeo per structure
 };
 
-// Schedules of the fine operator: LDS tile capacity (0: 768 nodes / 256 cells,
-// 1: 512 / 192, 2: 384 / 128, 3: none) and whether bricks are dealt to XCDs in
-// contiguous runs.
+// Schedules of the fine operator: the kind (0: node by node with the brick's
+// 768 nodes / 256 cells staged in LDS, 1: node by node from memory, 2: cell by cell
+// with LDS rows per node) and whether bricks are dealt to XCDs in contiguous runs.
 struct ApplySchedule {
   int tile, xcd;
 };
-constexpr ApplySchedule kSchedules[] = {{0, 1}, {1, 1}, {2, 1}, {3, 1}, {0, 0}, {1, 0}, {2, 0}, {3, 0}};
+constexpr ApplySchedule kSchedules[] = {{0, 1}, {1, 1}, {2, 1}, {0, 0}, {1, 0}, {2, 0}};
 constexpr int kNumSchedules = (int)(sizeof(kSchedules) / sizeof(kSchedules[0]));
 
 // One coarse level (aggregates of the level below, 6 DOF each), block-sparse.

@@ -258,6 +258,92 @@ __global__ __launch_bounds__(kBlock, ST_APPLY_WAVES) void k_apply(GeoD<T> g, con
   }
 }
 
+// The operator cell by cell (the other schedule): one workgroup per brick, one
+// thread per free cell applies the cell's 15 beams once and adds the loads of each
+// face into the LDS row of that face's node when the node is the brick's. A node
+// gets at most two adds, one from each of its cells, and 0 + a + b is the same in
+// either order, so the rows are exact and deterministic. A node on the brick's lower
+// faces whose other cell lies in another brick takes that cell's five beams ending on
+// it (the halo). Then the epilogue runs per node. blist: the bricks of this launch
+// (bricks are launched in classes by node count, each with LDS for its largest).
+template <class T, class Epi>
+__global__ __launch_bounds__(kBlock) void k_apply_cells(GeoD<T> g, const uint32_t* __restrict__ blist,
+                                                        const T* __restrict__ x, Epi epi) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* yl = reinterpret_cast<T*>(smem_raw);
+  __shared__ double red[kBlock / 64];
+  const uint32_t b = blist[g.xcd ? xcd_brick(blockIdx.x, gridDim.x) : blockIdx.x];
+  const uint32_t s = g.bstruct[b];
+  if (epi.skip(s)) {
+    if (Epi::kReduce && threadIdx.x == 0) epi.partial[b] = 0.0;
+    return;
+  }
+  const uint32_t n0 = g.nbase[b], nn = g.nbase[b + 1] - n0;
+  const uint32_t c0 = g.cbase[b], c1 = g.cbase[b + 1];
+  for (uint32_t i = threadIdx.x; i < 6 * nn; i += blockDim.x) yl[i] = T(0);
+  const BeamGeo G = g.bgeo[s];
+  __syncthreads();
+  for (uint32_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+    const uint2* fp = reinterpret_cast<const uint2*>(g.cfaces + 6 * (size_t)c);
+    const uint2 f01 = fp[0], f23 = fp[1], f45 = fp[2];
+    const uint32_t fi[6] = {f01.x, f01.y, f23.x, f23.y, f45.x, f45.y};
+    T xv[6][6], yv[6][6];
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+      if (fi[f] & kNotFree) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) xv[f][k] = T(0);
+      } else {
+        load6<T>(x + 6 * (size_t)fi[f], xv[f]);
+      }
+#pragma unroll
+      for (int k = 0; k < 6; k++) yv[f][k] = T(0);
+    }
+    const T e = g.ce[c], gg = g.cg[c];
+    cell_apply<T>(cell_coef<T>(G, 0, e, gg), cell_coef<T>(G, 1, e, gg), (uint32_t)g.cmask[c], xv, yv);
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+      if (fi[f] & kNotFree) continue;
+      const uint32_t l = fi[f] - n0;
+      if (l >= nn) continue;  // kept by the next brick, which takes this cell as its halo
+#pragma unroll
+      for (int k = 0; k < 6; k++) atomicAdd(&yl[6 * l + k], yv[f][k]);
+    }
+  }
+  Tile<T> none{};  // the halo reads memory
+  for (uint32_t l = threadIdx.x; l < nn; l += blockDim.x) {
+    const uint32_t v = n0 + l;
+    const uint8_t kind = g.nkind[v];
+    if ((kind & 1) || (g.nflags[v] & (kHeld | kGone))) continue;
+    const int32_t cm = g.ncell[2 * (size_t)v];
+    if (cm < 0 || ((uint32_t)cm >= c0 && (uint32_t)cm < c1)) continue;
+    T y6[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    const int o = kind >> 1;
+    if (o == 0)
+      cell_face_row<T, 1>(g, none, G, x, (uint32_t)cm, y6);
+    else if (o == 1)
+      cell_face_row<T, 3>(g, none, G, x, (uint32_t)cm, y6);
+    else
+      cell_face_row<T, 5>(g, none, G, x, (uint32_t)cm, y6);
+#pragma unroll
+    for (int k = 0; k < 6; k++) atomicAdd(&yl[6 * l + k], y6[k]);
+  }
+  __syncthreads();
+  double part = 0.0;
+  for (uint32_t l = threadIdx.x; l < nn; l += blockDim.x) {
+    const uint32_t v = n0 + l;
+    T y[6], xo[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) y[k] = yl[6 * l + k];
+    load6<T>(x + 6 * (size_t)v, xo);
+    part += epi.node(v, y, s, xo);
+  }
+  if (Epi::kReduce) {
+    const double tot = block_sum(part, red);
+    if (threadIdx.x == 0) epi.partial[b] = tot;
+  }
+}
+
 // ---------------------------------------------------------------- epilogues
 struct EpiBase {
   const uint8_t* done = nullptr;  // per structure: skip when set

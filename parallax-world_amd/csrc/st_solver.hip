@@ -1453,29 +1453,63 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
   __shared__ int slot[kH];
   __shared__ int sorted[kSlots];
   __shared__ int ncount;
-  __shared__ double acc[36 * kSlots];
+  __shared__ double acc[pass ? 36 * kSlots : 1];
+  // the column's nodes, a chunk at a time: each node's P0 block, its ten beams' other
+  // ends (kDeadBeam where the beam is absent or dead), its orientation, cells' e and g
+  constexpr int kV = 128;
+  constexpr uint32_t kDeadBeam = 0xfffffffeu;
+  __shared__ int s_blk[kV], s_v[kV], s_sidx[kV];
+  __shared__ uint8_t s_o[kV];
+  __shared__ uint32_t s_wf[10][kV];
+  __shared__ double s_e[2][kV], s_g[2][kV];
   if ((int64_t)blockIdx.x >= n1) return;
   const int I = rows ? rows[blockIdx.x] : (int)blockIdx.x;
   for (int h = threadIdx.x; h < kH; h += blockDim.x) keys[h] = -1;
   __syncthreads();
   if (threadIdx.x == 0) hinsert<kH>(keys, I);
   const int k0 = pt_ptr[I], nv = pt_ptr[I + 1] - k0;
-  const int items = nv * 10;
-  {
-    for (int it = threadIdx.x; it < items; it += blockDim.x) {
-      const int vi = it % nv, j = it / nv;
-      const int v = pblk_row[pt_blk[k0 + vi]];
-      if (j == 0)
+  auto load_chunk = [&](int base, int nvc) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < nvc; t += blockDim.x) {
+      const int blk = pt_blk[k0 + base + t];
+      const int v = pblk_row[blk];
+      const int o = nkind[v] >> 1;
+      s_blk[t] = blk;
+      s_v[t] = v;
+      s_o[t] = (uint8_t)o;
+      int sidx = 0;
+      for (int side = 0; side < 2; side++) {
+        const int c = ncell[2 * v + side];
+        const int f = side == 0 ? 2 * o + 1 : 2 * o;
+        const uint32_t mask = c >= 0 ? (uint32_t)cmask[c] : 0u;
+        for (int jj = 0; jj < 5; jj++) {
+          const int q = face_beam_q(f, jj);
+          s_wf[5 * side + jj][t] =
+              (mask & (1u << q)) ? cfaces[6 * c + (kPairA[q] == f ? kPairB[q] : kPairA[q])] : kDeadBeam;
+        }
+        if (c >= 0) {
+          s_e[side][t] = ce[c];
+          s_g[side][t] = cg[c];
+          sidx = (int)cstruct[c];
+        }
+      }
+      s_sidx[t] = sidx;
+    }
+    __syncthreads();
+  };
+  // columns of the row: the P0 rows of the column's nodes and of their beams' other ends
+  for (int base = 0; base < nv; base += kV) {
+    const int nvc = min(kV, nv - base);
+    load_chunk(base, nvc);
+    for (int it = threadIdx.x; it < nvc * 10; it += blockDim.x) {
+      const int vi = it % nvc, j = it / nvc;
+      if (j == 0) {
+        const int v = s_v[vi];
         for (int k2 = prow[v]; k2 < prow[v + 1]; k2++) hinsert<kH>(keys, pcol[k2]);
-      const int side = j / 5, o = nkind[v] >> 1;
-      const int c = ncell[2 * v + side];
-      if (c < 0) continue;
-      const int f = side == 0 ? 2 * o + 1 : 2 * o;
-      const int q = face_beam_q(f, j % 5);
-      if (!(cmask[c] & (1u << q))) continue;
-      const uint32_t wf = cfaces[6 * c + (kPairA[q] == f ? kPairB[q] : kPairA[q])];
-      if (!(wf & kNotFree))
-        for (int k2 = prow[wf]; k2 < prow[wf + 1]; k2++) hinsert<kH>(keys, pcol[k2]);
+      }
+      const uint32_t wf = s_wf[j][vi];
+      if (wf == kDeadBeam || (wf & kNotFree)) continue;
+      for (int k2 = prow[wf]; k2 < prow[wf + 1]; k2++) hinsert<kH>(keys, pcol[k2]);
     }
   }
   const int m = row_columns<kH>(pass, I, keys, slot, sorted, kSlots, &ncount);
@@ -1488,46 +1522,46 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
     return;
   }
   for (int x = threadIdx.x; x < 36 * m; x += blockDim.x) acc[x] = 0.0;
-  __syncthreads();
   // products split by output row r: thread task (row r, node, beam)
-  for (int it = threadIdx.x; it < items * 6; it += blockDim.x) {
-    const int r = it % 6, rest = it / 6;
-    const int vi = rest % nv, j = rest / nv;
-    const int blk = pt_blk[k0 + vi];
-    const int v = pblk_row[blk];
-    const int side = j / 5, o = nkind[v] >> 1;
-    const int c = ncell[2 * v + side];
-    if (c < 0) continue;
-    const int f = side == 0 ? 2 * o + 1 : 2 * o;
-    const int q = face_beam_q(f, j % 5);
-    if (!(cmask[c] & (1u << q))) continue;
-    const int sv = kPairA[q] == f ? -1 : 1;
-    const uint32_t wf = cfaces[6 * c + (kPairA[q] == f ? kPairB[q] : kPairA[q])];
-    const bool wfree = !(wf & kNotFree);
-    const double* PvI = pval + 36 * (int64_t)blk;
-    double pc[6];  // column r of P0[v, I]
-    for (int i = 0; i < 6; i++) pc[i] = PvI[6 * i + r];
-    const BeamGeo G = bgeo[cstruct[c]];
-    double n[3];
-    int cls;
-    pair_n(q, G, n, cls);
-    const double e = ce[c], g = cg[c];
-    for (int sd = 0; sd < (wfree ? 2 : 1); sd++) {
-      double qr[6];  // row r of P0[v,I]^T K_b
-      beam_row(pc, sv, sd == 0 ? sv : -sv, n, G.hd[cls], G.K4[cls], G.R2[cls], e, g, qr);
-      const int row = sd == 0 ? v : (int)wf;
-      // start each task's column loop at a different column, so that the tasks of a
-      // wave (mostly nodes of one aggregate) do not add into the same block at once
-      const int k2a = prow[row], len = prow[row + 1] - k2a;
-      for (int kk = 0; kk < len; kk++) {
-        const int k2 = k2a + (kk + rest) % len;
-        const int sl = hfind<kH>(keys, slot, pcol[k2]);
-        const double* Pj = pval + 36 * (int64_t)k2;
-        double* out = acc + 36 * sl + 6 * r;
-        double o6[6] = {0, 0, 0, 0, 0, 0};
-        for (int k = 0; k < 6; k++)
-          for (int cc = 0; cc < 6; cc++) o6[cc] += qr[k] * Pj[6 * k + cc];
-        for (int cc = 0; cc < 6; cc++) atomicAdd(&out[cc], o6[cc]);
+  for (int base = 0; base < nv; base += kV) {
+    const int nvc = min(kV, nv - base);
+    load_chunk(base, nvc);
+    for (int it = threadIdx.x; it < nvc * 60; it += blockDim.x) {
+      const int r = it % 6, rest = it / 6;
+      const int vi = rest % nvc, j = rest / nvc;
+      const uint32_t wf = s_wf[j][vi];
+      if (wf == kDeadBeam) continue;
+      const int side = j / 5, o = s_o[vi];
+      const int f = side == 0 ? 2 * o + 1 : 2 * o;
+      const int q = face_beam_q(f, j % 5);
+      const int sv = kPairA[q] == f ? -1 : 1;
+      const bool wfree = !(wf & kNotFree);
+      const int v = s_v[vi];
+      const double* PvI = pval + 36 * (int64_t)s_blk[vi];
+      double pc[6];  // column r of P0[v, I]
+      for (int i = 0; i < 6; i++) pc[i] = PvI[6 * i + r];
+      const BeamGeo G = bgeo[s_sidx[vi]];
+      double n[3];
+      int cls;
+      pair_n(q, G, n, cls);
+      const double e = s_e[side][vi], g = s_g[side][vi];
+      for (int sd = 0; sd < (wfree ? 2 : 1); sd++) {
+        double qr[6];  // row r of P0[v,I]^T K_b
+        beam_row(pc, sv, sd == 0 ? sv : -sv, n, G.hd[cls], G.K4[cls], G.R2[cls], e, g, qr);
+        const int row = sd == 0 ? v : (int)wf;
+        // start each task's column loop at a different column, so that the tasks of a
+        // wave (mostly nodes of one aggregate) do not add into the same block at once
+        const int k2a = prow[row], len = prow[row + 1] - k2a;
+        for (int kk = 0; kk < len; kk++) {
+          const int k2 = k2a + (kk + rest) % len;
+          const int sl = hfind<kH>(keys, slot, pcol[k2]);
+          const double* Pj = pval + 36 * (int64_t)k2;
+          double* out = acc + 36 * sl + 6 * r;
+          double o6[6] = {0, 0, 0, 0, 0, 0};
+          for (int k = 0; k < 6; k++)
+            for (int cc = 0; cc < 6; cc++) o6[cc] += qr[k] * Pj[6 * k + cc];
+          for (int cc = 0; cc < 6; cc++) atomicAdd(&out[cc], o6[cc]);
+        }
       }
     }
   }
@@ -2540,7 +2574,7 @@ void release_levels(Ctx& c) {
 
 void release_all(Ctx& c) {
   DevGeo& d = c.geo;
-  for (DBuf* b : {&d.cbase, &d.nbase, &d.bstruct, &d.ce64, &d.cg64, &d.ce32, &d.cg32,
+  for (DBuf* b : {&d.cbase, &d.nbase, &d.bstruct, &d.blist, &d.ce64, &d.cg64, &d.ce32, &d.cg32,
                   &d.cmask_user, &d.cmask, &d.cfaces, &d.ccoord, &d.cstruct, &d.nflags, &d.ncell, &d.nkind, &d.nkey,
                   &d.bgeo})
     b->release();

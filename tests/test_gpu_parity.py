@@ -44,6 +44,27 @@ def test_apply_parity_generated(ctx, seed):
     assert rel(y_dev, y_or) < 1e-13
 
 
+def test_apply_parity_every_schedule(ctx, monkeypatch):
+    """Each schedule of the operator (node by node with and without LDS tiles, cell by cell;
+    XCD brick mapping on and off) against the oracle, with dead beams and a gone set."""
+    for k in range(6):
+        monkeypatch.setenv("STATICS_HIP_SCHEDULE", str(k))
+        P = generated(ctx, 20, 4)
+        rng = np.random.default_rng(k)
+        beam_alive = rng.random(P.L.n_bonds // 4) > 0.1
+        alive = np.repeat(beam_alive, 4).astype(np.uint8)
+        jo = P.L.joined_to_held(alive)
+        jg = P.from_dev(ctx.joined_to_held(P.beam_masks(alive.astype(bool))))
+        assert (jo == jg).all()
+        x = rng.normal(size=(P.L.n_nodes, 6))
+        y = P.from_dev(ctx.apply(P.to_dev(x)))
+        assert rel(y, P.L.apply(x, alive=alive, gone=(jo == 0).astype(np.uint8))) < 1e-13, k
+        f = gravity_z(P.L)
+        x_dev, st = ctx.solve(P.to_dev(f), precision=sh.MIXED, tol=1e-10)
+        x_or, _ = P.L.motions(f, alive=alive, gone=(jo == 0).astype(np.uint8), tol=1e-13, max_iter=5000)
+        assert st.status == 0 and rel(P.from_dev(x_dev), x_or) < 1e-8, k
+
+
 def test_apply_parity_dead_beams_and_gone(ctx):
     P = generated(ctx, 16, 7)
     rng = np.random.default_rng(7)
