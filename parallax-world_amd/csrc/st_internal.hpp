@@ -141,6 +141,11 @@ struct Level {
   DBuf step_d;               // per structure (double)
   // V-cycle vectors at this level
   DBuf r64, x64, t64, u64, v64, r32, x32, t32, u32, v32;
+  std::vector<int64_t> seg;  // per structure: its aggregates [seg[s], seg[s+1])
+  // dense top: per structure the inverse of its (6 n_s)^2 operator, FP64, symmetric
+  bool dense = false;
+  int32_t dense_mmax = 0;
+  DBuf dense_a, dense_off, dense_seg;  // matrices; per structure offset (int64, ns+1); seg (int64)
 };
 
 struct Ctx {
@@ -171,6 +176,7 @@ struct Ctx {
   int gamma_to = 1000;   // ... up to this index
   int f64_from = 2;      // first level (0: level 1) kept in FP64 by the mixed V-cycle
   bool graphs = false;   // replay each CG iteration's tail as a captured graph
+  bool dense_top = true; // invert a small coarse level densely and drop the levels above
   int apply_sched[2] = {0, 0};  // operator schedule per precision (FP32, FP64), see kSchedules
   bool tuned = false;           // apply_sched chosen for the uploaded structures
   std::vector<Level> levels;        // coarse levels 1..

@@ -835,6 +835,97 @@ __global__ void k_axpy(int64_t n, T a, const T* x, T* y) {
   if (i < n) y[i] = a == T(0) ? x[i] : a * y[i] + x[i];
 }
 
+// ---- dense top: a coarse level's operator per structure, inverted by Gauss-Jordan
+// (SPD: no pivoting), symmetrised, and applied as a dense product.
+// Scatter the level's blocks into each structure's dense matrix (zeroed before).
+__global__ void k_dense_fill(int64_t n, const uint32_t* ast, const int64_t* seg, const int64_t* off,
+                             const int* rowptr, const int* col, const double* A, double* D) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  const uint32_t s = ast[I];
+  const int64_t a0 = seg[s], m = 6 * (seg[s + 1] - a0);
+  double* Ds = D + off[s];
+  for (int b = rowptr[I]; b < rowptr[I + 1]; b++) {
+    const int64_t J = col[b];
+    for (int r = 0; r < 6; r++)
+      for (int cc = 0; cc < 6; cc++) Ds[(6 * (I - a0) + r) * m + 6 * (J - a0) + cc] = A[36 * (int64_t)b + 6 * r + cc];
+  }
+}
+// The Gauss-Jordan kernels run over the structures still active at step k (those with
+// more than k rows): ord lists the structures by size, largest first; pre1/pre2 are the
+// running sums of m and m^2 in that order, so thread t finds its structure by bisection.
+__device__ __forceinline__ int seg_of(const int64_t* pre, int n, int64_t t) {
+  int lo = 0, hi = n - 1;  // the last i with pre[i] <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+// Step k, first half: save column k and row k (row k with a 1 at k), zero column k.
+__global__ void k_gj_save(int k, int nact, const int* ord, const int64_t* pre1, const int64_t* seg, const int64_t* off,
+                          double* D, double* colk, double* rowk, int mmax) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= pre1[nact]) return;
+  const int i = seg_of(pre1, nact, t);
+  const int s = ord[i];
+  const int64_t m = 6 * (seg[s + 1] - seg[s]), r = t - pre1[i];
+  double* Ds = D + off[s];
+  colk[(int64_t)s * mmax + r] = Ds[r * m + k];
+  rowk[(int64_t)s * mmax + r] = r == k ? 1.0 : Ds[(int64_t)k * m + r];
+  if (r != k) Ds[r * m + k] = 0.0;
+}
+// Step k, second half: row k /= pivot; every other row -= its column-k entry times row k.
+__global__ void k_gj_update(int k, int nact, const int* ord, const int64_t* pre2, const int64_t* seg,
+                            const int64_t* off, double* D, const double* colk, const double* rowk, int mmax) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= pre2[nact]) return;
+  const int i0 = seg_of(pre2, nact, t);
+  const int s = ord[i0];
+  const int64_t m = 6 * (seg[s + 1] - seg[s]), e = t - pre2[i0];
+  const int64_t i = e / m, j = e % m;
+  const double p = 1.0 / colk[(int64_t)s * mmax + k];
+  double* Ds = D + off[s];
+  const double rk = rowk[(int64_t)s * mmax + j] * p;
+  if (i == k)
+    Ds[e] = rk;
+  else
+    Ds[e] -= colk[(int64_t)s * mmax + i] * rk;
+}
+// (D + D^T) / 2 in place, every structure
+__global__ void k_dense_sym(int nact, const int* ord, const int64_t* pre2, const int64_t* seg, const int64_t* off,
+                            double* D) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= pre2[nact]) return;
+  const int i0 = seg_of(pre2, nact, t);
+  const int s = ord[i0];
+  const int64_t m = 6 * (seg[s + 1] - seg[s]), e = t - pre2[i0];
+  const int64_t i = e / m, j = e % m;
+  if (j <= i) return;
+  double* Ds = D + off[s];
+  const double a = 0.5 * (Ds[i * m + j] + Ds[j * m + i]);
+  Ds[i * m + j] = a;
+  Ds[j * m + i] = a;
+}
+// x = D r per structure (D symmetric: row i read as column i, coalesced across threads)
+template <class T>
+__global__ void k_dense_apply(const int64_t* seg, const int64_t* off, const double* D, const uint8_t* done,
+                              const T* r, T* x) {
+  const int s = blockIdx.y;
+  if (done && done[s]) return;
+  const int64_t a0 = seg[s], m = 6 * (seg[s + 1] - a0);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const double* Ds = D + off[s];
+  const T* rs = r + 6 * a0;
+  double acc = 0.0;
+  for (int64_t j = 0; j < m; j++) acc += Ds[j * m + i] * (double)rs[j];
+  x[6 * a0 + i] = (T)acc;
+}
+
 // Coarse block rows are worked by groups of 8 lanes: lane i < 6 of a group computes
 // component i of its row, reading row i of every 6x6 block (the group reads each
 // block as one contiguous 144-byte run); lanes 6 and 7 idle.
@@ -2140,6 +2231,7 @@ struct Solver {
         break;
       }
       lv.n = cnt;
+      lv.seg = seg;
       lv.agg_struct.ensure(cnt * sizeof(uint32_t));
       k_agg_struct<<<nblk(cnt), 256, 0, st>>>(cnt, lv.rep.as<int>(), g.ncell.as<int32_t>(), g.cstruct.as<uint32_t>(),
                                               lv.agg_struct.as<uint32_t>());
@@ -2216,9 +2308,91 @@ struct Solver {
       if (c.levels.back().nblocks == c.levels.back().n) break;  // block diagonal: exact top
     }
     label_tmp.release();
+    if (c.dense_top) make_dense_top();
     if (mixed)  // the FP32 levels' FP64 copies served the Galerkin products and the step estimates only
       for (size_t li = 0; li < c.levels.size() && (int)li < c.f64_from; li++)
         for (DBuf* b : {&c.levels[li].a64, &c.levels[li].dinv64, &c.levels[li].pval64}) b->release();
+  }
+
+  // The first level from index 2 up where every structure has at most kDenseMax
+  // aggregates becomes the top: its operator is inverted densely per structure, and
+  // the levels above it are dropped (visiting them cost hundreds of tiny launches an
+  // iteration in the gamma-cycles).
+  void make_dense_top() {
+    constexpr int64_t kDenseMax = 256;
+    int ld = -1;
+    for (size_t li = 2; li + 1 < c.levels.size(); li++) {
+      const auto& sg = c.levels[li].seg;
+      int64_t mx = 0;
+      for (int s2 = 0; s2 < ns; s2++) mx = std::max(mx, sg[s2 + 1] - sg[s2]);
+      if (mx <= kDenseMax) {
+        ld = (int)li;
+        break;
+      }
+    }
+    if (ld < 0) return;
+    for (size_t li = ld + 1; li < c.levels.size(); li++) {
+      Level& l2 = c.levels[li];
+      for (DBuf* b : {&l2.rowptr, &l2.col, &l2.a64, &l2.a32, &l2.dinv64, &l2.dinv32, &l2.centre, &l2.parent, &l2.rep,
+                      &l2.key, &l2.kid_ptr, &l2.kids, &l2.agg_struct, &l2.step_d, &l2.r64, &l2.x64, &l2.t64, &l2.u64,
+                      &l2.v64, &l2.r32, &l2.x32, &l2.t32, &l2.u32, &l2.v32, &l2.prow, &l2.pcol, &l2.pval64,
+                      &l2.pval32, &l2.pt_ptr, &l2.pt_blk, &l2.pblk_row})
+        b->release();
+    }
+    c.levels.resize(ld + 1);
+    Level& lv = c.levels[ld];
+    std::vector<int64_t> off(ns + 1, 0);
+    int64_t mmax = 0;
+    for (int s2 = 0; s2 < ns; s2++) {
+      const int64_t m = 6 * (lv.seg[s2 + 1] - lv.seg[s2]);
+      off[s2 + 1] = off[s2] + m * m;
+      mmax = std::max(mmax, m);
+    }
+    lv.dense = true;
+    lv.dense_mmax = (int32_t)mmax;
+    lv.dense_a.ensure(std::max<int64_t>(off[ns], 1) * sizeof(double));
+    lv.dense_off.ensure((ns + 1) * sizeof(int64_t));
+    lv.dense_seg.ensure((ns + 1) * sizeof(int64_t));
+    ST_HIP(hipMemcpyAsync(lv.dense_off.p, off.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    ST_HIP(hipMemcpyAsync(lv.dense_seg.p, lv.seg.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    ST_HIP(hipMemsetAsync(lv.dense_a.p, 0, std::max<int64_t>(off[ns], 1) * sizeof(double), st));
+    if (!mmax) return;
+    const int64_t* sgd = lv.dense_seg.as<int64_t>();
+    const int64_t* ofd = lv.dense_off.as<int64_t>();
+    double* D = lv.dense_a.as<double>();
+    k_dense_fill<<<nblk(lv.n), 256, 0, st>>>(lv.n, lv.agg_struct.as<uint32_t>(), sgd, ofd, lv.rowptr.as<int>(),
+                                             lv.col.as<int>(), lv.a64.as<double>(), D);
+    // structures by size, largest first, and running sums of m and m^2 in that order
+    std::vector<int> ord(ns);
+    for (int s2 = 0; s2 < ns; s2++) ord[s2] = s2;
+    auto msz = [&](int s2) { return 6 * (lv.seg[s2 + 1] - lv.seg[s2]); };
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return msz(a) > msz(b); });
+    std::vector<int64_t> pre1(ns + 1, 0), pre2(ns + 1, 0);
+    for (int i = 0; i < ns; i++) {
+      pre1[i + 1] = pre1[i] + msz(ord[i]);
+      pre2[i + 1] = pre2[i] + msz(ord[i]) * msz(ord[i]);
+    }
+    DBuf colk, rowk, ordd, pre1d, pre2d;
+    colk.ensure((size_t)ns * mmax * sizeof(double));
+    rowk.ensure((size_t)ns * mmax * sizeof(double));
+    ordd.ensure(ns * sizeof(int));
+    pre1d.ensure((ns + 1) * sizeof(int64_t));
+    pre2d.ensure((ns + 1) * sizeof(int64_t));
+    ST_HIP(hipMemcpyAsync(ordd.p, ord.data(), ns * sizeof(int), hipMemcpyHostToDevice, st));
+    ST_HIP(hipMemcpyAsync(pre1d.p, pre1.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    ST_HIP(hipMemcpyAsync(pre2d.p, pre2.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    int nact = ns;
+    for (int k = 0; k < mmax; k++) {
+      while (nact > 0 && msz(ord[nact - 1]) <= k) nact--;  // structures with more than k rows
+      k_gj_save<<<nblk(pre1[nact]), 256, 0, st>>>(k, nact, ordd.as<int>(), pre1d.as<int64_t>(), sgd, ofd, D,
+                                                  colk.as<double>(), rowk.as<double>(), (int)mmax);
+      k_gj_update<<<nblk(pre2[nact]), 256, 0, st>>>(k, nact, ordd.as<int>(), pre2d.as<int64_t>(), sgd, ofd, D,
+                                                    colk.as<double>(), rowk.as<double>(), (int)mmax);
+    }
+    k_dense_sym<<<nblk(pre2[ns]), 256, 0, st>>>(ns, ordd.as<int>(), pre2d.as<int64_t>(), sgd, ofd, D);
+    ST_HIP(hipGetLastError());
+    ST_HIP(hipStreamSynchronize(st));
+    for (DBuf* b2 : {&colk, &rowk, &ordd, &pre1d, &pre2d}) b2->release();
   }
 
   // --------------------------------------------------------------- V-cycle
@@ -2266,6 +2440,12 @@ struct Solver {
     const int64_t n = lv.n;
     const uint32_t* ast = lv.agg_struct.as<uint32_t>();
     const bool top = li + 1 == c.levels.size();
+    if (lv.dense) {  // the dense top: exact
+      const dim3 gd((unsigned)((lv.dense_mmax + 63) / 64), (unsigned)ns);
+      k_dense_apply<T><<<gd, 64, 0, st>>>(lv.dense_seg.as<int64_t>(), lv.dense_off.as<int64_t>(),
+                                          lv.dense_a.as<double>(), done, r, x);
+      return;
+    }
     if (top && lv.nblocks == lv.n) {  // block diagonal: exact
       k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), nullptr, r, x);
       return;
@@ -2566,7 +2746,8 @@ void release_levels(Ctx& c) {
   for (auto& lv : c.levels)
     for (DBuf* b : {&lv.rowptr, &lv.col, &lv.a64, &lv.a32, &lv.dinv64, &lv.dinv32, &lv.centre, &lv.parent, &lv.rep,
                     &lv.key, &lv.kid_ptr, &lv.kids, &lv.agg_struct, &lv.step_d, &lv.r64, &lv.x64, &lv.t64, &lv.u64,
-                    &lv.v64, &lv.r32, &lv.x32, &lv.t32, &lv.u32, &lv.v32, &lv.prow, &lv.pcol, &lv.pval64, &lv.pval32, &lv.pt_ptr, &lv.pt_blk,
+                    &lv.v64, &lv.r32, &lv.x32, &lv.t32, &lv.u32, &lv.v32, &lv.dense_a, &lv.dense_off,
+                    &lv.dense_seg, &lv.prow, &lv.pcol, &lv.pval64, &lv.pval32, &lv.pt_ptr, &lv.pt_blk,
                     &lv.pblk_row})
       b->release();
   c.levels.clear();
@@ -2635,6 +2816,7 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   // graph replay of the iteration tail: opt-in (STATICS_HIP_GRAPHS=1); at 256^3 the launch
   // queue stays full without it, and rocprofv3's kernel tracing does not follow captures
   c.graphs = std::getenv("STATICS_HIP_GRAPHS") != nullptr;
+  c.dense_top = std::getenv("STATICS_HIP_NODENSE") == nullptr;
   c.nu1 = 1;
   if (const char* e = std::getenv("STATICS_HIP_NU1")) c.nu1 = std::max(1, std::atoi(e));
   c.nu_c = 1;
