@@ -910,20 +910,31 @@ __global__ void k_dense_sym(int nact, const int* ord, const int64_t* pre2, const
   Ds[i * m + j] = a;
   Ds[j * m + i] = a;
 }
-// x = D r per structure (D symmetric: row i read as column i, coalesced across threads)
+// x = D r per structure (D symmetric: row i read as column i). A 1024-thread block takes
+// 32 rows; its 32 thread groups split the sum over j and add their partial sums in a
+// fixed order.
 template <class T>
-__global__ void k_dense_apply(const int64_t* seg, const int64_t* off, const double* D, const uint8_t* done,
-                              const T* r, T* x) {
+__global__ __launch_bounds__(1024) void k_dense_apply(const int64_t* seg, const int64_t* off, const double* D,
+                                                      const uint8_t* done, const T* r, T* x) {
+  __shared__ double part[32][33];
   const int s = blockIdx.y;
   if (done && done[s]) return;
   const int64_t a0 = seg[s], m = 6 * (seg[s + 1] - a0);
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
+  const int il = threadIdx.x & 31, jg = threadIdx.x >> 5;
+  const int64_t i = (int64_t)blockIdx.x * 32 + il;
+  if ((int64_t)blockIdx.x * 32 >= m) return;
   const double* Ds = D + off[s];
   const T* rs = r + 6 * a0;
   double acc = 0.0;
-  for (int64_t j = 0; j < m; j++) acc += Ds[j * m + i] * (double)rs[j];
-  x[6 * a0 + i] = (T)acc;
+  if (i < m)
+    for (int64_t j = jg; j < m; j += 32) acc += Ds[j * m + i] * (double)rs[j];
+  part[jg][il] = acc;
+  __syncthreads();
+  if (jg == 0 && i < m) {
+    double t = 0.0;
+    for (int k = 0; k < 32; k++) t += part[k][il];
+    x[6 * a0 + i] = (T)t;
+  }
 }
 
 // Coarse block rows are worked by groups of 8 lanes: lane i < 6 of a group computes
@@ -1477,7 +1488,9 @@ __global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, c
         const int k2a = prow[j], len = prow[j + 1] - k2a;
         for (int kk = 0; kk < len; kk++) {
           const int k2 = k2a + (kk + it) % len;
-          const int sl = hfind<kH>(keys, slot, pcol[k2]);
+          const int J = pcol[k2];
+          if (J < I) continue;  // the lower blocks are the transposes of other rows' upper ones
+          const int sl = hfind<kH>(keys, slot, J);
           const double* Pj = pval + 36 * (int64_t)k2;
           double* out = acc + 36 * sl + 6 * r;
           double o6[6] = {0, 0, 0, 0, 0, 0};
@@ -1516,6 +1529,30 @@ __global__ void k_row_classes(int64_t n, const int* cnt, int* lists, int* ctr) {
   if (threadIdx.x < 4) base[threadIdx.x] = local[threadIdx.x] ? atomicAdd(&ctr[threadIdx.x], local[threadIdx.x]) : 0;
   __syncthreads();
   if (k >= 0) lists[k * n + base[k] + pos] = (int)i;
+}
+
+// The blocks below the diagonal of a symmetric operator from the transposes of those
+// above it: (I, J) with J < I is (J, I)^T, found in row J by bisection (its columns
+// are ascending after the diagonal).
+__global__ void k_fill_lower(int64_t n, const int* rowptr, const int* col, double* A) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  for (int p = rowptr[I] + 1; p < rowptr[I + 1]; p++) {
+    const int J = col[p];
+    if (J >= I) break;
+    int lo = rowptr[J] + 1, hi = rowptr[J + 1] - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (col[mid] < (int)I)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    const double* src = A + 36 * (int64_t)lo;
+    double* dst = A + 36 * (int64_t)p;
+    for (int r = 0; r < 6; r++)
+      for (int c2 = 0; c2 < 6; c2++) dst[6 * r + c2] = src[6 * c2 + r];
+  }
 }
 
 // The five beams touching face f (pair indices, ascending).
@@ -1645,7 +1682,9 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
         const int k2a = prow[row], len = prow[row + 1] - k2a;
         for (int kk = 0; kk < len; kk++) {
           const int k2 = k2a + (kk + rest) % len;
-          const int sl = hfind<kH>(keys, slot, pcol[k2]);
+          const int J = pcol[k2];
+          if (J < I) continue;  // the lower blocks are the transposes of other rows' upper ones
+          const int sl = hfind<kH>(keys, slot, J);
           const double* Pj = pval + 36 * (int64_t)k2;
           double* out = acc + 36 * sl + 6 * r;
           double o6[6] = {0, 0, 0, 0, 0, 0};
@@ -2040,9 +2079,11 @@ struct Solver {
     ST_HIP(hipMemcpyAsync(count, ctr.p, sizeof(count), hipMemcpyDeviceToHost, st));
     ST_HIP(hipStreamSynchronize(st));
     finish_rows(lv, cnt, [&](int pass) {
-      if (pass == 1)
+      if (pass == 1) {
         for (int k = 0; k < 4; k++)
           if (count[k]) launch(1, kRowClass[k], cnt.as<int>(), lists.as<int>() + k * n, (int64_t)count[k]);
+        k_fill_lower<<<nblk(n), 256, 0, st>>>(n, lv.rowptr.as<int>(), lv.col.as<int>(), lv.a64.as<double>());
+      }
     });
     cnt.release();
     lists.release();
@@ -2441,8 +2482,8 @@ struct Solver {
     const uint32_t* ast = lv.agg_struct.as<uint32_t>();
     const bool top = li + 1 == c.levels.size();
     if (lv.dense) {  // the dense top: exact
-      const dim3 gd((unsigned)((lv.dense_mmax + 63) / 64), (unsigned)ns);
-      k_dense_apply<T><<<gd, 64, 0, st>>>(lv.dense_seg.as<int64_t>(), lv.dense_off.as<int64_t>(),
+      const dim3 gd((unsigned)((lv.dense_mmax + 31) / 32), (unsigned)ns);
+      k_dense_apply<T><<<gd, 1024, 0, st>>>(lv.dense_seg.as<int64_t>(), lv.dense_off.as<int64_t>(),
                                           lv.dense_a.as<double>(), done, r, x);
       return;
     }
