@@ -30,29 +30,39 @@ sys.path.insert(0, os.path.join(HERE, "parallax-world_amd"))
 import statics_hip as sh  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
-ROOFLINE_KERNEL = "k_apply<double, st::EpiDot<double> >"
 COPY_BYTES = 1 << 30
-TRAFFIC_FILE = os.path.join(HERE, "profiles", "k_apply_epidot_f64_256_traffic.txt")
+# The roofline kernel is the CG's K p with the fused p.Ap in FP64 (epilogue EpiDot<double>),
+# in whichever schedule the autotuner chose: k_apply (node-centric, one dispatch per
+# application) or k_apply_cells (cell-centric, one dispatch per brick class).
+ROOFLINE_EPILOGUE = "EpiDot<double>"
+
+
+def traffic_file(kernel):
+    return os.path.join(HERE, "profiles", f"{kernel}_epidot_f64_256_traffic.txt")
 
 
 def _pmc_per_dispatch(csv_path, counter):
     """Mean counter value per dispatch for the roofline kernel and for the three calibration
-    copies (the last three copy dispatches of the run)."""
+    copies (the last three copy dispatches of the run), and the roofline kernel's name."""
     import csv
-    apply, copies = [], []
+    apply, copies, names = [], [], set()
     with open(csv_path) as f:
         for row in csv.DictReader(f):
             if row.get("Counter_Name") != counter:
                 continue
             name = row["Kernel_Name"]
             v = float(row["Counter_Value"])
-            if "EpiDot<double>" in name:
+            if ROOFLINE_EPILOGUE in name:
                 apply.append(v)
+                names.add("k_apply_cells" if "k_apply_cells" in name else "k_apply")
             elif "copyBuffer" in name:
                 copies.append((int(row["Dispatch_Id"]), v))
+    if len(names) > 1:
+        raise RuntimeError(f"{csv_path}: the run used more than one schedule for the roofline kernel: {names}")
     copies = [v for _, v in sorted(copies)[-3:]]
     vals = {"apply": apply, "copy": copies}
-    return {k: (sum(v) / len(v) if v else None) for k, v in vals.items()}, {k: len(v) for k, v in vals.items()}
+    mean = {k: (sum(v) / len(v) if v else None) for k, v in vals.items()}
+    return mean, {k: len(v) for k, v in vals.items()}, (names.pop() if names else None)
 
 
 def copy_calibration():
@@ -73,22 +83,31 @@ def copy_calibration():
     hip.hipFree(b)
 
 
-def traffic_summary(fetch_csv, write_csv):
+def traffic_summary(fetch_csv, write_csv, schedule):
     """FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the roofline kernel, each scaled by the
-    ratio of a 1 GiB device copy's known bytes to what the same counter reads for it."""
-    fe, nf = _pmc_per_dispatch(fetch_csv, "FETCH_SIZE")
-    wr, nw = _pmc_per_dispatch(write_csv, "WRITE_SIZE")
+    ratio of a 1 GiB device copy's known bytes to what the same counter reads for it, and
+    summed over the dispatches of one application of the operator."""
+    fe, nf, kname = _pmc_per_dispatch(fetch_csv, "FETCH_SIZE")
+    wr, nw, kname_w = _pmc_per_dispatch(write_csv, "WRITE_SIZE")
+    if kname is None or kname != kname_w:
+        raise RuntimeError(f"roofline kernel not found in both runs ({kname}, {kname_w})")
+    per_app = 2 if kname == "k_apply_cells" else 1
     kf = COPY_BYTES / (fe["copy"] * 1024.0)
     kw = COPY_BYTES / (wr["copy"] * 1024.0)
-    fetch_b = fe["apply"] * 1024.0 * kf
-    write_b = wr["apply"] * 1024.0 * kw
+    fetch_b = fe["apply"] * 1024.0 * kf * per_app
+    write_b = wr["apply"] * 1024.0 * kw * per_app
     lines = [
-        f"# HBM traffic per launch of {ROOFLINE_KERNEL}, 256^3 bench workload",
+        f"# HBM traffic per application of {kname}<double, EpiDot<double>> (the CG's K p with the fused p.Ap),",
+        f"# 256^3 bench workload; {per_app} dispatch(es) per application",
         "# commands (GPU, from the repository root, cd /tmp && export TMPDIR=/tmp first):",
-        "#   rocprofv3 --pmc FETCH_SIZE --kernel-include-regex 'EpiDot<double>|copyBuffer' -d <dir> -o fetch "
-        "--output-format csv -- python3 bench.py --steps 1 --warmup 0 --cpu-sample 0 --no-fp64 --copy-calib",
-        "#   rocprofv3 --pmc WRITE_SIZE (same otherwise, -o write)",
-        f"#   python3 bench.py --traffic <fetch_counter_collection.csv> <write_counter_collection.csv>",
+        f"#   STATICS_HIP_SCHEDULE={schedule} rocprofv3 --pmc FETCH_SIZE --kernel-include-regex "
+        "'EpiDot<double>|copyBuffer' -d <dir> -o fetch --output-format csv -- python3 bench.py --steps 1 "
+        "--warmup 0 --cpu-sample 0 --no-fp64 --copy-calib",
+        "#   the same with --pmc WRITE_SIZE and -o write",
+        f"#   python3 bench.py --traffic <fetch_counter_collection.csv> <write_counter_collection.csv> "
+        f"--traffic-schedule {schedule}",
+        f"kernel: {kname}",
+        f"dispatches_per_application: {per_app}",
         f"fetch_kib_per_dispatch_raw: {fe['apply']:.1f} (dispatches {nf['apply']})",
         f"write_kib_per_dispatch_raw: {wr['apply']:.1f} (dispatches {nw['apply']})",
         f"copy_1GiB_fetch_kib_raw: {fe['copy']:.1f} (dispatches {nf['copy']}) -> fetch scale {kf:.4f}",
@@ -97,16 +116,18 @@ def traffic_summary(fetch_csv, write_csv):
         f"write_bytes_per_launch: {write_b:.6e}",
         f"traffic_bytes_per_launch: {fetch_b + write_b:.6e}",
     ]
-    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
-    with open(TRAFFIC_FILE, "w") as f:
+    path = traffic_file(kname)
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
 
-def measured_traffic():
-    """HBM bytes per launch of the roofline kernel from the committed counter summary, or None."""
+def measured_traffic(kernel):
+    """HBM bytes per application of the roofline kernel from the committed counter summary
+    for that kernel, or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(traffic_file(kernel)) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":")[1])
@@ -336,10 +357,12 @@ def main():
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="offline: HBM bytes per launch of the roofline kernel from two rocprofv3 --pmc "
                          "counter_collection.csv files (FETCH_SIZE, WRITE_SIZE) of a --copy-calib run; "
-                         "writes " + TRAFFIC_FILE)
+                         "writes " + traffic_file("<kernel>"))
+    ap.add_argument("--traffic-schedule", type=int, default=None,
+                    help="with --traffic: the STATICS_HIP_SCHEDULE the counter runs were pinned to")
     args = ap.parse_args()
     if args.traffic:
-        traffic_summary(*args.traffic)
+        traffic_summary(*args.traffic, schedule=args.traffic_schedule)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is single-GPU by design", file=sys.stderr)
@@ -413,6 +436,7 @@ def main():
     nl, tms, bpl = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
     ctx._check(ctx.L.st_kernel_timing(ctx.h, ctypes.byref(nl), ctypes.byref(tms), ctypes.byref(bpl)), "timing")
     ctx._check(ctx.L.st_kernel_timing_enable(ctx.h, 0, 0), "timing")
+    kidx, kname, kdesc, kdisp = ctx.apply_schedule(fp64=True)
     kernel_ms = tms.value / max(nl.value, 1)
     achieved = bpl.value / (kernel_ms * 1e-3) / 1e9 if nl.value else 0.0
     ms_per_step = 1e3 * sum(times) / len(times)
@@ -469,10 +493,12 @@ def main():
         "fp64_forward_solves_per_s": (1e3 / fp64["ms"]) if fp64 else None,
         "fp64_forward_iters": fp64["iters"] if fp64 else None,
         "mixed_vs_fp64_forward": (fp64["ms"] / mixed_fwd_ms) if fp64 else None,
-        "roofline": {"bound": "hbm", "kernel": ROOFLINE_KERNEL + " (the CG's K p with the fused p.Ap, FP64)",
+        "roofline": {"bound": "hbm",
+                     "kernel": f"{kname}<double, EpiDot<double>> (the CG's K p with the fused p.Ap, FP64; "
+                               f"{kdesc}; {kdisp} dispatch(es) per launch)", "schedule": kidx,
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": measured_traffic(), "launches": int(nl.value), "avg_ms": kernel_ms,
-                     "algorithmic_bytes_per_launch": bpl.value},
+                     "traffic": measured_traffic(kname), "launches": int(nl.value), "avg_ms": kernel_ms,
+                     "dispatches_per_launch": kdisp, "algorithmic_bytes_per_launch": bpl.value},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -83,7 +83,7 @@ typedef struct {
   int32_t refreshes;      /* FP64 true-residual recomputations */
   int32_t status;         /* StStatus of the solve */
   double rel_residual;    /* final ||f - K x|| / ||f||, recomputed in FP64 */
-  double setup_ms;        /* classification, numbering, hierarchy */
+  double setup_ms;        /* numbering, smoother, hierarchy (bookkeeping only when kept) */
   double solve_ms;        /* the iterations */
   int64_t nodes;          /* face nodes (held and gone included) */
   int64_t dof_nodes;      /* free nodes: not held, not gone */
@@ -140,7 +140,12 @@ int st_apply(StCtx* ctx, const double* x, double* y);
 /* ---- solve ------------------------------------------------------------ */
 /* Replaces: bodies::motions (statics/src/bodies.rs:429-468). loads: 6 doubles
  * per node in device order (force, moment), read for free nodes only. x_out:
- * 6 doubles per node; 0 for held and gone nodes. */
+ * 6 doubles per node; 0 for held and gone nodes.
+ * The multigrid hierarchy built by a solve is kept for the next solve of the same
+ * operator in the same precision mode (st_solve, st_solve_resident, st_backward's
+ * adjoint): st_upload, st_set_beams, st_set_gone, st_joined_to_held, st_classify
+ * and st_set_stiffness drop it, and the solve after them rebuilds it. A solve on a
+ * kept hierarchy reports only its per-solve bookkeeping as setup_ms. */
 int st_solve(StCtx* ctx, const double* loads, const StSolveOpts* opts, double* x_out, StStats* stats);
 
 /* The same solve in three steps, for callers that keep the loads and the
@@ -167,6 +172,16 @@ int st_classify(StCtx* ctx, int64_t* joined_count);
  * six face indices per free cell). */
 int st_kernel_timing_enable(StCtx* ctx, int which, int enable);
 int st_kernel_timing(StCtx* ctx, int64_t* launches, double* total_ms, double* bytes_per_launch);
+
+/* No counterpart in the reference: the schedule the autotuner chose for the fine
+ * operator in FP64 (fp64 != 0) or FP32: its index (the value of the
+ * STATICS_HIP_SCHEDULE environment variable that forces it), the name of its
+ * kernel and a short description (static strings), and the number of kernel
+ * dispatches one application takes (the cell-centric schedule launches its
+ * small- and large-brick classes separately). Tunes first if the upload is not
+ * tuned yet. Any output pointer may be NULL. */
+int st_apply_schedule(StCtx* ctx, int fp64, int* index, const char** kernel, const char** description,
+                      int* dispatches);
 
 /* Replaces: bodies::moved (statics/src/bodies.rs:420-423) for every live bond:
  * per cell (in upload order) 15 beams x 4 bonds x 3 doubles, the force on

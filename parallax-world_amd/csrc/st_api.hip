@@ -534,7 +534,8 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
         return;
       }
       ST_HIP(hipMemcpyAsync(mz.p, mzh, 2 * sizeof(double), hipMemcpyHostToDevice, s));
-      // adjoint: K lambda = dJ/dx, solved with the same operator and a rebuilt hierarchy
+      // adjoint: K lambda = dJ/dx, solved with the same operator (and the forward solve's
+      // hierarchy, when its precision mode is the same)
       c->f64.ensure(bytes);
       c->x64.ensure(bytes);
       ST_HIP(hipMemsetAsync(c->f64.p, 0, bytes, s));

@@ -53,6 +53,7 @@ void copy_geo_to_device(Ctx& c) {
   d.nnodes = (int64_t)h.nflags.size();
   d.nstruct = (int32_t)h.structs.size();
   c.tuned = false;
+  c.hier_valid = false;
   {
     // bricks in two classes by node count (Morton order kept inside each)
     constexpr uint32_t kSmall = 640;
@@ -98,6 +99,7 @@ void copy_geo_to_device(Ctx& c) {
 
 // Applies the node flags (held, gone) to the cells' face lists and live masks.
 void refresh_cell_flags(Ctx& c) {
+  c.hier_valid = false;  // the operator's live beams may have changed
   DevGeo& d = c.geo;
   if (!d.ncells) return;
   k_cell_flags<<<(unsigned)((d.ncells + 255) / 256), 256, 0, c.stream>>>(d.ncells, d.cfaces.as<uint32_t>(), d.nflags.as<uint8_t>(),
@@ -405,6 +407,7 @@ int st_set_stiffness(StCtx* c, const double* e, const double* g) {
     }
     ST_HIP(hipStreamSynchronize(c->stream));
     c->have_solution = false;
+    c->hier_valid = false;
   });
 }
 
@@ -428,6 +431,26 @@ int st_kernel_timing(StCtx* c, int64_t* launches, double* total_ms, double* byte
     if (launches) *launches = c->timing_launches;
     if (total_ms) *total_ms = c->timing_ms_done;
     if (bytes) *bytes = c->timing_bytes;
+  });
+}
+
+int st_apply_schedule(StCtx* c, int fp64, int* index, const char** kernel, const char** description,
+                      int* dispatches) {
+  return guard(c, [&] {
+    need_upload(c);
+    autotune_apply(*c);
+    const int sched = c->apply_sched[fp64 ? 1 : 0];
+    const ApplySchedule& k = kSchedules[sched];
+    if (index) *index = sched;
+    static const char* const names[3] = {"k_apply", "k_apply", "k_apply_cells"};
+    static const char* const what[3][2] = {
+        {"node-centric, LDS tile of 768 nodes / 256 cells, bricks in launch order",
+         "node-centric, LDS tile of 768 nodes / 256 cells, XCD brick mapping"},
+        {"node-centric, no LDS tile, bricks in launch order", "node-centric, no LDS tile, XCD brick mapping"},
+        {"cell-centric, brick rows in LDS, bricks in launch order", "cell-centric, brick rows in LDS, XCD brick mapping"}};
+    if (kernel) *kernel = names[k.tile];
+    if (description) *description = what[k.tile][k.xcd];
+    if (dispatches) *dispatches = k.tile == 2 ? (c->geo.nclass[0] > 0) + (c->geo.nclass[1] > 0) : 1;
   });
 }
 

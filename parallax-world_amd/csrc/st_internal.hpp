@@ -180,6 +180,19 @@ struct Ctx {
   int apply_sched[2] = {0, 0};  // operator schedule per precision (FP32, FP64), see kSchedules
   bool tuned = false;           // apply_sched chosen for the uploaded structures
   std::vector<Level> levels;        // coarse levels 1..
+  // The hierarchy (fine smoother and coarse levels) is kept between solves of the same
+  // operator: anything that changes the operator clears hier_valid; hier_key holds the
+  // settings it was built with.
+  struct HierKey {
+    bool mixed, smoothed, dense_top;
+    int sa_levels, f64_from;
+    bool operator==(const HierKey& o) const {
+      return mixed == o.mixed && smoothed == o.smoothed && dense_top == o.dense_top && sa_levels == o.sa_levels &&
+             f64_from == o.f64_from;
+    }
+  };
+  bool hier_valid = false;
+  HierKey hier_key{};
   // solver vectors
   DBuf f64, x64, r64, p64, ap64, z64, t64, u64, w64;
   DBuf x32, r32, p32, ap32, z32, t32, w32;

@@ -955,6 +955,33 @@ __device__ __forceinline__ void group_gather6(T a, T* all) {
   for (int j = 0; j < 6; j++) all[j] = __shfl(a, base + j, 64);
 }
 
+// Row i of block row I times x, summed over the row's blocks b0..b1 in order; four
+// blocks' loads are issued before their products, to keep more loads in flight.
+template <class T>
+__device__ __forceinline__ T block_row_dot(int b0, int b1, const int* col, const T* A, const T* x, int i) {
+  T acc = 0;
+  int b = b0;
+  for (; b + 4 <= b1; b += 4) {
+    int cb[4];
+    T xv[4][6], av[4][6];
+#pragma unroll
+    for (int k = 0; k < 4; k++) cb[k] = col[b + k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      load6<T>(x + 6 * (int64_t)cb[k], xv[k]);
+      load6<T>(A + 36 * (int64_t)(b + k) + 6 * i, av[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc += row6_dot<T>(av[k], xv[k]);
+  }
+  for (; b < b1; b++) {
+    T xv[6];
+    load6<T>(x + 6 * (int64_t)col[b], xv);
+    acc += row6_dot<T>(A + 36 * (int64_t)b + 6 * i, xv);
+  }
+  return acc;
+}
+
 // mode 0: out = r - A x; mode 1: out = x + w D^-1 (r - A x)
 template <class T>
 __global__ void k_c_resid(int mode, int64_t n, const uint32_t* ast, const uint8_t* done, const int* rowptr, const int* col,
@@ -964,15 +991,7 @@ __global__ void k_c_resid(int mode, int64_t n, const uint32_t* ast, const uint8_
   const int i = (int)(t & 7);
   const bool live = I < n && !(done && done[ast[I]]);
   T acc = 0;
-  if (live && i < 6) {
-    acc = r[6 * I + i];
-    const int b1 = rowptr[I + 1];
-    for (int b = rowptr[I]; b < b1; b++) {
-      T xv[6];
-      load6<T>(x + 6 * (int64_t)col[b], xv);
-      acc -= row6_dot<T>(A + 36 * (int64_t)b + 6 * i, xv);
-    }
-  }
+  if (live && i < 6) acc = r[6 * I + i] - block_row_dot<T>(rowptr[I], rowptr[I + 1], col, A, x, i);
   if (mode == 0) {
     if (live && i < 6) out[6 * I + i] = acc;
     return;
@@ -992,14 +1011,7 @@ __global__ void k_c_dinvA(int64_t n, const int* rowptr, const int* col, const T*
   const int64_t I = t >> 3;
   const int i = (int)(t & 7);
   T acc = 0;
-  if (I < n && i < 6) {
-    const int b1 = rowptr[I + 1];
-    for (int b = rowptr[I]; b < b1; b++) {
-      T xv[6];
-      load6<T>(x + 6 * (int64_t)col[b], xv);
-      acc += row6_dot<T>(A + 36 * (int64_t)b + 6 * i, xv);
-    }
-  }
+  if (I < n && i < 6) acc = block_row_dot<T>(rowptr[I], rowptr[I + 1], col, A, x, i);
   T all[6];
   group_gather6<T>(acc, all);
   if (I < n && i < 6) w[6 * I + i] = row6_dot<T>(dinv + 36 * I + 6 * i, all);
@@ -1718,10 +1730,31 @@ __global__ void k_restrict_p(int64_t nJ, const uint32_t* ast, const uint8_t* don
   const int a = (int)(g & 7);
   if (J >= nJ || a >= 6) return;
   if (done && done[ast[J]]) return;
+  // column a of each block of column J of P, four blocks' loads issued together
   T acc = 0;
-  for (int kk = pt_ptr[J]; kk < pt_ptr[J + 1]; kk++) {
+  int kk = pt_ptr[J];
+  const int k1 = pt_ptr[J + 1];
+  for (; kk + 4 <= k1; kk += 4) {
+    int blk[4], row[4];
+    T tv[4][6], mv[4][6];
+#pragma unroll
+    for (int q = 0; q < 4; q++) blk[q] = pt_blk[kk + q];
+#pragma unroll
+    for (int q = 0; q < 4; q++) row[q] = pblk_row[blk[q]];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      load6<T>(t + 6 * (int64_t)row[q], tv[q]);
+#pragma unroll
+      for (int b = 0; b < 6; b++) mv[q][b] = pval[36 * (int64_t)blk[q] + 6 * b + a];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int b = 0; b < 6; b++) acc += mv[q][b] * tv[q][b];
+  }
+  for (; kk < k1; kk++) {
     const int blk = pt_blk[kk];
-    const T* M = pval + 36 * (int64_t)blk + a;  // column a of the block
+    const T* M = pval + 36 * (int64_t)blk + a;
     T tv[6];
     load6<T>(t + 6 * (int64_t)pblk_row[blk], tv);
 #pragma unroll
@@ -1738,13 +1771,7 @@ __global__ void k_prolong_p(int64_t n, const uint32_t* ast, const uint8_t* done,
   const int a = (int)(g & 7);
   if (i >= n || a >= 6) return;
   if (done && done[ast[i]]) return;
-  T acc = x[6 * i + a];
-  for (int k = prow[i]; k < prow[i + 1]; k++) {
-    T yv[6];
-    load6<T>(y + 6 * (int64_t)pcol[k], yv);
-    acc += row6_dot<T>(pval + 36 * (int64_t)k + 6 * a, yv);
-  }
-  x[6 * i + a] = acc;
+  x[6 * i + a] += block_row_dot<T>(prow[i], prow[i + 1], pcol, pval, y, a);
 }
 
 // rigid prolongation level 1 -> fine, overwriting: x[v] = Sh(r_v) y[label[v]] (0 off aggregates)
@@ -2784,6 +2811,7 @@ struct Solver {
 // ============================================================ entry helpers
 
 void release_levels(Ctx& c) {
+  c.hier_valid = false;
   for (auto& lv : c.levels)
     for (DBuf* b : {&lv.rowptr, &lv.col, &lv.a64, &lv.a32, &lv.dinv64, &lv.dinv32, &lv.centre, &lv.parent, &lv.rep,
                     &lv.key, &lv.kid_ptr, &lv.kids, &lv.agg_struct, &lv.step_d, &lv.r64, &lv.x64, &lv.t64, &lv.u64,
@@ -2877,8 +2905,16 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   autotune_apply(c);
   S.mixed = o.precision == ST_MIXED;
   S.init_struct_arrays();
-  S.setup_fine();
-  S.setup_hierarchy();
+  const Ctx::HierKey key{S.mixed, c.smoothed, c.dense_top, c.sa_levels, c.f64_from};
+  const bool kept = c.hier_valid && c.hier_key == key;
+  if (kept) {
+    S.ndof = c.last_ndof;
+  } else {
+    S.setup_fine();
+    S.setup_hierarchy();
+    c.hier_valid = true;
+    c.hier_key = key;
+  }
   ST_HIP(hipEventRecord(e1, c.stream));
   if (std::getenv("STATICS_HIP_VERBOSE")) {  // the hierarchy's shape, for tuning
     size_t mfree = 0, mtotal = 0;

@@ -22,7 +22,7 @@ VALUE_ONLY = 256  # or-ed into the objective: J alone
 EXPORTS = ["st_create", "st_destroy", "st_last_error", "st_upload", "st_sizes", "st_node_faces", "st_lookup_faces",
            "st_node_held", "st_joined_to_held", "st_set_gone", "st_set_beams", "st_apply", "st_solve",
            "st_set_loads", "st_solve_resident", "st_get_solution", "st_classify", "st_kernel_timing_enable",
-           "st_kernel_timing", "st_bond_forces", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness"]
+           "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness"]
 
 UNIT = 0.125  # metres to a level-0 cell (statics/src/lattice.rs:16)
 
@@ -90,6 +90,7 @@ def lib():
             "st_classify": [vp, vp],
             "st_kernel_timing_enable": [vp, ctypes.c_int, ctypes.c_int],
             "st_kernel_timing": [vp, vp, vp, vp],
+            "st_apply_schedule": [vp, ctypes.c_int, vp, vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -262,6 +263,13 @@ class Context:
         e = np.ascontiguousarray(e, np.float64)
         g = np.ascontiguousarray(g, np.float64)
         self._check(self.L.st_set_stiffness(self.h, _p(e), _p(g)), "st_set_stiffness")
+
+    def apply_schedule(self, fp64=True):
+        """(index, kernel, description, dispatches per application) of the fine operator's tuned schedule."""
+        i, k, d, n = ctypes.c_int(), ctypes.c_char_p(), ctypes.c_char_p(), ctypes.c_int()
+        self._check(self.L.st_apply_schedule(self.h, 1 if fp64 else 0, ctypes.byref(i), ctypes.byref(k),
+                                             ctypes.byref(d), ctypes.byref(n)), "st_apply_schedule")
+        return i.value, k.value.decode(), d.value.decode(), n.value
 
     def solve_batch(self, cell_sets, loads=None, precision=MIXED, tol=1e-8, max_iter=10000, want_x=True):
         """cell_sets: list of (sites, held, e, g). Gravity (weight 1 per free cell, -z) where loads[i] is None."""

@@ -63,6 +63,42 @@ def test_apply_parity_every_schedule(ctx, monkeypatch):
         x_dev, st = ctx.solve(P.to_dev(f), precision=sh.MIXED, tol=1e-10)
         x_or, _ = P.L.motions(f, alive=alive, gone=(jo == 0).astype(np.uint8), tol=1e-13, max_iter=5000)
         assert st.status == 0 and rel(P.from_dev(x_dev), x_or) < 1e-8, k
+        for fp64 in (True, False):
+            idx, kernel, desc, disp = ctx.apply_schedule(fp64)
+            assert idx == k and kernel == ("k_apply_cells" if k % 3 == 2 else "k_apply"), (k, kernel)
+            assert ("XCD" in desc) == (k < 3) and disp in ((1, 2) if k % 3 == 2 else (1,)), (k, desc, disp)
+
+
+def test_hierarchy_kept_and_dropped(ctx):
+    """A second solve of the same operator keeps the hierarchy (same iterations and solution,
+    setup down to bookkeeping); st_set_stiffness drops it, so the solve after it matches one
+    after a fresh upload of the new stiffness; a change of precision mode rebuilds too."""
+    occ, theta = sh.generate(32, 32, 32, seed=4, sigma=0.5)
+    sites, held, e, g = sh.cells_from_occ(occ, theta)
+
+    def fresh(e_, g_):
+        ctx.upload(sites, held, e_, g_)
+        ctx.joined_to_held()
+        f = np.zeros((ctx.nodes, 6))
+        f[:, 2] = -1.0
+        f[ctx.node_held().astype(bool)] = 0
+        return f
+
+    f = fresh(e, g)
+    x1, s1 = ctx.solve(f, precision=sh.MIXED, tol=1e-10)
+    x2, s2 = ctx.solve(f, precision=sh.MIXED, tol=1e-10)
+    assert s1.status == 0 and s2.status == 0 and s2.iters == s1.iters
+    assert rel(x2, x1) < 1e-13 and s2.setup_ms < 0.5 * s1.setup_ms, (s1.setup_ms, s2.setup_ms)
+    # stiffen the upper half a hundredfold: a kept hierarchy would no longer fit the operator
+    scale = np.where(sites[:, 2] >= 16, 100.0, 1.0)
+    ctx.set_stiffness(e * scale, g * scale)
+    x3, s3 = ctx.solve(f, precision=sh.MIXED, tol=1e-10)
+    f4 = fresh(e * scale, g * scale)
+    assert (f4 == f).all()
+    x4, s4 = ctx.solve(f, precision=sh.MIXED, tol=1e-10)
+    assert s3.status == 0 and s3.iters == s4.iters and rel(x3, x4) < 1e-12, (s3.iters, s4.iters)
+    x5, s5 = ctx.solve(f, precision=sh.FP64, tol=1e-10)
+    assert s5.status == 0 and rel(x5, x4) < 1e-8 and s5.setup_ms > 2 * s2.setup_ms
 
 
 def test_apply_parity_dead_beams_and_gone(ctx):
