@@ -189,6 +189,34 @@ int st_apply_schedule(StCtx* ctx, int fp64, int* index, const char** kernel, con
  * (statics/src/lattice.rs:76-84); 0 for dead beams. x: 6 per node. */
 int st_bond_forces(StCtx* ctx, const double* x, double* forces_out);
 
+/* Replaces: bodies::used (statics/src/bodies.rs:544-546) over the bonds of
+ * bodies::moved, with Bond::used and Mat::used (statics/src/model.rs:123-129,
+ * 66-68): per cell (in upload order) 15 beams x 4 bonds, the share of the
+ * cell's strengths that the bond's force under motions x uses (> 1: the bond
+ * breaks); 0 for dead beams and held cells. strength: 3 doubles per cell, the
+ * material's ft, fc, fs (model.rs:43-58). x: 6 per node. */
+int st_bond_used(StCtx* ctx, const double* strength, const double* x, double* used_out);
+
+/* Replaces: bodies::settle (statics/src/bodies.rs:511-541) on the device. From
+ * the context's beam masks (all alive after st_upload with beams NULL, as the
+ * reference starts), rounds of: gone = !joined_to_held (bodies.rs:518-519); the
+ * solve under `loads` (the reference's forces: gravity, bodies.rs:407-409,
+ * given here as 6 doubles per node in device order); every live beam one of
+ * whose bonds has used > 1 (strength as st_bond_used) dies whole
+ * (bodies.rs:523-533); until a round breaks nothing. opts->warm_start != 0 starts
+ * each round after the first from the previous round's solution (the reference
+ * starts from 0; the converged solution is the same). The reference settles at
+ * tol 1e-6 (bodies.rs:432): rounds after breaks can leave systems whose FP64 true
+ * residual floors above 1e-8. Outputs (each may be NULL):
+ * gone_out per node (device order), beams_out the final 15-bit masks per cell
+ * (upload order; 0 for held cells), rounds_out, and stats of the last round with
+ * iters, setup_ms and solve_ms summed over the rounds. The fallen share
+ * (bodies.rs:535-540) is the mass of the gone nodes over the free mass, which
+ * the caller holds. Afterwards the context keeps the final masks and gone set,
+ * and the last round's solution. */
+int st_settle(StCtx* ctx, const double* strength, const double* loads, const StSolveOpts* opts, uint8_t* gone_out,
+              uint16_t* beams_out, int32_t* rounds_out, StStats* stats);
+
 /* ---- gradients (no counterpart in the reference) ---------------------- */
 typedef enum { ST_COMPLIANCE = 0, ST_SOFTMAX_STRESS = 1, ST_VALUE_ONLY = 256 } StObjective;
 /* After st_solve or st_solve_resident: J and dJ/dtheta per cell (upload order; 0 for held

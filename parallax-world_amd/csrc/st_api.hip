@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 #include "st_solver.hpp"
@@ -59,8 +60,45 @@ __device__ __forceinline__ void load_faces(const uint32_t* cfaces, int64_t c, co
   }
 }
 
-// Per bond force on the j side (statics/src/bodies.rs:420-423 with model.rs:115-119):
-// out[c][q][k][3], zero for dead beams.
+// The forces on the j side of beam q's four bonds (statics/src/bodies.rs:420-423 with
+// model.rs:115-119), f[k] in the reference's bond order: offsets +r e1, -r e1, +r e2, -r e2
+// (lattice.rs:76-84).
+__device__ __forceinline__ void beam_bond_forces(const BeamState& b, double e, double g, const double* xi,
+                                                 const double* xj, double (*f)[3]) {
+  double d[3], dw[3], e1[3], e2[3];
+  beam_delta(b, xi, xj, d, dw);
+  pair_frame(b.n, e1, e2);
+  for (int k = 0; k < 4; k++) {
+    const double* ek = k < 2 ? e1 : e2;
+    const double sg = (k & 1) ? -b.r : b.r;
+    const double off[3] = {sg * ek[0], sg * ek[1], sg * ek[2]};
+    const double dl[3] = {d[0] + dw[1] * off[2] - dw[2] * off[1], d[1] + dw[2] * off[0] - dw[0] * off[2],
+                          d[2] + dw[0] * off[1] - dw[1] * off[0]};
+    const double along = dl[0] * b.n[0] + dl[1] * b.n[1] + dl[2] * b.n[2];
+    for (int t = 0; t < 3; t++) f[k][t] = b.n[t] * (e * b.k * along) + (dl[t] - b.n[t] * along) * (g * b.k);
+  }
+}
+
+// Bond::used (statics/src/model.rs:123-129) with Mat::used (model.rs:66-68): the share of
+// the material's strengths (ft, fc, fs) that force f uses in a bond of area a along n.
+__device__ __forceinline__ double bond_used(const double* f, const double* n, double a, double ft, double fc,
+                                            double fs) {
+  const double ia = 1.0 / a;
+  const double t[3] = {f[0] * ia, f[1] * ia, f[2] * ia};
+  const double sigma = t[0] * n[0] + t[1] * n[1] + t[2] * n[2];
+  const double tau2 = fmax(t[0] * t[0] + t[1] * t[1] + t[2] * t[2] - sigma * sigma, 0.0);
+  const double r = sqrt(sigma * sigma / 4.0 + tau2);
+  const double s1 = sigma / 2.0 + r, s3 = sigma / 2.0 - r;
+  return fmax(fmax(s1 / ft, -s3 / fc), (s1 - s3) / (2.0 * fs));
+}
+
+// A bond's area: a quarter of its beam's section SECTION * s^2 (lattice.rs:21-25, 81-83).
+__device__ __forceinline__ double bond_area(const BeamGeo& G) {
+  const double s = 2.0 * G.hd[0];
+  return kSection * s * s / 4.0;
+}
+
+// Per bond force on the j side: out[c][q][k][3], zero for dead beams.
 __global__ void k_bond_forces(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, const double* ce,
                               const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* x,
                               double* out) {
@@ -79,18 +117,69 @@ __global__ void k_bond_forces(int64_t ncells, const uint32_t* cfaces, const uint
     }
     BeamState b;
     beam_state(G, q, b);
-    double d[3], dw[3], e1[3], e2[3];
-    beam_delta(b, xv[kPairA[q]], xv[kPairB[q]], d, dw);
-    pair_frame(b.n, e1, e2);
-    for (int k = 0; k < 4; k++) {
-      const double* ek = k < 2 ? e1 : e2;
-      const double sg = (k & 1) ? -b.r : b.r;
-      const double off[3] = {sg * ek[0], sg * ek[1], sg * ek[2]};
-      const double dl[3] = {d[0] + dw[1] * off[2] - dw[2] * off[1], d[1] + dw[2] * off[0] - dw[0] * off[2],
-                            d[2] + dw[0] * off[1] - dw[1] * off[0]};
-      const double along = dl[0] * b.n[0] + dl[1] * b.n[1] + dl[2] * b.n[2];
-      for (int t = 0; t < 3; t++) o[3 * k + t] = b.n[t] * (e * b.k * along) + (dl[t] - b.n[t] * along) * (g * b.k);
+    double f[4][3];
+    beam_bond_forces(b, e, g, xv[kPairA[q]], xv[kPairB[q]], f);
+    for (int k = 0; k < 4; k++)
+      for (int t = 0; t < 3; t++) o[3 * k + t] = f[k][t];
+  }
+}
+
+// Per bond used share (bodies.rs:544-546): out[c][q][k], zero for dead beams.
+// strength: ft, fc, fs per cell.
+__global__ void k_bond_used(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, const double* ce,
+                            const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* strength,
+                            const double* x, double* out) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  double xv[6][6];
+  load_faces(cfaces, c, x, xv);
+  const BeamGeo G = bgeo[cstruct[c]];
+  const double e = ce[c], g = cg[c], a = bond_area(G);
+  const double ft = strength[3 * c], fc = strength[3 * c + 1], fs = strength[3 * c + 2];
+  const uint32_t m = cmask[c];
+  for (int q = 0; q < 15; q++) {
+    double* o = out + ((size_t)c * 15 + q) * 4;
+    if (!(m & (1u << q))) {
+      for (int k = 0; k < 4; k++) o[k] = 0.0;
+      continue;
     }
+    BeamState b;
+    beam_state(G, q, b);
+    double f[4][3];
+    beam_bond_forces(b, e, g, xv[kPairA[q]], xv[kPairB[q]], f);
+    for (int k = 0; k < 4; k++) o[k] = bond_used(f[k], b.n, a, ft, fc, fs);
+  }
+}
+
+// One round's breaking (statics/src/bodies.rs:523-533): a live beam (alive, neither end
+// gone) one of whose four bonds uses more than its strength (used > 1) dies whole; its
+// bit is cleared in the caller's mask. Counts the beams broken.
+__global__ void k_break(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, uint16_t* cmask_user,
+                        const double* ce, const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo,
+                        const double* strength, const double* x, int* broken) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const uint32_t m = cmask[c];
+  if (!m) return;
+  double xv[6][6];
+  load_faces(cfaces, c, x, xv);
+  const BeamGeo G = bgeo[cstruct[c]];
+  const double e = ce[c], g = cg[c], a = bond_area(G);
+  const double ft = strength[3 * c], fc = strength[3 * c + 1], fs = strength[3 * c + 2];
+  uint32_t dead = 0;
+  for (int q = 0; q < 15; q++) {
+    if (!(m & (1u << q))) continue;
+    BeamState b;
+    beam_state(G, q, b);
+    double f[4][3];
+    beam_bond_forces(b, e, g, xv[kPairA[q]], xv[kPairB[q]], f);
+    bool any = false;
+    for (int k = 0; k < 4; k++) any |= bond_used(f[k], b.n, a, ft, fc, fs) > 1.0;
+    if (any) dead |= 1u << q;
+  }
+  if (dead) {
+    cmask_user[c] = (uint16_t)(cmask_user[c] & ~dead);
+    atomicAdd(broken, __popc(dead));
   }
 }
 
@@ -473,6 +562,131 @@ int st_bond_forces(StCtx* c, const double* x, double* out) {
         std::memcpy(out + 180 * i, h.data() + 180 * (size_t)coi[i], 180 * sizeof(double));
     }
   });
+}
+
+namespace {
+// ft, fc, fs per device cell from 3 per input cell (held cells have no beams and are skipped).
+void upload_strength(StCtx* c, const double* strength, DBuf& out) {
+  const auto& coi = c->host.cell_of_input;
+  const int64_t ncl = c->geo.ncells;
+  std::vector<double> h((size_t)std::max<int64_t>(ncl, 1) * 3, 1.0);
+  for (size_t i = 0; i < coi.size(); i++) {
+    const int64_t k = coi[i];
+    if (k < 0) continue;
+    for (int t = 0; t < 3; t++) {
+      const double v = strength[3 * i + t];
+      if (!(v > 0) || !std::isfinite(v))
+        throw StError{ST_BAD_INPUT, "strength: a free cell has a non-positive or non-finite ft, fc or fs"};
+      h[3 * (size_t)k + t] = v;
+    }
+  }
+  out.ensure(h.size() * sizeof(double));
+  ST_HIP(hipMemcpyAsync(out.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+}
+}  // namespace
+
+int st_bond_used(StCtx* c, const double* strength, const double* x, double* out) {
+  return guard(c, [&] {
+    need_upload(c);
+    if (!strength || !x || !out) throw StError{ST_BAD_INPUT, "bond used: strength, x or out is NULL"};
+    ST_HIP(hipSetDevice(c->device));
+    const int64_t nn = c->geo.nnodes, ncl = c->geo.ncells;
+    DBuf sd, xd, od;
+    upload_strength(c, strength, sd);
+    xd.ensure(std::max<int64_t>(nn, 1) * 6 * sizeof(double));
+    od.ensure(std::max<int64_t>(ncl, 1) * 60 * sizeof(double));
+    if (nn) ST_HIP(hipMemcpyAsync(xd.p, x, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (ncl)
+      k_bond_used<<<nblk(ncl, 128), 128, 0, c->stream>>>(ncl, c->geo.cfaces.as<uint32_t>(), c->geo.cmask.as<uint16_t>(),
+                                                    c->geo.ce64.as<double>(), c->geo.cg64.as<double>(),
+                                                    c->geo.cstruct.as<uint32_t>(), c->geo.bgeo.as<BeamGeo>(),
+                                                    sd.as<double>(), xd.as<double>(), od.as<double>());
+    ST_HIP(hipGetLastError());
+    std::vector<double> h((size_t)ncl * 60);
+    if (ncl) ST_HIP(hipMemcpyAsync(h.data(), od.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    const auto& coi = c->host.cell_of_input;
+    for (size_t i = 0; i < coi.size(); i++) {
+      if (coi[i] < 0)
+        std::memset(out + 60 * i, 0, 60 * sizeof(double));
+      else
+        std::memcpy(out + 60 * i, h.data() + 60 * (size_t)coi[i], 60 * sizeof(double));
+    }
+  });
+}
+
+int st_settle(StCtx* c, const double* strength, const double* loads, const StSolveOpts* opts, uint8_t* gone_out,
+              uint16_t* beams_out, int32_t* rounds_out, StStats* stats) {
+  std::vector<StStats> st_out;
+  int rc = guard(c, [&] {
+    need_upload(c);
+    check_opts(opts);
+    if (!strength || !loads) throw StError{ST_BAD_INPUT, "settle: strength or loads is NULL"};
+    ST_HIP(hipSetDevice(c->device));
+    DevGeo& g = c->geo;
+    const int64_t nn = g.nnodes, ncl = g.ncells;
+    DBuf sd, broken;
+    upload_strength(c, strength, sd);
+    set_loads(c, loads);
+    broken.ensure(sizeof(int));
+    StSolveOpts o = *opts;
+    StStats total{};
+    int rounds = 0;
+    for (;;) {
+      rounds++;
+      // bodies.rs:518-519: what no chain of alive beams joins to a held node is gone
+      classify_device(*c, nullptr, nullptr);
+      // bodies.rs:520: forces under the loads (later rounds may start from the last round's x)
+      o.warm_start = rounds > 1 ? opts->warm_start : 0;
+      st_out = solve_resident(c, o);
+      const int bad = worst(st_out);
+      if (bad != ST_OK) {
+        char msg[160];
+        std::snprintf(msg, sizeof msg, "settle: the solve of round %d failed after %d iterations (residual %.3g)",
+                      rounds, st_out[0].iters, st_out[0].rel_residual);
+        throw StError{bad, msg};
+      }
+      total.iters += st_out[0].iters;
+      total.setup_ms += st_out[0].setup_ms;
+      total.solve_ms += st_out[0].solve_ms;
+      // bodies.rs:523-533: beams with an overused bond break
+      ST_HIP(hipMemsetAsync(broken.p, 0, sizeof(int), c->stream));
+      if (ncl)
+        k_break<<<nblk(ncl, 128), 128, 0, c->stream>>>(ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(),
+                                                      g.cmask_user.as<uint16_t>(), g.ce64.as<double>(),
+                                                      g.cg64.as<double>(), g.cstruct.as<uint32_t>(),
+                                                      g.bgeo.as<BeamGeo>(), sd.as<double>(), c->xsol.as<double>(),
+                                                      broken.as<int>());
+      ST_HIP(hipGetLastError());
+      int nb = 0;
+      ST_HIP(hipMemcpyAsync(&nb, broken.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      ST_HIP(hipStreamSynchronize(c->stream));
+      if (!nb) break;
+      refresh_cell_flags(*c);  // the masks changed: the kept hierarchy no longer fits
+    }
+    // the host copy of the masks follows the device's
+    HostGeo& h = c->host;
+    if (!h.cmask.empty())
+      ST_HIP(hipMemcpyAsync(h.cmask.data(), g.cmask_user.p, h.cmask.size() * sizeof(uint16_t), hipMemcpyDeviceToHost,
+                            c->stream));
+    std::vector<uint8_t> fl(nn);
+    if (nn) ST_HIP(hipMemcpyAsync(fl.data(), g.nflags.p, nn, hipMemcpyDeviceToHost, c->stream));
+    ST_HIP(hipStreamSynchronize(c->stream));
+    if (gone_out)
+      for (int64_t v = 0; v < nn; v++) gone_out[v] = (fl[v] & kGone) ? 1 : 0;
+    if (beams_out) {
+      const auto& coi = h.cell_of_input;
+      for (size_t i = 0; i < coi.size(); i++) beams_out[i] = coi[i] < 0 ? 0 : h.cmask[coi[i]];
+    }
+    if (rounds_out) *rounds_out = rounds;
+    if (stats) {
+      *stats = st_out[0];
+      stats->iters = total.iters;
+      stats->setup_ms = total.setup_ms;
+      stats->solve_ms = total.solve_ms;
+    }
+  });
+  return rc;
 }
 
 int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, double* J_out, double* dJ,

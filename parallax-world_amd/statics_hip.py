@@ -22,7 +22,7 @@ VALUE_ONLY = 256  # or-ed into the objective: J alone
 EXPORTS = ["st_create", "st_destroy", "st_last_error", "st_upload", "st_sizes", "st_node_faces", "st_lookup_faces",
            "st_node_held", "st_joined_to_held", "st_set_gone", "st_set_beams", "st_apply", "st_solve",
            "st_set_loads", "st_solve_resident", "st_get_solution", "st_classify", "st_kernel_timing_enable",
-           "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness"]
+           "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_bond_used", "st_settle", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness"]
 
 UNIT = 0.125  # metres to a level-0 cell (statics/src/lattice.rs:16)
 
@@ -81,6 +81,8 @@ def lib():
             "st_apply": [vp, vp, vp],
             "st_solve": [vp, vp, ctypes.POINTER(StSolveOpts), vp, ctypes.POINTER(StStats)],
             "st_bond_forces": [vp, vp, vp],
+            "st_bond_used": [vp, vp, vp, vp],
+            "st_settle": [vp, vp, vp, ctypes.POINTER(StSolveOpts), vp, vp, vp, ctypes.POINTER(StStats)],
             "st_backward": [vp, ctypes.c_int, dbl, ctypes.POINTER(StSolveOpts), vp, vp, ctypes.POINTER(StStats)],
             "st_solve_batch": [vp, i32, vp, vp, ctypes.POINTER(StSolveOpts), vp, vp],
             "st_generate": [ctypes.POINTER(StGenParams), vp, vp],
@@ -240,6 +242,28 @@ class Context:
         out = np.zeros((len(self._keep[0]), 15, 4, 3))
         self._check(self.L.st_bond_forces(self.h, _p(x), _p(out)), "st_bond_forces")
         return out
+
+    def bond_used(self, strength, x):
+        """Per uploaded cell [15][4] used shares; strength: (cells, 3) ft, fc, fs."""
+        x = np.ascontiguousarray(x, np.float64).reshape(self.nodes, 6)
+        s = np.ascontiguousarray(strength, np.float64).reshape(-1, 3)
+        out = np.zeros((len(self._keep[0]), 15, 4))
+        self._check(self.L.st_bond_used(self.h, _p(s), _p(x), _p(out)), "st_bond_used")
+        return out
+
+    def settle(self, strength, loads, precision=FP64, tol=1e-6, max_iter=10000, warm_start=False):
+        """bodies::settle on the device: (gone per node, final beam masks per cell, rounds, stats).
+        tol: the reference settles at 1e-6 (statics/src/bodies.rs:432)."""
+        s = np.ascontiguousarray(strength, np.float64).reshape(-1, 3)
+        loads = np.ascontiguousarray(loads, np.float64).reshape(self.nodes, 6)
+        o = StSolveOpts(precision, tol, max_iter, 0, 1 if warm_start else 0)
+        gone = np.zeros(self.nodes, np.uint8)
+        beams = np.zeros(len(self._keep[0]), np.uint16)
+        rounds = ctypes.c_int32()
+        st = StStats()
+        self._check(self.L.st_settle(self.h, _p(s), _p(loads), ctypes.byref(o), _p(gone), _p(beams),
+                                     ctypes.byref(rounds), ctypes.byref(st)), "st_settle")
+        return gone, beams, rounds.value, st
 
     def backward(self, objective=COMPLIANCE, beta=20.0, precision=FP64, tol=1e-8, max_iter=10000):
         o = StSolveOpts(precision, tol, max_iter, 0, 0)

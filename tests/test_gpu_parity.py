@@ -161,6 +161,70 @@ def test_bond_forces_parity(ctx):
     assert rel(P.bond_forces_dev(x), P.L.moved(x)) < 1e-12
 
 
+def test_bond_used_parity(ctx):
+    """st_bond_used against Bond::used of the oracle's bond forces, per-cell strengths."""
+    P = generated(ctx, 16, 3)
+    rng = np.random.default_rng(3)
+    x = rng.normal(size=(P.L.n_nodes, 6)) * 1e-2
+    x[P.L.held.astype(bool)] = 0
+    strength = rng.uniform(0.5, 2.0, size=(len(P.sites), 3))
+    u_dev = ctx.bond_used(strength, P.to_dev(x))[P.beam_cell, P.beam_q].reshape(-1)
+    u_or = P.L.used(P.L.moved(x), np.repeat(strength[P.beam_cell], 4, axis=0))
+    assert u_or.max() > 1.0 and rel(u_dev, u_or) < 1e-12
+
+
+def settle_both(P, **kw):
+    """The oracle's settle and st_settle on the same cells (gravity along -y, bodies.rs:408).
+    The oracle solves to 1e-10 (its recursive residual); the device to the reference's own
+    settle tolerance, 1e-6 (bodies.rs:432), in the FP64 true residual: after the first breaks
+    some rounds' systems floor the true residual near 3e-6 (the portico's second round)."""
+    o = P.L.settle(tol=1e-10)
+    gone, beams, rounds, st = P.ctx.settle(P.mats[:, 3:], P.to_dev(P.L.gravity()), tol=1e-6, **kw)
+    assert st.status == 0
+    gone = P.from_dev(gone)
+    free = ~P.L.held.astype(bool)
+    fallen = P.L.m[gone.astype(bool)].sum() / P.L.m[free].sum()
+    return o, gone, beams, rounds, fallen
+
+
+@pytest.mark.parametrize("name,scene,kw", [
+    ("stone beam 4 m", scenes.cantilever(orc.STONE, 0.5, 4.0), dict(precision=sh.MIXED, warm_start=True)),
+    ("stone beam 8 m", scenes.cantilever(orc.STONE, 0.5, 8.0), dict(precision=sh.FP64)),
+    ("portico, two columns cut", scenes.portico(2, 0.5), dict(precision=sh.MIXED, warm_start=True)),
+    ("tree cut 30 %", scenes.tree(8.0, -1.0, 0.3), dict(precision=sh.MIXED, warm_start=True)),
+    ("tree cut 70 %", scenes.tree(8.0, -1.0, 0.7), dict(precision=sh.MIXED)),
+])
+def test_settle_parity_scenes(ctx, name, scene, kw):
+    """bodies::settle on the reference's scenes (statics/examples/statics.rs): the same rounds,
+    the same gone nodes (bit-exact) and the same fallen share as the oracle."""
+    sites, mats, held = scenes.solids(0, *scene)
+    P = Pair(ctx, sites, mats, held)
+    o, gone, beams, rounds, fallen = settle_both(P, **kw)
+    assert rounds == o["rounds"], (name, rounds, o["rounds"])
+    assert (gone == o["gone"]).all(), (name, int(gone.sum()), int(o["gone"].sum()))
+    assert abs(fallen - o["fallen"]) <= 1e-12, (name, fallen, o["fallen"])
+    if rounds == 1:
+        assert (beams[held == 0] == 0x7FFF).all()
+
+
+def test_settle_parity_generated(ctx):
+    """A generated structure whose strengths its most stressed bonds overuse: breaking cascades
+    over several rounds, identically on both sides; the final masks kill whole beams only."""
+    occ, theta = sh.generate(20, 20, 20, seed=5, sigma=0.5)
+    sites, held, e, g = sh.cells_from_occ(occ, theta)
+    P0 = Pair(ctx, sites, mats_from(e, g), held)
+    u = P0.L.used_standing(tol=1e-10)
+    s = float(np.quantile(u[u > 0], 0.995))
+    P = Pair(ctx, sites, mats_from(e, g, strength=(s, s, s)), held)
+    o, gone, beams, rounds, fallen = settle_both(P, precision=sh.MIXED, warm_start=True)
+    assert o["rounds"] >= 2 and rounds == o["rounds"], (rounds, o["rounds"])
+    assert (gone == o["gone"]).all() and abs(fallen - o["fallen"]) <= 1e-12
+    # the context keeps the final masks: classifying with them gives the same gone set
+    jg = P.from_dev(ctx.joined_to_held())
+    assert ((jg == 0) == gone.astype(bool)).all()
+    assert (beams[held == 0] != 0x7FFF).any()
+
+
 def softmax_J(L, x, beta, live):
     f = L.moved(x)
     sigma = ((f * L.n).sum(axis=1) / L.a)[live]
