@@ -331,6 +331,92 @@ def fd_check(args, eps=1e-4, beta=20.0):
     print(json.dumps(out), flush=True)
 
 
+def settle_cpu_sample(size, seed, quantile, threads):
+    """The oracle's settle (bodies.rs:511-541, gravity -y, tol 1e-6) on a size^3 structure of the
+    same generator, strengths at the same quantile of its standing used shares."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as orc
+    occ, theta = structure(size, seed)
+    sites, held, e, g = sh.cells_from_occ(occ, theta)
+    mats = np.zeros((len(sites), 6))
+    mats[:, 0], mats[:, 1], mats[:, 2], mats[:, 3:] = 1.0, e, g, 1.0
+    orc.lib().orc_set_threads(threads)
+    u = orc.Lattice(0, sites, mats, held).used_standing(tol=1e-6)
+    mats[:, 3:] = float(np.quantile(u[u > 0], quantile))
+    L = orc.Lattice(0, sites, mats, held)
+    t0 = time.perf_counter()
+    o = L.settle(tol=1e-6)
+    sec = time.perf_counter() - t0
+    dof = 6 * int((L.held == 0).sum())
+    return dict(seconds=sec, rounds=o["rounds"], fallen=o["fallen"], dof=dof)
+
+
+def settle_bench(args):
+    """bodies::settle on the device (st_settle) for one --settle^3 structure of the bench's generator
+    under gravity along -y (the reference's forces), every strength at the --settle-quantile of the
+    standing structure's used shares, so that the most used bonds break and breaking cascades over
+    rounds; mixed solves to the reference's settle tolerance 1e-6 under its stopping rule (the
+    recursive residual decides), rounds warm-started."""
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    n = args.settle
+    occ, theta = structure(n, args.seed, spacing=args.spacing)
+    sites, held, e, g = sh.cells_from_occ(occ, theta)
+    ctx = sh.Context().upload(sites, held, e, g)
+    faces = ctx.node_faces()
+    joined = ctx.joined_to_held()
+    nheld = ctx.node_held()
+    f = node_loads(occ, faces, joined, nheld, np.random.default_rng(args.seed), n_points=0)
+    f = np.ascontiguousarray(f[:, [0, 2, 1, 3, 4, 5]])  # the weights along -y
+    x, st = ctx.solve(f, precision=sh.MIXED, tol=1e-6, max_iter=20000)
+    u = ctx.bond_used(np.ones((len(sites), 3)), x)
+    s = float(np.quantile(u[u > 0], args.settle_quantile))
+    del u
+    strength = np.full((len(sites), 3), s)
+    log(f"[bench] settle {n}^3: {ctx.cells} free cells, strength {s:.4g} (quantile {args.settle_quantile})")
+    times, last = [], None
+    for i in range(args.warmup + args.steps):
+        ctx.set_beams(None)  # every beam alive again
+        t0 = time.perf_counter()
+        gone, beams, rounds, st = ctx.settle(strength, f, precision=sh.MIXED, tol=1e-6, max_iter=20000,
+                                             warm_start=True, refresh=-1)
+        dt = time.perf_counter() - t0
+        log(f"[bench] settle {i}: {dt * 1e3:.1f} ms, {rounds} rounds, {st.iters} CG iterations, "
+            f"{int(gone.sum())} nodes gone")
+        if i >= args.warmup:
+            times.append(dt)
+        last = (gone, beams, rounds, st)
+    gone, beams, rounds, st = last
+    ms = 1e3 * sum(times) / len(times)
+    dof = 6 * int(st.dof_nodes) if st.dof_nodes else 6 * ctx.nodes
+    dof0 = 6 * int((nheld == 0).sum())
+    cpu = None
+    if args.settle_cpu_sample > 0:
+        cb = settle_cpu_sample(args.settle_cpu_sample, args.seed, args.settle_quantile, args.cpu_threads)
+        per_s = (1.0 / cb["seconds"]) * (cb["dof"] * cb["rounds"]) / (dof0 * rounds)
+        cpu = {"value": per_s, "unit": f"settles/sec ({n}^3, extrapolated)", "cores": args.cpu_threads,
+               "kind": "port",
+               "sample": f"oracle settle (C++ FP64 restatement of statics/src/bodies.rs:511-541) on the "
+                         f"{args.settle_cpu_sample}^3 structure of the same generator and strength rule: "
+                         f"{cb['seconds']:.2f} s, {cb['rounds']} rounds, {cb['dof']} DOF; scaled by DOF x rounds "
+                         f"to the {n}^3 settle's {dof0} DOF x {rounds} rounds"}
+    out = {
+        "metric": f"settles/sec of one {n}^3 structure (bodies::settle on the device)",
+        "value": 1e3 / ms, "unit": "settles/sec", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 CG + f32 V-cycle", "data": "synthetic",
+        "config": {"workload": f"one {n}^3 pillar/slab structure (pillar spacing {args.spacing:g}), lognormal "
+                               f"stiffness sigma 0.5, gravity along -y, strength at the {args.settle_quantile} "
+                               f"quantile of the standing used shares; step = st_settle (classify, solve to 1e-6, "
+                               f"break, until nothing breaks)",
+                   "lattice": n, "free_cells": int(ctx.cells), "dof_standing": dof0},
+        "rounds": rounds, "cg_iterations": int(st.iters), "nodes_gone": int(gone.sum()),
+        "beams_broken": int(np.unpackbits((beams[held == 0] ^ np.uint16(0x7FFF)).view(np.uint8)).sum()),
+        "setup_ms_total": st.setup_ms, "solve_ms_total": st.solve_ms,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -352,6 +438,11 @@ def main():
                     help="instead: the gradient check, d(compliance) and d(soft-max stress)/d(log-stiffness) "
                          "against central differences (eps 1e-4, FP64 solves to 1e-10) at this many cells "
                          "of a --size structure (64 at 128: the adjoint config)")
+    ap.add_argument("--settle", type=int, default=0, metavar="N",
+                    help="instead: bodies::settle on the device for one N^3 structure (see settle_bench)")
+    ap.add_argument("--settle-cpu-sample", type=int, default=32, help="settle: edge of the CPU oracle's sample")
+    ap.add_argument("--settle-quantile", type=float, default=0.999,
+                    help="settle: every strength at this quantile of the standing structure's used shares")
     ap.add_argument("--copy-calib", action="store_true",
                     help="end with three device copies of 1 GiB (the calibration kernel of the traffic counters)")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -372,6 +463,9 @@ def main():
         return
     if args.fd_check:
         fd_check(args)
+        return
+    if args.settle:
+        settle_bench(args)
         return
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
 

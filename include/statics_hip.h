@@ -73,7 +73,10 @@ typedef struct {
   int32_t precision;   /* StPrecision */
   double tol;          /* stop when ||f - K x||_2 <= tol * ||f||_2 (statics/src/bodies.rs:350-352) */
   int32_t max_iter;    /* cap on CG iterations (the reference: 100,000) */
-  int32_t refresh;     /* reserved, ignored (0) */
+  int32_t refresh;     /* 0: the FP64 true residual must meet tol (the solve restarts from
+                          it when the recursive residual does, and stops when it stalls);
+                          < 0: the reference's rule (statics/src/bodies.rs:350-352), the
+                          recursive residual decides; rel_residual reports the true one */
   int32_t warm_start;  /* nonzero: start from x_out's contents instead of 0 */
 } StSolveOpts;
 

@@ -4,6 +4,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "st_solver.hpp"
@@ -335,7 +336,7 @@ __global__ void k_mask_rows2(int64_t n, const uint8_t* nflags, double* v) {
 void check_opts(const StSolveOpts* o) {
   if (!o) throw StError{ST_BAD_INPUT, "opts is NULL"};
   if (o->precision != ST_FP64 && o->precision != ST_MIXED) throw StError{ST_BAD_INPUT, "opts.precision"};
-  if (!(o->tol >= 0) || o->max_iter < 0 || o->refresh < 0) throw StError{ST_BAD_INPUT, "opts: tol, max_iter or refresh"};
+  if (!(o->tol >= 0) || o->max_iter < 0) throw StError{ST_BAD_INPUT, "opts: tol or max_iter"};
 }
 
 int worst(const std::vector<StStats>& s) {
@@ -363,10 +364,8 @@ void set_loads(StCtx* c, const double* loads) {
   const int64_t nn = c->geo.nnodes;
   const size_t bytes = (size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double);
   c->fres.ensure(bytes);
-  if (nn) {
-    ST_HIP(hipMemcpyAsync(c->fres.p, loads, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->fres.as<double>());
-  }
+  // kept as given: each solve masks the rows of the nodes held or gone at that time
+  if (nn) ST_HIP(hipMemcpyAsync(c->fres.p, loads, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
   c->loads_set = true;
 }
 
@@ -377,9 +376,11 @@ std::vector<StStats> solve_resident(StCtx* c, const StSolveOpts& o) {
   const size_t bytes = (size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double);
   c->xsol.ensure(bytes);
   c->x64.ensure(bytes);
+  c->f64.ensure(bytes);
   if (nn) {
-    // the rows of held and gone nodes may have changed since the loads were set
-    k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->fres.as<double>());
+    // the loads of the nodes free now (the held and gone ones may have changed since they were set)
+    ST_HIP(hipMemcpyAsync(c->f64.p, c->fres.p, nn * 6 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->f64.as<double>());
     if (o.warm_start && c->have_solution) {
       ST_HIP(hipMemcpyAsync(c->x64.p, c->xsol.p, nn * 6 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
       k_mask_rows2<<<nblk(nn), 256, 0, c->stream>>>(nn, c->geo.nflags.as<uint8_t>(), c->x64.as<double>());
@@ -387,15 +388,8 @@ std::vector<StStats> solve_resident(StCtx* c, const StSolveOpts& o) {
       ST_HIP(hipMemsetAsync(c->x64.p, 0, nn * 6 * sizeof(double), c->stream));
     }
   }
-  std::swap(c->f64, c->fres);
   std::vector<StStats> out;
-  try {
-    solve_all(*c, o, out);
-  } catch (...) {
-    std::swap(c->f64, c->fres);
-    throw;
-  }
-  std::swap(c->f64, c->fres);
+  solve_all(*c, o, out);
   std::swap(c->x64, c->xsol);
   c->have_solution = true;
   c->last_opts = o;
@@ -649,6 +643,7 @@ int st_settle(StCtx* c, const double* strength, const double* loads, const StSol
       total.iters += st_out[0].iters;
       total.setup_ms += st_out[0].setup_ms;
       total.solve_ms += st_out[0].solve_ms;
+      const bool verbose = std::getenv("STATICS_HIP_VERBOSE") != nullptr;
       // bodies.rs:523-533: beams with an overused bond break
       ST_HIP(hipMemsetAsync(broken.p, 0, sizeof(int), c->stream));
       if (ncl)
@@ -661,6 +656,9 @@ int st_settle(StCtx* c, const double* strength, const double* loads, const StSol
       int nb = 0;
       ST_HIP(hipMemcpyAsync(&nb, broken.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
       ST_HIP(hipStreamSynchronize(c->stream));
+      if (verbose)
+        std::fprintf(stderr, "statics_hip: settle round %d: %d iterations, residual %.3g, %d beams broken\n", rounds,
+                     st_out[0].iters, st_out[0].rel_residual, nb);
       if (!nb) break;
       refresh_cell_flags(*c);  // the masks changed: the kept hierarchy no longer fits
     }

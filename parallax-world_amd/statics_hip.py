@@ -251,12 +251,12 @@ class Context:
         self._check(self.L.st_bond_used(self.h, _p(s), _p(x), _p(out)), "st_bond_used")
         return out
 
-    def settle(self, strength, loads, precision=FP64, tol=1e-6, max_iter=10000, warm_start=False):
+    def settle(self, strength, loads, precision=FP64, tol=1e-6, max_iter=10000, warm_start=False, refresh=0):
         """bodies::settle on the device: (gone per node, final beam masks per cell, rounds, stats).
-        tol: the reference settles at 1e-6 (statics/src/bodies.rs:432)."""
+        tol: the reference settles at 1e-6 (statics/src/bodies.rs:432); refresh -1: its stopping rule."""
         s = np.ascontiguousarray(strength, np.float64).reshape(-1, 3)
         loads = np.ascontiguousarray(loads, np.float64).reshape(self.nodes, 6)
-        o = StSolveOpts(precision, tol, max_iter, 0, 1 if warm_start else 0)
+        o = StSolveOpts(precision, tol, max_iter, refresh, 1 if warm_start else 0)
         gone = np.zeros(self.nodes, np.uint8)
         beams = np.zeros(len(self._keep[0]), np.uint16)
         rounds = ctypes.c_int32()

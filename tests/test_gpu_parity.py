@@ -153,6 +153,19 @@ def test_solve_parity_reference_scenes(ctx, scene):
     assert abs(u_dev.max() - u_or.max()) <= 1e-5 * u_or.max()
 
 
+def test_reference_stopping_rule(ctx):
+    """refresh < 0: the recursive residual decides (bodies.rs:350-352); the solution still meets
+    the bar and rel_residual reports the true residual."""
+    P = generated(ctx, 24, 5)
+    gone = classify_both(P)
+    f = gravity_z(P.L)
+    x_or, _ = P.L.motions(f, gone=gone, tol=1e-13, max_iter=5000)
+    for prec in (sh.FP64, sh.MIXED):
+        x_dev, st = ctx.solve(P.to_dev(f), precision=prec, tol=1e-8, refresh=-1)
+        assert st.status == 0 and st.refreshes == 0 and st.rel_residual < 1e-7
+        assert rel(P.from_dev(x_dev), x_or) < 1e-6
+
+
 def test_bond_forces_parity(ctx):
     P = generated(ctx, 16, 3)
     rng = np.random.default_rng(3)
@@ -173,13 +186,13 @@ def test_bond_used_parity(ctx):
     assert u_or.max() > 1.0 and rel(u_dev, u_or) < 1e-12
 
 
-def settle_both(P, **kw):
+def settle_both(P, tol=1e-6, **kw):
     """The oracle's settle and st_settle on the same cells (gravity along -y, bodies.rs:408).
     The oracle solves to 1e-10 (its recursive residual); the device to the reference's own
     settle tolerance, 1e-6 (bodies.rs:432), in the FP64 true residual: after the first breaks
     some rounds' systems floor the true residual near 3e-6 (the portico's second round)."""
     o = P.L.settle(tol=1e-10)
-    gone, beams, rounds, st = P.ctx.settle(P.mats[:, 3:], P.to_dev(P.L.gravity()), tol=1e-6, **kw)
+    gone, beams, rounds, st = P.ctx.settle(P.mats[:, 3:], P.to_dev(P.L.gravity()), tol=tol, **kw)
     assert st.status == 0
     gone = P.from_dev(gone)
     free = ~P.L.held.astype(bool)
@@ -191,6 +204,8 @@ def settle_both(P, **kw):
     ("stone beam 4 m", scenes.cantilever(orc.STONE, 0.5, 4.0), dict(precision=sh.MIXED, warm_start=True)),
     ("stone beam 8 m", scenes.cantilever(orc.STONE, 0.5, 8.0), dict(precision=sh.FP64)),
     ("portico, two columns cut", scenes.portico(2, 0.5), dict(precision=sh.MIXED, warm_start=True)),
+    ("portico at 1e-8, the reference's stopping rule", scenes.portico(2, 0.5),
+     dict(precision=sh.MIXED, warm_start=True, tol=1e-8, refresh=-1)),
     ("tree cut 30 %", scenes.tree(8.0, -1.0, 0.3), dict(precision=sh.MIXED, warm_start=True)),
     ("tree cut 70 %", scenes.tree(8.0, -1.0, 0.7), dict(precision=sh.MIXED)),
 ])
@@ -223,6 +238,11 @@ def test_settle_parity_generated(ctx):
     jg = P.from_dev(ctx.joined_to_held())
     assert ((jg == 0) == gone.astype(bool)).all()
     assert (beams[held == 0] != 0x7FFF).any()
+    # every beam alive again: a second settle repeats the first (the loads set while most
+    # nodes were gone still reach every node)
+    ctx.set_beams(None)
+    o2, gone2, beams2, rounds2, fallen2 = settle_both(P, precision=sh.MIXED, warm_start=True)
+    assert rounds2 == rounds and (gone2 == gone).all() and (beams2 == beams).all()
 
 
 def softmax_J(L, x, beta, live):
