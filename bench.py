@@ -259,9 +259,9 @@ def batch_bench(args):
 
 
 def fd_check(args, eps=1e-4, beta=20.0):
-    import ctypes
     """Adjoint gradients against central differences of FP64 solves (tol 1e-10: the FP64 floor of
     these systems lies near 1e-11; the achieved residuals are reported) at random free cells."""
+    import ctypes
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     n = args.size
     occ, theta = structure(n, args.seed)
@@ -272,8 +272,9 @@ def fd_check(args, eps=1e-4, beta=20.0):
     faces = ctx.node_faces()
     f = node_loads(occ, faces, joined, nheld, np.random.default_rng(args.seed), n_points=0)
     t0 = time.perf_counter()
-    _, st0 = ctx.solve(f, precision=sh.FP64, tol=1e-10, max_iter=20000, check=False)
+    x0, st0 = ctx.solve(f, precision=sh.FP64, tol=1e-10, max_iter=20000, check=False)
     worst = st0.rel_residual
+    y0 = ctx.apply(x0)
     grads = {}
     for name, obj in (("compliance", sh.COMPLIANCE), ("softmax_stress", sh.SOFTMAX_STRESS)):
         o = sh.StSolveOpts(sh.FP64, 1e-10, 20000, 0, 0)
@@ -296,11 +297,15 @@ def fd_check(args, eps=1e-4, beta=20.0):
             e2[c] *= np.exp(sgn * eps)
             g2[c] *= np.exp(sgn * eps)
             ctx.set_stiffness(e2, g2)
-            xp, stp = ctx.solve(f, precision=sh.FP64, tol=1e-10, max_iter=20000, check=False)
-            worst = max(worst, stp.rel_residual)
-            # compliance in its stationary form 2 f.x - x.K x: its error is second order in the
-            # solve's error (f.x alone would carry the 1e-10 residual into a difference of 1e-4)
-            Jpm["compliance"].append(2.0 * float((f * xp).sum()) - float((xp * ctx.apply(xp)).sum()))
+            # x(theta +- eps) = x0 + d with K(theta +- eps) d = -(K(theta +- eps) - K0) x0: solving for
+            # the difference d puts the solve's error on d, not on x, so C(+) - C(-) = f.d(+) - f.d(-)
+            # keeps its digits (two solves of x to 1e-10 would leave ~1e-10 C of noise in each C)
+            b = y0 - ctx.apply(x0)
+            d, std = ctx.solve(b, precision=sh.FP64, tol=1e-10, max_iter=20000, check=False)
+            # the context's solution becomes x0 + d (already converged) for the soft-max objective
+            xp, stp = ctx.solve(f, precision=sh.FP64, tol=1e-10, max_iter=20000, x0=x0 + d, check=False)
+            worst = max(worst, stp.rel_residual, std.rel_residual * float(np.abs(b).max() > 0))
+            Jpm["compliance"].append(float((f * d).sum()))
             Jpm["softmax_stress"].append(ctx.objective(sh.SOFTMAX_STRESS, beta))
         for k in grads:
             fd = (Jpm[k][0] - Jpm[k][1]) / (2 * eps)
@@ -314,7 +319,8 @@ def fd_check(args, eps=1e-4, beta=20.0):
     ctx.set_stiffness(e, g)
     out = {
         "check": f"adjoint gradients vs central differences, {n}^3 structure, {len(pick)} random free cells, "
-                 f"eps {eps}, FP64 solves to 1e-10, beta {beta}",
+                 f"eps {eps}, FP64 solves to 1e-10, beta {beta}; C(+-eps) - C(0) = f.d with d solved from "
+                 f"K(+-eps) d = -(K(+-eps) - K0) x0",
         "worst_rel_residual": worst,
         "free_cells": int(len(free)),
         "max_rel_err_compliance": float(max(errs["compliance"])),

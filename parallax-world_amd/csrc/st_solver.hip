@@ -983,26 +983,114 @@ __device__ __forceinline__ T block_row_dot(int b0, int b1, const int* col, const
   return acc;
 }
 
+// Coarse residual and smoothing, lanes mapped onto each block's 36 entries: a row is
+// worked by L = 36 / E lanes (9 in FP32, 18 in FP64), lane i loading entries
+// E i .. E i + E - 1 of every block of the row as one 16-byte load, so a group reads
+// each block as one contiguous 144 or 288-byte run. Each lane keeps its entries' sums
+// over the row's blocks; at the row's end the per-entry sums meet in LDS and are
+// added in a fixed order. 64 / L rows per wave, four waves per block.
 // mode 0: out = r - A x; mode 1: out = x + w D^-1 (r - A x)
 template <class T>
-__global__ void k_c_resid(int mode, int64_t n, const uint32_t* ast, const uint8_t* done, const int* rowptr, const int* col,
-                          const T* A, const T* dinv, const double* step, const T* r, const T* x, T* out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t I = t >> 3;
-  const int i = (int)(t & 7);
-  const bool live = I < n && !(done && done[ast[I]]);
-  T acc = 0;
-  if (live && i < 6) acc = r[6 * I + i] - block_row_dot<T>(rowptr[I], rowptr[I + 1], col, A, x, i);
-  if (mode == 0) {
-    if (live && i < 6) out[6 * I + i] = acc;
-    return;
+struct CoarseLanes {
+  static constexpr int E = 16 / (int)sizeof(T);  // entries per lane
+  static constexpr int L = 36 / E;               // lanes per row
+  static constexpr int R = 64 / L;               // rows per wave
+  static constexpr int kRowsPerBlock = 4 * R;
+};
+template <class T>
+__device__ __forceinline__ void load_entries(const T* p, T* v);
+template <>
+__device__ __forceinline__ void load_entries<float>(const float* p, float* v) {
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+template <>
+__device__ __forceinline__ void load_entries<double>(const double* p, double* v) {
+  const double2 a = *reinterpret_cast<const double2*>(p);
+  v[0] = a.x; v[1] = a.y;
+}
+// the x components lane entries E i .. E i + E - 1 multiply: columns c0, c0 + 1, ... mod 6
+template <class T>
+__device__ __forceinline__ void load_columns(const T* xr, int c0, T* v);
+template <>
+__device__ __forceinline__ void load_columns<float>(const float* xr, int c0, float* v) {
+  const float2 a = *reinterpret_cast<const float2*>(xr + c0);
+  const float2 b = *reinterpret_cast<const float2*>(xr + (c0 + 2) % 6);
+  v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+template <>
+__device__ __forceinline__ void load_columns<double>(const double* xr, int c0, double* v) {
+  const double2 a = *reinterpret_cast<const double2*>(xr + c0);
+  v[0] = a.x; v[1] = a.y;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_c_resid(int mode, int64_t n, const uint32_t* ast, const uint8_t* done,
+                                                 const int* rowptr, const int* col, const T* A, const T* dinv,
+                                                 const double* step, const T* r, const T* x, T* out) {
+  using CL = CoarseLanes<T>;
+  constexpr int E = CL::E, L = CL::L, R = CL::R;
+  __shared__ T part[4][R][36];
+  __shared__ T vres[4][R][6];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = lane / L, i = lane - g * L;
+  const int64_t I = ((int64_t)blockIdx.x * 4 + wv) * R + g;
+  const bool live = g < R && I < n && !(done && done[ast[I]]);
+  if (live) {
+    T acc[E];
+#pragma unroll
+    for (int k = 0; k < E; k++) acc[k] = T(0);
+    const int c0 = (E * i) % 6;
+    int b = rowptr[I];
+    const int b1 = rowptr[I + 1];
+    for (; b + 4 <= b1; b += 4) {
+      int cb[4];
+      T av[4][E], xv[4][E];
+#pragma unroll
+      for (int q = 0; q < 4; q++) cb[q] = col[b + q];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        load_entries<T>(A + 36 * (int64_t)(b + q) + E * i, av[q]);
+        load_columns<T>(x + 6 * (int64_t)cb[q], c0, xv[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int k = 0; k < E; k++) acc[k] += av[q][k] * xv[q][k];
+    }
+    for (; b < b1; b++) {
+      T av[E], xv[E];
+      load_entries<T>(A + 36 * (int64_t)b + E * i, av);
+      load_columns<T>(x + 6 * (int64_t)col[b], c0, xv);
+#pragma unroll
+      for (int k = 0; k < E; k++) acc[k] += av[k] * xv[k];
+    }
+#pragma unroll
+    for (int k = 0; k < E; k++) part[wv][g][E * i + k] = acc[k];
   }
-  T all[6];
-  group_gather6<T>(acc, all);
+  __syncthreads();
+  T v = T(0);
   if (live && i < 6) {
-    const T d = row6_dot<T>(dinv + 36 * I + 6 * i, all);
+    T s = T(0);
+#pragma unroll
+    for (int c = 0; c < 6; c++) s += part[wv][g][6 * i + c];
+    v = r[6 * I + i] - s;
+    if (mode == 0) out[6 * I + i] = v;
+    vres[wv][g][i] = v;
+  }
+  if (mode == 0) return;
+  __syncthreads();
+  if (live && i < 6) {
+    T d = T(0);
+#pragma unroll
+    for (int j = 0; j < 6; j++) d += dinv[36 * I + 6 * i + j] * vres[wv][g][j];
     out[6 * I + i] = x[6 * I + i] + (T)step[ast[I]] * d;
   }
+}
+template <class T>
+inline unsigned coarse_blocks(int64_t n) {
+  const int64_t rpb = CoarseLanes<T>::kRowsPerBlock;
+  return (unsigned)std::max<int64_t>(1, (n + rpb - 1) / rpb);
 }
 
 // y = A x (for power iterations), then partial sums of x.x per aggregate are not needed: w = D^-1 A x
@@ -2524,7 +2612,7 @@ struct Solver {
     T* t = lvbuf<T>(lv, 2);
     T* v = lvbuf<T>(lv, 4);
     for (int k = 1; k < reps; k++) {
-      k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
+      k_c_resid<T><<<coarse_blocks<T>(n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
                                                nullptr, nullptr, r, x, t);
       k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(0), x, v);  // v = x
       one_cycle<T>(li, t, x);
@@ -2543,7 +2631,7 @@ struct Solver {
     if (li + 1 == c.levels.size()) {  // not block diagonal (should not happen): smoothing sweeps
       k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), rhs, x);
       for (int k = 0; k < 8; k++) {
-        k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
+        k_c_resid<T><<<coarse_blocks<T>(n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
                                                  lvD<T>(lv), lv.step_d.as<double>(), rhs, x, u);
         k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(0), u, x);
       }
@@ -2554,7 +2642,7 @@ struct Solver {
     // x <- x + w D^-1 (rhs - A x), k times
     auto smooth = [&](int k) {
       for (int i = 0; i < k; i++) {
-        k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
+        k_c_resid<T><<<coarse_blocks<T>(n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
                                                  lvD<T>(lv), lv.step_d.as<double>(), rhs, x, u);
         k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(0), u, x);
       }
@@ -2562,7 +2650,7 @@ struct Solver {
     k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), rhs, x);
     smooth(nu - 1);
     const T* pv = std::is_same<T, double>::value ? (const T*)lv.pval64.p : (const T*)lv.pval32.p;
-    k_c_resid<T><<<nblk(8 * n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
+    k_c_resid<T><<<coarse_blocks<T>(n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
                                              nullptr, nullptr, rhs, x, u);
     if (lv.pblocks)
       k_restrict_p<T><<<nblk(8 * up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(),
