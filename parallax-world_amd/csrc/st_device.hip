@@ -201,6 +201,8 @@ ST_INST_APPLY(double, EpiStore<double>)
 ST_INST_APPLY(float, EpiStore<float>)
 ST_INST_APPLY(double, EpiDot<double>)
 ST_INST_APPLY(float, EpiDot<float>)
+ST_INST_APPLY(double, EpiResidRestrict<double>)
+ST_INST_APPLY(float, EpiResidRestrict<float>)
 ST_INST_APPLY(double, SINGLE_ARG(EpiResid<double, double>))
 ST_INST_APPLY(float, SINGLE_ARG(EpiResid<float, float>))
 ST_INST_APPLY(float, SINGLE_ARG(EpiResid<float, double>))

@@ -163,6 +163,11 @@ struct Ctx {
   // fine-node aggregation (level 1) and the union-find scratch
   DBuf label1, uf, uf2, flag, scan_tmp, sort_tmp;
   DBuf nodes_by_agg, agg_ptr;
+  // per brick, its level-1 aggregates [brick_agg1[b], brick_agg1[b + 1]) (int32, nbricks + 1);
+  // restrict_fused: every level-1 aggregate lies in one brick, so the operator's epilogue
+  // restricts the residual brick by brick
+  DBuf brick_agg1;
+  bool restrict_fused = false;
   // smoothed prolongator from level 1 to the fine nodes (rows: nodes), used to build level 1
   DBuf p0row, p0col, p0val, p0t_ptr, p0t_blk, p0blk_row;
   int64_t p0blocks = 0;

@@ -256,6 +256,10 @@ __global__ __launch_bounds__(kBlock, ST_APPLY_WAVES) void k_apply(GeoD<T> g, con
     const double tot = block_sum(part, red);
     if (threadIdx.x == 0) epi.partial[b] = tot;
   }
+  if constexpr (Epi::kTail) {
+    __syncthreads();
+    epi.tail(b, s);
+  }
 }
 
 // The operator cell by cell (the other schedule): one workgroup per brick, one
@@ -342,6 +346,37 @@ __global__ __launch_bounds__(kBlock) void k_apply_cells(GeoD<T> g, const uint32_
     const double tot = block_sum(part, red);
     if (threadIdx.x == 0) epi.partial[b] = tot;
   }
+  if constexpr (Epi::kTail) {
+    __syncthreads();
+    epi.tail(b, s);
+  }
+}
+
+// ---------------------------------------------------------------- node geometry
+// Position of node v (its face's centre, metres, in its structure's frame) at spacing s.
+__device__ __forceinline__ void node_pos(uint32_t v, const uint32_t* nkey, const uint8_t* nkind, double s, double* p) {
+  const uint32_t k = nkey[v];
+  const int kind = nkind[v], o = kind >> 1;
+  p[0] = (double)(k & 1023u) + 0.5;
+  p[1] = (double)((k >> 10) & 1023u) + 0.5;
+  p[2] = (double)((k >> 20) & 1023u) + 0.5;
+  p[o] += (kind & 1) ? 0.5 : -0.5;
+  p[0] *= s;
+  p[1] *= s;
+  p[2] *= s;
+}
+
+// A load [f; m] at offset r taken to the centre, [f; m + r x f]: the transpose of the
+// rigid shift [u + w x r; w] (statics/src/bodies.rs:44-47).
+template <class T>
+__device__ __forceinline__ void shiftT_add(const T* f, const double* r, T* out) {
+  const T r0 = (T)r[0], r1 = (T)r[1], r2 = (T)r[2];
+  out[0] += f[0];
+  out[1] += f[1];
+  out[2] += f[2];
+  out[3] += f[3] + (r1 * f[2] - r2 * f[1]);
+  out[4] += f[4] + (r2 * f[0] - r0 * f[2]);
+  out[5] += f[5] + (r0 * f[1] - r1 * f[0]);
 }
 
 // ---------------------------------------------------------------- epilogues
@@ -349,6 +384,10 @@ struct EpiBase {
   const uint8_t* done = nullptr;  // per structure: skip when set
   double* partial = nullptr;
   __device__ bool skip(uint32_t s) const { return done && done[s]; }
+  // kTail: after every node of the brick is done (and the block synchronised), the
+  // workgroup runs tail(b, s) once more over the brick
+  static constexpr bool kTail = false;
+  __device__ void tail(uint32_t, uint32_t) const {}
 };
 
 template <class T>
@@ -395,6 +434,67 @@ struct EpiResid : EpiBase {
     }
     store6<T>(r + 6 * (size_t)n, rv);
     return d;
+  }
+};
+
+// The residual u = b - K x (into r, in place allowed) followed, per brick, by the rigid
+// restriction to level 1, c_I = sum over the aggregate's nodes v of Sh(p_v - centre_I)^T u_v
+// (statics/src/bodies.rs:44-47 transposed). Eight lanes take an aggregate, lane k its nodes
+// k, k + 8, ... in node order, and a butterfly over the eight lanes adds their sums (a fixed
+// order). Level-1 aggregates lie inside one brick (checked at setup), so the workgroup that
+// wrote a brick's residuals restricts them; brick_agg[b] .. brick_agg[b + 1] are the
+// brick's aggregates.
+template <class T>
+struct EpiResidRestrict : EpiBase {
+  static constexpr bool kReduce = false;
+  static constexpr bool kTail = true;
+  const T* b;
+  T* r;
+  const int *brick_agg, *agg_ptr, *nodes;
+  const double* centre;
+  const uint32_t* nkey;
+  const uint8_t* nkind;
+  const BeamGeo* bgeo;
+  T* c;
+  __device__ double node(uint32_t n, const T* a, uint32_t, const T*) const {
+    T bv[6], rv[6];
+    load6<T>(b + 6 * (size_t)n, bv);
+#pragma unroll
+    for (int k = 0; k < 6; k++) rv[k] = bv[k] - a[k];
+    store6<T>(r + 6 * (size_t)n, rv);
+    return 0.0;
+  }
+  __device__ void tail(uint32_t bk, uint32_t s) const {
+    const double sp = 2.0 * bgeo[s].hd[0];
+    const int lane = (int)(threadIdx.x & 7), group = (int)(threadIdx.x >> 3), groups = (int)(blockDim.x >> 3);
+    const int a0 = brick_agg[bk], a1 = brick_agg[bk + 1];
+    // every lane of a wave runs the same number of rounds, so the shuffles see whole groups
+    const int rounds = (a1 - a0 + groups - 1) / groups;
+    for (int q = 0; q < rounds; q++) {
+      const int I = a0 + q * groups + group;
+      T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+      if (I < a1) {
+        double cI[3];
+        for (int k = 0; k < 3; k++) cI[k] = centre[3 * (size_t)I + k];
+        for (int i = agg_ptr[I] + lane; i < agg_ptr[I + 1]; i += 8) {
+          const uint32_t v = (uint32_t)nodes[i];
+          double p[3];
+          node_pos(v, nkey, nkind, sp, p);
+          for (int k = 0; k < 3; k++) p[k] -= cI[k];
+          T tv[6];
+          load6<T>(r + 6 * (size_t)v, tv);
+          shiftT_add<T>(tv, p, acc);
+        }
+      }
+#pragma unroll
+      for (int m = 1; m < 8; m <<= 1)
+#pragma unroll
+        for (int k = 0; k < 6; k++) acc[k] += __shfl_xor(acc[k], m, 64);
+      T o = acc[0];
+#pragma unroll
+      for (int k = 1; k < 6; k++) o = lane == k ? acc[k] : o;
+      if (I < a1 && lane < 6) c[6 * (size_t)I + lane] = o;
+    }
   }
 };
 

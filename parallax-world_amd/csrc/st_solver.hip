@@ -52,18 +52,6 @@ constexpr int kMaxJ = 96;  // distinct neighbours of one aggregate
 inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)std::max<int64_t>(1, (n + b - 1) / b); }
 
 // --------------------------------------------------------------- geometry
-__device__ __forceinline__ void node_pos(uint32_t v, const uint32_t* nkey, const uint8_t* nkind, double s, double* p) {
-  const uint32_t k = nkey[v];
-  const int kind = nkind[v], o = kind >> 1;
-  p[0] = (double)(k & 1023u) + 0.5;
-  p[1] = (double)((k >> 10) & 1023u) + 0.5;
-  p[2] = (double)((k >> 20) & 1023u) + 0.5;
-  p[o] += (kind & 1) ? 0.5 : -0.5;
-  p[0] *= s;
-  p[1] *= s;
-  p[2] *= s;
-}
-
 // Rigid motion y about a centre seen at offset r: [u + w x r; w] (statics/src/bodies.rs:44-47)
 template <class T>
 __device__ __forceinline__ void shift_add(const T* y, const double* r, T* out) {
@@ -75,18 +63,6 @@ __device__ __forceinline__ void shift_add(const T* y, const double* r, T* out) {
   out[4] += y[4];
   out[5] += y[5];
 }
-// Its transpose: a load [f; m] at offset r taken to the centre: [f; m + r x f]
-template <class T>
-__device__ __forceinline__ void shiftT_add(const T* f, const double* r, T* out) {
-  const T r0 = (T)r[0], r1 = (T)r[1], r2 = (T)r[2];
-  out[0] += f[0];
-  out[1] += f[1];
-  out[2] += f[2];
-  out[3] += f[3] + (r1 * f[2] - r2 * f[1]);
-  out[4] += f[4] + (r2 * f[0] - r0 * f[2]);
-  out[5] += f[5] + (r0 * f[1] - r1 * f[0]);
-}
-
 // 6x6 inverse by Gauss-Jordan with partial pivoting (FP64). Returns false if singular.
 __device__ bool invert6(const double* a_in, double* inv) {
   double a[36];
@@ -1122,6 +1098,44 @@ __global__ void k_c_init(int64_t n, double* x) {
 }
 
 // restrict fine -> level 1: c[I] = sum over nodes v of I of Sh(r_v)^T t[v]
+// brick_agg[b] = the first level-1 aggregate whose first node lies in brick b or later
+// (aggregates are numbered in the order of their first nodes); brick_agg[nb] = n1
+__global__ void k_brick_agg1(int64_t nb, const uint32_t* nbase, int64_t n1, const int* agg_ptr, const int* nodes,
+                             int* brick_agg) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nb) return;
+  if (b == nb) {
+    brick_agg[b] = (int)n1;
+    return;
+  }
+  const uint32_t start = nbase[b];
+  int64_t lo = 0, hi = n1;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((uint32_t)nodes[agg_ptr[mid]] < start)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  brick_agg[b] = (int)lo;
+}
+// Flags an aggregate whose nodes (sorted) do not all lie in the brick of its first node.
+__global__ void k_agg_straddle(int64_t n1, const int* agg_ptr, const int* nodes, int64_t nb, const uint32_t* nbase,
+                               int* bad) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n1) return;
+  const uint32_t first = (uint32_t)nodes[agg_ptr[I]], last = (uint32_t)nodes[agg_ptr[I + 1] - 1];
+  int64_t lo = 0, hi = nb;  // the brick of `first`: the last b with nbase[b] <= first
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (nbase[mid] <= first)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  if (last >= nbase[lo + 1] || first < nbase[lo]) atomicOr(bad, 1);
+}
+
 template <class T>
 __global__ void k_restrict1(int64_t n1, const uint32_t* ast, const uint8_t* done, const int* agg_ptr, const int* nodes,
                             const double* centre, const uint32_t* nkey, const uint8_t* nkind, const BeamGeo* bgeo,
@@ -2412,6 +2426,14 @@ struct Solver {
         keys.release();
         vals.release();
         skeys.release();
+        // the brick ranges of the level-1 aggregates, for the fused restriction
+        c.brick_agg1.ensure((g.nbricks + 1) * sizeof(int));
+        k_brick_agg1<<<nblk(g.nbricks + 1), 256, 0, st>>>(g.nbricks, g.nbase.as<uint32_t>(), cnt, c.agg_ptr.as<int>(),
+                                                         c.nodes_by_agg.as<int>(), c.brick_agg1.as<int>());
+        ST_HIP(hipMemsetAsync(sbad.p, 0, sizeof(int), st));
+        k_agg_straddle<<<nblk(cnt), 256, 0, st>>>(cnt, c.agg_ptr.as<int>(), c.nodes_by_agg.as<int>(), g.nbricks,
+                                                  g.nbase.as<uint32_t>(), sbad.as<int>());
+        c.restrict_fused = read_int(sbad.as<int>()) == 0 && std::getenv("STATICS_HIP_NOFUSE") == nullptr;
         if (c.smoothed)
           build_sa_level1(lv);
         else
@@ -2670,6 +2692,38 @@ struct Solver {
     smooth(nu);
   }
 
+  // u = b - K x into u, and its rigid restriction P_rigid^T u into level 1's right-hand side:
+  // fused into the operator's epilogue when every level-1 aggregate lies in one brick
+  template <class T>
+  void residual_restrict(const T* x, const T* b, T* u, Level& l1) {
+    const uint8_t* done = c.done.as<uint8_t>();
+    if (c.restrict_fused) {
+      EpiResidRestrict<T> er;
+      er.done = done;
+      er.b = b;
+      er.r = u;
+      er.brick_agg = c.brick_agg1.as<int>();
+      er.agg_ptr = c.agg_ptr.as<int>();
+      er.nodes = c.nodes_by_agg.as<int>();
+      er.centre = l1.centre.as<double>();
+      er.nkey = g.nkey.as<uint32_t>();
+      er.nkind = g.nkind.as<uint8_t>();
+      er.bgeo = g.bgeo.as<BeamGeo>();
+      er.c = lvbuf<T>(l1, 0);
+      launch_apply<T>(c, x, er);
+      return;
+    }
+    EpiResid<T, T> er;
+    er.done = done;
+    er.partial = c.partial.as<double>();
+    er.b = b;
+    er.r = u;
+    launch_apply<T>(c, x, er);
+    k_restrict1<T><<<nblk(l1.n), 256, 0, st>>>(l1.n, l1.agg_struct.as<uint32_t>(), done, c.agg_ptr.as<int>(),
+                                               c.nodes_by_agg.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
+                                               g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), u, lvbuf<T>(l1, 0));
+  }
+
   // z = M^-1 r at the fine level; partial r.z per brick into c.partial
   // rdot: if set, the partial is rdot . z in FP64 instead of r . z
   template <class T>
@@ -2705,16 +2759,8 @@ struct Solver {
       ej.r1 = r1;
       ej.sout = s1;
       launch_apply<T>(c, x1, ej);
-      EpiResid<T, T> er;
-      er.done = done;
-      er.partial = c.partial.as<double>();
-      er.b = r1;
-      er.r = r1;
-      launch_apply<T>(c, s1, er);
       Level& l1 = c.levels[0];
-      k_restrict1<T><<<nblk(l1.n), 256, 0, st>>>(l1.n, l1.agg_struct.as<uint32_t>(), done, c.agg_ptr.as<int>(),
-                                                 c.nodes_by_agg.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
-                                                 g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), r1, lvbuf<T>(l1, 0));
+      residual_restrict<T>(s1, r1, r1, l1);
       descend<T>(0);
       k_prolong1_set<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done,
                                                c.label1.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
@@ -2726,16 +2772,8 @@ struct Solver {
       ep.x = x1;
       launch_apply<T>(c, s1, ep);
     } else if (!c.levels.empty()) {
-      EpiResid<T, T> er;
-      er.done = done;
-      er.partial = c.partial.as<double>();
-      er.b = r;
-      er.r = r1;
-      launch_apply<T>(c, x1, er);
       Level& l1 = c.levels[0];
-      k_restrict1<T><<<nblk(l1.n), 256, 0, st>>>(l1.n, l1.agg_struct.as<uint32_t>(), done, c.agg_ptr.as<int>(),
-                                                 c.nodes_by_agg.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
-                                                 g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), r1, lvbuf<T>(l1, 0));
+      residual_restrict<T>(x1, r, r1, l1);
       descend<T>(0);
       k_prolong1<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done, c.label1.as<int>(),
                                            l1.centre.as<double>(), g.nkey.as<uint32_t>(), g.nkind.as<uint8_t>(),
@@ -2903,6 +2941,7 @@ struct Solver {
 
 void release_levels(Ctx& c) {
   c.hier_valid = false;
+  c.restrict_fused = false;
   for (auto& lv : c.levels)
     for (DBuf* b : {&lv.rowptr, &lv.col, &lv.a64, &lv.a32, &lv.dinv64, &lv.dinv32, &lv.centre, &lv.parent, &lv.rep,
                     &lv.key, &lv.kid_ptr, &lv.kids, &lv.agg_struct, &lv.step_d, &lv.r64, &lv.x64, &lv.t64, &lv.u64,
@@ -2921,7 +2960,7 @@ void release_all(Ctx& c) {
     b->release();
   for (DBuf* b : {&c.fres, &c.xsol, &c.dinv64, &c.dinv32, &c.step0_d, &c.label1, &c.uf, &c.uf2, &c.flag, &c.scan_tmp, &c.sort_tmp,
                   &c.nodes_by_agg, &c.agg_ptr, &c.f64, &c.x64, &c.r64, &c.p64, &c.ap64, &c.z64, &c.t64, &c.u64, &c.x32,
-                  &c.r32, &c.p32, &c.ap32, &c.z32, &c.t32, &c.partial, &c.scal, &c.done, &c.w64, &c.w32, &c.p0row,
+                  &c.r32, &c.p32, &c.ap32, &c.z32, &c.t32, &c.partial, &c.scal, &c.done, &c.w64, &c.w32, &c.brick_agg1, &c.p0row,
                   &c.p0col, &c.p0val, &c.p0t_ptr, &c.p0t_blk, &c.p0blk_row})
     b->release();
   release_levels(c);

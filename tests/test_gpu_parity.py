@@ -153,6 +153,25 @@ def test_solve_parity_reference_scenes(ctx, scene):
     assert abs(u_dev.max() - u_or.max()) <= 1e-5 * u_or.max()
 
 
+def test_fused_restriction_matches(ctx, monkeypatch):
+    """The level-1 restriction fused into the operator's epilogue against the separate kernel:
+    the same sums in the same order per aggregate, compiled inside another kernel (rounding may
+    differ in the last bits): the same iterations and the same solution to 1e-8."""
+    monkeypatch.setenv("STATICS_HIP_SCHEDULE", "1")
+    out = []
+    for fuse in (True, False):
+        if not fuse:
+            monkeypatch.setenv("STATICS_HIP_NOFUSE", "1")
+        P = generated(ctx, 24, 6)
+        classify_both(P)
+        for prec in (sh.FP64, sh.MIXED):
+            x, st = ctx.solve(P.to_dev(gravity_z(P.L)), precision=prec, tol=1e-10)
+            assert st.status == 0
+            out.append((prec, st.iters, x))
+    for (p1, i1, x1), (p2, i2, x2) in zip(out[:2], out[2:]):
+        assert p1 == p2 and i1 == i2 and rel(x1, x2) < 1e-8, (p1, i1, i2, rel(x1, x2))
+
+
 def test_reference_stopping_rule(ctx):
     """refresh < 0: the recursive residual decides (bodies.rs:350-352); the solution still meets
     the bar and rel_residual reports the true residual."""
