@@ -18,6 +18,7 @@ sample of the same generator, scaled to 256^3 by its degrees of freedom.
 """
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -54,7 +55,8 @@ def _pmc_per_dispatch(csv_path, counter):
             v = float(row["Counter_Value"])
             if ROOFLINE_EPILOGUE in name:
                 apply.append(v)
-                names.add("k_apply_cells" if "k_apply_cells" in name else "k_apply")
+                m = re.search(r"(k_apply\w*)<", name)
+                names.add(m.group(1) if m else "k_apply")
             elif "copyBuffer" in name:
                 copies.append((int(row["Dispatch_Id"]), v))
     if len(names) > 1:
