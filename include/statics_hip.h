@@ -64,8 +64,8 @@ typedef struct {
 
 /* ST_FP64: everything in FP64. ST_MIXED: the CG (vectors, K p, inner products,
  * step lengths) stays in FP64 and the V-cycle preconditioner runs in FP32 on the
- * fine level and the first two coarse levels (their operators, transfers and
- * smoothers), FP64 above. Both recompute the FP64 true residual when the
+ * fine level and the first four coarse levels (their operators, transfers and
+ * smoothers), FP64 above and in the dense top. Both recompute the FP64 true residual when the
  * recursive one meets the tolerance, and go on from it if it does not. */
 typedef enum { ST_FP64 = 0, ST_MIXED = 1 } StPrecision;
 

@@ -181,7 +181,7 @@ struct Ctx {
   int gamma_to = 1000;   // ... up to this index
   std::vector<int> gammas;  // if set: cycles per visit of each level index (1 beyond the list);
                             // if empty, gamma on the indices (gamma_from, gamma_to]
-  int f64_from = 2;      // first level (0: level 1) kept in FP64 by the mixed V-cycle
+  int f64_from = 4;      // first level (0: level 1) kept in FP64 by the mixed V-cycle
   bool graphs = false;   // replay each CG iteration's tail as a captured graph
   bool dense_top = true; // invert a small coarse level densely and drop the levels above
   int apply_sched[2] = {0, 0};  // operator schedule per precision (FP32, FP64), see kSchedules
