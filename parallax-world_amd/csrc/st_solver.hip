@@ -3024,9 +3024,11 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   // gamma-cycles: each visit of coarse level index i runs gammas[i] cycles (the later
   // ones on the remaining residual). Measured at 256^3 (mixed, 1e-8): one cycle
   // everywhere 360 iterations; two on levels 3-6, 144; (1, 2, 4, 2, 2) on levels 1-5,
-  // 84 iterations and the least time. STATICS_HIP_GAMMAS="a,b,..." sets the list;
-  // STATICS_HIP_GAMMA / GFROM / GTO set one count for a range of level indices instead.
-  c.gammas = {1, 2, 4, 2, 2};
+  // 84 iterations and the least time (level 6 is its dense top). On the 512^3 world
+  // (10 levels) two cycles on level 6 as well take 192 iterations to 168.
+  // STATICS_HIP_GAMMAS="a,b,..." sets the list; STATICS_HIP_GAMMA / GFROM / GTO set
+  // one count for a range of level indices instead.
+  c.gammas = {1, 2, 4, 2, 2, 2};
   c.gamma = 2;
   c.gamma_from = 1;
   c.gamma_to = 5;
