@@ -105,6 +105,31 @@ const char* st_last_error(const StCtx* ctx);
  * into device bricks and numbers the face nodes; no lattice of bonds is ever
  * materialised. */
 int st_upload(StCtx* ctx, const StCells* cells);
+/* A world tile as the reference's brick grid, lattice::Grid
+ * (lattice/src/lib.rs:182-205), as its caller holds it: */
+typedef struct {
+  int32_t min[3];         /* the box's least site (Grid.min) */
+  int32_t n[3];           /* the box's size in sites (Grid.n) */
+  const int32_t* bricks;  /* Grid.bricks: per 8^3 place, z-major, then y, then x, where its
+                             sites start in Grid.cells in units of 512; -1 if empty */
+  int64_t nbricks;        /* length of bricks: ceil(n/8) over the three axes */
+  const uint32_t* solid;  /* Grid.solid: bit i % 32 of word i / 32 for site i of Grid.cells */
+  int64_t nsites;         /* length of Grid.cells (a multiple of 512) */
+} StGrid;
+
+/* Replaces: statics::lattice::solids + hold_bulk (statics/src/lattice.rs:87-119)
+ * for a tile held as a lattice::Grid. The cells are the sites whose Grid.solid bit
+ * is set (Grid::solid, lattice/src/lib.rs:332-335), in solids' order (x outermost,
+ * z innermost); each is held if every site within hold_radius metres (at spacing
+ * UNIT * 2^level) is a cell, as hold_bulk decides it. Both run on the device.
+ * Writes the count to *count_out; with sites_out (3 int32 per cell, grid units),
+ * held_out (1 per cell) and index_out (the cell's index in Grid.cells, for its
+ * material) not NULL, also the cells, each array sized for the count (a first
+ * call with them NULL gives it). The result is st_upload's input, with e and g
+ * per cell from the caller's materials. */
+int st_grid_cells(StCtx* ctx, const StGrid* grid, int32_t level, double hold_radius, int64_t* count_out,
+                  int32_t* sites_out, uint8_t* held_out, int64_t* index_out);
+
 /* Counts of what st_upload built. */
 int st_sizes(StCtx* ctx, int64_t* nodes, int64_t* cells, int64_t* bricks);
 /* Each node's face, 4 int32 per node: o, x, y, z (the + side cell), in device order. */

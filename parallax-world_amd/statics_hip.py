@@ -22,7 +22,7 @@ VALUE_ONLY = 256  # or-ed into the objective: J alone
 EXPORTS = ["st_create", "st_destroy", "st_last_error", "st_upload", "st_sizes", "st_node_faces", "st_lookup_faces",
            "st_node_held", "st_joined_to_held", "st_set_gone", "st_set_beams", "st_apply", "st_solve",
            "st_set_loads", "st_solve_resident", "st_get_solution", "st_classify", "st_kernel_timing_enable",
-           "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_bond_used", "st_settle", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness"]
+           "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_bond_used", "st_settle", "st_grid_cells", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness"]
 
 UNIT = 0.125  # metres to a level-0 cell (statics/src/lattice.rs:16)
 
@@ -31,6 +31,11 @@ class StCells(ctypes.Structure):
     _fields_ = [("n_cells", ctypes.c_int64), ("sites", ctypes.c_void_p), ("held", ctypes.c_void_p),
                 ("e", ctypes.c_void_p), ("g", ctypes.c_void_p), ("beams", ctypes.c_void_p),
                 ("spacing", ctypes.c_double)]
+
+
+class StGrid(ctypes.Structure):
+    _fields_ = [("min", ctypes.c_int32 * 3), ("n", ctypes.c_int32 * 3), ("bricks", ctypes.c_void_p),
+                ("nbricks", ctypes.c_int64), ("solid", ctypes.c_void_p), ("nsites", ctypes.c_int64)]
 
 
 class StSolveOpts(ctypes.Structure):
@@ -82,6 +87,7 @@ def lib():
             "st_solve": [vp, vp, ctypes.POINTER(StSolveOpts), vp, ctypes.POINTER(StStats)],
             "st_bond_forces": [vp, vp, vp],
             "st_bond_used": [vp, vp, vp, vp],
+            "st_grid_cells": [vp, ctypes.POINTER(StGrid), ctypes.c_int32, dbl, vp, vp, vp, vp],
             "st_settle": [vp, vp, vp, ctypes.POINTER(StSolveOpts), vp, vp, vp, ctypes.POINTER(StStats)],
             "st_backward": [vp, ctypes.c_int, dbl, ctypes.POINTER(StSolveOpts), vp, vp, ctypes.POINTER(StStats)],
             "st_solve_batch": [vp, i32, vp, vp, ctypes.POINTER(StSolveOpts), vp, vp],
@@ -242,6 +248,22 @@ class Context:
         out = np.zeros((len(self._keep[0]), 15, 4, 3))
         self._check(self.L.st_bond_forces(self.h, _p(x), _p(out)), "st_bond_forces")
         return out
+
+    def grid_cells(self, min_site, n, bricks, solid, nsites, level=0, hold_radius=0.0):
+        """solids + hold_bulk for a lattice::Grid: (sites (m, 3) int32, held (m,), index into Grid.cells (m,))."""
+        bricks = np.ascontiguousarray(bricks, np.int32)
+        solid = np.ascontiguousarray(solid, np.uint32)
+        gr = StGrid((ctypes.c_int32 * 3)(*min_site), (ctypes.c_int32 * 3)(*n), _p(bricks), len(bricks), _p(solid),
+                    int(nsites))
+        m = ctypes.c_int64()
+        self._check(self.L.st_grid_cells(self.h, ctypes.byref(gr), level, hold_radius, ctypes.byref(m), None, None,
+                                         None), "st_grid_cells")
+        sites = np.zeros((m.value, 3), np.int32)
+        held = np.zeros(m.value, np.uint8)
+        index = np.zeros(m.value, np.int64)
+        self._check(self.L.st_grid_cells(self.h, ctypes.byref(gr), level, hold_radius, ctypes.byref(m), _p(sites),
+                                         _p(held), _p(index)), "st_grid_cells")
+        return sites, held, index
 
     def bond_used(self, strength, x):
         """Per uploaded cell [15][4] used shares; strength: (cells, 3) ft, fc, fs."""

@@ -403,3 +403,66 @@ def test_edge_cases(ctx):
         ctx.upload(np.array([[0, 0, 0], [0, 0, 0]]), np.zeros(2, np.uint8), np.ones(2), np.ones(2))
     with pytest.raises(sh.StaticsError):
         ctx.upload(np.array([[0, 0, 0]]), np.zeros(1, np.uint8), -np.ones(1), np.ones(1))
+
+
+def grid_decode(min_site, n, bricks, solid, nsites):
+    """Grid::solid over the box in solids' order (lattice/src/lib.rs:293-335; statics/src/lattice.rs:87-102)."""
+    nb = [(v + 7) // 8 for v in n]
+    sites, index = [], []
+    for x in range(n[0]):
+        for y in range(n[1]):
+            for z in range(n[2]):
+                at = bricks[((z >> 3) * nb[1] + (y >> 3)) * nb[0] + (x >> 3)]
+                if at < 0:
+                    continue
+                i = at * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7)
+                if i < nsites and (solid[i >> 5] >> (i & 31)) & 1:
+                    sites.append((x + min_site[0], y + min_site[1], z + min_site[2]))
+                    index.append(i)
+    return np.array(sites, np.int64).reshape(-1, 3), np.array(index, np.int64)
+
+
+@pytest.mark.parametrize("level,radius", [(0, 0.2), (0, 0.3), (1, 0.5)])
+def test_grid_cells_parity(ctx, level, radius):
+    """st_grid_cells against Grid's own indexing and the oracle's hold_bulk (bit-exact): a box
+    that is not a whole number of bricks, empty and shared bricks, an offset origin."""
+    rng = np.random.default_rng(level * 10 + int(radius * 10))
+    n = (37, 29, 45)
+    min_site = (-5, 3, 10)
+    nb = [(v + 7) // 8 for v in n]
+    nstarts = 30
+    bricks = rng.integers(0, nstarts, size=nb[0] * nb[1] * nb[2]).astype(np.int32)
+    bricks[rng.random(len(bricks)) < 0.15] = -1
+    nsites = nstarts * 512
+    bits = (rng.random(nsites) < 0.9).astype(np.uint64)
+    solid = np.zeros(nsites // 32, np.uint32)
+    for k in range(32):
+        solid |= (bits[k::32] << np.uint64(k)).astype(np.uint32)
+    sites, held, index = ctx.grid_cells(min_site, n, bricks, solid, nsites, level=level, hold_radius=radius)
+    want_sites, want_index = grid_decode(min_site, n, bricks, solid, nsites)
+    assert (sites == want_sites).all() and (index == want_index).all()
+    want_held = orc.hold_bulk(level, want_sites.astype(np.int64) << level, radius)
+    assert (held == want_held).all() and 0 < held.sum() < len(held)
+
+
+def test_grid_cells_block():
+    """A 3^3 block in one brick: with hold radius one cell only the centre is held; the cells
+    upload and solve."""
+    ctx = sh.Context()
+    solid = np.zeros(512 // 32, np.uint32)
+    for x in range(2, 5):
+        for y in range(1, 4):
+            for z in range(3, 6):
+                i = (z * 8 + y) * 8 + x
+                solid[i >> 5] |= np.uint32(1 << (i & 31))
+    sites, held, index = ctx.grid_cells((0, 0, 0), (8, 8, 8), np.array([0], np.int32), solid, 512, 0, 0.125)
+    assert len(sites) == 27 and held.sum() == 1 and tuple(sites[held == 1][0]) == (3, 2, 4)
+    assert (sites[:, 0][:-1] <= sites[:, 0][1:]).all()  # x outermost
+    e = np.ones(len(sites))
+    ctx.upload(sites, held, e, 0.4 * e)
+    ctx.joined_to_held()
+    f = np.zeros((ctx.nodes, 6))
+    f[:, 2] = -1.0
+    f[ctx.node_held().astype(bool)] = 0
+    x, st = ctx.solve(f, tol=1e-10)
+    assert st.status == 0 and np.isfinite(x).all()
