@@ -425,6 +425,66 @@ def settle_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def grid_of(occ):
+    """The occupied sites of occ (x, y, z) as a lattice::Grid (lattice/src/lib.rs:182-205): 8^3
+    places z-major, each non-empty one its own 512 sites, z-major within; Grid.solid bits."""
+    n = occ.shape
+    nb = [(v + 7) // 8 for v in n]
+    occ_p = np.zeros([8 * v for v in nb], np.uint8)
+    occ_p[:n[0], :n[1], :n[2]] = occ > 0
+    blocks = occ_p.reshape(nb[0], 8, nb[1], 8, nb[2], 8).transpose(4, 2, 0, 5, 3, 1).reshape(-1, 512)
+    nonempty = blocks.any(axis=1)
+    bricks = np.where(nonempty, np.cumsum(nonempty) - 1, -1).astype(np.int32)
+    bits = blocks[nonempty].reshape(-1)
+    solid = np.packbits(bits, bitorder="little").view(np.uint32)
+    return bricks, solid, len(bits)
+
+
+def ingest_bench(args):
+    """solids + hold_bulk from a lattice::Grid on the device (st_grid_cells, the count call and
+    the fill call) for the --ingest^3 structure of the bench's generator; the oracle's
+    hold_bulk on a 128^3 sample as the CPU baseline (scaled by cells)."""
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    n, radius = args.ingest, 0.3
+    occ, _ = structure(n, args.seed, spacing=args.spacing)
+    bricks, solid, nsites = grid_of(occ)
+    del occ
+    ctx = sh.Context()
+    times = []
+    for i in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        sites, held, index = ctx.grid_cells((0, 0, 0), (n, n, n), bricks, solid, nsites, 0, radius)
+        dt = time.perf_counter() - t0
+        if i >= args.warmup:
+            times.append(dt)
+        log(f"[bench] ingest {i}: {dt * 1e3:.1f} ms, {len(sites)} cells, {int(held.sum())} held")
+    ms = 1e3 * sum(times) / len(times)
+    cpu = None
+    if args.cpu_sample > 0:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle as orc
+        m = min(args.cpu_sample * 2, n)
+        occ_s, _ = structure(m, args.seed, spacing=args.spacing)
+        sx, sy, sz = np.nonzero(occ_s > 0)
+        s_sites = np.stack([sx, sy, sz], axis=1).astype(np.int64)
+        t0 = time.perf_counter()
+        orc.hold_bulk(0, s_sites, radius)
+        sec = time.perf_counter() - t0
+        cpu = {"value": len(s_sites) / sec, "unit": "cells/sec", "cores": 1, "kind": "port",
+               "sample": f"oracle hold_bulk (statics/src/lattice.rs:107-119 restated, hash set) on the {m}^3 "
+                         f"structure's {len(s_sites)} cells, radius {radius}: {sec:.2f} s (the cells are given, "
+                         f"no grid decode)"}
+    out = {"metric": f"cells/sec classified from a lattice::Grid ({n}^3 tile: solids + hold_bulk on the device)",
+           "value": len(sites) / (ms * 1e-3), "unit": "cells/sec", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u32 bits", "data": "synthetic",
+           "config": {"workload": f"the bench generator's {n}^3 structure as a lattice::Grid, hold radius {radius} m; "
+                                  f"step = st_grid_cells count + fill, host copies included",
+                      "sites": n ** 3, "cells": int(len(sites)), "held": int(held.sum())},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -446,6 +506,8 @@ def main():
                     help="instead: the gradient check, d(compliance) and d(soft-max stress)/d(log-stiffness) "
                          "against central differences (eps 1e-4, FP64 solves to 1e-10) at this many cells "
                          "of a --size structure (64 at 128: the adjoint config)")
+    ap.add_argument("--ingest", type=int, default=0, metavar="N",
+                    help="instead: solids + hold_bulk from a lattice::Grid of one N^3 tile (st_grid_cells)")
     ap.add_argument("--settle", type=int, default=0, metavar="N",
                     help="instead: bodies::settle on the device for one N^3 structure (see settle_bench)")
     ap.add_argument("--settle-cpu-sample", type=int, default=32, help="settle: edge of the CPU oracle's sample")
@@ -474,6 +536,9 @@ def main():
         return
     if args.settle:
         settle_bench(args)
+        return
+    if args.ingest:
+        ingest_bench(args)
         return
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
 
