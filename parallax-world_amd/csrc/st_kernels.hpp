@@ -287,6 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_cells(GeoD<T> g, const uint32_
   for (uint32_t i = threadIdx.x; i < 6 * nn; i += blockDim.x) yl[i] = T(0);
   const BeamGeo G = g.bgeo[s];
   __syncthreads();
+  double part = 0.0;
   for (uint32_t c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
     const uint2* fp = reinterpret_cast<const uint2*>(g.cfaces + 6 * (size_t)c);
     const uint2 f01 = fp[0], f23 = fp[1], f45 = fp[2];
@@ -305,6 +306,12 @@ __global__ __launch_bounds__(kBlock) void k_apply_cells(GeoD<T> g, const uint32_
     }
     const T e = g.ce[c], gg = g.cg[c];
     cell_apply<T>(cell_coef<T>(G, 0, e, gg), cell_coef<T>(G, 1, e, gg), (uint32_t)g.cmask[c], xv, yv);
+    if constexpr (Epi::kCellDot) {  // every (cell, face) counted once, in the cell's own brick
+#pragma unroll
+      for (int f = 0; f < 6; f++)
+#pragma unroll
+        for (int k = 0; k < 6; k++) part += (double)xv[f][k] * (double)yv[f][k];
+    }
 #pragma unroll
     for (int f = 0; f < 6; f++) {
       if (fi[f] & kNotFree) continue;
@@ -333,14 +340,18 @@ __global__ __launch_bounds__(kBlock) void k_apply_cells(GeoD<T> g, const uint32_
     for (int k = 0; k < 6; k++) atomicAdd(&yl[6 * l + k], y6[k]);
   }
   __syncthreads();
-  double part = 0.0;
   for (uint32_t l = threadIdx.x; l < nn; l += blockDim.x) {
     const uint32_t v = n0 + l;
-    T y[6], xo[6];
+    T y[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) y[k] = yl[6 * l + k];
-    load6<T>(x + 6 * (size_t)v, xo);
-    part += epi.node(v, y, s, xo);
+    if constexpr (Epi::kCellDot) {
+      epi.store(v, y);
+    } else {
+      T xo[6];
+      load6<T>(x + 6 * (size_t)v, xo);
+      part += epi.node(v, y, s, xo);
+    }
   }
   if (Epi::kReduce) {
     const double tot = block_sum(part, red);
@@ -388,6 +399,10 @@ struct EpiBase {
   // workgroup runs tail(b, s) once more over the brick
   static constexpr bool kTail = false;
   __device__ void tail(uint32_t, uint32_t) const {}
+  // kCellDot: the reduced value is x.(K x), which the cell-by-cell schedule sums cell by
+  // cell from the motions and loads it already holds (x^T K x = sum over cells of
+  // x_c . (K_c x_c)), so that its epilogue only stores the row (store()).
+  static constexpr bool kCellDot = false;
 };
 
 template <class T>
@@ -404,7 +419,9 @@ struct EpiStore : EpiBase {
 template <class T>
 struct EpiDot : EpiBase {
   static constexpr bool kReduce = true;
+  static constexpr bool kCellDot = true;
   T* y;
+  __device__ void store(uint32_t n, const T* a) const { store6<T>(y + 6 * (size_t)n, a); }
   __device__ double node(uint32_t n, const T* a, uint32_t, const T* pn) const {
     T pv[6];
     load6<T>(pn, pv);
