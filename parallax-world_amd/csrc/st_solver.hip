@@ -506,11 +506,18 @@ __global__ void k_cg_check(int ns, double* scal, uint8_t* done, int* status, int
     done[s] = 1;
   }
 }
-__global__ void k_cg_beta(int ns, double* scal, const uint8_t* done) {
+// r.z <= 0 while r is not yet small means the preconditioner is not positive on r (it can
+// lose definiteness on nearly singular structures): the structure stops as not SPD.
+__global__ void k_cg_beta(int ns, double* scal, uint8_t* done, int* status) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns || done[s]) return;
   double* S = scal + kScal * s;
   const double rzn = S[S_RZNEW];
+  if (!(rzn > 0.0) && S[S_RR] > S[S_GOAL2]) {
+    status[s] = ST_NOT_SPD;
+    done[s] = 1;
+    return;
+  }
   S[S_BETA] = S[S_RZ] != 0.0 ? rzn / S[S_RZ] : 0.0;
   S[S_RZ] = rzn;
 }
@@ -2878,7 +2885,7 @@ struct Solver {
         k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), refresh < 0 ? 0 : 1,
                                              swant.as<uint8_t>(), 0.0);
         precond(S_RZNEW);
-        k_cg_beta<<<nblk(ns), 256, 0, st>>>(ns, scal, done);
+        k_cg_beta<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>());
         if (mixed)
           k_update_p<double, float><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z32, p);
         else
@@ -3080,6 +3087,30 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
                    (long long)(l ? c.levels[l - 1].pblocks : c.p0blocks));
   }
   if (S.ndof > 0) S.pcg(o.precision, o.tol, o.max_iter, o.refresh);
+  // The fast cycle settings (gamma counts up to 4, FP32 on coarse levels 3-4) can lose
+  // definiteness on nearly singular structures, such as a settle round where the structure
+  // comes apart. A solve they fail is repeated once, from zero, with the robust settings:
+  // two cycles on levels 3-6 and FP64 from level 3 (a rebuilt hierarchy).
+  std::vector<int> it_first(ns, 0);
+  if (S.ndof > 0 && !std::getenv("STATICS_HIP_NORETRY")) {
+    std::vector<int> sts(ns);
+    ST_HIP(hipMemcpy(sts.data(), S.sstatus.p, ns * sizeof(int), hipMemcpyDeviceToHost));
+    const bool bad = std::any_of(sts.begin(), sts.end(), [](int v) { return v == ST_NOT_SPD || v == ST_NO_CONVERGENCE; });
+    const std::vector<int> robust = {1, 1, 2, 2, 2, 2};
+    if (bad && (c.gammas != robust || c.f64_from > 2)) {
+      ST_HIP(hipMemcpy(it_first.data(), S.siters.p, ns * sizeof(int), hipMemcpyDeviceToHost));
+      if (std::getenv("STATICS_HIP_VERBOSE"))
+        std::fprintf(stderr, "statics_hip: solve failed with the fast cycle settings; again with the robust ones\n");
+      c.gammas = robust;
+      c.f64_from = std::min(c.f64_from, 2);
+      S.setup_fine();
+      S.setup_hierarchy();
+      c.hier_valid = true;
+      c.hier_key = Ctx::HierKey{S.mixed, c.smoothed, c.dense_top, c.sa_levels, c.f64_from};
+      ST_HIP(hipMemsetAsync(c.x64.p, 0, (size_t)c.geo.nnodes * 6 * sizeof(double), c.stream));
+      S.pcg(o.precision, o.tol, o.max_iter, o.refresh);
+    }
+  }
   S.true_residual();
   ST_HIP(hipEventRecord(e2, c.stream));
   ST_HIP(hipEventSynchronize(e2));
@@ -3101,7 +3132,7 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   for (int s = 0; s < ns; s++) {
     StStats& t = stats[s];
     const StructInfo& si = c.host.structs[s];
-    t.iters = it[s];
+    t.iters = it[s] + it_first[s];
     t.levels = 1 + (int)c.levels.size();
     t.refreshes = rf[s];
     const double ff = sc[(size_t)s * kScal + S_FF], rr = sc[(size_t)s * kScal + S_RRTRUE];
