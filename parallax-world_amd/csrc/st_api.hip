@@ -352,14 +352,6 @@ double host_sum(const std::vector<double>& v) {
   return s;
 }
 
-}  // namespace
-}  // namespace st
-
-using namespace st;
-
-extern "C" {
-
-namespace {
 void set_loads(StCtx* c, const double* loads) {
   const int64_t nn = c->geo.nnodes;
   const size_t bytes = (size_t)std::max<int64_t>(nn, 1) * 6 * sizeof(double);
@@ -395,7 +387,32 @@ std::vector<StStats> solve_resident(StCtx* c, const StSolveOpts& o) {
   c->last_opts = o;
   return out;
 }
+
+// ft, fc, fs per device cell from 3 per input cell (held cells have no beams and are skipped).
+void upload_strength(StCtx* c, const double* strength, DBuf& out) {
+  const auto& coi = c->host.cell_of_input;
+  const int64_t ncl = c->geo.ncells;
+  std::vector<double> h((size_t)std::max<int64_t>(ncl, 1) * 3, 1.0);
+  for (size_t i = 0; i < coi.size(); i++) {
+    const int64_t k = coi[i];
+    if (k < 0) continue;
+    for (int t = 0; t < 3; t++) {
+      const double v = strength[3 * i + t];
+      if (!(v > 0) || !std::isfinite(v))
+        throw StError{ST_BAD_INPUT, "strength: a free cell has a non-positive or non-finite ft, fc or fs"};
+      h[3 * (size_t)k + t] = v;
+    }
+  }
+  out.ensure(h.size() * sizeof(double));
+  ST_HIP(hipMemcpyAsync(out.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+}
 }  // namespace
+}  // namespace st
+
+using namespace st;
+
+extern "C" {
+
 
 int st_set_loads(StCtx* c, const double* loads) {
   return guard(c, [&] {
@@ -558,26 +575,6 @@ int st_bond_forces(StCtx* c, const double* x, double* out) {
   });
 }
 
-namespace {
-// ft, fc, fs per device cell from 3 per input cell (held cells have no beams and are skipped).
-void upload_strength(StCtx* c, const double* strength, DBuf& out) {
-  const auto& coi = c->host.cell_of_input;
-  const int64_t ncl = c->geo.ncells;
-  std::vector<double> h((size_t)std::max<int64_t>(ncl, 1) * 3, 1.0);
-  for (size_t i = 0; i < coi.size(); i++) {
-    const int64_t k = coi[i];
-    if (k < 0) continue;
-    for (int t = 0; t < 3; t++) {
-      const double v = strength[3 * i + t];
-      if (!(v > 0) || !std::isfinite(v))
-        throw StError{ST_BAD_INPUT, "strength: a free cell has a non-positive or non-finite ft, fc or fs"};
-      h[3 * (size_t)k + t] = v;
-    }
-  }
-  out.ensure(h.size() * sizeof(double));
-  ST_HIP(hipMemcpyAsync(out.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-}
-}  // namespace
 
 int st_bond_used(StCtx* c, const double* strength, const double* x, double* out) {
   return guard(c, [&] {
