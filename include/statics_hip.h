@@ -115,11 +115,19 @@ typedef struct {
   int64_t nbricks;        /* length of bricks: ceil(n/8) over the three axes */
   const uint32_t* solid;  /* Grid.solid: bit i % 32 of word i / 32 for site i of Grid.cells */
   int64_t nsites;         /* length of Grid.cells (a multiple of 512) */
+  /* Or a worldgen::grid::VoxelGrid (worldgen/src/grid.rs:69-81, the same brick layout
+   * and site indexing, worldgen/src/grid.rs:530-537): with solid_bytes not NULL, a
+   * site is a cell when VoxelGrid.solid (how much of ALPHA = 255 of the site stops a
+   * walking body) is at least solid_min, and `solid` is not read; nsites is then the
+   * length of VoxelGrid.solid. */
+  const uint8_t* solid_bytes;
+  uint8_t solid_min;
 } StGrid;
 
 /* Replaces: statics::lattice::solids + hold_bulk (statics/src/lattice.rs:87-119)
- * for a tile held as a lattice::Grid. The cells are the sites whose Grid.solid bit
- * is set (Grid::solid, lattice/src/lib.rs:332-335), in solids' order (x outermost,
+ * for a tile held as a lattice::Grid (or a VoxelGrid, above). The cells are the
+ * sites whose Grid.solid bit is set (Grid::solid, lattice/src/lib.rs:332-335) or
+ * whose VoxelGrid.solid byte reaches solid_min, in solids' order (x outermost,
  * z innermost); each is held if every site within hold_radius metres (at spacing
  * UNIT * 2^level) is a cell, as hold_bulk decides it. Both run on the device.
  * Writes the count to *count_out; with sites_out (3 int32 per cell, grid units),

@@ -1,7 +1,8 @@
 // Cells of a world tile held as the reference's brick grid (lattice::Grid,
-// lattice/src/lib.rs:182-205): which sites are cells, in the order statics'
-// `solids` takes them, and which of them hold_bulk holds (statics/src/lattice.rs:107-119),
-// both on the device.
+// lattice/src/lib.rs:182-205, or worldgen's VoxelGrid, worldgen/src/grid.rs:69-81, which
+// shares its brick layout): which sites are cells, in the order statics' `solids` takes
+// them, and which of them hold_bulk holds (statics/src/lattice.rs:107-119), both on the
+// device.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -18,11 +19,14 @@ struct GridView {
   int n0, n1, n2;   // the box's size in sites
   int nb0, nb1;     // bricks along x and y
   const int32_t* bricks;
-  const uint32_t* solid;
+  const uint32_t* solid;  // a bit per site (lattice::Grid), or
+  const uint8_t* bytes;   // a byte per site (worldgen's VoxelGrid), a cell from bmin up
+  uint8_t bmin;
   int64_t nsites;
 };
 
-// Grid::solid at a site relative to the box's least one (place, index: lattice/src/lib.rs:293-313)
+// Grid::solid (or VoxelGrid::solid >= bmin) at a site relative to the box's least one
+// (place, index: lattice/src/lib.rs:293-313, worldgen/src/grid.rs:530-537)
 __device__ __forceinline__ bool grid_solid(const GridView& g, int x, int y, int z, int64_t* index = nullptr) {
   if (x < 0 || x >= g.n0 || y < 0 || y >= g.n1 || z < 0 || z >= g.n2) return false;
   const int64_t place = ((int64_t)(z >> 3) * g.nb1 + (y >> 3)) * g.nb0 + (x >> 3);
@@ -31,6 +35,7 @@ __device__ __forceinline__ bool grid_solid(const GridView& g, int x, int y, int 
   const int64_t i = (int64_t)at * 512 + (((z & 7) * kGridBrick + (y & 7)) * kGridBrick + (x & 7));
   if (i >= g.nsites) return false;
   if (index) *index = i;
+  if (g.bytes) return g.bytes[i] >= g.bmin;
   return (g.solid[i >> 5] >> (i & 31)) & 1u;
 }
 
@@ -123,17 +128,24 @@ int st_grid_cells(StCtx* c, const StGrid* grid, int32_t level, double hold_radiu
               nb2 = (n2 + kGridBrick - 1) / kGridBrick;
     if (grid->nbricks != (int64_t)nb0 * nb1 * nb2 || (grid->nbricks && !grid->bricks))
       throw StError{ST_BAD_INPUT, "grid cells: bricks does not cover the box (one per 8^3 place)"};
-    if (grid->nsites < 0 || grid->nsites % 512 || (grid->nsites && !grid->solid))
+    const bool bytes = grid->solid_bytes != nullptr;
+    if (bytes && grid->solid_min == 0) throw StError{ST_BAD_INPUT, "grid cells: solid_min must be at least 1"};
+    if (grid->nsites < 0 || (!bytes && (grid->nsites % 512 || (grid->nsites && !grid->solid))))
       throw StError{ST_BAD_INPUT, "grid cells: nsites is not a multiple of 512 or solid is NULL"};
     for (int64_t b = 0; b < grid->nbricks; b++)
       if (grid->bricks[b] >= 0 && ((int64_t)grid->bricks[b] + 1) * 512 > grid->nsites)
-        throw StError{ST_BAD_INPUT, "grid cells: a brick starts past the grid's sites"};
+        throw StError{ST_BAD_INPUT, "grid cells: a brick's sites run past the grid's"};
     ST_HIP(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     DBuf bricks, solid, flag, bsum, boff, sites, index, nearb, held;
     upload_vec(bricks, grid->bricks, (size_t)grid->nbricks, s);
-    upload_vec(solid, grid->solid, (size_t)(grid->nsites / 32), s);
-    GridView g{n0, n1, n2, nb0, nb1, bricks.as<int32_t>(), solid.as<uint32_t>(), grid->nsites};
+    DBuf sbytes;
+    if (bytes)
+      upload_vec(sbytes, grid->solid_bytes, (size_t)grid->nsites, s);
+    else
+      upload_vec(solid, grid->solid, (size_t)(grid->nsites / 32), s);
+    GridView g{n0, n1, n2, nb0, nb1, bricks.as<int32_t>(), solid.as<uint32_t>(), sbytes.as<uint8_t>(),
+               grid->solid_min, grid->nsites};
     const int64_t total = (int64_t)n0 * n1 * n2;
     const int64_t nblk = (total + 1023) / 1024;
     flag.ensure(std::max<int64_t>(total, 1) * sizeof(int));

@@ -35,7 +35,8 @@ class StCells(ctypes.Structure):
 
 class StGrid(ctypes.Structure):
     _fields_ = [("min", ctypes.c_int32 * 3), ("n", ctypes.c_int32 * 3), ("bricks", ctypes.c_void_p),
-                ("nbricks", ctypes.c_int64), ("solid", ctypes.c_void_p), ("nsites", ctypes.c_int64)]
+                ("nbricks", ctypes.c_int64), ("solid", ctypes.c_void_p), ("nsites", ctypes.c_int64),
+                ("solid_bytes", ctypes.c_void_p), ("solid_min", ctypes.c_uint8)]
 
 
 class StSolveOpts(ctypes.Structure):
@@ -249,12 +250,18 @@ class Context:
         self._check(self.L.st_bond_forces(self.h, _p(x), _p(out)), "st_bond_forces")
         return out
 
-    def grid_cells(self, min_site, n, bricks, solid, nsites, level=0, hold_radius=0.0):
-        """solids + hold_bulk for a lattice::Grid: (sites (m, 3) int32, held (m,), index into Grid.cells (m,))."""
+    def grid_cells(self, min_site, n, bricks, solid, nsites, level=0, hold_radius=0.0, solid_min=None):
+        """solids + hold_bulk for a lattice::Grid (solid: its bits as uint32 words) or, with solid_min
+        set, a VoxelGrid (solid: a byte per site): (sites (m, 3) int32, held (m,), index (m,))."""
         bricks = np.ascontiguousarray(bricks, np.int32)
-        solid = np.ascontiguousarray(solid, np.uint32)
-        gr = StGrid((ctypes.c_int32 * 3)(*min_site), (ctypes.c_int32 * 3)(*n), _p(bricks), len(bricks), _p(solid),
-                    int(nsites))
+        if solid_min is None:
+            solid = np.ascontiguousarray(solid, np.uint32)
+            gr = StGrid((ctypes.c_int32 * 3)(*min_site), (ctypes.c_int32 * 3)(*n), _p(bricks), len(bricks),
+                        _p(solid), int(nsites), None, 0)
+        else:
+            solid = np.ascontiguousarray(solid, np.uint8)
+            gr = StGrid((ctypes.c_int32 * 3)(*min_site), (ctypes.c_int32 * 3)(*n), _p(bricks), len(bricks), None,
+                        int(nsites), _p(solid), int(solid_min))
         m = ctypes.c_int64()
         self._check(self.L.st_grid_cells(self.h, ctypes.byref(gr), level, hold_radius, ctypes.byref(m), None, None,
                                          None), "st_grid_cells")

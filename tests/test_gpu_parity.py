@@ -445,6 +445,30 @@ def test_grid_cells_parity(ctx, level, radius):
     assert (held == want_held).all() and 0 < held.sum() < len(held)
 
 
+def test_voxelgrid_cells_parity(ctx):
+    """The same for a worldgen VoxelGrid: a byte per site (how much of it stops a walking
+    body), a cell from the threshold up (worldgen/src/grid.rs:530-537, 617-622)."""
+    rng = np.random.default_rng(77)
+    n = (30, 17, 41)
+    min_site = (4, -9, 0)
+    nb = [(v + 7) // 8 for v in n]
+    nstarts = 24
+    bricks = rng.integers(0, nstarts, size=nb[0] * nb[1] * nb[2]).astype(np.int32)
+    bricks[rng.random(len(bricks)) < 0.1] = -1
+    nsites = nstarts * 512
+    sb = rng.integers(0, 256, size=nsites).astype(np.uint8)
+    sb[rng.random(nsites) < 0.6] = 255
+    thr = 200
+    sites, held, index = ctx.grid_cells(min_site, n, bricks, sb, nsites, level=0, hold_radius=0.2, solid_min=thr)
+    bits = (sb >= thr).astype(np.uint64)
+    solid = np.zeros(nsites // 32, np.uint32)
+    for k in range(32):
+        solid |= (bits[k::32] << np.uint64(k)).astype(np.uint32)
+    want_sites, want_index = grid_decode(min_site, n, bricks, solid, nsites)
+    assert (sites == want_sites).all() and (index == want_index).all()
+    assert (held == orc.hold_bulk(0, want_sites.astype(np.int64), 0.2)).all() and 0 < held.sum() < len(held)
+
+
 def test_grid_cells_block():
     """A 3^3 block in one brick: with hold radius one cell only the centre is held; the cells
     upload and solve."""
