@@ -37,9 +37,11 @@ def host_cpu():
 
 
 def build():
-    """Compiles the restatement with the host's g++ (-O3 -march=native, as timed)."""
+    """Compiles the restatement with the host's g++ (-O3 -march=native, as timed; no FMA
+    contraction, as Rust does not contract)."""
     tmp = LIB_PATH + ".tmp"
-    cmd = ["g++", "-O3", "-march=native", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o", tmp, SRC]
+    cmd = ["g++", "-O3", "-march=native", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o",
+           tmp, SRC]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB_PATH)
     with open(CPU_TAG, "w") as f:
@@ -88,6 +90,14 @@ def lib():
         L.orc_coarse_site.argtypes = [vp, ctypes.c_uint32, vp]
         L.orc_spacing.restype = ctypes.c_double
         L.orc_spacing.argtypes = [ctypes.c_uint32]
+        L.orc_gathered.restype = vp
+        L.orc_gathered.argtypes = [vp, ctypes.c_int]
+        L.orc_level.restype = ctypes.c_uint32
+        L.orc_level.argtypes = [vp]
+        L.orc_mats.argtypes = [vp, vp, vp]
+        L.orc_gather_solve.argtypes = [ctypes.c_size_t, ctypes.c_size_t, vp, vp]
+        L.orc_beams.restype = ctypes.c_size_t
+        L.orc_beams.argtypes = [ctypes.c_size_t, vp, vp, vp, vp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_threads.restype = ctypes.c_int
         _lib = L
@@ -115,6 +125,22 @@ class Lattice:
         self.held_cells = np.ascontiguousarray(held, np.uint8)
         self.h = ctypes.c_void_p(lib().orc_build(level, len(self.sites), _p(self.sites), _p(self.mats),
                                                  _p(self.held_cells)))
+        self._read()
+
+    @classmethod
+    def _of_handle(cls, h):
+        lat = cls.__new__(cls)
+        lat.h = h
+        lat.level = lib().orc_level(h)
+        lat.sites = lat.mats = lat.held_cells = None
+        lat._read()
+        return lat
+
+    def gathered(self, against_full=True):
+        """gather::gathered (statics/src/gather.rs:193-195), or measured(fine, false) (:228)."""
+        return Lattice._of_handle(ctypes.c_void_p(lib().orc_gathered(self.h, 1 if against_full else 0)))
+
+    def _read(self):
         nn, nb = ctypes.c_size_t(), ctypes.c_size_t()
         lib().orc_sizes(self.h, ctypes.byref(nn), ctypes.byref(nb))
         self.n_nodes, self.n_bonds = nn.value, nb.value
@@ -134,6 +160,9 @@ class Lattice:
         self.beam_cell = np.zeros(nbm // 4, np.int64)
         lib().orc_bonds(self.h, _p(self.ij), _p(self.p), _p(self.n), _p(self.a), _p(self.d), _p(self.e), _p(self.g),
                         _p(self.beam_cell))
+        self.node_mat = np.zeros((self.n_nodes, 6))
+        self.bond_mat = np.zeros((nbm, 6))
+        lib().orc_mats(self.h, _p(self.node_mat), _p(self.bond_mat))
 
     def __del__(self):
         try:
@@ -216,8 +245,32 @@ def invert6(m):
     return out
 
 
+def gather_solve(a, rhs):
+    """gather.rs:165-186 on a (n, n + rhs) matrix: the (n, rhs) solutions."""
+    a = np.ascontiguousarray(a, np.float64)
+    n = a.shape[0]
+    x = np.zeros((n, rhs))
+    lib().orc_gather_solve(n, rhs, _p(a), _p(x))
+    return x
+
+
+def beams(at, k, joined):
+    """gather.rs:116-161: [(a, b, stiffness)] for handles at `at`, stiffness k, joined (nh, nh)."""
+    at = np.ascontiguousarray(at, np.float64)
+    nh = len(at)
+    out = np.zeros((max(nh * (nh - 1) // 2, 1), 3))
+    m = lib().orc_beams(nh, _p(at), _p(np.ascontiguousarray(k, np.float64)),
+                        _p(np.ascontiguousarray(joined, np.uint8)), _p(out))
+    return [(int(a), int(b), float(v)) for a, b, v in out[:m]]
+
+
 def coarse_site(p, l):
     p = np.ascontiguousarray(p, np.int64)
     out = np.zeros(3, np.int64)
     lib().orc_coarse_site(_p(p), l, _p(out))
     return out
+
+
+def spacing(level):
+    """statics/src/lattice.rs:34-36"""
+    return lib().orc_spacing(level)

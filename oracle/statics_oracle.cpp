@@ -9,7 +9,10 @@
 //                           jacobi, ask, coarser}, merge, Cholesky, conjugate_gradients,
 //                           cycle, forces, moved, motions, bodies_of, settle, used,
 //                           used_standing, joined_to_held, invert, find
-// Rust semantics mirrored: sums are sequential left folds; `max_by` keeps the
+//   statics/src/gather.rs   Cell::{handle, pieces}, responses, beams, solve, full,
+//                           measured, gathered (Bond::energy inline, model.rs:132-136)
+// Rust semantics mirrored: no contraction of a * b + c (built with -ffp-contract=off);
+// sums are sequential left folds; `max_by` keeps the
 // LAST maximum (invert's pivot); `>>` on i64 floors; f64::round is half away
 // from zero (std::round); HashMaps are only looked up, never iterated.
 //
@@ -35,7 +38,11 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <map>
+#include <mutex>
+#include <stdexcept>
 #include <thread>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -919,6 +926,442 @@ std::vector<double> used_standing(const Lattice& lat, double tol) {
   return used(lat, forces(lat, alive, gone, tol, st));
 }
 
+// ---------------------------------------------------------------- gather.rs
+// A coarser level measured from a finer one. Materials are told apart by their
+// six values here; the reference tells them apart by name, which is the same for
+// its materials (each name has its own values; only bonds' copies are made
+// stronger, and gathering reads nodes' materials).
+
+bool same_mat(const Mat& a, const Mat& b) {
+  return a.rho == b.rho && a.e == b.e && a.g == b.g && a.ft == b.ft && a.fc == b.fc && a.fs == b.fs;
+}
+constexpr Mat STONE_MAT{2500.0, 30e9, 12e9, 5e6, 100e6, 10e6};  // model.rs:53
+
+// bodies.rs:628-634
+size_t find_root(std::vector<size_t>& p, size_t a) {
+  while (p[a] != a) {
+    p[a] = p[p[a]];
+    a = p[a];
+  }
+  return a;
+}
+
+// gather.rs:25-51: free parts (finer nodes inside), faces (named by the cell across),
+// the finer bonds between its places (free parts first, then a handle per face)
+struct GCell {
+  std::vector<size_t> free, across;
+  std::vector<std::array<size_t, 3>> bonds;
+  size_t handle(size_t other) {
+    size_t h = 0;
+    while (h < across.size() && across[h] != other) h++;
+    if (h == across.size()) across.push_back(other);
+    return free.size() + h;
+  }
+  std::vector<size_t> pieces() const {
+    size_t nf = free.size();
+    std::vector<size_t> parent(nf + across.size());
+    for (size_t k = 0; k < parent.size(); k++) parent[k] = k;
+    for (auto& b : bonds) {
+      size_t a = find_root(parent, b[0]), c = find_root(parent, b[1]);
+      parent[a] = c;
+    }
+    std::vector<size_t> out(across.size());
+    for (size_t h = 0; h < across.size(); h++) out[h] = find_root(parent, nf + h);
+    return out;
+  }
+};
+
+// gather.rs:165-186: elimination with partial pivoting (the LAST greatest |a[x][c]|, as
+// max_by), a direction it cannot tell solved as zero. a: n rows of n + rhs.
+std::vector<std::vector<double>> gsolve(std::vector<std::vector<double>> a, size_t rhs) {
+  size_t n = a.size();
+  double scale = 0.0;
+  for (size_t r = 0; r < n; r++) scale = std::max(scale, std::fabs(a[r][r]));
+  auto small = [&](double v) { return std::fabs(v) <= 1e-12 * scale; };
+  for (size_t c = 0; c < n; c++) {
+    size_t p = c;
+    for (size_t x = c + 1; x < n; x++)
+      if (!(std::fabs(a[x][c]) < std::fabs(a[p][c]))) p = x;
+    std::swap(a[c], a[p]);
+    if (small(a[c][c])) continue;
+    std::vector<double> pivot = a[c];
+    for (size_t r = 0; r < n; r++) {
+      double f = a[r][c] / pivot[c];
+      if (r != c && f != 0.0)
+        for (size_t k = c; k < n + rhs; k++) a[r][k] -= f * pivot[k];
+    }
+  }
+  std::vector<std::vector<double>> x(n, std::vector<double>(rhs));
+  for (size_t r = 0; r < n; r++)
+    for (size_t k = 0; k < rhs; k++) x[r][k] = small(a[r][r]) ? 0.0 : a[r][n + k] / a[r][r];
+  return x;
+}
+
+// gather.rs:53-114: the cell's matter as a lattice of its own (handles held at `at`), and
+// the finer bonds' forces for each handle moved alone along each axis, the free parts
+// at rest where that leaves them.
+std::pair<Lattice, std::vector<std::vector<V3>>> responses(const Lattice& fine, const GCell& cell,
+                                                           const std::vector<V3>& at) {
+  size_t nf = cell.free.size(), nh = cell.across.size();
+  Lattice lat;
+  lat.level = fine.level;
+  for (size_t v : cell.free) {
+    Node nd = fine.nodes[v];
+    nd.held = false;
+    lat.nodes.push_back(nd);
+  }
+  for (const V3& x : at) lat.nodes.push_back(Node{x, 0.0, STONE_MAT, true});
+  for (auto& t : cell.bonds) {
+    Bond b = fine.bonds[t[2]];
+    b.i = t[0];
+    b.j = t[1];
+    lat.bonds.push_back(b);
+  }
+  size_t n = 6 * nf, m = 3 * nh;
+  auto column = [&](size_t v, size_t d, V3 p) -> V3 {
+    V3 e{0.0, 0.0, 0.0};
+    e[d % 3] = 1.0;
+    return d < 3 ? e : cross(e, sub(p, lat.nodes[v].x));
+  };
+  auto dof = [&](size_t v, size_t d) -> int64_t {
+    if (v < nf) return (int64_t)(6 * v + d);
+    if (d < 3) return (int64_t)(n + 3 * (v - nf) + d);
+    return -1;
+  };
+  std::vector<std::vector<double>> a(n, std::vector<double>(n + m, 0.0));
+  for (const Bond& b : lat.bonds) {
+    M3 st = b.stiffness();
+    const std::pair<size_t, double> ends[2] = {{b.i, -1.0}, {b.j, 1.0}};
+    for (auto [v, sv] : ends)
+      for (size_t d = 0; d < 6; d++) {
+        int64_t r = dof(v, d);
+        if (r < 0 || (size_t)r >= n) continue;
+        V3 cv = column(v, d, b.p);
+        V3 scv{dot(st[0], cv), dot(st[1], cv), dot(st[2], cv)};
+        for (auto [w, sw] : ends)
+          for (size_t e = 0; e < 6; e++) {
+            int64_t c = dof(w, e);
+            if (c < 0) continue;
+            double sign = (size_t)c < n ? 1.0 : -1.0;
+            a[r][c] += sign * sv * sw * dot(scv, column(w, e, b.p));
+          }
+      }
+  }
+  auto x = gsolve(std::move(a), m);
+  std::vector<std::vector<V3>> cases;
+  for (size_t c = 0; c < m; c++) {
+    std::vector<Six> motion(nf + nh, Six{});
+    for (size_t v = 0; v < nf; v++)
+      for (size_t d = 0; d < 6; d++) motion[v][d] = x[6 * v + d][c];
+    motion[nf + c / 3][c % 3] = 1.0;
+    cases.push_back(moved(lat, motion));
+  }
+  return {std::move(lat), std::move(cases)};
+}
+
+struct Found {
+  size_t a, b;
+  double k, stronger;
+};
+
+// gather.rs:116-161: the beams (a < b, stiffness along each) nearest a stiffness between
+// handles, none negative (least squares, dropping the most negative till none is; the
+// FIRST most negative, as min_by), none between handles the matter does not join.
+std::vector<std::tuple<size_t, size_t, double>> beams(const std::vector<std::vector<double>>& k,
+                                                      const std::vector<V3>& at,
+                                                      const std::function<bool(size_t, size_t)>& joined) {
+  size_t n = 3 * at.size();
+  std::vector<std::pair<size_t, size_t>> pairs;
+  for (size_t a = 0; a < at.size(); a++)
+    for (size_t b = a + 1; b < at.size(); b++)
+      if (joined(a, b)) pairs.push_back({a, b});
+  std::vector<std::vector<double>> units;
+  for (auto [a, b] : pairs) {
+    V3 e = unit(sub(at[b], at[a]));
+    std::vector<double> d(n * n, 0.0);
+    for (size_t r = 0; r < 3; r++)
+      for (size_t c = 0; c < 3; c++) {
+        double v = e[r] * e[c];
+        d[(3 * a + r) * n + 3 * a + c] += v;
+        d[(3 * b + r) * n + 3 * b + c] += v;
+        d[(3 * a + r) * n + 3 * b + c] -= v;
+        d[(3 * b + r) * n + 3 * a + c] -= v;
+      }
+    units.push_back(std::move(d));
+  }
+  std::vector<double> target(n * n);
+  for (size_t i = 0; i < n * n; i++) target[i] = 0.5 * (k[i / n][i % n] + k[i % n][i / n]);
+  auto vdot = [](const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); i++) s += a[i] * b[i];
+    return s;
+  };
+  std::vector<char> on(pairs.size(), 1);
+  for (;;) {
+    std::vector<size_t> idx;
+    for (size_t p = 0; p < pairs.size(); p++)
+      if (on[p]) idx.push_back(p);
+    size_t m = idx.size();
+    std::vector<std::vector<double>> g(m, std::vector<double>(m + 1, 0.0));
+    for (size_t r = 0; r < m; r++) {
+      for (size_t c = 0; c < m; c++) g[r][c] = vdot(units[idx[r]], units[idx[c]]);
+      g[r][m] = vdot(units[idx[r]], target);
+    }
+    auto x = gsolve(std::move(g), 1);
+    int64_t worst = -1;
+    for (size_t r = 0; r < m; r++)
+      if (x[r][0] < 0.0 && (worst < 0 || x[r][0] < x[(size_t)worst][0])) worst = (int64_t)r;
+    if (worst >= 0) {
+      on[idx[(size_t)worst]] = 0;
+      continue;
+    }
+    std::vector<std::tuple<size_t, size_t, double>> out;
+    for (size_t r = 0; r < m; r++)
+      if (x[r][0] > 0.0) out.push_back({pairs[idx[r]].first, pairs[idx[r]].second, x[r][0]});
+    return out;
+  }
+}
+
+using Full = std::array<std::pair<double, double>, 2>;
+Lattice measured(const Lattice& fine, bool against_full);
+
+// lattice.rs:87-102 (solids), for a box all of one material
+std::vector<Solid> solids_full(uint32_t level, V3 lo, V3 hi, Mat mat) {
+  double s = spacing(level);
+  std::vector<Solid> found;
+  for (int64_t a = (int64_t)std::floor(lo[0] / s); a < (int64_t)std::ceil(hi[0] / s); a++)
+    for (int64_t b = (int64_t)std::floor(lo[1] / s); b < (int64_t)std::ceil(hi[1] / s); b++)
+      for (int64_t c = (int64_t)std::floor(lo[2] / s); c < (int64_t)std::ceil(hi[2] / s); c++)
+        found.push_back(Solid{{a << level, b << level, c << level}, mat, false});
+  return found;
+}
+
+// gather.rs:200-223: a full cell's beams by class (section over spacing squared, how much
+// stronger than the material), measured once per material
+Full full(Mat mat) {
+  static std::mutex lock;
+  static std::vector<std::pair<Mat, Full>> known;
+  {
+    std::lock_guard<std::mutex> g(lock);
+    for (auto& k : known)
+      if (same_mat(k.first, mat)) return k.second;
+  }
+  double s = spacing(1);
+  Lattice coarse = measured(build(0, solids_full(0, {-1.5 * s, -1.5 * s, -1.5 * s}, {1.5 * s, 1.5 * s, 1.5 * s}, mat)),
+                            false);
+  double sum[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  const Site zero{0, 0, 0};
+  for (size_t q = 0; q < coarse.bonds.size(); q += PER_BEAM) {
+    const Bond& b = coarse.bonds[q];
+    auto has = [&](size_t v) { return coarse.around[v][0] == zero || coarse.around[v][1] == zero; };
+    if (has(b.i) && has(b.j)) {
+      size_t k = cls(coarse.nodes[b.i].x, coarse.nodes[b.j].x, centre(zero, 1));
+      sum[k][0] += 4.0 * b.a / (s * s);
+      sum[k][1] += b.mat.ft / mat.ft;
+      sum[k][2] += 1.0;
+    }
+  }
+  Full f;
+  for (int k = 0; k < 2; k++)
+    f[k] = sum[k][2] > 0.0 ? std::make_pair(sum[k][0] / sum[k][2], sum[k][1] / sum[k][2]) : std::make_pair(1.0, 1.0);
+  std::lock_guard<std::mutex> g(lock);
+  known.push_back({mat, f});
+  return f;
+}
+
+// gather.rs:225-395
+Lattice measured(const Lattice& fine, bool against_full) {
+  uint32_t level = fine.level + 1;
+  std::unordered_map<Site, size_t, SiteHash> index;
+  std::vector<Site> sites;
+  auto cell_of = [&](Site site) {
+    Site key = coarse_site(site, level);
+    auto it = index.find(key);
+    if (it != index.end()) return it->second;
+    sites.push_back(key);
+    index.emplace(key, sites.size() - 1);
+    return sites.size() - 1;
+  };
+  std::vector<std::array<size_t, 2>> between;
+  for (auto& a : fine.around) {
+    size_t x = cell_of(a[0]);
+    size_t y = cell_of(a[1]);
+    between.push_back({x, y});
+  }
+  std::vector<size_t> lies;
+  for (auto& b : fine.bonds) {
+    const Around& ai = fine.around[b.i];
+    const Around& aj = fine.around[b.j];
+    const Site& s0 = (ai[0] == aj[0] || ai[0] == aj[1]) ? ai[0] : ai[1];
+    if (!(s0 == aj[0] || s0 == aj[1])) throw std::runtime_error("a beam is inside a cell");
+    lies.push_back(cell_of(s0));
+  }
+  size_t n = sites.size();
+  auto face = [&](size_t a, size_t b) { return sites[a] < sites[b] ? std::make_pair(a, b) : std::make_pair(b, a); };
+  std::vector<GCell> cells(n);
+  std::vector<std::vector<std::pair<Mat, double>>> mats(n);
+  std::map<std::pair<size_t, size_t>, std::pair<double, bool>> faces;
+  for (size_t v = 0; v < between.size(); v++) {
+    size_t x = between[v][0], y = between[v][1];
+    const Node& part = fine.nodes[v];
+    for (size_t c : (x == y ? std::vector<size_t>{x} : std::vector<size_t>{x, y})) {
+      bool hit = false;
+      for (auto& mw : mats[c])
+        if (same_mat(mw.first, part.mat)) {
+          mw.second += part.m;
+          hit = true;
+          break;
+        }
+      if (!hit) mats[c].push_back({part.mat, part.m});
+    }
+    if (x == y) {
+      cells[x].free.push_back(v);
+    } else {
+      auto& f = faces[face(x, y)];
+      f.first += part.m;
+      f.second = f.second || part.held;
+    }
+  }
+  std::vector<size_t> place(fine.nodes.size(), SIZE_MAX);
+  for (auto& cell : cells)
+    for (size_t k = 0; k < cell.free.size(); k++) place[cell.free[k]] = k;
+  for (size_t b = 0; b < fine.bonds.size(); b++) {
+    size_t c = lies[b];
+    auto at = [&](size_t v) {
+      size_t x = between[v][0], y = between[v][1];
+      if (x == c && y == c) return place[v];
+      return cells[c].handle(x == c ? y : x);
+    };
+    size_t i = at(fine.bonds[b].i);
+    size_t j = at(fine.bonds[b].j);
+    cells[c].bonds.push_back({i, j, b});
+  }
+  std::vector<char> held(n);
+  for (size_t c = 0; c < n; c++) {
+    bool all = !cells[c].free.empty();
+    for (size_t v : cells[c].free) all = all && fine.nodes[v].held;
+    held[c] = all;
+  }
+  for (size_t c = 0; c < n; c++)
+    for (size_t o : cells[c].across) faces[face(c, o)];
+  auto mat_of = [&](std::vector<size_t> cs) {
+    std::vector<std::pair<Mat, double>> all;
+    for (size_t c : cs)
+      for (auto& mw : mats[c]) {
+        bool hit = false;
+        for (auto& x : all)
+          if (same_mat(x.first, mw.first)) {
+            x.second += mw.second;
+            hit = true;
+            break;
+          }
+        if (!hit) all.push_back(mw);
+      }
+    if (all.empty()) return STONE_MAT;
+    size_t best = 0;
+    for (size_t k = 1; k < all.size(); k++)
+      if (!(all[k].second < all[best].second)) best = k;  // max_by keeps the last maximum
+    return all[best].first;
+  };
+  auto ccentre = [&](size_t c) { return centre(sites[c], level); };
+  std::map<std::pair<size_t, size_t>, size_t> node;
+  Lattice out;
+  out.level = level;
+  for (auto& [f, mh] : faces) {
+    node[f] = out.nodes.size();
+    out.nodes.push_back(Node{mul(add(ccentre(f.first), ccentre(f.second)), 0.5), mh.first,
+                             mat_of({f.first, f.second}), mh.second || held[f.first] || held[f.second]});
+    out.around.push_back({sites[f.first], sites[f.second]});
+  }
+  double s = spacing(level);
+  std::vector<std::vector<size_t>> ends(n);
+  std::vector<std::vector<V3>> at(n);
+  std::vector<Mat> mat(n);
+  std::vector<Full> fulls(n);
+  for (size_t c = 0; c < n; c++) {
+    for (size_t o : cells[c].across) ends[c].push_back(node[face(c, o)]);
+    for (size_t k : ends[c]) at[c].push_back(out.nodes[k].x);
+    mat[c] = mat_of({c});
+  }
+  for (size_t c = 0; c < n; c++) fulls[c] = against_full ? full(mat[c]) : Full{{{1.0, 1.0}, {1.0, 1.0}}};
+  auto found = each<std::vector<Found>>(n, [&](size_t c) {
+    const GCell& cell = cells[c];
+    const std::vector<V3>& pts = at[c];
+    const Mat& mt = mat[c];
+    const Full& fl = fulls[c];
+    std::vector<Found> fd;
+    if (held[c] || pts.empty()) return fd;
+    auto [lat, cases] = responses(fine, cell, pts);
+    size_t nf = cell.free.size(), m = cases.size();
+    std::vector<std::vector<double>> k(m, std::vector<double>(m, 0.0));
+    for (size_t cs = 0; cs < m; cs++)
+      for (size_t b = 0; b < lat.bonds.size(); b++) {
+        const Bond& bd = lat.bonds[b];
+        const V3& f = cases[cs][b];
+        const std::pair<size_t, double> sides[2] = {{bd.j, 1.0}, {bd.i, -1.0}};
+        for (auto [v, sign] : sides)
+          if (v >= nf)
+            for (size_t q = 0; q < 3; q++) k[3 * (v - nf) + q][cs] += sign * f[q];
+      }
+    std::vector<size_t> piece = cell.pieces();
+    for (auto [a, b, stiff] : beams(k, pts, [&](size_t x, size_t y) { return piece[x] == piece[y]; })) {
+      V3 e = unit(sub(pts[b], pts[a]));
+      std::vector<V3> pulled(lat.bonds.size());
+      for (size_t q = 0; q < lat.bonds.size(); q++) {
+        V3 f{0.0, 0.0, 0.0};
+        for (size_t t = 0; t < 3; t++)
+          f = add(f, add(mul(cases[3 * b + t][q], e[t] / 2.0), mul(cases[3 * a + t][q], -e[t] / 2.0)));
+        pulled[q] = f;
+      }
+      double hd = 0.0, work = 0.0;
+      std::vector<double> u = used(lat, pulled);
+      for (size_t q = 0; q < lat.bonds.size(); q++) {
+        const Bond& bd = lat.bonds[q];
+        double kk = bd.a / bd.d;
+        double along = dot(pulled[q], bd.n);
+        double w = 0.5 * (along * along / (bd.mat.e * kk) +
+                          (dot(pulled[q], pulled[q]) - along * along) / (bd.mat.g * kk));  // model.rs:132-136
+        hd += w * u[q];
+        work += w;
+      }
+      double fine_used = work > 0.0 ? hd / work : 0.0;
+      double length = norm(sub(pts[b], pts[a]));
+      double own = mt.used(mt.e / length, 0.0);
+      double stronger = fine_used > 0.0 ? own / fine_used : 1.0;
+      size_t kc = cls(pts[a], pts[b], ccentre(c));
+      double section = stiff * length / mt.e / (s * s);
+      if (against_full) section = section * SECTION[kc] / fl[kc].first;
+      fd.push_back(Found{a, b, section * s * s * mt.e / length, stronger / fl[kc].second});
+    }
+    return fd;
+  });
+  for (size_t c = 0; c < n; c++) {
+    const GCell& cell = cells[c];
+    double mass = 0.0;
+    for (size_t v : cell.free) mass += fine.nodes[v].m;
+    std::vector<double> joined(at[c].size(), 0.0);
+    for (auto& f : found[c]) {
+      joined[f.a] += f.k;
+      joined[f.b] += f.k;
+    }
+    double total = 0.0;
+    for (double j : joined) total += j;
+    for (size_t h = 0; h < ends[c].size(); h++)
+      out.nodes[ends[c][h]].m += mass * (total > 0.0 ? joined[h] / total : 1.0 / (double)at[c].size());
+    for (auto& f : found[c])
+      if (!(out.nodes[ends[c][f.a]].held && out.nodes[ends[c][f.b]].held)) {
+        double length = norm(sub(at[c][f.b], at[c][f.a]));
+        Mat st = mat[c];
+        st.ft *= f.stronger;
+        st.fc *= f.stronger;
+        st.fs *= f.stronger;
+        beam(ends[c][f.a], ends[c][f.b], at[c][f.a], at[c][f.b], f.k * length / mat[c].e, st, out.bonds);
+        out.beam_cell.push_back(c);
+      }
+  }
+  return out;
+}
+
 }  // namespace
 
 // ======================================================================= C ABI
@@ -1116,4 +1559,50 @@ void orc_coarse_site(const int64_t* p, uint32_t l, int64_t* out) {
 }
 
 double orc_spacing(uint32_t level) { return spacing(level); }
+
+// gather::gathered (against_full = 1) or measured(fine, false): a new lattice, freed by orc_free.
+OrcLattice* orc_gathered(const OrcLattice* h, int against_full) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  return reinterpret_cast<OrcLattice*>(new Lattice(measured(*L, against_full != 0)));
+}
+
+uint32_t orc_level(const OrcLattice* h) { return reinterpret_cast<const Lattice*>(h)->level; }
+
+// materials: 6 per node (rho, e, g, ft, fc, fs), 6 per bond
+void orc_mats(const OrcLattice* h, double* node_mat, double* bond_mat) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  auto put = [](const Mat& m, double* o) {
+    o[0] = m.rho, o[1] = m.e, o[2] = m.g, o[3] = m.ft, o[4] = m.fc, o[5] = m.fs;
+  };
+  for (size_t v = 0; v < L->nodes.size(); v++) put(L->nodes[v].mat, node_mat + 6 * v);
+  for (size_t b = 0; b < L->bonds.size(); b++) put(L->bonds[b].mat, bond_mat + 6 * b);
+}
+
+// gather.rs:165-186 on an n x (n + rhs) row-major matrix; x: n x rhs
+void orc_gather_solve(size_t n, size_t rhs, const double* a, double* x) {
+  std::vector<std::vector<double>> m(n, std::vector<double>(n + rhs));
+  for (size_t r = 0; r < n; r++)
+    for (size_t c = 0; c < n + rhs; c++) m[r][c] = a[r * (n + rhs) + c];
+  auto s = gsolve(std::move(m), rhs);
+  for (size_t r = 0; r < n; r++)
+    for (size_t k = 0; k < rhs; k++) x[r * rhs + k] = s[r][k];
+}
+
+// gather.rs:116-161 for handles at `at` (nh x 3), k (3nh x 3nh), joined (nh x nh bytes).
+// out: up to nh(nh-1)/2 triples (a, b, stiffness); returns how many.
+size_t orc_beams(size_t nh, const double* at, const double* k, const uint8_t* joined, double* out) {
+  std::vector<V3> pts(nh);
+  for (size_t h = 0; h < nh; h++) pts[h] = {at[3 * h], at[3 * h + 1], at[3 * h + 2]};
+  size_t n = 3 * nh;
+  std::vector<std::vector<double>> km(n, std::vector<double>(n));
+  for (size_t r = 0; r < n; r++)
+    for (size_t c = 0; c < n; c++) km[r][c] = k[r * n + c];
+  auto found = beams(km, pts, [&](size_t a, size_t b) { return joined[a * nh + b] != 0; });
+  for (size_t q = 0; q < found.size(); q++) {
+    out[3 * q] = (double)std::get<0>(found[q]);
+    out[3 * q + 1] = (double)std::get<1>(found[q]);
+    out[3 * q + 2] = std::get<2>(found[q]);
+  }
+  return found.size();
+}
 }

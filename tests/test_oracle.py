@@ -271,3 +271,127 @@ def test_settle_cantilever_against_beam_theory():
         else:
             lo = mid
     assert 0.9 <= lo * S0 / theory <= 1.1
+
+
+# ---------------------------------------------------------------- gather.rs
+
+
+def test_gather_bar_weighs_as_much():
+    """statics/src/gather.rs:400-412: a steel bar keeps its weight through three gatherings."""
+    fill = lambda x: (orc.STEEL, False) if (abs(x[0]) < 4.0 and abs(x[1]) < 0.2 and abs(x[2]) < 0.2) else None
+    lat = orc.Lattice(0, *scenes.solids(0, [-4.5, -1.0, -1.0], [4.5, 1.0, 1.0], fill))
+    whole = lat.m.sum()
+    for level in (1, 2, 3):
+        lat = lat.gathered()
+        assert lat.level == level and lat.n_bonds > 0
+        assert abs(lat.m.sum() / whole - 1.0) < 1e-9
+
+
+def _centre_beams(lat):
+    """The beams of the coarse cell at the origin: (class, area, ft) of each."""
+    s = orc.spacing(lat.level)
+    c = (np.zeros(3) + 0.5 * s)
+    inc = lambda v: (lat.around[v][0] == 0).all() or (lat.around[v][1] == 0).all()
+    out = []
+    for q in range(lat.n_bonds // 4):
+        i, j = lat.ij[4 * q]
+        if inc(i) and inc(j):
+            u, v = lat.x[i] - c, lat.x[j] - c
+            k = int(round(1.0 + u @ v / (np.linalg.norm(u) * np.linalg.norm(v))))
+            out.append((k, 4.0 * lat.a[4 * q], lat.bond_mat[4 * q, 3]))
+    return out
+
+
+@pytest.mark.parametrize("mat", [orc.STONE, orc.WOOD])
+def test_gather_full_cell_is_section(mat):
+    """A full cell gathered against a full cell's measure is SECTION of its material: all 15
+    beams (3 between opposite faces, 12 between neighbouring ones) of section SECTION s^2
+    (statics/src/lattice.rs:20-25) and the material's own strength (gather.rs:187-192)."""
+    s = orc.spacing(1)
+    lat = orc.Lattice(0, *scenes.solids(0, [-1.5 * s] * 3, [1.5 * s] * 3, lambda x: (mat, False)))
+    beams = _centre_beams(lat.gathered())
+    assert sorted(k for k, _, _ in beams) == [0] * 3 + [1] * 12
+    for _, area, ft in beams:
+        assert area == pytest.approx(0.336 * s * s, rel=1e-12)
+        assert ft == pytest.approx(mat[3], rel=1e-12)
+    # measured as it is, the same cell's beams differ by class but not within one
+    raw = _centre_beams(lat.gathered(against_full=False))
+    for k in (0, 1):
+        a = [area for kk, area, _ in raw if kk == k]
+        assert max(a) == pytest.approx(min(a), rel=1e-9)
+
+
+def test_gather_truss_stays_truss():
+    """Matter that only lies near other matter in a cell does not join it (gather.rs:14-15):
+    two level-0 cells touching at a corner inside one coarse cell give two separate sets of
+    beams, three among each one's own three handles and none across."""
+    lat = orc.Lattice(0, np.array([[0, 0, 0], [1, 1, 1]]), [orc.STONE] * 2, np.zeros(2, np.uint8))
+    g = lat.gathered()
+    assert g.n_nodes == 6 and g.n_bonds == 6 * 4
+    low = lambda v: int(g.around[v].min() < 0)  # a face on the coarse cell's lower side
+    for q in range(6):
+        i, j = g.ij[4 * q]
+        assert low(i) == low(j)
+    assert g.m.sum() == pytest.approx(lat.m.sum(), rel=1e-12)
+
+
+def test_gather_solve():
+    """gather.rs:165-186: numpy's solution for a regular system; a direction it cannot tell
+    (a zero row and column, last: the pivot search keeps the last of equal maxima, so a zero
+    column earlier would swap a later row in) solved as zero, the rest as without it."""
+    rng = np.random.default_rng(5)
+    n, rhs = 12, 3
+    a = rng.standard_normal((n, n)) + n * np.eye(n)
+    b = rng.standard_normal((n, rhs))
+    x = orc.gather_solve(np.hstack([a, b]), rhs)
+    assert np.allclose(x, np.linalg.solve(a, b), rtol=1e-12, atol=1e-12)
+    a[n - 1, :] = 0.0
+    a[:, n - 1] = 0.0
+    b[n - 1] = 0.0
+    x = orc.gather_solve(np.hstack([a, b]), rhs)
+    keep = list(range(n - 1))
+    assert (x[n - 1] == 0.0).all()
+    assert np.allclose(x[keep], np.linalg.solve(a[np.ix_(keep, keep)], b[keep]), rtol=1e-12, atol=1e-12)
+
+
+def _units(at, pairs):
+    n = 3 * len(at)
+    out = []
+    for a, b in pairs:
+        e = (at[b] - at[a]) / np.linalg.norm(at[b] - at[a])
+        u = np.zeros((n, n))
+        ee = np.outer(e, e)
+        u[3 * a:3 * a + 3, 3 * a:3 * a + 3] += ee
+        u[3 * b:3 * b + 3, 3 * b:3 * b + 3] += ee
+        u[3 * a:3 * a + 3, 3 * b:3 * b + 3] -= ee
+        u[3 * b:3 * b + 3, 3 * a:3 * a + 3] -= ee
+        out.append(u)
+    return out
+
+
+def test_gather_beams_fit():
+    """gather.rs:116-161: a stiffness made of beams is fitted by those beams; a beam asked to
+    be negative is dropped and the rest refitted (numpy's least squares on what is kept);
+    pairs not joined get none."""
+    at = np.array([[-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0], [0, 0, -1], [0, 0, 1]], float) * 0.25
+    pairs = [(a, b) for a in range(6) for b in range(a + 1, 6)]
+    u = _units(at, pairs)
+    want = {(0, 1): 3.0, (0, 2): 1.5, (2, 5): 0.7, (3, 4): 2.2}
+    k = sum(v * u[pairs.index(p)] for p, v in want.items())
+    joined = np.ones((6, 6), np.uint8)
+    got = orc.beams(at, k, joined)
+    big = {(a, b): v for a, b, v in got if v > 1e-9}
+    assert set(big) == set(want)
+    for p, v in want.items():
+        assert big[p] == pytest.approx(v, rel=1e-9)
+    # one beam negative: dropped, the rest is numpy's least squares over the kept pairs
+    k2 = k - 2.0 * u[pairs.index((1, 3))]
+    got = orc.beams(at, k2, joined)
+    assert all(v > 0 for _, _, v in got) and (1, 3) not in [(a, b) for a, b, _ in got]
+    kept = [p for p in pairs if p in [(a, b) for a, b, _ in got]]
+    A = np.stack([u[pairs.index(p)].ravel() for p in kept], 1)
+    lsq = np.linalg.lstsq(A, k2.ravel(), rcond=None)[0]
+    assert np.allclose([v for _, _, v in got], lsq, rtol=1e-9, atol=1e-12)
+    # not joined: no beam between 0 and 1
+    joined[0, 1] = joined[1, 0] = 0
+    assert (0, 1) not in [(a, b) for a, b, _ in orc.beams(at, k, joined)]
