@@ -135,6 +135,41 @@ typedef struct {
  * material) not NULL, also the cells, each array sized for the count (a first
  * call with them NULL gives it). The result is st_upload's input, with e and g
  * per cell from the caller's materials. */
+/* A lattice as the reference holds it (statics/src/lattice.rs:61-71, model.rs:92-111): per
+ * node its position, weight, material, held flag and the two sites it lies between
+ * (Lattice.around: lesser first, level-0 units); per bond its nodes, point, direction,
+ * share of the section, beam length and material. A material is six values, rho, e, g,
+ * ft, fc, fs (model.rs:43-51); two are the same material when all six are equal (the
+ * reference compares names, one to each set of values among its materials). */
+typedef struct {
+  uint32_t level;
+  int64_t nnodes;
+  const double* x;         /* 3 per node */
+  const double* m;
+  const double* node_mat;  /* 6 per node */
+  const uint8_t* held;
+  const int64_t* around;   /* 6 per node */
+  int64_t nbonds;
+  const int64_t* ij;       /* 2 per bond */
+  const double* p;         /* 3 per bond */
+  const double* n;         /* 3 per bond */
+  const double* a;
+  const double* d;
+  const double* bond_mat;  /* 6 per bond */
+} StLattice;
+
+/* Replaces: statics::gather::gathered (statics/src/gather.rs:193-195; against_full = 1)
+ * and measured (:228-395; against_full = 0): the next level up, each of its cells' beams
+ * measured from the finer matter in it. The finer lattice can be one built at level 0 or
+ * one gathered before; a coarse cell holds at most 12 free finer nodes, 6 faces and 480
+ * bonds (a lattice the reference builds or gathers never has more; more is ST_BAD_INPUT).
+ * The result is kept in the context until the next st_gather: its sizes go to
+ * nnodes_out and nbonds_out, st_gather_read copies it out in StLattice's layout, in the
+ * reference's order (nodes by face, bonds cell by cell, PER_BEAM to a beam). */
+int st_gather(StCtx* ctx, const StLattice* fine, int against_full, int64_t* nnodes_out, int64_t* nbonds_out);
+int st_gather_read(StCtx* ctx, uint32_t* level, double* x, double* m, double* node_mat, uint8_t* held,
+                   int64_t* around, int64_t* ij, double* p, double* n, double* a, double* d, double* bond_mat);
+
 int st_grid_cells(StCtx* ctx, const StGrid* grid, int32_t level, double hold_radius, int64_t* count_out,
                   int32_t* sites_out, uint8_t* held_out, int64_t* index_out);
 

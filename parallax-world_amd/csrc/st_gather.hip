@@ -1,0 +1,777 @@
+// The next level up, measured from a finer one (statics/src/gather.rs:193-395): each
+// coarse cell's matter condensed onto its faces, and the beams nearest that stiffness.
+//
+// The host does the bookkeeping the reference does in `measured` (which coarse cell each
+// finer node and bond lies in, the faces, the handles, the materials, the weights); the
+// device measures the cells, one lane per cell: the elimination of the free parts
+// (responses, :59-114), the stiffness between handles, the non-negative beam fit (beams,
+// :119-161) and each beam's share of the finer strength. Every choice in that measure (a
+// pivot, a beam dropped for being negative, a beam kept for being positive) follows from
+// FP64 values compared exactly, and full cells are symmetric, so ties are the rule, not
+// the exception: the lane computes each value with the reference's operations in the
+// reference's order, without contraction (the pragma below), so the choices, and the
+// lattice, are the reference's.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <unordered_map>
+#include <vector>
+
+#include "st_solver.hpp"
+
+#pragma clang fp contract(off)
+
+namespace st {
+namespace {
+
+constexpr int kMaxFree = 12;   // finer nodes inside a coarse cell: its 12 inner faces
+constexpr int kMaxHandles = 6;  // a coarse cell's faces
+constexpr int kMaxBonds = 480;  // 8 finer cells x 15 beams x PER_BEAM
+constexpr int kMaxPairs = kMaxHandles * (kMaxHandles - 1) / 2;
+constexpr int kRow = 6 * kMaxFree + 3 * kMaxHandles;  // the elimination's row pitch (90)
+constexpr int kN2 = 3 * kMaxHandles;                   // handle motions (18)
+constexpr double kSection[2] = {0.336, 0.336};         // lattice.rs:25
+constexpr double kUnit = 0.125;                        // lattice.rs:16
+constexpr int kPerBeam = 4;                            // lattice.rs:19
+
+// per-lane workspace, in doubles: the elimination, the cases' bond forces, the stiffness
+// between handles, the beam units, the target, the beams' normal equations, a pulled beam
+constexpr int64_t kOffA = 0;
+constexpr int64_t kOffCases = kOffA + (int64_t)6 * kMaxFree * kRow;
+constexpr int64_t kOffK = kOffCases + (int64_t)kN2 * kMaxBonds * 3;
+constexpr int64_t kOffU = kOffK + kN2 * kN2;
+constexpr int64_t kOffT = kOffU + (int64_t)kMaxPairs * kN2 * kN2;
+constexpr int64_t kOffG = kOffT + kN2 * kN2;
+constexpr int64_t kOffPull = kOffG + kMaxPairs * (kMaxPairs + 1);
+constexpr int64_t kWsDoubles = kOffPull + (int64_t)kMaxBonds * 3;
+
+struct G3 {
+  double c[3];
+};
+__host__ __device__ inline G3 gadd(G3 a, G3 b) { return {{a.c[0] + b.c[0], a.c[1] + b.c[1], a.c[2] + b.c[2]}}; }
+__host__ __device__ inline G3 gsub(G3 a, G3 b) { return {{a.c[0] - b.c[0], a.c[1] - b.c[1], a.c[2] - b.c[2]}}; }
+__host__ __device__ inline G3 gmul(G3 a, double s) { return {{a.c[0] * s, a.c[1] * s, a.c[2] * s}}; }
+__host__ __device__ inline double gdot(G3 a, G3 b) { return a.c[0] * b.c[0] + a.c[1] * b.c[1] + a.c[2] * b.c[2]; }
+__host__ __device__ inline G3 gcross(G3 a, G3 b) {
+  return {{a.c[1] * b.c[2] - a.c[2] * b.c[1], a.c[2] * b.c[0] - a.c[0] * b.c[2], a.c[0] * b.c[1] - a.c[1] * b.c[0]}};
+}
+__host__ __device__ inline double gnorm(G3 a) { return sqrt(gdot(a, a)); }
+__host__ __device__ inline G3 gunit(G3 a) { return gmul(a, 1.0 / gnorm(a)); }
+// std::max's choice (the first unless the second is greater), as the reference's f64::max
+// gives for the values it meets here
+__host__ __device__ inline double gmax(double a, double b) { return a < b ? b : a; }
+// lattice.rs:29-32
+__host__ __device__ inline int gclass(G3 a, G3 b, G3 c) {
+  G3 u = gsub(a, c), v = gsub(b, c);
+  return (int)round(1.0 + gdot(u, v) / (gnorm(u) * gnorm(v)));
+}
+// Mat::used (model.rs:66-68) for strengths ft, fc, fs
+__host__ __device__ inline double mat_used(double ft, double fc, double fs, double s1, double s3) {
+  return gmax(gmax(s1 / ft, -s3 / fc), (s1 - s3) / (2.0 * fs));
+}
+
+// The cells to measure, structure of arrays; a cell's bonds, free parts and handles are
+// contiguous ranges.
+struct GJob {
+  int32_t n;
+  const int32_t *nf, *nh, *nb;
+  const int64_t *boff, *foff, *hoff;
+  const double* fx;       // free parts' positions, 3 each
+  const double* hx;       // handles' positions, 3 each
+  const int32_t* piece;   // per handle, the piece of matter it is joined to
+  const int32_t* bij;     // per bond, its two places (free parts first, then handles)
+  const double* bgeo;     // per bond: p (3), n (3), a, d
+  const double* bmat;     // per bond: e, g, ft, fc, fs
+  const double* cmat;     // per cell: e, ft, fc, fs of its material
+  const double* cfull;    // per cell: section and strength ratio of both classes
+  const double* ccentre;  // per cell: its centre
+  double s;               // the coarse spacing
+  int32_t against_full;   // sections against a full cell's (gathered) or as measured
+  int32_t* nfound;        // per cell: how many beams
+  int32_t* fab;           // per cell, kMaxPairs x 2: the beams' handles
+  double* fval;           // per cell, kMaxPairs x 2: stiffness along, how much stronger
+};
+
+struct Ws {
+  double* base;
+  int64_t stride;
+  __device__ double& operator[](int64_t e) const { return base[e * stride]; }
+};
+
+// gather.rs:165-186 on rows of pitch `pitch` at `off`: n rows, n + rhs columns, in place.
+// Returns the size under which a pivot is taken for nothing (its scale: the greatest
+// diagonal entry before the elimination).
+__device__ double lane_solve(const Ws& w, int64_t off, int pitch, int n, int rhs) {
+  auto A = [&](int r, int c) -> double& { return w[off + (int64_t)r * pitch + c]; };
+  double scale = 0.0;
+  for (int r = 0; r < n; r++) scale = gmax(scale, fabs(A(r, r)));
+  const double tiny = 1e-12 * scale;
+  for (int c = 0; c < n; c++) {
+    int p = c;
+    for (int x = c + 1; x < n; x++)
+      if (!(fabs(A(x, c)) < fabs(A(p, c)))) p = x;  // the last of the greatest, as max_by
+    if (p != c)
+      for (int k = 0; k < n + rhs; k++) {
+        double t = A(c, k);
+        A(c, k) = A(p, k);
+        A(p, k) = t;
+      }
+    const double pv = A(c, c);
+    if (fabs(pv) <= tiny) continue;
+    for (int r = 0; r < n; r++) {
+      if (r == c) continue;
+      const double f = A(r, c) / pv;
+      if (f != 0.0)
+        for (int k = c; k < n + rhs; k++) A(r, k) -= f * A(c, k);
+    }
+  }
+  return tiny;
+}
+// the solution's row r, right-hand side k
+__device__ inline double lane_x(const Ws& w, int64_t off, int pitch, int n, int r, int k, double tiny) {
+  const double d = w[off + (int64_t)r * pitch + r];
+  return fabs(d) <= tiny ? 0.0 : w[off + (int64_t)r * pitch + n + k] / d;
+}
+
+__global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_t stride) {
+  const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= j.n) return;
+  const Ws w{ws + c, stride};
+  const int nf = j.nf[c], nh = j.nh[c], nb = j.nb[c];
+  const int n = 6 * nf, m = 3 * nh;
+  const double* fx = j.fx + 3 * j.foff[c];
+  const double* hx = j.hx + 3 * j.hoff[c];
+  const int32_t* piece = j.piece + j.hoff[c];
+  const int32_t* bij = j.bij + 2 * j.boff[c];
+  const double* bgeo = j.bgeo + 8 * j.boff[c];
+  const double* bmat = j.bmat + 5 * j.boff[c];
+  auto pos = [&](int v) -> G3 {
+    const double* q = v < nf ? fx + 3 * v : hx + 3 * (v - nf);
+    return {{q[0], q[1], q[2]}};
+  };
+  auto dof = [&](int v, int d) -> int { return v < nf ? 6 * v + d : (d < 3 ? n + 3 * (v - nf) + d : -1); };
+  auto column = [&](int v, int d, G3 p) -> G3 {
+    G3 e{{0.0, 0.0, 0.0}};
+    e.c[d % 3] = 1.0;
+    return d < 3 ? e : gcross(e, gsub(p, pos(v)));
+  };
+  auto A = [&](int r, int k) -> double& { return w[kOffA + (int64_t)r * kRow + k]; };
+  auto P3 = [&](const double* q) -> G3 { return {{q[0], q[1], q[2]}}; };
+
+  // responses (gather.rs:59-103): the free parts' equations, the handles' columns on the right
+  for (int r = 0; r < n; r++)
+    for (int k = 0; k < n + m; k++) A(r, k) = 0.0;
+  for (int b = 0; b < nb; b++) {
+    const G3 p = P3(bgeo + 8 * b), nv = P3(bgeo + 8 * b + 3);
+    const double kk = bgeo[8 * b + 6] / bgeo[8 * b + 7];
+    const double e = bmat[5 * b], g = bmat[5 * b + 1];
+    double st[3][3];  // Bond::stiffness (model.rs:139-149)
+    for (int r = 0; r < 3; r++)
+      for (int q = 0; q < 3; q++) {
+        const double nn = nv.c[r] * nv.c[q];
+        st[r][q] = e * kk * nn + g * kk * ((r == q ? 1.0 : 0.0) - nn);
+      }
+    const int vs[2] = {bij[2 * b], bij[2 * b + 1]};
+    const double sg[2] = {-1.0, 1.0};
+    for (int x = 0; x < 2; x++)
+      for (int d = 0; d < 6; d++) {
+        const int r = dof(vs[x], d);
+        if (r < 0 || r >= n) continue;
+        const G3 cv = column(vs[x], d, p);
+        const G3 scv{{gdot(G3{{st[0][0], st[0][1], st[0][2]}}, cv), gdot(G3{{st[1][0], st[1][1], st[1][2]}}, cv),
+                      gdot(G3{{st[2][0], st[2][1], st[2][2]}}, cv)}};
+        for (int y = 0; y < 2; y++)
+          for (int d2 = 0; d2 < 6; d2++) {
+            const int col = dof(vs[y], d2);
+            if (col < 0) continue;
+            const double sign = col < n ? 1.0 : -1.0;
+            A(r, col) += sign * sg[x] * sg[y] * gdot(scv, column(vs[y], d2, p));
+          }
+      }
+  }
+  const double tiny_a = lane_solve(w, kOffA, kRow, n, m);
+
+  // each handle moved alone along each axis: the bonds' forces (moved, bodies.rs:420-423)
+  auto CASE = [&](int cs, int b, int q) -> double& { return w[kOffCases + ((int64_t)cs * kMaxBonds + b) * 3 + q]; };
+  for (int cs = 0; cs < m; cs++)
+    for (int b = 0; b < nb; b++) {
+      const G3 p = P3(bgeo + 8 * b), nv = P3(bgeo + 8 * b + 3);
+      G3 tr[2];
+      for (int x = 0; x < 2; x++) {
+        const int v = bij[2 * b + x];
+        double mo[6];
+        if (v < nf) {
+          for (int d = 0; d < 6; d++) mo[d] = lane_x(w, kOffA, kRow, n, 6 * v + d, cs, tiny_a);
+        } else {
+          for (int d = 0; d < 6; d++) mo[d] = 0.0;
+          if (v - nf == cs / 3) mo[cs % 3] = 1.0;
+        }
+        // shift (bodies.rs:44-47): the translation at the bond's point
+        tr[x] = gadd(G3{{mo[0], mo[1], mo[2]}}, gcross(G3{{mo[3], mo[4], mo[5]}}, gsub(p, pos(v))));
+      }
+      const G3 delta = gsub(tr[1], tr[0]);
+      // Bond::force (model.rs:115-119)
+      const double along = gdot(delta, nv);
+      const double kk = bgeo[8 * b + 6] / bgeo[8 * b + 7];
+      const G3 f = gadd(gmul(nv, bmat[5 * b] * kk * along), gmul(gsub(delta, gmul(nv, along)), bmat[5 * b + 1] * kk));
+      for (int q = 0; q < 3; q++) CASE(cs, b, q) = f.c[q];
+    }
+
+  // the stiffness between handles: what holds each, each case (gather.rs:291-303)
+  auto K = [&](int r, int q) -> double& { return w[kOffK + r * kN2 + q]; };
+  for (int r = 0; r < m; r++)
+    for (int q = 0; q < m; q++) K(r, q) = 0.0;
+  for (int cs = 0; cs < m; cs++)
+    for (int b = 0; b < nb; b++) {
+      const int sides[2] = {bij[2 * b + 1], bij[2 * b]};
+      const double sign[2] = {1.0, -1.0};
+      for (int x = 0; x < 2; x++)
+        if (sides[x] >= nf)
+          for (int q = 0; q < 3; q++) K(3 * (sides[x] - nf) + q, cs) += sign[x] * CASE(cs, b, q);
+    }
+
+  // beams (gather.rs:119-161)
+  const int n2 = m;
+  int pa[kMaxPairs], pb[kMaxPairs], np = 0;
+  for (int a = 0; a < nh; a++)
+    for (int b = a + 1; b < nh; b++)
+      if (piece[a] == piece[b]) {
+        pa[np] = a;
+        pb[np] = b;
+        np++;
+      }
+  auto U = [&](int p, int i) -> double& { return w[kOffU + (int64_t)p * kN2 * kN2 + i]; };
+  auto T = [&](int i) -> double& { return w[kOffT + i]; };
+  for (int p = 0; p < np; p++) {
+    const int a = pa[p], b = pb[p];
+    const G3 e = gunit(gsub(pos(nf + b), pos(nf + a)));
+    for (int i = 0; i < n2 * n2; i++) U(p, i) = 0.0;
+    for (int r = 0; r < 3; r++)
+      for (int q = 0; q < 3; q++) {
+        const double v = e.c[r] * e.c[q];
+        U(p, (3 * a + r) * n2 + 3 * a + q) += v;
+        U(p, (3 * b + r) * n2 + 3 * b + q) += v;
+        U(p, (3 * a + r) * n2 + 3 * b + q) -= v;
+        U(p, (3 * b + r) * n2 + 3 * a + q) -= v;
+      }
+  }
+  for (int i = 0; i < n2 * n2; i++) T(i) = 0.5 * (K(i / n2, i % n2) + K(i % n2, i / n2));
+  const int gp = kMaxPairs + 1;
+  auto G = [&](int r, int q) -> double& { return w[kOffG + r * gp + q]; };
+  uint32_t on = np ? ((1u << np) - 1u) : 0u;
+  int idx[kMaxPairs], mm = 0;
+  double xs[kMaxPairs];
+  for (;;) {
+    mm = 0;
+    for (int p = 0; p < np; p++)
+      if (on >> p & 1u) idx[mm++] = p;
+    for (int r = 0; r < mm; r++) {
+      for (int q = 0; q < mm; q++) {
+        double s = 0.0;
+        for (int i = 0; i < n2 * n2; i++) s += U(idx[r], i) * U(idx[q], i);
+        G(r, q) = s;
+      }
+      double s = 0.0;
+      for (int i = 0; i < n2 * n2; i++) s += U(idx[r], i) * T(i);
+      G(r, mm) = s;
+    }
+    const double tiny_g = lane_solve(w, kOffG, gp, mm, 1);
+    int worst = -1;
+    for (int r = 0; r < mm; r++) {
+      xs[r] = lane_x(w, kOffG, gp, mm, r, 0, tiny_g);
+      if (xs[r] < 0.0 && (worst < 0 || xs[r] < xs[worst])) worst = r;  // the first most negative
+    }
+    if (worst < 0) break;
+    on &= ~(1u << idx[worst]);
+  }
+
+  // each beam kept: pulled alone, a unit apart, how much of the finer strength it uses
+  const double e_c = j.cmat[4 * c], ft_c = j.cmat[4 * c + 1], fc_c = j.cmat[4 * c + 2], fs_c = j.cmat[4 * c + 3];
+  const G3 centre = P3(j.ccentre + 3 * c);
+  const double s = j.s;
+  auto PULL = [&](int b, int q) -> double& { return w[kOffPull + (int64_t)b * 3 + q]; };
+  int found = 0;
+  for (int r = 0; r < mm; r++) {
+    if (!(xs[r] > 0.0)) continue;
+    const int a = pa[idx[r]], b = pb[idx[r]];
+    const double stiff = xs[r];
+    const G3 at_a = pos(nf + a), at_b = pos(nf + b);
+    const G3 e = gunit(gsub(at_b, at_a));
+    for (int q = 0; q < nb; q++) {
+      G3 f{{0.0, 0.0, 0.0}};
+      for (int t = 0; t < 3; t++) {
+        const G3 cb{{CASE(3 * b + t, q, 0), CASE(3 * b + t, q, 1), CASE(3 * b + t, q, 2)}};
+        const G3 ca{{CASE(3 * a + t, q, 0), CASE(3 * a + t, q, 1), CASE(3 * a + t, q, 2)}};
+        f = gadd(f, gadd(gmul(cb, e.c[t] / 2.0), gmul(ca, -e.c[t] / 2.0)));
+      }
+      for (int k = 0; k < 3; k++) PULL(q, k) = f.c[k];
+    }
+    double held = 0.0, work = 0.0;
+    for (int q = 0; q < nb; q++) {
+      const G3 f{{PULL(q, 0), PULL(q, 1), PULL(q, 2)}};
+      const G3 nv = P3(bgeo + 8 * q + 3);
+      const double ba = bgeo[8 * q + 6], bd = bgeo[8 * q + 7];
+      const double* bm = bmat + 5 * q;
+      // Bond::used (model.rs:123-129)
+      const G3 t = gmul(f, 1.0 / ba);
+      const double sigma = gdot(t, nv);
+      const double tau2 = gmax(gdot(t, t) - sigma * sigma, 0.0);
+      const double rr = sqrt(sigma * sigma / 4.0 + tau2);
+      const double u = mat_used(bm[2], bm[3], bm[4], sigma / 2.0 + rr, sigma / 2.0 - rr);
+      // Bond::energy (model.rs:132-136)
+      const double kk = ba / bd;
+      const double along = gdot(f, nv);
+      const double wq = 0.5 * (along * along / (bm[0] * kk) + (gdot(f, f) - along * along) / (bm[1] * kk));
+      held += wq * u;
+      work += wq;
+    }
+    const double fine_used = work > 0.0 ? held / work : 0.0;
+    const double length = gnorm(gsub(at_b, at_a));
+    const double own = mat_used(ft_c, fc_c, fs_c, e_c / length, 0.0);
+    const double stronger = fine_used > 0.0 ? own / fine_used : 1.0;
+    const int kc = gclass(at_a, at_b, centre);
+    double section = stiff * length / e_c / (s * s);
+    if (j.against_full) section = section * kSection[kc] / j.cfull[4 * c + 2 * kc];
+    j.fab[(int64_t)c * 2 * kMaxPairs + 2 * found] = a;
+    j.fab[(int64_t)c * 2 * kMaxPairs + 2 * found + 1] = b;
+    j.fval[(int64_t)c * 2 * kMaxPairs + 2 * found] = section * s * s * e_c / length;
+    j.fval[(int64_t)c * 2 * kMaxPairs + 2 * found + 1] = stronger / j.cfull[4 * c + 2 * kc + 1];
+    found++;
+  }
+  j.nfound[c] = found;
+}
+
+// ------------------------------------------------------------------ host side
+
+using Mat6 = std::array<double, 6>;  // rho, e, g, ft, fc, fs
+using Site3 = std::array<int64_t, 3>;
+struct Site3Hash {
+  size_t operator()(const Site3& s) const {
+    uint64_t h = 0x9e3779b97f4a7c15ull;
+    for (int64_t v : s) h = (h ^ (uint64_t)v) * 0x100000001b3ull + (h >> 29);
+    return (size_t)h;
+  }
+};
+
+// A lattice in host memory (lattice.rs:61-71)
+struct HLat {
+  uint32_t level = 0;
+  std::vector<G3> x;
+  std::vector<double> m;
+  std::vector<Mat6> nmat;
+  std::vector<uint8_t> held;
+  std::vector<std::array<Site3, 2>> around;
+  std::vector<std::array<int64_t, 2>> ij;
+  std::vector<G3> p, n;
+  std::vector<double> a, d;
+  std::vector<Mat6> bmat;
+};
+
+double hspacing(uint32_t level) { return kUnit * (double)(uint64_t(1) << level); }  // lattice.rs:34-36
+G3 hcentre(const Site3& site, uint32_t level) {                                      // lattice.rs:39-41
+  G3 c;
+  for (int k = 0; k < 3; k++) c.c[k] = ((double)site[k] + 0.5 * (double)(uint64_t(1) << level)) * kUnit;
+  return c;
+}
+
+// lattice.rs:76-84: PER_BEAM bonds about the beam's middle
+void hbeam(HLat& L, int64_t i, int64_t j, G3 a, G3 b, double area, const Mat6& mat) {
+  const G3 n = gunit(gsub(b, a));
+  const G3 e1 = gunit(gcross(n, fabs(n.c[0]) < 0.9 ? G3{{1.0, 0.0, 0.0}} : G3{{0.0, 1.0, 0.0}}));
+  const G3 e2 = gcross(n, e1);
+  const double r = sqrt(area / (2.0 * M_PI));
+  const G3 middle = gmul(gadd(a, b), 0.5);
+  const double d = gnorm(gsub(b, a));
+  const G3 off[kPerBeam] = {gmul(e1, r), gmul(e1, -r), gmul(e2, r), gmul(e2, -r)};
+  for (const G3& o : off) {
+    L.ij.push_back({i, j});
+    L.p.push_back(gadd(middle, o));
+    L.n.push_back(n);
+    L.a.push_back(area / 4.0);
+    L.d.push_back(d);
+    L.bmat.push_back(mat);
+  }
+}
+
+void measure(Ctx* c, const HLat& fine, bool against_full, HLat& out);
+
+// gather.rs:200-223: a full cell's beams by class, measured once per material. The
+// reference measures the cell at the origin of a 6^3 block of level-0 cells; the cell's
+// own problem (its 8 finer cells, their nodes and bonds, in the same order, none held) is
+// the same in the 2^3 block of those 8 cells alone, and so are its beams, which are all
+// the block's beams with both ends on the origin cell's faces.
+std::array<double, 4> full_of(Ctx* c, const Mat6& mat) {
+  for (auto& k : c->full_known)
+    if (std::equal(mat.begin(), mat.end(), k.begin())) return {k[6], k[7], k[8], k[9]};
+  HLat block;  // build(0, the 8 cells) (lattice.rs:124-163; weights left out: unused)
+  const int faces[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  std::map<std::array<Site3, 2>, int64_t> index;
+  for (int64_t ax = 0; ax < 2; ax++)
+    for (int64_t ay = 0; ay < 2; ay++)
+      for (int64_t az = 0; az < 2; az++) {
+        const Site3 site{ax, ay, az};
+        int64_t ends[6];
+        for (int f = 0; f < 6; f++) {
+          const Site3 other{site[0] + faces[f][0], site[1] + faces[f][1], site[2] + faces[f][2]};
+          const std::array<Site3, 2> key = site < other ? std::array<Site3, 2>{site, other}
+                                                        : std::array<Site3, 2>{other, site};
+          auto it = index.find(key);
+          if (it == index.end()) {
+            it = index.emplace(key, (int64_t)block.x.size()).first;
+            block.x.push_back(gmul(gadd(hcentre(site, 0), hcentre(other, 0)), 0.5));
+            block.m.push_back(0.0);
+            block.nmat.push_back(mat);
+            block.held.push_back(0);
+            block.around.push_back(key);
+          }
+          ends[f] = it->second;
+        }
+        const G3 middle = hcentre(site, 0);
+        const double s0 = hspacing(0);
+        for (int a = 0; a < 6; a++)
+          for (int b = a + 1; b < 6; b++) {
+            const G3 x = block.x[ends[a]], y = block.x[ends[b]];
+            hbeam(block, ends[a], ends[b], x, y, kSection[gclass(x, y, middle)] * s0 * s0, mat);
+          }
+      }
+  HLat coarse;
+  measure(c, block, false, coarse);
+  const double s = hspacing(1);
+  const Site3 zero{0, 0, 0};
+  double sum[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  for (size_t q = 0; q < coarse.a.size(); q += kPerBeam) {
+    const int64_t i = coarse.ij[q][0], j = coarse.ij[q][1];
+    auto on = [&](int64_t v) { return coarse.around[v][0] == zero || coarse.around[v][1] == zero; };
+    if (on(i) && on(j)) {
+      const int k = gclass(coarse.x[i], coarse.x[j], hcentre(zero, 1));
+      sum[k][0] += 4.0 * coarse.a[q] / (s * s);
+      sum[k][1] += coarse.bmat[q][3] / mat[3];
+      sum[k][2] += 1.0;
+    }
+  }
+  std::array<double, 4> f;
+  for (int k = 0; k < 2; k++) {
+    f[2 * k] = sum[k][2] > 0.0 ? sum[k][0] / sum[k][2] : 1.0;
+    f[2 * k + 1] = sum[k][2] > 0.0 ? sum[k][1] / sum[k][2] : 1.0;
+  }
+  std::array<double, 10> kept;
+  std::copy(mat.begin(), mat.end(), kept.begin());
+  std::copy(f.begin(), f.end(), kept.begin() + 6);
+  c->full_known.push_back(kept);
+  return f;
+}
+
+template <class T>
+void put(DBuf& d, const std::vector<T>& h, hipStream_t s) {
+  d.ensure(std::max<size_t>(h.size(), 1) * sizeof(T));
+  if (!h.empty()) ST_HIP(hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// gather.rs:228-395
+void measure(Ctx* ctx, const HLat& fine, bool against_full, HLat& out) {
+  const uint32_t level = fine.level + 1;
+  // the coarse cells, numbered as first met: each finer node's two, then each bond's
+  std::unordered_map<Site3, int64_t, Site3Hash> index;
+  std::vector<Site3> sites;
+  auto cell_of = [&](const Site3& site) {
+    Site3 key;
+    for (int k = 0; k < 3; k++) key[k] = (site[k] >> level) << level;
+    auto it = index.find(key);
+    if (it != index.end()) return it->second;
+    sites.push_back(key);
+    index.emplace(key, (int64_t)sites.size() - 1);
+    return (int64_t)sites.size() - 1;
+  };
+  const size_t nn = fine.x.size(), nbond = fine.a.size();
+  std::vector<std::array<int64_t, 2>> between(nn);
+  for (size_t v = 0; v < nn; v++) {
+    between[v][0] = cell_of(fine.around[v][0]);
+    between[v][1] = cell_of(fine.around[v][1]);
+  }
+  std::vector<int64_t> lies(nbond);
+  for (size_t b = 0; b < nbond; b++) {
+    const auto& ai = fine.around[fine.ij[b][0]];
+    const auto& aj = fine.around[fine.ij[b][1]];
+    const Site3& in = (ai[0] == aj[0] || ai[0] == aj[1]) ? ai[0] : ai[1];
+    if (!(in == aj[0] || in == aj[1])) throw StError{ST_BAD_INPUT, "gather: a bond's nodes share no cell"};
+    lies[b] = cell_of(in);
+  }
+  const int64_t ncell = (int64_t)sites.size();
+  auto face = [&](int64_t a, int64_t b) { return sites[a] < sites[b] ? std::make_pair(a, b) : std::make_pair(b, a); };
+  struct Cell {
+    std::vector<int64_t> free, across;
+    std::vector<std::array<int64_t, 3>> bonds;  // places i, j and the finer bond
+  };
+  std::vector<Cell> cells(ncell);
+  std::vector<std::vector<std::pair<Mat6, double>>> mats(ncell);
+  auto add_mat = [](std::vector<std::pair<Mat6, double>>& to, const Mat6& m, double w) {
+    for (auto& x : to)
+      if (x.first == m) {
+        x.second += w;
+        return;
+      }
+    to.push_back({m, w});
+  };
+  std::map<std::pair<int64_t, int64_t>, std::pair<double, bool>> faces;
+  for (size_t v = 0; v < nn; v++) {
+    const int64_t x = between[v][0], y = between[v][1];
+    add_mat(mats[x], fine.nmat[v], fine.m[v]);
+    if (x != y) add_mat(mats[y], fine.nmat[v], fine.m[v]);
+    if (x == y) {
+      cells[x].free.push_back((int64_t)v);
+    } else {
+      auto& f = faces[face(x, y)];
+      f.first += fine.m[v];
+      f.second = f.second || fine.held[v];
+    }
+  }
+  std::vector<int64_t> place(nn, -1);
+  for (auto& cell : cells)
+    for (size_t k = 0; k < cell.free.size(); k++) place[cell.free[k]] = (int64_t)k;
+  for (size_t b = 0; b < nbond; b++) {
+    Cell& cell = cells[lies[b]];
+    const int64_t c = lies[b];
+    auto at = [&](int64_t v) -> int64_t {
+      const int64_t x = between[v][0], y = between[v][1];
+      if (x == c && y == c) return place[v];
+      const int64_t other = x == c ? y : x;
+      size_t h = 0;
+      while (h < cell.across.size() && cell.across[h] != other) h++;
+      if (h == cell.across.size()) cell.across.push_back(other);
+      return (int64_t)(cell.free.size() + h);
+    };
+    const int64_t i = at(fine.ij[b][0]);
+    const int64_t j = at(fine.ij[b][1]);
+    cell.bonds.push_back({i, j, (int64_t)b});
+  }
+  std::vector<uint8_t> held(ncell);
+  for (int64_t c = 0; c < ncell; c++) {
+    bool all = !cells[c].free.empty();
+    for (int64_t v : cells[c].free) all = all && fine.held[v];
+    held[c] = all;
+  }
+  for (int64_t c = 0; c < ncell; c++)
+    for (int64_t o : cells[c].across) faces[face(c, o)];
+  auto mat_of = [&](std::initializer_list<int64_t> cs) {
+    std::vector<std::pair<Mat6, double>> all;
+    for (int64_t c : cs)
+      for (auto& mw : mats[c]) add_mat(all, mw.first, mw.second);
+    if (all.empty()) return Mat6{2500.0, 30e9, 12e9, 5e6, 100e6, 10e6};  // STONE (model.rs:53)
+    size_t best = 0;
+    for (size_t k = 1; k < all.size(); k++)
+      if (!(all[k].second < all[best].second)) best = k;  // the last of the heaviest, as max_by
+    return all[best].first;
+  };
+  out = HLat();
+  out.level = level;
+  std::map<std::pair<int64_t, int64_t>, int64_t> node;
+  for (auto& [f, mh] : faces) {
+    node[f] = (int64_t)out.x.size();
+    out.x.push_back(gmul(gadd(hcentre(sites[f.first], level), hcentre(sites[f.second], level)), 0.5));
+    out.m.push_back(mh.first);
+    out.nmat.push_back(mat_of({f.first, f.second}));
+    out.held.push_back(mh.second || held[f.first] || held[f.second]);
+    out.around.push_back({sites[f.first], sites[f.second]});
+  }
+  const double s = hspacing(level);
+  std::vector<std::vector<int64_t>> ends(ncell);
+  std::vector<Mat6> mat(ncell);
+  for (int64_t c = 0; c < ncell; c++) {
+    for (int64_t o : cells[c].across) ends[c].push_back(node[face(c, o)]);
+    mat[c] = mat_of({c});
+  }
+
+  // the cells to measure, packed for the device
+  std::vector<int64_t> job;
+  for (int64_t c = 0; c < ncell; c++)
+    if (!held[c] && !cells[c].across.empty()) job.push_back(c);
+  const int64_t nj = (int64_t)job.size();
+  std::vector<int32_t> nf(nj), nh(nj), nb(nj), piece, bij;
+  std::vector<int64_t> boff(nj), foff(nj), hoff(nj);
+  std::vector<double> fx, hx, bgeo, bmat, cmat(4 * nj), cfull(4 * nj), ccentre(3 * nj);
+  for (int64_t q = 0; q < nj; q++) {
+    const int64_t c = job[q];
+    const Cell& cell = cells[c];
+    if (cell.free.size() > (size_t)kMaxFree || cell.across.size() > (size_t)kMaxHandles ||
+        cell.bonds.size() > (size_t)kMaxBonds)
+      throw StError{ST_BAD_INPUT, "gather: a coarse cell holds more than 12 free nodes, 6 faces or 480 bonds"};
+    nf[q] = (int32_t)cell.free.size();
+    nh[q] = (int32_t)cell.across.size();
+    nb[q] = (int32_t)cell.bonds.size();
+    foff[q] = (int64_t)fx.size() / 3;
+    hoff[q] = (int64_t)hx.size() / 3;
+    boff[q] = (int64_t)bij.size() / 2;
+    for (int64_t v : cell.free)
+      for (int k = 0; k < 3; k++) fx.push_back(fine.x[v].c[k]);
+    for (int64_t k : ends[c])
+      for (int t = 0; t < 3; t++) hx.push_back(out.x[k].c[t]);
+    // which of its handles its matter joins (gather.rs:42-50): a piece of each
+    std::vector<int64_t> parent(cell.free.size() + cell.across.size());
+    for (size_t k = 0; k < parent.size(); k++) parent[k] = (int64_t)k;
+    auto root = [&](int64_t a) {
+      while (parent[a] != a) a = parent[a] = parent[parent[a]];
+      return a;
+    };
+    for (auto& t : cell.bonds) parent[root(t[0])] = root(t[1]);
+    for (size_t h = 0; h < cell.across.size(); h++) piece.push_back((int32_t)root((int64_t)(cell.free.size() + h)));
+    for (auto& t : cell.bonds) {
+      bij.push_back((int32_t)t[0]);
+      bij.push_back((int32_t)t[1]);
+      const int64_t b = t[2];
+      for (int k = 0; k < 3; k++) bgeo.push_back(fine.p[b].c[k]);
+      for (int k = 0; k < 3; k++) bgeo.push_back(fine.n[b].c[k]);
+      bgeo.push_back(fine.a[b]);
+      bgeo.push_back(fine.d[b]);
+      for (int k = 1; k < 6; k++) bmat.push_back(fine.bmat[b][k]);
+    }
+    cmat[4 * q] = mat[c][1];
+    for (int k = 0; k < 3; k++) cmat[4 * q + 1 + k] = mat[c][3 + k];
+    const std::array<double, 4> f = against_full ? full_of(ctx, mat[c]) : std::array<double, 4>{1.0, 1.0, 1.0, 1.0};
+    for (int k = 0; k < 4; k++) cfull[4 * q + k] = f[k];
+    const G3 ce = hcentre(sites[c], level);
+    for (int k = 0; k < 3; k++) ccentre[3 * q + k] = ce.c[k];
+  }
+  std::vector<int32_t> nfound(nj), fab((size_t)nj * 2 * kMaxPairs);
+  std::vector<double> fval((size_t)nj * 2 * kMaxPairs);
+  if (nj) {
+    ST_HIP(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    DBuf dnf, dnh, dnb, dboff, dfoff, dhoff, dfx, dhx, dpiece, dbij, dbgeo, dbmat, dcmat, dcfull, dcc, dfound, dfab,
+        dfval, dws;
+    put(dnf, nf, st), put(dnh, nh, st), put(dnb, nb, st), put(dboff, boff, st), put(dfoff, foff, st);
+    put(dhoff, hoff, st), put(dfx, fx, st), put(dhx, hx, st), put(dpiece, piece, st), put(dbij, bij, st);
+    put(dbgeo, bgeo, st), put(dbmat, bmat, st), put(dcmat, cmat, st), put(dcfull, cfull, st), put(dcc, ccentre, st);
+    dfound.ensure(nj * sizeof(int32_t));
+    dfab.ensure(fab.size() * sizeof(int32_t));
+    dfval.ensure(fval.size() * sizeof(double));
+    // lanes per launch: the workspace is 316 KB a lane
+    const int64_t chunk = std::min<int64_t>(nj, 16384);
+    dws.ensure((size_t)chunk * kWsDoubles * sizeof(double));
+    for (int64_t c0 = 0; c0 < nj; c0 += chunk) {
+      const int64_t cn = std::min(chunk, nj - c0);
+      GJob j;
+      j.n = (int32_t)cn;
+      j.nf = dnf.as<int32_t>() + c0, j.nh = dnh.as<int32_t>() + c0, j.nb = dnb.as<int32_t>() + c0;
+      j.boff = dboff.as<int64_t>() + c0, j.foff = dfoff.as<int64_t>() + c0, j.hoff = dhoff.as<int64_t>() + c0;
+      j.fx = dfx.as<double>(), j.hx = dhx.as<double>(), j.piece = dpiece.as<int32_t>();
+      j.bij = dbij.as<int32_t>(), j.bgeo = dbgeo.as<double>(), j.bmat = dbmat.as<double>();
+      j.cmat = dcmat.as<double>() + 4 * c0, j.cfull = dcfull.as<double>() + 4 * c0, j.ccentre = dcc.as<double>() + 3 * c0;
+      j.s = s;
+      j.against_full = against_full ? 1 : 0;
+      j.nfound = dfound.as<int32_t>() + c0;
+      j.fab = dfab.as<int32_t>() + c0 * 2 * kMaxPairs;
+      j.fval = dfval.as<double>() + c0 * 2 * kMaxPairs;
+      k_measure_cells<<<(unsigned)((cn + 63) / 64), 64, 0, st>>>(j, dws.as<double>(), cn);
+      ST_HIP(hipGetLastError());
+    }
+    ST_HIP(hipMemcpyAsync(nfound.data(), dfound.p, nj * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipMemcpyAsync(fab.data(), dfab.p, fab.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipMemcpyAsync(fval.data(), dfval.p, fval.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    ST_HIP(hipStreamSynchronize(st));
+  }
+
+  // each cell's weight on its faces, as much as each is joined; its beams (gather.rs:369-392)
+  std::vector<int64_t> slot(ncell, -1);
+  for (int64_t q = 0; q < nj; q++) slot[job[q]] = q;
+  for (int64_t c = 0; c < ncell; c++) {
+    const Cell& cell = cells[c];
+    const int64_t q = slot[c];
+    const int nfd = q >= 0 ? nfound[q] : 0;
+    const int32_t* ab = q >= 0 ? fab.data() + q * 2 * kMaxPairs : nullptr;
+    const double* val = q >= 0 ? fval.data() + q * 2 * kMaxPairs : nullptr;
+    double mass = 0.0;
+    for (int64_t v : cell.free) mass += fine.m[v];
+    std::vector<double> joined(ends[c].size(), 0.0);
+    for (int k = 0; k < nfd; k++) {
+      joined[ab[2 * k]] += val[2 * k];
+      joined[ab[2 * k + 1]] += val[2 * k];
+    }
+    double total = 0.0;
+    for (double jv : joined) total += jv;
+    for (size_t h = 0; h < ends[c].size(); h++)
+      out.m[ends[c][h]] += mass * (total > 0.0 ? joined[h] / total : 1.0 / (double)ends[c].size());
+    for (int k = 0; k < nfd; k++) {
+      const int64_t ea = ends[c][ab[2 * k]], eb = ends[c][ab[2 * k + 1]];
+      if (out.held[ea] && out.held[eb]) continue;
+      const G3 xa = out.x[ea], xb = out.x[eb];
+      const double length = gnorm(gsub(xb, xa));
+      Mat6 st = mat[c];
+      for (int t = 3; t < 6; t++) st[t] *= val[2 * k + 1];  // Mat::stronger (model.rs:71-73)
+      hbeam(out, ea, eb, xa, xb, val[2 * k] * length / mat[c][1], st);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace st
+
+using namespace st;
+
+int st_gather(StCtx* c, const StLattice* fine, int against_full, int64_t* nnodes_out, int64_t* nbonds_out) {
+  return guard(c, [&] {
+    if (!fine || fine->nnodes < 0 || fine->nbonds < 0 || fine->level > 40)
+      throw StError{ST_BAD_INPUT, "gather: no lattice, or negative sizes"};
+    if (fine->nnodes && (!fine->x || !fine->m || !fine->node_mat || !fine->held || !fine->around))
+      throw StError{ST_BAD_INPUT, "gather: a node array is NULL"};
+    if (fine->nbonds && (!fine->ij || !fine->p || !fine->n || !fine->a || !fine->d || !fine->bond_mat))
+      throw StError{ST_BAD_INPUT, "gather: a bond array is NULL"};
+    HLat L;
+    L.level = fine->level;
+    const int64_t nn = fine->nnodes, nb = fine->nbonds;
+    L.x.resize(nn), L.m.resize(nn), L.nmat.resize(nn), L.held.resize(nn), L.around.resize(nn);
+    for (int64_t v = 0; v < nn; v++) {
+      for (int k = 0; k < 3; k++) L.x[v].c[k] = fine->x[3 * v + k];
+      L.m[v] = fine->m[v];
+      for (int k = 0; k < 6; k++) L.nmat[v][k] = fine->node_mat[6 * v + k];
+      L.held[v] = fine->held[v] != 0;
+      for (int a = 0; a < 2; a++)
+        for (int k = 0; k < 3; k++) L.around[v][a][k] = fine->around[6 * v + 3 * a + k];
+    }
+    L.ij.resize(nb), L.p.resize(nb), L.n.resize(nb), L.a.resize(nb), L.d.resize(nb), L.bmat.resize(nb);
+    for (int64_t b = 0; b < nb; b++) {
+      for (int k = 0; k < 2; k++) {
+        L.ij[b][k] = fine->ij[2 * b + k];
+        if (L.ij[b][k] < 0 || L.ij[b][k] >= nn) throw StError{ST_BAD_INPUT, "gather: a bond's node is out of range"};
+      }
+      for (int k = 0; k < 3; k++) L.p[b].c[k] = fine->p[3 * b + k], L.n[b].c[k] = fine->n[3 * b + k];
+      L.a[b] = fine->a[b];
+      L.d[b] = fine->d[b];
+      for (int k = 0; k < 6; k++) L.bmat[b][k] = fine->bond_mat[6 * b + k];
+    }
+    HLat out;
+    measure(c, L, against_full != 0, out);
+    GatherOut& g = c->gathered;
+    g = GatherOut();
+    g.level = out.level;
+    const size_t on = out.x.size(), ob = out.a.size();
+    g.x.resize(3 * on), g.m = out.m, g.node_mat.resize(6 * on), g.held = out.held, g.around.resize(6 * on);
+    for (size_t v = 0; v < on; v++) {
+      for (int k = 0; k < 3; k++) g.x[3 * v + k] = out.x[v].c[k];
+      for (int k = 0; k < 6; k++) g.node_mat[6 * v + k] = out.nmat[v][k];
+      for (int a = 0; a < 2; a++)
+        for (int k = 0; k < 3; k++) g.around[6 * v + 3 * a + k] = out.around[v][a][k];
+    }
+    g.ij.resize(2 * ob), g.p.resize(3 * ob), g.n.resize(3 * ob), g.a = out.a, g.d = out.d, g.bond_mat.resize(6 * ob);
+    for (size_t b = 0; b < ob; b++) {
+      g.ij[2 * b] = out.ij[b][0], g.ij[2 * b + 1] = out.ij[b][1];
+      for (int k = 0; k < 3; k++) g.p[3 * b + k] = out.p[b].c[k], g.n[3 * b + k] = out.n[b].c[k];
+      for (int k = 0; k < 6; k++) g.bond_mat[6 * b + k] = out.bmat[b][k];
+    }
+    if (nnodes_out) *nnodes_out = (int64_t)on;
+    if (nbonds_out) *nbonds_out = (int64_t)ob;
+  });
+}
+
+int st_gather_read(StCtx* c, uint32_t* level, double* x, double* m, double* node_mat, uint8_t* held, int64_t* around,
+                   int64_t* ij, double* p, double* n, double* a, double* d, double* bond_mat) {
+  return guard(c, [&] {
+    const GatherOut& g = c->gathered;
+    auto copy = [](auto* to, const auto& from) {
+      if (to && !from.empty()) std::memcpy(to, from.data(), from.size() * sizeof(from[0]));
+    };
+    if (level) *level = g.level;
+    copy(x, g.x), copy(m, g.m), copy(node_mat, g.node_mat), copy(held, g.held), copy(around, g.around);
+    copy(ij, g.ij), copy(p, g.p), copy(n, g.n), copy(a, g.a), copy(d, g.d), copy(bond_mat, g.bond_mat);
+  });
+}

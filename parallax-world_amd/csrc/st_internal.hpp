@@ -18,6 +18,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -148,6 +149,14 @@ struct Level {
   DBuf dense_a, dense_off, dense_seg;  // matrices; per structure offset (int64, ns+1); seg (int64)
 };
 
+// A lattice as the reference holds it (nodes, then bonds; materials six values each)
+struct GatherOut {
+  uint32_t level = 0;
+  std::vector<double> x, m, node_mat, p, n, a, d, bond_mat;
+  std::vector<uint8_t> held;
+  std::vector<int64_t> around, ij;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -218,6 +227,10 @@ struct Ctx {
   double timing_bytes = 0.0;
   double timing_ms_done = 0.0;
   int64_t timing_launches = 0;
+  // the last gathered lattice, kept until read, and each material's full-cell measure
+  // (rho, e, g, ft, fc, fs, then section and strength ratio of both beam classes)
+  GatherOut gathered;
+  std::vector<std::array<double, 10>> full_known;
 };
 
 // host upload (st_upload.cpp)

@@ -22,9 +22,30 @@ VALUE_ONLY = 256  # or-ed into the objective: J alone
 EXPORTS = ["st_create", "st_destroy", "st_last_error", "st_upload", "st_sizes", "st_node_faces", "st_lookup_faces",
            "st_node_held", "st_joined_to_held", "st_set_gone", "st_set_beams", "st_apply", "st_solve",
            "st_set_loads", "st_solve_resident", "st_get_solution", "st_classify", "st_kernel_timing_enable",
-           "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_bond_used", "st_settle", "st_grid_cells", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness"]
+           "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_bond_used", "st_settle", "st_grid_cells", "st_gather", "st_gather_read", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness"]
 
 UNIT = 0.125  # metres to a level-0 cell (statics/src/lattice.rs:16)
+
+
+class StLattice(ctypes.Structure):
+    _fields_ = [("level", ctypes.c_uint32), ("nnodes", ctypes.c_int64), ("x", ctypes.c_void_p), ("m", ctypes.c_void_p),
+                ("node_mat", ctypes.c_void_p), ("held", ctypes.c_void_p), ("around", ctypes.c_void_p),
+                ("nbonds", ctypes.c_int64), ("ij", ctypes.c_void_p), ("p", ctypes.c_void_p), ("n", ctypes.c_void_p),
+                ("a", ctypes.c_void_p), ("d", ctypes.c_void_p), ("bond_mat", ctypes.c_void_p)]
+
+
+class GatheredLattice:
+    """A lattice as the reference holds it (statics/src/lattice.rs:61-71): node arrays x (n, 3), m,
+    node_mat (n, 6: rho, e, g, ft, fc, fs), held, around (n, 2, 3); bond arrays ij (b, 2), p, n (b, 3),
+    a, d, bond_mat (b, 6)."""
+
+    def __init__(self, level, nn, nb):
+        self.level = level
+        self.x, self.m = np.zeros((nn, 3)), np.zeros(nn)
+        self.node_mat, self.held = np.zeros((nn, 6)), np.zeros(nn, np.uint8)
+        self.around = np.zeros((nn, 2, 3), np.int64)
+        self.ij, self.p, self.n = np.zeros((nb, 2), np.int64), np.zeros((nb, 3)), np.zeros((nb, 3))
+        self.a, self.d, self.bond_mat = np.zeros(nb), np.zeros(nb), np.zeros((nb, 6))
 
 
 class StCells(ctypes.Structure):
@@ -89,6 +110,8 @@ def lib():
             "st_bond_forces": [vp, vp, vp],
             "st_bond_used": [vp, vp, vp, vp],
             "st_grid_cells": [vp, ctypes.POINTER(StGrid), ctypes.c_int32, dbl, vp, vp, vp, vp],
+            "st_gather": [vp, ctypes.POINTER(StLattice), ctypes.c_int, vp, vp],
+            "st_gather_read": [vp] * 13,
             "st_settle": [vp, vp, vp, ctypes.POINTER(StSolveOpts), vp, vp, vp, ctypes.POINTER(StStats)],
             "st_backward": [vp, ctypes.c_int, dbl, ctypes.POINTER(StSolveOpts), vp, vp, ctypes.POINTER(StStats)],
             "st_solve_batch": [vp, i32, vp, vp, ctypes.POINTER(StSolveOpts), vp, vp],
@@ -271,6 +294,27 @@ class Context:
         self._check(self.L.st_grid_cells(self.h, ctypes.byref(gr), level, hold_radius, ctypes.byref(m), _p(sites),
                                          _p(held), _p(index)), "st_grid_cells")
         return sites, held, index
+
+    def gather(self, fine, against_full=True):
+        """gather::gathered (against_full) or measured(fine, false) (statics/src/gather.rs:193-395) of a
+        lattice with the attributes GatheredLattice names: the next level up, as a GatheredLattice."""
+        arr = {k: np.ascontiguousarray(getattr(fine, k), t) for k, t in
+               [("x", np.float64), ("m", np.float64), ("node_mat", np.float64), ("held", np.uint8),
+                ("around", np.int64), ("ij", np.int64), ("p", np.float64), ("n", np.float64), ("a", np.float64),
+                ("d", np.float64), ("bond_mat", np.float64)]}
+        lat = StLattice(int(fine.level), len(arr["m"]), _p(arr["x"]), _p(arr["m"]), _p(arr["node_mat"]),
+                        _p(arr["held"]), _p(arr["around"]), len(arr["a"]), _p(arr["ij"]), _p(arr["p"]),
+                        _p(arr["n"]), _p(arr["a"]), _p(arr["d"]), _p(arr["bond_mat"]))
+        nn, nb = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.L.st_gather(self.h, ctypes.byref(lat), 1 if against_full else 0, ctypes.byref(nn),
+                                     ctypes.byref(nb)), "st_gather")
+        out = GatheredLattice(0, nn.value, nb.value)
+        level = ctypes.c_uint32()
+        self._check(self.L.st_gather_read(self.h, ctypes.byref(level), _p(out.x), _p(out.m), _p(out.node_mat),
+                                          _p(out.held), _p(out.around), _p(out.ij), _p(out.p), _p(out.n),
+                                          _p(out.a), _p(out.d), _p(out.bond_mat)), "st_gather_read")
+        out.level = level.value
+        return out
 
     def bond_used(self, strength, x):
         """Per uploaded cell [15][4] used shares; strength: (cells, 3) ft, fc, fs."""

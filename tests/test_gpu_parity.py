@@ -506,3 +506,60 @@ def test_settle_parity_collapse(ctx):
     o, gone, beams, rounds, fallen = settle_both(P, precision=sh.MIXED, warm_start=True, refresh=-1)
     assert o["rounds"] >= 3 and rounds == o["rounds"], (rounds, o["rounds"])
     assert (gone == o["gone"]).all() and abs(fallen - o["fallen"]) <= 1e-12
+
+
+# ---------------------------------------------------------------- gather.rs
+
+GATHER_FIELDS = ["x", "m", "node_mat", "held", "around", "ij", "p", "n", "a", "d", "bond_mat"]
+
+
+def gather_same(dev, want):
+    """The device's gathered lattice against the oracle's: the same nodes and bonds in the same
+    order, every value equal (the measure runs the reference's operations in its order)."""
+    assert dev.level == want.level
+    assert len(dev.m) == want.n_nodes and len(dev.a) == want.n_bonds, (len(dev.m), want.n_nodes, len(dev.a),
+                                                                       want.n_bonds)
+    for f in GATHER_FIELDS:
+        a, b = getattr(dev, f), getattr(want, f)
+        b = b.reshape(a.shape)
+        if not np.array_equal(a, b):
+            diff = np.abs(a.astype(float) - b.astype(float))
+            k = int(np.argmax(diff))
+            raise AssertionError(f"{f}: {int((a != b).sum())} of {a.size} differ, worst at {k}: "
+                                 f"{a.ravel()[k]!r} against {b.ravel()[k]!r}")
+
+
+def _two_material_blob(n, seed):
+    rng = np.random.default_rng(seed)
+    sites, mats, held = [], [], []
+    for x in range(n):
+        for y in range(n):
+            for z in range(n):
+                if rng.random() < 0.7 or y == 0:
+                    sites.append((x - n // 2, y, z - n // 2))
+                    mats.append(orc.STONE if rng.random() < 0.6 else orc.WOOD)
+                    held.append(1 if y == 0 else 0)
+    return orc.Lattice(0, np.array(sites), np.array(mats), np.array(held, np.uint8))
+
+
+@pytest.mark.parametrize("case", ["bar", "blob", "blob_measured", "truss", "cantilever"])
+def test_gather_parity(ctx, case):
+    """gather::gathered on the device equals the oracle's, level after level."""
+    if case == "bar":
+        fill = lambda x: (orc.STEEL, False) if (abs(x[0]) < 4.0 and abs(x[1]) < 0.2 and abs(x[2]) < 0.2) else None
+        lat = orc.Lattice(0, *scenes.solids(0, [-4.5, -1.0, -1.0], [4.5, 1.0, 1.0], fill))
+        levels, full = 3, True
+    elif case.startswith("blob"):
+        lat, levels, full = _two_material_blob(12, 3), 2, case == "blob"
+    elif case == "truss":
+        lat = orc.Lattice(0, np.array([[0, 0, 0], [1, 1, 1]]), [orc.STONE] * 2, np.zeros(2, np.uint8))
+        levels, full = 1, True
+    else:
+        lat = orc.Lattice(0, *scenes.solids(0, *scenes.cantilever(orc.WOOD, 0.5, 3.0)))
+        levels, full = 2, True
+    print(f"gather {case}: {lat.n_nodes} nodes, {lat.n_bonds} bonds", flush=True)
+    for _ in range(levels):
+        want = lat.gathered(against_full=full)
+        dev = ctx.gather(lat, against_full=full)
+        gather_same(dev, want)
+        lat = want
