@@ -485,6 +485,126 @@ def ingest_bench(args):
     print(json.dumps(out), flush=True)
 
 
+FACES6 = np.array([[-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0], [0, 0, -1], [0, 0, 1]], np.int64)
+
+
+def level0_lattice(sites, held, mat):
+    """lattice::build (statics/src/lattice.rs:124-163) at level 0, all cells of one material, in
+    numpy: nodes in the order build makes them (each free cell's faces in VOXEL_FACES order), its
+    beams cell by cell, PER_BEAM bonds each (lattice.rs:76-84)."""
+    s = sh.UNIT
+    sites = np.asarray(sites, np.int64)
+    held = np.asarray(held, bool)
+    lo_all = sites.min(0) - 1
+    dims = sites.max(0) - lo_all + 2
+    hgrid = np.zeros(dims, bool)
+    hs = sites[held] - lo_all
+    hgrid[hs[:, 0], hs[:, 1], hs[:, 2]] = True
+    free = sites[~held]
+    other = free[:, None, :] + FACES6[None]
+    plus = (np.arange(6) % 2 == 1)[None, :, None]
+    lo = np.where(plus, free[:, None, :], other)
+    hi = np.where(plus, other, free[:, None, :])
+    rel = lo - lo_all
+    code = ((rel[..., 0] * dims[1] + rel[..., 1]) * dims[2] + rel[..., 2]) * 3 + np.arange(6)[None] // 2
+    uniq, first, inv = np.unique(code.ravel(), return_index=True, return_inverse=True)
+    rank = np.empty(len(uniq), np.int64)
+    rank[np.argsort(first)] = np.arange(len(uniq))
+    node = rank[inv].reshape(-1, 6)
+    nn = len(uniq)
+    centre = lambda q: (q.astype(np.float64) + 0.5) * s  # noqa: E731
+    lat = sh.GatheredLattice(0, 0, 0)
+    lat.level = 0
+    x = np.zeros((nn, 3))
+    x[node.ravel()] = ((centre(free[:, None, :].repeat(6, 1)) + centre(other)) * 0.5).reshape(-1, 3)
+    lat.x = x
+    lat.m = np.bincount(node.ravel(), minlength=nn) * (mat[0] * s * s * s / 6.0)
+    lat.node_mat = np.tile(np.asarray(mat, np.float64), (nn, 1))
+    oh = other - lo_all
+    nheld = np.zeros(nn, np.uint8)
+    nheld[node.ravel()] = hgrid[oh[..., 0], oh[..., 1], oh[..., 2]].ravel()
+    lat.held = nheld
+    around = np.zeros((nn, 2, 3), np.int64)
+    around[node.ravel(), 0] = lo.reshape(-1, 3)
+    around[node.ravel(), 1] = hi.reshape(-1, 3)
+    lat.around = around
+    pa, pb = np.triu_indices(6, 1)
+    ea, eb = node[:, pa], node[:, pb]
+    keep = ~(nheld[ea].astype(bool) & nheld[eb].astype(bool))
+    ea, eb = ea[keep], eb[keep]
+    xa, xb = x[ea], x[eb]
+    unit = lambda v: v / np.linalg.norm(v, axis=-1, keepdims=True)  # noqa: E731
+    nv = unit(xb - xa)
+    ref = np.where((np.abs(nv[:, 0]) < 0.9)[:, None], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0])
+    e1 = unit(np.cross(nv, ref))
+    e2 = np.cross(nv, e1)
+    area = 0.336 * s * s
+    r = np.sqrt(area / (2.0 * np.pi))
+    mid = (xa + xb) * 0.5
+    off = np.stack([e1 * r, -e1 * r, e2 * r, -e2 * r], 1)
+    nb = len(ea) * 4
+    lat.ij = np.stack([ea, eb], 1).repeat(4, 0)
+    lat.p = (mid[:, None, :] + off).reshape(-1, 3)
+    lat.n = nv.repeat(4, 0)
+    lat.a = np.full(nb, area / 4.0)
+    lat.d = np.linalg.norm(xb - xa, axis=1).repeat(4)
+    lat.bond_mat = np.tile(np.asarray(mat, np.float64), (nb, 1))
+    return lat
+
+
+def gather_bench(args):
+    """gather::gathered on the device (st_gather: the host's bookkeeping, the cells measured on the
+    device, the coarse lattice assembled) for the bench generator's --gather^3 structure built as a
+    level-0 lattice of stone; the oracle's gathered on a smaller structure as the CPU baseline."""
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    n = args.gather
+    stone = (2500.0, 30e9, 12e9, 5e6, 100e6, 10e6)  # statics/src/model.rs:53
+    occ, _ = structure(n, args.seed, spacing=args.spacing)
+    sites, held, _, _ = sh.cells_from_occ(occ)
+    del occ
+    t0 = time.perf_counter()
+    lat = level0_lattice(sites, held, stone)
+    log(f"[bench] gather: {len(sites)} cells ({int(held.sum())} held), {len(lat.m)} nodes, {len(lat.a)} bonds "
+        f"built in {time.perf_counter() - t0:.1f} s")
+    ctx = sh.Context()
+    times = []
+    for i in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        g = ctx.gather(lat)
+        dt = time.perf_counter() - t0
+        if i >= args.warmup:
+            times.append(dt)
+        log(f"[bench] gather {i}: {dt * 1e3:.1f} ms, {len(g.m)} nodes, {len(g.a)} bonds, "
+            f"weight kept to {abs(g.m.sum() / lat.m.sum() - 1):.1e}")
+    ms = 1e3 * sum(times) / len(times)
+    cpu = None
+    if args.cpu_sample > 0:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle as orc
+        m = min(args.gather_cpu_sample, n)
+        occ_s, _ = structure(m, args.seed, spacing=args.spacing)
+        s_sites, s_held, _, _ = sh.cells_from_occ(occ_s)
+        olat = orc.Lattice(0, s_sites.astype(np.int64), stone, s_held)
+        orc.lib().orc_set_threads(args.cpu_threads)
+        t0 = time.perf_counter()
+        olat.gathered()
+        sec = time.perf_counter() - t0
+        cpu = {"value": len(s_sites) / sec, "unit": "cells/sec", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"oracle gathered (statics/src/gather.rs:228-395 restated, cells over {args.cpu_threads} "
+                         f"threads as the reference's `each`) on the {m}^3 structure's {len(s_sites)} cells: "
+                         f"{sec:.2f} s"}
+    out = {"metric": f"level-0 cells/sec gathered to level 1 ({n}^3 structure, gather::gathered on the device)",
+           "value": len(sites) / (ms * 1e-3), "unit": "cells/sec", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"the bench generator's {n}^3 structure, all stone, as a level-0 lattice; "
+                                  f"step = st_gather + st_gather_read, host bookkeeping and copies included",
+                      "cells": int(len(sites)), "nodes": int(len(lat.m)), "bonds": int(len(lat.a)),
+                      "coarse_nodes": int(len(g.m)), "coarse_bonds": int(len(g.a))},
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -508,6 +628,9 @@ def main():
                          "of a --size structure (64 at 128: the adjoint config)")
     ap.add_argument("--ingest", type=int, default=0, metavar="N",
                     help="instead: solids + hold_bulk from a lattice::Grid of one N^3 tile (st_grid_cells)")
+    ap.add_argument("--gather", type=int, default=0, metavar="N",
+                    help="instead: gather::gathered on the device for the N^3 structure (see gather_bench)")
+    ap.add_argument("--gather-cpu-sample", type=int, default=96, help="gather: edge of the CPU oracle's sample")
     ap.add_argument("--settle", type=int, default=0, metavar="N",
                     help="instead: bodies::settle on the device for one N^3 structure (see settle_bench)")
     ap.add_argument("--settle-cpu-sample", type=int, default=32, help="settle: edge of the CPU oracle's sample")
@@ -536,6 +659,9 @@ def main():
         return
     if args.settle:
         settle_bench(args)
+        return
+    if args.gather:
+        gather_bench(args)
         return
     if args.ingest:
         ingest_bench(args)

@@ -12,7 +12,10 @@
 // reference's order, without contraction (the pragma below), so the choices, and the
 // lattice, are the reference's.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <unordered_map>
@@ -36,13 +39,11 @@ constexpr double kUnit = 0.125;                        // lattice.rs:16
 constexpr int kPerBeam = 4;                            // lattice.rs:19
 
 // per-lane workspace, in doubles: the elimination, the cases' bond forces, the stiffness
-// between handles, the beam units, the target, the beams' normal equations, a pulled beam
+// between handles, the beams' normal equations, a pulled beam
 constexpr int64_t kOffA = 0;
 constexpr int64_t kOffCases = kOffA + (int64_t)6 * kMaxFree * kRow;
 constexpr int64_t kOffK = kOffCases + (int64_t)kN2 * kMaxBonds * 3;
-constexpr int64_t kOffU = kOffK + kN2 * kN2;
-constexpr int64_t kOffT = kOffU + (int64_t)kMaxPairs * kN2 * kN2;
-constexpr int64_t kOffG = kOffT + kN2 * kN2;
+constexpr int64_t kOffG = kOffK + kN2 * kN2;
 constexpr int64_t kOffPull = kOffG + kMaxPairs * (kMaxPairs + 1);
 constexpr int64_t kWsDoubles = kOffPull + (int64_t)kMaxBonds * 3;
 
@@ -80,9 +81,9 @@ struct GJob {
   const double* fx;       // free parts' positions, 3 each
   const double* hx;       // handles' positions, 3 each
   const int32_t* piece;   // per handle, the piece of matter it is joined to
-  const int32_t* bij;     // per bond, its two places (free parts first, then handles)
-  const double* bgeo;     // per bond: p (3), n (3), a, d
-  const double* bmat;     // per bond: e, g, ft, fc, fs
+  const int32_t* bij;     // per bond of a cell, its two places (free parts first, then handles)
+  const int64_t* bidx;    // per bond of a cell, the finer lattice's bond
+  const double *bp, *bn, *ba, *bd, *bm;  // the finer lattice's bonds: p, n (3 each), a, d, material (6)
   const double* cmat;     // per cell: e, ft, fc, fs of its material
   const double* cfull;    // per cell: section and strength ratio of both classes
   const double* ccentre;  // per cell: its centre
@@ -111,19 +112,27 @@ __device__ double lane_solve(const Ws& w, int64_t off, int pitch, int n, int rhs
     int p = c;
     for (int x = c + 1; x < n; x++)
       if (!(fabs(A(x, c)) < fabs(A(p, c)))) p = x;  // the last of the greatest, as max_by
+    const int wd = n + rhs;
     if (p != c)
-      for (int k = 0; k < n + rhs; k++) {
-        double t = A(c, k);
-        A(c, k) = A(p, k);
-        A(p, k) = t;
+      for (int k0 = 0; k0 < wd; k0 += 8) {
+        double u[8], v[8];
+        const int kn = min(8, wd - k0);
+        for (int t = 0; t < kn; t++) u[t] = A(c, k0 + t), v[t] = A(p, k0 + t);
+        for (int t = 0; t < kn; t++) A(c, k0 + t) = v[t], A(p, k0 + t) = u[t];
       }
     const double pv = A(c, c);
     if (fabs(pv) <= tiny) continue;
     for (int r = 0; r < n; r++) {
       if (r == c) continue;
       const double f = A(r, c) / pv;
-      if (f != 0.0)
-        for (int k = c; k < n + rhs; k++) A(r, k) -= f * A(c, k);
+      if (f == 0.0) continue;
+      // eight entries at a time: the pivot row's and this row's loaded together
+      for (int k0 = c; k0 < wd; k0 += 8) {
+        double u[8], v[8];
+        const int kn = min(8, wd - k0);
+        for (int t = 0; t < kn; t++) u[t] = A(c, k0 + t), v[t] = A(r, k0 + t);
+        for (int t = 0; t < kn; t++) A(r, k0 + t) = v[t] - f * u[t];
+      }
     }
   }
   return tiny;
@@ -144,8 +153,18 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
   const double* hx = j.hx + 3 * j.hoff[c];
   const int32_t* piece = j.piece + j.hoff[c];
   const int32_t* bij = j.bij + 2 * j.boff[c];
-  const double* bgeo = j.bgeo + 8 * j.boff[c];
-  const double* bmat = j.bmat + 5 * j.boff[c];
+  const int64_t* bidx = j.bidx + j.boff[c];
+  auto BP = [&](int b) -> G3 {
+    const double* q = j.bp + 3 * bidx[b];
+    return {{q[0], q[1], q[2]}};
+  };
+  auto BN = [&](int b) -> G3 {
+    const double* q = j.bn + 3 * bidx[b];
+    return {{q[0], q[1], q[2]}};
+  };
+  auto BA = [&](int b) { return j.ba[bidx[b]]; };
+  auto BD = [&](int b) { return j.bd[bidx[b]]; };
+  auto BM = [&](int b, int k) { return j.bm[6 * bidx[b] + k]; };  // rho, e, g, ft, fc, fs
   auto pos = [&](int v) -> G3 {
     const double* q = v < nf ? fx + 3 * v : hx + 3 * (v - nf);
     return {{q[0], q[1], q[2]}};
@@ -163,9 +182,9 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
   for (int r = 0; r < n; r++)
     for (int k = 0; k < n + m; k++) A(r, k) = 0.0;
   for (int b = 0; b < nb; b++) {
-    const G3 p = P3(bgeo + 8 * b), nv = P3(bgeo + 8 * b + 3);
-    const double kk = bgeo[8 * b + 6] / bgeo[8 * b + 7];
-    const double e = bmat[5 * b], g = bmat[5 * b + 1];
+    const G3 p = BP(b), nv = BN(b);
+    const double kk = BA(b) / BD(b);
+    const double e = BM(b, 1), g = BM(b, 2);
     double st[3][3];  // Bond::stiffness (model.rs:139-149)
     for (int r = 0; r < 3; r++)
       for (int q = 0; q < 3; q++) {
@@ -181,13 +200,19 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
         const G3 cv = column(vs[x], d, p);
         const G3 scv{{gdot(G3{{st[0][0], st[0][1], st[0][2]}}, cv), gdot(G3{{st[1][0], st[1][1], st[1][2]}}, cv),
                       gdot(G3{{st[2][0], st[2][1], st[2][2]}}, cv)}};
+        // the row's twelve entries this bond adds to are distinct: load them together
+        double add[12], cur[12];
+        int cols[12];
         for (int y = 0; y < 2; y++)
           for (int d2 = 0; d2 < 6; d2++) {
             const int col = dof(vs[y], d2);
-            if (col < 0) continue;
+            cols[6 * y + d2] = col;
             const double sign = col < n ? 1.0 : -1.0;
-            A(r, col) += sign * sg[x] * sg[y] * gdot(scv, column(vs[y], d2, p));
+            add[6 * y + d2] = col < 0 ? 0.0 : sign * sg[x] * sg[y] * gdot(scv, column(vs[y], d2, p));
           }
+        for (int t = 0; t < 12; t++) cur[t] = cols[t] < 0 ? 0.0 : A(r, cols[t]);
+        for (int t = 0; t < 12; t++)
+          if (cols[t] >= 0) A(r, cols[t]) = cur[t] + add[t];
       }
   }
   const double tiny_a = lane_solve(w, kOffA, kRow, n, m);
@@ -196,7 +221,7 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
   auto CASE = [&](int cs, int b, int q) -> double& { return w[kOffCases + ((int64_t)cs * kMaxBonds + b) * 3 + q]; };
   for (int cs = 0; cs < m; cs++)
     for (int b = 0; b < nb; b++) {
-      const G3 p = P3(bgeo + 8 * b), nv = P3(bgeo + 8 * b + 3);
+      const G3 p = BP(b), nv = BN(b);
       G3 tr[2];
       for (int x = 0; x < 2; x++) {
         const int v = bij[2 * b + x];
@@ -213,8 +238,8 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
       const G3 delta = gsub(tr[1], tr[0]);
       // Bond::force (model.rs:115-119)
       const double along = gdot(delta, nv);
-      const double kk = bgeo[8 * b + 6] / bgeo[8 * b + 7];
-      const G3 f = gadd(gmul(nv, bmat[5 * b] * kk * along), gmul(gsub(delta, gmul(nv, along)), bmat[5 * b + 1] * kk));
+      const double kk = BA(b) / BD(b);
+      const G3 f = gadd(gmul(nv, BM(b, 1) * kk * along), gmul(gsub(delta, gmul(nv, along)), BM(b, 2) * kk));
       for (int q = 0; q < 3; q++) CASE(cs, b, q) = f.c[q];
     }
 
@@ -226,13 +251,16 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
     for (int b = 0; b < nb; b++) {
       const int sides[2] = {bij[2 * b + 1], bij[2 * b]};
       const double sign[2] = {1.0, -1.0};
+      double f[3], cur[6];
+      for (int q = 0; q < 3; q++) f[q] = CASE(cs, b, q);
+      for (int x = 0; x < 2; x++)  // two handles, six distinct entries
+        for (int q = 0; q < 3; q++) cur[3 * x + q] = sides[x] >= nf ? K(3 * (sides[x] - nf) + q, cs) : 0.0;
       for (int x = 0; x < 2; x++)
         if (sides[x] >= nf)
-          for (int q = 0; q < 3; q++) K(3 * (sides[x] - nf) + q, cs) += sign[x] * CASE(cs, b, q);
+          for (int q = 0; q < 3; q++) K(3 * (sides[x] - nf) + q, cs) = cur[3 * x + q] + sign[x] * f[q];
     }
 
   // beams (gather.rs:119-161)
-  const int n2 = m;
   int pa[kMaxPairs], pb[kMaxPairs], np = 0;
   for (int a = 0; a < nh; a++)
     for (int b = a + 1; b < nh; b++)
@@ -241,22 +269,37 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
         pb[np] = b;
         np++;
       }
-  auto U = [&](int p, int i) -> double& { return w[kOffU + (int64_t)p * kN2 * kN2 + i]; };
-  auto T = [&](int i) -> double& { return w[kOffT + i]; };
-  for (int p = 0; p < np; p++) {
-    const int a = pa[p], b = pb[p];
-    const G3 e = gunit(gsub(pos(nf + b), pos(nf + a)));
-    for (int i = 0; i < n2 * n2; i++) U(p, i) = 0.0;
-    for (int r = 0; r < 3; r++)
-      for (int q = 0; q < 3; q++) {
-        const double v = e.c[r] * e.c[q];
-        U(p, (3 * a + r) * n2 + 3 * a + q) += v;
-        U(p, (3 * b + r) * n2 + 3 * b + q) += v;
-        U(p, (3 * a + r) * n2 + 3 * b + q) -= v;
-        U(p, (3 * b + r) * n2 + 3 * a + q) -= v;
+  // A unit (a beam of unit stiffness along its direction, as a matrix) is nonzero only in the
+  // four 3x3 blocks of its two handles: +e e^T on the diagonal blocks, -e e^T off them. Its
+  // products with another matrix are summed over those 36 entries in increasing index; the
+  // reference sums over all (3 nh)^2, but every other term is a signed zero, which leaves a sum
+  // begun at +0 unchanged, so the sums are the same to the bit.
+  G3 ue[kMaxPairs];
+  for (int p = 0; p < np; p++) ue[p] = gunit(gsub(pos(nf + pb[p]), pos(nf + pa[p])));
+  // unit p's entry (row, col), or 0
+  auto U = [&](int p, int row, int col) -> double {
+    const int hr = row / 3, hc = col / 3;
+    const bool ra = hr == pa[p], rb = hr == pb[p], ca = hc == pa[p], cb = hc == pb[p];
+    if (!((ra || rb) && (ca || cb))) return 0.0;
+    const double v = ue[p].c[row % 3] * ue[p].c[col % 3];
+    return (ra == ca) ? 0.0 + v : 0.0 - v;
+  };
+  auto T = [&](int row, int col) { return 0.5 * (K(row, col) + K(col, row)); };
+  // unit p with unit q (q >= 0) or with the target (q < 0)
+  auto udot = [&](int p, int q) {
+    const int hs[2] = {pa[p], pb[p]};
+    double sum = 0.0;
+    for (int x = 0; x < 2; x++)
+      for (int r = 0; r < 3; r++) {
+        const int row = 3 * hs[x] + r;
+        for (int y = 0; y < 2; y++)
+          for (int c2 = 0; c2 < 3; c2++) {
+            const int col = 3 * hs[y] + c2;
+            sum += U(p, row, col) * (q >= 0 ? U(q, row, col) : T(row, col));
+          }
       }
-  }
-  for (int i = 0; i < n2 * n2; i++) T(i) = 0.5 * (K(i / n2, i % n2) + K(i % n2, i / n2));
+    return sum;
+  };
   const int gp = kMaxPairs + 1;
   auto G = [&](int r, int q) -> double& { return w[kOffG + r * gp + q]; };
   uint32_t on = np ? ((1u << np) - 1u) : 0u;
@@ -267,14 +310,8 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
     for (int p = 0; p < np; p++)
       if (on >> p & 1u) idx[mm++] = p;
     for (int r = 0; r < mm; r++) {
-      for (int q = 0; q < mm; q++) {
-        double s = 0.0;
-        for (int i = 0; i < n2 * n2; i++) s += U(idx[r], i) * U(idx[q], i);
-        G(r, q) = s;
-      }
-      double s = 0.0;
-      for (int i = 0; i < n2 * n2; i++) s += U(idx[r], i) * T(i);
-      G(r, mm) = s;
+      for (int q = 0; q < mm; q++) G(r, q) = udot(idx[r], idx[q]);
+      G(r, mm) = udot(idx[r], -1);
     }
     const double tiny_g = lane_solve(w, kOffG, gp, mm, 1);
     int worst = -1;
@@ -310,19 +347,18 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
     double held = 0.0, work = 0.0;
     for (int q = 0; q < nb; q++) {
       const G3 f{{PULL(q, 0), PULL(q, 1), PULL(q, 2)}};
-      const G3 nv = P3(bgeo + 8 * q + 3);
-      const double ba = bgeo[8 * q + 6], bd = bgeo[8 * q + 7];
-      const double* bm = bmat + 5 * q;
+      const G3 nv = BN(q);
+      const double ba = BA(q), bd = BD(q);
       // Bond::used (model.rs:123-129)
       const G3 t = gmul(f, 1.0 / ba);
       const double sigma = gdot(t, nv);
       const double tau2 = gmax(gdot(t, t) - sigma * sigma, 0.0);
       const double rr = sqrt(sigma * sigma / 4.0 + tau2);
-      const double u = mat_used(bm[2], bm[3], bm[4], sigma / 2.0 + rr, sigma / 2.0 - rr);
+      const double u = mat_used(BM(q, 3), BM(q, 4), BM(q, 5), sigma / 2.0 + rr, sigma / 2.0 - rr);
       // Bond::energy (model.rs:132-136)
       const double kk = ba / bd;
       const double along = gdot(f, nv);
-      const double wq = 0.5 * (along * along / (bm[0] * kk) + (gdot(f, f) - along * along) / (bm[1] * kk));
+      const double wq = 0.5 * (along * along / (BM(q, 1) * kk) + (gdot(f, f) - along * along) / (BM(q, 2) * kk));
       held += wq * u;
       work += wq;
     }
@@ -394,7 +430,24 @@ void hbeam(HLat& L, int64_t i, int64_t j, G3 a, G3 b, double area, const Mat6& m
   }
 }
 
-void measure(Ctx* c, const HLat& fine, bool against_full, HLat& out);
+// A finer lattice as flat arrays (StLattice's layout), read in place
+struct LatIn {
+  uint32_t level;
+  int64_t nn, nb;
+  const double *x, *m, *nmat;
+  const uint8_t* held;
+  const int64_t *around, *ij;
+  const double *p, *n, *a, *d, *bmat;
+};
+LatIn view_of(const HLat& L) {
+  return {L.level, (int64_t)L.m.size(), (int64_t)L.a.size(), reinterpret_cast<const double*>(L.x.data()),
+          L.m.data(), reinterpret_cast<const double*>(L.nmat.data()), L.held.data(),
+          reinterpret_cast<const int64_t*>(L.around.data()), reinterpret_cast<const int64_t*>(L.ij.data()),
+          reinterpret_cast<const double*>(L.p.data()), reinterpret_cast<const double*>(L.n.data()), L.a.data(),
+          L.d.data(), reinterpret_cast<const double*>(L.bmat.data())};
+}
+
+void measure(Ctx* c, const LatIn& fine, bool against_full, HLat& out);
 
 // gather.rs:200-223: a full cell's beams by class, measured once per material. The
 // reference measures the cell at the origin of a 6^3 block of level-0 cells; the cell's
@@ -436,7 +489,7 @@ std::array<double, 4> full_of(Ctx* c, const Mat6& mat) {
           }
       }
   HLat coarse;
-  measure(c, block, false, coarse);
+  measure(c, view_of(block), false, coarse);
   const double s = hspacing(1);
   const Site3 zero{0, 0, 0};
   double sum[2][3] = {{0, 0, 0}, {0, 0, 0}};
@@ -469,42 +522,52 @@ void put(DBuf& d, const std::vector<T>& h, hipStream_t s) {
 }
 
 // gather.rs:228-395
-void measure(Ctx* ctx, const HLat& fine, bool against_full, HLat& out) {
+void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
+  using clock = std::chrono::steady_clock;
+  const auto t0 = clock::now();
   const uint32_t level = fine.level + 1;
-  // the coarse cells, numbered as first met: each finer node's two, then each bond's
-  std::unordered_map<Site3, int64_t, Site3Hash> index;
-  std::vector<Site3> sites;
-  auto cell_of = [&](const Site3& site) {
-    Site3 key;
-    for (int k = 0; k < 3; k++) key[k] = (site[k] >> level) << level;
-    auto it = index.find(key);
-    if (it != index.end()) return it->second;
-    sites.push_back(key);
-    index.emplace(key, (int64_t)sites.size() - 1);
-    return (int64_t)sites.size() - 1;
+  const int64_t nn = fine.nn, nbond = fine.nb;
+  auto site_of = [&](int64_t v, int k) {
+    const int64_t* q = fine.around + 6 * v + 3 * k;
+    return Site3{q[0], q[1], q[2]};
   };
-  const size_t nn = fine.x.size(), nbond = fine.a.size();
-  std::vector<std::array<int64_t, 2>> between(nn);
-  for (size_t v = 0; v < nn; v++) {
-    between[v][0] = cell_of(fine.around[v][0]);
-    between[v][1] = cell_of(fine.around[v][1]);
-  }
-  std::vector<int64_t> lies(nbond);
-  for (size_t b = 0; b < nbond; b++) {
-    const auto& ai = fine.around[fine.ij[b][0]];
-    const auto& aj = fine.around[fine.ij[b][1]];
-    const Site3& in = (ai[0] == aj[0] || ai[0] == aj[1]) ? ai[0] : ai[1];
-    if (!(in == aj[0] || in == aj[1])) throw StError{ST_BAD_INPUT, "gather: a bond's nodes share no cell"};
-    lies[b] = cell_of(in);
-  }
+  // the coarse cells, numbered as first met, each finer node's two in turn
+  std::unordered_map<Site3, int64_t, Site3Hash> index;
+  index.reserve((size_t)nn / 2 + 16);
+  std::vector<Site3> sites;
+  std::vector<std::array<int64_t, 2>> between((size_t)nn);
+  for (int64_t v = 0; v < nn; v++)
+    for (int k = 0; k < 2; k++) {
+      Site3 key = site_of(v, k);
+      for (int t = 0; t < 3; t++) key[t] = (key[t] >> level) << level;
+      auto it = index.find(key);
+      if (it == index.end()) {
+        it = index.emplace(key, (int64_t)sites.size()).first;
+        sites.push_back(key);
+      }
+      between[v][k] = it->second;
+    }
+  // each bond's cell: that of the site its two nodes share (the first of node i's two that
+  // node j also lies by), already numbered through node i
   const int64_t ncell = (int64_t)sites.size();
-  auto face = [&](int64_t a, int64_t b) { return sites[a] < sites[b] ? std::make_pair(a, b) : std::make_pair(b, a); };
+  std::vector<int64_t> lies((size_t)nbond), bcount((size_t)ncell + 1, 0);
+  for (int64_t b = 0; b < nbond; b++) {
+    const int64_t i = fine.ij[2 * b], j = fine.ij[2 * b + 1];
+    const Site3 i0 = site_of(i, 0), i1 = site_of(i, 1), j0 = site_of(j, 0), j1 = site_of(j, 1);
+    int k = (i0 == j0 || i0 == j1) ? 0 : 1;
+    if (k == 1 && !(i1 == j0 || i1 == j1)) throw StError{ST_BAD_INPUT, "gather: a bond's nodes share no cell"};
+    lies[b] = between[i][k];
+    bcount[lies[b] + 1]++;
+  }
+  for (int64_t c = 0; c < ncell; c++) bcount[c + 1] += bcount[c];
+  auto face_key = [&](int64_t a, int64_t b) {
+    return sites[a] < sites[b] ? std::array<int64_t, 2>{a, b} : std::array<int64_t, 2>{b, a};
+  };
   struct Cell {
     std::vector<int64_t> free, across;
-    std::vector<std::array<int64_t, 3>> bonds;  // places i, j and the finer bond
   };
-  std::vector<Cell> cells(ncell);
-  std::vector<std::vector<std::pair<Mat6, double>>> mats(ncell);
+  std::vector<Cell> cells((size_t)ncell);
+  std::vector<std::vector<std::pair<Mat6, double>>> mats((size_t)ncell);
   auto add_mat = [](std::vector<std::pair<Mat6, double>>& to, const Mat6& m, double w) {
     for (auto& x : to)
       if (x.first == m) {
@@ -513,25 +576,37 @@ void measure(Ctx* ctx, const HLat& fine, bool against_full, HLat& out) {
       }
     to.push_back({m, w});
   };
-  std::map<std::pair<int64_t, int64_t>, std::pair<double, bool>> faces;
-  for (size_t v = 0; v < nn; v++) {
+  auto nmat = [&](int64_t v) {
+    Mat6 m;
+    for (int k = 0; k < 6; k++) m[k] = fine.nmat[6 * v + k];
+    return m;
+  };
+  // the faces: (key, finer node on it, or -1 for a face only named by a handle), to be
+  // sorted by key as the reference's ordered map keeps them
+  struct FaceRec {
+    std::array<int64_t, 2> key;
+    int64_t v;
+  };
+  std::vector<FaceRec> recs;
+  for (int64_t v = 0; v < nn; v++) {
     const int64_t x = between[v][0], y = between[v][1];
-    add_mat(mats[x], fine.nmat[v], fine.m[v]);
-    if (x != y) add_mat(mats[y], fine.nmat[v], fine.m[v]);
-    if (x == y) {
-      cells[x].free.push_back((int64_t)v);
-    } else {
-      auto& f = faces[face(x, y)];
-      f.first += fine.m[v];
-      f.second = f.second || fine.held[v];
-    }
+    const Mat6 mv = nmat(v);
+    add_mat(mats[x], mv, fine.m[v]);
+    if (x != y) add_mat(mats[y], mv, fine.m[v]);
+    if (x == y)
+      cells[x].free.push_back(v);
+    else
+      recs.push_back({face_key(x, y), v});
   }
-  std::vector<int64_t> place(nn, -1);
+  std::vector<int64_t> place((size_t)nn, -1);
   for (auto& cell : cells)
     for (size_t k = 0; k < cell.free.size(); k++) place[cell.free[k]] = (int64_t)k;
-  for (size_t b = 0; b < nbond; b++) {
-    Cell& cell = cells[lies[b]];
+  // each cell's bonds in order, as places (free parts first, then a handle per face)
+  std::vector<int32_t> bij(2 * (size_t)nbond);
+  std::vector<int64_t> bidx((size_t)nbond), bfill(bcount.begin(), bcount.end() - 1);
+  for (int64_t b = 0; b < nbond; b++) {
     const int64_t c = lies[b];
+    Cell& cell = cells[c];
     auto at = [&](int64_t v) -> int64_t {
       const int64_t x = between[v][0], y = between[v][1];
       if (x == c && y == c) return place[v];
@@ -541,18 +616,19 @@ void measure(Ctx* ctx, const HLat& fine, bool against_full, HLat& out) {
       if (h == cell.across.size()) cell.across.push_back(other);
       return (int64_t)(cell.free.size() + h);
     };
-    const int64_t i = at(fine.ij[b][0]);
-    const int64_t j = at(fine.ij[b][1]);
-    cell.bonds.push_back({i, j, (int64_t)b});
+    const int64_t q = bfill[c]++;
+    bij[2 * q] = (int32_t)at(fine.ij[2 * b]);
+    bij[2 * q + 1] = (int32_t)at(fine.ij[2 * b + 1]);
+    bidx[q] = b;
   }
-  std::vector<uint8_t> held(ncell);
+  std::vector<uint8_t> held((size_t)ncell);
   for (int64_t c = 0; c < ncell; c++) {
     bool all = !cells[c].free.empty();
     for (int64_t v : cells[c].free) all = all && fine.held[v];
     held[c] = all;
+    for (int64_t o : cells[c].across) recs.push_back({face_key(c, o), -1});
   }
-  for (int64_t c = 0; c < ncell; c++)
-    for (int64_t o : cells[c].across) faces[face(c, o)];
+  std::stable_sort(recs.begin(), recs.end(), [](const FaceRec& a, const FaceRec& b) { return a.key < b.key; });
   auto mat_of = [&](std::initializer_list<int64_t> cs) {
     std::vector<std::pair<Mat6, double>> all;
     for (int64_t c : cs)
@@ -565,45 +641,58 @@ void measure(Ctx* ctx, const HLat& fine, bool against_full, HLat& out) {
   };
   out = HLat();
   out.level = level;
-  std::map<std::pair<int64_t, int64_t>, int64_t> node;
-  for (auto& [f, mh] : faces) {
-    node[f] = (int64_t)out.x.size();
-    out.x.push_back(gmul(gadd(hcentre(sites[f.first], level), hcentre(sites[f.second], level)), 0.5));
-    out.m.push_back(mh.first);
-    out.nmat.push_back(mat_of({f.first, f.second}));
-    out.held.push_back(mh.second || held[f.first] || held[f.second]);
-    out.around.push_back({sites[f.first], sites[f.second]});
+  std::vector<std::array<int64_t, 2>> keys;
+  for (size_t r = 0; r < recs.size();) {
+    const auto key = recs[r].key;
+    double m = 0.0;
+    bool on_held = false;
+    for (; r < recs.size() && recs[r].key == key; r++)
+      if (recs[r].v >= 0) {
+        m += fine.m[recs[r].v];
+        on_held = on_held || fine.held[recs[r].v];
+      }
+    keys.push_back(key);
+    out.x.push_back(gmul(gadd(hcentre(sites[key[0]], level), hcentre(sites[key[1]], level)), 0.5));
+    out.m.push_back(m);
+    out.nmat.push_back(mat_of({key[0], key[1]}));
+    out.held.push_back(on_held || held[key[0]] || held[key[1]]);
+    out.around.push_back({sites[key[0]], sites[key[1]]});
   }
+  recs = std::vector<FaceRec>();
   const double s = hspacing(level);
-  std::vector<std::vector<int64_t>> ends(ncell);
-  std::vector<Mat6> mat(ncell);
+  std::vector<std::vector<int64_t>> ends((size_t)ncell);
+  std::vector<Mat6> mat((size_t)ncell);
   for (int64_t c = 0; c < ncell; c++) {
-    for (int64_t o : cells[c].across) ends[c].push_back(node[face(c, o)]);
+    for (int64_t o : cells[c].across) {
+      const auto key = face_key(c, o);
+      ends[c].push_back(std::lower_bound(keys.begin(), keys.end(), key) - keys.begin());
+    }
     mat[c] = mat_of({c});
   }
+  const auto t1 = clock::now();
 
-  // the cells to measure, packed for the device
+  // the cells to measure
   std::vector<int64_t> job;
   for (int64_t c = 0; c < ncell; c++)
     if (!held[c] && !cells[c].across.empty()) job.push_back(c);
   const int64_t nj = (int64_t)job.size();
-  std::vector<int32_t> nf(nj), nh(nj), nb(nj), piece, bij;
+  std::vector<int32_t> nf(nj), nh(nj), nb(nj), piece;
   std::vector<int64_t> boff(nj), foff(nj), hoff(nj);
-  std::vector<double> fx, hx, bgeo, bmat, cmat(4 * nj), cfull(4 * nj), ccentre(3 * nj);
+  std::vector<double> fx, hx, cmat(4 * nj), cfull(4 * nj), ccentre(3 * nj);
   for (int64_t q = 0; q < nj; q++) {
     const int64_t c = job[q];
     const Cell& cell = cells[c];
-    if (cell.free.size() > (size_t)kMaxFree || cell.across.size() > (size_t)kMaxHandles ||
-        cell.bonds.size() > (size_t)kMaxBonds)
+    const int64_t cb = bcount[c + 1] - bcount[c];
+    if (cell.free.size() > (size_t)kMaxFree || cell.across.size() > (size_t)kMaxHandles || cb > kMaxBonds)
       throw StError{ST_BAD_INPUT, "gather: a coarse cell holds more than 12 free nodes, 6 faces or 480 bonds"};
     nf[q] = (int32_t)cell.free.size();
     nh[q] = (int32_t)cell.across.size();
-    nb[q] = (int32_t)cell.bonds.size();
+    nb[q] = (int32_t)cb;
     foff[q] = (int64_t)fx.size() / 3;
     hoff[q] = (int64_t)hx.size() / 3;
-    boff[q] = (int64_t)bij.size() / 2;
+    boff[q] = bcount[c];
     for (int64_t v : cell.free)
-      for (int k = 0; k < 3; k++) fx.push_back(fine.x[v].c[k]);
+      for (int k = 0; k < 3; k++) fx.push_back(fine.x[3 * v + k]);
     for (int64_t k : ends[c])
       for (int t = 0; t < 3; t++) hx.push_back(out.x[k].c[t]);
     // which of its handles its matter joins (gather.rs:42-50): a piece of each
@@ -613,18 +702,11 @@ void measure(Ctx* ctx, const HLat& fine, bool against_full, HLat& out) {
       while (parent[a] != a) a = parent[a] = parent[parent[a]];
       return a;
     };
-    for (auto& t : cell.bonds) parent[root(t[0])] = root(t[1]);
-    for (size_t h = 0; h < cell.across.size(); h++) piece.push_back((int32_t)root((int64_t)(cell.free.size() + h)));
-    for (auto& t : cell.bonds) {
-      bij.push_back((int32_t)t[0]);
-      bij.push_back((int32_t)t[1]);
-      const int64_t b = t[2];
-      for (int k = 0; k < 3; k++) bgeo.push_back(fine.p[b].c[k]);
-      for (int k = 0; k < 3; k++) bgeo.push_back(fine.n[b].c[k]);
-      bgeo.push_back(fine.a[b]);
-      bgeo.push_back(fine.d[b]);
-      for (int k = 1; k < 6; k++) bmat.push_back(fine.bmat[b][k]);
+    for (int64_t t = bcount[c]; t < bcount[c + 1]; t++) {
+      const int64_t ra = root(bij[2 * t]);
+      parent[ra] = root(bij[2 * t + 1]);
     }
+    for (size_t h = 0; h < cell.across.size(); h++) piece.push_back((int32_t)root((int64_t)(cell.free.size() + h)));
     cmat[4 * q] = mat[c][1];
     for (int k = 0; k < 3; k++) cmat[4 * q + 1 + k] = mat[c][3 + k];
     const std::array<double, 4> f = against_full ? full_of(ctx, mat[c]) : std::array<double, 4>{1.0, 1.0, 1.0, 1.0};
@@ -632,21 +714,29 @@ void measure(Ctx* ctx, const HLat& fine, bool against_full, HLat& out) {
     const G3 ce = hcentre(sites[c], level);
     for (int k = 0; k < 3; k++) ccentre[3 * q + k] = ce.c[k];
   }
+  const auto t2 = clock::now();
   std::vector<int32_t> nfound(nj), fab((size_t)nj * 2 * kMaxPairs);
   std::vector<double> fval((size_t)nj * 2 * kMaxPairs);
   if (nj) {
     ST_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
-    DBuf dnf, dnh, dnb, dboff, dfoff, dhoff, dfx, dhx, dpiece, dbij, dbgeo, dbmat, dcmat, dcfull, dcc, dfound, dfab,
-        dfval, dws;
+    DBuf dnf, dnh, dnb, dboff, dfoff, dhoff, dfx, dhx, dpiece, dbij, dbidx, dbp, dbn, dba, dbd, dbm, dcmat, dcfull,
+        dcc, dfound, dfab, dfval, dws;
     put(dnf, nf, st), put(dnh, nh, st), put(dnb, nb, st), put(dboff, boff, st), put(dfoff, foff, st);
     put(dhoff, hoff, st), put(dfx, fx, st), put(dhx, hx, st), put(dpiece, piece, st), put(dbij, bij, st);
-    put(dbgeo, bgeo, st), put(dbmat, bmat, st), put(dcmat, cmat, st), put(dcfull, cfull, st), put(dcc, ccentre, st);
+    put(dbidx, bidx, st), put(dcmat, cmat, st), put(dcfull, cfull, st), put(dcc, ccentre, st);
+    auto raw = [&](DBuf& d, const double* h, size_t count) {
+      d.ensure(std::max<size_t>(count, 1) * sizeof(double));
+      if (count) ST_HIP(hipMemcpyAsync(d.p, h, count * sizeof(double), hipMemcpyHostToDevice, st));
+    };
+    raw(dbp, fine.p, 3 * (size_t)nbond), raw(dbn, fine.n, 3 * (size_t)nbond), raw(dba, fine.a, (size_t)nbond);
+    raw(dbd, fine.d, (size_t)nbond), raw(dbm, fine.bmat, 6 * (size_t)nbond);
     dfound.ensure(nj * sizeof(int32_t));
     dfab.ensure(fab.size() * sizeof(int32_t));
     dfval.ensure(fval.size() * sizeof(double));
-    // lanes per launch: the workspace is 316 KB a lane
-    const int64_t chunk = std::min<int64_t>(nj, 16384);
+    // lanes per launch (the workspace is 275 KB a lane): each lane's work is a long chain of
+    // dependent loads, so as many lanes as memory allows are in flight at once
+    const int64_t chunk = std::min<int64_t>(nj, 65536);
     dws.ensure((size_t)chunk * kWsDoubles * sizeof(double));
     for (int64_t c0 = 0; c0 < nj; c0 += chunk) {
       const int64_t cn = std::min(chunk, nj - c0);
@@ -655,7 +745,9 @@ void measure(Ctx* ctx, const HLat& fine, bool against_full, HLat& out) {
       j.nf = dnf.as<int32_t>() + c0, j.nh = dnh.as<int32_t>() + c0, j.nb = dnb.as<int32_t>() + c0;
       j.boff = dboff.as<int64_t>() + c0, j.foff = dfoff.as<int64_t>() + c0, j.hoff = dhoff.as<int64_t>() + c0;
       j.fx = dfx.as<double>(), j.hx = dhx.as<double>(), j.piece = dpiece.as<int32_t>();
-      j.bij = dbij.as<int32_t>(), j.bgeo = dbgeo.as<double>(), j.bmat = dbmat.as<double>();
+      j.bij = dbij.as<int32_t>(), j.bidx = dbidx.as<int64_t>();
+      j.bp = dbp.as<double>(), j.bn = dbn.as<double>(), j.ba = dba.as<double>(), j.bd = dbd.as<double>();
+      j.bm = dbm.as<double>();
       j.cmat = dcmat.as<double>() + 4 * c0, j.cfull = dcfull.as<double>() + 4 * c0, j.ccentre = dcc.as<double>() + 3 * c0;
       j.s = s;
       j.against_full = against_full ? 1 : 0;
@@ -670,9 +762,10 @@ void measure(Ctx* ctx, const HLat& fine, bool against_full, HLat& out) {
     ST_HIP(hipMemcpyAsync(fval.data(), dfval.p, fval.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     ST_HIP(hipStreamSynchronize(st));
   }
+  const auto t3 = clock::now();
 
   // each cell's weight on its faces, as much as each is joined; its beams (gather.rs:369-392)
-  std::vector<int64_t> slot(ncell, -1);
+  std::vector<int64_t> slot((size_t)ncell, -1);
   for (int64_t q = 0; q < nj; q++) slot[job[q]] = q;
   for (int64_t c = 0; c < ncell; c++) {
     const Cell& cell = cells[c];
@@ -701,6 +794,12 @@ void measure(Ctx* ctx, const HLat& fine, bool against_full, HLat& out) {
       hbeam(out, ea, eb, xa, xb, val[2 * k] * length / mat[c][1], st);
     }
   }
+  if (std::getenv("STATICS_HIP_VERBOSE")) {
+    auto ms = [](clock::time_point a, clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    std::fprintf(stderr, "[statics-hip] gather: %lld cells (%lld measured): bookkeeping %.1f ms, packing %.1f ms, "
+                 "device %.1f ms, assembly %.1f ms\n", (long long)ncell, (long long)nj, ms(t0, t1), ms(t1, t2),
+                 ms(t2, t3), ms(t3, clock::now()));
+  }
 }
 
 }  // namespace
@@ -716,29 +815,11 @@ int st_gather(StCtx* c, const StLattice* fine, int against_full, int64_t* nnodes
       throw StError{ST_BAD_INPUT, "gather: a node array is NULL"};
     if (fine->nbonds && (!fine->ij || !fine->p || !fine->n || !fine->a || !fine->d || !fine->bond_mat))
       throw StError{ST_BAD_INPUT, "gather: a bond array is NULL"};
-    HLat L;
-    L.level = fine->level;
-    const int64_t nn = fine->nnodes, nb = fine->nbonds;
-    L.x.resize(nn), L.m.resize(nn), L.nmat.resize(nn), L.held.resize(nn), L.around.resize(nn);
-    for (int64_t v = 0; v < nn; v++) {
-      for (int k = 0; k < 3; k++) L.x[v].c[k] = fine->x[3 * v + k];
-      L.m[v] = fine->m[v];
-      for (int k = 0; k < 6; k++) L.nmat[v][k] = fine->node_mat[6 * v + k];
-      L.held[v] = fine->held[v] != 0;
-      for (int a = 0; a < 2; a++)
-        for (int k = 0; k < 3; k++) L.around[v][a][k] = fine->around[6 * v + 3 * a + k];
-    }
-    L.ij.resize(nb), L.p.resize(nb), L.n.resize(nb), L.a.resize(nb), L.d.resize(nb), L.bmat.resize(nb);
-    for (int64_t b = 0; b < nb; b++) {
-      for (int k = 0; k < 2; k++) {
-        L.ij[b][k] = fine->ij[2 * b + k];
-        if (L.ij[b][k] < 0 || L.ij[b][k] >= nn) throw StError{ST_BAD_INPUT, "gather: a bond's node is out of range"};
-      }
-      for (int k = 0; k < 3; k++) L.p[b].c[k] = fine->p[3 * b + k], L.n[b].c[k] = fine->n[3 * b + k];
-      L.a[b] = fine->a[b];
-      L.d[b] = fine->d[b];
-      for (int k = 0; k < 6; k++) L.bmat[b][k] = fine->bond_mat[6 * b + k];
-    }
+    for (int64_t b = 0; b < 2 * fine->nbonds; b++)
+      if (fine->ij[b] < 0 || fine->ij[b] >= fine->nnodes)
+        throw StError{ST_BAD_INPUT, "gather: a bond's node is out of range"};
+    const LatIn L{fine->level, fine->nnodes, fine->nbonds, fine->x, fine->m, fine->node_mat, fine->held,
+                  fine->around, fine->ij, fine->p, fine->n, fine->a, fine->d, fine->bond_mat};
     HLat out;
     measure(c, L, against_full != 0, out);
     GatherOut& g = c->gathered;
