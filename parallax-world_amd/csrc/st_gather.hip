@@ -3,14 +3,15 @@
 //
 // The host does the bookkeeping the reference does in `measured` (which coarse cell each
 // finer node and bond lies in, the faces, the handles, the materials, the weights); the
-// device measures the cells, one lane per cell: the elimination of the free parts
+// device measures the cells, one wave per cell: the elimination of the free parts
 // (responses, :59-114), the stiffness between handles, the non-negative beam fit (beams,
 // :119-161) and each beam's share of the finer strength. Every choice in that measure (a
 // pivot, a beam dropped for being negative, a beam kept for being positive) follows from
 // FP64 values compared exactly, and full cells are symmetric, so ties are the rule, not
-// the exception: the lane computes each value with the reference's operations in the
-// reference's order, without contraction (the pragma below), so the choices, and the
-// lattice, are the reference's.
+// the exception: each value is computed with the reference's operations in the reference's
+// order, without contraction (the pragma below), so the choices, and the lattice, are the
+// reference's. A wave measures a cell; its lanes split only work whose values do not depend
+// on how it is split.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -38,14 +39,6 @@ constexpr double kSection[2] = {0.336, 0.336};         // lattice.rs:25
 constexpr double kUnit = 0.125;                        // lattice.rs:16
 constexpr int kPerBeam = 4;                            // lattice.rs:19
 
-// per-lane workspace, in doubles: the elimination, the cases' bond forces, the stiffness
-// between handles, the beams' normal equations, a pulled beam
-constexpr int64_t kOffA = 0;
-constexpr int64_t kOffCases = kOffA + (int64_t)6 * kMaxFree * kRow;
-constexpr int64_t kOffK = kOffCases + (int64_t)kN2 * kMaxBonds * 3;
-constexpr int64_t kOffG = kOffK + kN2 * kN2;
-constexpr int64_t kOffPull = kOffG + kMaxPairs * (kMaxPairs + 1);
-constexpr int64_t kWsDoubles = kOffPull + (int64_t)kMaxBonds * 3;
 
 struct G3 {
   double c[3];
@@ -94,64 +87,85 @@ struct GJob {
   double* fval;           // per cell, kMaxPairs x 2: stiffness along, how much stronger
 };
 
-struct Ws {
-  double* base;
-  int64_t stride;
-  __device__ double& operator[](int64_t e) const { return base[e * stride]; }
-};
+// the solution's row r, right-hand side k
+__device__ inline double sol_x(const double* A, int pitch, int n, int r, int k, double tiny) {
+  const double d = A[r * pitch + r];
+  return fabs(d) <= tiny ? 0.0 : A[r * pitch + n + k] / d;
+}
 
-// gather.rs:165-186 on rows of pitch `pitch` at `off`: n rows, n + rhs columns, in place.
-// Returns the size under which a pivot is taken for nothing (its scale: the greatest
-// diagonal entry before the elimination).
-__device__ double lane_solve(const Ws& w, int64_t off, int pitch, int n, int rhs) {
-  auto A = [&](int r, int c) -> double& { return w[off + (int64_t)r * pitch + c]; };
+// gather.rs:165-186 by one wave (every lane calls it): each step's pivot found over the
+// lanes (the last of the greatest, as max_by), the other rows updated one lane each, eight
+// entries at a time, each entry as the reference updates it. Returns the scale's tiny.
+__device__ double wave_solve(double* M, int pitch, int n, int rhs, int lane) {
   double scale = 0.0;
-  for (int r = 0; r < n; r++) scale = gmax(scale, fabs(A(r, r)));
+  for (int r = 0; r < n; r++) scale = gmax(scale, fabs(M[r * pitch + r]));
   const double tiny = 1e-12 * scale;
-  for (int c = 0; c < n; c++) {
-    int p = c;
-    for (int x = c + 1; x < n; x++)
-      if (!(fabs(A(x, c)) < fabs(A(p, c)))) p = x;  // the last of the greatest, as max_by
-    const int wd = n + rhs;
-    if (p != c)
-      for (int k0 = 0; k0 < wd; k0 += 8) {
-        double u[8], v[8];
-        const int kn = min(8, wd - k0);
-        for (int t = 0; t < kn; t++) u[t] = A(c, k0 + t), v[t] = A(p, k0 + t);
-        for (int t = 0; t < kn; t++) A(c, k0 + t) = v[t], A(p, k0 + t) = u[t];
-      }
-    const double pv = A(c, c);
-    if (fabs(pv) <= tiny) continue;
-    for (int r = 0; r < n; r++) {
-      if (r == c) continue;
-      const double f = A(r, c) / pv;
-      if (f == 0.0) continue;
-      // eight entries at a time: the pivot row's and this row's loaded together
-      for (int k0 = c; k0 < wd; k0 += 8) {
-        double u[8], v[8];
-        const int kn = min(8, wd - k0);
-        for (int t = 0; t < kn; t++) u[t] = A(c, k0 + t), v[t] = A(r, k0 + t);
-        for (int t = 0; t < kn; t++) A(r, k0 + t) = v[t] - f * u[t];
-      }
+  const int W = n + rhs;
+  for (int cc = 0; cc < n; cc++) {
+    double bv = 0.0;
+    int bi = -1;
+    for (int x = cc + lane; x < n; x += 64) {
+      const double v = fabs(M[x * pitch + cc]);
+      if (bi < 0 || !(v < bv)) bv = v, bi = x;
     }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(bv, o);
+      const int oi = __shfl_xor(bi, o);
+      if (oi >= 0 && (bi < 0 || bv < ov || (!(ov < bv) && oi > bi))) bv = ov, bi = oi;
+    }
+    const int p = __shfl(bi, 0);  // every lane holds the same; taken from one so the wave agrees
+    if (p != cc) {
+      for (int k = lane; k < W; k += 64) {
+        const double t = M[cc * pitch + k];
+        M[cc * pitch + k] = M[p * pitch + k];
+        M[p * pitch + k] = t;
+      }
+      __syncthreads();
+    }
+    const double pv = M[cc * pitch + cc];
+    if (fabs(pv) <= tiny) continue;
+    for (int r = lane; r < n; r += 64) {
+      if (r == cc) continue;
+      const double f = M[r * pitch + cc] / pv;
+      if (f == 0.0) continue;
+      int k = cc;
+      for (; k + 8 <= W; k += 8) {
+        double u[8], v[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) u[t] = M[cc * pitch + k + t], v[t] = M[r * pitch + k + t];
+#pragma unroll
+        for (int t = 0; t < 8; t++) M[r * pitch + k + t] = v[t] - f * u[t];
+      }
+      for (; k < W; k++) M[r * pitch + k] -= f * M[cc * pitch + k];
+    }
+    __syncthreads();
   }
   return tiny;
 }
-// the solution's row r, right-hand side k
-__device__ inline double lane_x(const Ws& w, int64_t off, int pitch, int n, int r, int k, double tiny) {
-  const double d = w[off + (int64_t)r * pitch + r];
-  return fabs(d) <= tiny ? 0.0 : w[off + (int64_t)r * pitch + n + k] / d;
-}
 
-__global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_t stride) {
-  const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (c >= j.n) return;
-  const Ws w{ws + c, stride};
+constexpr int kRowL = kRow + 1;  // LDS row pitch of the elimination (odd: a column spreads over the banks)
+constexpr int64_t kCaseDoubles = (int64_t)kN2 * kMaxBonds * 3;  // global workspace per cell: the cases
+
+// One wave measures one coarse cell. Lanes share the work only where each value's operations
+// and their order stay those of the reference: the entries one bond adds to (distinct), the
+// rows of an elimination step, the handle motions, the bonds of a pulled beam; the sums over
+// bonds run in bond order, each on one lane.
+__global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws) {
+  const int c = blockIdx.x;
+  const int lane = threadIdx.x;
+  __shared__ double A[6 * kMaxFree * kRowL];
+  __shared__ double X[6 * kMaxFree * kN2];
+  __shared__ double Ks[kN2 * kN2];
+  __shared__ double Gs[kMaxPairs * (kMaxPairs + 1)];
+  __shared__ double fx_s[kMaxPairs];
+  __shared__ int fa_s[kMaxPairs], fb_s[kMaxPairs];
+  __shared__ int nfound_s;
+  double* cases = ws + (int64_t)c * kCaseDoubles;
   const int nf = j.nf[c], nh = j.nh[c], nb = j.nb[c];
-  const int n = 6 * nf, m = 3 * nh;
+  const int n = 6 * nf, m = 3 * nh, W = n + m;
+  const int32_t* piece = j.piece + j.hoff[c];
   const double* fx = j.fx + 3 * j.foff[c];
   const double* hx = j.hx + 3 * j.hoff[c];
-  const int32_t* piece = j.piece + j.hoff[c];
   const int32_t* bij = j.bij + 2 * j.boff[c];
   const int64_t* bidx = j.bidx + j.boff[c];
   auto BP = [&](int b) -> G3 {
@@ -175,51 +189,48 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
     e.c[d % 3] = 1.0;
     return d < 3 ? e : gcross(e, gsub(p, pos(v)));
   };
-  auto A = [&](int r, int k) -> double& { return w[kOffA + (int64_t)r * kRow + k]; };
-  auto P3 = [&](const double* q) -> G3 { return {{q[0], q[1], q[2]}}; };
+  auto CASE = [&](int cs, int b, int q) -> double& { return cases[((int64_t)cs * kMaxBonds + b) * 3 + q]; };
 
-  // responses (gather.rs:59-103): the free parts' equations, the handles' columns on the right
-  for (int r = 0; r < n; r++)
-    for (int k = 0; k < n + m; k++) A(r, k) = 0.0;
+  // responses (gather.rs:59-103): the free parts' equations, the handles' columns on the right;
+  // a bond adds to 144 distinct entries, one lane each, bond after bond
+  for (int t = lane; t < n * kRowL; t += 64) A[t] = 0.0;
+  __syncthreads();
   for (int b = 0; b < nb; b++) {
     const G3 p = BP(b), nv = BN(b);
     const double kk = BA(b) / BD(b);
     const double e = BM(b, 1), g = BM(b, 2);
-    double st[3][3];  // Bond::stiffness (model.rs:139-149)
-    for (int r = 0; r < 3; r++)
-      for (int q = 0; q < 3; q++) {
-        const double nn = nv.c[r] * nv.c[q];
-        st[r][q] = e * kk * nn + g * kk * ((r == q ? 1.0 : 0.0) - nn);
-      }
     const int vs[2] = {bij[2 * b], bij[2 * b + 1]};
     const double sg[2] = {-1.0, 1.0};
-    for (int x = 0; x < 2; x++)
-      for (int d = 0; d < 6; d++) {
-        const int r = dof(vs[x], d);
-        if (r < 0 || r >= n) continue;
-        const G3 cv = column(vs[x], d, p);
-        const G3 scv{{gdot(G3{{st[0][0], st[0][1], st[0][2]}}, cv), gdot(G3{{st[1][0], st[1][1], st[1][2]}}, cv),
-                      gdot(G3{{st[2][0], st[2][1], st[2][2]}}, cv)}};
-        // the row's twelve entries this bond adds to are distinct: load them together
-        double add[12], cur[12];
-        int cols[12];
-        for (int y = 0; y < 2; y++)
-          for (int d2 = 0; d2 < 6; d2++) {
-            const int col = dof(vs[y], d2);
-            cols[6 * y + d2] = col;
-            const double sign = col < n ? 1.0 : -1.0;
-            add[6 * y + d2] = col < 0 ? 0.0 : sign * sg[x] * sg[y] * gdot(scv, column(vs[y], d2, p));
-          }
-        for (int t = 0; t < 12; t++) cur[t] = cols[t] < 0 ? 0.0 : A(r, cols[t]);
-        for (int t = 0; t < 12; t++)
-          if (cols[t] >= 0) A(r, cols[t]) = cur[t] + add[t];
-      }
+    for (int t = lane; t < 144; t += 64) {
+      const int x = t / 72, d = (t % 72) / 12, y = (t % 12) / 6, d2 = t % 6;
+      const int r = dof(vs[x], d), col = dof(vs[y], d2);
+      if (r < 0 || r >= n || col < 0) continue;
+      double st[3][3];  // Bond::stiffness (model.rs:139-149)
+      for (int a = 0; a < 3; a++)
+        for (int q = 0; q < 3; q++) {
+          const double nn = nv.c[a] * nv.c[q];
+          st[a][q] = e * kk * nn + g * kk * ((a == q ? 1.0 : 0.0) - nn);
+        }
+      const G3 cv = column(vs[x], d, p);
+      const G3 scv{{gdot(G3{{st[0][0], st[0][1], st[0][2]}}, cv), gdot(G3{{st[1][0], st[1][1], st[1][2]}}, cv),
+                    gdot(G3{{st[2][0], st[2][1], st[2][2]}}, cv)}};
+      const double sign = col < n ? 1.0 : -1.0;
+      A[r * kRowL + col] += sign * sg[x] * sg[y] * gdot(scv, column(vs[y], d2, p));
+    }
+    __syncthreads();
   }
-  const double tiny_a = lane_solve(w, kOffA, kRow, n, m);
 
-  // each handle moved alone along each axis: the bonds' forces (moved, bodies.rs:420-423)
-  auto CASE = [&](int cs, int b, int q) -> double& { return w[kOffCases + ((int64_t)cs * kMaxBonds + b) * 3 + q]; };
-  for (int cs = 0; cs < m; cs++)
+  // gather.rs:165-186, then the solutions x (the free parts' motions for each handle motion)
+  const double tiny_a = wave_solve(A, kRowL, n, m, lane);
+  for (int t = lane; t < n * m; t += 64) X[t] = sol_x(A, kRowL, n, t / m, t % m, tiny_a);
+  __syncthreads();
+
+  // each handle moved alone along each axis (one lane each): the bonds' forces (moved,
+  // bodies.rs:420-423), and what holds each handle (gather.rs:291-303), bond after bond
+  for (int t = lane; t < kN2 * kN2; t += 64) Ks[t] = 0.0;
+  __syncthreads();
+  if (lane < m) {
+    const int cs = lane;
     for (int b = 0; b < nb; b++) {
       const G3 p = BP(b), nv = BN(b);
       G3 tr[2];
@@ -227,7 +238,7 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
         const int v = bij[2 * b + x];
         double mo[6];
         if (v < nf) {
-          for (int d = 0; d < 6; d++) mo[d] = lane_x(w, kOffA, kRow, n, 6 * v + d, cs, tiny_a);
+          for (int d = 0; d < 6; d++) mo[d] = X[(6 * v + d) * m + cs];
         } else {
           for (int d = 0; d < 6; d++) mo[d] = 0.0;
           if (v - nf == cs / 3) mo[cs % 3] = 1.0;
@@ -241,51 +252,45 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
       const double kk = BA(b) / BD(b);
       const G3 f = gadd(gmul(nv, BM(b, 1) * kk * along), gmul(gsub(delta, gmul(nv, along)), BM(b, 2) * kk));
       for (int q = 0; q < 3; q++) CASE(cs, b, q) = f.c[q];
+      const int vj = bij[2 * b + 1], vi = bij[2 * b];
+      if (vj >= nf)
+        for (int q = 0; q < 3; q++) Ks[(3 * (vj - nf) + q) * kN2 + cs] += 1.0 * f.c[q];
+      if (vi >= nf)
+        for (int q = 0; q < 3; q++) Ks[(3 * (vi - nf) + q) * kN2 + cs] += -1.0 * f.c[q];
     }
+  }
+  __syncthreads();
 
-  // the stiffness between handles: what holds each, each case (gather.rs:291-303)
-  auto K = [&](int r, int q) -> double& { return w[kOffK + r * kN2 + q]; };
-  for (int r = 0; r < m; r++)
-    for (int q = 0; q < m; q++) K(r, q) = 0.0;
-  for (int cs = 0; cs < m; cs++)
-    for (int b = 0; b < nb; b++) {
-      const int sides[2] = {bij[2 * b + 1], bij[2 * b]};
-      const double sign[2] = {1.0, -1.0};
-      double f[3], cur[6];
-      for (int q = 0; q < 3; q++) f[q] = CASE(cs, b, q);
-      for (int x = 0; x < 2; x++)  // two handles, six distinct entries
-        for (int q = 0; q < 3; q++) cur[3 * x + q] = sides[x] >= nf ? K(3 * (sides[x] - nf) + q, cs) : 0.0;
-      for (int x = 0; x < 2; x++)
-        if (sides[x] >= nf)
-          for (int q = 0; q < 3; q++) K(3 * (sides[x] - nf) + q, cs) = cur[3 * x + q] + sign[x] * f[q];
-    }
-
-  // beams (gather.rs:119-161)
-  int pa[kMaxPairs], pb[kMaxPairs], np = 0;
+  // beams (gather.rs:119-161): the normal equations' entries one lane each, their
+  // elimination over the wave; every lane keeps the same active set
+  __shared__ int pa[kMaxPairs], pb[kMaxPairs];
+  __shared__ double ue[3 * kMaxPairs];
+  int np = 0;
   for (int a = 0; a < nh; a++)
     for (int b = a + 1; b < nh; b++)
       if (piece[a] == piece[b]) {
-        pa[np] = a;
-        pb[np] = b;
+        if (lane == 0) pa[np] = a, pb[np] = b;
         np++;
       }
+  __syncthreads();
+  for (int p = lane; p < np; p += 64) {
+    const G3 u = gunit(gsub(pos(nf + pb[p]), pos(nf + pa[p])));
+    for (int k = 0; k < 3; k++) ue[3 * p + k] = u.c[k];
+  }
+  __syncthreads();
   // A unit (a beam of unit stiffness along its direction, as a matrix) is nonzero only in the
   // four 3x3 blocks of its two handles: +e e^T on the diagonal blocks, -e e^T off them. Its
   // products with another matrix are summed over those 36 entries in increasing index; the
-  // reference sums over all (3 nh)^2, but every other term is a signed zero, which leaves a sum
-  // begun at +0 unchanged, so the sums are the same to the bit.
-  G3 ue[kMaxPairs];
-  for (int p = 0; p < np; p++) ue[p] = gunit(gsub(pos(nf + pb[p]), pos(nf + pa[p])));
-  // unit p's entry (row, col), or 0
+  // reference sums over all (3 nh)^2, but every other term is a signed zero, which leaves a
+  // sum begun at +0 unchanged, so the sums are the same to the bit.
   auto U = [&](int p, int row, int col) -> double {
     const int hr = row / 3, hc = col / 3;
     const bool ra = hr == pa[p], rb = hr == pb[p], ca = hc == pa[p], cb = hc == pb[p];
     if (!((ra || rb) && (ca || cb))) return 0.0;
-    const double v = ue[p].c[row % 3] * ue[p].c[col % 3];
+    const double v = ue[3 * p + row % 3] * ue[3 * p + col % 3];
     return (ra == ca) ? 0.0 + v : 0.0 - v;
   };
-  auto T = [&](int row, int col) { return 0.5 * (K(row, col) + K(col, row)); };
-  // unit p with unit q (q >= 0) or with the target (q < 0)
+  auto T = [&](int row, int col) { return 0.5 * (Ks[row * kN2 + col] + Ks[col * kN2 + row]); };
   auto udot = [&](int p, int q) {
     const int hs[2] = {pa[p], pb[p]};
     double sum = 0.0;
@@ -301,7 +306,6 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
     return sum;
   };
   const int gp = kMaxPairs + 1;
-  auto G = [&](int r, int q) -> double& { return w[kOffG + r * gp + q]; };
   uint32_t on = np ? ((1u << np) - 1u) : 0u;
   int idx[kMaxPairs], mm = 0;
   double xs[kMaxPairs];
@@ -309,73 +313,89 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws, int64_
     mm = 0;
     for (int p = 0; p < np; p++)
       if (on >> p & 1u) idx[mm++] = p;
-    for (int r = 0; r < mm; r++) {
-      for (int q = 0; q < mm; q++) G(r, q) = udot(idx[r], idx[q]);
-      G(r, mm) = udot(idx[r], -1);
+    for (int e = lane; e < mm * (mm + 1); e += 64) {
+      const int r = e / (mm + 1), q = e % (mm + 1);
+      Gs[r * gp + q] = q < mm ? udot(idx[r], idx[q]) : udot(idx[r], -1);
     }
-    const double tiny_g = lane_solve(w, kOffG, gp, mm, 1);
+    __syncthreads();
+    const double tiny_g = wave_solve(Gs, gp, mm, 1, lane);
     int worst = -1;
     for (int r = 0; r < mm; r++) {
-      xs[r] = lane_x(w, kOffG, gp, mm, r, 0, tiny_g);
+      xs[r] = sol_x(Gs, gp, mm, r, 0, tiny_g);
       if (xs[r] < 0.0 && (worst < 0 || xs[r] < xs[worst])) worst = r;  // the first most negative
     }
+    __syncthreads();  // every lane has read the solutions before the next round writes
     if (worst < 0) break;
     on &= ~(1u << idx[worst]);
   }
+  if (lane == 0) {
+    int found = 0;
+    for (int r = 0; r < mm; r++)
+      if (xs[r] > 0.0) {
+        fa_s[found] = pa[idx[r]];
+        fb_s[found] = pb[idx[r]];
+        fx_s[found] = xs[r];
+        found++;
+      }
+    nfound_s = found;
+  }
+  __syncthreads();
 
-  // each beam kept: pulled alone, a unit apart, how much of the finer strength it uses
+  // each beam kept: pulled alone, a unit apart (lanes on bonds), how much of the finer
+  // strength it uses (summed in bond order on one lane)
   const double e_c = j.cmat[4 * c], ft_c = j.cmat[4 * c + 1], fc_c = j.cmat[4 * c + 2], fs_c = j.cmat[4 * c + 3];
-  const G3 centre = P3(j.ccentre + 3 * c);
+  const G3 centre{{j.ccentre[3 * c], j.ccentre[3 * c + 1], j.ccentre[3 * c + 2]}};
   const double s = j.s;
-  auto PULL = [&](int b, int q) -> double& { return w[kOffPull + (int64_t)b * 3 + q]; };
-  int found = 0;
-  for (int r = 0; r < mm; r++) {
-    if (!(xs[r] > 0.0)) continue;
-    const int a = pa[idx[r]], b = pb[idx[r]];
-    const double stiff = xs[r];
+  double* wu = A;  // the elimination is done with: the bonds' work and its used share
+  const int nfd = nfound_s;
+  for (int k = 0; k < nfd; k++) {
+    const int a = fa_s[k], b = fb_s[k];
     const G3 at_a = pos(nf + a), at_b = pos(nf + b);
     const G3 e = gunit(gsub(at_b, at_a));
-    for (int q = 0; q < nb; q++) {
+    for (int q = lane; q < nb; q += 64) {
       G3 f{{0.0, 0.0, 0.0}};
       for (int t = 0; t < 3; t++) {
         const G3 cb{{CASE(3 * b + t, q, 0), CASE(3 * b + t, q, 1), CASE(3 * b + t, q, 2)}};
         const G3 ca{{CASE(3 * a + t, q, 0), CASE(3 * a + t, q, 1), CASE(3 * a + t, q, 2)}};
         f = gadd(f, gadd(gmul(cb, e.c[t] / 2.0), gmul(ca, -e.c[t] / 2.0)));
       }
-      for (int k = 0; k < 3; k++) PULL(q, k) = f.c[k];
-    }
-    double held = 0.0, work = 0.0;
-    for (int q = 0; q < nb; q++) {
-      const G3 f{{PULL(q, 0), PULL(q, 1), PULL(q, 2)}};
       const G3 nv = BN(q);
       const double ba = BA(q), bd = BD(q);
       // Bond::used (model.rs:123-129)
-      const G3 t = gmul(f, 1.0 / ba);
-      const double sigma = gdot(t, nv);
-      const double tau2 = gmax(gdot(t, t) - sigma * sigma, 0.0);
+      const G3 tq = gmul(f, 1.0 / ba);
+      const double sigma = gdot(tq, nv);
+      const double tau2 = gmax(gdot(tq, tq) - sigma * sigma, 0.0);
       const double rr = sqrt(sigma * sigma / 4.0 + tau2);
       const double u = mat_used(BM(q, 3), BM(q, 4), BM(q, 5), sigma / 2.0 + rr, sigma / 2.0 - rr);
       // Bond::energy (model.rs:132-136)
       const double kk = ba / bd;
       const double along = gdot(f, nv);
       const double wq = 0.5 * (along * along / (BM(q, 1) * kk) + (gdot(f, f) - along * along) / (BM(q, 2) * kk));
-      held += wq * u;
-      work += wq;
+      wu[2 * q] = wq * u;
+      wu[2 * q + 1] = wq;
     }
-    const double fine_used = work > 0.0 ? held / work : 0.0;
-    const double length = gnorm(gsub(at_b, at_a));
-    const double own = mat_used(ft_c, fc_c, fs_c, e_c / length, 0.0);
-    const double stronger = fine_used > 0.0 ? own / fine_used : 1.0;
-    const int kc = gclass(at_a, at_b, centre);
-    double section = stiff * length / e_c / (s * s);
-    if (j.against_full) section = section * kSection[kc] / j.cfull[4 * c + 2 * kc];
-    j.fab[(int64_t)c * 2 * kMaxPairs + 2 * found] = a;
-    j.fab[(int64_t)c * 2 * kMaxPairs + 2 * found + 1] = b;
-    j.fval[(int64_t)c * 2 * kMaxPairs + 2 * found] = section * s * s * e_c / length;
-    j.fval[(int64_t)c * 2 * kMaxPairs + 2 * found + 1] = stronger / j.cfull[4 * c + 2 * kc + 1];
-    found++;
+    __syncthreads();
+    if (lane == 0) {
+      double held = 0.0, work = 0.0;
+      for (int q = 0; q < nb; q++) {
+        held += wu[2 * q];
+        work += wu[2 * q + 1];
+      }
+      const double fine_used = work > 0.0 ? held / work : 0.0;
+      const double length = gnorm(gsub(at_b, at_a));
+      const double own = mat_used(ft_c, fc_c, fs_c, e_c / length, 0.0);
+      const double stronger = fine_used > 0.0 ? own / fine_used : 1.0;
+      const int kc = gclass(at_a, at_b, centre);
+      double section = fx_s[k] * length / e_c / (s * s);
+      if (j.against_full) section = section * kSection[kc] / j.cfull[4 * c + 2 * kc];
+      j.fab[(int64_t)c * 2 * kMaxPairs + 2 * k] = a;
+      j.fab[(int64_t)c * 2 * kMaxPairs + 2 * k + 1] = b;
+      j.fval[(int64_t)c * 2 * kMaxPairs + 2 * k] = section * s * s * e_c / length;
+      j.fval[(int64_t)c * 2 * kMaxPairs + 2 * k + 1] = stronger / j.cfull[4 * c + 2 * kc + 1];
+    }
+    __syncthreads();
   }
-  j.nfound[c] = found;
+  if (lane == 0) j.nfound[c] = nfd;
 }
 
 // ------------------------------------------------------------------ host side
@@ -734,10 +754,9 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
     dfound.ensure(nj * sizeof(int32_t));
     dfab.ensure(fab.size() * sizeof(int32_t));
     dfval.ensure(fval.size() * sizeof(double));
-    // lanes per launch (the workspace is 275 KB a lane): each lane's work is a long chain of
-    // dependent loads, so as many lanes as memory allows are in flight at once
-    const int64_t chunk = std::min<int64_t>(nj, 65536);
-    dws.ensure((size_t)chunk * kWsDoubles * sizeof(double));
+    // cells per launch (a wave each; the cases are 207 KB a cell in global memory)
+    const int64_t chunk = std::min<int64_t>(nj, 16384);
+    dws.ensure((size_t)chunk * kCaseDoubles * sizeof(double));
     for (int64_t c0 = 0; c0 < nj; c0 += chunk) {
       const int64_t cn = std::min(chunk, nj - c0);
       GJob j;
@@ -754,7 +773,7 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
       j.nfound = dfound.as<int32_t>() + c0;
       j.fab = dfab.as<int32_t>() + c0 * 2 * kMaxPairs;
       j.fval = dfval.as<double>() + c0 * 2 * kMaxPairs;
-      k_measure_cells<<<(unsigned)((cn + 63) / 64), 64, 0, st>>>(j, dws.as<double>(), cn);
+      k_measure_cells<<<(unsigned)cn, 64, 0, st>>>(j, dws.as<double>());
       ST_HIP(hipGetLastError());
     }
     ST_HIP(hipMemcpyAsync(nfound.data(), dfound.p, nj * sizeof(int32_t), hipMemcpyDeviceToHost, st));

@@ -563,3 +563,39 @@ def test_gather_parity(ctx, case):
         dev = ctx.gather(lat, against_full=full)
         gather_same(dev, want)
         lat = want
+
+
+def test_gather_generated_structure(ctx):
+    """A 32^3 structure of the bench's generator (pillars, slabs, holes; a held ground layer),
+    gathered twice: equal to the oracle's, and its weight kept (gather.rs:400-412)."""
+    occ, _ = sh.generate(32, 32, 32, seed=4, sigma=0.5, pillar_spacing=8.0, pillar_min=3, pillar_max=7)
+    sites, held, _, _ = sh.cells_from_occ(occ)
+    lat = orc.Lattice(0, sites.astype(np.int64), orc.MASONRY, held)
+    whole = lat.m.sum()
+    print(f"gather generated: {lat.n_nodes} nodes, {lat.n_bonds} bonds", flush=True)
+    for _ in range(2):
+        want = lat.gathered()
+        dev = ctx.gather(lat)
+        gather_same(dev, want)
+        assert abs(dev.m.sum() / whole - 1.0) < 1e-12
+        lat = want
+
+
+def test_gather_empty_and_refused(ctx):
+    """An empty lattice gathers to an empty one a level up; a bond naming a node that is not
+    there, or a cell past the measure's limits, is refused with a message."""
+    empty = sh.GatheredLattice(3, 0, 0)
+    g = ctx.gather(empty)
+    assert g.level == 4 and len(g.m) == 0 and len(g.a) == 0
+    lat = orc.Lattice(0, np.array([[0, 0, 0]]), [orc.STONE], np.zeros(1, np.uint8))
+    bad = sh.GatheredLattice(0, lat.n_nodes, lat.n_bonds)
+    for f in GATHER_FIELDS:
+        setattr(bad, f, getattr(lat, f).copy().reshape(getattr(bad, f).shape))
+    bad.ij[5, 1] = lat.n_nodes
+    with pytest.raises(RuntimeError, match="out of range"):
+        ctx.gather(bad)
+    # a bond whose nodes share no cell
+    bad.ij[5, 1] = lat.ij[5, 1]
+    bad.around[0] = [[100, 100, 100], [101, 100, 100]]
+    with pytest.raises(RuntimeError, match="share no cell"):
+        ctx.gather(bad)
