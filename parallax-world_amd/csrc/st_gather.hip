@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -469,6 +470,39 @@ LatIn view_of(const HLat& L) {
 
 void measure(Ctx* c, const LatIn& fine, bool against_full, HLat& out);
 
+// f(begin, end) over [0, n) in contiguous ranges on up to 16 host threads (pure arithmetic only)
+template <class F>
+void par_for(int64_t n, F f) {
+  const int64_t t = std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency()));
+  if (n < 65536 || t == 1) {
+    f((int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const int64_t chunk = (n + t - 1) / t;
+  for (int64_t a = 0; a < n; a += chunk) pool.emplace_back(f, a, std::min(n, a + chunk));
+  for (auto& th : pool) th.join();
+}
+
+// lattice.rs:76-84 into bonds [q, q + PER_BEAM) of L
+void hbeam_at(HLat& L, int64_t q, int64_t i, int64_t j, G3 a, G3 b, double area, const Mat6& mat) {
+  const G3 n = gunit(gsub(b, a));
+  const G3 e1 = gunit(gcross(n, fabs(n.c[0]) < 0.9 ? G3{{1.0, 0.0, 0.0}} : G3{{0.0, 1.0, 0.0}}));
+  const G3 e2 = gcross(n, e1);
+  const double r = sqrt(area / (2.0 * M_PI));
+  const G3 middle = gmul(gadd(a, b), 0.5);
+  const double d = gnorm(gsub(b, a));
+  const G3 off[kPerBeam] = {gmul(e1, r), gmul(e1, -r), gmul(e2, r), gmul(e2, -r)};
+  for (int k = 0; k < kPerBeam; k++) {
+    L.ij[q + k] = {i, j};
+    L.p[q + k] = gadd(middle, off[k]);
+    L.n[q + k] = n;
+    L.a[q + k] = area / 4.0;
+    L.d[q + k] = d;
+    L.bmat[q + k] = mat;
+  }
+}
+
 // gather.rs:200-223: a full cell's beams by class, measured once per material. The
 // reference measures the cell at the origin of a 6^3 block of level-0 cells; the cell's
 // own problem (its 8 finer cells, their nodes and bonds, in the same order, none held) is
@@ -783,9 +817,11 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
   }
   const auto t3 = clock::now();
 
-  // each cell's weight on its faces, as much as each is joined; its beams (gather.rs:369-392)
+  // each cell's weight on its faces, as much as each is joined (in cell order: a face takes
+  // from both its cells), and where its beams go (gather.rs:369-392)
   std::vector<int64_t> slot((size_t)ncell, -1);
   for (int64_t q = 0; q < nj; q++) slot[job[q]] = q;
+  std::vector<int64_t> first((size_t)ncell + 1, 0);
   for (int64_t c = 0; c < ncell; c++) {
     const Cell& cell = cells[c];
     const int64_t q = slot[c];
@@ -803,16 +839,32 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
     for (double jv : joined) total += jv;
     for (size_t h = 0; h < ends[c].size(); h++)
       out.m[ends[c][h]] += mass * (total > 0.0 ? joined[h] / total : 1.0 / (double)ends[c].size());
-    for (int k = 0; k < nfd; k++) {
-      const int64_t ea = ends[c][ab[2 * k]], eb = ends[c][ab[2 * k + 1]];
-      if (out.held[ea] && out.held[eb]) continue;
-      const G3 xa = out.x[ea], xb = out.x[eb];
-      const double length = gnorm(gsub(xb, xa));
-      Mat6 st = mat[c];
-      for (int t = 3; t < 6; t++) st[t] *= val[2 * k + 1];  // Mat::stronger (model.rs:71-73)
-      hbeam(out, ea, eb, xa, xb, val[2 * k] * length / mat[c][1], st);
-    }
+    int64_t kept = 0;
+    for (int k = 0; k < nfd; k++) kept += !(out.held[ends[c][ab[2 * k]]] && out.held[ends[c][ab[2 * k + 1]]]);
+    first[c + 1] = first[c] + kPerBeam * kept;
   }
+  const int64_t nbo = first[ncell];
+  out.ij.resize(nbo), out.p.resize(nbo), out.n.resize(nbo), out.a.resize(nbo), out.d.resize(nbo);
+  out.bmat.resize(nbo);
+  par_for(ncell, [&](int64_t c0, int64_t c1) {
+    for (int64_t c = c0; c < c1; c++) {
+      const int64_t q = slot[c];
+      if (q < 0) continue;
+      const int32_t* ab = fab.data() + q * 2 * kMaxPairs;
+      const double* val = fval.data() + q * 2 * kMaxPairs;
+      int64_t at = first[c];
+      for (int k = 0; k < nfound[q]; k++) {
+        const int64_t ea = ends[c][ab[2 * k]], eb = ends[c][ab[2 * k + 1]];
+        if (out.held[ea] && out.held[eb]) continue;
+        const G3 xa = out.x[ea], xb = out.x[eb];
+        const double length = gnorm(gsub(xb, xa));
+        Mat6 st = mat[c];
+        for (int t = 3; t < 6; t++) st[t] *= val[2 * k + 1];  // Mat::stronger (model.rs:71-73)
+        hbeam_at(out, at, ea, eb, xa, xb, val[2 * k] * length / mat[c][1], st);
+        at += kPerBeam;
+      }
+    }
+  });
   if (std::getenv("STATICS_HIP_VERBOSE")) {
     auto ms = [](clock::time_point a, clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
     std::fprintf(stderr, "[statics-hip] gather: %lld cells (%lld measured): bookkeeping %.1f ms, packing %.1f ms, "
@@ -853,11 +905,13 @@ int st_gather(StCtx* c, const StLattice* fine, int against_full, int64_t* nnodes
         for (int k = 0; k < 3; k++) g.around[6 * v + 3 * a + k] = out.around[v][a][k];
     }
     g.ij.resize(2 * ob), g.p.resize(3 * ob), g.n.resize(3 * ob), g.a = out.a, g.d = out.d, g.bond_mat.resize(6 * ob);
-    for (size_t b = 0; b < ob; b++) {
-      g.ij[2 * b] = out.ij[b][0], g.ij[2 * b + 1] = out.ij[b][1];
-      for (int k = 0; k < 3; k++) g.p[3 * b + k] = out.p[b].c[k], g.n[3 * b + k] = out.n[b].c[k];
-      for (int k = 0; k < 6; k++) g.bond_mat[6 * b + k] = out.bmat[b][k];
-    }
+    par_for((int64_t)ob, [&](int64_t b0, int64_t b1) {
+      for (int64_t b = b0; b < b1; b++) {
+        g.ij[2 * b] = out.ij[b][0], g.ij[2 * b + 1] = out.ij[b][1];
+        for (int k = 0; k < 3; k++) g.p[3 * b + k] = out.p[b].c[k], g.n[3 * b + k] = out.n[b].c[k];
+        for (int k = 0; k < 6; k++) g.bond_mat[6 * b + k] = out.bmat[b][k];
+      }
+    });
     if (nnodes_out) *nnodes_out = (int64_t)on;
     if (nbonds_out) *nbonds_out = (int64_t)ob;
   });
