@@ -599,3 +599,20 @@ def test_gather_empty_and_refused(ctx):
     bad.around[0] = [[100, 100, 100], [101, 100, 100]]
     with pytest.raises(RuntimeError, match="share no cell"):
         ctx.gather(bad)
+
+
+def test_gather_parity_two_launches(ctx):
+    """More coarse cells than one launch measures (16,384): a 72^3 structure of the generator,
+    gathered once, equal to the oracle's throughout."""
+    occ, _ = sh.generate(72, 72, 72, seed=2, sigma=0.5, pillar_spacing=8.0, pillar_min=3, pillar_max=7)
+    sites, held, _, _ = sh.cells_from_occ(occ)
+    lat = orc.Lattice(0, sites.astype(np.int64), orc.STONE, held)
+    # the coarse cells holding bonds: each bond's cell is the site its two nodes share
+    i, j = lat.ij[::4, 0], lat.ij[::4, 1]
+    ai, aj = lat.around[i], lat.around[j]
+    first = (ai[:, 0] == aj[:, 0]).all(1) | (ai[:, 0] == aj[:, 1]).all(1)
+    shared = np.where(first[:, None], ai[:, 0], ai[:, 1])
+    cells = np.unique((shared >> 1) << 1, axis=0)
+    assert len(cells) > 16384, len(cells)
+    print(f"gather 72^3: {lat.n_nodes} nodes, {lat.n_bonds} bonds, {len(cells)} coarse cells with bonds", flush=True)
+    gather_same(ctx.gather(lat), lat.gathered())
