@@ -13,6 +13,7 @@
 // reference's. A wave measures a cell; its lanes split only work whose values do not depend
 // on how it is split.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -605,14 +606,18 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
   // node j also lies by), already numbered through node i
   const int64_t ncell = (int64_t)sites.size();
   std::vector<int64_t> lies((size_t)nbond), bcount((size_t)ncell + 1, 0);
-  for (int64_t b = 0; b < nbond; b++) {
-    const int64_t i = fine.ij[2 * b], j = fine.ij[2 * b + 1];
-    const Site3 i0 = site_of(i, 0), i1 = site_of(i, 1), j0 = site_of(j, 0), j1 = site_of(j, 1);
-    int k = (i0 == j0 || i0 == j1) ? 0 : 1;
-    if (k == 1 && !(i1 == j0 || i1 == j1)) throw StError{ST_BAD_INPUT, "gather: a bond's nodes share no cell"};
-    lies[b] = between[i][k];
-    bcount[lies[b] + 1]++;
-  }
+  std::atomic<bool> apart{false};
+  par_for(nbond, [&](int64_t b0, int64_t b1) {
+    for (int64_t b = b0; b < b1; b++) {
+      const int64_t i = fine.ij[2 * b], j = fine.ij[2 * b + 1];
+      const Site3 i0 = site_of(i, 0), i1 = site_of(i, 1), j0 = site_of(j, 0), j1 = site_of(j, 1);
+      const int k = (i0 == j0 || i0 == j1) ? 0 : 1;
+      if (k == 1 && !(i1 == j0 || i1 == j1)) apart = true;
+      lies[b] = between[i][k];
+    }
+  });
+  if (apart) throw StError{ST_BAD_INPUT, "gather: a bond's nodes share no cell"};
+  for (int64_t b = 0; b < nbond; b++) bcount[lies[b] + 1]++;
   for (int64_t c = 0; c < ncell; c++) bcount[c + 1] += bcount[c];
   auto face_key = [&](int64_t a, int64_t b) {
     return sites[a] < sites[b] ? std::array<int64_t, 2>{a, b} : std::array<int64_t, 2>{b, a};
@@ -655,26 +660,30 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
   std::vector<int64_t> place((size_t)nn, -1);
   for (auto& cell : cells)
     for (size_t k = 0; k < cell.free.size(); k++) place[cell.free[k]] = (int64_t)k;
-  // each cell's bonds in order, as places (free parts first, then a handle per face)
+  // each cell's bonds in order, as places (free parts first, then a handle per face, in
+  // the order the cell's bonds first name them): cells over threads
   std::vector<int32_t> bij(2 * (size_t)nbond);
   std::vector<int64_t> bidx((size_t)nbond), bfill(bcount.begin(), bcount.end() - 1);
-  for (int64_t b = 0; b < nbond; b++) {
-    const int64_t c = lies[b];
-    Cell& cell = cells[c];
-    auto at = [&](int64_t v) -> int64_t {
-      const int64_t x = between[v][0], y = between[v][1];
-      if (x == c && y == c) return place[v];
-      const int64_t other = x == c ? y : x;
-      size_t h = 0;
-      while (h < cell.across.size() && cell.across[h] != other) h++;
-      if (h == cell.across.size()) cell.across.push_back(other);
-      return (int64_t)(cell.free.size() + h);
-    };
-    const int64_t q = bfill[c]++;
-    bij[2 * q] = (int32_t)at(fine.ij[2 * b]);
-    bij[2 * q + 1] = (int32_t)at(fine.ij[2 * b + 1]);
-    bidx[q] = b;
-  }
+  for (int64_t b = 0; b < nbond; b++) bidx[bfill[lies[b]]++] = b;
+  par_for(ncell, [&](int64_t c0, int64_t c1) {
+    for (int64_t c = c0; c < c1; c++) {
+      Cell& cell = cells[c];
+      auto at = [&](int64_t v) -> int64_t {
+        const int64_t x = between[v][0], y = between[v][1];
+        if (x == c && y == c) return place[v];
+        const int64_t other = x == c ? y : x;
+        size_t h = 0;
+        while (h < cell.across.size() && cell.across[h] != other) h++;
+        if (h == cell.across.size()) cell.across.push_back(other);
+        return (int64_t)(cell.free.size() + h);
+      };
+      for (int64_t q = bcount[c]; q < bcount[c + 1]; q++) {
+        const int64_t b = bidx[q];
+        bij[2 * q] = (int32_t)at(fine.ij[2 * b]);
+        bij[2 * q + 1] = (int32_t)at(fine.ij[2 * b + 1]);
+      }
+    }
+  });
   std::vector<uint8_t> held((size_t)ncell);
   for (int64_t c = 0; c < ncell; c++) {
     bool all = !cells[c].free.empty();
