@@ -480,8 +480,15 @@ void par_for(int64_t n, F f) {
     return;
   }
   std::vector<std::thread> pool;
+  pool.reserve((size_t)t);
   const int64_t chunk = (n + t - 1) / t;
-  for (int64_t a = 0; a < n; a += chunk) pool.emplace_back(f, a, std::min(n, a + chunk));
+  int64_t a = 0;
+  try {
+    for (; a < n; a += chunk) pool.emplace_back(f, a, std::min(n, a + chunk));
+  } catch (const std::exception&) {
+    // a thread that could not start: its range, and the rest, run here
+  }
+  for (; a < n; a += chunk) f(a, std::min(n, a + chunk));
   for (auto& th : pool) th.join();
 }
 
@@ -665,6 +672,8 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
   std::vector<int32_t> bij(2 * (size_t)nbond);
   std::vector<int64_t> bidx((size_t)nbond), bfill(bcount.begin(), bcount.end() - 1);
   for (int64_t b = 0; b < nbond; b++) bidx[bfill[lies[b]]++] = b;
+  for (auto& cell : cells) cell.across.reserve(kMaxHandles);  // the threads below never allocate
+  std::atomic<bool> too_many{false};
   par_for(ncell, [&](int64_t c0, int64_t c1) {
     for (int64_t c = c0; c < c1; c++) {
       Cell& cell = cells[c];
@@ -674,7 +683,13 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
         const int64_t other = x == c ? y : x;
         size_t h = 0;
         while (h < cell.across.size() && cell.across[h] != other) h++;
-        if (h == cell.across.size()) cell.across.push_back(other);
+        if (h == cell.across.size()) {
+          if (h == (size_t)kMaxHandles) {  // a cell has at most six faces
+            too_many = true;
+            return 0;
+          }
+          cell.across.push_back(other);
+        }
         return (int64_t)(cell.free.size() + h);
       };
       for (int64_t q = bcount[c]; q < bcount[c + 1]; q++) {
@@ -684,6 +699,7 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
       }
     }
   });
+  if (too_many) throw StError{ST_BAD_INPUT, "gather: a coarse cell's bonds name more than six faces"};
   std::vector<uint8_t> held((size_t)ncell);
   for (int64_t c = 0; c < ncell; c++) {
     bool all = !cells[c].free.empty();

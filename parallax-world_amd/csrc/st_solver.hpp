@@ -1,5 +1,6 @@
 // Declarations shared by the library's translation units.
 #pragma once
+#include <exception>
 #include "st_kernels.hpp"
 
 #define SINGLE_ARG(...) __VA_ARGS__
@@ -41,6 +42,9 @@ int guard(StCtx* c, F f) {
   } catch (const std::bad_alloc&) {
     c->err = "host allocation failed";
     return ST_OOM;
+  } catch (const std::exception& e) {  // e.g. a host thread that could not start
+    c->err = std::string("host error: ") + e.what();
+    return ST_DEVICE;
   }
 }
 inline void need_upload(StCtx* c) {

@@ -599,6 +599,17 @@ def test_gather_empty_and_refused(ctx):
     bad.around[0] = [[100, 100, 100], [101, 100, 100]]
     with pytest.raises(RuntimeError, match="share no cell"):
         ctx.gather(bad)
+    # more bonds in one coarse cell than any built or gathered lattice has (9 x 60 > 480)
+    many = sh.GatheredLattice(0, lat.n_nodes, 9 * lat.n_bonds)
+    for f in ["x", "m", "node_mat", "held", "around"]:
+        setattr(many, f, getattr(lat, f).copy().reshape(getattr(many, f).shape))
+    for f in ["ij", "p", "n", "a", "d", "bond_mat"]:
+        v = getattr(lat, f).reshape(getattr(bad, f).shape)
+        setattr(many, f, np.concatenate([v] * 9))
+    with pytest.raises(RuntimeError, match="480 bonds"):
+        ctx.gather(many)
+    # and the context is still good for the next call
+    gather_same(ctx.gather(lat), lat.gathered())
 
 
 def test_gather_parity_two_launches(ctx):
