@@ -149,9 +149,9 @@ constexpr int kRowL = kRow + 1;  // LDS row pitch of the elimination (odd: a col
 constexpr int64_t kCaseDoubles = (int64_t)kN2 * kMaxBonds * 3;  // global workspace per cell: the cases
 
 // One wave measures one coarse cell. Lanes share the work only where each value's operations
-// and their order stay those of the reference: the entries one bond adds to (distinct), the
-// rows of an elimination step, the handle motions, the bonds of a pulled beam; the sums over
-// bonds run in bond order, each on one lane.
+// and their order stay those of the reference: the rows of the assembly and of an
+// elimination step, the handle motions, the bonds of a pulled beam; the sums over bonds run
+// in bond order, each on one lane.
 __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws) {
   const int c = blockIdx.x;
   const int lane = threadIdx.x;
@@ -193,34 +193,39 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws) {
   };
   auto CASE = [&](int cs, int b, int q) -> double& { return cases[((int64_t)cs * kMaxBonds + b) * 3 + q]; };
 
-  // responses (gather.rs:59-103): the free parts' equations, the handles' columns on the right;
-  // a bond adds to 144 distinct entries, one lane each, bond after bond
+  // responses (gather.rs:59-103): the free parts' equations, the handles' columns on the
+  // right. A lane owns whole rows and walks the bonds in order, adding each bond's twelve
+  // entries of its row: every entry takes its bonds in the reference's order, no barrier.
   for (int t = lane; t < n * kRowL; t += 64) A[t] = 0.0;
-  __syncthreads();
-  for (int b = 0; b < nb; b++) {
-    const G3 p = BP(b), nv = BN(b);
-    const double kk = BA(b) / BD(b);
-    const double e = BM(b, 1), g = BM(b, 2);
-    const int vs[2] = {bij[2 * b], bij[2 * b + 1]};
-    const double sg[2] = {-1.0, 1.0};
-    for (int t = lane; t < 144; t += 64) {
-      const int x = t / 72, d = (t % 72) / 12, y = (t % 12) / 6, d2 = t % 6;
-      const int r = dof(vs[x], d), col = dof(vs[y], d2);
-      if (r < 0 || r >= n || col < 0) continue;
+  for (int r = lane; r < n; r += 64) {
+    const int v = r / 6, d = r % 6;
+    for (int b = 0; b < nb; b++) {
+      const int vs[2] = {bij[2 * b], bij[2 * b + 1]};
+      const int x = vs[0] == v ? 0 : (vs[1] == v ? 1 : -1);
+      if (x < 0) continue;
+      const G3 p = BP(b), nv = BN(b);
+      const double kk = BA(b) / BD(b);
+      const double e = BM(b, 1), g = BM(b, 2);
+      const double sg[2] = {-1.0, 1.0};
       double st[3][3];  // Bond::stiffness (model.rs:139-149)
-      for (int a = 0; a < 3; a++)
+      for (int a2 = 0; a2 < 3; a2++)
         for (int q = 0; q < 3; q++) {
-          const double nn = nv.c[a] * nv.c[q];
-          st[a][q] = e * kk * nn + g * kk * ((a == q ? 1.0 : 0.0) - nn);
+          const double nn = nv.c[a2] * nv.c[q];
+          st[a2][q] = e * kk * nn + g * kk * ((a2 == q ? 1.0 : 0.0) - nn);
         }
-      const G3 cv = column(vs[x], d, p);
+      const G3 cv = column(v, d, p);
       const G3 scv{{gdot(G3{{st[0][0], st[0][1], st[0][2]}}, cv), gdot(G3{{st[1][0], st[1][1], st[1][2]}}, cv),
                     gdot(G3{{st[2][0], st[2][1], st[2][2]}}, cv)}};
-      const double sign = col < n ? 1.0 : -1.0;
-      A[r * kRowL + col] += sign * sg[x] * sg[y] * gdot(scv, column(vs[y], d2, p));
+      for (int y = 0; y < 2; y++)
+        for (int d2 = 0; d2 < 6; d2++) {
+          const int col = dof(vs[y], d2);
+          if (col < 0) continue;
+          const double sign = col < n ? 1.0 : -1.0;
+          A[r * kRowL + col] += sign * sg[x] * sg[y] * gdot(scv, column(vs[y], d2, p));
+        }
     }
-    __syncthreads();
   }
+  __syncthreads();
 
   // gather.rs:165-186, then the solutions x (the free parts' motions for each handle motion)
   const double tiny_a = wave_solve(A, kRowL, n, m, lane);
