@@ -98,6 +98,9 @@ def lib():
         L.orc_gather_solve.argtypes = [ctypes.c_size_t, ctypes.c_size_t, vp, vp]
         L.orc_beams.restype = ctypes.c_size_t
         L.orc_beams.argtypes = [ctypes.c_size_t, vp, vp, vp, vp]
+        L.orc_cell_stiffness.restype = ctypes.c_size_t
+        L.orc_cell_stiffness.argtypes = [vp, ctypes.c_size_t, vp, vp]
+        L.orc_energy.argtypes = [vp, vp, vp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_threads.restype = ctypes.c_int
         _lib = L
@@ -227,6 +230,25 @@ class Lattice:
         fallen, rounds, work = ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
         lib().orc_settle(self.h, tol, _p(gone), ctypes.byref(fallen), ctypes.byref(rounds), ctypes.byref(work))
         return dict(fallen=fallen.value, gone=gone, rounds=rounds.value, work=work.value)
+
+    def cell_stiffness(self, cap=4096):
+        """Each measured cell's stiffness between its handles (statics/src/gather.rs:291-303), as
+        measured(self, false) finds it: a list of (3 nh, 3 nh) arrays."""
+        out = np.zeros(cap * (1 + 18 * 18))
+        n = ctypes.c_size_t()
+        cells = lib().orc_cell_stiffness(self.h, cap, _p(out), ctypes.byref(n))
+        ks, at = [], 0
+        for _ in range(cells):
+            m = 3 * int(out[at])
+            ks.append(out[at + 1:at + 1 + m * m].reshape(m, m).copy())
+            at += 1 + m * m
+        return ks
+
+    def energy(self, f):
+        """Bond::energy (statics/src/model.rs:132-136) of each bond bearing f (bonds, 3)."""
+        out = np.zeros(self.n_bonds)
+        lib().orc_energy(self.h, _p(np.ascontiguousarray(f, np.float64)), _p(out))
+        return out
 
     def scale_cell(self, cell, by):
         lib().orc_scale_cell(self.h, int(cell), float(by))

@@ -10,7 +10,7 @@
 //                           cycle, forces, moved, motions, bodies_of, settle, used,
 //                           used_standing, joined_to_held, invert, find
 //   statics/src/gather.rs   Cell::{handle, pieces}, responses, beams, solve, full,
-//                           measured, gathered (Bond::energy inline, model.rs:132-136)
+//                           measured, gathered (and Bond::energy, model.rs:132-136)
 // Rust semantics mirrored: no contraction of a * b + c (built with -ffp-contract=off);
 // sums are sequential left folds; `max_by` keeps the
 // LAST maximum (invert's pivot); `>>` on i64 floors; f64::round is half away
@@ -103,6 +103,12 @@ struct Bond {
     double tau2 = std::max(dot(t, t) - sigma * sigma, 0.0);
     double r = std::sqrt(sigma * sigma / 4.0 + tau2);
     return mat.used(sigma / 2.0 + r, sigma / 2.0 - r);
+  }
+  // model.rs:132-136
+  double energy(V3 f) const {
+    double k = a / d;
+    double along = dot(f, n);
+    return 0.5 * (along * along / (mat.e * k) + (dot(f, f) - along * along) / (mat.g * k));
   }
   // model.rs:139-149
   M3 stiffness() const {
@@ -1124,6 +1130,8 @@ std::vector<std::tuple<size_t, size_t, double>> beams(const std::vector<std::vec
 
 using Full = std::array<std::pair<double, double>, 2>;
 Lattice measured(const Lattice& fine, bool against_full);
+// When set, measured() keeps each measured cell's stiffness between its handles (a test hook).
+std::vector<std::vector<double>>* g_keep_k = nullptr;
 
 // lattice.rs:87-102 (solids), for a box all of one material
 std::vector<Solid> solids_full(uint32_t level, V3 lo, V3 hi, Mat mat) {
@@ -1284,6 +1292,7 @@ Lattice measured(const Lattice& fine, bool against_full) {
     mat[c] = mat_of({c});
   }
   for (size_t c = 0; c < n; c++) fulls[c] = against_full ? full(mat[c]) : Full{{{1.0, 1.0}, {1.0, 1.0}}};
+  if (g_keep_k) g_keep_k->assign(n, {});
   auto found = each<std::vector<Found>>(n, [&](size_t c) {
     const GCell& cell = cells[c];
     const std::vector<V3>& pts = at[c];
@@ -1303,6 +1312,13 @@ Lattice measured(const Lattice& fine, bool against_full) {
           if (v >= nf)
             for (size_t q = 0; q < 3; q++) k[3 * (v - nf) + q][cs] += sign * f[q];
       }
+    if (g_keep_k) {
+      std::vector<double> flat;
+      for (auto& row : k) flat.insert(flat.end(), row.begin(), row.end());
+      static std::mutex keep;
+      std::lock_guard<std::mutex> g(keep);
+      (*g_keep_k)[c] = std::move(flat);
+    }
     std::vector<size_t> piece = cell.pieces();
     for (auto [a, b, stiff] : beams(k, pts, [&](size_t x, size_t y) { return piece[x] == piece[y]; })) {
       V3 e = unit(sub(pts[b], pts[a]));
@@ -1316,11 +1332,7 @@ Lattice measured(const Lattice& fine, bool against_full) {
       double hd = 0.0, work = 0.0;
       std::vector<double> u = used(lat, pulled);
       for (size_t q = 0; q < lat.bonds.size(); q++) {
-        const Bond& bd = lat.bonds[q];
-        double kk = bd.a / bd.d;
-        double along = dot(pulled[q], bd.n);
-        double w = 0.5 * (along * along / (bd.mat.e * kk) +
-                          (dot(pulled[q], pulled[q]) - along * along) / (bd.mat.g * kk));  // model.rs:132-136
+        double w = lat.bonds[q].energy(pulled[q]);
         hd += w * u[q];
         work += w;
       }
@@ -1576,6 +1588,33 @@ void orc_mats(const OrcLattice* h, double* node_mat, double* bond_mat) {
   };
   for (size_t v = 0; v < L->nodes.size(); v++) put(L->nodes[v].mat, node_mat + 6 * v);
   for (size_t b = 0; b < L->bonds.size(); b++) put(L->bonds[b].mat, bond_mat + 6 * b);
+}
+
+// measured(fine, false)'s stiffness between handles of each measured cell (gather.rs:291-303):
+// k_out gets, cell after cell, nh_out[i] then its (3 nh)^2 entries row by row; returns how many
+// cells (at most cap) and the doubles written in *len.
+size_t orc_cell_stiffness(const OrcLattice* h, size_t cap, double* k_out, size_t* len) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  std::vector<std::vector<double>> ks;
+  g_keep_k = &ks;
+  Lattice out = measured(*L, false);
+  g_keep_k = nullptr;
+  size_t cells = 0, at = 0;
+  for (auto& k : ks) {
+    if (k.empty() || cells == cap) continue;
+    size_t m = (size_t)std::llround(std::sqrt((double)k.size()));
+    k_out[at++] = (double)(m / 3);
+    for (double v : k) k_out[at++] = v;
+    cells++;
+  }
+  *len = at;
+  return cells;
+}
+
+// Bond::energy (model.rs:132-136) of each bond bearing f (3 per bond)
+void orc_energy(const OrcLattice* h, const double* f, double* out) {
+  auto* L = reinterpret_cast<const Lattice*>(h);
+  for (size_t b = 0; b < L->bonds.size(); b++) out[b] = L->bonds[b].energy({f[3 * b], f[3 * b + 1], f[3 * b + 2]});
 }
 
 // gather.rs:165-186 on an n x (n + rhs) row-major matrix; x: n x rhs

@@ -395,3 +395,54 @@ def test_gather_beams_fit():
     # not joined: no beam between 0 and 1
     joined[0, 1] = joined[1, 0] = 0
     assert (0, 1) not in [(a, b) for a, b, _ in orc.beams(at, k, joined)]
+
+
+def _blob(n, seed):
+    rng = np.random.default_rng(seed)
+    sites, mats, held = [], [], []
+    for x in range(n):
+        for y in range(n):
+            for z in range(n):
+                if rng.random() < 0.7 or y == 0:
+                    sites.append((x - n // 2, y, z - n // 2))
+                    mats.append(orc.STONE if rng.random() < 0.6 else orc.WOOD)
+                    held.append(1 if y == 0 else 0)
+    return orc.Lattice(0, np.array(sites), np.array(mats), np.array(held, np.uint8))
+
+
+@pytest.mark.parametrize("levels", [0, 1])
+def test_gather_cell_stiffness_invariants(levels):
+    """gather.rs:59-114, 291-303: a cell's stiffness between its handles is its matter's
+    stiffness condensed onto them, so it is symmetric, positive semi-definite, and moving every
+    handle by one translation (the free parts following) asks no force. A wrong sign, a dropped
+    term or a transposed product in the elimination, the handle columns or the reactions breaks
+    one of these; a full cell's normalisation cannot hide it."""
+    lat = _blob(8, 11)
+    for _ in range(levels):
+        lat = lat.gathered()
+    ks = lat.cell_stiffness()
+    assert len(ks) >= 8
+    for k in ks:
+        m = len(k)
+        scale = np.abs(k).max()
+        assert scale > 0
+        assert np.abs(k - k.T).max() <= 1e-9 * scale
+        for t in np.eye(3):
+            assert np.abs(k @ np.tile(t, m // 3)).max() <= 1e-9 * scale * m
+        ev = np.linalg.eigvalsh(0.5 * (k + k.T))
+        assert ev.min() >= -1e-9 * ev.max()
+        assert (ev > 1e-6 * ev.max()).sum() >= 1
+
+
+def test_gather_energy_is_compliance():
+    """Bond::energy (model.rs:132-136), which weighs the finer bonds' shares of a pulled beam,
+    is half the force through the inverse of the bond's own spring matrix."""
+    lat = _blob(4, 2)
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((lat.n_bonds, 3)) * 1e3
+    w = lat.energy(f)
+    for b in rng.choice(lat.n_bonds, 50, replace=False):
+        k = lat.a[b] / lat.d[b]
+        nn = np.outer(lat.n[b], lat.n[b])
+        s = lat.e[b] * k * nn + lat.g[b] * k * (np.eye(3) - nn)
+        assert w[b] == pytest.approx(0.5 * f[b] @ np.linalg.solve(s, f[b]), rel=1e-12)
