@@ -2,7 +2,7 @@
 
 Bars: classification (joined_to_held) bit-exact; the operator and bond forces to FP64
 rounding; displacements within 1e-6 relative of the oracle's tightly converged solve
-(the north star's bar; the tests ask for 1e-8 at tol 1e-10); gradients within 1e-4 of
+(the parity bar for solves; the tests ask for 1e-8 at tol 1e-10); gradients within 1e-4 of
 central differences (eps = 1e-4) of the oracle's solve.
 """
 import numpy as np
@@ -139,7 +139,7 @@ def test_solve_parity_reference_scenes(ctx, scene):
     f = P.L.gravity()  # -y, as forces (statics/src/bodies.rs:408)
     x_or, _ = P.L.motions(f, gone=gone, tol=1e-13, max_iter=5000)
     # stone's stiff slab floors the FP64 true residual near 5e-10 for the oracle too (its recursive
-    # residual reaches 1e-14): these scenes are held to the north star's bar, 1e-8 and 1e-6
+    # residual reaches 1e-14): these scenes are held to the solve parity bar, tol 1e-8 and 1e-6
     for prec in (sh.FP64, sh.MIXED):
         x_dev, st = ctx.solve(P.to_dev(f), precision=prec, tol=1e-8)
         assert st.status == 0
