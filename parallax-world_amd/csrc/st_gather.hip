@@ -594,6 +594,36 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
   const auto t0 = clock::now();
   const uint32_t level = fine.level + 1;
   const int64_t nn = fine.nn, nbond = fine.nb;
+  // The caller's bond arrays go to the device on a host thread while the bookkeeping runs
+  // (only the kernel reads them). The guard joins it however this call ends.
+  DBuf dbp, dbn, dba, dbd, dbm;
+  hipError_t up_err = hipSuccess;
+  std::thread uploader;
+  struct Join {
+    std::thread& t;
+    ~Join() {
+      if (t.joinable()) t.join();
+    }
+  } join{uploader};
+  if (nbond > 0) {
+    ST_HIP(hipSetDevice(ctx->device));
+    const size_t nb8 = (size_t)nbond * sizeof(double);
+    dbp.ensure(3 * nb8), dbn.ensure(3 * nb8), dba.ensure(nb8), dbd.ensure(nb8), dbm.ensure(6 * nb8);
+    auto upload = [&, nb8] {
+      hipError_t e = hipSetDevice(ctx->device);
+      const std::pair<DBuf*, std::pair<const double*, size_t>> parts[5] = {
+          {&dbp, {fine.p, 3 * nb8}}, {&dbn, {fine.n, 3 * nb8}}, {&dba, {fine.a, nb8}}, {&dbd, {fine.d, nb8}},
+          {&dbm, {fine.bmat, 6 * nb8}}};
+      for (auto& pt : parts)
+        if (e == hipSuccess) e = hipMemcpy(pt.first->p, pt.second.first, pt.second.second, hipMemcpyHostToDevice);
+      up_err = e;
+    };
+    try {
+      uploader = std::thread(upload);
+    } catch (const std::exception&) {
+      upload();  // no thread to be had: upload here
+    }
+  }
   auto site_of = [&](int64_t v, int k) {
     const int64_t* q = fine.around + 6 * v + 3 * k;
     return Site3{q[0], q[1], q[2]};
@@ -804,17 +834,15 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
   if (nj) {
     ST_HIP(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
-    DBuf dnf, dnh, dnb, dboff, dfoff, dhoff, dfx, dhx, dpiece, dbij, dbidx, dbp, dbn, dba, dbd, dbm, dcmat, dcfull,
-        dcc, dfound, dfab, dfval, dws;
+    DBuf dnf, dnh, dnb, dboff, dfoff, dhoff, dfx, dhx, dpiece, dbij, dbidx, dcmat, dcfull, dcc, dfound, dfab, dfval,
+        dws;
     put(dnf, nf, st), put(dnh, nh, st), put(dnb, nb, st), put(dboff, boff, st), put(dfoff, foff, st);
     put(dhoff, hoff, st), put(dfx, fx, st), put(dhx, hx, st), put(dpiece, piece, st), put(dbij, bij, st);
     put(dbidx, bidx, st), put(dcmat, cmat, st), put(dcfull, cfull, st), put(dcc, ccentre, st);
-    auto raw = [&](DBuf& d, const double* h, size_t count) {
-      d.ensure(std::max<size_t>(count, 1) * sizeof(double));
-      if (count) ST_HIP(hipMemcpyAsync(d.p, h, count * sizeof(double), hipMemcpyHostToDevice, st));
-    };
-    raw(dbp, fine.p, 3 * (size_t)nbond), raw(dbn, fine.n, 3 * (size_t)nbond), raw(dba, fine.a, (size_t)nbond);
-    raw(dbd, fine.d, (size_t)nbond), raw(dbm, fine.bmat, 6 * (size_t)nbond);
+    if (uploader.joinable()) uploader.join();
+    if (up_err != hipSuccess)
+      throw StError{up_err == hipErrorOutOfMemory ? ST_OOM : ST_DEVICE,
+                    std::string("gather: bond upload: ") + hipGetErrorString(up_err)};
     dfound.ensure(nj * sizeof(int32_t));
     dfab.ensure(fab.size() * sizeof(int32_t));
     dfval.ensure(fval.size() * sizeof(double));
