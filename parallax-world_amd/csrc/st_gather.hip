@@ -629,21 +629,26 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
     return Site3{q[0], q[1], q[2]};
   };
   // the coarse cells, numbered as first met, each finer node's two in turn
-  std::unordered_map<Site3, int64_t, Site3Hash> index;
-  index.reserve((size_t)nn / 2 + 16);
+  // (an open-addressing table of cell numbers: slots at least twice the nodes, linear probing)
+  size_t cap = 64;
+  while (cap < 2 * (size_t)nn + 16) cap <<= 1;
+  std::vector<int64_t> table(cap, -1);
+  const Site3Hash hasher;
   std::vector<Site3> sites;
   std::vector<std::array<int64_t, 2>> between((size_t)nn);
   for (int64_t v = 0; v < nn; v++)
     for (int k = 0; k < 2; k++) {
       Site3 key = site_of(v, k);
       for (int t = 0; t < 3; t++) key[t] = (key[t] >> level) << level;
-      auto it = index.find(key);
-      if (it == index.end()) {
-        it = index.emplace(key, (int64_t)sites.size()).first;
+      size_t h = hasher(key) & (cap - 1);
+      while (table[h] >= 0 && sites[table[h]] != key) h = (h + 1) & (cap - 1);
+      if (table[h] < 0) {
+        table[h] = (int64_t)sites.size();
         sites.push_back(key);
       }
-      between[v][k] = it->second;
+      between[v][k] = table[h];
     }
+  table = std::vector<int64_t>();
   // each bond's cell: that of the site its two nodes share (the first of node i's two that
   // node j also lies by), already numbered through node i
   const int64_t ncell = (int64_t)sites.size();
@@ -742,7 +747,22 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
     held[c] = all;
     for (int64_t o : cells[c].across) recs.push_back({face_key(c, o), -1});
   }
-  std::stable_sort(recs.begin(), recs.end(), [](const FaceRec& a, const FaceRec& b) { return a.key < b.key; });
+  {  // stable sort by key: counting sort on the first cell, then each bucket by the second
+    std::vector<int64_t> start((size_t)ncell + 1, 0);
+    for (const FaceRec& r : recs) start[r.key[0] + 1]++;
+    for (int64_t c = 0; c < ncell; c++) start[c + 1] += start[c];
+    std::vector<FaceRec> sorted(recs.size());
+    std::vector<int64_t> fill(start.begin(), start.end() - 1);
+    for (const FaceRec& r : recs) sorted[fill[r.key[0]]++] = r;
+    for (int64_t c = 0; c < ncell; c++)  // a few each: insertion sort, which keeps equal keys in order
+      for (int64_t i = start[c] + 1; i < start[c + 1]; i++) {
+        const FaceRec r = sorted[i];
+        int64_t j = i;
+        for (; j > start[c] && sorted[j - 1].key[1] > r.key[1]; j--) sorted[j] = sorted[j - 1];
+        sorted[j] = r;
+      }
+    recs.swap(sorted);
+  }
   auto mat_of = [&](std::initializer_list<int64_t> cs) {
     std::vector<std::pair<Mat6, double>> all;
     for (int64_t c : cs)
