@@ -810,9 +810,11 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
   for (int64_t c = 0; c < ncell; c++)
     if (!held[c] && !cells[c].across.empty()) job.push_back(c);
   const int64_t nj = (int64_t)job.size();
-  std::vector<int32_t> nf(nj), nh(nj), nb(nj), piece;
-  std::vector<int64_t> boff(nj), foff(nj), hoff(nj);
-  std::vector<double> fx, hx, cmat(4 * nj), cfull(4 * nj), ccentre(3 * nj);
+  std::vector<int32_t> nf(nj), nh(nj), nb(nj);
+  std::vector<int64_t> boff(nj), foff(nj + 1, 0), hoff(nj + 1, 0);
+  std::vector<double> cmat(4 * nj), cfull(4 * nj), ccentre(3 * nj);
+  // sizes and offsets in order; the full-cell measures (each material's once, which may
+  // measure a block of its own) in order too
   for (int64_t q = 0; q < nj; q++) {
     const int64_t c = job[q];
     const Cell& cell = cells[c];
@@ -822,32 +824,43 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
     nf[q] = (int32_t)cell.free.size();
     nh[q] = (int32_t)cell.across.size();
     nb[q] = (int32_t)cb;
-    foff[q] = (int64_t)fx.size() / 3;
-    hoff[q] = (int64_t)hx.size() / 3;
+    foff[q + 1] = foff[q] + nf[q];
+    hoff[q + 1] = hoff[q] + nh[q];
     boff[q] = bcount[c];
-    for (int64_t v : cell.free)
-      for (int k = 0; k < 3; k++) fx.push_back(fine.x[3 * v + k]);
-    for (int64_t k : ends[c])
-      for (int t = 0; t < 3; t++) hx.push_back(out.x[k].c[t]);
-    // which of its handles its matter joins (gather.rs:42-50): a piece of each
-    std::vector<int64_t> parent(cell.free.size() + cell.across.size());
-    for (size_t k = 0; k < parent.size(); k++) parent[k] = (int64_t)k;
-    auto root = [&](int64_t a) {
-      while (parent[a] != a) a = parent[a] = parent[parent[a]];
-      return a;
-    };
-    for (int64_t t = bcount[c]; t < bcount[c + 1]; t++) {
-      const int64_t ra = root(bij[2 * t]);
-      parent[ra] = root(bij[2 * t + 1]);
-    }
-    for (size_t h = 0; h < cell.across.size(); h++) piece.push_back((int32_t)root((int64_t)(cell.free.size() + h)));
-    cmat[4 * q] = mat[c][1];
-    for (int k = 0; k < 3; k++) cmat[4 * q + 1 + k] = mat[c][3 + k];
     const std::array<double, 4> f = against_full ? full_of(ctx, mat[c]) : std::array<double, 4>{1.0, 1.0, 1.0, 1.0};
     for (int k = 0; k < 4; k++) cfull[4 * q + k] = f[k];
-    const G3 ce = hcentre(sites[c], level);
-    for (int k = 0; k < 3; k++) ccentre[3 * q + k] = ce.c[k];
   }
+  std::vector<double> fx(3 * (size_t)foff[nj]), hx(3 * (size_t)hoff[nj]);
+  std::vector<int32_t> piece((size_t)hoff[nj]);
+  par_for(nj, [&](int64_t q0, int64_t q1) {
+    int64_t parent[kMaxFree + kMaxHandles];  // (the threads allocate nothing)
+    for (int64_t q = q0; q < q1; q++) {
+      const int64_t c = job[q];
+      const Cell& cell = cells[c];
+      for (size_t k = 0; k < cell.free.size(); k++)
+        for (int t = 0; t < 3; t++) fx[3 * (foff[q] + (int64_t)k) + t] = fine.x[3 * cell.free[k] + t];
+      for (size_t k = 0; k < ends[c].size(); k++)
+        for (int t = 0; t < 3; t++) hx[3 * (hoff[q] + (int64_t)k) + t] = out.x[ends[c][k]].c[t];
+      // which of its handles its matter joins (gather.rs:42-50): a piece of each
+      for (size_t k = 0; k < cell.free.size() + cell.across.size(); k++) parent[k] = (int64_t)k;
+      auto root = [&](int64_t a) {
+        while (parent[a] != a) a = parent[a] = parent[parent[a]];
+        return a;
+      };
+      for (int64_t t = bcount[c]; t < bcount[c + 1]; t++) {
+        const int64_t ra = root(bij[2 * t]);
+        parent[ra] = root(bij[2 * t + 1]);
+      }
+      for (size_t h = 0; h < cell.across.size(); h++)
+        piece[hoff[q] + (int64_t)h] = (int32_t)root((int64_t)(cell.free.size() + h));
+      cmat[4 * q] = mat[c][1];
+      for (int k = 0; k < 3; k++) cmat[4 * q + 1 + k] = mat[c][3 + k];
+      const G3 ce = hcentre(sites[c], level);
+      for (int k = 0; k < 3; k++) ccentre[3 * q + k] = ce.c[k];
+    }
+  });
+  foff.pop_back();
+  hoff.pop_back();
   const auto t2 = clock::now();
   std::vector<int32_t> nfound(nj), fab((size_t)nj * 2 * kMaxPairs);
   std::vector<double> fval((size_t)nj * 2 * kMaxPairs);
