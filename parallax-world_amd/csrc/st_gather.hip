@@ -41,7 +41,6 @@ constexpr double kSection[2] = {0.336, 0.336};         // lattice.rs:25
 constexpr double kUnit = 0.125;                        // lattice.rs:16
 constexpr int kPerBeam = 4;                            // lattice.rs:19
 
-
 struct G3 {
   double c[3];
 };
@@ -197,6 +196,7 @@ __global__ __launch_bounds__(64) void k_measure_cells(GJob j, double* ws) {
   // right. A lane owns whole rows and walks the bonds in order, adding each bond's twelve
   // entries of its row: every entry takes its bonds in the reference's order, no barrier.
   for (int t = lane; t < n * kRowL; t += 64) A[t] = 0.0;
+  __syncthreads();  // rows are zeroed by other lanes than their owners
   for (int r = lane; r < n; r += 64) {
     const int v = r / 6, d = r % 6;
     for (int b = 0; b < nb; b++) {
