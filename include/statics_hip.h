@@ -78,6 +78,10 @@ typedef struct {
                           < 0: the reference's rule (statics/src/bodies.rs:350-352), the
                           recursive residual decides; rel_residual reports the true one */
   int32_t warm_start;  /* nonzero: start from x_out's contents instead of 0 */
+  int32_t release_hierarchy; /* nonzero: free the multigrid hierarchy when the solve ends,
+                                so the next solve (st_backward's adjoint included) recomputes
+                                it instead of reading a stored one; 0: keep it for the next
+                                solve of the same operator in the same precision mode */
 } StSolveOpts;
 
 typedef struct {
@@ -92,7 +96,19 @@ typedef struct {
   int64_t dof_nodes;      /* free nodes: not held, not gone */
   int64_t cells;          /* free cells */
   int64_t bricks;         /* bricks on the device */
+  int32_t retries;        /* 1 if the solve was repeated from zero with the robust cycle
+                             settings (two cycles on coarse levels 3-6, FP64 from level 3)
+                             after the fast ones failed; 0 otherwise */
+  int32_t first_status;   /* with retries = 1: the StStatus the fast settings ended with */
+  int64_t mem_peak;       /* the most device memory (bytes) the library's buffers held in
+                             this process during the call */
 } StStats;
+
+/* The layouts a binding must reproduce (x86-64 and aarch64 C ABI; statics_hip checks
+ * them at compile time): StSolveOpts is 32 bytes, precision at 0, tol at 8, max_iter at
+ * 16, refresh at 20, warm_start at 24, release_hierarchy at 28. StStats is 88 bytes:
+ * iters 0, levels 4, refreshes 8, status 12, rel_residual 16, setup_ms 24, solve_ms 32,
+ * nodes 40, dof_nodes 48, cells 56, bricks 64, retries 72, first_status 76, mem_peak 80. */
 
 /* ---- context ---------------------------------------------------------- */
 int st_create(StCtx** ctx, int device);
@@ -284,7 +300,10 @@ int st_bond_used(StCtx* ctx, const double* strength, const double* x, double* us
  * iters, setup_ms and solve_ms summed over the rounds. The fallen share
  * (bodies.rs:535-540) is the mass of the gone nodes over the free mass, which
  * the caller holds. Afterwards the context keeps the final masks and gone set,
- * and the last round's solution. */
+ * and the last round's solution. stats->retries counts the rounds whose solve was
+ * repeated with the robust cycle settings; stats->first_status is ST_NO_CONVERGENCE
+ * when some round under the reference's rule (refresh < 0) ended with a true residual
+ * above tol (its rel_residual is the worst such round's), else ST_OK. */
 int st_settle(StCtx* ctx, const double* strength, const double* loads, const StSolveOpts* opts, uint8_t* gone_out,
               uint16_t* beams_out, int32_t* rounds_out, StStats* stats);
 
@@ -294,8 +313,12 @@ typedef enum { ST_COMPLIANCE = 0, ST_SOFTMAX_STRESS = 1, ST_VALUE_ONLY = 256 } S
  * cells), where theta is the cell's log-stiffness (e and g both scale by
  * exp(theta)). ST_COMPLIANCE: J = f.x. ST_SOFTMAX_STRESS: J = (1/beta) log
  * sum_b exp(beta sigma_b) over live bonds b, sigma_b = (f_b . n_b)/a_b, the
- * bond's normal stress. The adjoint solve reuses the operator and rebuilds
- * the hierarchy; it runs with the precision and tolerance of opts.
+ * bond's normal stress. The adjoint solve K lambda = dJ/dx reuses the operator (K is
+ * symmetric) and runs with the precision and tolerance of opts. Its multigrid hierarchy
+ * is the one the forward solve kept if that solve ran with release_hierarchy = 0 in the
+ * same precision mode; otherwise (release_hierarchy = 1 on the forward solve, the
+ * recompute-instead-of-store mode) the adjoint builds it anew and, with
+ * opts->release_hierarchy, frees it again when done.
  * dJ_dtheta_out may be NULL (the gradient then stays on the device).
  * objective | ST_VALUE_ONLY: J alone, no adjoint solve, no gradient. */
 int st_backward(StCtx* ctx, int objective, double beta, const StSolveOpts* opts, double* J_out,

@@ -4,10 +4,23 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
 
 #include "st_solver.hpp"
+
+// The layouts include/statics_hip.h states for bindings in other languages.
+static_assert(sizeof(StSolveOpts) == 32 && offsetof(StSolveOpts, tol) == 8 && offsetof(StSolveOpts, max_iter) == 16 &&
+                  offsetof(StSolveOpts, refresh) == 20 && offsetof(StSolveOpts, warm_start) == 24 &&
+                  offsetof(StSolveOpts, release_hierarchy) == 28,
+              "StSolveOpts layout");
+static_assert(sizeof(StStats) == 88 && offsetof(StStats, rel_residual) == 16 && offsetof(StStats, setup_ms) == 24 &&
+                  offsetof(StStats, solve_ms) == 32 && offsetof(StStats, nodes) == 40 &&
+                  offsetof(StStats, dof_nodes) == 48 && offsetof(StStats, cells) == 56 &&
+                  offsetof(StStats, bricks) == 64 && offsetof(StStats, retries) == 72 &&
+                  offsetof(StStats, first_status) == 76 && offsetof(StStats, mem_peak) == 80,
+              "StStats layout");
 
 namespace st {
 namespace {
@@ -640,6 +653,14 @@ int st_settle(StCtx* c, const double* strength, const double* loads, const StSol
       total.iters += st_out[0].iters;
       total.setup_ms += st_out[0].setup_ms;
       total.solve_ms += st_out[0].solve_ms;
+      total.retries += st_out[0].retries;
+      total.mem_peak = std::max(total.mem_peak, st_out[0].mem_peak);
+      // under the reference's rule the recursive residual decides: a round whose true
+      // residual is above tol is flagged (first_status) with the worst such residual
+      if (o.refresh < 0 && !(st_out[0].rel_residual <= o.tol * (1.0 + 1e-9))) {
+        total.first_status = ST_NO_CONVERGENCE;
+        total.rel_residual = std::max(total.rel_residual, st_out[0].rel_residual);
+      }
       const bool verbose = std::getenv("STATICS_HIP_VERBOSE") != nullptr;
       // bodies.rs:523-533: beams with an overused bond break
       ST_HIP(hipMemsetAsync(broken.p, 0, sizeof(int), c->stream));
@@ -679,6 +700,10 @@ int st_settle(StCtx* c, const double* strength, const double* loads, const StSol
       stats->iters = total.iters;
       stats->setup_ms = total.setup_ms;
       stats->solve_ms = total.solve_ms;
+      stats->retries = total.retries;
+      stats->first_status = total.first_status;
+      stats->mem_peak = total.mem_peak;
+      if (total.first_status != ST_OK) stats->rel_residual = total.rel_residual;
     }
   });
   return rc;

@@ -166,13 +166,16 @@ void launch_apply(Ctx& c, const T* x, const Epi& epi) {
       k_apply<T, Epi, 0, 0><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
       break;
     default: {
-      static size_t attr_bytes = 0;  // one per instantiation
+      // the dynamic LDS limit is raised once per context and kernel instance (a context
+      // owns its device; each keeps what it set, so contexts on other threads or devices
+      // neither share nor race on it)
+      const void* fn = reinterpret_cast<const void*>(&k_apply_cells<T, Epi>);
       const DevGeo& d = c.geo;
       const size_t need = (size_t)std::max(d.class_nodes[0], d.class_nodes[1]) * 6 * sizeof(T);
-      if (need > attr_bytes) {
-        ST_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_apply_cells<T, Epi>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-        attr_bytes = need;
+      size_t& have = c.lds_attr[fn];
+      if (need > have) {
+        ST_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+        have = need;
       }
       const uint32_t* bl = d.blist.as<uint32_t>();
       for (int k = 0; k < 2; k++) {

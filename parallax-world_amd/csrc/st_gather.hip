@@ -616,6 +616,9 @@ void measure(Ctx* ctx, const LatIn& fine, bool against_full, HLat& out) {
           {&dbm, {fine.bmat, 6 * nb8}}};
       for (auto& pt : parts)
         if (e == hipSuccess) e = hipMemcpy(pt.first->p, pt.second.first, pt.second.second, hipMemcpyHostToDevice);
+      // a pageable host-to-device hipMemcpy may return before its DMA lands; the kernel
+      // reading these arrays runs on another stream, so wait for the device here
+      if (e == hipSuccess) e = hipDeviceSynchronize();
       up_err = e;
     };
     try {

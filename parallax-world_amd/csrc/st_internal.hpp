@@ -19,7 +19,9 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <atomic>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -44,20 +46,40 @@ struct StError {
                           std::string(#call) + ": " + hipGetErrorString(e_)};                 \
   } while (0)
 
+// Device bytes the library's buffers hold in this process, and the most they held since
+// the last reset (StStats.mem_peak).
+struct MemTally {
+  std::atomic<int64_t> held{0}, peak{0};
+  void add(int64_t b) {
+    const int64_t now = held.fetch_add(b) + b;
+    int64_t p = peak.load();
+    while (now > p && !peak.compare_exchange_weak(p, now)) {
+    }
+  }
+  void reset_peak() { peak.store(held.load()); }
+};
+inline MemTally& mem_tally() {
+  static MemTally t;
+  return t;
+}
+
 // A device buffer that grows on demand and is never shrunk (freed at context end).
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
   void ensure(size_t n) {
     if (n <= bytes) return;
-    if (p) ST_HIP(hipFree(p));
-    p = nullptr;
-    bytes = 0;
-    ST_HIP(hipMalloc(&p, n < 256 ? 256 : n));
-    bytes = n < 256 ? 256 : n;
+    release();
+    const size_t want = n < 256 ? 256 : n;
+    ST_HIP(hipMalloc(&p, want));
+    bytes = want;
+    mem_tally().add((int64_t)want);
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      mem_tally().add(-(int64_t)bytes);
+    }
     p = nullptr;
     bytes = 0;
   }
@@ -194,6 +216,7 @@ struct Ctx {
   bool graphs = false;   // replay each CG iteration's tail as a captured graph
   bool dense_top = true; // invert a small coarse level densely and drop the levels above
   int apply_sched[2] = {0, 0};  // operator schedule per precision (FP32, FP64), see kSchedules
+  std::map<const void*, size_t> lds_attr;  // per kernel: the dynamic LDS limit this context set
   bool tuned = false;           // apply_sched chosen for the uploaded structures
   std::vector<Level> levels;        // coarse levels 1..
   // The hierarchy (fine smoother and coarse levels) is kept between solves of the same

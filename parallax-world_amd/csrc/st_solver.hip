@@ -2960,6 +2960,16 @@ void release_levels(Ctx& c) {
   c.levels.clear();
 }
 
+// The hierarchy's device memory: the coarse levels, the fine smoother and the level-1
+// aggregation with its prolongator. The next solve builds them again.
+void release_hierarchy(Ctx& c) {
+  release_levels(c);
+  for (DBuf* b : {&c.dinv64, &c.dinv32, &c.label1, &c.nodes_by_agg, &c.agg_ptr, &c.brick_agg1, &c.p0row, &c.p0col,
+                  &c.p0val, &c.p0t_ptr, &c.p0t_blk, &c.p0blk_row})
+    b->release();
+  c.p0blocks = 0;
+}
+
 void release_all(Ctx& c) {
   DevGeo& d = c.geo;
   for (DBuf* b : {&d.cbase, &d.nbase, &d.bstruct, &d.blist, &d.ce64, &d.cg64, &d.ce32, &d.cg32,
@@ -3005,6 +3015,7 @@ void set_beam_masks(Ctx& c, const uint16_t* beams) {
 
 // Runs setup + PCG for all uploaded structures. loads: device FP64 (c.f64), x0 in c.x64.
 void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
+  mem_tally().reset_peak();
   Solver S(c);
   const int ns = S.ns;
   hipEvent_t e0, e1, e2;
@@ -3091,13 +3102,16 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   // definiteness on nearly singular structures, such as a settle round where the structure
   // comes apart. A solve they fail is repeated once, from zero, with the robust settings:
   // two cycles on levels 3-6 and FP64 from level 3 (a rebuilt hierarchy).
-  std::vector<int> it_first(ns, 0);
+  std::vector<int> it_first(ns, 0), st_first(ns, ST_OK);
+  bool retried = false;
   if (S.ndof > 0 && !std::getenv("STATICS_HIP_NORETRY")) {
     std::vector<int> sts(ns);
     ST_HIP(hipMemcpy(sts.data(), S.sstatus.p, ns * sizeof(int), hipMemcpyDeviceToHost));
     const bool bad = std::any_of(sts.begin(), sts.end(), [](int v) { return v == ST_NOT_SPD || v == ST_NO_CONVERGENCE; });
     const std::vector<int> robust = {1, 1, 2, 2, 2, 2};
     if (bad && (c.gammas != robust || c.f64_from > 2)) {
+      retried = true;
+      st_first = sts;
       ST_HIP(hipMemcpy(it_first.data(), S.siters.p, ns * sizeof(int), hipMemcpyDeviceToHost));
       if (std::getenv("STATICS_HIP_VERBOSE"))
         std::fprintf(stderr, "statics_hip: solve failed with the fast cycle settings; again with the robust ones\n");
@@ -3152,7 +3166,12 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
     t.cells = si.ncells;
     t.bricks = si.nbricks;
     t.dof_nodes = s < (int)c.dof_per_struct.size() ? c.dof_per_struct[s] : 0;
+    t.retries = retried ? 1 : 0;
+    t.first_status = retried ? st_first[s] : ST_OK;
   }
+  if (o.release_hierarchy) release_hierarchy(c);
+  const int64_t peak = mem_tally().peak.load();
+  for (auto& t : stats) t.mem_peak = peak;
 }
 
 __global__ void k_gone_from_joined(int64_t n, const uint8_t* joined, uint8_t* nflags, int* count) {
@@ -3236,3 +3255,4 @@ void joined_to_held(Ctx& c, std::vector<uint8_t>& joined) {
 }
 
 }  // namespace st
+

@@ -14,6 +14,7 @@ void launch_apply(Ctx& c, const T* x, const Epi& epi);
 void refresh_cell_flags(Ctx& c);
 void release_all(Ctx& c);
 void release_levels(Ctx& c);
+void release_hierarchy(Ctx& c);
 void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats);
 void joined_to_held(Ctx& c, std::vector<uint8_t>& joined);
 void copy_geo_to_device(Ctx& c);

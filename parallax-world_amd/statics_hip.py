@@ -62,14 +62,15 @@ class StGrid(ctypes.Structure):
 
 class StSolveOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("tol", ctypes.c_double), ("max_iter", ctypes.c_int32),
-                ("refresh", ctypes.c_int32), ("warm_start", ctypes.c_int32)]
+                ("refresh", ctypes.c_int32), ("warm_start", ctypes.c_int32), ("release_hierarchy", ctypes.c_int32)]
 
 
 class StStats(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int32), ("levels", ctypes.c_int32), ("refreshes", ctypes.c_int32),
                 ("status", ctypes.c_int32), ("rel_residual", ctypes.c_double), ("setup_ms", ctypes.c_double),
                 ("solve_ms", ctypes.c_double), ("nodes", ctypes.c_int64), ("dof_nodes", ctypes.c_int64),
-                ("cells", ctypes.c_int64), ("bricks", ctypes.c_int64)]
+                ("cells", ctypes.c_int64), ("bricks", ctypes.c_int64), ("retries", ctypes.c_int32),
+                ("first_status", ctypes.c_int32), ("mem_peak", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -255,10 +256,11 @@ class Context:
         self._check(self.L.st_apply(self.h, _p(x), _p(y)), "st_apply")
         return y
 
-    def solve(self, loads, precision=FP64, tol=1e-8, max_iter=10000, refresh=0, x0=None, check=True):
+    def solve(self, loads, precision=FP64, tol=1e-8, max_iter=10000, refresh=0, x0=None, check=True,
+              release_hierarchy=False):
         loads = np.ascontiguousarray(loads, np.float64).reshape(self.nodes, 6)
         x = np.zeros_like(loads) if x0 is None else np.ascontiguousarray(x0, np.float64).copy()
-        o = StSolveOpts(precision, tol, max_iter, refresh, 0 if x0 is None else 1)
+        o = StSolveOpts(precision, tol, max_iter, refresh, 0 if x0 is None else 1, 1 if release_hierarchy else 0)
         s = StStats()
         rc = self.L.st_solve(self.h, _p(loads), ctypes.byref(o), _p(x), ctypes.byref(s))
         if check:
@@ -329,7 +331,7 @@ class Context:
         tol: the reference settles at 1e-6 (statics/src/bodies.rs:432); refresh -1: its stopping rule."""
         s = np.ascontiguousarray(strength, np.float64).reshape(-1, 3)
         loads = np.ascontiguousarray(loads, np.float64).reshape(self.nodes, 6)
-        o = StSolveOpts(precision, tol, max_iter, refresh, 1 if warm_start else 0)
+        o = StSolveOpts(precision, tol, max_iter, refresh, 1 if warm_start else 0, 0)
         gone = np.zeros(self.nodes, np.uint8)
         beams = np.zeros(len(self._keep[0]), np.uint16)
         rounds = ctypes.c_int32()
@@ -338,8 +340,9 @@ class Context:
                                      ctypes.byref(rounds), ctypes.byref(st)), "st_settle")
         return gone, beams, rounds.value, st
 
-    def backward(self, objective=COMPLIANCE, beta=20.0, precision=FP64, tol=1e-8, max_iter=10000):
-        o = StSolveOpts(precision, tol, max_iter, 0, 0)
+    def backward(self, objective=COMPLIANCE, beta=20.0, precision=FP64, tol=1e-8, max_iter=10000,
+                 release_hierarchy=False):
+        o = StSolveOpts(precision, tol, max_iter, 0, 0, 1 if release_hierarchy else 0)
         J = ctypes.c_double()
         d = np.zeros(len(self._keep[0]))
         s = StStats()
@@ -349,7 +352,7 @@ class Context:
 
     def objective(self, objective=COMPLIANCE, beta=20.0):
         """J of the last solve alone (no adjoint solve)."""
-        o = StSolveOpts(FP64, 1e-8, 1, 0, 0)
+        o = StSolveOpts(FP64, 1e-8, 1, 0, 0, 0)
         J = ctypes.c_double()
         self._check(self.L.st_backward(self.h, objective | VALUE_ONLY, beta, ctypes.byref(o), ctypes.byref(J),
                                        None, None), "st_backward")
@@ -388,7 +391,7 @@ class Context:
         stats = (StStats * n)()
         xs = [None] * n
         xptr = (ctypes.c_void_p * n)()
-        o = StSolveOpts(precision, tol, max_iter, 0, 0)
+        o = StSolveOpts(precision, tol, max_iter, 0, 0, 0)
         # node counts are known only after the upload: a first call without outputs would cost a solve,
         # so x comes back through a second, zero-iteration-free route: size by an upper bound of 6 faces per cell
         if want_x:

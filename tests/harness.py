@@ -78,3 +78,47 @@ def generated(ctx, n, seed, sigma=0.5, **kw):
     occ, theta = sh.generate(n, n, n, seed=seed, sigma=sigma, **kw)
     sites, held, e, g = sh.cells_from_occ(occ, theta)
     return Pair(ctx, sites, mats_from(e, g), held)
+
+
+def point_loads(P, gone, rng, n_points, gravity=True):
+    """The configurations' loads in the oracle's node order: gravity of weight 1 per free cell,
+    a sixth on each of its faces, along -z, plus n_points point loads of magnitude U(1, 10) in
+    uniformly random directions on the translations of distinct free nodes (not held, not
+    gone). The reference takes loads through motions' closure, only for free nodes
+    (statics/src/bodies.rs:429, 462)."""
+    L = P.L
+    f = np.zeros((L.n_nodes, 6))
+    if gravity:
+        f[:, 2] = -L.m / orc.UNIT ** 3
+    free = np.flatnonzero((L.held == 0) & (gone == 0))
+    pick = rng.choice(free, size=min(n_points, len(free)), replace=False)
+    d = rng.normal(size=(len(pick), 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    f[pick, :3] += d * rng.uniform(1.0, 10.0, size=(len(pick), 1))
+    return f
+
+
+def device_loads(ctx, occ, joined, rng, n_points):
+    """The same loads for a structure uploaded from occ (no oracle lattice), in the device's node
+    order: each face's weight is a sixth of each free cell beside it."""
+    faces = ctx.node_faces()
+    nheld = ctx.node_held().astype(bool)
+    o = faces[:, 0]
+    plus = faces[:, 1:4]
+    minus = plus - np.eye(3, dtype=np.int32)[o]
+    n = np.array(occ.shape)
+
+    def free_cell(c):
+        ok = np.all((c >= 0) & (c < n), axis=1)
+        cc = np.clip(c, 0, n - 1)
+        return ok & (occ[cc[:, 0], cc[:, 1], cc[:, 2]] == 1)
+
+    f = np.zeros((len(faces), 6))
+    f[:, 2] = -(free_cell(plus).astype(np.float64) + free_cell(minus)) / 6.0
+    free = np.flatnonzero(joined.astype(bool) & ~nheld)
+    pick = rng.choice(free, size=min(n_points, len(free)), replace=False)
+    d = rng.normal(size=(len(pick), 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    f[pick, :3] += d * rng.uniform(1.0, 10.0, size=(len(pick), 1))
+    f[nheld | ~joined.astype(bool)] = 0.0
+    return f

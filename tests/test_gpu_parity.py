@@ -101,6 +101,31 @@ def test_hierarchy_kept_and_dropped(ctx):
     assert s5.status == 0 and rel(x5, x4) < 1e-8 and s5.setup_ms > 2 * s2.setup_ms
 
 
+def test_hierarchy_recomputed(ctx):
+    """release_hierarchy: the solve frees its hierarchy, so the next solve and the adjoint build
+    it again (setup well above the kept solve's bookkeeping) and give the same results as with
+    the hierarchy kept (a rebuilt hierarchy agrees to rounding, so to the tolerance)."""
+    occ, theta = sh.generate(32, 32, 32, seed=6, sigma=0.5)
+    sites, held, e, g = sh.cells_from_occ(occ, theta)
+    ctx.upload(sites, held, e, g)
+    ctx.joined_to_held()
+    f = np.zeros((ctx.nodes, 6))
+    f[:, 2] = -1.0
+    f[ctx.node_held().astype(bool)] = 0
+    x1, s1 = ctx.solve(f, precision=sh.MIXED, tol=1e-10)
+    J1, d1, a1 = ctx.backward(sh.SOFTMAX_STRESS, beta=20.0, precision=sh.MIXED, tol=1e-10)
+    assert a1.setup_ms < 0.5 * s1.setup_ms  # kept: bookkeeping only
+    x2, s2 = ctx.solve(f, precision=sh.MIXED, tol=1e-10, release_hierarchy=True)
+    assert s2.setup_ms < 0.5 * s1.setup_ms  # this solve still found the kept one
+    J2, d2, a2 = ctx.backward(sh.SOFTMAX_STRESS, beta=20.0, precision=sh.MIXED, tol=1e-10, release_hierarchy=True)
+    assert a2.setup_ms > 2 * a1.setup_ms and a2.status == 0  # recomputed
+    x3, s3 = ctx.solve(f, precision=sh.MIXED, tol=1e-10)
+    assert s3.setup_ms > 2 * s2.setup_ms  # the adjoint released it again
+    assert rel(x2, x1) < 1e-12 and rel(x3, x1) < 1e-8
+    assert abs(J2 - J1) <= 1e-12 * abs(J1) and rel(d2, d1) < 1e-7
+    assert s1.mem_peak > 0 and s1.retries == 0
+
+
 def test_apply_parity_dead_beams_and_gone(ctx):
     P = generated(ctx, 16, 7)
     rng = np.random.default_rng(7)

@@ -52,3 +52,19 @@ def test_generator_rejects_bad_params():
     p = sh.StGenParams(0, 4, 4, 0, 8.0, 2, 6, 8, 16, 0.2, 0.0)
     occ = np.zeros(1, np.uint8)
     assert sh.lib().st_generate(ctypes.byref(p), sh._p(occ), None) == sh.ST_BAD_INPUT
+
+
+def test_struct_layouts_match_the_header():
+    """The binding's structures have the sizes and offsets the header states (and the
+    library asserts at compile time)."""
+    assert ctypes.sizeof(sh.StSolveOpts) == 32
+    assert [getattr(sh.StSolveOpts, f).offset for f in
+            ("precision", "tol", "max_iter", "refresh", "warm_start", "release_hierarchy")] == [0, 8, 16, 20, 24, 28]
+    assert ctypes.sizeof(sh.StStats) == 88
+    assert [getattr(sh.StStats, f).offset for f in
+            ("iters", "levels", "refreshes", "status", "rel_residual", "setup_ms", "solve_ms", "nodes", "dof_nodes",
+             "cells", "bricks", "retries", "first_status", "mem_peak")] == \
+        [0, 4, 8, 12, 16, 24, 32, 40, 48, 56, 64, 72, 76, 80]
+    with open(os.path.join(ROOT, "include", "statics_hip.h")) as f:
+        text = f.read()
+    assert "StSolveOpts is 32 bytes" in text and "StStats is 88 bytes" in text
