@@ -244,6 +244,19 @@ int st_set_loads(StCtx* ctx, const double* loads);
 int st_solve_resident(StCtx* ctx, const StSolveOpts* opts, StStats* stats);
 int st_get_solution(StCtx* ctx, double* x_out);
 
+/* No counterpart in the reference: st_joined_to_held (keep_gone = 1, the context's
+ * current beam masks) for callers that cannot allocate device memory, such as code under
+ * HIP graph capture or a tight serving loop. It allocates nothing, copies nothing and does
+ * not wait: it only enqueues kernels on `stream` (a hipStream_t; NULL: the context's own
+ * stream). joined_out: device memory, one byte per node (device order), 1 where a chain of
+ * alive beams joins the node to a held node; count_out: device memory for one int32, the
+ * number joined, or NULL. Its scratch (union-find parents and held-root flags) is a fixed
+ * device array of the library, 32 MiB on each GPU it runs on, held until the process
+ * ends, so structures of more than 4,194,304 nodes are refused (ST_BAD_INPUT). Every call
+ * on one GPU shares that array: calls that overlap on one GPU (from several contexts or
+ * streams) race, and the caller must order them. */
+int st_joined_to_held_static(StCtx* ctx, void* stream, uint8_t* joined_out, int32_t* count_out);
+
 /* Replaces: bodies::joined_to_held (statics/src/bodies.rs:557-574) when the
  * caller only needs `gone`: classifies on the device with the current beam
  * masks and sets gone = !joined without a host round trip. Writes the number

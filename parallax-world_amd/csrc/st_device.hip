@@ -293,6 +293,14 @@ int st_create(StCtx** out, int device) {
     delete c;
     return ST_DEVICE;
   }
+  // the fixed scratch of st_joined_to_held_static: its address on this device
+  try {
+    scratch_setup(device);
+  } catch (const StError& e) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return e.code;
+  }
   *out = c;
   return ST_OK;
 }

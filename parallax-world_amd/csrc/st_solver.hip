@@ -38,6 +38,7 @@
 #include <cstring>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 
 #include "st_solver.hpp"
 
@@ -3216,6 +3217,83 @@ void classify_device(Ctx& c, int64_t* joined_count, uint8_t* joined_host) {
   c.have_solution = false;
 }
 
+// ---- joined_to_held without allocating: the scratch lives in a fixed device array
+//
+// The union-find parents and the held-root flags of st_joined_to_held_static, for up to
+// kStaticNodes nodes, and one counter: 2 * 4 Mi + 64 int32 (32 MiB + 256 B), which each
+// GPU the library runs on holds until the process ends. Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device address): it
+// passes the address hipGetSymbolAddress gave for the device at st_create.
+namespace {
+constexpr int64_t kStaticNodes = int64_t(1) << 22;
+__device__ __align__(256) int scratch[2 * kStaticNodes + 64];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+int* scratch_dev[kMaxDevices] = {};
+
+// Sums joined[0..n) into *count, one atomic add per block.
+__global__ __launch_bounds__(256) void k_count_joined(int64_t n, const uint8_t* joined, int* count) {
+  __shared__ int part[256];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  part[threadIdx.x] = (i < n && joined[i]) ? 1 : 0;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && part[0]) atomicAdd(count, part[0]);
+}
+
+}  // namespace
+
+// The device address of the scratch array on `device`, which must be current. Called by
+// st_create (once per device; later calls find it resolved).
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) throw StError{ST_BAD_INPUT, "device index beyond 63"};
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_dev[device]) return;
+  void* ptr = nullptr;
+  ST_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) throw StError{ST_DEVICE, "hipGetSymbolAddress gave no address for the scratch array"};
+  scratch_dev[device] = static_cast<int*>(ptr);
+}
+
+// joined_to_held over the context's beam masks into joined (device memory, one byte per
+// node) and, if count is not NULL, the number joined into *count (device); the context's
+// gone set is left as it is. Launches only, on stream s: no allocation, no copy, no wait.
+void joined_static(Ctx& c, hipStream_t s, uint8_t* joined, int32_t* count) {
+  DevGeo& g = c.geo;
+  const int64_t nn = g.nnodes;
+  if (nn > kStaticNodes) {
+    char msg[160];
+    std::snprintf(msg, sizeof msg, "st_joined_to_held_static: %lld nodes, more than its scratch holds (%lld)",
+                  (long long)nn, (long long)kStaticNodes);
+    throw StError{ST_BAD_INPUT, msg};
+  }
+  int* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    base = (c.device >= 0 && c.device < kMaxDevices) ? scratch_dev[c.device] : nullptr;
+  }
+  if (!base) throw StError{ST_DEVICE, "st_joined_to_held_static: the scratch array was not set up on this device"};
+  int* uf = base;
+  int* flag = base + kStaticNodes;
+  int* cnt = count ? count : base + 2 * kStaticNodes;
+  k_fill_int<<<1, 64, 0, s>>>(1, cnt, 0);
+  if (nn) {
+    k_iota<<<nblk(nn), 256, 0, s>>>(nn, uf);
+    if (g.ncells)
+      k_unite_all<<<nblk(g.ncells), 256, 0, s>>>(g.ncells, g.cfaces.as<uint32_t>(), g.cmask_user.as<uint16_t>(), uf);
+    k_compress<<<nblk(nn), 256, 0, s>>>(nn, uf);
+    k_fill_int<<<nblk(nn), 256, 0, s>>>(nn, flag, 0);
+    k_held_roots<<<nblk(nn), 256, 0, s>>>(nn, uf, g.nflags.as<uint8_t>(), flag);
+    k_joined<<<nblk(nn), 256, 0, s>>>(nn, uf, flag, joined);
+    k_count_joined<<<nblk(nn), 256, 0, s>>>(nn, joined, cnt);
+  }
+  ST_HIP(hipGetLastError());
+}
+
 double device_dot_f_x(Ctx& c, const double* a, const double* b) {
   DevGeo& g = c.geo;
   if (!g.nbricks) return 0.0;
@@ -3256,3 +3334,13 @@ void joined_to_held(Ctx& c, std::vector<uint8_t>& joined) {
 
 }  // namespace st
 
+using namespace st;
+
+extern "C" int st_joined_to_held_static(StCtx* c, void* stream, uint8_t* joined_out, int32_t* count_out) {
+  return guard(c, [&] {
+    need_upload(c);
+    if (!joined_out) throw StError{ST_BAD_INPUT, "st_joined_to_held_static: joined_out is NULL"};
+    ST_HIP(hipSetDevice(c->device));
+    joined_static(*c, stream ? static_cast<hipStream_t>(stream) : c->stream, joined_out, count_out);
+  });
+}

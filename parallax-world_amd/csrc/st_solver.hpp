@@ -24,6 +24,8 @@ double device_dot_f_x(Ctx& c, const double* a, const double* b);
 void set_gone_flags(Ctx& c, const uint8_t* gone);
 void set_beam_masks(Ctx& c, const uint16_t* beams);
 void autotune_apply(Ctx& c);
+void scratch_setup(int device);
+void joined_static(Ctx& c, hipStream_t s, uint8_t* joined, int32_t* count);
 
 }  // namespace st
 

@@ -22,7 +22,8 @@ VALUE_ONLY = 256  # or-ed into the objective: J alone
 EXPORTS = ["st_create", "st_destroy", "st_last_error", "st_upload", "st_sizes", "st_node_faces", "st_lookup_faces",
            "st_node_held", "st_joined_to_held", "st_set_gone", "st_set_beams", "st_apply", "st_solve",
            "st_set_loads", "st_solve_resident", "st_get_solution", "st_classify", "st_kernel_timing_enable",
-           "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_bond_used", "st_settle", "st_grid_cells", "st_gather", "st_gather_read", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness"]
+           "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_bond_used", "st_settle", "st_grid_cells", "st_gather", "st_gather_read", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness",
+           "st_joined_to_held_static"]
 
 UNIT = 0.125  # metres to a level-0 cell (statics/src/lattice.rs:16)
 
@@ -124,6 +125,7 @@ def lib():
             "st_kernel_timing_enable": [vp, ctypes.c_int, ctypes.c_int],
             "st_kernel_timing": [vp, vp, vp, vp],
             "st_apply_schedule": [vp, ctypes.c_int, vp, vp, vp, vp],
+            "st_joined_to_held_static": [vp, vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
