@@ -1,20 +1,23 @@
 #!/usr/bin/env python3
 """Structure solves/sec to 1e-8 relative residual at 256^3 on one MI355X.
 
-Workload: one structure on a 256^3 lattice, ~30% solid: a held ground layer,
-square pillars 3-7 cells wide at Poisson-disk spacing 8, one-cell floor slabs
-every 8-16 cells with 20% 2x2 holes (seed 1); per-cell log-stiffness
-theta ~ N(0, 0.5^2) (e = exp(theta), g = 0.4 e); gravity of weight 1 per free
-cell (a sixth on each face, -z) plus 64 point loads of magnitude U(1, 10) in
-uniformly random directions on random free nodes. Synthetic, generated here.
+Workload: one structure on a 256^3 lattice, ~29% solid: a held ground layer,
+square pillars 2-6 cells wide at Poisson-disk spacing 7 (the configurations'
+rule; spacing 7 rather than 8 reaches their ~30% fill with 2-6 wide pillars),
+one-cell floor slabs every 8-16 cells with 20% 2x2 holes (seed 1); per-cell
+log-stiffness theta ~ N(0, 0.5^2) (e = exp(theta), g = 0.4 e); gravity of weight
+1 per free cell (a sixth on each face, -z) plus 64 point loads of magnitude
+U(1, 10) in uniformly random directions on random free nodes. Synthetic,
+generated here.
 
-One step = classify (joined to held -> gone) + mixed-precision solve to 1e-8 +
-the adjoint of the soft-max (beta = 20) bond stress w.r.t. per-cell
-log-stiffness (a second solve, mixed, 1e-8, hierarchy rebuilt). The timed
-steps run with the loads and the solution resident in device memory.
-Alongside: the plain FP64 solve (classify + FP64 solve to 1e-8), and the CPU
-oracle (the C++ restatement of the reference's solver, FP64) on a bounded
-sample of the same generator, scaled to 256^3 by its degrees of freedom.
+One step = classify (joined to held -> gone) + mixed-precision solve to 1e-8 that
+frees its multigrid hierarchy when done + the adjoint of the soft-max (beta = 20)
+bond stress w.r.t. per-cell log-stiffness (a second solve, mixed, 1e-8, which
+recomputes the hierarchy instead of reading a stored one, and frees it again).
+The timed steps run with the loads and the solution resident in device memory.
+Alongside: the same step in plain FP64 (like for like), the forward solves
+alone, the error of each result against a tight FP64 solve, and the CPU
+baseline.
 """
 import argparse
 import json
@@ -138,7 +141,7 @@ def measured_traffic(kernel):
     return None
 
 
-def structure(n, seed, sigma=0.5, spacing=8.0, wmin=3, wmax=7):
+def structure(n, seed, sigma=0.5, spacing=7.0, wmin=2, wmax=6):
     occ, theta = sh.generate(n, n, n, seed=seed, sigma=sigma, pillar_spacing=spacing, pillar_min=wmin,
                              pillar_max=wmax)
     return occ, theta
@@ -195,15 +198,16 @@ def cpu_baseline(size, threads, seed=1):
 
 def batch_structures(count, seed):
     """The batch workload: side L = 16 * 8^U (16..128), solid fraction about U(0.15, 0.40) reached
-    through the pillar spacing (widths 3-7; fill 0.32 at spacing 8, ~ 1/spacing^2), lognormal stiffness sigma 0.5."""
+    through the pillar spacing (widths 2-6; fill ~0.32 at spacing 6.5, ~ 1/spacing^2), lognormal stiffness
+    sigma 0.5."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
         side = int(16 * 8 ** rng.uniform(0.0, 1.0))
         phi = rng.uniform(0.15, 0.40)
-        spacing = 8.0 * np.sqrt(0.32 / phi)
+        spacing = 6.5 * np.sqrt(0.32 / phi)
         occ, theta = sh.generate(side, side, side, seed=seed * 100003 + i, sigma=0.5, pillar_spacing=spacing,
-                                 pillar_min=3, pillar_max=7)
+                                 pillar_min=2, pillar_max=6)
         out.append(sh.cells_from_occ(occ, theta))
     return out
 
@@ -612,12 +616,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--spacing", type=float, default=8.0,
-                    help="pillar Poisson-disk spacing (8: ~32%% solid; 11.7: ~15%%, the 512^3 sparse world)")
+    ap.add_argument("--spacing", type=float, default=7.0,
+                    help="pillar Poisson-disk spacing (7: ~29%% solid; 12.8: ~14%%, the 512^3 sparse world)")
     ap.add_argument("--points", type=int, default=64, help="point loads on top of gravity (1024: the 512^3 world)")
     ap.add_argument("--cpu-sample", type=int, default=64, help="edge of the CPU oracle's sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-errors", dest="errors", action="store_false",
+                    help="skip the error report (displacements and gradient against tight FP64 solves)")
     ap.add_argument("--batch", type=int, default=0,
                     help="instead: the batch workload, this many structures (512: the batch config)")
     ap.add_argument("--batch-cells", type=float, default=14e6,
@@ -686,46 +692,55 @@ def main():
     ctx._check(ctx.L.st_set_loads(ctx.h, sh._p(loads)), "st_set_loads")
 
     import ctypes
-    opts_mixed = sh.StSolveOpts(sh.MIXED, 1e-8, 20000, 0, 0)
-    opts_fp64 = sh.StSolveOpts(sh.FP64, 1e-8, 20000, 0, 0)
+    # the north star's mode: each solve frees its hierarchy, the adjoint recomputes it
+    opts_mixed = sh.StSolveOpts(sh.MIXED, 1e-8, 20000, 0, 0, 1)
+    opts_fp64 = sh.StSolveOpts(sh.FP64, 1e-8, 20000, 0, 0, 1)
 
     def classify():
         c = ctypes.c_int64()
         ctx._check(ctx.L.st_classify(ctx.h, ctypes.byref(c)), "st_classify")
         return c.value
 
-    def solve(opts):
+    def solve(opts, check=True):
         s = sh.StStats()
-        ctx._check(ctx.L.st_solve_resident(ctx.h, ctypes.byref(opts), ctypes.byref(s)), "st_solve_resident")
+        rc = ctx.L.st_solve_resident(ctx.h, ctypes.byref(opts), ctypes.byref(s))
+        if check or rc != sh.ST_NO_CONVERGENCE:
+            ctx._check(rc, "st_solve_resident")
         return s
 
-    def adjoint(opts):
+    def adjoint(opts, dJ=None):
         s = sh.StStats()
         J = ctypes.c_double()
-        ctx._check(ctx.L.st_backward(ctx.h, sh.SOFTMAX_STRESS, 20.0, ctypes.byref(opts), ctypes.byref(J), None,
-                                     ctypes.byref(s)), "st_backward")
+        ctx._check(ctx.L.st_backward(ctx.h, sh.SOFTMAX_STRESS, 20.0, ctypes.byref(opts), ctypes.byref(J),
+                                     None if dJ is None else sh._p(dJ), ctypes.byref(s)), "st_backward")
         return J.value, s
 
-    def step():
+    def step(opts):
         classify()
-        a = solve(opts_mixed)
-        J, b = adjoint(opts_mixed)
+        a = solve(opts)
+        J, b = adjoint(opts)
         return a, b, J
 
+    def solution():
+        x = np.zeros((ctx.nodes, 6))
+        ctx._check(ctx.L.st_get_solution(ctx.h, sh._p(x)), "st_get_solution")
+        return x
+
     for i in range(args.warmup):
-        a, b, J = step()
+        a, b, J = step(opts_mixed)
         log(f"[bench] warmup {i}: forward {a.iters} it ({a.setup_ms:.1f}+{a.solve_ms:.1f} ms, res {a.rel_residual:.2e}), "
             f"adjoint {b.iters} it ({b.setup_ms:.1f}+{b.solve_ms:.1f} ms, res {b.rel_residual:.2e})")
     ctx._check(ctx.L.st_kernel_timing_enable(ctx.h, 0, 1), "timing")
     times, fwd, adj = [], [], []
     for i in range(args.steps):
         t0 = time.perf_counter()
-        a, b, J = step()
+        a, b, J = step(opts_mixed)
         times.append(time.perf_counter() - t0)
         fwd.append(a)
         adj.append(b)
-        log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms; forward {a.iters} it, res {a.rel_residual:.2e}; "
-            f"adjoint {b.iters} it, res {b.rel_residual:.2e}; J {J:.6g}")
+        log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms; forward {a.iters} it ({a.setup_ms:.0f}+{a.solve_ms:.0f} ms), "
+            f"res {a.rel_residual:.2e}; adjoint {b.iters} it ({b.setup_ms:.0f}+{b.solve_ms:.0f} ms), "
+            f"res {b.rel_residual:.2e}; J {J:.6g}")
     nl, tms, bpl = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
     ctx._check(ctx.L.st_kernel_timing(ctx.h, ctypes.byref(nl), ctypes.byref(tms), ctypes.byref(bpl)), "timing")
     ctx._check(ctx.L.st_kernel_timing_enable(ctx.h, 0, 0), "timing")
@@ -733,32 +748,82 @@ def main():
     kernel_ms = tms.value / max(nl.value, 1)
     achieved = bpl.value / (kernel_ms * 1e-3) / 1e9 if nl.value else 0.0
     ms_per_step = 1e3 * sum(times) / len(times)
+    dof = 6 * fwd[-1].dof_nodes
 
-    # mixed forward alone and plain FP64 forward (classify + solve), after the timed steps
+    # the whole PCG iteration against the HBM roofline: the canonical 25 vector passes of
+    # the reference's algorithm per iteration on the fine level, 11 of them FP64 (the CG)
+    # and 14 FP32 (the V-cycle) in the mixed mode: 144 D bytes; 200 D in plain FP64
+    def iteration_roofline(stats_list, bytes_per_dof):
+        its = sum(s.iters for s in stats_list)
+        ms = sum(s.solve_ms for s in stats_list)
+        per_it = ms / max(its, 1)
+        gbs = bytes_per_dof * dof / (per_it * 1e-3) / 1e9
+        return {"bound": "hbm", "ms_per_iteration": per_it, "bytes_per_iteration": bytes_per_dof * dof,
+                "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+                "count": f"{bytes_per_dof} B per DOF per iteration (25 canonical vector passes), D = {dof}; "
+                         "coarse levels excluded; iteration time = solve time / iterations, forward and adjoint"}
+
+    rl_iter_mixed = iteration_roofline([fwd[-1], adj[-1]], 144)
+
+    # the same step in plain FP64, like for like (recomputed hierarchy in both solves)
+    fp64_step = None
+    if not args.no_fp64:
+        step(opts_fp64)  # warm
+        t0 = time.perf_counter()
+        a64, b64, J64 = step(opts_fp64)
+        fp64_step = dict(ms=(time.perf_counter() - t0) * 1e3, fwd_iters=a64.iters, adj_iters=b64.iters,
+                         fwd_setup_ms=a64.setup_ms, adj_setup_ms=b64.setup_ms, J=J64,
+                         iteration_roofline=iteration_roofline([a64, b64], 200))
+        log(f"[bench] fp64 step: {fp64_step['ms']:.1f} ms; forward {a64.iters} it, adjoint {b64.iters} it")
+
+    # the forward solves alone (classify + solve), and their errors against a tight FP64 solve
     t0 = time.perf_counter()
     classify()
     m1 = solve(opts_mixed)
     mixed_fwd_ms = (time.perf_counter() - t0) * 1e3
+    x_mixed = solution()
     fp64 = None
     if not args.no_fp64:
         t0 = time.perf_counter()
         classify()
         p = solve(opts_fp64)
         fp64 = dict(ms=(time.perf_counter() - t0) * 1e3, iters=p.iters, rel_residual=p.rel_residual)
+        x_fp64 = solution()
         log(f"[bench] fp64 forward: {fp64['ms']:.1f} ms, {p.iters} it, res {p.rel_residual:.2e}")
     log(f"[bench] mixed forward: {mixed_fwd_ms:.1f} ms, {m1.iters} it, res {m1.rel_residual:.2e}")
+    errors = None
+    if args.errors:
+        # reference: FP64 as far as its true residual goes in 400 iterations (it floors near 4e-10)
+        ref = solve(sh.StSolveOpts(sh.FP64, 1e-12, 400, 0, 0, 0), check=False)
+        x_ref = solution()
+        scale = np.abs(x_ref).max()
+        errors = {"reference": f"FP64 device solve, {ref.iters} iterations, true residual {ref.rel_residual:.2e}",
+                  "mixed_forward_max_rel": float(np.abs(x_mixed - x_ref).max() / scale)}
+        if fp64:
+            errors["fp64_forward_max_rel"] = float(np.abs(x_fp64 - x_ref).max() / scale)
+        del x_ref
+        # the soft-max gradient of the mixed step against FP64 solves to 1e-10
+        g_ref = np.zeros(len(sites))
+        solve(sh.StSolveOpts(sh.FP64, 1e-10, 2000, 0, 0, 0))
+        adjoint(sh.StSolveOpts(sh.FP64, 1e-10, 2000, 0, 0, 0), g_ref)
+        g_mix = np.zeros(len(sites))
+        solve(opts_mixed)
+        adjoint(opts_mixed, g_mix)
+        errors["mixed_softmax_gradient_max_rel"] = float(np.abs(g_mix - g_ref).max() / np.abs(g_ref).max())
+        errors["gradient_reference"] = "FP64 forward and adjoint solves to 1e-10"
+        log(f"[bench] errors: {errors}")
+    del x_mixed
 
     cpu = None
     if args.cpu_sample > 0:
         log(f"[bench] cpu oracle on a {args.cpu_sample}^3 sample, {args.cpu_threads} threads")
         cb = cpu_baseline(args.cpu_sample, args.cpu_threads, args.seed)
-        dof_gpu = 6 * fwd[-1].dof_nodes
-        per_s = (1.0 / cb["seconds"]) * (cb["dof"] / dof_gpu)
+        per_s = (1.0 / cb["seconds"]) * (cb["dof"] / dof)
         cpu = {"value": per_s, "unit": "structure solves/sec (256^3, extrapolated)", "cores": args.cpu_threads,
                "kind": "port",
                "sample": f"oracle (C++ FP64 restatement of statics/src/bodies.rs) classify+motions to 1e-8 on the "
                          f"{args.cpu_sample}^3 structure of the same generator: {cb['seconds']:.2f} s, {cb['iters']} "
-                         f"iterations, {cb['dof']} DOF; scaled by DOF to the 256^3 structure's {dof_gpu} DOF"}
+                         f"iterations, {cb['dof']} DOF; scaled by DOF to the 256^3 structure's {dof} DOF"}
         log(f"[bench] cpu sample: {cb}")
 
     value = 1e3 / ms_per_step
@@ -775,23 +840,32 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 CG + f32 V-cycle",
         "data": "synthetic",
-        "config": {"workload": f"one {n}^3 pillar/slab structure, {len(sites) / n ** 3:.0%} solid (pillar spacing "
-                               f"{args.spacing:g}), lognormal stiffness sigma 0.5, gravity + {args.points} point "
-                               f"loads; step = classify + mixed solve + soft-max stress adjoint",
+        "config": {"workload": f"one {n}^3 pillar/slab structure, {len(sites) / n ** 3:.0%} solid (pillars 2-6 wide "
+                               f"at Poisson spacing {args.spacing:g}), lognormal stiffness sigma 0.5, gravity + "
+                               f"{args.points} point loads; step = classify + mixed solve + soft-max stress adjoint, "
+                               f"each solve recomputing its multigrid hierarchy",
                    "lattice": n, "free_cells": int(fwd[-1].cells), "free_nodes": int(fwd[-1].dof_nodes),
-                   "dof": int(6 * fwd[-1].dof_nodes), "tol": 1e-8},
+                   "dof": int(dof), "tol": 1e-8, "hierarchy": "recomputed per solve (release_hierarchy)"},
         "forward_iters": int(fwd[-1].iters), "adjoint_iters": int(adj[-1].iters),
+        "forward_setup_ms": fwd[-1].setup_ms, "forward_solve_ms": fwd[-1].solve_ms,
+        "adjoint_setup_ms": adj[-1].setup_ms, "adjoint_solve_ms": adj[-1].solve_ms,
         "forward_rel_residual": fwd[-1].rel_residual, "adjoint_rel_residual": adj[-1].rel_residual,
+        "device_memory_peak_gb": max(fwd[-1].mem_peak, adj[-1].mem_peak) / 1e9,
+        "fp64_step_ms": fp64_step["ms"] if fp64_step else None,
+        "mixed_vs_fp64_step": (fp64_step["ms"] / ms_per_step) if fp64_step else None,
+        "fp64_step": fp64_step,
         "mixed_forward_solves_per_s": 1e3 / mixed_fwd_ms,
         "fp64_forward_solves_per_s": (1e3 / fp64["ms"]) if fp64 else None,
         "fp64_forward_iters": fp64["iters"] if fp64 else None,
         "mixed_vs_fp64_forward": (fp64["ms"] / mixed_fwd_ms) if fp64 else None,
+        "errors": errors,
         "roofline": {"bound": "hbm",
                      "kernel": f"{kname}<double, EpiDot<double>> (the CG's K p with the fused p.Ap, FP64; "
                                f"{kdesc}; {kdisp} dispatch(es) per launch)", "schedule": kidx,
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": measured_traffic(kname), "launches": int(nl.value), "avg_ms": kernel_ms,
                      "dispatches_per_launch": kdisp, "algorithmic_bytes_per_launch": bpl.value},
+        "roofline_iteration": rl_iter_mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

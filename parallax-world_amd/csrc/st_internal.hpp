@@ -160,6 +160,7 @@ struct Level {
   int64_t pblocks = 0;
   DBuf prow, pcol, pval64, pval32, pt_ptr, pt_blk, pblk_row;
   DBuf agg_struct;      // per aggregate: structure
+  bool wide32 = false, wide64 = false;  // the residual kernel's layout per precision (wide rows)
   std::vector<double> step;  // smoother step per structure
   DBuf step_d;               // per structure (double)
   // V-cycle vectors at this level

@@ -1008,33 +1008,50 @@ __device__ __forceinline__ void load_columns<double>(const double* xr, int c0, d
   v[0] = a.x; v[1] = a.y;
 }
 
-template <class T>
+// kWide = 0 ("narrow"): R rows per wave, one group of L lanes per row walking the row's
+// blocks in order. kWide = 1: one row per wave, its blocks dealt round-robin to the
+// wave's 64 / L groups (G), whose per-entry sums meet in LDS and are added in group
+// order: a short chain of loads per lane where a level has too few rows to fill the
+// GPU with narrow rows (its rows are then latency-bound chains of dependent loads).
+// The sums' order is fixed either way (per lane in block order, then lanes/groups in
+// order), so each variant is deterministic; they differ from each other by rounding.
+//
+// mode 0: out = r - A x, and (out2 not NULL) out2 = w D^-1 (r - A x), the damped Jacobi
+//         start of a cycle whose right-hand side is this residual;
+// mode 1: out = x + w D^-1 (r - A x) (+ add, when add is not NULL).
+template <class T, int kWide>
 __global__ __launch_bounds__(256) void k_c_resid(int mode, int64_t n, const uint32_t* ast, const uint8_t* done,
                                                  const int* rowptr, const int* col, const T* A, const T* dinv,
-                                                 const double* step, const T* r, const T* x, T* out) {
+                                                 const double* step, const T* r, const T* x, T* out,
+                                                 const T* add, T* out2) {
   using CL = CoarseLanes<T>;
-  constexpr int E = CL::E, L = CL::L, R = CL::R;
-  __shared__ T part[4][R][36];
+  constexpr int E = CL::E, L = CL::L;
+  constexpr int R = kWide ? 1 : CL::R;  // rows per wave
+  constexpr int G = kWide ? 64 / L : 1;  // lane groups per row
+  constexpr int NG = 64 / L;             // lane groups per wave
+  __shared__ T part[4][NG][36];
   __shared__ T vres[4][R][6];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int g = lane / L, i = lane - g * L;
+  const int grp = lane / L, i = lane - grp * L;
+  const int g = kWide ? 0 : grp;   // the row of the wave this group works
+  const int gi = kWide ? grp : 0;  // the group's place among the row's groups
   const int64_t I = ((int64_t)blockIdx.x * 4 + wv) * R + g;
-  const bool live = g < R && I < n && !(done && done[ast[I]]);
+  const bool live = grp < NG && g < R && I < n && !(done && done[ast[I]]);
   if (live) {
     T acc[E];
 #pragma unroll
     for (int k = 0; k < E; k++) acc[k] = T(0);
     const int c0 = (E * i) % 6;
-    int b = rowptr[I];
+    int b = rowptr[I] + gi;
     const int b1 = rowptr[I + 1];
-    for (; b + 4 <= b1; b += 4) {
+    for (; b + 3 * G < b1; b += 4 * G) {
       int cb[4];
       T av[4][E], xv[4][E];
 #pragma unroll
-      for (int q = 0; q < 4; q++) cb[q] = col[b + q];
+      for (int q = 0; q < 4; q++) cb[q] = col[b + q * G];
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        load_entries<T>(A + 36 * (int64_t)(b + q) + E * i, av[q]);
+        load_entries<T>(A + 36 * (int64_t)(b + q * G) + E * i, av[q]);
         load_columns<T>(x + 6 * (int64_t)cb[q], c0, xv[q]);
       }
 #pragma unroll
@@ -1042,7 +1059,7 @@ __global__ __launch_bounds__(256) void k_c_resid(int mode, int64_t n, const uint
 #pragma unroll
         for (int k = 0; k < E; k++) acc[k] += av[q][k] * xv[q][k];
     }
-    for (; b < b1; b++) {
+    for (; b < b1; b += G) {
       T av[E], xv[E];
       load_entries<T>(A + 36 * (int64_t)b + E * i, av);
       load_columns<T>(x + 6 * (int64_t)col[b], c0, xv);
@@ -1050,30 +1067,39 @@ __global__ __launch_bounds__(256) void k_c_resid(int mode, int64_t n, const uint
       for (int k = 0; k < E; k++) acc[k] += av[k] * xv[k];
     }
 #pragma unroll
-    for (int k = 0; k < E; k++) part[wv][g][E * i + k] = acc[k];
+    for (int k = 0; k < E; k++) part[wv][grp][E * i + k] = acc[k];
   }
   __syncthreads();
   T v = T(0);
-  if (live && i < 6) {
+  if (live && gi == 0 && i < 6) {
     T s = T(0);
 #pragma unroll
-    for (int c = 0; c < 6; c++) s += part[wv][g][6 * i + c];
+    for (int c = 0; c < 6; c++) {
+      T e = part[wv][g][6 * i + c];
+#pragma unroll
+      for (int h = 1; h < G; h++) e += part[wv][h][6 * i + c];
+      s += e;
+    }
     v = r[6 * I + i] - s;
     if (mode == 0) out[6 * I + i] = v;
     vres[wv][g][i] = v;
   }
-  if (mode == 0) return;
+  if (mode == 0 && !out2) return;
   __syncthreads();
-  if (live && i < 6) {
+  if (live && gi == 0 && i < 6) {
     T d = T(0);
 #pragma unroll
     for (int j = 0; j < 6; j++) d += dinv[36 * I + 6 * i + j] * vres[wv][g][j];
-    out[6 * I + i] = x[6 * I + i] + (T)step[ast[I]] * d;
+    d *= (T)step[ast[I]];
+    if (mode == 0)
+      out2[6 * I + i] = d;
+    else
+      out[6 * I + i] = x[6 * I + i] + d + (add ? add[6 * I + i] : T(0));
   }
 }
 template <class T>
-inline unsigned coarse_blocks(int64_t n) {
-  const int64_t rpb = CoarseLanes<T>::kRowsPerBlock;
+inline unsigned coarse_blocks(int64_t n, bool wide) {
+  const int64_t rpb = wide ? 4 : CoarseLanes<T>::kRowsPerBlock;
   return (unsigned)std::max<int64_t>(1, (n + rpb - 1) / rpb);
 }
 
@@ -1189,9 +1215,12 @@ __global__ __launch_bounds__(kBlock) void k_prolong1(const uint32_t* nbase, cons
 }
 
 // restrict level l -> l+1 over children lists
+// With x0 not NULL, also the damped Jacobi start of the coarse level's first cycle,
+// x0 = w D^-1 c (dinv, step: the coarse level's).
 template <class T>
 __global__ void k_restrict_c(int64_t n, const uint32_t* ast, const uint8_t* done, const int* kid_ptr, const int* kids,
-                             const double* centre, const double* centre_c, const T* t, T* c) {
+                             const double* centre, const double* centre_c, const T* t, T* c, const T* dinv,
+                             const double* step, T* x0) {
   const int64_t J = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (J >= n) return;
   if (done && done[ast[J]]) return;
@@ -1205,6 +1234,14 @@ __global__ void k_restrict_c(int64_t n, const uint32_t* ast, const uint8_t* done
     shiftT_add<T>(tv, r, acc);
   }
   store6<T>(c + 6 * J, acc);
+  if (x0) {
+    T d[6];
+    mat6v<T>(dinv + 36 * J, acc, d);
+    const T w = (T)step[ast[J]];
+#pragma unroll
+    for (int k = 0; k < 6; k++) d[k] *= w;
+    store6<T>(x0 + 6 * J, d);
+  }
 }
 
 // prolong level l+1 -> l: x[i] += Sh(r_i) y[parent[i]]
@@ -1833,18 +1870,21 @@ __global__ void k_blk_rows(int64_t n, const int* prow, int* blk_row) {
 }
 
 // V-cycle transfers with an explicit P. restrict: c[J] = sum over column J of P^T t.
+// With x0 not NULL, also x0 = w D^-1 c for the coarse level (as k_restrict_c).
 template <class T>
 __global__ void k_restrict_p(int64_t nJ, const uint32_t* ast, const uint8_t* done, const int* pt_ptr, const int* pt_blk,
-                             const int* pblk_row, const T* pval, const T* t, T* c) {
+                             const int* pblk_row, const T* pval, const T* t, T* c, const T* dinv, const double* step,
+                             T* x0) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t J = g >> 3;
   const int a = (int)(g & 7);
-  if (J >= nJ || a >= 6) return;
+  // the group of 8 lanes of one J stays whole (lanes 6 and 7 idle) for the shuffles below
+  if (J >= nJ) return;
   if (done && done[ast[J]]) return;
   // column a of each block of column J of P, four blocks' loads issued together
   T acc = 0;
-  int kk = pt_ptr[J];
-  const int k1 = pt_ptr[J + 1];
+  int kk = a < 6 ? pt_ptr[J] : 0;
+  const int k1 = a < 6 ? pt_ptr[J + 1] : 0;
   for (; kk + 4 <= k1; kk += 4) {
     int blk[4], row[4];
     T tv[4][6], mv[4][6];
@@ -1871,7 +1911,12 @@ __global__ void k_restrict_p(int64_t nJ, const uint32_t* ast, const uint8_t* don
 #pragma unroll
     for (int b = 0; b < 6; b++) acc += M[6 * b] * tv[b];
   }
-  c[6 * J + a] = acc;
+  if (a < 6) c[6 * J + a] = acc;
+  if (x0) {
+    T all[6];
+    group_gather6<T>(acc, all);
+    if (a < 6) x0[6 * J + a] = (T)step[ast[J]] * row6_dot<T>(dinv + 36 * J + 6 * a, all);
+  }
 }
 // prolong: x[i] += sum over row i of P y
 template <class T>
@@ -2499,6 +2544,51 @@ struct Solver {
     if (mixed)  // the FP32 levels' FP64 copies served the Galerkin products and the step estimates only
       for (size_t li = 0; li < c.levels.size() && (int)li < c.f64_from; li++)
         for (DBuf* b : {&c.levels[li].a64, &c.levels[li].dinv64, &c.levels[li].pval64}) b->release();
+    tune_coarse_layout();
+  }
+
+  // Per coarse level, the faster layout of its residual kernel (narrow or wide rows, see
+  // k_c_resid) in the precision the solve runs it in, timed on the level itself: three
+  // launches each after a warm one. STATICS_HIP_WIDE=0 or 1 forces one.
+  void tune_coarse_layout() {
+    const char* force = std::getenv("STATICS_HIP_WIDE");
+    hipEvent_t e0, e1;
+    ST_HIP(hipEventCreate(&e0));
+    ST_HIP(hipEventCreate(&e1));
+    for (size_t li = 0; li < c.levels.size(); li++) {
+      Level& lv = c.levels[li];
+      if (lv.dense || exact_level(li)) continue;
+      const bool f32 = mixed && (int)li < c.f64_from;
+      if (force) {
+        (f32 ? lv.wide32 : lv.wide64) = std::atoi(force) != 0;
+        continue;
+      }
+      float ms[2] = {0, 0};
+      for (int w = 0; w < 2; w++) {
+        (f32 ? lv.wide32 : lv.wide64) = w != 0;
+        auto run = [&] {
+          if (f32) {
+            ST_HIP(hipMemsetAsync(lv.x32.p, 0, lv.n * 6 * sizeof(float), st));
+            c_resid<float>(lv, 0, lv.r32.as<float>(), lv.x32.as<float>(), lv.t32.as<float>());
+          } else {
+            ST_HIP(hipMemsetAsync(lv.x64.p, 0, lv.n * 6 * sizeof(double), st));
+            c_resid<double>(lv, 0, lv.r64.as<double>(), lv.x64.as<double>(), lv.t64.as<double>());
+          }
+        };
+        run();
+        ST_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < 3; k++) run();
+        ST_HIP(hipEventRecord(e1, st));
+        ST_HIP(hipEventSynchronize(e1));
+        ST_HIP(hipEventElapsedTime(&ms[w], e0, e1));
+      }
+      (f32 ? lv.wide32 : lv.wide64) = ms[1] < ms[0];
+      if (std::getenv("STATICS_HIP_VERBOSE"))
+        std::fprintf(stderr, "statics_hip: level %zu residual layout %s (narrow %.1f us, wide %.1f us, %s)\n", li + 1,
+                     ms[1] < ms[0] ? "wide" : "narrow", 1e3 * ms[0] / 3, 1e3 * ms[1] / 3, f32 ? "fp32" : "fp64");
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   }
 
   // The first level from index 2 up where every structure has at most kDenseMax
@@ -2598,107 +2688,137 @@ struct Solver {
     return std::is_same<T, double>::value ? (const T*)lv.dinv64.p : (const T*)lv.dinv32.p;
   }
 
-  // The coarse cycle of level li for a V-cycle in precision T: levels from
-  // c.f64_from up run in FP64 whatever T is (their operators are the most
-  // ill-conditioned and the smallest), the right-hand side and the result
+  // Level li is solved exactly by one launch: its dense top, or a block-diagonal top.
+  bool exact_level(size_t li) const {
+    const Level& lv = c.levels[li];
+    return lv.dense || (li + 1 == c.levels.size() && lv.nblocks == lv.n);
+  }
+
+  // The coarse residual kernel of level lv in its tuned layout (narrow or wide rows).
+  // mode 0: out = r - A x (and out2 = w D^-1 (r - A x) when out2 is set);
+  // mode 1: out = x + w D^-1 (r - A x) (+ add).
+  template <class T>
+  void c_resid(Level& lv, int mode, const T* r, const T* x, T* out, const T* add = nullptr, T* out2 = nullptr) {
+    const bool wide = std::is_same<T, double>::value ? lv.wide64 : lv.wide32;
+    const int64_t n = lv.n;
+    const T* dinv = (mode == 1 || out2) ? lvD<T>(lv) : nullptr;
+    const double* step = (mode == 1 || out2) ? lv.step_d.as<double>() : nullptr;
+    if (wide)
+      k_c_resid<T, 1><<<coarse_blocks<T>(n, true), 256, 0, st>>>(mode, n, lv.agg_struct.as<uint32_t>(),
+                                                               c.done.as<uint8_t>(), lv.rowptr.as<int>(),
+                                                               lv.col.as<int>(), lvA<T>(lv), dinv, step, r, x, out,
+                                                               add, out2);
+    else
+      k_c_resid<T, 0><<<coarse_blocks<T>(n, false), 256, 0, st>>>(mode, n, lv.agg_struct.as<uint32_t>(),
+                                                                c.done.as<uint8_t>(), lv.rowptr.as<int>(),
+                                                                lv.col.as<int>(), lvA<T>(lv), dinv, step, r, x, out,
+                                                                add, out2);
+  }
+
+  // The coarse solve of level li for a V-cycle in precision T, on the right-hand side in
+  // the level's buffer 0; returns the buffer holding the result. x0_ready: whoever wrote
+  // the right-hand side also wrote the first cycle's damped Jacobi start w D^-1 r into
+  // buffer 1. Levels from c.f64_from up run in FP64 whatever T is (their operators are
+  // the most ill-conditioned and the smallest), the right-hand side and the result
   // converted at the switch.
   template <class T>
-  void descend(size_t li) {
+  T* descend(size_t li, bool x0_ready) {
     if (std::is_same<T, float>::value && (int)li >= c.f64_from) {
       Level& lv = c.levels[li];
       const int64_t m = lv.n * 6;
       k_convert<float, double><<<nblk(m), 256, 0, st>>>(m, lv.r32.as<float>(), lv.r64.as<double>());
-      coarse_cycle<double>(li);
-      k_convert<double, float><<<nblk(m), 256, 0, st>>>(m, lv.x64.as<double>(), lv.x32.as<float>());
-    } else {
-      coarse_cycle<T>(li);
+      const double* y = coarse_cycle<double>(li, false);
+      k_convert<double, float><<<nblk(m), 256, 0, st>>>(m, y, lv.x32.as<float>());
+      return (T*)lv.x32.p;
     }
+    return coarse_cycle<T>(li, x0_ready);
   }
 
-  // Solves A_li x = r approximately, r in lv.r, x into lv.x: one cycle, or (levels
-  // above c.gamma_from + 1) gamma of them, the later ones on the remaining residual
-  // (x += cycle(r - A x)), which keeps the preconditioner symmetric.
+  // Solves A_li x = r approximately: one cycle, or (levels with a gamma count above one)
+  // gamma of them, each later one on the remaining residual (x += cycle(r - A x)), which
+  // keeps the preconditioner symmetric. Buffers: 0 r, 1 each cycle's start, 3 each later
+  // cycle's right-hand side, 2 and 4 the results in turn.
   template <class T>
-  void coarse_cycle(size_t li) {
+  T* coarse_cycle(size_t li, bool x0_ready) {
     Level& lv = c.levels[li];
     const uint8_t* done = c.done.as<uint8_t>();
     T* r = lvbuf<T>(lv, 0);
     T* x = lvbuf<T>(lv, 1);
     const int64_t n = lv.n;
     const uint32_t* ast = lv.agg_struct.as<uint32_t>();
-    const bool top = li + 1 == c.levels.size();
     if (lv.dense) {  // the dense top: exact
       const dim3 gd((unsigned)((lv.dense_mmax + 31) / 32), (unsigned)ns);
       k_dense_apply<T><<<gd, 1024, 0, st>>>(lv.dense_seg.as<int64_t>(), lv.dense_off.as<int64_t>(),
                                           lv.dense_a.as<double>(), done, r, x);
-      return;
+      return x;
     }
-    if (top && lv.nblocks == lv.n) {  // block diagonal: exact
+    if (exact_level(li)) {  // block diagonal: exact
       k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), nullptr, r, x);
-      return;
+      return x;
     }
-    one_cycle<T>(li, r, x);
+    if (!x0_ready) k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), r, x);
+    T* res = one_cycle<T>(li, r, x, lvbuf<T>(lv, 2), nullptr);
     const int reps = !c.gammas.empty() ? (li < c.gammas.size() ? c.gammas[li] : 1)
                                        : (((int)li > c.gamma_from && (int)li <= c.gamma_to) ? c.gamma : 1);
-    T* t = lvbuf<T>(lv, 2);
-    T* v = lvbuf<T>(lv, 4);
+    T* rk = lvbuf<T>(lv, 3);
     for (int k = 1; k < reps; k++) {
-      k_c_resid<T><<<coarse_blocks<T>(n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
-                                               nullptr, nullptr, r, x, t);
-      k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(0), x, v);  // v = x
-      one_cycle<T>(li, t, x);
-      k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(1), v, x);  // x += v
+      c_resid<T>(lv, 0, r, res, rk, nullptr, x);  // rk = r - A res; x = w D^-1 rk
+      T* out = res == lvbuf<T>(lv, 2) ? lvbuf<T>(lv, 4) : lvbuf<T>(lv, 2);
+      res = one_cycle<T>(li, rk, x, out, res);  // res + cycle(rk)
     }
+    return res;
   }
 
-  // One cycle at level li: x = cycle(rhs); uses lv.u as scratch (and lv.t for smoothing).
+  // One cycle at level li on rhs from the damped Jacobi start x (updated in place): the
+  // residual, restricted; the coarser solve, prolonged onto x; the post-smoothing
+  // x + w D^-1 (rhs - A x) (+ add) into out, which also serves as the residual's buffer
+  // (out differs from rhs, x and add). Returns out.
   template <class T>
-  void one_cycle(size_t li, const T* rhs, T* x) {
+  T* one_cycle(size_t li, const T* rhs, T* x, T* out, const T* add) {
     Level& lv = c.levels[li];
     const uint8_t* done = c.done.as<uint8_t>();
     const int64_t n = lv.n;
     const uint32_t* ast = lv.agg_struct.as<uint32_t>();
-    T* u = lvbuf<T>(lv, 3);
     if (li + 1 == c.levels.size()) {  // not block diagonal (should not happen): smoothing sweeps
-      k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), rhs, x);
       for (int k = 0; k < 8; k++) {
-        k_c_resid<T><<<coarse_blocks<T>(n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
-                                                 lvD<T>(lv), lv.step_d.as<double>(), rhs, x, u);
-        k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(0), u, x);
+        c_resid<T>(lv, 1, rhs, x, out);
+        k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(0), out, x);
       }
-      return;
+      c_resid<T>(lv, 1, rhs, x, out, add);
+      return out;
     }
     Level& up = c.levels[li + 1];
     const int nu = li == 0 ? c.nu1 : c.nu_c;
-    // x <- x + w D^-1 (rhs - A x), k times
-    auto smooth = [&](int k) {
-      for (int i = 0; i < k; i++) {
-        k_c_resid<T><<<coarse_blocks<T>(n), 256, 0, st>>>(1, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
-                                                 lvD<T>(lv), lv.step_d.as<double>(), rhs, x, u);
-        k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(0), u, x);
-      }
-    };
-    k_c_jacobi<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lvD<T>(lv), lv.step_d.as<double>(), rhs, x);
-    smooth(nu - 1);
+    for (int i = 1; i < nu; i++) {  // further smoothing steps x <- x + w D^-1 (rhs - A x)
+      c_resid<T>(lv, 1, rhs, x, out);
+      k_axpy<T><<<nblk(6 * n), 256, 0, st>>>(6 * n, T(0), out, x);
+    }
     const T* pv = std::is_same<T, double>::value ? (const T*)lv.pval64.p : (const T*)lv.pval32.p;
-    k_c_resid<T><<<coarse_blocks<T>(n), 256, 0, st>>>(0, n, ast, done, lv.rowptr.as<int>(), lv.col.as<int>(), lvA<T>(lv),
-                                             nullptr, nullptr, rhs, x, u);
+    T* u = out;
+    c_resid<T>(lv, 0, rhs, x, u);
+    // the coarser level's first cycle starts from w D^-1 of its right-hand side: written
+    // with it, unless that level is solved exactly or switches precision
+    const bool fuse = !exact_level(li + 1) && !(std::is_same<T, float>::value && (int)(li + 1) >= c.f64_from);
+    T* up_r = lvbuf<T>(up, 0);
+    T* up_x0 = fuse ? lvbuf<T>(up, 1) : nullptr;
+    const T* up_d = fuse ? lvD<T>(up) : nullptr;
+    const double* up_step = fuse ? up.step_d.as<double>() : nullptr;
     if (lv.pblocks)
       k_restrict_p<T><<<nblk(8 * up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(),
-                                                     lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv, u,
-                                                     lvbuf<T>(up, 0));
+                                                     lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv, u, up_r, up_d,
+                                                     up_step, up_x0);
     else
       k_restrict_c<T><<<nblk(up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, up.kid_ptr.as<int>(),
                                                   up.kids.as<int>(), up.centre.as<double>(), lv.centre.as<double>(), u,
-                                                  lvbuf<T>(up, 0));
-    descend<T>(li + 1);
+                                                  up_r, up_d, up_step, up_x0);
+    const T* y = descend<T>(li + 1, fuse);
     if (lv.pblocks)
-      k_prolong_p<T><<<nblk(8 * n), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.pcol.as<int>(), pv,
-                                                 lvbuf<T>(up, 1), x);
+      k_prolong_p<T><<<nblk(8 * n), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.pcol.as<int>(), pv, y, x);
     else
       k_prolong_c<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lv.parent.as<int>(), lv.centre.as<double>(),
-                                              up.centre.as<double>(), lvbuf<T>(up, 1), x);
-    smooth(nu);
+                                              up.centre.as<double>(), y, x);
+    c_resid<T>(lv, 1, rhs, x, out, add);
+    return out;
   }
 
   // u = b - K x into u, and its rigid restriction P_rigid^T u into level 1's right-hand side:
@@ -2770,10 +2890,10 @@ struct Solver {
       launch_apply<T>(c, x1, ej);
       Level& l1 = c.levels[0];
       residual_restrict<T>(s1, r1, r1, l1);
-      descend<T>(0);
+      const T* y1 = descend<T>(0, false);
       k_prolong1_set<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done,
                                                c.label1.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
-                                               g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), lvbuf<T>(l1, 1), s1);
+                                               g.nkind.as<uint8_t>(), g.bgeo.as<BeamGeo>(), y1, s1);
       EpiProlong<T> ep;
       ep.done = done;
       ep.dinv = dinv;
@@ -2783,10 +2903,10 @@ struct Solver {
     } else if (!c.levels.empty()) {
       Level& l1 = c.levels[0];
       residual_restrict<T>(x1, r, r1, l1);
-      descend<T>(0);
+      const T* y1 = descend<T>(0, false);
       k_prolong1<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done, c.label1.as<int>(),
                                            l1.centre.as<double>(), g.nkey.as<uint32_t>(), g.nkind.as<uint8_t>(),
-                                           g.bgeo.as<BeamGeo>(), lvbuf<T>(l1, 1), x1);
+                                           g.bgeo.as<BeamGeo>(), y1, x1);
     }
     smooth_steps(c.nu - 1);
     EpiSmooth<T> es;
