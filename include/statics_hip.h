@@ -183,6 +183,33 @@ typedef struct {
  * nnodes_out and nbonds_out, st_gather_read copies it out in StLattice's layout, in the
  * reference's order (nodes by face, bonds cell by cell, PER_BEAM to a beam). */
 int st_gather(StCtx* ctx, const StLattice* fine, int against_full, int64_t* nnodes_out, int64_t* nbonds_out);
+
+/* The cells a lattice was built from, for st_upload. */
+typedef struct {
+  int64_t n_cells;        /* out: cells (those with beams first, then held ones) */
+  double spacing;         /* out: UNIT * 2^level */
+  int32_t* sites;         /* 3 per cell, level units; NULL on the sizing call */
+  uint8_t* held;          /* 1 per cell, or NULL */
+  double* e;              /* 1 per cell, or NULL (0 for held cells) */
+  double* g;              /* 1 per cell, or NULL */
+  double* rho;            /* 1 per cell, or NULL */
+  uint16_t* beams;        /* 1 per cell, or NULL: the beams the lattice holds (bit q, pair q) */
+  double* strength;       /* 45 per cell, or NULL: ft, fc, fs of each beam q (0 where absent) */
+  int64_t* beam_of_bond;  /* 1 per bond, or NULL: 15 * cell + q of the bond's beam */
+} StLatticeCells;
+
+/* Replaces: the caller's side of lattice::build (statics/src/lattice.rs:124-163) run
+ * backwards, for a caller that holds a Lattice (statics/src/lattice.rs:61-71) rather than
+ * its cells: each beam's cell is the site its end nodes share (as statics/src/gather.rs:245
+ * finds it), its pair the two faces of that cell its nodes are; a held node's other cell is
+ * held (lattice.rs:138). Host only; no context. Takes a lattice that build could have made
+ * at its level, strengths per beam allowed (bodies::as_generated, statics/src/bodies.rs:
+ * 611-625): every beam four bonds in lattice::beam's order and geometry (lattice.rs:76-84)
+ * of section SECTION s^2 between face centres, one e and g to a cell, one material to a
+ * beam. Anything else (a gathered lattice, whose beams have sections of their own,
+ * statics/src/gather.rs:386-390) is ST_BAD_INPUT, with the reason in message. Call once
+ * with out->sites NULL for n_cells, then with arrays of that size. */
+int st_lattice_cells(const StLattice* lat, StLatticeCells* out, char* message, int64_t message_len);
 int st_gather_read(StCtx* ctx, uint32_t* level, double* x, double* m, double* node_mat, uint8_t* held,
                    int64_t* around, int64_t* ij, double* p, double* n, double* a, double* d, double* bond_mat);
 
@@ -205,6 +232,12 @@ int st_node_held(StCtx* ctx, uint8_t* held_out);
  * The result also becomes the context's `gone` (gone = !joined) unless
  * keep_gone is nonzero. */
 int st_joined_to_held(StCtx* ctx, const uint16_t* beams_alive, uint8_t* joined_out, int keep_gone);
+/* Replaces: bodies::joined_to_held (statics/src/bodies.rs:557-574) with the reference's own
+ * arguments, a lattice's nodes and bonds (no upload needed): per node, 1 if a chain of alive
+ * bonds joins it to a held node. held: 1 per node; ij: 2 per bond (the bond's nodes i, j);
+ * alive: 1 per bond, or NULL (all alive). Bit-exact (reachability is order-free). */
+int st_joined_edges(StCtx* ctx, int64_t nnodes, const uint8_t* held, int64_t nbonds, const int64_t* ij,
+                    const uint8_t* alive, uint8_t* joined_out);
 /* Sets the nodes taken out of the solve (the `gone` argument of motions,
  * statics/src/bodies.rs:429); NULL clears it. Beams touching a gone node are
  * not live (statics/src/bodies.rs:434). */
@@ -293,16 +326,19 @@ int st_bond_forces(StCtx* ctx, const double* x, double* forces_out);
  * bodies::moved, with Bond::used and Mat::used (statics/src/model.rs:123-129,
  * 66-68): per cell (in upload order) 15 beams x 4 bonds, the share of the
  * cell's strengths that the bond's force under motions x uses (> 1: the bond
- * breaks); 0 for dead beams and held cells. strength: 3 doubles per cell, the
- * material's ft, fc, fs (model.rs:43-58). x: 6 per node. */
-int st_bond_used(StCtx* ctx, const double* strength, const double* x, double* used_out);
+ * breaks); 0 for dead beams and held cells. strength: per_beam = 0, 3 doubles per
+ * cell, the material's ft, fc, fs (model.rs:43-58); per_beam != 0, 45 per cell, each
+ * beam's own (beam q at 3 q), as lattices whose beams were made stronger one by one
+ * hold them (bodies::as_generated, statics/src/bodies.rs:611-625; st_lattice_cells
+ * gives them). x: 6 per node. */
+int st_bond_used(StCtx* ctx, const double* strength, int per_beam, const double* x, double* used_out);
 
 /* Replaces: bodies::settle (statics/src/bodies.rs:511-541) on the device. From
  * the context's beam masks (all alive after st_upload with beams NULL, as the
  * reference starts), rounds of: gone = !joined_to_held (bodies.rs:518-519); the
  * solve under `loads` (the reference's forces: gravity, bodies.rs:407-409,
  * given here as 6 doubles per node in device order); every live beam one of
- * whose bonds has used > 1 (strength as st_bond_used) dies whole
+ * whose bonds has used > 1 (strength and per_beam as st_bond_used) dies whole
  * (bodies.rs:523-533); until a round breaks nothing. opts->warm_start != 0 starts
  * each round after the first from the previous round's solution (the reference
  * starts from 0; the converged solution is the same). The reference settles at
@@ -317,7 +353,7 @@ int st_bond_used(StCtx* ctx, const double* strength, const double* x, double* us
  * repeated with the robust cycle settings; stats->first_status is ST_NO_CONVERGENCE
  * when some round under the reference's rule (refresh < 0) ended with a true residual
  * above tol (its rel_residual is the worst such round's), else ST_OK. */
-int st_settle(StCtx* ctx, const double* strength, const double* loads, const StSolveOpts* opts, uint8_t* gone_out,
+int st_settle(StCtx* ctx, const double* strength, int per_beam, const double* loads, const StSolveOpts* opts, uint8_t* gone_out,
               uint16_t* beams_out, int32_t* rounds_out, StStats* stats);
 
 /* ---- gradients (no counterpart in the reference) ---------------------- */

@@ -86,6 +86,7 @@ def lib():
         L.orc_used_standing.argtypes = [vp, ctypes.c_double, vp]
         L.orc_settle.argtypes = [vp, ctypes.c_double, vp, vp, vp, vp]
         L.orc_scale_cell.argtypes = [vp, ctypes.c_int64, ctypes.c_double]
+        L.orc_as_generated.argtypes = [vp, vp, ctypes.c_double]
         L.orc_invert6.argtypes = [vp, vp]
         L.orc_coarse_site.argtypes = [vp, ctypes.c_uint32, vp]
         L.orc_spacing.restype = ctypes.c_double
@@ -249,6 +250,12 @@ class Lattice:
         out = np.zeros(self.n_bonds)
         lib().orc_energy(self.h, _p(np.ascontiguousarray(f, np.float64)), _p(out))
         return out
+
+    def as_generated(self, generated, tol=1e-6):
+        """bodies::as_generated (statics/src/bodies.rs:604-625): each beam's bonds made MARGIN times
+        as strong as generated's same beam needs standing (tol: used_standing's solve)."""
+        lib().orc_as_generated(self.h, generated.h, tol)
+        self._read()
 
     def scale_cell(self, cell, by):
         lib().orc_scale_cell(self.h, int(cell), float(by))

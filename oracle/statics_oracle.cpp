@@ -8,7 +8,7 @@
 //   statics/src/bodies.rs   shift, each, times*, Level::{new, of, most_asked, apply,
 //                           jacobi, ask, coarser}, merge, Cholesky, conjugate_gradients,
 //                           cycle, forces, moved, motions, bodies_of, settle, used,
-//                           used_standing, joined_to_held, invert, find
+//                           used_standing, joined_to_held, invert, find, as_generated
 //   statics/src/gather.rs   Cell::{handle, pieces}, responses, beams, solve, full,
 //                           measured, gathered (and Bond::energy, model.rs:132-136)
 // Rust semantics mirrored: no contraction of a * b + c (built with -ffp-contract=off);
@@ -932,6 +932,36 @@ std::vector<double> used_standing(const Lattice& lat, double tol) {
   return used(lat, forces(lat, alive, gone, tol, st));
 }
 
+// bodies.rs:604-625: strength no less than what `generated` needs under its own weight,
+// MARGIN times, beam by beam (a beam named by its end nodes' sites, lesser first).
+constexpr double MARGIN = 2.0;
+void as_generated(Lattice& lattice, const Lattice& generated, double tol) {
+  using Key = std::array<Around, 2>;
+  auto beam = [](const Lattice& l, const Bond& b) {
+    const Around& i = l.around[b.i];
+    const Around& j = l.around[b.j];
+    return i < j ? Key{i, j} : Key{j, i};
+  };
+  std::map<Key, double> need;
+  std::vector<double> u = used_standing(generated, tol);
+  for (size_t b = 0; b < generated.bonds.size(); b++) {
+    auto it = need.emplace(beam(generated, generated.bonds[b]), 0.0).first;
+    it->second = std::max(it->second, u[b]);
+  }
+  std::vector<double> by(lattice.bonds.size());
+  for (size_t b = 0; b < lattice.bonds.size(); b++) {
+    auto it = need.find(beam(lattice, lattice.bonds[b]));
+    by[b] = it == need.end() ? 1.0 : std::max(it->second * MARGIN, 1.0);
+  }
+  // Mat::stronger (model.rs:71-73)
+  for (size_t b = 0; b < lattice.bonds.size(); b++) {
+    Mat& m = lattice.bonds[b].mat;
+    m.ft *= by[b];
+    m.fc *= by[b];
+    m.fs *= by[b];
+  }
+}
+
 // ---------------------------------------------------------------- gather.rs
 // A coarser level measured from a finer one. Materials are told apart by their
 // six values here; the reference tells them apart by name, which is the same for
@@ -1545,6 +1575,10 @@ void orc_settle(const OrcLattice* h, double tol, uint8_t* gone_out, double* fall
 }
 
 // Scales the stiffness (e and g) of every bond of cell `cell` by `by` (a log-stiffness step of log(by)).
+void orc_as_generated(OrcLattice* h, const OrcLattice* generated, double tol) {
+  as_generated(*reinterpret_cast<Lattice*>(h), *reinterpret_cast<const Lattice*>(generated), tol);
+}
+
 void orc_scale_cell(OrcLattice* h, int64_t cell, double by) {
   auto* L = reinterpret_cast<Lattice*>(h);
   for (size_t q = 0; q < L->beam_cell.size(); q++)

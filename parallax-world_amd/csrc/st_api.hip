@@ -139,7 +139,7 @@ __global__ void k_bond_forces(int64_t ncells, const uint32_t* cfaces, const uint
 }
 
 // Per bond used share (bodies.rs:544-546): out[c][q][k], zero for dead beams.
-// strength: ft, fc, fs per cell.
+// strength: ft, fc, fs per beam, 45 per cell (beam q at 3 q).
 __global__ void k_bond_used(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, const double* ce,
                             const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* strength,
                             const double* x, double* out) {
@@ -149,9 +149,10 @@ __global__ void k_bond_used(int64_t ncells, const uint32_t* cfaces, const uint16
   load_faces(cfaces, c, x, xv);
   const BeamGeo G = bgeo[cstruct[c]];
   const double e = ce[c], g = cg[c], a = bond_area(G);
-  const double ft = strength[3 * c], fc = strength[3 * c + 1], fs = strength[3 * c + 2];
   const uint32_t m = cmask[c];
   for (int q = 0; q < 15; q++) {
+    const double* sq = strength + 45 * c + 3 * q;
+    const double ft = sq[0], fc = sq[1], fs = sq[2];
     double* o = out + ((size_t)c * 15 + q) * 4;
     if (!(m & (1u << q))) {
       for (int k = 0; k < 4; k++) o[k] = 0.0;
@@ -179,10 +180,11 @@ __global__ void k_break(int64_t ncells, const uint32_t* cfaces, const uint16_t* 
   load_faces(cfaces, c, x, xv);
   const BeamGeo G = bgeo[cstruct[c]];
   const double e = ce[c], g = cg[c], a = bond_area(G);
-  const double ft = strength[3 * c], fc = strength[3 * c + 1], fs = strength[3 * c + 2];
   uint32_t dead = 0;
   for (int q = 0; q < 15; q++) {
     if (!(m & (1u << q))) continue;
+    const double* sq = strength + 45 * c + 3 * q;
+    const double ft = sq[0], fc = sq[1], fs = sq[2];
     BeamState b;
     beam_state(G, q, b);
     double f[4][3];
@@ -401,23 +403,32 @@ std::vector<StStats> solve_resident(StCtx* c, const StSolveOpts& o) {
   return out;
 }
 
-// ft, fc, fs per device cell from 3 per input cell (held cells have no beams and are skipped).
-void upload_strength(StCtx* c, const double* strength, DBuf& out) {
+// ft, fc, fs of every beam of every device cell (45 per cell) from the caller's: 3 per
+// input cell (per_beam = 0, the cell's material) or 45 (per_beam != 0: beam q's at 3 q,
+// as st_lattice_cells gives them for lattices whose beams are of their own strengths).
+// Held cells have no beams and are skipped; so are beams a cell does not hold.
+void upload_strength(StCtx* c, const double* strength, int per_beam, DBuf& out) {
   const auto& coi = c->host.cell_of_input;
   const int64_t ncl = c->geo.ncells;
-  std::vector<double> h((size_t)std::max<int64_t>(ncl, 1) * 3, 1.0);
+  const int stride = per_beam ? 45 : 3;
+  std::vector<double> h((size_t)std::max<int64_t>(ncl, 1) * 45, 1.0);
   for (size_t i = 0; i < coi.size(); i++) {
     const int64_t k = coi[i];
     if (k < 0) continue;
-    for (int t = 0; t < 3; t++) {
-      const double v = strength[3 * i + t];
-      if (!(v > 0) || !std::isfinite(v))
-        throw StError{ST_BAD_INPUT, "strength: a free cell has a non-positive or non-finite ft, fc or fs"};
-      h[3 * (size_t)k + t] = v;
-    }
+    const uint32_t held_beams = c->host.cmask[k];
+    for (int q = 0; q < 15; q++)
+      for (int t = 0; t < 3; t++) {
+        const double v = strength[stride * i + (per_beam ? 3 * q : 0) + t];
+        if (!(v > 0) || !std::isfinite(v)) {
+          if (per_beam && !(held_beams & (1u << q))) continue;  // a beam the cell does not hold
+          throw StError{ST_BAD_INPUT, "strength: a free cell has a non-positive or non-finite ft, fc or fs"};
+        }
+        h[45 * (size_t)k + 3 * q + t] = v;
+      }
   }
   out.ensure(h.size() * sizeof(double));
   ST_HIP(hipMemcpyAsync(out.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  ST_HIP(hipStreamSynchronize(c->stream));
 }
 }  // namespace
 }  // namespace st
@@ -589,14 +600,14 @@ int st_bond_forces(StCtx* c, const double* x, double* out) {
 }
 
 
-int st_bond_used(StCtx* c, const double* strength, const double* x, double* out) {
+int st_bond_used(StCtx* c, const double* strength, int per_beam, const double* x, double* out) {
   return guard(c, [&] {
     need_upload(c);
     if (!strength || !x || !out) throw StError{ST_BAD_INPUT, "bond used: strength, x or out is NULL"};
     ST_HIP(hipSetDevice(c->device));
     const int64_t nn = c->geo.nnodes, ncl = c->geo.ncells;
     DBuf sd, xd, od;
-    upload_strength(c, strength, sd);
+    upload_strength(c, strength, per_beam, sd);
     xd.ensure(std::max<int64_t>(nn, 1) * 6 * sizeof(double));
     od.ensure(std::max<int64_t>(ncl, 1) * 60 * sizeof(double));
     if (nn) ST_HIP(hipMemcpyAsync(xd.p, x, nn * 6 * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -619,7 +630,7 @@ int st_bond_used(StCtx* c, const double* strength, const double* x, double* out)
   });
 }
 
-int st_settle(StCtx* c, const double* strength, const double* loads, const StSolveOpts* opts, uint8_t* gone_out,
+int st_settle(StCtx* c, const double* strength, int per_beam, const double* loads, const StSolveOpts* opts, uint8_t* gone_out,
               uint16_t* beams_out, int32_t* rounds_out, StStats* stats) {
   std::vector<StStats> st_out;
   int rc = guard(c, [&] {
@@ -630,7 +641,7 @@ int st_settle(StCtx* c, const double* strength, const double* loads, const StSol
     DevGeo& g = c->geo;
     const int64_t nn = g.nnodes, ncl = g.ncells;
     DBuf sd, broken;
-    upload_strength(c, strength, sd);
+    upload_strength(c, strength, per_beam, sd);
     set_loads(c, loads);
     broken.ensure(sizeof(int));
     StSolveOpts o = *opts;

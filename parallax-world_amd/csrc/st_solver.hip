@@ -3452,9 +3452,60 @@ void joined_to_held(Ctx& c, std::vector<uint8_t>& joined) {
   out.release();
 }
 
+// Unites the ends of every alive edge of a general graph (a lattice's bonds).
+__global__ void k_unite_edges(int64_t ne, const int64_t* ij, const uint8_t* alive, int* uf) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ne || (alive && !alive[b])) return;
+  uf_unite(uf, (int)ij[2 * b], (int)ij[2 * b + 1]);
+}
+
+// bodies::joined_to_held (statics/src/bodies.rs:557-574) over a lattice's nodes and bonds as
+// given: reachability from held nodes over the alive bonds, by union-find on the device.
+void joined_edges(Ctx& c, int64_t nn, const uint8_t* held, int64_t ne, const int64_t* ij, const uint8_t* alive,
+                  uint8_t* joined) {
+  if (nn <= 0) return;
+  if (nn >= (int64_t(1) << 31)) throw StError{ST_BAD_INPUT, "joined_edges: more nodes than int32 indexes"};
+  for (int64_t b = 0; b < ne; b++)
+    if (ij[2 * b] < 0 || ij[2 * b] >= nn || ij[2 * b + 1] < 0 || ij[2 * b + 1] >= nn)
+      throw StError{ST_BAD_INPUT, "joined_edges: a bond names a missing node"};
+  hipStream_t st = c.stream;
+  DBuf uf, flag, hd, ijd, ad, out;
+  uf.ensure(nn * sizeof(int));
+  flag.ensure(nn * sizeof(int));
+  hd.ensure(nn);
+  out.ensure(nn);
+  ijd.ensure(std::max<int64_t>(ne, 1) * 2 * sizeof(int64_t));
+  if (alive) ad.ensure(std::max<int64_t>(ne, 1));
+  for (int64_t v = 0; v < nn; v++)
+    if (held[v] > 1) throw StError{ST_BAD_INPUT, "joined_edges: held is 0 or 1 per node"};
+  ST_HIP(hipMemcpyAsync(hd.p, held, nn, hipMemcpyHostToDevice, st));
+  if (ne) ST_HIP(hipMemcpyAsync(ijd.p, ij, ne * 2 * sizeof(int64_t), hipMemcpyHostToDevice, st));
+  if (alive && ne) ST_HIP(hipMemcpyAsync(ad.p, alive, ne, hipMemcpyHostToDevice, st));
+  k_iota<<<nblk(nn), 256, 0, st>>>(nn, uf.as<int>());
+  if (ne)
+    k_unite_edges<<<nblk(ne), 256, 0, st>>>(ne, ijd.as<int64_t>(), alive ? ad.as<uint8_t>() : nullptr, uf.as<int>());
+  k_compress<<<nblk(nn), 256, 0, st>>>(nn, uf.as<int>());
+  k_fill_int<<<nblk(nn), 256, 0, st>>>(nn, flag.as<int>(), 0);
+  k_held_roots<<<nblk(nn), 256, 0, st>>>(nn, uf.as<int>(), hd.as<uint8_t>(), flag.as<int>());
+  k_joined<<<nblk(nn), 256, 0, st>>>(nn, uf.as<int>(), flag.as<int>(), out.as<uint8_t>());
+  ST_HIP(hipGetLastError());
+  ST_HIP(hipMemcpyAsync(joined, out.p, nn, hipMemcpyDeviceToHost, st));
+  ST_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace st
 
 using namespace st;
+
+extern "C" int st_joined_edges(StCtx* c, int64_t nnodes, const uint8_t* held, int64_t nbonds, const int64_t* ij,
+                               const uint8_t* alive, uint8_t* joined_out) {
+  return guard(c, [&] {
+    if (nnodes < 0 || nbonds < 0 || (nnodes && (!held || !joined_out)) || (nbonds && !ij))
+      throw StError{ST_BAD_INPUT, "st_joined_edges: bad arguments"};
+    ST_HIP(hipSetDevice(c->device));
+    joined_edges(*c, nnodes, held, nbonds, ij, alive, joined_out);
+  });
+}
 
 extern "C" int st_joined_to_held_static(StCtx* c, void* stream, uint8_t* joined_out, int32_t* count_out) {
   return guard(c, [&] {

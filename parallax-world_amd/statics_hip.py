@@ -23,7 +23,7 @@ EXPORTS = ["st_create", "st_destroy", "st_last_error", "st_upload", "st_sizes", 
            "st_node_held", "st_joined_to_held", "st_set_gone", "st_set_beams", "st_apply", "st_solve",
            "st_set_loads", "st_solve_resident", "st_get_solution", "st_classify", "st_kernel_timing_enable",
            "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_bond_used", "st_settle", "st_grid_cells", "st_gather", "st_gather_read", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness",
-           "st_joined_to_held_static"]
+           "st_joined_to_held_static", "st_lattice_cells", "st_joined_edges"]
 
 UNIT = 0.125  # metres to a level-0 cell (statics/src/lattice.rs:16)
 
@@ -59,6 +59,12 @@ class StGrid(ctypes.Structure):
     _fields_ = [("min", ctypes.c_int32 * 3), ("n", ctypes.c_int32 * 3), ("bricks", ctypes.c_void_p),
                 ("nbricks", ctypes.c_int64), ("solid", ctypes.c_void_p), ("nsites", ctypes.c_int64),
                 ("solid_bytes", ctypes.c_void_p), ("solid_min", ctypes.c_uint8)]
+
+
+class StLatticeCells(ctypes.Structure):
+    _fields_ = [("n_cells", ctypes.c_int64), ("spacing", ctypes.c_double), ("sites", ctypes.c_void_p),
+                ("held", ctypes.c_void_p), ("e", ctypes.c_void_p), ("g", ctypes.c_void_p), ("rho", ctypes.c_void_p),
+                ("beams", ctypes.c_void_p), ("strength", ctypes.c_void_p), ("beam_of_bond", ctypes.c_void_p)]
 
 
 class StSolveOpts(ctypes.Structure):
@@ -110,11 +116,13 @@ def lib():
             "st_apply": [vp, vp, vp],
             "st_solve": [vp, vp, ctypes.POINTER(StSolveOpts), vp, ctypes.POINTER(StStats)],
             "st_bond_forces": [vp, vp, vp],
-            "st_bond_used": [vp, vp, vp, vp],
+            "st_bond_used": [vp, vp, ctypes.c_int, vp, vp],
+            "st_lattice_cells": [ctypes.POINTER(StLattice), ctypes.POINTER(StLatticeCells), ctypes.c_char_p,
+                                 ctypes.c_int64],
             "st_grid_cells": [vp, ctypes.POINTER(StGrid), ctypes.c_int32, dbl, vp, vp, vp, vp],
             "st_gather": [vp, ctypes.POINTER(StLattice), ctypes.c_int, vp, vp],
             "st_gather_read": [vp] * 13,
-            "st_settle": [vp, vp, vp, ctypes.POINTER(StSolveOpts), vp, vp, vp, ctypes.POINTER(StStats)],
+            "st_settle": [vp, vp, ctypes.c_int, vp, ctypes.POINTER(StSolveOpts), vp, vp, vp, ctypes.POINTER(StStats)],
             "st_backward": [vp, ctypes.c_int, dbl, ctypes.POINTER(StSolveOpts), vp, vp, ctypes.POINTER(StStats)],
             "st_solve_batch": [vp, i32, vp, vp, ctypes.POINTER(StSolveOpts), vp, vp],
             "st_generate": [ctypes.POINTER(StGenParams), vp, vp],
@@ -126,6 +134,7 @@ def lib():
             "st_kernel_timing": [vp, vp, vp, vp],
             "st_apply_schedule": [vp, ctypes.c_int, vp, vp, vp, vp],
             "st_joined_to_held_static": [vp, vp, vp, vp],
+            "st_joined_edges": [vp, ctypes.c_int64, vp, ctypes.c_int64, vp, vp, vp],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -175,6 +184,48 @@ def gravity_loads(faces, held_or_gone, count_cells, weight=1.0, axis=2):
     f[:, axis] = -weight * count_cells / 6.0
     f[held_or_gone.astype(bool)] = 0.0
     return f
+
+
+def _strength(strength, ncells):
+    a = np.ascontiguousarray(strength, np.float64)
+    if a.size == ncells * 45:
+        return a.reshape(ncells, 45), 1
+    return a.reshape(ncells, 3), 0
+
+
+def lattice_arrays(lat):
+    """A lattice's arrays as StLattice wants them, and the StLattice (which points into them)."""
+    arr = {k: np.ascontiguousarray(getattr(lat, k), t) for k, t in
+           [("x", np.float64), ("m", np.float64), ("node_mat", np.float64), ("held", np.uint8),
+            ("around", np.int64), ("ij", np.int64), ("p", np.float64), ("n", np.float64), ("a", np.float64),
+            ("d", np.float64), ("bond_mat", np.float64)]}
+    st = StLattice(int(lat.level), len(arr["m"]), _p(arr["x"]), _p(arr["m"]), _p(arr["node_mat"]), _p(arr["held"]),
+                   _p(arr["around"]), len(arr["a"]), _p(arr["ij"]), _p(arr["p"]), _p(arr["n"]), _p(arr["a"]),
+                   _p(arr["d"]), _p(arr["bond_mat"]))
+    return arr, st
+
+
+def lattice_cells(lat):
+    """st_lattice_cells: the cells a built lattice comes from, as a dict of arrays (sites, held, e,
+    g, rho, beams, strength (cells, 15, 3), beam_of_bond) and spacing. Raises StaticsError with the
+    library's reason for a lattice build could not have made (a gathered one, say)."""
+    arr, st = lattice_arrays(lat)
+    out = StLatticeCells()
+    msg = ctypes.create_string_buffer(512)
+    rc = lib().st_lattice_cells(ctypes.byref(st), ctypes.byref(out), msg, 512)
+    if rc != 0:
+        raise StaticsError(rc, msg.value.decode())
+    n = out.n_cells
+    res = {"sites": np.zeros((n, 3), np.int32), "held": np.zeros(n, np.uint8), "e": np.zeros(n), "g": np.zeros(n),
+           "rho": np.zeros(n), "beams": np.zeros(n, np.uint16), "strength": np.zeros((n, 15, 3)),
+           "beam_of_bond": np.zeros(len(arr["a"]), np.int64)}
+    for k in ("sites", "held", "e", "g", "rho", "beams", "strength", "beam_of_bond"):
+        setattr(out, k, res[k].ctypes.data)
+    rc = lib().st_lattice_cells(ctypes.byref(st), ctypes.byref(out), msg, 512)
+    if rc != 0:
+        raise StaticsError(rc, msg.value.decode())
+    res["spacing"] = out.spacing
+    return res
 
 
 class Context:
@@ -244,6 +295,16 @@ class Context:
         self._check(self.L.st_joined_to_held(self.h, _p(b), _p(j), 1 if keep_gone else 0), "st_joined_to_held")
         return j
 
+    def joined_edges(self, held, ij, alive=None):
+        """bodies::joined_to_held(nodes, bonds, alive) over a lattice's own nodes and bonds."""
+        held = np.ascontiguousarray(held, np.uint8)
+        ij = np.ascontiguousarray(ij, np.int64).reshape(-1, 2)
+        a = None if alive is None else np.ascontiguousarray(alive, np.uint8)
+        out = np.zeros(len(held), np.uint8)
+        self._check(self.L.st_joined_edges(self.h, len(held), _p(held), len(ij), _p(ij), _p(a), _p(out)),
+                    "st_joined_edges")
+        return out
+
     def set_gone(self, gone):
         g = None if gone is None else np.ascontiguousarray(gone, np.uint8)
         self._check(self.L.st_set_gone(self.h, _p(g)), "st_set_gone")
@@ -302,13 +363,7 @@ class Context:
     def gather(self, fine, against_full=True):
         """gather::gathered (against_full) or measured(fine, false) (statics/src/gather.rs:193-395) of a
         lattice with the attributes GatheredLattice names: the next level up, as a GatheredLattice."""
-        arr = {k: np.ascontiguousarray(getattr(fine, k), t) for k, t in
-               [("x", np.float64), ("m", np.float64), ("node_mat", np.float64), ("held", np.uint8),
-                ("around", np.int64), ("ij", np.int64), ("p", np.float64), ("n", np.float64), ("a", np.float64),
-                ("d", np.float64), ("bond_mat", np.float64)]}
-        lat = StLattice(int(fine.level), len(arr["m"]), _p(arr["x"]), _p(arr["m"]), _p(arr["node_mat"]),
-                        _p(arr["held"]), _p(arr["around"]), len(arr["a"]), _p(arr["ij"]), _p(arr["p"]),
-                        _p(arr["n"]), _p(arr["a"]), _p(arr["d"]), _p(arr["bond_mat"]))
+        arr, lat = lattice_arrays(fine)
         nn, nb = ctypes.c_int64(), ctypes.c_int64()
         self._check(self.L.st_gather(self.h, ctypes.byref(lat), 1 if against_full else 0, ctypes.byref(nn),
                                      ctypes.byref(nb)), "st_gather")
@@ -321,24 +376,25 @@ class Context:
         return out
 
     def bond_used(self, strength, x):
-        """Per uploaded cell [15][4] used shares; strength: (cells, 3) ft, fc, fs."""
+        """Per uploaded cell [15][4] used shares; strength: (cells, 3) ft, fc, fs per cell, or
+        (cells, 15, 3) per beam."""
         x = np.ascontiguousarray(x, np.float64).reshape(self.nodes, 6)
-        s = np.ascontiguousarray(strength, np.float64).reshape(-1, 3)
+        s, per_beam = _strength(strength, len(self._keep[0]))
         out = np.zeros((len(self._keep[0]), 15, 4))
-        self._check(self.L.st_bond_used(self.h, _p(s), _p(x), _p(out)), "st_bond_used")
+        self._check(self.L.st_bond_used(self.h, _p(s), per_beam, _p(x), _p(out)), "st_bond_used")
         return out
 
     def settle(self, strength, loads, precision=FP64, tol=1e-6, max_iter=10000, warm_start=False, refresh=0):
         """bodies::settle on the device: (gone per node, final beam masks per cell, rounds, stats).
         tol: the reference settles at 1e-6 (statics/src/bodies.rs:432); refresh -1: its stopping rule."""
-        s = np.ascontiguousarray(strength, np.float64).reshape(-1, 3)
+        s, per_beam = _strength(strength, len(self._keep[0]))
         loads = np.ascontiguousarray(loads, np.float64).reshape(self.nodes, 6)
         o = StSolveOpts(precision, tol, max_iter, refresh, 1 if warm_start else 0, 0)
         gone = np.zeros(self.nodes, np.uint8)
         beams = np.zeros(len(self._keep[0]), np.uint16)
         rounds = ctypes.c_int32()
         st = StStats()
-        self._check(self.L.st_settle(self.h, _p(s), _p(loads), ctypes.byref(o), _p(gone), _p(beams),
+        self._check(self.L.st_settle(self.h, _p(s), per_beam, _p(loads), ctypes.byref(o), _p(gone), _p(beams),
                                      ctypes.byref(rounds), ctypes.byref(st)), "st_settle")
         return gone, beams, rounds.value, st
 
