@@ -65,9 +65,14 @@ typedef struct {
 /* ST_FP64: everything in FP64. ST_MIXED: the CG (vectors, K p, inner products,
  * step lengths) stays in FP64 and the V-cycle preconditioner runs in FP32 on the
  * fine level and the first four coarse levels (their operators, transfers and
- * smoothers), FP64 above and in the dense top. Both recompute the FP64 true residual when the
- * recursive one meets the tolerance, and go on from it if it does not. */
-typedef enum { ST_FP64 = 0, ST_MIXED = 1 } StPrecision;
+ * smoothers), FP64 above and in the dense top. ST_MIXED32: the V-cycle as ST_MIXED, and the
+ * CG's vectors and its K p in FP32 too, with FP64 kept for the inner products, the step
+ * lengths, the solution (the FP32 steps are added into it in FP64) and the residual: each
+ * time the FP32 recursive residual has fallen a hundredfold, and when it meets the
+ * tolerance, the FP64 true residual f - K x replaces it (iterative refinement by residual
+ * replacement). All recompute the FP64 true residual when the recursive one meets the
+ * tolerance, and go on from it if it does not. */
+typedef enum { ST_FP64 = 0, ST_MIXED = 1, ST_MIXED32 = 2 } StPrecision;
 
 typedef struct {
   int32_t precision;   /* StPrecision */
@@ -96,10 +101,12 @@ typedef struct {
   int64_t dof_nodes;      /* free nodes: not held, not gone */
   int64_t cells;          /* free cells */
   int64_t bricks;         /* bricks on the device */
-  int32_t retries;        /* 1 if the solve was repeated from zero with the robust cycle
-                             settings (two cycles on coarse levels 3-6, FP64 from level 3)
-                             after the fast ones failed; 0 otherwise */
-  int32_t first_status;   /* with retries = 1: the StStatus the fast settings ended with */
+  int32_t retries;        /* restarts, 0 to 2: under ST_MIXED32, the FP64 CG taking over from
+                             an FP32 CG that stopped short (from its solution); and the
+                             solve repeated from zero with the robust cycle settings (two
+                             cycles on coarse levels 3-6, FP64 from level 3) after the fast
+                             ones failed */
+  int32_t first_status;   /* with retries > 0: the StStatus the first attempt ended with */
   int64_t mem_peak;       /* the most device memory (bytes) the library's buffers held in
                              this process during the call */
 } StStats;
@@ -298,8 +305,8 @@ int st_classify(StCtx* ctx, int64_t* joined_count);
 
 /* Device time of the launches of one kernel, for measuring it against its
  * roofline: enable != 0 starts recording (and clears the totals) for kernel
- * `which` (0: the fine operator K p of the CG with its fused p.Ap, FP64 in both
- * precision modes); enable == 0 stops. st_kernel_timing reads the totals so far:
+ * `which` (0: the fine operator K p of the CG with its fused p.Ap: FP64 under ST_FP64
+ * and ST_MIXED, FP32 under ST_MIXED32); enable == 0 stops. st_kernel_timing reads the totals so far:
  * launches, summed milliseconds, and the algorithmic bytes of one launch
  * (6 values per free node read and written, plus e, g, the beam mask and the
  * six face indices per free cell). */

@@ -350,7 +350,8 @@ __global__ void k_mask_rows2(int64_t n, const uint8_t* nflags, double* v) {
 
 void check_opts(const StSolveOpts* o) {
   if (!o) throw StError{ST_BAD_INPUT, "opts is NULL"};
-  if (o->precision != ST_FP64 && o->precision != ST_MIXED) throw StError{ST_BAD_INPUT, "opts.precision"};
+  if (o->precision != ST_FP64 && o->precision != ST_MIXED && o->precision != ST_MIXED32)
+    throw StError{ST_BAD_INPUT, "opts.precision"};
   if (!(o->tol >= 0) || o->max_iter < 0) throw StError{ST_BAD_INPUT, "opts: tol or max_iter"};
 }
 

@@ -54,6 +54,7 @@ void copy_geo_to_device(Ctx& c) {
   d.nstruct = (int32_t)h.structs.size();
   c.tuned = false;
   c.hier_valid = false;
+  c.coarse_layouts.clear();
   {
     // bricks in two classes by node count (Morton order kept inside each)
     constexpr uint32_t kSmall = 640;
@@ -142,7 +143,8 @@ void timing_collect(Ctx& c) {
 template <class T, class Epi>
 void launch_apply(Ctx& c, const T* x, const Epi& epi) {
   if (!c.geo.nbricks) return;
-  const bool timed = std::is_same<Epi, EpiDot<T>>::value && sizeof(T) == 8 && c.timing_which == 0;
+  // the CG's K p: FP64, or FP32 under ST_MIXED32
+  const bool timed = std::is_same<Epi, EpiDot<T>>::value && c.timing_which == 0;
   if (timed) {
     if (c.tev_used * 2 + 2 > c.tev.size()) {
       if (c.tev.size() >= 4096) timing_collect(c);

@@ -160,7 +160,7 @@ struct Level {
   int64_t pblocks = 0;
   DBuf prow, pcol, pval64, pval32, pt_ptr, pt_blk, pblk_row;
   DBuf agg_struct;      // per aggregate: structure
-  bool wide32 = false, wide64 = false;  // the residual kernel's layout per precision (wide rows)
+  int layout32 = 0, layout64 = 0;  // the residual kernel's tuned layout per precision (see k_c_resid)
   std::vector<double> step;  // smoother step per structure
   DBuf step_d;               // per structure (double)
   // V-cycle vectors at this level
@@ -220,6 +220,9 @@ struct Ctx {
   std::map<const void*, size_t> lds_attr;  // per kernel: the dynamic LDS limit this context set
   bool tuned = false;           // apply_sched chosen for the uploaded structures
   std::vector<Level> levels;        // coarse levels 1..
+  // the coarse residual layout chosen per (level index, FP32, aggregates, blocks), kept
+  // for the hierarchies rebuilt on this upload (cleared by st_upload)
+  std::vector<std::pair<std::array<int64_t, 4>, int>> coarse_layouts;
   // The hierarchy (fine smoother and coarse levels) is kept between solves of the same
   // operator: anything that changes the operator clears hier_valid; hier_key holds the
   // settings it was built with.

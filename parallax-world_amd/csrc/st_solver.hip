@@ -432,6 +432,20 @@ __global__ __launch_bounds__(kBlock) void k_copy_fine(const uint32_t* nbase, con
 }
 
 
+__global__ __launch_bounds__(kBlock) void k_zero_fine(const uint32_t* nbase, float* v) {
+  const uint32_t b = blockIdx.x;
+  for (uint32_t i = 6 * nbase[b] + threadIdx.x; i < 6 * nbase[b + 1]; i += blockDim.x) v[i] = 0.0f;
+}
+
+// x += (double) dx and dx = 0, for every structure (the FP32 CG's steps into the FP64 solution)
+__global__ __launch_bounds__(kBlock) void k_flush_fine(const uint32_t* nbase, float* dx, double* x) {
+  const uint32_t b = blockIdx.x;
+  for (uint32_t i = 6 * nbase[b] + threadIdx.x; i < 6 * nbase[b + 1]; i += blockDim.x) {
+    x[i] += (double)dx[i];
+    dx[i] = 0.0f;
+  }
+}
+
 // zero rows of held and gone nodes (a load on them is not read)
 __global__ void k_mask_rows(int64_t n, const uint8_t* nflags, double* v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -525,8 +539,11 @@ __global__ void k_cg_beta(int ns, double* scal, uint8_t* done, int* status) {
 // After a true residual: it is the residual from now on. A structure whose true
 // residual has not fallen by a tenth over kStall replacements stops (the
 // precision's attainable accuracy is reached) and is reported as not converged.
+// (stall_limit and improve, the squared fall that counts, are kStall and 0.81 for the FP64
+// CG; the FP32 CG gives up sooner, after 3 replacements gaining less than half each.)
 constexpr int kStall = 8;
-__global__ void k_cg_true(int ns, double* scal, uint8_t* done, int* status, uint8_t* want, int* refreshes) {
+__global__ void k_cg_true(int ns, double* scal, uint8_t* done, int* status, uint8_t* want, int* refreshes,
+                          int stall_limit, double improve) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns || done[s]) return;
   double* S = scal + kScal * s;
@@ -541,16 +558,37 @@ __global__ void k_cg_true(int ns, double* scal, uint8_t* done, int* status, uint
   } else if (rt <= S[S_GOAL2]) {
     done[s] = 1;
   } else {
-    if (rt < 0.81 * S[S_BEST]) {
+    if (rt < improve * S[S_BEST]) {
       S[S_BEST] = rt;
       S[S_STALL] = 0.0;
     } else {
       S[S_STALL] += 1.0;
-      if (S[S_STALL] >= kStall) {
+      if (S[S_STALL] >= stall_limit) {
         status[s] = ST_NO_CONVERGENCE;
         done[s] = 1;
       }
     }
+  }
+}
+
+// ST_MIXED32 under the reference's rule: a structure whose recursive residual met the
+// tolerance is done; the others take the true residual as their residual and go on.
+__global__ void k_cg_true_rec(int ns, double* scal, uint8_t* done, int* status, uint8_t* want, int* refreshes) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns || done[s]) return;
+  double* S = scal + kScal * s;
+  refreshes[s] += 1;
+  const bool met = S[S_RR] <= S[S_GOAL2];
+  const double rt = S[S_RRTRUE];
+  want[s] = 0;
+  if (!isfinite(rt)) {
+    status[s] = ST_NAN;
+    done[s] = 1;
+  } else if (met) {
+    done[s] = 1;  // S_RR keeps the recursive residual the rule judged
+  } else {
+    S[S_RR] = rt;
+    S[S_RRLAST] = rt;
   }
 }
 
@@ -993,6 +1031,22 @@ __device__ __forceinline__ void load_entries<double>(const double* p, double* v)
   const double2 a = *reinterpret_cast<const double2*>(p);
   v[0] = a.x; v[1] = a.y;
 }
+// The same loads marked non-temporal (the blocks stream through once per product; the
+// gathered x entries are what the caches should keep).
+typedef float st_f4 __attribute__((ext_vector_type(4)));
+typedef double st_d2 __attribute__((ext_vector_type(2)));
+template <class T>
+__device__ __forceinline__ void load_entries_nt(const T* p, T* v);
+template <>
+__device__ __forceinline__ void load_entries_nt<float>(const float* p, float* v) {
+  const st_f4 a = __builtin_nontemporal_load(reinterpret_cast<const st_f4*>(p));
+  v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+template <>
+__device__ __forceinline__ void load_entries_nt<double>(const double* p, double* v) {
+  const st_d2 a = __builtin_nontemporal_load(reinterpret_cast<const st_d2*>(p));
+  v[0] = a.x; v[1] = a.y;
+}
 // the x components lane entries E i .. E i + E - 1 multiply: columns c0, c0 + 1, ... mod 6
 template <class T>
 __device__ __forceinline__ void load_columns(const T* xr, int c0, T* v);
@@ -1019,7 +1073,9 @@ __device__ __forceinline__ void load_columns<double>(const double* xr, int c0, d
 // mode 0: out = r - A x, and (out2 not NULL) out2 = w D^-1 (r - A x), the damped Jacobi
 //         start of a cycle whose right-hand side is this residual;
 // mode 1: out = x + w D^-1 (r - A x) (+ add, when add is not NULL).
-template <class T, int kWide>
+// kU blocks' loads are issued together per lane (4 or 8); kNT: the blocks' loads are
+// non-temporal. Neither changes the sums or their order.
+template <class T, int kWide, int kU, int kNT>
 __global__ __launch_bounds__(256) void k_c_resid(int mode, int64_t n, const uint32_t* ast, const uint8_t* done,
                                                  const int* rowptr, const int* col, const T* A, const T* dinv,
                                                  const double* step, const T* r, const T* x, T* out,
@@ -1044,24 +1100,30 @@ __global__ __launch_bounds__(256) void k_c_resid(int mode, int64_t n, const uint
     const int c0 = (E * i) % 6;
     int b = rowptr[I] + gi;
     const int b1 = rowptr[I + 1];
-    for (; b + 3 * G < b1; b += 4 * G) {
-      int cb[4];
-      T av[4][E], xv[4][E];
+    for (; b + (kU - 1) * G < b1; b += kU * G) {
+      int cb[kU];
+      T av[kU][E], xv[kU][E];
 #pragma unroll
-      for (int q = 0; q < 4; q++) cb[q] = col[b + q * G];
+      for (int q = 0; q < kU; q++) cb[q] = col[b + q * G];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        load_entries<T>(A + 36 * (int64_t)(b + q * G) + E * i, av[q]);
+      for (int q = 0; q < kU; q++) {
+        if (kNT)
+          load_entries_nt<T>(A + 36 * (int64_t)(b + q * G) + E * i, av[q]);
+        else
+          load_entries<T>(A + 36 * (int64_t)(b + q * G) + E * i, av[q]);
         load_columns<T>(x + 6 * (int64_t)cb[q], c0, xv[q]);
       }
 #pragma unroll
-      for (int q = 0; q < 4; q++)
+      for (int q = 0; q < kU; q++)
 #pragma unroll
         for (int k = 0; k < E; k++) acc[k] += av[q][k] * xv[q][k];
     }
     for (; b < b1; b += G) {
       T av[E], xv[E];
-      load_entries<T>(A + 36 * (int64_t)b + E * i, av);
+      if (kNT)
+        load_entries_nt<T>(A + 36 * (int64_t)b + E * i, av);
+      else
+        load_entries<T>(A + 36 * (int64_t)b + E * i, av);
       load_columns<T>(x + 6 * (int64_t)col[b], c0, xv);
 #pragma unroll
       for (int k = 0; k < E; k++) acc[k] += av[k] * xv[k];
@@ -2547,9 +2609,10 @@ struct Solver {
     tune_coarse_layout();
   }
 
-  // Per coarse level, the faster layout of its residual kernel (narrow or wide rows, see
-  // k_c_resid) in the precision the solve runs it in, timed on the level itself: three
-  // launches each after a warm one. STATICS_HIP_WIDE=0 or 1 forces one.
+  // Per coarse level, the fastest layout of its residual kernel (narrow or wide rows, four
+  // or eight blocks in flight, plain or non-temporal block loads, see k_c_resid) in the
+  // precision the solve runs it in, timed on the level itself: three launches each after a
+  // warm one. STATICS_HIP_WIDE=k (0-7) forces layout k.
   void tune_coarse_layout() {
     const char* force = std::getenv("STATICS_HIP_WIDE");
     hipEvent_t e0, e1;
@@ -2560,12 +2623,22 @@ struct Solver {
       if (lv.dense || exact_level(li)) continue;
       const bool f32 = mixed && (int)li < c.f64_from;
       if (force) {
-        (f32 ? lv.wide32 : lv.wide64) = std::atoi(force) != 0;
+        (f32 ? lv.layout32 : lv.layout64) = std::max(0, std::min(7, std::atoi(force)));
         continue;
       }
-      float ms[2] = {0, 0};
-      for (int w = 0; w < 2; w++) {
-        (f32 ? lv.wide32 : lv.wide64) = w != 0;
+      // a rebuilt hierarchy of the same upload has levels of the same shapes: the choice
+      // made for a level of this index, precision and shape stands
+      const std::array<int64_t, 4> shape{(int64_t)li, f32 ? 1 : 0, lv.n, lv.nblocks};
+      bool known = false;
+      for (auto& kv : c.coarse_layouts)
+        if (kv.first == shape) {
+          (f32 ? lv.layout32 : lv.layout64) = kv.second;
+          known = true;
+        }
+      if (known) continue;
+      float ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (int w = 0; w < 8; w++) {
+        (f32 ? lv.layout32 : lv.layout64) = w;
         auto run = [&] {
           if (f32) {
             ST_HIP(hipMemsetAsync(lv.x32.p, 0, lv.n * 6 * sizeof(float), st));
@@ -2582,10 +2655,16 @@ struct Solver {
         ST_HIP(hipEventSynchronize(e1));
         ST_HIP(hipEventElapsedTime(&ms[w], e0, e1));
       }
-      (f32 ? lv.wide32 : lv.wide64) = ms[1] < ms[0];
+      int best = 0;
+      for (int w = 1; w < 8; w++)
+        if (ms[w] < ms[best]) best = w;
+      (f32 ? lv.layout32 : lv.layout64) = best;
+      c.coarse_layouts.emplace_back(shape, best);
       if (std::getenv("STATICS_HIP_VERBOSE"))
-        std::fprintf(stderr, "statics_hip: level %zu residual layout %s (narrow %.1f us, wide %.1f us, %s)\n", li + 1,
-                     ms[1] < ms[0] ? "wide" : "narrow", 1e3 * ms[0] / 3, 1e3 * ms[1] / 3, f32 ? "fp32" : "fp64");
+        std::fprintf(stderr,
+                     "statics_hip: level %zu residual layout %d (%s): %.1f %.1f %.1f %.1f %.1f %.1f %.1f %.1f us\n",
+                     li + 1, best, f32 ? "fp32" : "fp64", 1e3 * ms[0] / 3, 1e3 * ms[1] / 3, 1e3 * ms[2] / 3,
+                     1e3 * ms[3] / 3, 1e3 * ms[4] / 3, 1e3 * ms[5] / 3, 1e3 * ms[6] / 3, 1e3 * ms[7] / 3);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -2697,22 +2776,28 @@ struct Solver {
   // The coarse residual kernel of level lv in its tuned layout (narrow or wide rows).
   // mode 0: out = r - A x (and out2 = w D^-1 (r - A x) when out2 is set);
   // mode 1: out = x + w D^-1 (r - A x) (+ add).
+  template <class T, int W, int U, int NT>
+  void c_resid_v(Level& lv, int mode, const T* r, const T* x, T* out, const T* add, T* out2, const T* dinv,
+                 const double* step) {
+    k_c_resid<T, W, U, NT><<<coarse_blocks<T>(lv.n, W != 0), 256, 0, st>>>(
+        mode, lv.n, lv.agg_struct.as<uint32_t>(), c.done.as<uint8_t>(), lv.rowptr.as<int>(), lv.col.as<int>(),
+        lvA<T>(lv), dinv, step, r, x, out, add, out2);
+  }
   template <class T>
   void c_resid(Level& lv, int mode, const T* r, const T* x, T* out, const T* add = nullptr, T* out2 = nullptr) {
-    const bool wide = std::is_same<T, double>::value ? lv.wide64 : lv.wide32;
-    const int64_t n = lv.n;
+    const int layout = std::is_same<T, double>::value ? lv.layout64 : lv.layout32;
     const T* dinv = (mode == 1 || out2) ? lvD<T>(lv) : nullptr;
     const double* step = (mode == 1 || out2) ? lv.step_d.as<double>() : nullptr;
-    if (wide)
-      k_c_resid<T, 1><<<coarse_blocks<T>(n, true), 256, 0, st>>>(mode, n, lv.agg_struct.as<uint32_t>(),
-                                                               c.done.as<uint8_t>(), lv.rowptr.as<int>(),
-                                                               lv.col.as<int>(), lvA<T>(lv), dinv, step, r, x, out,
-                                                               add, out2);
-    else
-      k_c_resid<T, 0><<<coarse_blocks<T>(n, false), 256, 0, st>>>(mode, n, lv.agg_struct.as<uint32_t>(),
-                                                                c.done.as<uint8_t>(), lv.rowptr.as<int>(),
-                                                                lv.col.as<int>(), lvA<T>(lv), dinv, step, r, x, out,
-                                                                add, out2);
+    switch (layout) {  // bit 0: wide rows; bit 1: eight blocks in flight; bit 2: non-temporal blocks
+      case 0: c_resid_v<T, 0, 4, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
+      case 1: c_resid_v<T, 1, 4, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
+      case 2: c_resid_v<T, 0, 8, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
+      case 3: c_resid_v<T, 1, 8, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
+      case 4: c_resid_v<T, 0, 4, 1>(lv, mode, r, x, out, add, out2, dinv, step); break;
+      case 5: c_resid_v<T, 1, 4, 1>(lv, mode, r, x, out, add, out2, dinv, step); break;
+      case 6: c_resid_v<T, 0, 8, 1>(lv, mode, r, x, out, add, out2, dinv, step); break;
+      default: c_resid_v<T, 1, 8, 1>(lv, mode, r, x, out, add, out2, dinv, step); break;
+    }
   }
 
   // The coarse solve of level li for a V-cycle in precision T, on the right-hand side in
@@ -2956,6 +3041,10 @@ struct Solver {
     }
     k_cg_start<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), tol, mixed ? 1 : 0,
                                          swant.as<uint8_t>());
+    if (precision == ST_MIXED32) {
+      pcg32(max_iter, refresh);
+      return;
+    }
     {
       c.p64.ensure(v64);
       c.ap64.ensure(v64);
@@ -2992,10 +3081,12 @@ struct Solver {
         k_copy_fine<double, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, z, p);
       std::vector<uint8_t> dh, wh;
       int it = 0;
-      const int chunk = 4;
-      // Everything of an iteration after K p (step length, updates, the preconditioner
-      // with its hundreds of coarse-level launches, the next direction) has fixed
-      // arguments: it is captured once into a graph and replayed.
+      // Every launch of an iteration (K p, the step length, the updates, the preconditioner
+      // with its hundreds of coarse-level launches, the next direction) has fixed arguments:
+      // with c.graphs, `chunk` whole iterations are captured once into one graph and each
+      // chunk is one graph launch; the device-side done flags make a converged structure's
+      // work a no-op inside it, and the host looks at the flags once per chunk.
+      const int chunk = c.graphs ? 8 : 4;
       auto tail = [&]() {
         reduce_bricks(c.partial, S_PAP);
         k_cg_alpha<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>());
@@ -3012,19 +3103,24 @@ struct Solver {
         else
           k_update_p<double, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z, p);
       };
-      GraphExec tail_graph;
+      auto iteration = [&]() {
+        EpiDot<double> ed;
+        ed.done = done;
+        ed.partial = partial;
+        ed.y = ap;
+        launch_apply<double>(c, p, ed);
+        tail();
+      };
+      GraphExec chunk_graph;
       while (it < max_iter) {
         const int k_end = std::min(max_iter, it + chunk);
-        for (; it < k_end; it++) {
-          EpiDot<double> ed;
-          ed.done = done;
-          ed.partial = partial;
-          ed.y = ap;
-          launch_apply<double>(c, p, ed);
-          if (c.graphs)
-            tail_graph.run(st, tail);
-          else
-            tail();
+        if (c.graphs && k_end - it == chunk) {
+          chunk_graph.run(st, [&] {
+            for (int k = 0; k < chunk; k++) iteration();
+          });
+          it = k_end;
+        } else {
+          for (; it < k_end; it++) iteration();
         }
         ST_HIP(hipGetLastError());
         sync_flags(dh, wh);
@@ -3040,12 +3136,99 @@ struct Solver {
           launch_apply<double>(c, x, er);
           reduce_bricks(c.partial, S_RRTRUE);
           k_cg_true<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), swant.as<uint8_t>(),
-                                              srefresh.as<int>());
+                                              srefresh.as<int>(), kStall, 0.81);
           sync_flags(dh, wh);
         }
         if (std::all_of(dh.begin(), dh.end(), [](uint8_t d) { return d != 0; })) break;
       }
     }
+    ST_HIP(hipGetLastError());
+  }
+
+  // The CG in FP32 (ST_MIXED32) after pcg's start (||f||^2, r64 = f - K x64, the scalars):
+  // r, p, K p, z and the steps dx since the last replacement are FP32; inner products,
+  // step lengths and x stay FP64. When a structure's FP32 residual has fallen a hundredfold
+  // since the last replacement, or meets the tolerance, the steps are added into x in FP64
+  // and the FP64 true residual f - K x replaces r (the direction p is kept).
+  void pcg32(int max_iter, int refresh) {
+    const unsigned nb = (unsigned)g.nbricks;
+    const int64_t nn = g.nnodes;
+    const size_t v32 = (size_t)std::max<int64_t>(nn, 1) * 6 * 4;
+    const uint32_t* nbase = g.nbase.as<uint32_t>();
+    const uint32_t* bst = g.bstruct.as<uint32_t>();
+    uint8_t* done = c.done.as<uint8_t>();
+    double* scal = c.scal.as<double>();
+    double* partial = c.partial.as<double>();
+    for (DBuf* b : {&c.r32, &c.z32, &c.t32, &c.p32, &c.w32, &c.ap32, &c.x32}) b->ensure(v32);
+    float *r = c.r32.as<float>(), *z = c.z32.as<float>(), *p = c.p32.as<float>(), *q = c.ap32.as<float>(),
+          *dx = c.x32.as<float>();
+    double* x = c.x64.as<double>();
+    // a fall of the recursive residual's square by this much since the last replacement asks for one
+    const double drop2 = 1e-4;
+    k_copy_fine<float, double><<<nb, kBlock, 0, st>>>(nbase, bst, nullptr, c.r64.as<double>(), r);
+    k_zero_fine<<<nb, kBlock, 0, st>>>(nbase, dx);
+    // (the V-cycle's scratch: t32, q (free until the next K p) and w32)
+    auto precond = [&](int slot) {
+      vcycle<float>(r, z, c.t32.as<float>(), q, c.w32.as<float>());
+      reduce_bricks(c.partial, slot);
+    };
+    precond(S_RZ);
+    k_copy_fine<float, float><<<nb, kBlock, 0, st>>>(nbase, bst, done, z, p);
+    std::vector<uint8_t> dh, wh;
+    int it = 0, last_replace = 0;
+    const int chunk = 4;
+    // a replacement at least every 32 iterations: an FP32 recursion that stagnates (stiff
+    // materials) is then seen by the stall count, and the FP64 CG takes over
+    const int every = 32;
+    while (it < max_iter) {
+      const int k_end = std::min(max_iter, it + chunk);
+      for (; it < k_end; it++) {
+        EpiDot<float> ed;
+        ed.done = done;
+        ed.partial = partial;
+        ed.y = q;
+        launch_apply<float>(c, p, ed);
+        reduce_bricks(c.partial, S_PAP);
+        k_cg_alpha<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>());
+        k_update_xr<float><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, dx, r, p, q, partial, nullptr);
+        reduce_bricks(c.partial, S_RR);
+        // refresh < 0: the reference's rule, the recursive residual decides (bodies.rs:350-352);
+        // it still asks for the replacements, which keep the FP32 recursion near the true residual
+        k_cg_check<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), siters.as<int>(), 1,
+                                             swant.as<uint8_t>(), drop2);
+        precond(S_RZNEW);
+        k_cg_beta<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>());
+        k_update_p<float, float><<<nb, kBlock, 0, st>>>(nbase, bst, done, scal, z, p);
+      }
+      ST_HIP(hipGetLastError());
+      sync_flags(dh, wh);
+      bool any_want = false;
+      for (int s2 = 0; s2 < ns; s2++) any_want |= !dh[s2] && wh[s2];
+      if (any_want || it >= max_iter || it - last_replace >= every) {
+        last_replace = it;
+        k_flush_fine<<<nb, kBlock, 0, st>>>(nbase, dx, x);
+        EpiResid<double, double> er;
+        er.done = done;
+        er.partial = partial;
+        er.b = c.f64.as<double>();
+        er.r = c.r64.as<double>();
+        launch_apply<double>(c, x, er);
+        reduce_bricks(c.partial, S_RRTRUE);
+        if (refresh < 0) {
+          // the recursive residual decides: the structures whose FP32 residual met the
+          // tolerance stop here; the others go on from the true residual
+          k_cg_true_rec<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), swant.as<uint8_t>(),
+                                                  srefresh.as<int>());
+        } else {
+          k_cg_true<<<nblk(ns), 256, 0, st>>>(ns, scal, done, sstatus.as<int>(), swant.as<uint8_t>(),
+                                              srefresh.as<int>(), 3, 0.25);
+        }
+        k_copy_fine<float, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, c.r64.as<double>(), r);
+        sync_flags(dh, wh);
+      }
+      if (std::all_of(dh.begin(), dh.end(), [](uint8_t d) { return d != 0; })) break;
+    }
+    k_flush_fine<<<nb, kBlock, 0, st>>>(nbase, dx, x);
     ST_HIP(hipGetLastError());
   }
 
@@ -3154,7 +3337,13 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   if (const char* e = std::getenv("STATICS_HIP_NU")) c.nu = std::max(1, std::atoi(e));
   // graph replay of the iteration tail: opt-in (STATICS_HIP_GRAPHS=1); at 256^3 the launch
   // queue stays full without it, and rocprofv3's kernel tracing does not follow captures
-  c.graphs = std::getenv("STATICS_HIP_GRAPHS") != nullptr;
+  // graph replay of whole iterations: on for batches (where the launches of many small
+  // structures' tails would otherwise cost more than their work), off while K p is timed
+  // (its events are host-side records) and for one structure, where the launch queue
+  // stays full (STATICS_HIP_GRAPHS=0 or 1 forces it)
+  c.graphs = c.geo.nstruct > 1 && c.timing_which < 0;
+  if (const char* e = std::getenv("STATICS_HIP_GRAPHS")) c.graphs = std::atoi(e) != 0;
+  if (c.timing_which >= 0) c.graphs = false;
   c.dense_top = std::getenv("STATICS_HIP_NODENSE") == nullptr;
   c.nu1 = 1;
   if (const char* e = std::getenv("STATICS_HIP_NU1")) c.nu1 = std::max(1, std::atoi(e));
@@ -3194,7 +3383,7 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   c.f64_from = 4;
   if (const char* e = std::getenv("STATICS_HIP_F64FROM")) c.f64_from = std::max(0, std::atoi(e));
   autotune_apply(c);
-  S.mixed = o.precision == ST_MIXED;
+  S.mixed = o.precision != ST_FP64;  // the FP32 V-cycle
   S.init_struct_arrays();
   const Ctx::HierKey key{S.mixed, c.smoothed, c.dense_top, c.sa_levels, c.f64_from};
   const bool kept = c.hier_valid && c.hier_key == key;
@@ -3218,7 +3407,29 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
                    c.levels[l].n ? (double)c.levels[l].nblocks / c.levels[l].n : 0.0,
                    (long long)(l ? c.levels[l - 1].pblocks : c.p0blocks));
   }
-  if (S.ndof > 0) S.pcg(o.precision, o.tol, o.max_iter, o.refresh);
+  // ST_MIXED32: a structure whose FP32 CG stops short (stiff materials: FP32 K p loses the
+  // cancellation a 1e-8 residual needs) goes on with the FP64 CG from where it stopped, on
+  // the same FP32 V-cycle and hierarchy (the others are done at once: their true residual
+  // already meets the tolerance).
+  int prec_used = o.precision;
+  std::vector<int> it_fp32(ns, 0), st_fp32(ns, ST_OK);
+  bool fell_back = false;
+  if (S.ndof > 0) {
+    S.pcg(o.precision, o.tol, o.max_iter, o.refresh);
+    if (o.precision == ST_MIXED32) {
+      std::vector<int> sts(ns);
+      ST_HIP(hipMemcpy(sts.data(), S.sstatus.p, ns * sizeof(int), hipMemcpyDeviceToHost));
+      if (std::any_of(sts.begin(), sts.end(), [](int v) { return v == ST_NOT_SPD || v == ST_NO_CONVERGENCE; })) {
+        fell_back = true;
+        st_fp32 = sts;
+        ST_HIP(hipMemcpy(it_fp32.data(), S.siters.p, ns * sizeof(int), hipMemcpyDeviceToHost));
+        if (std::getenv("STATICS_HIP_VERBOSE"))
+          std::fprintf(stderr, "statics_hip: the FP32 CG stopped short; on with the FP64 CG\n");
+        prec_used = ST_MIXED;
+        S.pcg(prec_used, o.tol, o.max_iter, o.refresh);
+      }
+    }
+  }
   // The fast cycle settings (gamma counts up to 4, FP32 on coarse levels 3-4) can lose
   // definiteness on nearly singular structures, such as a settle round where the structure
   // comes apart. A solve they fail is repeated once, from zero, with the robust settings:
@@ -3243,7 +3454,7 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
       c.hier_valid = true;
       c.hier_key = Ctx::HierKey{S.mixed, c.smoothed, c.dense_top, c.sa_levels, c.f64_from};
       ST_HIP(hipMemsetAsync(c.x64.p, 0, (size_t)c.geo.nnodes * 6 * sizeof(double), c.stream));
-      S.pcg(o.precision, o.tol, o.max_iter, o.refresh);
+      S.pcg(prec_used, o.tol, o.max_iter, o.refresh);
     }
   }
   S.true_residual();
@@ -3267,7 +3478,7 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   for (int s = 0; s < ns; s++) {
     StStats& t = stats[s];
     const StructInfo& si = c.host.structs[s];
-    t.iters = it[s] + it_first[s];
+    t.iters = it[s] + it_first[s] + it_fp32[s];
     t.levels = 1 + (int)c.levels.size();
     t.refreshes = rf[s];
     const double ff = sc[(size_t)s * kScal + S_FF], rr = sc[(size_t)s * kScal + S_RRTRUE];
@@ -3287,8 +3498,8 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
     t.cells = si.ncells;
     t.bricks = si.nbricks;
     t.dof_nodes = s < (int)c.dof_per_struct.size() ? c.dof_per_struct[s] : 0;
-    t.retries = retried ? 1 : 0;
-    t.first_status = retried ? st_first[s] : ST_OK;
+    t.retries = (fell_back ? 1 : 0) + (retried ? 1 : 0);
+    t.first_status = fell_back ? st_fp32[s] : retried ? st_first[s] : ST_OK;
   }
   if (o.release_hierarchy) release_hierarchy(c);
   const int64_t peak = mem_tally().peak.load();

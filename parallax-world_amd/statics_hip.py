@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libstatics_hip.so")
 ST_OK, ST_BAD_INPUT, ST_NO_CONVERGENCE, ST_NOT_SPD, ST_NAN, ST_OOM, ST_DEVICE, ST_NO_STRUCTURE = range(8)
 STATUS = {0: "OK", 1: "BAD_INPUT", 2: "NO_CONVERGENCE", 3: "NOT_SPD", 4: "NAN", 5: "OOM", 6: "DEVICE",
           7: "NO_STRUCTURE"}
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED32 = 0, 1, 2
 COMPLIANCE, SOFTMAX_STRESS = 0, 1
 VALUE_ONLY = 256  # or-ed into the objective: J alone
 

@@ -54,7 +54,7 @@ def test_config1_point_loads_parity(ctx):
     f = point_loads(P, gone, np.random.default_rng(21), 64)
     x_or, st_or = P.L.motions(f, gone=gone, tol=1e-13, max_iter=5000)
     assert not st_or["capped"]
-    for prec in (sh.FP64, sh.MIXED):
+    for prec in (sh.FP64, sh.MIXED, sh.MIXED32):
         x_dev, st = ctx.solve(P.to_dev(f), precision=prec, tol=1e-8, max_iter=2000)
         assert st.status == 0 and st.rel_residual <= 1e-8 and st.retries == 0
         err = rel(P.from_dev(x_dev), x_or)
@@ -160,7 +160,7 @@ def test_config1_full_size_properties(ctx):
     print(f"config 1 at 256^3: FP64 reference {s_ref.iters} iterations, residual {s_ref.rel_residual:.2e}",
           flush=True)
     assert s_ref.rel_residual <= 1e-9
-    for prec in (sh.MIXED, sh.FP64):
+    for prec in (sh.MIXED, sh.MIXED32, sh.FP64):
         x, st = ctx.solve(f, precision=prec, tol=1e-8, max_iter=2000)
         err = rel(x, x_ref)
         print(f"config 1 at 256^3, precision {prec}: {st.iters} iterations, residual {st.rel_residual:.2e}, "

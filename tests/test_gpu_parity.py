@@ -144,7 +144,7 @@ def test_apply_parity_dead_beams_and_gone(ctx):
     assert rel(P.from_dev(ctx.apply(P.to_dev(x))), P.L.apply(x, gone=gone2)) < 1e-13
 
 
-@pytest.mark.parametrize("precision", [sh.FP64, sh.MIXED])
+@pytest.mark.parametrize("precision", [sh.FP64, sh.MIXED, sh.MIXED32])
 @pytest.mark.parametrize("seed", [0, 1])
 def test_solve_parity_generated(ctx, precision, seed):
     P = generated(ctx, 32, seed)
@@ -165,7 +165,7 @@ def test_solve_parity_reference_scenes(ctx, scene):
     x_or, _ = P.L.motions(f, gone=gone, tol=1e-13, max_iter=5000)
     # stone's stiff slab floors the FP64 true residual near 5e-10 for the oracle too (its recursive
     # residual reaches 1e-14): these scenes are held to the solve parity bar, tol 1e-8 and 1e-6
-    for prec in (sh.FP64, sh.MIXED):
+    for prec in (sh.FP64, sh.MIXED, sh.MIXED32):
         x_dev, st = ctx.solve(P.to_dev(f), precision=prec, tol=1e-8)
         assert st.status == 0
         assert rel(P.from_dev(x_dev), x_or) < 1e-6
@@ -208,6 +208,10 @@ def test_reference_stopping_rule(ctx):
         x_dev, st = ctx.solve(P.to_dev(f), precision=prec, tol=1e-8, refresh=-1)
         assert st.status == 0 and st.refreshes == 0 and st.rel_residual < 1e-7
         assert rel(P.from_dev(x_dev), x_or) < 1e-6
+    # MIXED32 replaces its FP32 residual by the true one as it goes, under either rule
+    x_dev, st = ctx.solve(P.to_dev(f), precision=sh.MIXED32, tol=1e-8, refresh=-1)
+    assert st.status == 0 and st.refreshes >= 1 and st.rel_residual < 1e-7
+    assert rel(P.from_dev(x_dev), x_or) < 1e-6
 
 
 def test_bond_forces_parity(ctx):
@@ -237,6 +241,8 @@ def settle_both(P, tol=1e-6, **kw):
     some rounds' systems floor the true residual near 3e-6 (the portico's second round)."""
     o = P.L.settle(tol=1e-10)
     gone, beams, rounds, st = P.ctx.settle(P.mats[:, 3:], P.to_dev(P.L.gravity()), tol=tol, **kw)
+    print(f"settle: {rounds} rounds, {st.iters} iterations, restarts {st.retries}, flagged {st.first_status}, "
+          f"residual {st.rel_residual:.2e}")
     assert st.status == 0
     gone = P.from_dev(gone)
     free = ~P.L.held.astype(bool)
@@ -394,6 +400,30 @@ def test_batch_matches_single(ctx):
         assert s1.status == sb.status
         if s1.status == 0:
             assert rel(xb, x1) < 1e-8
+
+
+def test_batch_graph_replay(ctx, monkeypatch):
+    """A batch's iterations replayed as captured graphs (eight iterations a launch) against the
+    same batch launched kernel by kernel: the same kernels in the same order. Each batch call
+    uploads and builds its hierarchy anew, and the Galerkin products' LDS atomics make a
+    rebuilt hierarchy equal to rounding only, so the solutions agree to the solve's accuracy
+    (1e-7 relative, the iteration counts within one); twice in a row, eager solves between."""
+    sets = []
+    for n, seed in ((16, 21), (28, 22), (20, 23), (36, 24)):
+        occ, theta = sh.generate(n, n, n, seed=seed, sigma=0.5)
+        sets.append(sh.cells_from_occ(occ, theta))
+    monkeypatch.setenv("STATICS_HIP_GRAPHS", "0")
+    xe, se = ctx.solve_batch(sets, precision=sh.MIXED, tol=1e-8, max_iter=2000)
+    for rep in range(2):
+        monkeypatch.setenv("STATICS_HIP_GRAPHS", "1")
+        xg, sg = ctx.solve_batch(sets, precision=sh.MIXED, tol=1e-8, max_iter=2000)
+        assert all(abs(a.iters - b.iters) <= 1 for a, b in zip(sg, se)) and all(s.status == 0 for s in sg)
+        for a, b in zip(xg, xe):
+            assert rel(a, b) < 1e-7
+        monkeypatch.setenv("STATICS_HIP_GRAPHS", "0")
+        xe2, _ = ctx.solve_batch(sets, precision=sh.MIXED, tol=1e-8, max_iter=2000)
+        for a, b in zip(xe2, xe):
+            assert rel(a, b) < 1e-7
 
 
 def test_edge_cases(ctx):
