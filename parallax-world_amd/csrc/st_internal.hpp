@@ -149,6 +149,8 @@ struct Level {
   DBuf rowptr;          // n+1 (int32): blocks of row I at [rowptr[I], rowptr[I+1]); diagonal first
   DBuf col;             // nblocks (int32)
   DBuf a64, a32;        // 36 per block, row-major
+  DBuf a16, ascale;     // the FP16 copy of the blocks, and each block's scale (when half)
+  bool half = false;    // the mixed V-cycle reads the FP16 copy
   DBuf dinv64, dinv32;  // 36 per aggregate
   DBuf centre;          // 3 per aggregate (double, metres, structure-local)
   DBuf parent;          // per aggregate of this level: its aggregate in the next level (-1 none)
@@ -214,6 +216,7 @@ struct Ctx {
   std::vector<int> gammas;  // if set: cycles per visit of each level index (1 beyond the list);
                             // if empty, gamma on the indices (gamma_from, gamma_to]
   int f64_from = 4;      // first level (0: level 1) kept in FP64 by the mixed V-cycle
+  bool half_coarse = true;  // the mixed V-cycle's largest coarse operators stored in FP16
   bool graphs = false;   // replay each CG iteration's tail as a captured graph
   bool dense_top = true; // invert a small coarse level densely and drop the levels above
   int apply_sched[2] = {0, 0};  // operator schedule per precision (FP32, FP64), see kSchedules
@@ -229,9 +232,10 @@ struct Ctx {
   struct HierKey {
     bool mixed, smoothed, dense_top;
     int sa_levels, f64_from;
+    bool half;
     bool operator==(const HierKey& o) const {
       return mixed == o.mixed && smoothed == o.smoothed && dense_top == o.dense_top && sa_levels == o.sa_levels &&
-             f64_from == o.f64_from;
+             f64_from == o.f64_from && half == o.half;
     }
   };
   bool hier_valid = false;

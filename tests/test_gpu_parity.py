@@ -178,6 +178,25 @@ def test_solve_parity_reference_scenes(ctx, scene):
     assert abs(u_dev.max() - u_or.max()) <= 1e-5 * u_or.max()
 
 
+@pytest.mark.parametrize("case", ["generated", "stone cantilever", "portico"])
+def test_half_coarse_operators(ctx, monkeypatch, case):
+    """The mixed V-cycle's coarse operators stored in FP16 (block-scaled, the default) against
+    the oracle: the solves meet the parity bar, on the stiff stone scenes too."""
+    monkeypatch.setenv("STATICS_HIP_HALF", "1")
+    if case == "generated":
+        P = generated(ctx, 32, 3)
+        f = gravity_z(P.L)
+    else:
+        scene = scenes.cantilever(orc.STONE, 0.5, 3.0) if case == "stone cantilever" else scenes.portico(1, 0.5)
+        P = Pair(ctx, *scenes.solids(0, *scene))
+        f = P.L.gravity()
+    gone = classify_both(P)
+    x_or, _ = P.L.motions(f, gone=gone, tol=1e-13, max_iter=5000)
+    x_dev, st = ctx.solve(P.to_dev(f), precision=sh.MIXED, tol=1e-8)
+    print(f"half coarse operators, {case}: {st.iters} iterations, restarts {st.retries}")
+    assert st.status == 0 and rel(P.from_dev(x_dev), x_or) < 1e-6
+
+
 def test_fused_restriction_matches(ctx, monkeypatch):
     """The level-1 restriction fused into the operator's epilogue against the separate kernel:
     the same sums in the same order per aggregate, compiled inside another kernel (rounding may
