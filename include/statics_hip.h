@@ -60,6 +60,11 @@ typedef struct {
   const double* g;      /* 1 per cell */
   const uint16_t* beams;/* 1 per cell, or NULL */
   double spacing;       /* metres per cell: UNIT * 2^level (statics/src/lattice.rs:34-36) */
+  const double* section;/* 15 per cell, or NULL: beam q's section over SECTION s^2, each beam's
+                           own section as gather::measured gives it (statics/src/gather.rs:
+                           386-390): its force terms scale with it, its couple with its square
+                           (K4 = A / d, R2 = A^2 / (4 pi d)). NULL: 1 for every beam, the
+                           sections lattice::build gives (lattice.rs:151-158) */
 } StCells;
 
 /* ST_FP64: everything in FP64. ST_MIXED: the CG (vectors, K p, inner products,
@@ -202,20 +207,28 @@ typedef struct {
   double* rho;            /* 1 per cell, or NULL */
   uint16_t* beams;        /* 1 per cell, or NULL: the beams the lattice holds (bit q, pair q) */
   double* strength;       /* 45 per cell, or NULL: ft, fc, fs of each beam q (0 where absent) */
-  int64_t* beam_of_bond;  /* 1 per bond, or NULL: 15 * cell + q of the bond's beam */
+  int64_t* bond_index;    /* 1 per bond, or NULL: the bond's place 60 * cell + 4 * q + k in the
+                             per-bond outputs (st_bond_used; st_bond_forces at 3 times it), or
+                             -1 minus that place for a beam that runs from the greater face to
+                             the lesser (gather::measured makes such): the force on its j side
+                             is then the negated st_bond_forces value, its used share the same */
+  double* section;        /* 15 per cell, or NULL: each beam's section over SECTION s^2 (1 where
+                             absent), for StCells.section */
 } StLatticeCells;
 
 /* Replaces: the caller's side of lattice::build (statics/src/lattice.rs:124-163) run
  * backwards, for a caller that holds a Lattice (statics/src/lattice.rs:61-71) rather than
  * its cells: each beam's cell is the site its end nodes share (as statics/src/gather.rs:245
  * finds it), its pair the two faces of that cell its nodes are; a held node's other cell is
- * held (lattice.rs:138). Host only; no context. Takes a lattice that build could have made
- * at its level, strengths per beam allowed (bodies::as_generated, statics/src/bodies.rs:
- * 611-625): every beam four bonds in lattice::beam's order and geometry (lattice.rs:76-84)
- * of section SECTION s^2 between face centres, one e and g to a cell, one material to a
- * beam. Anything else (a gathered lattice, whose beams have sections of their own,
- * statics/src/gather.rs:386-390) is ST_BAD_INPUT, with the reason in message. Call once
- * with out->sites NULL for n_cells, then with arrays of that size. */
+ * held (lattice.rs:138). Host only; no context. Takes a lattice of full cells at its level
+ * as build makes them (strengths per beam allowed, bodies::as_generated,
+ * statics/src/bodies.rs:611-625), or as gather::measured makes the next level's (beams of
+ * their own sections and strengths, statics/src/gather.rs:386-390; out->section then
+ * receives the sections, and a lattice with any section other than SECTION s^2 is refused
+ * when out->section is NULL): every beam four bonds in lattice::beam's order and geometry
+ * (lattice.rs:76-84) between face centres, one e and g to a cell, one material to a beam.
+ * Anything else is ST_BAD_INPUT, with the reason in message. Call once with out->sites
+ * NULL for n_cells, then with arrays of that size. */
 int st_lattice_cells(const StLattice* lat, StLatticeCells* out, char* message, int64_t message_len);
 int st_gather_read(StCtx* ctx, uint32_t* level, double* x, double* m, double* node_mat, uint8_t* held,
                    int64_t* around, int64_t* ij, double* p, double* n, double* a, double* d, double* bond_mat);

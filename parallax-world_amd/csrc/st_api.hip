@@ -46,15 +46,20 @@ struct BeamState {
   double n[3], hd, K4, R2, k;  // k = a/d per bond
   double r;                    // offset radius sqrt(A / 2 pi)
 };
-__device__ __forceinline__ void beam_state(const BeamGeo& G, int q, BeamState& b) {
+// Beam q of a cell whose section is rho SECTION s^2 (rho 1: lattice::build's).
+__device__ __forceinline__ void beam_state(const BeamGeo& G, int q, double rho, BeamState& b) {
   const int cls = pair_opposite(q) ? 0 : 1;
   for (int k = 0; k < 3; k++) b.n[k] = G.nscale[cls] * (double)pair_dir(q, k);
   b.hd = G.hd[cls];
-  b.K4 = G.K4[cls];
-  b.R2 = G.R2[cls];
-  b.k = 0.25 * G.K4[cls];
+  b.K4 = G.K4[cls] * rho;
+  b.R2 = G.R2[cls] * (rho * rho);
+  b.k = 0.25 * b.K4;
   // A = K4 * d ; r = sqrt(A / (2 pi))
-  b.r = sqrt(G.K4[cls] * 2.0 * G.hd[cls] / (2.0 * kPi));
+  b.r = sqrt(b.K4 * 2.0 * G.hd[cls] / (2.0 * kPi));
+}
+// Beam q of cell c: its section over SECTION s^2 (1 when the cells gave none).
+__device__ __forceinline__ double beam_sec(const double* csec, int64_t c, int q) {
+  return csec ? csec[15 * c + q] : 1.0;
 }
 
 // middle relative motion delta and dw of a beam between end motions xi, xj
@@ -106,15 +111,15 @@ __device__ __forceinline__ double bond_used(const double* f, const double* n, do
   return fmax(fmax(s1 / ft, -s3 / fc), (s1 - s3) / (2.0 * fs));
 }
 
-// A bond's area: a quarter of its beam's section SECTION * s^2 (lattice.rs:21-25, 81-83).
-__device__ __forceinline__ double bond_area(const BeamGeo& G) {
+// A bond's area: a quarter of its beam's section rho SECTION s^2 (lattice.rs:21-25, 81-83).
+__device__ __forceinline__ double bond_area(const BeamGeo& G, double rho) {
   const double s = 2.0 * G.hd[0];
-  return kSection * s * s / 4.0;
+  return kSection * s * s / 4.0 * rho;
 }
 
 // Per bond force on the j side: out[c][q][k][3], zero for dead beams.
 __global__ void k_bond_forces(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, const double* ce,
-                              const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* x,
+                              const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* csec, const double* x,
                               double* out) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
@@ -130,7 +135,7 @@ __global__ void k_bond_forces(int64_t ncells, const uint32_t* cfaces, const uint
       continue;
     }
     BeamState b;
-    beam_state(G, q, b);
+    beam_state(G, q, beam_sec(csec, c, q), b);
     double f[4][3];
     beam_bond_forces(b, e, g, xv[kPairA[q]], xv[kPairB[q]], f);
     for (int k = 0; k < 4; k++)
@@ -141,14 +146,14 @@ __global__ void k_bond_forces(int64_t ncells, const uint32_t* cfaces, const uint
 // Per bond used share (bodies.rs:544-546): out[c][q][k], zero for dead beams.
 // strength: ft, fc, fs per beam, 45 per cell (beam q at 3 q).
 __global__ void k_bond_used(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, const double* ce,
-                            const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* strength,
+                            const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* csec, const double* strength,
                             const double* x, double* out) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
   double xv[6][6];
   load_faces(cfaces, c, x, xv);
   const BeamGeo G = bgeo[cstruct[c]];
-  const double e = ce[c], g = cg[c], a = bond_area(G);
+  const double e = ce[c], g = cg[c];
   const uint32_t m = cmask[c];
   for (int q = 0; q < 15; q++) {
     const double* sq = strength + 45 * c + 3 * q;
@@ -159,10 +164,10 @@ __global__ void k_bond_used(int64_t ncells, const uint32_t* cfaces, const uint16
       continue;
     }
     BeamState b;
-    beam_state(G, q, b);
+    beam_state(G, q, beam_sec(csec, c, q), b);
     double f[4][3];
     beam_bond_forces(b, e, g, xv[kPairA[q]], xv[kPairB[q]], f);
-    for (int k = 0; k < 4; k++) o[k] = bond_used(f[k], b.n, a, ft, fc, fs);
+    for (int k = 0; k < 4; k++) o[k] = bond_used(f[k], b.n, bond_area(G, beam_sec(csec, c, q)), ft, fc, fs);
   }
 }
 
@@ -170,7 +175,7 @@ __global__ void k_bond_used(int64_t ncells, const uint32_t* cfaces, const uint16
 // gone) one of whose four bonds uses more than its strength (used > 1) dies whole; its
 // bit is cleared in the caller's mask. Counts the beams broken.
 __global__ void k_break(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, uint16_t* cmask_user,
-                        const double* ce, const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo,
+                        const double* ce, const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* csec,
                         const double* strength, const double* x, int* broken) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
@@ -179,17 +184,18 @@ __global__ void k_break(int64_t ncells, const uint32_t* cfaces, const uint16_t* 
   double xv[6][6];
   load_faces(cfaces, c, x, xv);
   const BeamGeo G = bgeo[cstruct[c]];
-  const double e = ce[c], g = cg[c], a = bond_area(G);
+  const double e = ce[c], g = cg[c];
   uint32_t dead = 0;
   for (int q = 0; q < 15; q++) {
     if (!(m & (1u << q))) continue;
     const double* sq = strength + 45 * c + 3 * q;
     const double ft = sq[0], fc = sq[1], fs = sq[2];
     BeamState b;
-    beam_state(G, q, b);
+    beam_state(G, q, beam_sec(csec, c, q), b);
     double f[4][3];
     beam_bond_forces(b, e, g, xv[kPairA[q]], xv[kPairB[q]], f);
     bool any = false;
+    const double a = bond_area(G, beam_sec(csec, c, q));
     for (int k = 0; k < 4; k++) any |= bond_used(f[k], b.n, a, ft, fc, fs) > 1.0;
     if (any) dead |= 1u << q;
   }
@@ -218,7 +224,7 @@ __device__ __forceinline__ void bond_oxn(const BeamState& b, int k, double* ox) 
 
 // pass 0: per-cell max sigma; pass 1: per-cell sum exp(beta (sigma - m))
 __global__ void k_softmax_stats(int pass, int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask,
-                                const double* ce, const uint32_t* cstruct, const BeamGeo* bgeo, const double* x,
+                                const double* ce, const uint32_t* cstruct, const BeamGeo* bgeo, const double* csec, const double* x,
                                 double beta, const double* mref, double* out) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
@@ -232,7 +238,7 @@ __global__ void k_softmax_stats(int pass, int64_t ncells, const uint32_t* cfaces
     if (!(m & (1u << q))) continue;
     if (cfaces[6 * c + kPairA[q]] & cfaces[6 * c + kPairB[q]] & kNotFree) continue;  // held-held: no such beam
     BeamState b;
-    beam_state(G, q, b);
+    beam_state(G, q, beam_sec(csec, c, q), b);
     double d[3], dw[3];
     beam_delta(b, xv[kPairA[q]], xv[kPairB[q]], d, dw);
     for (int k = 0; k < 4; k++) {
@@ -251,7 +257,7 @@ __global__ void k_softmax_stats(int pass, int64_t ncells, const uint32_t* cfaces
 // d J / d x for the soft-max: per bond weight w = exp(beta(sigma - m)) / Z, scattered to the
 // bond's end nodes (global FP64 atomics; run once per backward).
 __global__ void k_softmax_rhs(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, const double* ce,
-                              const uint32_t* cstruct, const BeamGeo* bgeo, const double* x, double beta,
+                              const uint32_t* cstruct, const BeamGeo* bgeo, const double* csec, const double* x, double beta,
                               const double* mz, double* g) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
@@ -265,7 +271,7 @@ __global__ void k_softmax_rhs(int64_t ncells, const uint32_t* cfaces, const uint
     if (!(m & (1u << q))) continue;
     if (cfaces[6 * c + kPairA[q]] & cfaces[6 * c + kPairB[q]] & kNotFree) continue;  // held-held: no such beam
     BeamState b;
-    beam_state(G, q, b);
+    beam_state(G, q, beam_sec(csec, c, q), b);
     double d[3], dw[3];
     beam_delta(b, xv[kPairA[q]], xv[kPairB[q]], d, dw);
     double gu[3] = {0, 0, 0}, gw[3] = {0, 0, 0};
@@ -295,7 +301,7 @@ __global__ void k_softmax_rhs(int64_t ncells, const uint32_t* cfaces, const uint
 
 // Per-cell sensitivity: -lambda_c^T K_c x_c (+ the soft-max's explicit sum w sigma when beta > 0).
 __global__ void k_sensitivity(int64_t ncells, const uint32_t* cfaces, const uint16_t* cmask, const double* ce,
-                              const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* x,
+                              const double* cg, const uint32_t* cstruct, const BeamGeo* bgeo, const double* csec, const double* x,
                               const double* lam, int softmax, double beta, const double* mz, double* out) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncells) return;
@@ -310,7 +316,7 @@ __global__ void k_sensitivity(int64_t ncells, const uint32_t* cfaces, const uint
     if (!(m & (1u << q))) continue;
     if (cfaces[6 * c + kPairA[q]] & cfaces[6 * c + kPairB[q]] & kNotFree) continue;  // held-held: no such beam
     BeamState b;
-    beam_state(G, q, b);
+    beam_state(G, q, beam_sec(csec, c, q), b);
     double d[3], dw[3], dl[3], dwl[3];
     beam_delta(b, xv[kPairA[q]], xv[kPairB[q]], d, dw);
     beam_delta(b, lv[kPairA[q]], lv[kPairB[q]], dl, dwl);
@@ -581,7 +587,7 @@ int st_bond_forces(StCtx* c, const double* x, double* out) {
     if (ncl)
       k_bond_forces<<<nblk(ncl, 128), 128, 0, c->stream>>>(ncl, c->geo.cfaces.as<uint32_t>(), c->geo.cmask.as<uint16_t>(),
                                                       c->geo.ce64.as<double>(), c->geo.cg64.as<double>(),
-                                                      c->geo.cstruct.as<uint32_t>(), c->geo.bgeo.as<BeamGeo>(),
+                                                      c->geo.cstruct.as<uint32_t>(), c->geo.bgeo.as<BeamGeo>(), c->geo.csec64.as<double>(),
                                                       xd.as<double>(), od.as<double>());
     ST_HIP(hipGetLastError());
     std::vector<double> h((size_t)ncl * 180);
@@ -615,7 +621,7 @@ int st_bond_used(StCtx* c, const double* strength, int per_beam, const double* x
     if (ncl)
       k_bond_used<<<nblk(ncl, 128), 128, 0, c->stream>>>(ncl, c->geo.cfaces.as<uint32_t>(), c->geo.cmask.as<uint16_t>(),
                                                     c->geo.ce64.as<double>(), c->geo.cg64.as<double>(),
-                                                    c->geo.cstruct.as<uint32_t>(), c->geo.bgeo.as<BeamGeo>(),
+                                                    c->geo.cstruct.as<uint32_t>(), c->geo.bgeo.as<BeamGeo>(), c->geo.csec64.as<double>(),
                                                     sd.as<double>(), xd.as<double>(), od.as<double>());
     ST_HIP(hipGetLastError());
     std::vector<double> h((size_t)ncl * 60);
@@ -680,7 +686,7 @@ int st_settle(StCtx* c, const double* strength, int per_beam, const double* load
         k_break<<<nblk(ncl, 128), 128, 0, c->stream>>>(ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(),
                                                       g.cmask_user.as<uint16_t>(), g.ce64.as<double>(),
                                                       g.cg64.as<double>(), g.cstruct.as<uint32_t>(),
-                                                      g.bgeo.as<BeamGeo>(), sd.as<double>(), c->xsol.as<double>(),
+                                                      g.bgeo.as<BeamGeo>(), g.csec64.as<double>(), sd.as<double>(), c->xsol.as<double>(),
                                                       broken.as<int>());
       ST_HIP(hipGetLastError());
       int nb = 0;
@@ -754,7 +760,7 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
       // J = m + log(sum exp(beta (sigma - m))) / beta, m the greatest sigma
       if (ncl) {
         k_softmax_stats<<<nblk(ncl, 128), 128, 0, s>>>(0, ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(),
-                                                  g.ce64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
+                                                  g.ce64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), g.csec64.as<double>(),
                                                   xd.as<double>(), beta, nullptr, cellv.as<double>());
         ST_HIP(hipMemcpyAsync(hc.data(), cellv.p, ncl * sizeof(double), hipMemcpyDeviceToHost, s));
         ST_HIP(hipStreamSynchronize(s));
@@ -765,7 +771,7 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
       mzh[0] = m;
       ST_HIP(hipMemcpyAsync(mz.p, mzh, 2 * sizeof(double), hipMemcpyHostToDevice, s));
       k_softmax_stats<<<nblk(ncl, 128), 128, 0, s>>>(1, ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(),
-                                                g.ce64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
+                                                g.ce64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), g.csec64.as<double>(),
                                                 xd.as<double>(), beta, mz.as<double>(), cellv.as<double>());
       ST_HIP(hipMemcpyAsync(hc.data(), cellv.p, ncl * sizeof(double), hipMemcpyDeviceToHost, s));
       ST_HIP(hipStreamSynchronize(s));
@@ -786,7 +792,7 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
       c->x64.ensure(bytes);
       ST_HIP(hipMemsetAsync(c->f64.p, 0, bytes, s));
       k_softmax_rhs<<<nblk(ncl, 128), 128, 0, s>>>(ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(), g.ce64.as<double>(),
-                                              g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), xd.as<double>(), beta,
+                                              g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), g.csec64.as<double>(), xd.as<double>(), beta,
                                               mz.as<double>(), c->f64.as<double>());
       k_mask_rows2<<<nblk(nn), 256, 0, s>>>(nn, g.nflags.as<uint8_t>(), c->f64.as<double>());
       ST_HIP(hipMemsetAsync(c->x64.p, 0, bytes, s));
@@ -795,7 +801,7 @@ int st_backward(StCtx* c, int objective, double beta, const StSolveOpts* opts, d
     }
     if (ncl && !value_only)
       k_sensitivity<<<nblk(ncl, 128), 128, 0, s>>>(ncl, g.cfaces.as<uint32_t>(), g.cmask.as<uint16_t>(), g.ce64.as<double>(),
-                                              g.cg64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
+                                              g.cg64.as<double>(), g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), g.csec64.as<double>(),
                                               xd.as<double>(), lam.as<double>(), soft ? 1 : 0, beta, mz.as<double>(),
                                               cellv.as<double>());
     ST_HIP(hipGetLastError());

@@ -77,6 +77,20 @@ ST_HD CellCoef<T> cell_coef(const BeamGeo& G, int cls, T e, T g) {
   return c;
 }
 
+// A beam of its own section rho * SECTION s^2 (a gathered lattice's, statics/src/gather.rs:
+// 386-390): the force terms scale with the section, the couple of the offsets with its
+// square (K4 = A / d, R2 = A^2 / (4 pi d)).
+template <class T>
+ST_HD CellCoef<T> scaled(const CellCoef<T>& c, T rho) {
+  CellCoef<T> o = c;
+  o.kg = c.kg * rho;
+  o.ke = c.ke * rho;
+  o.hk = c.hk * rho;
+  o.re = c.re * (rho * rho);
+  o.rx = c.rx * (rho * rho);
+  return o;
+}
+
 // Applies beam q of a cell to end motions xi = x[a], xj = x[b] (6 each: u, w) and adds
 // the beam's loads into yi, yj. n components nx, ny, nz in {-1,0,1} times c.ns.
 template <class T, int NX, int NY, int NZ>
@@ -143,10 +157,13 @@ ST_HD void beam_end(const CellCoef<T>& c, const T* xi, const T* xj, T* y) {
 // order), y[f][6] accumulates. mask bit q = beam q alive. only_face >= 0 restricts
 // to the beams touching that face (the others' loads are still added, but the
 // caller keeps only y[only_face]).
+// rho: the cell's 15 section factors, or NULL (every beam SECTION s^2).
 template <class T>
-ST_HD void cell_apply(const CellCoef<T>& c0, const CellCoef<T>& c1, uint32_t mask, const T (*x)[6], T (*y)[6]) {
+ST_HD void cell_apply(const CellCoef<T>& c0, const CellCoef<T>& c1, uint32_t mask, const T (*x)[6], T (*y)[6],
+                      const T* rho = nullptr) {
 #define ST_BEAM(q, a, b, cc, NX, NY, NZ) \
-  if (mask & (1u << q)) beam_apply<T, NX, NY, NZ>(cc, x[a], x[b], y[a], y[b]);
+  if (mask & (1u << q))                  \
+    beam_apply<T, NX, NY, NZ>(rho ? scaled<T>(cc, rho[q]) : cc, x[a], x[b], y[a], y[b]);
   ST_BEAM(0, 0, 1, c0, 1, 0, 0)
   ST_BEAM(1, 0, 2, c1, 1, -1, 0)
   ST_BEAM(2, 0, 3, c1, 1, 1, 0)

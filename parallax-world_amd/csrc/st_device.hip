@@ -96,6 +96,17 @@ void copy_geo_to_device(Ctx& c) {
     k_to_float<<<nb, 256, 0, s>>>(d.ncells, d.cg64.as<double>(), d.cg32.as<float>());
     ST_HIP(hipGetLastError());
   }
+  d.has_sec = !h.csec.empty();
+  if (d.has_sec) {
+    up(d.csec64, h.csec, s);
+    d.csec32.ensure(h.csec.size() * sizeof(float));
+    k_to_float<<<(unsigned)((h.csec.size() + 255) / 256), 256, 0, s>>>((int64_t)h.csec.size(),
+                                                                       d.csec64.as<double>(), d.csec32.as<float>());
+    ST_HIP(hipGetLastError());
+  } else {
+    d.csec64.release();
+    d.csec32.release();
+  }
 }
 
 // Applies the node flags (held, gone) to the cells' face lists and live masks.
@@ -123,6 +134,8 @@ GeoD<T> geo_view(Ctx& c) {
   g.cg = std::is_same<T, double>::value ? (const T*)d.cg64.p : (const T*)d.cg32.p;
   g.cmask = d.cmask.as<uint16_t>();
   g.bgeo = d.bgeo.as<BeamGeo>();
+  g.carea = !d.has_sec ? nullptr
+            : std::is_same<T, double>::value ? (const T*)d.csec64.p : (const T*)d.csec32.p;
   return g;
 }
 template GeoD<double> geo_view<double>(Ctx&);

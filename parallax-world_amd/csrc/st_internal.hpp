@@ -105,6 +105,8 @@ struct HostGeo {
   std::vector<uint32_t> cbase, nbase;         // per brick + 1
   std::vector<uint32_t> bstruct;              // per brick: structure
   std::vector<double> ce, cg;                 // per free cell
+  std::vector<double> csec;                   // 15 per free cell (beam sections over SECTION s^2),
+                                              // or empty: every beam SECTION s^2
   std::vector<uint16_t> cmask;                // per free cell, alive beams
   std::vector<uint32_t> cfaces;               // 6 per free cell
   std::vector<uint32_t> ccoord;               // per free cell: x | y << 10 | z << 20 (local)
@@ -128,6 +130,8 @@ struct DevGeo {
   int64_t nclass[2] = {0, 0};  // bricks per class
   uint32_t class_nodes[2] = {0, 0};  // largest node count in each class
   DBuf ce64, cg64, ce32, cg32, cmask_user, cmask, cfaces, ccoord, cstruct;
+  DBuf csec64, csec32;  // 15 per free cell when the cells gave beam sections
+  bool has_sec = false;
   DBuf nflags, ncell, nkind, nkey;
   DBuf bgeo;  // BeamGeo per structure
 };

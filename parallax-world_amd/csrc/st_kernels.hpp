@@ -26,6 +26,7 @@ struct GeoD {
   const T *ce, *cg;
   const uint16_t* cmask;
   const BeamGeo* bgeo;
+  const T* carea;  // 15 section factors per cell, or NULL (every beam SECTION s^2)
 };
 
 template <class T>
@@ -70,13 +71,15 @@ __device__ __forceinline__ void store6<float>(float* __restrict__ p, const float
 // so each beam's direction, class and end are constants.
 template <class T, int F, int Q>
 __device__ __forceinline__ void face_beam(const CellCoef<T>& c0, const CellCoef<T>& c1, uint32_t mask,
-                                          const T (*xv)[6], T* y) {
+                                          const T (*xv)[6], T* y, const T* rho) {
   if constexpr (kPairA[Q] == F || kPairB[Q] == F) {
-    if (mask & (1u << Q))
-      beam_end<T, pair_dir(Q, 0), pair_dir(Q, 1), pair_dir(Q, 2), kPairB[Q] == F>(pair_opposite(Q) ? c0 : c1,
+    if (mask & (1u << Q)) {
+      const CellCoef<T>& cc = pair_opposite(Q) ? c0 : c1;
+      beam_end<T, pair_dir(Q, 0), pair_dir(Q, 1), pair_dir(Q, 2), kPairB[Q] == F>(rho ? scaled<T>(cc, rho[Q]) : cc,
                                                                                  xv[kPairA[Q]], xv[kPairB[Q]], y);
+    }
   }
-  if constexpr (Q < 14) face_beam<T, F, Q + 1>(c0, c1, mask, xv, y);
+  if constexpr (Q < 14) face_beam<T, F, Q + 1>(c0, c1, mask, xv, y, rho);
 }
 
 // A brick's share of the operator's inputs staged in LDS: the motions of its first
@@ -136,7 +139,8 @@ __device__ __forceinline__ void cell_face_row(const GeoD<T>& g, const Tile<T>& t
     }
   }
   const T e = staged ? t.ce[lc] : g.ce[c], gg = staged ? t.cg[lc] : g.cg[c];
-  face_beam<T, F, 0>(cell_coef<T>(G, 0, e, gg), cell_coef<T>(G, 1, e, gg), mask, xv, y);
+  face_beam<T, F, 0>(cell_coef<T>(G, 0, e, gg), cell_coef<T>(G, 1, e, gg), mask, xv, y,
+                     g.carea ? g.carea + 15 * (size_t)c : nullptr);
 }
 
 // Row of K x at a free node of orientation O: face 2O+1 of the cell below (c-) and
@@ -305,7 +309,8 @@ __global__ __launch_bounds__(kBlock) void k_apply_cells(GeoD<T> g, const uint32_
       for (int k = 0; k < 6; k++) yv[f][k] = T(0);
     }
     const T e = g.ce[c], gg = g.cg[c];
-    cell_apply<T>(cell_coef<T>(G, 0, e, gg), cell_coef<T>(G, 1, e, gg), (uint32_t)g.cmask[c], xv, yv);
+    cell_apply<T>(cell_coef<T>(G, 0, e, gg), cell_coef<T>(G, 1, e, gg), (uint32_t)g.cmask[c], xv, yv,
+                  g.carea ? g.carea + 15 * (size_t)c : nullptr);
     if constexpr (Epi::kCellDot) {  // every (cell, face) counted once, in the cell's own brick
 #pragma unroll
       for (int f = 0; f < 6; f++)

@@ -1,16 +1,17 @@
 // st_lattice_cells: a Lattice as the reference holds it (statics/src/lattice.rs:61-71,
 // model.rs:92-111) back to the cells lattice::build made it from (lattice.rs:124-163):
 // each beam's cell is the site its two end nodes share (the technique of
-// statics/src/gather.rs:245), its pair q the two faces of that cell the nodes are; a cell
-// whose face node is held has the cell across that face held (lattice.rs:138).
+// statics/src/gather.rs:245), its pair q the two faces of that cell the nodes are, lesser
+// first; a cell whose face node is held has the cell across that face held (lattice.rs:138).
 //
-// The device's operator takes each cell's beams with the SECTION geometry of
-// lattice::beam (lattice.rs:76-84) and one e, g per cell. So a lattice is taken only when
-// it is one build could have made at its level, up to per-beam strengths (what
-// bodies::as_generated changes, statics/src/bodies.rs:611-625): every beam four bonds in
-// the order +r e1, -r e1, +r e2, -r e2, of section SECTION s^2 between face centres, the
-// bonds of a cell of one e and g, the bonds of a beam of one strength. A gathered lattice
-// (statics/src/gather.rs:386-390, beams of their own sections) is refused.
+// The device's operator takes each cell's beams between face centres with one e, g per
+// cell, and per beam a section (as a factor on SECTION s^2) and strengths. So a lattice is
+// taken when it is one build could have made at its level, up to per-beam strengths (what
+// bodies::as_generated changes, statics/src/bodies.rs:611-625), or one gather::measured
+// made (statics/src/gather.rs:386-390: beams of their own sections and strengths, one
+// material to a cell): every beam four bonds of one area a in the order +r e1, -r e1,
+// +r e2, -r e2 with r = sqrt(4 a / 2 pi) (lattice::beam, lattice.rs:76-84), the bonds of
+// a cell of one e and g, the bonds of a beam of one strength.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -73,11 +74,11 @@ void lattice_cells(const StLattice& L, StLatticeCells& out) {
   std::unordered_map<SiteKey, int64_t, SiteKeyHash> cell_of;
   std::vector<SiteKey> sites;
   std::vector<uint16_t> beams;
-  std::vector<double> e, g, rho, strength;  // strength: 45 per cell
+  std::vector<double> e, g, rho, strength, section;  // strength: 45 per cell, section: 15
   std::vector<uint8_t> have_mat;
-  std::vector<int64_t> beam_of_bond((size_t)nb);
-  const double area = 0.336 * s * s;  // statics/src/lattice.rs:21-25, both classes
-  const double r = std::sqrt(area / (2.0 * 3.14159265358979323846));
+  std::vector<int64_t> bond_index((size_t)nb);
+  const double full = 0.336 * s * s;  // SECTION s^2, statics/src/lattice.rs:21-25, both classes
+  int64_t first_own = -1;             // the first bond of a beam of a section of its own
   for (int64_t b = 0; b < nb; b += 4) {
     const int64_t i = L.ij[2 * b], j = L.ij[2 * b + 1];
     if (i < 0 || j < 0 || i >= nn || j >= nn || i == j) refuse("bond " + std::to_string(b) + " names a missing node");
@@ -98,14 +99,18 @@ void lattice_cells(const StLattice& L, StLatticeCells& out) {
     if (si < 0) refuse("beam at bond " + std::to_string(b) + " joins nodes of no common cell (not a built lattice)");
     const SiteKey c = *ci[si];
     // the face of the cell each end is: the lesser site's + face, or the greater site's - face
-    const int fa = 2 * ori[i] + (si == 0 ? 1 : 0), fb = 2 * ori[j] + (sj == 0 ? 1 : 0);
-    if (fa >= fb) refuse("beam at bond " + std::to_string(b) + ": its ends are not two faces a < b in VOXEL_FACES order");
+    const int fi = 2 * ori[i] + (si == 0 ? 1 : 0), fj = 2 * ori[j] + (sj == 0 ? 1 : 0);
+    if (fi == fj) refuse("beam at bond " + std::to_string(b) + ": its ends are one face");
+    // the device's beams run from the lesser face a to the greater b (build's, lattice.rs:
+    // 152-153); gather::measured's may run the other way (its handles' order, gather.rs:386-389)
+    const bool flip = fi > fj;
+    const int fa = flip ? fj : fi, fb = flip ? fi : fj;
     int q = -1;
     for (int t = 0; t < 15; t++)
       if (kPairA[t] == fa && kPairB[t] == fb) q = t;
-    // the geometry of lattice::beam between the two face centres
-    const double* xi = L.x + 3 * i;
-    const double* xj = L.x + 3 * j;
+    // the geometry of lattice::beam between the two face centres, from a to b
+    const double* xi = L.x + 3 * (flip ? j : i);
+    const double* xj = L.x + 3 * (flip ? i : j);
     double cc[3] = {((double)c.x + 0.5) * s, ((double)c.y + 0.5) * s, ((double)c.z + 0.5) * s};
     for (int t = 0; t < 3; t++) {
       const double want_i = cc[t] + 0.5 * s * face_normal(fa, t), want_j = cc[t] + 0.5 * s * face_normal(fb, t);
@@ -121,15 +126,24 @@ void lattice_cells(const StLattice& L, StLatticeCells& out) {
     const double l1 = std::sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
     for (double& t : e1) t /= l1;
     const double e2[3] = {nv[1] * e1[2] - nv[2] * e1[1], nv[2] * e1[0] - nv[0] * e1[2], nv[0] * e1[1] - nv[1] * e1[0]};
-    for (int k = 0; k < 4; k++) {
-      const int64_t bk = b + k;
-      if (!close(L.a[bk], area / 4.0, area))
-        refuse("beam at bond " + std::to_string(b) + " has a section other than SECTION s^2 (a gathered lattice?)");
+    // the beam's section: four times its bonds' area
+    const double area = 4.0 * L.a[b];
+    if (!(area > 0) || !std::isfinite(area)) refuse("beam at bond " + std::to_string(b) + ": a bond area is not positive");
+    if (first_own < 0 && !close(area, full, full)) first_own = b;
+    const double r = std::sqrt(area / (2.0 * 3.14159265358979323846));
+    // run from b to a, the beam's n and e1 are negated, e2 kept: its first two bonds are the
+    // device's in the other order
+    const double sn = flip ? -1.0 : 1.0;
+    for (int kl = 0; kl < 4; kl++) {
+      const int64_t bk = b + kl;
+      const int k = flip && kl < 2 ? kl ^ 1 : kl;  // the device's place of the bond
+      if (L.a[bk] != L.a[b]) refuse("beam at bond " + std::to_string(b) + ": its bonds differ in area");
       if (!close(L.d[bk], d, s)) refuse("beam at bond " + std::to_string(b) + ": d is not its length");
       const double* ek = k < 2 ? e1 : e2;
       const double sg = (k & 1) ? -r : r;
       for (int t = 0; t < 3; t++) {
-        if (!close(L.n[3 * bk + t], nv[t], 1.0)) refuse("beam at bond " + std::to_string(b) + ": n is not its direction");
+        if (!close(L.n[3 * bk + t], sn * nv[t], 1.0))
+          refuse("beam at bond " + std::to_string(b) + ": n is not its direction");
         if (!close(L.p[3 * bk + t], 0.5 * (xi[t] + xj[t]) + sg * ek[t], s))
           refuse("beam at bond " + std::to_string(b) + ": the bonds are not at middle +- r e1, +- r e2 in order");
       }
@@ -149,6 +163,7 @@ void lattice_cells(const StLattice& L, StLatticeCells& out) {
       g.push_back(0.0);
       rho.push_back(0.0);
       strength.resize(strength.size() + 45, 0.0);
+      section.resize(section.size() + 15, 1.0);
       have_mat.push_back(0);
     } else {
       ci_idx = it->second;
@@ -165,7 +180,11 @@ void lattice_cells(const StLattice& L, StLatticeCells& out) {
     if (beams[ci_idx] & (1u << q)) refuse("beam at bond " + std::to_string(b) + " appears twice");
     beams[ci_idx] |= (uint16_t)(1u << q);
     for (int t = 0; t < 3; t++) strength[45 * ci_idx + 3 * q + t] = m[3 + t];
-    for (int k = 0; k < 4; k++) beam_of_bond[b + k] = 15 * ci_idx + q;
+    section[15 * ci_idx + q] = area / full;
+    for (int kl = 0; kl < 4; kl++) {
+      const int64_t at = 60 * ci_idx + 4 * q + (flip && kl < 2 ? kl ^ 1 : kl);
+      bond_index[b + kl] = flip ? -1 - at : at;
+    }
   }
   const int64_t nfree = (int64_t)sites.size();
   // held cells: across each held node from a cell with beams (a held node lies between a
@@ -197,6 +216,9 @@ void lattice_cells(const StLattice& L, StLatticeCells& out) {
   out.n_cells = ncell;
   out.spacing = s;
   if (!out.sites) return;  // the sizing call
+  if (first_own >= 0 && !out.section)
+    refuse("beam at bond " + std::to_string(first_own) +
+           " has a section other than SECTION s^2 (a gathered lattice: give out->section for the sections)");
   for (int64_t k = 0; k < ncell; k++) {
     const SiteKey& c = sites[k];
     if (c.x < INT32_MIN || c.x > INT32_MAX || c.y < INT32_MIN || c.y > INT32_MAX || c.z < INT32_MIN || c.z > INT32_MAX)
@@ -212,8 +234,10 @@ void lattice_cells(const StLattice& L, StLatticeCells& out) {
     if (out.beams) out.beams[k] = f ? beams[k] : 0;
     if (out.strength)
       for (int t = 0; t < 45; t++) out.strength[45 * k + t] = f ? strength[45 * k + t] : 0.0;
+    if (out.section)
+      for (int t = 0; t < 15; t++) out.section[15 * k + t] = f ? section[15 * k + t] : 1.0;
   }
-  if (out.beam_of_bond) std::memcpy(out.beam_of_bond, beam_of_bond.data(), (size_t)nb * sizeof(int64_t));
+  if (out.bond_index) std::memcpy(out.bond_index, bond_index.data(), (size_t)nb * sizeof(int64_t));
 }
 
 }  // namespace st

@@ -110,12 +110,18 @@ __device__ __forceinline__ void pair_n(int q, const BeamGeo& G, double* n, int& 
   for (int k = 0; k < 3; k++) n[k] = ns * (double)pair_dir(q, k);
 }
 
+// Beam q of cell c: its section over SECTION s^2 (1 when the cells gave none).
+__device__ __forceinline__ double beam_sec(const double* csec, int64_t c, int q) {
+  return csec ? csec[15 * c + q] : 1.0;
+}
+
 // Own block of one beam end (6x6, row-major) added into D: sgn -1 for the beam's i end (face a), +1 for j.
-__device__ void beam_own_add(const BeamGeo& G, int q, int sgn, double e, double g, double* D) {
+// rho: the beam's section over SECTION s^2.
+__device__ void beam_own_add(const BeamGeo& G, int q, int sgn, double e, double g, double rho, double* D) {
   double n[3];
   int cls;
   pair_n(q, G, n, cls);
-  const double K4 = G.K4[cls], R2 = G.R2[cls], hd = G.hd[cls];
+  const double K4 = G.K4[cls] * rho, R2 = G.R2[cls] * (rho * rho), hd = G.hd[cls];
   const double kg = K4 * g, ke = K4 * (e - g);
   double nx[9] = {0, -n[2], n[1], n[2], 0, -n[0], -n[1], n[0], 0};  // [n]x
   for (int r = 0; r < 3; r++)
@@ -320,8 +326,8 @@ __global__ void k_sort_keys(int64_t n, const int* label, int* keys, int* vals) {
 // ------------------------------------------------------- fine smoother D^-1
 // D_v = sum of the own blocks of the live beams at node v (statics/src/bodies.rs:151-153).
 __global__ void k_dinv(int64_t nnodes, const uint8_t* nflags, const int32_t* ncell, const uint8_t* nkind,
-                       const double* ce, const double* cg, const uint16_t* cmask, const uint32_t* cstruct,
-                       const BeamGeo* bgeo, double* dinv64, float* dinv32, int* bad) {
+                       const double* ce, const double* cg, const double* csec, const uint16_t* cmask,
+                       const uint32_t* cstruct, const BeamGeo* bgeo, double* dinv64, float* dinv32, int* bad) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nnodes) return;
   double D[36], inv[36];
@@ -337,7 +343,7 @@ __global__ void k_dinv(int64_t nnodes, const uint8_t* nflags, const int32_t* nce
       const double e = ce[c], g = cg[c];
       const uint32_t m = (uint32_t)cmask[c] & face_beams(f);
       for (int q = 0; q < 15; q++)
-        if (m & (1u << q)) beam_own_add(G, q, kPairA[q] == f ? -1 : 1, e, g, D);
+        if (m & (1u << q)) beam_own_add(G, q, kPairA[q] == f ? -1 : 1, e, g, beam_sec(csec, c, q), D);
     }
     if (!invert6(D, inv)) {
       atomicExch(bad, 1);
@@ -628,8 +634,9 @@ __global__ void k_agg_struct(int64_t n, const int* rep, const int32_t* ncell, co
 // distinct neighbours; pass 1: fill blocks (row's diagonal first).
 __global__ void k_a1(int pass, int64_t n1, const int* agg_ptr, const int* nodes, const int* label, const double* centre,
                      const uint32_t* agg_struct, const uint32_t* nkey, const uint8_t* nkind, const int32_t* ncell,
-                     const uint32_t* cfaces, const double* ce, const double* cg, const uint16_t* cmask,
-                     const BeamGeo* bgeo, int* rowcnt, const int* rowptr, int* col, double* A, int* overflow) {
+                     const uint32_t* cfaces, const double* ce, const double* cg, const double* csec,
+                     const uint16_t* cmask, const BeamGeo* bgeo, int* rowcnt, const int* rowptr, int* col, double* A,
+                     int* overflow) {
   const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= n1) return;
   const BeamGeo G = bgeo[agg_struct[I]];
@@ -681,9 +688,10 @@ __global__ void k_a1(int pass, int64_t n1, const int* agg_ptr, const int* nodes,
         int cls;
         pair_n(q, G, n, cls);
         const int sv = kPairA[q] == f ? -1 : 1, sw = -sv;
+        const double rho = beam_sec(csec, c, q), K4 = G.K4[cls] * rho, R2 = G.R2[cls] * (rho * rho);
         double Gv[18];
         gmat(sv, n, G.hd[cls], rv, Gv);
-        beam_coarse_add(Gv, Gv, sv, sv, n, G.K4[cls], G.R2[cls], e, g, A + 36 * (int64_t)base);
+        beam_coarse_add(Gv, Gv, sv, sv, n, K4, R2, e, g, A + 36 * (int64_t)base);
         if (J >= 0) {
           double pw[3], rw[3];
           node_pos(wf, nkey, nkind, s, pw);
@@ -702,7 +710,7 @@ __global__ void k_a1(int pass, int64_t n1, const int* agg_ptr, const int* nodes,
                 break;
               }
           }
-          if (slot > 0) beam_coarse_add(Gv, Gw, sv, sw, n, G.K4[cls], G.R2[cls], e, g, A + 36 * ((int64_t)base + slot));
+          if (slot > 0) beam_coarse_add(Gv, Gw, sv, sw, n, K4, R2, e, g, A + 36 * ((int64_t)base + slot));
         }
       }
     }
@@ -1446,9 +1454,9 @@ __device__ int beams_at(uint32_t v, const uint8_t* nkind, const int32_t* ncell, 
 // reaching into J, K_b,vw Sh(r_w).
 __global__ void k_p0(int pass, int64_t nn, const uint8_t* nflags, const int* label, const double* centre,
                      const uint32_t* nkey, const uint8_t* nkind, const int32_t* ncell, const uint32_t* cfaces,
-                     const double* ce, const double* cg, const uint16_t* cmask, const uint32_t* cstruct,
-                     const BeamGeo* bgeo, const double* dinv, const double* step, int* cnt, const int* prow, int* pcol,
-                     double* pval, int* overflow) {
+                     const double* ce, const double* cg, const double* csec, const uint16_t* cmask,
+                     const uint32_t* cstruct, const BeamGeo* bgeo, const double* dinv, const double* step, int* cnt,
+                     const int* prow, int* pcol, double* pval, int* overflow) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nn) return;
   if (nflags[v] & (kHeld | kGone)) {
@@ -1495,16 +1503,17 @@ __global__ void k_p0(int pass, int64_t nn, const uint8_t* nflags, const int* lab
     int cls;
     pair_n(b.q, G, n, cls);
     const double e = ce[b.c], g = cg[b.c];
+    const double rho = beam_sec(csec, b.c, b.q), K4 = G.K4[cls] * rho, R2 = G.R2[cls] * (rho * rho);
     double Bv[18], Gv[18];
     gmat(b.sv, n, G.hd[cls], zero, Bv);
     gmat(b.sv, n, G.hd[cls], rv, Gv);
-    beam_coarse_add(Bv, Gv, b.sv, b.sv, n, G.K4[cls], G.R2[cls], e, g, out);  // K_b,vv Sh(r_v)
+    beam_coarse_add(Bv, Gv, b.sv, b.sv, n, K4, R2, e, g, out);  // K_b,vv Sh(r_v)
     if (Jw >= 0) {
       double pw[3], rw[3], Gw[18];
       node_pos(b.wf, nkey, nkind, sp, pw);
       for (int k = 0; k < 3; k++) rw[k] = pw[k] - centre[3 * (int64_t)Jw + k];
       gmat(-b.sv, n, G.hd[cls], rw, Gw);
-      beam_coarse_add(Bv, Gw, b.sv, -b.sv, n, G.K4[cls], G.R2[cls], e, g, out + 36 * slot_of(js, nj, Jw));
+      beam_coarse_add(Bv, Gw, b.sv, -b.sv, n, K4, R2, e, g, out + 36 * slot_of(js, nj, Jw));
     }
   }
   // P[v, J] = [J = I0] Sh(r_v) - w D^-1 (K chi_J)(v)
@@ -1828,7 +1837,8 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
                                                  const int* pblk_row, const int* prow, const int* pcol,
                                                  const double* pval, const uint8_t* nkind, const int32_t* ncell,
                                                  const uint32_t* cfaces, const double* ce, const double* cg,
-                                                 const uint16_t* cmask, const uint32_t* cstruct, const BeamGeo* bgeo,
+                                                 const double* csec, const uint16_t* cmask, const uint32_t* cstruct,
+                                                 const BeamGeo* bgeo,
                                                  int* rowcnt, const int* rowptr, int* col, double* A, int* overflow) {
   __shared__ int keys[kH];
   __shared__ int slot[kH];
@@ -1843,6 +1853,7 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
   __shared__ uint8_t s_o[kV];
   __shared__ uint32_t s_wf[10][kV];
   __shared__ double s_e[2][kV], s_g[2][kV];
+  __shared__ int s_c[2][kV];
   if ((int64_t)blockIdx.x >= n1) return;
   const int I = rows ? rows[blockIdx.x] : (int)blockIdx.x;
   for (int h = threadIdx.x; h < kH; h += blockDim.x) keys[h] = -1;
@@ -1868,6 +1879,7 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
           s_wf[5 * side + jj][t] =
               (mask & (1u << q)) ? cfaces[6 * c + (kPairA[q] == f ? kPairB[q] : kPairA[q])] : kDeadBeam;
         }
+        s_c[side][t] = c;
         if (c >= 0) {
           s_e[side][t] = ce[c];
           s_g[side][t] = cg[c];
@@ -1926,9 +1938,10 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
       int cls;
       pair_n(q, G, n, cls);
       const double e = s_e[side][vi], g = s_g[side][vi];
+      const double rho = beam_sec(csec, s_c[side][vi], q);
       for (int sd = 0; sd < (wfree ? 2 : 1); sd++) {
         double qr[6];  // row r of P0[v,I]^T K_b
-        beam_row(pc, sv, sd == 0 ? sv : -sv, n, G.hd[cls], G.K4[cls], G.R2[cls], e, g, qr);
+        beam_row(pc, sv, sd == 0 ? sv : -sv, n, G.hd[cls], G.K4[cls] * rho, G.R2[cls] * (rho * rho), e, g, qr);
         const int row = sd == 0 ? v : (int)wf;
         // start each task's column loop at a different column, so that the tasks of a
         // wave (mostly nodes of one aggregate) do not add into the same block at once
@@ -2166,7 +2179,7 @@ struct Solver {
     ST_HIP(hipMemsetAsync(sbad.p, 0, sizeof(int), st));
     if (nn)
       k_dinv<<<nblk(nn), 256, 0, st>>>(nn, g.nflags.as<uint8_t>(), g.ncell.as<int32_t>(), g.nkind.as<uint8_t>(),
-                                       g.ce64.as<double>(), g.cg64.as<double>(), g.cmask.as<uint16_t>(),
+                                       g.ce64.as<double>(), g.cg64.as<double>(), g.csec64.as<double>(), g.cmask.as<uint16_t>(),
                                        g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), c.dinv64.as<double>(),
                                        c.dinv32.as<float>(), sbad.as<int>());
     ST_HIP(hipGetLastError());
@@ -2248,7 +2261,7 @@ struct Solver {
       k_a1<<<nblk(n, 64), 64, 0, st>>>(pass, n, c.agg_ptr.as<int>(), c.nodes_by_agg.as<int>(), c.label1.as<int>(),
                                       lv.centre.as<double>(), lv.agg_struct.as<uint32_t>(), g.nkey.as<uint32_t>(),
                                       g.nkind.as<uint8_t>(), g.ncell.as<int32_t>(), g.cfaces.as<uint32_t>(),
-                                      g.ce64.as<double>(), g.cg64.as<double>(), g.cmask.as<uint16_t>(),
+                                      g.ce64.as<double>(), g.cg64.as<double>(), g.csec64.as<double>(), g.cmask.as<uint16_t>(),
                                       g.bgeo.as<BeamGeo>(), cnt.as<int>(), lv.rowptr.as<int>(), lv.col.as<int>(),
                                       lv.a64.as<double>(), sbad.as<int>());
       ST_HIP(hipGetLastError());
@@ -2375,7 +2388,7 @@ struct Solver {
                                                           lv.centre.as<double>(), g.nkey.as<uint32_t>(),
                                                           g.nkind.as<uint8_t>(), g.ncell.as<int32_t>(),
                                                           g.cfaces.as<uint32_t>(), g.ce64.as<double>(),
-                                                          g.cg64.as<double>(), g.cmask.as<uint16_t>(),
+                                                          g.cg64.as<double>(), g.csec64.as<double>(), g.cmask.as<uint16_t>(),
                                                           g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(),
                                                           c.dinv64.as<double>(), c.step0_d.as<double>(), cnt, prow,
                                                           pcol, pval, sbad.as<int>());
@@ -2386,7 +2399,7 @@ struct Solver {
                                                c.p0blk_row.as<int>(), c.p0row.as<int>(), c.p0col.as<int>(),         \
                                                c.p0val.as<double>(), g.nkind.as<uint8_t>(), g.ncell.as<int32_t>(),  \
                                                g.cfaces.as<uint32_t>(), g.ce64.as<double>(), g.cg64.as<double>(),   \
-                                               g.cmask.as<uint16_t>(), g.cstruct.as<uint32_t>(),                    \
+                                               g.csec64.as<double>(), g.cmask.as<uint16_t>(), g.cstruct.as<uint32_t>(), \
                                                g.bgeo.as<BeamGeo>(), cnt, lv.rowptr.as<int>(), lv.col.as<int>(),    \
                                                lv.a64.as<double>(), sbad.as<int>())
       if (pass == 0)
@@ -3333,8 +3346,8 @@ void release_hierarchy(Ctx& c) {
 
 void release_all(Ctx& c) {
   DevGeo& d = c.geo;
-  for (DBuf* b : {&d.cbase, &d.nbase, &d.bstruct, &d.blist, &d.ce64, &d.cg64, &d.ce32, &d.cg32,
-                  &d.cmask_user, &d.cmask, &d.cfaces, &d.ccoord, &d.cstruct, &d.nflags, &d.ncell, &d.nkind, &d.nkey,
+  for (DBuf* b : {&d.cbase, &d.nbase, &d.bstruct, &d.blist, &d.ce64, &d.cg64, &d.ce32, &d.cg32, &d.csec64,
+                  &d.csec32, &d.cmask_user, &d.cmask, &d.cfaces, &d.ccoord, &d.cstruct, &d.nflags, &d.ncell, &d.nkind, &d.nkey,
                   &d.bgeo})
     b->release();
   for (DBuf* b : {&c.fres, &c.xsol, &c.dinv64, &c.dinv32, &c.step0_d, &c.label1, &c.uf, &c.uf2, &c.flag, &c.scan_tmp, &c.sort_tmp,

@@ -126,6 +126,10 @@ void prepare(Prep& P) {
     else {
       if (!(C.e[i] > 0) || !(C.g[i] > 0) || !std::isfinite(C.e[i]) || !std::isfinite(C.g[i]))
         throw StError{ST_BAD_INPUT, "cells: a free cell has a non-positive or non-finite stiffness"};
+      if (C.section)
+        for (int q = 0; q < 15; q++)
+          if (!(C.section[15 * i + q] > 0) || !std::isfinite(C.section[15 * i + q]))
+            throw StError{ST_BAD_INPUT, "cells: a beam section is non-positive or non-finite"};
       o.freem[s >> 6] |= 1ull << (s & 63);
     }
   }
@@ -245,6 +249,8 @@ void fill(const Prep& P, int32_t si, HostGeo& hg, int64_t in_beg) {
         hg.cell_of_input[in_beg + in] = c;
         hg.ce[c] = C.e[in];
         hg.cg[c] = C.g[in];
+        if (C.section)
+          for (int q = 0; q < 15; q++) hg.csec[15 * c + q] = C.section[15 * in + q];
         hg.cmask[c] = C.beams ? (uint16_t)(C.beams[in] & 0x7fff) : (uint16_t)0x7fff;
         hg.ccoord[c] = (uint32_t)x | (uint32_t)y << 10 | (uint32_t)z << 20;
         hg.cstruct[c] = (uint32_t)si;
@@ -328,6 +334,9 @@ void build_host_geo(HostGeo& hg, const StCells* const* cells, int32_t n) {
   hg.nbase.assign(bricks + 1, 0);
   hg.bstruct.assign(bricks, 0);
   hg.ce.resize(ncells);
+  bool any_sec = false;
+  for (int32_t si = 0; si < n; si++) any_sec |= cells[si]->section != nullptr;
+  if (any_sec) hg.csec.assign(ncells * 15, 1.0);
   hg.cg.resize(ncells);
   hg.cmask.resize(ncells);
   hg.cfaces.resize(ncells * 6);

@@ -52,7 +52,7 @@ class GatheredLattice:
 class StCells(ctypes.Structure):
     _fields_ = [("n_cells", ctypes.c_int64), ("sites", ctypes.c_void_p), ("held", ctypes.c_void_p),
                 ("e", ctypes.c_void_p), ("g", ctypes.c_void_p), ("beams", ctypes.c_void_p),
-                ("spacing", ctypes.c_double)]
+                ("spacing", ctypes.c_double), ("section", ctypes.c_void_p)]
 
 
 class StGrid(ctypes.Structure):
@@ -64,7 +64,8 @@ class StGrid(ctypes.Structure):
 class StLatticeCells(ctypes.Structure):
     _fields_ = [("n_cells", ctypes.c_int64), ("spacing", ctypes.c_double), ("sites", ctypes.c_void_p),
                 ("held", ctypes.c_void_p), ("e", ctypes.c_void_p), ("g", ctypes.c_void_p), ("rho", ctypes.c_void_p),
-                ("beams", ctypes.c_void_p), ("strength", ctypes.c_void_p), ("beam_of_bond", ctypes.c_void_p)]
+                ("beams", ctypes.c_void_p), ("strength", ctypes.c_void_p), ("bond_index", ctypes.c_void_p),
+                ("section", ctypes.c_void_p)]
 
 
 class StSolveOpts(ctypes.Structure):
@@ -205,10 +206,12 @@ def lattice_arrays(lat):
     return arr, st
 
 
-def lattice_cells(lat):
-    """st_lattice_cells: the cells a built lattice comes from, as a dict of arrays (sites, held, e,
-    g, rho, beams, strength (cells, 15, 3), beam_of_bond) and spacing. Raises StaticsError with the
-    library's reason for a lattice build could not have made (a gathered one, say)."""
+def lattice_cells(lat, sections=True):
+    """st_lattice_cells: the cells a built or gathered lattice comes from, as a dict of arrays
+    (sites, held, e, g, rho, beams, strength (cells, 15, 3), section (cells, 15), bond_index) and
+    spacing. Raises StaticsError with the library's reason for a lattice neither build nor
+    gather::measured could have made. sections=False asks for no sections: a lattice whose
+    beams have sections of their own is then refused."""
     arr, st = lattice_arrays(lat)
     out = StLatticeCells()
     msg = ctypes.create_string_buffer(512)
@@ -218,9 +221,12 @@ def lattice_cells(lat):
     n = out.n_cells
     res = {"sites": np.zeros((n, 3), np.int32), "held": np.zeros(n, np.uint8), "e": np.zeros(n), "g": np.zeros(n),
            "rho": np.zeros(n), "beams": np.zeros(n, np.uint16), "strength": np.zeros((n, 15, 3)),
-           "beam_of_bond": np.zeros(len(arr["a"]), np.int64)}
-    for k in ("sites", "held", "e", "g", "rho", "beams", "strength", "beam_of_bond"):
+           "section": np.zeros((n, 15)), "bond_index": np.zeros(len(arr["a"]), np.int64)}
+    for k in ("sites", "held", "e", "g", "rho", "beams", "strength", "section", "bond_index"):
         setattr(out, k, res[k].ctypes.data)
+    if not sections:
+        out.section = None
+        del res["section"]
     rc = lib().st_lattice_cells(ctypes.byref(st), ctypes.byref(out), msg, 512)
     if rc != 0:
         raise StaticsError(rc, msg.value.decode())
@@ -256,16 +262,21 @@ class Context:
             raise StaticsError(rc, f"{what}: {self.L.st_last_error(self.h).decode()}")
 
     @staticmethod
-    def make_cells(sites, held, e, g, beams=None, spacing=UNIT):
+    def make_cells(sites, held, e, g, beams=None, spacing=UNIT, section=None):
         keep = [np.ascontiguousarray(sites, np.int32), np.ascontiguousarray(held, np.uint8),
                 np.ascontiguousarray(e, np.float64), np.ascontiguousarray(g, np.float64),
-                None if beams is None else np.ascontiguousarray(beams, np.uint16)]
+                None if beams is None else np.ascontiguousarray(beams, np.uint16),
+                None if section is None else np.ascontiguousarray(section, np.float64).reshape(-1, 15)]
+        if keep[5] is not None and len(keep[5]) != len(keep[0]):
+            raise ValueError("section: 15 per cell")
         c = StCells(len(keep[0]), keep[0].ctypes.data, keep[1].ctypes.data, keep[2].ctypes.data, keep[3].ctypes.data,
-                    None if keep[4] is None else keep[4].ctypes.data, spacing)
+                    None if keep[4] is None else keep[4].ctypes.data, spacing,
+                    None if keep[5] is None else keep[5].ctypes.data)
         return c, keep
 
-    def upload(self, sites, held, e, g, beams=None, spacing=UNIT):
-        c, keep = self.make_cells(sites, held, e, g, beams, spacing)
+    def upload(self, sites, held, e, g, beams=None, spacing=UNIT, section=None):
+        """st_upload. section: (cells, 15) beam sections over SECTION s^2, or None (all 1)."""
+        c, keep = self.make_cells(sites, held, e, g, beams, spacing, section)
         self._check(self.L.st_upload(self.h, ctypes.byref(c)), "st_upload")
         self._keep = keep
         n, nc, nb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

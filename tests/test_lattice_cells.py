@@ -7,7 +7,7 @@ import pytest
 
 import oracle as orc
 import statics_hip as sh
-from harness import FACES, PAIR_OF, mats_from
+from harness import FACES, PAIR_OF, PAIRS, mats_from
 
 
 def built(n=10, seed=3, level=0):
@@ -35,7 +35,9 @@ def test_cells_of_a_built_lattice(level):
     assert {tuple(s) for s, h in zip(c["sites"].tolist(), c["held"]) if h} == want_held
     # each bond's beam: its cell (the oracle's beam_cell) and its pair
     cell_of_site = {tuple(s): i for i, s in enumerate(sites.tolist())}
-    bob = c["beam_of_bond"]
+    bi = c["bond_index"]
+    assert (bi >= 0).all() and (bi % 4 == np.tile(np.arange(4), L.n_bonds // 4)).all()  # build's order
+    bob = bi // 4
     cell_dev = bob // 15
     assert (bob[0::4] == bob[1::4]).all() and (bob[0::4] == bob[3::4]).all()
     for b in range(0, L.n_bonds, 4 * 7):
@@ -46,12 +48,57 @@ def test_cells_of_a_built_lattice(level):
     assert nbeams == L.n_bonds // 4
 
 
-def test_gathered_lattice_is_refused():
+def test_gathered_lattice_needs_sections():
+    """Without room for the sections, a gathered lattice is refused (its beams have sections
+    of their own, statics/src/gather.rs:386-390); a built one is not."""
     L, *_ = built(n=12)
     G = L.gathered()
     with pytest.raises(sh.StaticsError) as e:
-        sh.lattice_cells(G)
+        sh.lattice_cells(G, sections=False)
     assert e.value.code == sh.ST_BAD_INPUT and "section" in str(e.value)
+    c = sh.lattice_cells(L, sections=False)
+    assert "section" not in c and c["beams"].any()
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+def test_gathered_lattice_cells(levels):
+    """A lattice gathered once and twice: each beam found in its cell at the pair its end nodes
+    are, with its own section (4 a over SECTION s^2) and strengths, and its cell's e and g."""
+    L, *_ = built(n=16, seed=4)
+    for _ in range(levels):
+        L = L.gathered()
+    c = sh.lattice_cells(L)
+    s = sh.UNIT * 2 ** L.level
+    assert c["spacing"] == s
+    bi = c["bond_index"]
+    flip = bi < 0
+    at = np.where(flip, -1 - bi, bi)
+    cell, q = at // 60, at // 4 % 15
+    # a beam's four bonds: one beam, all its bonds one way, the places 0..3 each once
+    assert (cell.reshape(-1, 4) == cell[0::4, None]).all() and (q.reshape(-1, 4) == q[0::4, None]).all()
+    assert (flip.reshape(-1, 4) == flip[0::4, None]).all() and flip.any() and not flip.all()
+    assert (np.sort((at % 4).reshape(-1, 4), axis=1) == np.arange(4)).all()
+    # run from the greater face, the first two bonds swap places (n and e1 negated, e2 kept)
+    want = np.tile(np.arange(4), L.n_bonds // 4)
+    want = np.where(flip & (want < 2), want ^ 1, want)
+    assert (at % 4 == want).all()
+    assert (c["beams"][cell] >> q & 1).all()
+    assert sum(bin(int(b)).count("1") for b in c["beams"]) == L.n_bonds // 4
+    sec = c["section"][cell, q]
+    np.testing.assert_allclose(sec * 0.336 * s * s, 4 * L.a, rtol=1e-15)
+    assert np.ptp(sec) > 0.5  # the sections differ beam by beam
+    np.testing.assert_array_equal(c["strength"][cell, q], L.bond_mat[:, 3:6])
+    np.testing.assert_array_equal(c["e"][cell], L.bond_mat[:, 1])
+    np.testing.assert_array_equal(c["g"][cell], L.bond_mat[:, 2])
+    # beams' ends: the two faces of the pair at the cell's site
+    site = c["sites"][cell].astype(np.int64)
+    for end in (0, 1):
+        v = np.where(flip, L.ij[:, 1 - end], L.ij[:, end])  # the node at the pair's lesser face, then greater
+        f = np.array([FACES[PAIRS[int(t)][end]] for t in q])
+        lo = site + np.minimum(f, 0)
+        hi = lo + np.abs(f)
+        np.testing.assert_array_equal(L.around[v, 0] >> L.level, lo)
+        np.testing.assert_array_equal(L.around[v, 1] >> L.level, hi)
 
 
 def test_broken_beams_are_refused():
@@ -104,6 +151,6 @@ def test_as_generated_strengths_per_beam():
     strong = need > 1.0
     assert np.allclose(u1.reshape(-1, 4).max(axis=1)[strong], 0.5, rtol=1e-9)
     c = sh.lattice_cells(G)
-    bob = c["beam_of_bond"][0::4]
+    bob = c["bond_index"][0::4] // 4
     s = c["strength"].reshape(-1, 3)[bob]
     assert (s[:, 0] == need).all() and (s[:, 1] == need).all() and (s[:, 2] == need).all()
