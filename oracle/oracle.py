@@ -11,6 +11,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "statics_oracle.cpp")
+MECH_SRC = os.path.join(HERE, "mechanics_oracle.cpp")
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 CPU_TAG = os.path.join(HERE, "liboracle.cpu")  # the host CPU the library was built for (-march=native)
 
@@ -41,7 +42,7 @@ def build():
     contraction, as Rust does not contract)."""
     tmp = LIB_PATH + ".tmp"
     cmd = ["g++", "-O3", "-march=native", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o",
-           tmp, SRC]
+           tmp, SRC, MECH_SRC]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB_PATH)
     with open(CPU_TAG, "w") as f:
@@ -50,7 +51,8 @@ def build():
 
 
 def _stale():
-    if not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(SRC):
+    if not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < max(os.path.getmtime(SRC),
+                                                                       os.path.getmtime(MECH_SRC)):
         return True
     try:
         with open(CPU_TAG) as f:
@@ -104,6 +106,19 @@ def lib():
         L.orc_energy.argtypes = [vp, vp, vp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_threads.restype = ctypes.c_int
+        L.mor_new.restype = vp
+        L.mor_new.argtypes = [ctypes.c_int64, vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_double]
+        L.mor_free.argtypes = [vp]
+        L.mor_solve.restype = ctypes.c_int64
+        L.mor_solve.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64, vp, vp, vp, vp, ctypes.c_double]
+        L.mor_precondition.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64, vp, vp, vp, vp]
+        L.mor_times.argtypes = [ctypes.c_int64, vp, ctypes.c_int64, vp, vp, vp, vp]
+        L.mor_levels.restype = ctypes.c_int
+        L.mor_levels.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.mor_level_map.argtypes = [vp, ctypes.c_int, vp, vp]
+        L.mor_lift.argtypes = [vp, vp, vp, vp]
+        L.mor_inverse6.argtypes = [vp, vp]
+        L.mor_unit_k.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -303,3 +318,99 @@ def coarse_site(p, l):
 def spacing(level):
     """statics/src/lattice.rs:34-36"""
     return lib().orc_spacing(level)
+
+
+# ---------------------------------------------------------------- mechanics::solve
+class MechSystem:
+    """mechanics::solve::System (mechanics/src/solve.rs:49-59) as arrays: sites (n, 3) int64,
+    at (n, 3), diag (n, 3, 3), pairs ij (m, 2) uint32 and their blocks (m, 3, 3)."""
+
+    def __init__(self, sites, at, diag, ij, blocks):
+        self.sites = np.ascontiguousarray(sites, np.int64).reshape(-1, 3)
+        self.at = np.ascontiguousarray(at, np.float64).reshape(-1, 3)
+        self.diag = np.ascontiguousarray(diag, np.float64).reshape(-1, 3, 3)
+        self.ij = np.ascontiguousarray(ij, np.uint32).reshape(-1, 2)
+        self.blocks = np.ascontiguousarray(blocks, np.float64).reshape(-1, 3, 3)
+
+    @property
+    def n(self):
+        return len(self.diag)
+
+    def times(self, x):
+        """System::times (solve.rs:61-72)."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        y = np.zeros_like(x)
+        lib().mor_times(self.n, _p(self.diag), len(self.ij), _p(self.ij), _p(self.blocks), _p(x), _p(y))
+        return y
+
+    def dense(self):
+        """The system as a dense (3n, 3n) matrix (tests' brute force)."""
+        n = self.n
+        a = np.zeros((3 * n, 3 * n))
+        for i in range(n):
+            a[3 * i:3 * i + 3, 3 * i:3 * i + 3] += self.diag[i]
+        for (i, j), b in zip(self.ij, self.blocks):
+            a[3 * i:3 * i + 3, 3 * j:3 * j + 3] += b
+            a[3 * j:3 * j + 3, 3 * i:3 * i + 3] += b.T
+        return a
+
+
+class MechSolver:
+    """mechanics::solve::Solver (solve.rs:379-537): new(system, size), solve(system, b, x, tol)."""
+
+    def __init__(self, system, size):
+        s = system
+        self.h = ctypes.c_void_p(lib().mor_new(s.n, _p(s.sites), _p(s.at), _p(s.diag), len(s.ij), _p(s.ij),
+                                               _p(s.blocks), size))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mor_free(self.h)
+            self.h = None
+
+    def solve(self, system, b, x0=None, tol=1e-3):
+        """(x, rounds) with rounds None where the reference returns None."""
+        s = system
+        b = np.ascontiguousarray(b, np.float64).reshape(-1, 3)
+        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, np.float64).reshape(-1, 3).copy()
+        r = lib().mor_solve(self.h, s.n, _p(s.diag), len(s.ij), _p(s.ij), _p(s.blocks), _p(b), _p(x), tol)
+        return x, (None if r < 0 else int(r))
+
+    def precondition(self, system, b):
+        s = system
+        b = np.ascontiguousarray(b, np.float64).reshape(-1, 3)
+        z = np.zeros_like(b)
+        lib().mor_precondition(self.h, s.n, _p(s.diag), len(s.ij), _p(s.ij), _p(s.blocks), _p(b), _p(z))
+        return z
+
+    def levels(self, cap=64):
+        """[(parts, pairs, step, direct)] per coarse level, and the finest level's step."""
+        parts, npairs = np.zeros(cap, np.int64), np.zeros(cap, np.int64)
+        steps, direct, step0 = np.zeros(cap), np.zeros(cap, np.int32), ctypes.c_double()
+        n = lib().mor_levels(self.h, cap, _p(parts), _p(npairs), _p(steps), _p(direct), ctypes.byref(step0))
+        return [(int(parts[l]), int(npairs[l]), float(steps[l]), bool(direct[l])) for l in range(n)], step0.value
+
+    def level_map(self, l, n_finer):
+        of, r = np.zeros(n_finer, np.uint32), np.zeros((n_finer, 3))
+        lib().mor_level_map(self.h, l, _p(of), _p(r))
+        return of, r
+
+
+def mech_lift(m, r, s):
+    m = np.ascontiguousarray(m, np.float64)
+    out = np.zeros((6, 6))
+    lib().mor_lift(_p(m), _p(np.asarray(r, np.float64)), _p(np.asarray(s, np.float64)), _p(out))
+    return out
+
+
+def mech_inverse6(m):
+    out = np.zeros((6, 6))
+    lib().mor_inverse6(_p(np.ascontiguousarray(m, np.float64)), _p(out))
+    return out
+
+
+def mech_unit_k():
+    """The unit cube's stiffness (mechanics/src/element.rs:49-76): (8, 8, 3, 3)."""
+    k = np.zeros((8, 8, 3, 3))
+    lib().mor_unit_k(_p(k))
+    return k

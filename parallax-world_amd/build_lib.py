@@ -22,7 +22,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-u
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 
-SOURCES = ["st_device.hip", "st_solver.hip", "st_api.hip", "st_grid.hip", "st_gather.hip", "st_upload.cpp", "st_generate.cpp", "st_lattice.cpp"]
+SOURCES = ["st_device.hip", "st_solver.hip", "st_api.hip", "st_grid.hip", "st_gather.hip", "st_upload.cpp", "st_generate.cpp", "st_lattice.cpp",
+           "me_solver.hip"]
 
 
 def _compile(src):

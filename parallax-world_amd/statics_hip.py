@@ -24,6 +24,8 @@ EXPORTS = ["st_create", "st_destroy", "st_last_error", "st_upload", "st_sizes", 
            "st_set_loads", "st_solve_resident", "st_get_solution", "st_classify", "st_kernel_timing_enable",
            "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_bond_used", "st_settle", "st_grid_cells", "st_gather", "st_gather_read", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness",
            "st_joined_to_held_static", "st_lattice_cells", "st_joined_edges"]
+# The entry points include/mechanics_hip.h declares.
+ME_EXPORTS = ["me_solver_new", "me_solve", "me_solver_free", "me_last_error", "me_solver_info"]
 
 UNIT = 0.125  # metres to a level-0 cell (statics/src/lattice.rs:16)
 
@@ -66,6 +68,11 @@ class StLatticeCells(ctypes.Structure):
                 ("held", ctypes.c_void_p), ("e", ctypes.c_void_p), ("g", ctypes.c_void_p), ("rho", ctypes.c_void_p),
                 ("beams", ctypes.c_void_p), ("strength", ctypes.c_void_p), ("bond_index", ctypes.c_void_p),
                 ("section", ctypes.c_void_p)]
+
+
+class MeSystem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("sites", ctypes.c_void_p), ("at", ctypes.c_void_p), ("diag", ctypes.c_void_p),
+                ("npairs", ctypes.c_int64), ("pairs", ctypes.c_void_p), ("blocks", ctypes.c_void_p)]
 
 
 class StSolveOpts(ctypes.Structure):
@@ -143,6 +150,18 @@ def lib():
             f.restype = ctypes.c_int
         L.st_last_error.argtypes = [vp]
         L.st_last_error.restype = ctypes.c_char_p
+        me = {
+            "me_solver_new": [ctypes.POINTER(vp), ctypes.POINTER(MeSystem), dbl, ctypes.c_int],
+            "me_solve": [vp, ctypes.POINTER(MeSystem), vp, vp, dbl, vp],
+            "me_solver_free": [vp],
+            "me_solver_info": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp],
+        }
+        for name, args in me.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.me_last_error.argtypes = [vp]
+        L.me_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -472,3 +491,61 @@ class Context:
         if want_x:
             xs = [x[:stats[i].nodes] for i, x in enumerate(xs)]
         return xs, [stats[i] for i in range(n)]
+
+
+class MechSolver:
+    """mechanics::solve::Solver on the device (include/mechanics_hip.h): made for a system
+    (sites, at, diag, pairs ij, blocks as numpy arrays, the shapes of mechanics::solve::System),
+    then solve(system, b, x0, tol) -> (x, rounds or None)."""
+
+    @staticmethod
+    def _system(s):
+        keep = [np.ascontiguousarray(s.sites, np.int64), np.ascontiguousarray(s.at, np.float64),
+                np.ascontiguousarray(s.diag, np.float64), np.ascontiguousarray(s.ij, np.uint32),
+                np.ascontiguousarray(s.blocks, np.float64)]
+        m = MeSystem(len(keep[2]), keep[0].ctypes.data, keep[1].ctypes.data, keep[2].ctypes.data, len(keep[3]),
+                     keep[3].ctypes.data if len(keep[3]) else None, keep[4].ctypes.data if len(keep[4]) else None)
+        return m, keep
+
+    def __init__(self, system, size, device=0):
+        self.L = lib()
+        m, keep = self._system(system)
+        h = ctypes.c_void_p()
+        rc = self.L.me_solver_new(ctypes.byref(h), ctypes.byref(m), size, device)
+        if rc != 0:
+            raise StaticsError(rc, "me_solver_new (the reason is on stderr)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.me_solver_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, system, b, x0=None, tol=1e-3):
+        m, keep = self._system(system)
+        b = np.ascontiguousarray(b, np.float64).reshape(-1, 3)
+        x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, np.float64).reshape(-1, 3).copy()
+        rounds = ctypes.c_int32()
+        rc = self.L.me_solve(self.h, ctypes.byref(m), _p(b), _p(x), tol, ctypes.byref(rounds))
+        if rc != 0:
+            raise StaticsError(rc, f"me_solve: {self.L.me_last_error(self.h).decode()}")
+        return x, (None if rounds.value < 0 else rounds.value)
+
+    def info(self, cap=64):
+        count = ctypes.c_int32()
+        parts, pairs = np.zeros(cap, np.int64), np.zeros(cap, np.int64)
+        steps, direct = np.zeros(cap), np.zeros(cap, np.int32)
+        step0, setup_ms, solve_ms = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        rc = self.L.me_solver_info(self.h, cap, ctypes.byref(count), _p(parts), _p(pairs), _p(steps), _p(direct),
+                                   ctypes.byref(step0), ctypes.byref(setup_ms), ctypes.byref(solve_ms))
+        if rc != 0:
+            raise StaticsError(rc, "me_solver_info")
+        n = count.value
+        return {"levels": [(int(parts[l]), int(pairs[l]), float(steps[l]), bool(direct[l])) for l in range(n)],
+                "step0": step0.value, "setup_ms": setup_ms.value, "solve_ms": solve_ms.value}

@@ -27,6 +27,17 @@ def test_exports_every_declared_symbol():
     assert set(names) == set(sh.EXPORTS)
 
 
+def test_exports_every_mechanics_symbol():
+    """include/mechanics_hip.h's entry points are exported and bound."""
+    with open(os.path.join(ROOT, "include", "mechanics_hip.h")) as f:
+        names = sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(me_\w+)\s*\(", f.read(), re.M)))
+    lib = ctypes.CDLL(sh.LIB_PATH)
+    assert len(names) == 5
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(sh.ME_EXPORTS)
+
+
 def test_no_gpu_fails_loudly():
     import torch
     if torch.cuda.is_available():
