@@ -178,11 +178,14 @@ def test_solve_parity_reference_scenes(ctx, scene):
     assert abs(u_dev.max() - u_or.max()) <= 1e-5 * u_or.max()
 
 
+@pytest.mark.parametrize("levels", ["2", "8"])
 @pytest.mark.parametrize("case", ["generated", "stone cantilever", "portico"])
-def test_half_coarse_operators(ctx, monkeypatch, case):
-    """The mixed V-cycle's coarse operators stored in FP16 (block-scaled, the default) against
-    the oracle: the solves meet the parity bar, on the stiff stone scenes too."""
+def test_half_coarse_operators(ctx, monkeypatch, case, levels):
+    """The mixed V-cycle's coarse operators stored in FP16 (block-scaled; by default on the two
+    largest coarse levels, here also on every FP32 one) against the oracle: the solves meet
+    the parity bar, on the stiff stone scenes too."""
     monkeypatch.setenv("STATICS_HIP_HALF", "1")
+    monkeypatch.setenv("STATICS_HIP_HALF_LEVELS", levels)
     if case == "generated":
         P = generated(ctx, 32, 3)
         f = gravity_z(P.L)
@@ -193,7 +196,7 @@ def test_half_coarse_operators(ctx, monkeypatch, case):
     gone = classify_both(P)
     x_or, _ = P.L.motions(f, gone=gone, tol=1e-13, max_iter=5000)
     x_dev, st = ctx.solve(P.to_dev(f), precision=sh.MIXED, tol=1e-8)
-    print(f"half coarse operators, {case}: {st.iters} iterations, restarts {st.retries}")
+    print(f"half coarse operators on {levels} levels, {case}: {st.iters} iterations, restarts {st.retries}")
     assert st.status == 0 and rel(P.from_dev(x_dev), x_or) < 1e-6
 
 
