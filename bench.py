@@ -619,6 +619,105 @@ def gather_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def mechanics_system(n, seed):
+    """The mechanics system of the bench generator's n^3 structure (mech_systems.system: the
+    engine's assembly, mechanics/src/lib.rs:638-676): every solid cell a wood cube of 0.125 m,
+    turned up to 0.2 rad, the corners of the held cells held; dt 1/60."""
+    import mech_systems as ms
+    occ, _ = structure(n, seed)
+    sites = np.argwhere(occ > 0)
+    hc = np.argwhere(occ == 2)
+    side = n + 2
+    code = lambda k: (k[:, 0] * side + k[:, 1]) * side + k[:, 2]  # noqa: E731
+    held_codes = np.unique(code((hc[:, None, :] + ms.CORNERS[None]).reshape(-1, 3)))
+    held = lambda k: np.isin(code(k), held_codes)  # noqa: E731
+    s, b, _, _ = ms.system(sites, held, h=sh.UNIT, turn=0.2, seed=seed)
+    return s, b, len(sites)
+
+
+def mechanics_cpu_sample(n, seed):
+    """The oracle's Solver::new + solve to 1e-3 on the n^3 system (one thread, as the reference's)."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as orc
+    s, b, cells = mechanics_system(n, seed)
+    o_sys = orc.MechSystem(s.sites, s.at, s.diag, s.ij, s.blocks)
+    v0 = np.random.default_rng(seed + 1).normal(size=b.shape) * 1e-2  # as mechanics_bench starts
+    orc.lib()  # built (for this host's CPU) before the clock starts
+    t0 = time.perf_counter()
+    o = orc.MechSolver(o_sys, sh.UNIT)
+    _, rounds = o.solve(o_sys, b, x0=v0, tol=1e-3)
+    return {"seconds": time.perf_counter() - t0, "unknowns": s.n, "rounds": rounds, "cells": cells}
+
+
+def mechanics_bench(args):
+    """mechanics::solve::Solver on the device for the system of one --mechanics^3 structure of
+    the bench generator: a step is what the engine does when its kept levels do not fit
+    (mechanics/src/lib.rs:741-746): Solver::new, then solve to the engine's 1e-3 from the
+    current speeds. Reported beside it: the solve alone, and a solve to 1e-8."""
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    n = args.mechanics
+    t = time.perf_counter()
+    s, b, cells = mechanics_system(n, args.seed)
+    log(f"[bench] mechanics {n}^3: {cells} cells, {s.n} unknowns, {len(s.ij)} pairs "
+        f"(built in {time.perf_counter() - t:.1f}s)")
+    v0 = np.random.default_rng(args.seed + 1).normal(size=b.shape) * 1e-2
+    times, solve_ms, rounds, solver = [], [], None, None
+    for i in range(args.warmup + args.steps):
+        if solver is not None:
+            solver.close()
+        t0 = time.perf_counter()
+        solver = sh.MechSolver(s, sh.UNIT)
+        t1 = time.perf_counter()
+        if i >= args.warmup:
+            solver.kernel_timing(1)
+        x, rounds = solver.solve(s, b, x0=v0, tol=1e-3)
+        t2 = time.perf_counter()
+        log(f"[bench] mechanics step {i}: new {1e3 * (t1 - t0):.1f} ms, solve {1e3 * (t2 - t1):.1f} ms, "
+            f"{rounds} rounds")
+        if i >= args.warmup:
+            times.append(t2 - t0)
+            solve_ms.append(1e3 * (t2 - t1))
+            launches, kp_ms, kp_bytes = solver.kernel_timing(-1)
+            solver.kernel_timing(0)
+    info = solver.info()
+    ms_step = 1e3 * sum(times) / len(times)
+    t0 = time.perf_counter()
+    _, rounds_tight = solver.solve(s, b, x0=v0, tol=1e-8)
+    tight_ms = 1e3 * (time.perf_counter() - t0)
+    avg = kp_ms / max(launches, 1)
+    roof = {"bound": "hbm", "kernel": "k_spmv<3, 0> (the CG's K p on the three-unknown level: a row of 3x3 blocks "
+                                      "per unknown, both halves of each pair)",
+            "achieved": kp_bytes / (avg * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+            "frac": kp_bytes / (avg * 1e-3) / 1e9 / 8000.0, "traffic": None, "launches": launches,
+            "avg_ms": avg, "algorithmic_bytes_per_launch": kp_bytes}
+    cpu = None
+    if args.mechanics_cpu_sample > 0:
+        m = min(args.mechanics_cpu_sample, n)
+        log(f"[bench] mechanics: the oracle on the {m}^3 system")
+        cb = mechanics_cpu_sample(m, args.seed)
+        scale = (cb["unknowns"] * cb["rounds"]) / (s.n * rounds) if m != n else 1.0
+        cpu = {"value": scale / cb["seconds"], "unit": f"steps/sec ({n}^3" + (", scaled from the sample)" if m != n else ")"),
+               "cores": 1, "kind": "port",
+               "sample": f"oracle (C++ FP64 restatement of mechanics/src/solve.rs, one thread as the reference's "
+                         f"Solver) Solver::new + solve to 1e-3 on the {m}^3 system of the same generator: "
+                         f"{cb['seconds']:.2f} s, {cb['rounds']} rounds, {cb['unknowns']} unknowns"
+                         + (f"; scaled by unknowns x rounds to the {n}^3 system's {s.n} x {rounds}" if m != n else "")}
+    out = {
+        "metric": f"mechanics solver steps/sec (Solver::new + solve to 1e-3) for the {n}^3 structure's system",
+        "value": 1e3 / ms_step, "unit": "steps/sec", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"the bench generator's {n}^3 pillar/slab structure as co-rotated wood cubes of "
+                               f"0.125 m (turned up to 0.2 rad), the held cells' corners held, dt 1/60: "
+                               f"(M + (dt^2 + 0.3 dt) K) v' = M v + dt M g from random speeds",
+                   "lattice": n, "cells": cells, "unknowns": int(s.n), "pairs": int(len(s.ij))},
+        "rounds": rounds, "solve_ms": sum(solve_ms) / len(solve_ms), "setup_ms": info["setup_ms"],
+        "levels": [lv[0] for lv in info["levels"]], "tight": {"tol": 1e-8, "rounds": rounds_tight, "ms": tight_ms},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -652,6 +751,9 @@ def main():
     ap.add_argument("--settle-cpu-sample", type=int, default=32, help="settle: edge of the CPU oracle's sample")
     ap.add_argument("--settle-quantile", type=float, default=0.999,
                     help="settle: every strength at this quantile of the standing structure's used shares")
+    ap.add_argument("--mechanics", type=int, default=0, metavar="N",
+                    help="instead: mechanics::solve::Solver on the device for the N^3 structure's system")
+    ap.add_argument("--mechanics-cpu-sample", type=int, default=48, help="mechanics: edge of the oracle's sample")
     ap.add_argument("--copy-calib", action="store_true",
                     help="end with three device copies of 1 GiB (the calibration kernel of the traffic counters)")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -675,6 +777,9 @@ def main():
         return
     if args.settle:
         settle_bench(args)
+        return
+    if args.mechanics:
+        mechanics_bench(args)
         return
     if args.gather:
         gather_bench(args)

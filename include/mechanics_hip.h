@@ -54,9 +54,17 @@ const char* me_last_error(const MeSolver* s);
 
 /* No counterpart in the reference: the levels made (count; per level up to cap: parts,
  * pairs, smoothing step, whether solved directly), the finest level's step, and the
- * device time of the last setup and solve, milliseconds. Any pointer may be NULL. */
+ * wall-clock time of the last setup and solve, milliseconds (host bookkeeping and
+ * copies included). Any pointer may be NULL. */
 int me_solver_info(const MeSolver* s, int32_t cap, int32_t* count, int64_t* parts, int64_t* pairs, double* steps,
                    int32_t* direct, double* step0, double* setup_ms, double* solve_ms);
+
+/* No counterpart in the reference: device time of the CG's K p (the level-0 product of
+ * me_solve, one launch per round) for measuring it against its roofline. enable 1 starts
+ * recording (clearing the totals), 0 stops, -1 only reads: launches and summed
+ * milliseconds so far, and the algorithmic bytes of one launch (the diagonal blocks, row
+ * pointers, column indices and pair blocks read once, x read once, the product written). */
+int me_kernel_timing(MeSolver* s, int enable, int64_t* launches, double* total_ms, double* bytes_per_launch);
 
 #ifdef __cplusplus
 }

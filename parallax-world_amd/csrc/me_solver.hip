@@ -503,8 +503,16 @@ struct Solver {
   DBuf x, r, z, p, ap, bvec, t1, t2, part, scal;
   double* hscal = nullptr;
   double setup_ms = 0.0, solve_ms = 0.0;
+  // device time of the CG's K p (the level-0 product), when asked for: event pairs around
+  // each launch, folded into the totals at the end of each solve
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  size_t tev_used = 0;
+  int64_t t_launches = 0;
+  double t_ms = 0.0;
 
   ~Solver();
+  void timing_collect();
   void upload_system(const MeSystem& s, bool setup);
   double power_step(int B, int64_t nn, const double* diag, const Csr& c, const double* inv, DBuf& v, DBuf& a, DBuf& w);
   void coarsen(const std::vector<Key>& keys, const std::vector<P3>& at, const std::vector<std::array<uint32_t, 2>>& fij,
@@ -540,8 +548,19 @@ void release_level(Level& L) {
     b->release();
 }
 
+void Solver::timing_collect() {
+  for (size_t i = 0; i < tev_used; i++) {
+    float ms = 0.0f;
+    ST_HIP(hipEventElapsedTime(&ms, tev[2 * i], tev[2 * i + 1]));
+    t_ms += ms;
+    t_launches++;
+  }
+  tev_used = 0;
+}
+
 Solver::~Solver() {
   if (stream) (void)hipStreamSynchronize(stream);
+  for (hipEvent_t e : tev) (void)hipEventDestroy(e);
   for (Level& L : levels) release_level(L);
   for (DBuf* b : {&inv0, &diag0, &blocks0, &ij0, &csr0.rowptr, &csr0.col, &csr0.src, &csr0.val, &x, &r, &z, &p, &ap,
                   &bvec, &t1, &t2, &part, &scal})
@@ -923,6 +942,7 @@ int32_t Solver::solve(const MeSystem& s, const double* bh, double* xh, double to
   auto finish = [&](int32_t rounds) {
     ST_HIP(hipMemcpyAsync(xh, x.p, n3 * sizeof(double), hipMemcpyDeviceToHost, stream));
     ST_HIP(hipStreamSynchronize(stream));
+    timing_collect();
     solve_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rounds;
   };
@@ -940,10 +960,19 @@ int32_t Solver::solve(const MeSystem& s, const double* bh, double* xh, double to
   for (int round = 0; round < kRounds; round++) {
     if (rz < 0.0) return finish(-1);
     if (rz <= target) return finish(round);
+    if (timing) {
+      while (tev.size() < 2 * tev_used + 2) {
+        hipEvent_t e;
+        ST_HIP(hipEventCreate(&e));
+        tev.push_back(e);
+      }
+      ST_HIP(hipEventRecord(tev[2 * tev_used], stream));
+    }
     k_spmv<3, 0><<<nblk(n), kBlock, 0, stream>>>(n, diag0.as<double>(), csr0.rowptr.as<int64_t>(),
                                                   csr0.col.as<int32_t>(), csr0.val.as<double>(), p.as<double>(),
                                                   nullptr, nullptr, 0.0, ap.as<double>());
     ST_HIP(hipGetLastError());
+    if (timing) ST_HIP(hipEventRecord(tev[2 * tev_used++ + 1], stream));
     dot(p.as<double>(), ap.as<double>(), n3, PAP, false);
     k_cg_update<<<nblk(n3), kBlock, 0, stream>>>(n3, p.as<double>(), ap.as<double>(), x.as<double>(),
                                                   r.as<double>(), scal.as<double>());
@@ -1041,4 +1070,24 @@ extern "C" int me_solver_info(const MeSolver* s, int32_t cap, int32_t* count, in
   if (setup_ms) *setup_ms = s->s.setup_ms;
   if (solve_ms) *solve_ms = s->s.solve_ms;
   return ST_OK;
+}
+
+extern "C" int me_kernel_timing(MeSolver* s, int enable, int64_t* launches, double* total_ms,
+                                double* bytes_per_launch) {
+  return guard(s, [&] {
+    me::Solver& v = s->s;
+    if (enable > 0) {
+      v.timing = true;
+      v.t_launches = 0;
+      v.t_ms = 0.0;
+    } else if (enable == 0) {
+      v.timing = false;
+    }
+    if (launches) *launches = v.t_launches;
+    if (total_ms) *total_ms = v.t_ms;
+    // read: the diagonal blocks, row pointers, column indices and blocks once, x once, y
+    // once; written: y
+    if (bytes_per_launch)
+      *bytes_per_launch = (double)v.n * (72.0 + 8.0 + 24.0 + 24.0) + 8.0 + (double)v.csr0.nslots * (4.0 + 72.0);
+  });
 }

@@ -25,7 +25,7 @@ EXPORTS = ["st_create", "st_destroy", "st_last_error", "st_upload", "st_sizes", 
            "st_kernel_timing", "st_apply_schedule", "st_bond_forces", "st_bond_used", "st_settle", "st_grid_cells", "st_gather", "st_gather_read", "st_backward", "st_solve_batch", "st_generate", "st_set_stiffness",
            "st_joined_to_held_static", "st_lattice_cells", "st_joined_edges"]
 # The entry points include/mechanics_hip.h declares.
-ME_EXPORTS = ["me_solver_new", "me_solve", "me_solver_free", "me_last_error", "me_solver_info"]
+ME_EXPORTS = ["me_solver_new", "me_solve", "me_solver_free", "me_last_error", "me_solver_info", "me_kernel_timing"]
 
 UNIT = 0.125  # metres to a level-0 cell (statics/src/lattice.rs:16)
 
@@ -155,6 +155,7 @@ def lib():
             "me_solve": [vp, ctypes.POINTER(MeSystem), vp, vp, dbl, vp],
             "me_solver_free": [vp],
             "me_solver_info": [vp, i32, vp, vp, vp, vp, vp, vp, vp, vp],
+            "me_kernel_timing": [vp, ctypes.c_int, vp, vp, vp],
         }
         for name, args in me.items():
             f = getattr(L, name)
@@ -536,6 +537,14 @@ class MechSolver:
         if rc != 0:
             raise StaticsError(rc, f"me_solve: {self.L.me_last_error(self.h).decode()}")
         return x, (None if rounds.value < 0 else rounds.value)
+
+    def kernel_timing(self, enable):
+        """me_kernel_timing: (launches, total ms, algorithmic bytes per launch) of the CG's K p."""
+        n, ms, b = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+        rc = self.L.me_kernel_timing(self.h, enable, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b))
+        if rc != 0:
+            raise StaticsError(rc, "me_kernel_timing")
+        return n.value, ms.value, b.value
 
     def info(self, cap=64):
         count = ctypes.c_int32()

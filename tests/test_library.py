@@ -32,7 +32,7 @@ def test_exports_every_mechanics_symbol():
     with open(os.path.join(ROOT, "include", "mechanics_hip.h")) as f:
         names = sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(me_\w+)\s*\(", f.read(), re.M)))
     lib = ctypes.CDLL(sh.LIB_PATH)
-    assert len(names) == 5
+    assert len(names) == 6
     for n in names:
         assert hasattr(lib, n), n
     assert set(names) == set(sh.ME_EXPORTS)

@@ -34,6 +34,12 @@ def test_unit_cube_turned_bears_nothing_and_stretched_is_as_stiff_as_its_materia
     assert np.abs(k - k.T).max() < 1e-15 and np.linalg.eigvalsh(k)[6] > 0  # six rigid modes, no more
 
 
+def test_system_builder_uses_the_oracles_unit_cube():
+    """The benchmark's system builder (parallax-world_amd/mech_systems.py) has its own unit
+    cube stiffness: it is the oracle's."""
+    assert np.abs(mh.unit_k() - orc.mech_unit_k()).max() < 1e-15
+
+
 def test_lift_is_the_rigid_transfer():
     rng = np.random.default_rng(0)
     for _ in range(5):
