@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -178,33 +179,47 @@ __global__ __launch_bounds__(64) void k_inverse(int64_t n, const double* diag, d
 
 // y = D x + sum of the row's blocks times x (System::times / Coarse::apply, solve.rs:61-72,
 // 330-340), then by mode: 0 out = y; 1 out = b - y; 2 out = x + step inv (b - y).
+// kRowLanes lanes to a row: lane g takes the row's blocks g, g + kRowLanes, ... (the lanes
+// of a row read neighbouring blocks), lane 0 the diagonal block; their sums meet by
+// shuffles in a fixed order.
+constexpr int kRowLanes = 8;
 template <int B, int MODE>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t n, const double* __restrict__ diag,
                                                  const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                                                  const double* __restrict__ val, const double* __restrict__ x,
                                                  const double* __restrict__ b, const double* __restrict__ inv,
                                                  double step, double* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / kRowLanes;
+  const int g = (int)(t % kRowLanes);
+  const bool live = i < n;
   double y[B], xi[B];
-  for (int k = 0; k < B; k++) xi[k] = x[(size_t)B * i + k];
-  const double* d = diag + (size_t)B * B * i;
-  for (int r = 0; r < B; r++) {
-    double s = 0.0;
-    for (int c = 0; c < B; c++) s += d[r * B + c] * xi[c];
-    y[r] = s;
-  }
-  for (int64_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
-    const int32_t j = col[e];
-    const double* m = val + (size_t)B * B * e;
-    double xj[B];
-    for (int k = 0; k < B; k++) xj[k] = x[(size_t)B * j + k];
-    for (int r = 0; r < B; r++) {
-      double s = 0.0;
-      for (int c = 0; c < B; c++) s += m[r * B + c] * xj[c];
-      y[r] += s;
+  for (int k = 0; k < B; k++) y[k] = 0.0;
+  if (live) {
+    if (g == 0) {
+      for (int k = 0; k < B; k++) xi[k] = x[(size_t)B * i + k];
+      const double* d = diag + (size_t)B * B * i;
+      for (int r = 0; r < B; r++) {
+        double s = 0.0;
+        for (int c = 0; c < B; c++) s += d[r * B + c] * xi[c];
+        y[r] = s;
+      }
+    }
+    for (int64_t e = rowptr[i] + g; e < rowptr[i + 1]; e += kRowLanes) {
+      const int32_t j = col[e];
+      const double* m = val + (size_t)B * B * e;
+      double xj[B];
+      for (int k = 0; k < B; k++) xj[k] = x[(size_t)B * j + k];
+      for (int r = 0; r < B; r++) {
+        double s = 0.0;
+        for (int c = 0; c < B; c++) s += m[r * B + c] * xj[c];
+        y[r] += s;
+      }
     }
   }
+  for (int o = 1; o < kRowLanes; o <<= 1)
+    for (int k = 0; k < B; k++) y[k] += __shfl_xor(y[k], o);
+  if (!live || g != 0) return;
   if constexpr (MODE == 0) {
     for (int k = 0; k < B; k++) out[(size_t)B * i + k] = y[k];
   } else {
@@ -609,12 +624,12 @@ double Solver::power_step(int B, int64_t nn, const double* diag, const Csr& c, c
   double most = 1.0;
   for (int it = 0; it < 10; it++) {
     if (B == 3) {
-      k_spmv<3, 0><<<nblk(nn), kBlock, 0, stream>>>(nn, diag, c.rowptr.as<int64_t>(), c.col.as<int32_t>(),
+      k_spmv<3, 0><<<nblk(nn * kRowLanes), kBlock, 0, stream>>>(nn, diag, c.rowptr.as<int64_t>(), c.col.as<int32_t>(),
                                                      c.val.as<double>(), v.as<double>(), nullptr, nullptr, 0.0,
                                                      a.as<double>());
       k_jacobi_set<3><<<nblk(nn), kBlock, 0, stream>>>(nn, inv, 1.0, a.as<double>(), w.as<double>());
     } else {
-      k_spmv<6, 0><<<nblk(nn), kBlock, 0, stream>>>(nn, diag, c.rowptr.as<int64_t>(), c.col.as<int32_t>(),
+      k_spmv<6, 0><<<nblk(nn * kRowLanes), kBlock, 0, stream>>>(nn, diag, c.rowptr.as<int64_t>(), c.col.as<int32_t>(),
                                                      c.val.as<double>(), v.as<double>(), nullptr, nullptr, 0.0,
                                                      a.as<double>());
       k_jacobi_set<6><<<nblk(nn), kBlock, 0, stream>>>(nn, inv, 1.0, a.as<double>(), w.as<double>());
@@ -688,50 +703,48 @@ void Solver::coarsen(const std::vector<Key>& keys, const std::vector<P3>& at,
     std::vector<int64_t> f(mptr.begin(), mptr.end() - 1);
     for (int64_t i = 0; i < nf; i++) mlist[f[of[i]]++] = (int32_t)i;
   }
-  std::vector<int64_t> iptr(n + 1, 0), ilist;
-  std::map<std::pair<uint32_t, uint32_t>, int64_t> slot;
-  std::vector<std::pair<int64_t, int64_t>> between;  // (finer pair k << 1 | transposed, slot)
+  // the pairs within each part (in order), and the pairs between parts bucketed by their
+  // lesser part (stable, so in order within a bucket), then each bucket sorted by the
+  // greater part: runs of one (a, c) are a coarse pair's finer pairs in order, the coarse
+  // pairs in (a, c) order
+  std::vector<int64_t> iptr(n + 1, 0), bptr(n + 1, 0);
   for (size_t k = 0; k < fij.size(); k++) {
     const uint32_t a = of[fij[k][0]], c = of[fij[k][1]];
-    if (a == c) {
+    if (a == c)
       iptr[a + 1]++;
-      continue;
-    }
-    const auto key = a < c ? std::make_pair(a, c) : std::make_pair(c, a);
-    auto it = slot.emplace(key, 0).first;
-    between.push_back({2 * (int64_t)k + (a > c ? 1 : 0), 0});
-    (void)it;
+    else
+      bptr[std::min(a, c) + 1]++;
   }
-  for (int64_t a = 0; a < n; a++) iptr[a + 1] += iptr[a];
-  ilist.resize(iptr[n]);
+  for (int64_t a = 0; a < n; a++) {
+    iptr[a + 1] += iptr[a];
+    bptr[a + 1] += bptr[a];
+  }
+  std::vector<int64_t> ilist(iptr[n]);
+  std::vector<std::pair<uint32_t, int64_t>> bl(bptr[n]);  // (greater part, finer pair k << 1 | transposed)
   {
-    std::vector<int64_t> f(iptr.begin(), iptr.end() - 1);
+    std::vector<int64_t> fi(iptr.begin(), iptr.end() - 1), fb(bptr.begin(), bptr.end() - 1);
     for (size_t k = 0; k < fij.size(); k++) {
       const uint32_t a = of[fij[k][0]], c = of[fij[k][1]];
-      if (a == c) ilist[f[a]++] = (int64_t)k;
+      if (a == c)
+        ilist[fi[a]++] = (int64_t)k;
+      else
+        bl[fb[std::min(a, c)]++] = {std::max(a, c), 2 * (int64_t)k + (a > c ? 1 : 0)};
     }
   }
-  // coarse pairs in (a, c) order; each one's finer pairs in order
-  int64_t np = 0;
+  for (int64_t a = 0; a < n; a++) std::sort(bl.begin() + bptr[a], bl.begin() + bptr[a + 1]);
   L.ij.clear();
-  for (auto& kv : slot) {
-    kv.second = np++;
-    L.ij.push_back({kv.first.first, kv.first.second});
-  }
+  std::vector<int64_t> sptr(1, 0), slist(bl.size());
+  for (int64_t a = 0; a < n; a++)
+    for (int64_t e = bptr[a]; e < bptr[a + 1]; e++) {
+      if (e == bptr[a] || bl[e].first != bl[e - 1].first) {
+        L.ij.push_back({(uint32_t)a, bl[e].first});
+        sptr.push_back(sptr.back());
+      }
+      slist[e] = bl[e].second;
+      sptr.back()++;
+    }
+  const int64_t np = (int64_t)L.ij.size();
   L.npairs = np;
-  std::vector<int64_t> sptr(np + 1, 0);
-  for (auto& b : between) {
-    const int64_t k = b.first >> 1;
-    const uint32_t a = of[fij[k][0]], c = of[fij[k][1]];
-    b.second = slot[a < c ? std::make_pair(a, c) : std::make_pair(c, a)];
-    sptr[b.second + 1]++;
-  }
-  for (int64_t s2 = 0; s2 < np; s2++) sptr[s2 + 1] += sptr[s2];
-  std::vector<int64_t> slist(between.size());
-  {
-    std::vector<int64_t> f(sptr.begin(), sptr.end() - 1);
-    for (const auto& b : between) slist[f[b.second]++] = b.first;
-  }
   // to the device: the maps, then the blocks
   up(L.of, of, stream);
   up(L.roff, roff, stream);
@@ -834,8 +847,16 @@ void Solver::factor(Level& L) {
 // Solver::new (solve.rs:387-420).
 void Solver::build(const MeSystem& s, double size) {
   const auto t0 = std::chrono::steady_clock::now();
+  const bool verbose = std::getenv("STATICS_HIP_VERBOSE") != nullptr;
+  auto mark = [&](const char* what) {
+    if (!verbose) return;
+    ST_HIP(hipStreamSynchronize(stream));
+    std::fprintf(stderr, "mechanics_hip: setup %s at %.1f ms\n", what,
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  };
   n = s.n;
   upload_system(s, true);
+  mark("system uploaded");
   inv0.ensure(std::max<size_t>(9 * sizeof(double) * n, 1));
   for (DBuf* b : {&x, &r, &z, &p, &ap, &bvec, &t1, &t2}) b->ensure(std::max<size_t>(3 * sizeof(double) * n, 1));
   part.ensure(kDotBlocks * sizeof(double));
@@ -846,6 +867,7 @@ void Solver::build(const MeSystem& s, double size) {
     ST_HIP(hipGetLastError());
   }
   step0 = power_step(3, n, diag0.as<double>(), csr0, inv0.as<double>(), t1, t2, z);
+  mark("level 0 inverses and step");
   for (Level& L : levels) release_level(L);
   levels.clear();
   std::vector<Key> keys(n);
@@ -859,6 +881,7 @@ void Solver::build(const MeSystem& s, double size) {
   levels.emplace_back();
   coarsen(keys, at, pattern, diag0.as<double>(), blocks0.as<double>(), ij0.as<uint32_t>(), 3, sz * 2.0,
           levels.back());
+  mark("level 1");
   for (;;) {
     sz *= 2.0;
     const int64_t nl = levels.back().n, before = levels.size() == 1 ? n : levels[levels.size() - 2].n;
@@ -875,6 +898,7 @@ void Solver::build(const MeSystem& s, double size) {
     Level& prev = levels[levels.size() - 2];
     coarsen(lkeys, centre, lij, prev.diag.as<double>(), prev.pairval.as<double>(), prev.pij.as<uint32_t>(), 6,
             sz * 2.0, levels.back());
+    mark("a level");
   }
   ST_HIP(hipStreamSynchronize(stream));
   setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -897,7 +921,7 @@ double* Solver::cycle(size_t l, double* b) {
   Level& U = levels[l + 1];
   double* x = L.x.as<double>();
   k_jacobi_set<6><<<nblk(nl), kBlock, 0, stream>>>(nl, L.inv.as<double>(), L.step, b, x);
-  k_spmv<6, 1><<<nblk(nl), kBlock, 0, stream>>>(nl, L.diag.as<double>(), L.csr.rowptr.as<int64_t>(),
+  k_spmv<6, 1><<<nblk(nl * kRowLanes), kBlock, 0, stream>>>(nl, L.diag.as<double>(), L.csr.rowptr.as<int64_t>(),
                                                  L.csr.col.as<int32_t>(), L.csr.val.as<double>(), x, b, nullptr, 0.0,
                                                  L.r.as<double>());
   k_restrict<6><<<nblk(U.n), kBlock, 0, stream>>>(U.n, U.mptr.as<int64_t>(), U.mlist.as<int32_t>(),
@@ -905,7 +929,7 @@ double* Solver::cycle(size_t l, double* b) {
   ST_HIP(hipGetLastError());
   const double* e = cycle(l + 1, U.b.as<double>());
   k_prolong<6><<<nblk(nl), kBlock, 0, stream>>>(nl, U.of.as<uint32_t>(), U.roff.as<double>(), e, x);
-  k_spmv<6, 2><<<nblk(nl), kBlock, 0, stream>>>(nl, L.diag.as<double>(), L.csr.rowptr.as<int64_t>(),
+  k_spmv<6, 2><<<nblk(nl * kRowLanes), kBlock, 0, stream>>>(nl, L.diag.as<double>(), L.csr.rowptr.as<int64_t>(),
                                                  L.csr.col.as<int32_t>(), L.csr.val.as<double>(), x, b,
                                                  L.inv.as<double>(), L.step, L.x2.as<double>());
   ST_HIP(hipGetLastError());
@@ -916,7 +940,7 @@ double* Solver::cycle(size_t l, double* b) {
 void Solver::precondition(const double* b, double* out) {
   Level& U = levels[0];
   k_jacobi_set<3><<<nblk(n), kBlock, 0, stream>>>(n, inv0.as<double>(), step0, b, t1.as<double>());
-  k_spmv<3, 1><<<nblk(n), kBlock, 0, stream>>>(n, diag0.as<double>(), csr0.rowptr.as<int64_t>(),
+  k_spmv<3, 1><<<nblk(n * kRowLanes), kBlock, 0, stream>>>(n, diag0.as<double>(), csr0.rowptr.as<int64_t>(),
                                                 csr0.col.as<int32_t>(), csr0.val.as<double>(), t1.as<double>(), b,
                                                 nullptr, 0.0, t2.as<double>());
   k_restrict<3><<<nblk(U.n), kBlock, 0, stream>>>(U.n, U.mptr.as<int64_t>(), U.mlist.as<int32_t>(),
@@ -924,7 +948,7 @@ void Solver::precondition(const double* b, double* out) {
   ST_HIP(hipGetLastError());
   const double* e = cycle(0, U.b.as<double>());
   k_prolong<3><<<nblk(n), kBlock, 0, stream>>>(n, U.of.as<uint32_t>(), U.roff.as<double>(), e, t1.as<double>());
-  k_spmv<3, 2><<<nblk(n), kBlock, 0, stream>>>(n, diag0.as<double>(), csr0.rowptr.as<int64_t>(),
+  k_spmv<3, 2><<<nblk(n * kRowLanes), kBlock, 0, stream>>>(n, diag0.as<double>(), csr0.rowptr.as<int64_t>(),
                                                 csr0.col.as<int32_t>(), csr0.val.as<double>(), t1.as<double>(), b,
                                                 inv0.as<double>(), step0, out);
   ST_HIP(hipGetLastError());
@@ -950,7 +974,7 @@ int32_t Solver::solve(const MeSystem& s, const double* bh, double* xh, double to
   const double target = tol * tol * dot(b, z.as<double>(), n3, TARGET, true);
   if (std::isnan(target) || target < 0.0) return finish(-1);
   // r = b - K x
-  k_spmv<3, 1><<<nblk(n), kBlock, 0, stream>>>(n, diag0.as<double>(), csr0.rowptr.as<int64_t>(),
+  k_spmv<3, 1><<<nblk(n * kRowLanes), kBlock, 0, stream>>>(n, diag0.as<double>(), csr0.rowptr.as<int64_t>(),
                                                 csr0.col.as<int32_t>(), csr0.val.as<double>(), x.as<double>(), b,
                                                 nullptr, 0.0, r.as<double>());
   ST_HIP(hipGetLastError());
@@ -968,7 +992,7 @@ int32_t Solver::solve(const MeSystem& s, const double* bh, double* xh, double to
       }
       ST_HIP(hipEventRecord(tev[2 * tev_used], stream));
     }
-    k_spmv<3, 0><<<nblk(n), kBlock, 0, stream>>>(n, diag0.as<double>(), csr0.rowptr.as<int64_t>(),
+    k_spmv<3, 0><<<nblk(n * kRowLanes), kBlock, 0, stream>>>(n, diag0.as<double>(), csr0.rowptr.as<int64_t>(),
                                                   csr0.col.as<int32_t>(), csr0.val.as<double>(), p.as<double>(),
                                                   nullptr, nullptr, 0.0, ap.as<double>());
     ST_HIP(hipGetLastError());
