@@ -17,6 +17,8 @@
 // of this build's own (rounding differs from the reference's sequential sums).
 #include <hip/hip_runtime.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -462,26 +464,68 @@ struct Csr {
   int64_t nslots = 0;
   DBuf rowptr, col, src, val;
 };
-void csr_symbolic(Csr& c, int64_t n, const std::vector<std::array<uint32_t, 2>>& ij, hipStream_t s) {
-  std::vector<int64_t> rowptr(n + 1, 0);
-  for (const auto& p : ij) {
-    rowptr[p[0] + 1]++;
-    rowptr[p[1] + 1]++;
+// Entry e of the both-halves listing: pair e >> 1, in the row of its first unknown (e even)
+// or its second (e odd, transposed).
+__global__ void k_csr_entries(int64_t ne, const uint32_t* ij, uint32_t* key, int64_t* val) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  key[e] = ij[e];  // ij[2k + h]: the row of half h of pair k
+  val[e] = e;
+}
+// After the stable sort by row: each row's first entry (binary search), and each entry's
+// column (the pair's other unknown).
+__global__ void k_csr_rows(int64_t n, int64_t ne, const uint32_t* key, int64_t* rowptr) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > n) return;
+  int64_t lo = 0, hi = ne;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) / 2;
+    if ((int64_t)key[mid] < r)
+      lo = mid + 1;
+    else
+      hi = mid;
   }
-  for (int64_t i = 0; i < n; i++) rowptr[i + 1] += rowptr[i];
-  std::vector<int64_t> fill(rowptr.begin(), rowptr.end() - 1), src(rowptr[n]);
-  std::vector<int32_t> col(rowptr[n]);
-  for (size_t k = 0; k < ij.size(); k++) {
-    const uint32_t i = ij[k][0], j = ij[k][1];
-    col[fill[i]] = (int32_t)j;
-    src[fill[i]++] = 2 * (int64_t)k;
-    col[fill[j]] = (int32_t)i;
-    src[fill[j]++] = 2 * (int64_t)k + 1;
+  rowptr[r] = lo;
+}
+__global__ void k_csr_cols(int64_t ne, const int64_t* src, const uint32_t* ij, int32_t* col) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  col[e] = (int32_t)ij[src[e] ^ 1];
+}
+
+// The block CSR of a level's pairs (device ij, np pairs) with both halves of each pair: row i
+// gets pair k as (j, 2k), row j as (i, 2k+1), each row's entries in pair order (a stable
+// sort of the listing by row).
+void csr_build(Csr& c, int64_t n, const uint32_t* dij, int64_t np, hipStream_t s) {
+  const int64_t ne = 2 * np;
+  c.nslots = ne;
+  c.rowptr.ensure((size_t)(n + 1) * sizeof(int64_t));
+  c.col.ensure(std::max<size_t>(ne * sizeof(int32_t), 1));
+  c.src.ensure(std::max<size_t>(ne * sizeof(int64_t), 1));
+  if (ne == 0) {
+    ST_HIP(hipMemsetAsync(c.rowptr.p, 0, (size_t)(n + 1) * sizeof(int64_t), s));
+    return;
   }
-  c.nslots = rowptr[n];
-  up(c.rowptr, rowptr, s);
-  up(c.col, col, s);
-  up(c.src, src, s);
+  if (ne >= (int64_t(1) << 31)) throw StError{ST_BAD_INPUT, "system: more than 2^30 pairs"};
+  DBuf kin, kout, vin, tmp;
+  kin.ensure(ne * sizeof(uint32_t));
+  kout.ensure(ne * sizeof(uint32_t));
+  vin.ensure(ne * sizeof(int64_t));
+  k_csr_entries<<<nblk(ne), kBlock, 0, s>>>(ne, dij, kin.as<uint32_t>(), vin.as<int64_t>());
+  ST_HIP(hipGetLastError());
+  int bits = 1;
+  while (bits < 32 && (int64_t(1) << bits) < n) bits++;
+  size_t bytes = 0;
+  ST_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin.as<uint32_t>(), kout.as<uint32_t>(),
+                                            vin.as<int64_t>(), c.src.as<int64_t>(), (int)ne, 0, bits, s));
+  tmp.ensure(bytes + 16);
+  ST_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, kin.as<uint32_t>(), kout.as<uint32_t>(),
+                                            vin.as<int64_t>(), c.src.as<int64_t>(), (int)ne, 0, bits, s));
+  k_csr_rows<<<nblk(n + 1), kBlock, 0, s>>>(n, ne, kout.as<uint32_t>(), c.rowptr.as<int64_t>());
+  k_csr_cols<<<nblk(ne), kBlock, 0, s>>>(ne, c.src.as<int64_t>(), dij, c.col.as<int32_t>());
+  ST_HIP(hipGetLastError());
+  ST_HIP(hipStreamSynchronize(s));
+  for (DBuf* b : {&kin, &kout, &vin, &tmp}) b->release();
 }
 
 struct Level {
@@ -596,7 +640,7 @@ void Solver::upload_system(const MeSystem& s, bool setup) {
     pattern = ij;
     npairs0 = s.npairs;
     up(ij0, ij, stream);
-    csr_symbolic(csr0, s.n, ij, stream);
+    csr_build(csr0, s.n, ij0.as<uint32_t>(), s.npairs, stream);
     csr0.val.ensure(std::max<size_t>(9 * sizeof(double) * csr0.nslots, 1));
   }
   if (csr0.nslots)
@@ -651,6 +695,13 @@ void Solver::coarsen(const std::vector<Key>& keys, const std::vector<P3>& at,
                      const std::vector<std::array<uint32_t, 2>>& fij, const double* fdiag, const double* fblocks,
                      const uint32_t* dfij, int FB, double size, Level& L) {
   const int64_t nf = (int64_t)keys.size();
+  const bool verbose = std::getenv("STATICS_HIP_VERBOSE") != nullptr && fij.size() > 1000000;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto mark = [&](const char* what) {
+    if (verbose)
+      std::fprintf(stderr, "mechanics_hip:   coarsen %s at %.1f ms\n", what,
+                   std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  };
   // the parts: union at the lesser root over the pairs within one octree cell a level up,
   // numbered as first met (solve.rs:210-250)
   std::vector<int64_t> upv(nf);
@@ -669,6 +720,7 @@ void Solver::coarsen(const std::vector<Key>& keys, const std::vector<P3>& at,
       const int64_t r = find(p[0]), s2 = find(p[1]);
       upv[std::max(r, s2)] = std::min(r, s2);
     }
+  mark("union-find");
   std::vector<uint32_t> index(nf, UINT32_MAX), of(nf);
   int64_t n = 0;
   for (int64_t i = 0; i < nf; i++) {
@@ -703,6 +755,7 @@ void Solver::coarsen(const std::vector<Key>& keys, const std::vector<P3>& at,
     std::vector<int64_t> f(mptr.begin(), mptr.end() - 1);
     for (int64_t i = 0; i < nf; i++) mlist[f[of[i]]++] = (int32_t)i;
   }
+  mark("parts, middles, members");
   // the pairs within each part (in order), and the pairs between parts bucketed by their
   // lesser part (stable, so in order within a bucket), then each bucket sorted by the
   // greater part: runs of one (a, c) are a coarse pair's finer pairs in order, the coarse
@@ -731,7 +784,9 @@ void Solver::coarsen(const std::vector<Key>& keys, const std::vector<P3>& at,
         bl[fb[std::min(a, c)]++] = {std::max(a, c), 2 * (int64_t)k + (a > c ? 1 : 0)};
     }
   }
+  mark("pairs bucketed");
   for (int64_t a = 0; a < n; a++) std::sort(bl.begin() + bptr[a], bl.begin() + bptr[a + 1]);
+  mark("buckets sorted");
   L.ij.clear();
   std::vector<int64_t> sptr(1, 0), slist(bl.size());
   for (int64_t a = 0; a < n; a++)
@@ -745,6 +800,7 @@ void Solver::coarsen(const std::vector<Key>& keys, const std::vector<P3>& at,
     }
   const int64_t np = (int64_t)L.ij.size();
   L.npairs = np;
+  mark("coarse pairs listed");
   // to the device: the maps, then the blocks
   up(L.of, of, stream);
   up(L.roff, roff, stream);
@@ -779,7 +835,9 @@ void Solver::coarsen(const std::vector<Key>& keys, const std::vector<P3>& at,
                                                        L.roff.as<double>(), fblocks, L.pairval.as<double>());
   }
   ST_HIP(hipGetLastError());
-  csr_symbolic(L.csr, n, L.ij, stream);
+  mark("maps uploaded, lifts launched");
+  csr_build(L.csr, n, L.pij.as<uint32_t>(), np, stream);
+  mark("csr built");
   L.csr.val.ensure(std::max<size_t>(36 * sizeof(double) * L.csr.nslots, 1));
   if (L.csr.nslots)
     k_scatter<6><<<nblk(L.csr.nslots), kBlock, 0, stream>>>(L.csr.nslots, L.csr.src.as<int64_t>(),
@@ -788,6 +846,7 @@ void Solver::coarsen(const std::vector<Key>& keys, const std::vector<P3>& at,
   for (DBuf* b : {&L.b, &L.x, &L.r, &L.x2}) b->ensure(std::max<size_t>(6 * sizeof(double) * n, 1));
   L.step = power_step(6, n, L.diag.as<double>(), L.csr, L.inv.as<double>(), L.x, L.r, L.x2);
   ST_HIP(hipStreamSynchronize(stream));
+  mark("step");
   for (DBuf* t : {&dip, &dil, &dsp, &dsl}) t->release();
   if (n <= kDirect) factor(L);
 }
