@@ -28,6 +28,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/mechanics_hip.h"
@@ -451,6 +452,23 @@ using Key = std::array<int64_t, 3>;
 using P3 = std::array<double, 3>;
 
 int64_t div_euclid2(int64_t v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
+
+// f(t, lo, hi) over [0, n) in contiguous chunks, one host thread each (chunk t before
+// chunk t + 1, so per-chunk results merged in chunk order keep the sequential order).
+int host_threads() {
+  static const int t = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  return t;
+}
+template <class F>
+void chunks(int64_t n, int nt, F f) {
+  if (nt <= 1 || n < 65536) {
+    f(0, (int64_t)0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++) th.emplace_back([&, t] { f(t, n * t / nt, n * (t + 1) / nt); });
+  for (auto& x : th) x.join();
+}
 Key parent(const Key& k) { return {div_euclid2(k[0]), div_euclid2(k[1]), div_euclid2(k[2])}; }
 
 template <class T>
@@ -713,11 +731,19 @@ void Solver::coarsen(const std::vector<Key>& keys, const std::vector<P3>& at,
     }
     return i;
   };
+  const int nt = host_threads();
   std::vector<Key> par(nf);
-  for (int64_t i = 0; i < nf; i++) par[i] = parent(keys[i]);
-  for (const auto& p : fij)
-    if (par[p[0]] == par[p[1]]) {
-      const int64_t r = find(p[0]), s2 = find(p[1]);
+  chunks(nf, nt, [&](int, int64_t lo, int64_t hi) {
+    for (int64_t i = lo; i < hi; i++) par[i] = parent(keys[i]);
+  });
+  const int64_t nfp = (int64_t)fij.size();
+  std::vector<uint8_t> within(nfp);
+  chunks(nfp, nt, [&](int, int64_t lo, int64_t hi) {
+    for (int64_t k = lo; k < hi; k++) within[k] = par[fij[k][0]] == par[fij[k][1]];
+  });
+  for (int64_t k = 0; k < nfp; k++)
+    if (within[k]) {
+      const int64_t r = find(fij[k][0]), s2 = find(fij[k][1]);
       upv[std::max(r, s2)] = std::min(r, s2);
     }
   mark("union-find");
@@ -760,32 +786,49 @@ void Solver::coarsen(const std::vector<Key>& keys, const std::vector<P3>& at,
   // lesser part (stable, so in order within a bucket), then each bucket sorted by the
   // greater part: runs of one (a, c) are a coarse pair's finer pairs in order, the coarse
   // pairs in (a, c) order
+  // per chunk of pairs its counts per part, then each chunk's place in each bucket (chunks in
+  // order), then the fills; at most 64 MiB of counts
+  const int ntb = (int)std::max<int64_t>(1, std::min<int64_t>(nt, (int64_t(1) << 26) / std::max<int64_t>(16 * n, 1)));
+  std::vector<std::vector<int64_t>> ci(ntb, std::vector<int64_t>(n, 0)), cb(ntb, std::vector<int64_t>(n, 0));
+  chunks(nfp, ntb, [&](int t, int64_t lo, int64_t hi) {
+    for (int64_t k = lo; k < hi; k++) {
+      const uint32_t a = of[fij[k][0]], c = of[fij[k][1]];
+      if (a == c)
+        ci[t][a]++;
+      else
+        cb[t][std::min(a, c)]++;
+    }
+  });
   std::vector<int64_t> iptr(n + 1, 0), bptr(n + 1, 0);
-  for (size_t k = 0; k < fij.size(); k++) {
-    const uint32_t a = of[fij[k][0]], c = of[fij[k][1]];
-    if (a == c)
-      iptr[a + 1]++;
-    else
-      bptr[std::min(a, c) + 1]++;
-  }
   for (int64_t a = 0; a < n; a++) {
-    iptr[a + 1] += iptr[a];
-    bptr[a + 1] += bptr[a];
+    int64_t si = iptr[a], sb = bptr[a];
+    for (int t = 0; t < ntb; t++) {
+      const int64_t xi = ci[t][a], xb = cb[t][a];
+      ci[t][a] = si;  // chunk t's first place in bucket a
+      cb[t][a] = sb;
+      si += xi;
+      sb += xb;
+    }
+    iptr[a + 1] = si;
+    bptr[a + 1] = sb;
   }
   std::vector<int64_t> ilist(iptr[n]);
   std::vector<std::pair<uint32_t, int64_t>> bl(bptr[n]);  // (greater part, finer pair k << 1 | transposed)
-  {
-    std::vector<int64_t> fi(iptr.begin(), iptr.end() - 1), fb(bptr.begin(), bptr.end() - 1);
-    for (size_t k = 0; k < fij.size(); k++) {
+  chunks(nfp, ntb, [&](int t, int64_t lo, int64_t hi) {
+    for (int64_t k = lo; k < hi; k++) {
       const uint32_t a = of[fij[k][0]], c = of[fij[k][1]];
       if (a == c)
-        ilist[fi[a]++] = (int64_t)k;
+        ilist[ci[t][a]++] = k;
       else
-        bl[fb[std::min(a, c)]++] = {std::max(a, c), 2 * (int64_t)k + (a > c ? 1 : 0)};
+        bl[cb[t][std::min(a, c)]++] = {std::max(a, c), 2 * k + (a > c ? 1 : 0)};
     }
-  }
+  });
+  ci.clear();
+  cb.clear();
   mark("pairs bucketed");
-  for (int64_t a = 0; a < n; a++) std::sort(bl.begin() + bptr[a], bl.begin() + bptr[a + 1]);
+  chunks(n, nt, [&](int, int64_t lo, int64_t hi) {
+    for (int64_t a = lo; a < hi; a++) std::sort(bl.begin() + bptr[a], bl.begin() + bptr[a + 1]);
+  });
   mark("buckets sorted");
   L.ij.clear();
   std::vector<int64_t> sptr(1, 0), slist(bl.size());
