@@ -1670,7 +1670,7 @@ __device__ __forceinline__ int block_scan256(int v, int* wsum) {
 // Next level = P^T A P by rows, one workgroup per row I (rows[blockIdx.x], or
 // blockIdx.x): row I += P[i,I]^T sum over blocks (i, j) of A of A[i,j] P[j,:], the
 // (column entry, block of A's row) pairs spread over the threads 256 at a time.
-template <int pass, int kSlots, int kH>
+template <int pass, int kSlots, int kH, int kRG = 1>
 __global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, const int* pt_ptr,
                                                   const int* pt_blk, const int* pblk_row, const int* prow,
                                                   const int* pcol, const double* pval, const int* rowptrA,
@@ -1720,9 +1720,9 @@ __global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, c
       if (threadIdx.x == 0) pre[0] = 0;
       __syncthreads();
       const int total = pre[nc];
-      const int split = run == 0 ? 1 : 6;  // run 1: one thread per output row r
+      const int split = run == 0 ? 1 : 6 / kRG;  // run 1: one thread per group of kRG output rows
       for (int it2 = threadIdx.x; it2 < total * split; it2 += blockDim.x) {
-        const int it = it2 / split, r = it2 % split;
+        const int it = it2 / split, rg = it2 % split;
         int lo = 0, hi = nc - 1;  // the last t with pre[t] <= it
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
@@ -1738,15 +1738,18 @@ __global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, c
           for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) hinsert<kH>(keys, pcol[k2]);
           continue;
         }
-        // row r of P[i,I]^T A[i,j], then row r of its products with P[j,:]
+        // rows rg kRG .. of P[i,I]^T A[i,j], then those rows of its products with P[j,:]
+        // (each block of A and P read once for the group's rows)
         const double* PiI = pval + 36 * (int64_t)blki[lo];
         const double* Ab = Aa + 36 * (int64_t)b;
-        double qr[6];
-        for (int k = 0; k < 6; k++) {
-          double t = 0;
-          for (int i2 = 0; i2 < 6; i2++) t += PiI[6 * i2 + r] * Ab[6 * i2 + k];
-          qr[k] = t;
-        }
+        double qr[kRG][6];
+        for (int rr = 0; rr < kRG; rr++)
+          for (int k = 0; k < 6; k++) qr[rr][k] = 0.0;
+        for (int i2 = 0; i2 < 6; i2++)
+          for (int k = 0; k < 6; k++) {
+            const double ab = Ab[6 * i2 + k];
+            for (int rr = 0; rr < kRG; rr++) qr[rr][k] += PiI[6 * i2 + rg * kRG + rr] * ab;
+          }
         const int k2a = prow[j], len = prow[j + 1] - k2a;
         for (int kk = 0; kk < len; kk++) {
           const int k2 = k2a + (kk + it) % len;
@@ -1754,11 +1757,17 @@ __global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, c
           if (J < I) continue;  // the lower blocks are the transposes of other rows' upper ones
           const int sl = hfind<kH>(keys, slot, J);
           const double* Pj = pval + 36 * (int64_t)k2;
-          double* out = acc + 36 * sl + 6 * r;
-          double o6[6] = {0, 0, 0, 0, 0, 0};
+          double o6[kRG][6];
+          for (int rr = 0; rr < kRG; rr++)
+            for (int cc = 0; cc < 6; cc++) o6[rr][cc] = 0.0;
           for (int k = 0; k < 6; k++)
-            for (int cc = 0; cc < 6; cc++) o6[cc] += qr[k] * Pj[6 * k + cc];
-          for (int cc = 0; cc < 6; cc++) atomicAdd(&out[cc], o6[cc]);
+            for (int cc = 0; cc < 6; cc++) {
+              const double pk = Pj[6 * k + cc];
+              for (int rr = 0; rr < kRG; rr++) o6[rr][cc] += qr[rr][k] * pk;
+            }
+          double* out = acc + 36 * sl + 6 * rg * kRG;
+          for (int rr = 0; rr < kRG; rr++)
+            for (int cc = 0; cc < 6; cc++) atomicAdd(&out[6 * rr + cc], o6[rr][cc]);
         }
       }
       __syncthreads();
@@ -1831,7 +1840,7 @@ __device__ __forceinline__ int face_beam_q(int f, int t) {
 // Level 1 = P0^T K P0 by rows, one workgroup per row I: work items are (node v in
 // column I of P0, one of the ten beams ending on v), adjacent threads on different
 // nodes. kH: hash slots (4x kSlots; pass 0 sizes with the largest instance).
-template <int pass, int kSlots, int kH>
+template <int pass, int kSlots, int kH, int kRG = 1>
 __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, const int* pt_ptr,
                                                  const int* pt_blk,
                                                  const int* pblk_row, const int* prow, const int* pcol,
@@ -1915,12 +1924,14 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
     return;
   }
   for (int x = threadIdx.x; x < 36 * m; x += blockDim.x) acc[x] = 0.0;
-  // products split by output row r: thread task (row r, node, beam)
+  // products split by groups of kRG output rows: thread task (row group, node, beam); each
+  // task reads a column block P0[w, J] once for its kRG rows
+  constexpr int NRG = 6 / kRG;
   for (int base = 0; base < nv; base += kV) {
     const int nvc = min(kV, nv - base);
     load_chunk(base, nvc);
-    for (int it = threadIdx.x; it < nvc * 60; it += blockDim.x) {
-      const int r = it % 6, rest = it / 6;
+    for (int it = threadIdx.x; it < nvc * 10 * NRG; it += blockDim.x) {
+      const int rg = it % NRG, rest = it / NRG;
       const int vi = rest % nvc, j = rest / nvc;
       const uint32_t wf = s_wf[j][vi];
       if (wf == kDeadBeam) continue;
@@ -1931,8 +1942,9 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
       const bool wfree = !(wf & kNotFree);
       const int v = s_v[vi];
       const double* PvI = pval + 36 * (int64_t)s_blk[vi];
-      double pc[6];  // column r of P0[v, I]
-      for (int i = 0; i < 6; i++) pc[i] = PvI[6 * i + r];
+      double pc[kRG][6];  // columns rg kRG .. of P0[v, I]
+      for (int rr = 0; rr < kRG; rr++)
+        for (int i = 0; i < 6; i++) pc[rr][i] = PvI[6 * i + rg * kRG + rr];
       const BeamGeo G = bgeo[s_sidx[vi]];
       double n[3];
       int cls;
@@ -1940,8 +1952,10 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
       const double e = s_e[side][vi], g = s_g[side][vi];
       const double rho = beam_sec(csec, s_c[side][vi], q);
       for (int sd = 0; sd < (wfree ? 2 : 1); sd++) {
-        double qr[6];  // row r of P0[v,I]^T K_b
-        beam_row(pc, sv, sd == 0 ? sv : -sv, n, G.hd[cls], G.K4[cls] * rho, G.R2[cls] * (rho * rho), e, g, qr);
+        double qr[kRG][6];  // rows rg kRG .. of P0[v,I]^T K_b
+        for (int rr = 0; rr < kRG; rr++)
+          beam_row(pc[rr], sv, sd == 0 ? sv : -sv, n, G.hd[cls], G.K4[cls] * rho, G.R2[cls] * (rho * rho), e, g,
+                   qr[rr]);
         const int row = sd == 0 ? v : (int)wf;
         // start each task's column loop at a different column, so that the tasks of a
         // wave (mostly nodes of one aggregate) do not add into the same block at once
@@ -1952,11 +1966,17 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
           if (J < I) continue;  // the lower blocks are the transposes of other rows' upper ones
           const int sl = hfind<kH>(keys, slot, J);
           const double* Pj = pval + 36 * (int64_t)k2;
-          double* out = acc + 36 * sl + 6 * r;
-          double o6[6] = {0, 0, 0, 0, 0, 0};
+          double o6[kRG][6];
+          for (int rr = 0; rr < kRG; rr++)
+            for (int cc = 0; cc < 6; cc++) o6[rr][cc] = 0.0;
           for (int k = 0; k < 6; k++)
-            for (int cc = 0; cc < 6; cc++) o6[cc] += qr[k] * Pj[6 * k + cc];
-          for (int cc = 0; cc < 6; cc++) atomicAdd(&out[cc], o6[cc]);
+            for (int cc = 0; cc < 6; cc++) {
+              const double pk = Pj[6 * k + cc];
+              for (int rr = 0; rr < kRG; rr++) o6[rr][cc] += qr[rr][k] * pk;
+            }
+          double* out = acc + 36 * sl + 6 * rg * kRG;
+          for (int rr = 0; rr < kRG; rr++)
+            for (int cc = 0; cc < 6; cc++) atomicAdd(&out[6 * rr + cc], o6[rr][cc]);
         }
       }
     }
@@ -2393,9 +2413,14 @@ struct Solver {
                                                           c.dinv64.as<double>(), c.step0_d.as<double>(), cnt, prow,
                                                           pcol, pval, sbad.as<int>());
                          });
+    // output rows per task of the A1 product (STATICS_HIP_A1_ROWS: 1, 2, 3 or 6): three rows
+    // read each P0 block once for three; 275 ms at 256^3 with one row, 207 ms with three
+    // (six rows take 256 VGPRs, one wave per SIMD)
+    int a1_rows = 3;
+    if (const char* e = std::getenv("STATICS_HIP_A1_ROWS")) a1_rows = std::atoi(e);
     wave_rows(lv, [&](int pass, int slots, int* cnt, const int* rows, int64_t nr) {
-#define ST_A1(P, S, H)                                                                                                   \
-  k_sa_a1_w<P, S, H><<<(unsigned)nr, 256, 0, st>>>(rows, nr, c.p0t_ptr.as<int>(), c.p0t_blk.as<int>(),          \
+#define ST_A1(P, S, H, RG)                                                                                               \
+  k_sa_a1_w<P, S, H, RG><<<(unsigned)nr, 256, 0, st>>>(rows, nr, c.p0t_ptr.as<int>(), c.p0t_blk.as<int>(),          \
                                                c.p0blk_row.as<int>(), c.p0row.as<int>(), c.p0col.as<int>(),         \
                                                c.p0val.as<double>(), g.nkind.as<uint8_t>(), g.ncell.as<int32_t>(),  \
                                                g.cfaces.as<uint32_t>(), g.ce64.as<double>(), g.cg64.as<double>(),   \
@@ -2403,15 +2428,22 @@ struct Solver {
                                                g.bgeo.as<BeamGeo>(), cnt, lv.rowptr.as<int>(), lv.col.as<int>(),    \
                                                lv.a64.as<double>(), sbad.as<int>())
       if (pass == 0)
-        ST_A1(0, 1, 2048);
-      else if (slots <= 64)
-        ST_A1(1, 64, 256);
-      else if (slots <= 128)
-        ST_A1(1, 128, 512);
+        ST_A1(0, 1, 2048, 1);
+      else if (slots <= 64) {
+        if (a1_rows == 6)
+          ST_A1(1, 64, 256, 6);
+        else if (a1_rows == 3)
+          ST_A1(1, 64, 256, 3);
+        else if (a1_rows == 2)
+          ST_A1(1, 64, 256, 2);
+        else
+          ST_A1(1, 64, 256, 1);
+      } else if (slots <= 128)
+        ST_A1(1, 128, 512, 1);
       else if (slots <= 256)
-        ST_A1(1, 256, 1024);
+        ST_A1(1, 256, 1024, 1);
       else
-        ST_A1(1, 448, 2048);
+        ST_A1(1, 448, 2048, 1);
 #undef ST_A1
       ST_HIP(hipGetLastError());
     });
@@ -2433,27 +2465,42 @@ struct Solver {
                                                                   lv.centre.as<double>(), cnt, prow, pcol, pval,
                                                                   sbad.as<int>());
                             });
+    // output rows per task of the product (STATICS_HIP_GAL_ROWS: 1, 2 or 3): three rows read
+    // each block of A and P once for three; at 256^3 the setup's Galerkin products take
+    // 58 ms less than with one
+    int gal_rows = 3;
+    if (const char* e = std::getenv("STATICS_HIP_GAL_ROWS")) gal_rows = std::atoi(e);
     below.pval32.ensure((size_t)std::max<int64_t>(below.pblocks, 1) * 36 * sizeof(float));
     k_to_f32<<<nblk(below.pblocks * 36), 256, 0, st>>>(below.pblocks * 36, below.pval64.as<double>(),
                                                        below.pval32.as<float>());
     wave_rows(lv, [&](int pass, int slots, int* cnt, const int* rows, int64_t nr) {
-#define ST_GAL(P, S, H)                                                                                                  \
-  k_sa_gal_w<P, S, H><<<(unsigned)nr, 256, 0, st>>>(rows, nr, below.pt_ptr.as<int>(), below.pt_blk.as<int>(),    \
+#define ST_GAL(P, S, H, RG)                                                                                              \
+  k_sa_gal_w<P, S, H, RG><<<(unsigned)nr, 256, 0, st>>>(rows, nr, below.pt_ptr.as<int>(), below.pt_blk.as<int>(),    \
                                                 below.pblk_row.as<int>(), below.prow.as<int>(),                     \
                                                 below.pcol.as<int>(), below.pval64.as<double>(),                    \
                                                 below.rowptr.as<int>(), below.col.as<int>(), below.a64.as<double>(),\
                                                 cnt, lv.rowptr.as<int>(), lv.col.as<int>(), lv.a64.as<double>(),    \
                                                 sbad.as<int>())
       if (pass == 0)
-        ST_GAL(0, 1, 2048);
-      else if (slots <= 64)
-        ST_GAL(1, 64, 256);
-      else if (slots <= 128)
-        ST_GAL(1, 128, 512);
-      else if (slots <= 256)
-        ST_GAL(1, 256, 1024);
+        ST_GAL(0, 1, 2048, 1);
+      else if (slots <= 64) {
+        if (gal_rows == 3)
+          ST_GAL(1, 64, 256, 3);
+        else if (gal_rows == 2)
+          ST_GAL(1, 64, 256, 2);
+        else
+          ST_GAL(1, 64, 256, 1);
+      } else if (slots <= 128) {
+        if (gal_rows == 3)
+          ST_GAL(1, 128, 512, 3);
+        else if (gal_rows == 2)
+          ST_GAL(1, 128, 512, 2);
+        else
+          ST_GAL(1, 128, 512, 1);
+      } else if (slots <= 256)
+        ST_GAL(1, 256, 1024, 1);
       else
-        ST_GAL(1, 448, 2048);
+        ST_GAL(1, 448, 2048, 1);
 #undef ST_GAL
       ST_HIP(hipGetLastError());
     });
