@@ -57,23 +57,26 @@ inline BeamGeo beam_geo(double s) {
   return g;
 }
 
-// Per-cell coefficients for one class: F = kg*delta + ke*(n.delta) n; M = re*dw + rx*(n.dw) n;
-// the lever term -(d/2) n x F = -hk * (n x delta) with hk = hd*kg.
+// Per-cell coefficients for one class, for a beam direction n = ns N with N in {-1, 0, 1}^3
+// (ns = 1 for opposite faces, 1/sqrt(2) for perpendicular ones):
+//   delta = du + hdn (N x w),  F = kg delta + ken2 (N.delta) N,
+//   M = re dw + rxn2 (N.dw) N,  lever -(d/2) n x F = hkn (N x delta),
+// with hdn = hd ns, ken2 = K4 (e - g) ns^2, rxn2 = R2 (2g - e) ns^2, hkn = -hd ns K4 g.
 template <class T>
 struct CellCoef {
-  T kg, ke, re, rx, hk, ns, hd;
+  T kg, ken2, re, rxn2, hdn, hkn;
 };
 
 template <class T>
 ST_HD CellCoef<T> cell_coef(const BeamGeo& G, int cls, T e, T g) {
+  const T ns2 = cls == 0 ? T(1) : T(0.5);
   CellCoef<T> c;
   c.kg = (T)G.K4[cls] * g;
-  c.ke = (T)G.K4[cls] * (e - g);
+  c.ken2 = (T)G.K4[cls] * (e - g) * ns2;
   c.re = (T)G.R2[cls] * e;
-  c.rx = (T)G.R2[cls] * (g + g - e);
-  c.hk = (T)G.hd[cls] * c.kg;
-  c.ns = (T)G.nscale[cls];
-  c.hd = (T)G.hd[cls];
+  c.rxn2 = (T)G.R2[cls] * (g + g - e) * ns2;
+  c.hdn = (T)(G.hd[cls] * G.nscale[cls]);
+  c.hkn = -c.hdn * c.kg;
   return c;
 }
 
@@ -84,102 +87,247 @@ template <class T>
 ST_HD CellCoef<T> scaled(const CellCoef<T>& c, T rho) {
   CellCoef<T> o = c;
   o.kg = c.kg * rho;
-  o.ke = c.ke * rho;
-  o.hk = c.hk * rho;
+  o.ken2 = c.ken2 * rho;
+  o.hkn = c.hkn * rho;
   o.re = c.re * (rho * rho);
-  o.rx = c.rx * (rho * rho);
+  o.rxn2 = c.rxn2 * (rho * rho);
+  return o;
+}
+
+// A a + B b for compile-time A, B in {-1, 0, 1}, not both 0, with no operation on a zero
+// coefficient (0 * x cannot be folded away in IEEE arithmetic, so it is never written).
+template <int A, int B, class T>
+ST_HD T lin2(T a, T b) {
+  static_assert(A != 0 || B != 0, "lin2 of two zero coefficients");
+  if constexpr (A == 0)
+    return B > 0 ? b : -b;
+  else if constexpr (B == 0)
+    return A > 0 ? a : -a;
+  else if constexpr (A > 0)
+    return B > 0 ? a + b : a - b;
+  else
+    return B > 0 ? b - a : -(a + b);
+}
+template <int A, int B, int C, class T>
+ST_HD T lin3(T a, T b, T c) {
+  if constexpr (A == 0)
+    return lin2<B, C>(b, c);
+  else if constexpr (B == 0 && C == 0)
+    return A > 0 ? a : -a;
+  else {
+    const T r = lin2<B, C == 0 ? 0 : C>(b, c);  // B or C non-zero here
+    return A > 0 ? a + r : r - a;
+  }
+}
+// (N x v)_K for N = (NX, NY, NZ): N_{K+1} v_{K+2} - N_{K+2} v_{K+1}; structurally zero when
+// both of those components of N are.
+template <int K, int NX, int NY, int NZ>
+ST_HD constexpr bool cross_zero() {
+  constexpr int N[3] = {NX, NY, NZ};
+  return N[(K + 1) % 3] == 0 && N[(K + 2) % 3] == 0;
+}
+template <int K, int NX, int NY, int NZ, class T>
+ST_HD T cross_n(const T* v) {
+  constexpr int N[3] = {NX, NY, NZ};
+  return lin2<N[(K + 1) % 3], -N[(K + 2) % 3]>(v[(K + 2) % 3], v[(K + 1) % 3]);
+}
+template <int K, int NX, int NY, int NZ>
+ST_HD constexpr int ncomp() {
+  constexpr int N[3] = {NX, NY, NZ};
+  return N[K];
+}
+
+// The beam's end loads from its end motions xi, xj (6 each: u, w): the force F on end j
+// (-F on end i) and the rotational rows Lj = L + M (end j) and Li = L - M (end i).
+template <class T, int NX, int NY, int NZ>
+struct BeamLoads {
+  T F[3], Lj[3], Li[3];
+};
+template <int K, class T, int NX, int NY, int NZ>
+ST_HD void beam_delta(const CellCoef<T>& c, const T* xi, const T* xj, const T* w, T* d) {
+  const T du = xj[K] - xi[K];
+  if constexpr (cross_zero<K, NX, NY, NZ>())
+    d[K] = du;
+  else
+    d[K] = du + c.hdn * cross_n<K, NX, NY, NZ>(w);
+}
+template <int K, class T, int NX, int NY, int NZ>
+ST_HD void beam_rows(const CellCoef<T>& c, const T* d, const T* q, T t, T tq, BeamLoads<T, NX, NY, NZ>& o) {
+  constexpr int N = ncomp<K, NX, NY, NZ>();
+  T f = c.kg * d[K], m = c.re * q[K];
+  if constexpr (N > 0) {
+    f = f + t;
+    m = m + tq;
+  } else if constexpr (N < 0) {
+    f = f - t;
+    m = m - tq;
+  }
+  o.F[K] = f;
+  if constexpr (cross_zero<K, NX, NY, NZ>()) {
+    o.Lj[K] = m;
+    o.Li[K] = -m;
+  } else {
+    const T l = c.hkn * cross_n<K, NX, NY, NZ>(d);
+    o.Lj[K] = l + m;
+    o.Li[K] = l - m;
+  }
+}
+template <class T, int NX, int NY, int NZ>
+ST_HD BeamLoads<T, NX, NY, NZ> beam_loads(const CellCoef<T>& c, const T* xi, const T* xj) {
+  T w[3], d[3], q[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    w[k] = xi[3 + k] + xj[3 + k];  // (components the direction does not use are dropped)
+    q[k] = xj[3 + k] - xi[3 + k];
+  }
+  beam_delta<0, T, NX, NY, NZ>(c, xi, xj, w, d);
+  beam_delta<1, T, NX, NY, NZ>(c, xi, xj, w, d);
+  beam_delta<2, T, NX, NY, NZ>(c, xi, xj, w, d);
+  const T t = c.ken2 * lin3<NX, NY, NZ>(d[0], d[1], d[2]);
+  const T tq = c.rxn2 * lin3<NX, NY, NZ>(q[0], q[1], q[2]);
+  BeamLoads<T, NX, NY, NZ> o;
+  beam_rows<0, T, NX, NY, NZ>(c, d, q, t, tq, o);
+  beam_rows<1, T, NX, NY, NZ>(c, d, q, t, tq, o);
+  beam_rows<2, T, NX, NY, NZ>(c, d, q, t, tq, o);
   return o;
 }
 
 // Applies beam q of a cell to end motions xi = x[a], xj = x[b] (6 each: u, w) and adds
-// the beam's loads into yi, yj. n components nx, ny, nz in {-1,0,1} times c.ns.
+// the beam's loads into yi, yj. N = (NX, NY, NZ) in {-1,0,1}^3, the direction over ns.
 template <class T, int NX, int NY, int NZ>
 ST_HD void beam_apply(const CellCoef<T>& c, const T* xi, const T* xj, T* yi, T* yj) {
-  const T ns = c.ns;
-  const T n0 = (T)NX * ns, n1 = (T)NY * ns, n2 = (T)NZ * ns;
-  // w = w_i + w_j; delta = du + hd * n x w
-  const T w0 = xi[3] + xj[3], w1 = xi[4] + xj[4], w2 = xi[5] + xj[5];
-  const T cx0 = n1 * w2 - n2 * w1, cx1 = n2 * w0 - n0 * w2, cx2 = n0 * w1 - n1 * w0;
-  const T d0 = (xj[0] - xi[0]) + c.hd * cx0;
-  const T d1 = (xj[1] - xi[1]) + c.hd * cx1;
-  const T d2 = (xj[2] - xi[2]) + c.hd * cx2;
-  const T nd = n0 * d0 + n1 * d1 + n2 * d2;
-  const T F0 = c.kg * d0 + c.ke * nd * n0;
-  const T F1 = c.kg * d1 + c.ke * nd * n1;
-  const T F2 = c.kg * d2 + c.ke * nd * n2;
-  const T q0 = xj[3] - xi[3], q1 = xj[4] - xi[4], q2 = xj[5] - xi[5];
-  const T nq = n0 * q0 + n1 * q1 + n2 * q2;
-  const T M0 = c.re * q0 + c.rx * nq * n0;
-  const T M1 = c.re * q1 + c.rx * nq * n1;
-  const T M2 = c.re * q2 + c.rx * nq * n2;
-  // lever: -(d/2) n x F = -hk (n x delta)
-  const T L0 = -c.hk * (n1 * d2 - n2 * d1);
-  const T L1 = -c.hk * (n2 * d0 - n0 * d2);
-  const T L2 = -c.hk * (n0 * d1 - n1 * d0);
-  yj[0] += F0; yj[1] += F1; yj[2] += F2;
-  yi[0] -= F0; yi[1] -= F1; yi[2] -= F2;
-  yj[3] += L0 + M0; yj[4] += L1 + M1; yj[5] += L2 + M2;
-  yi[3] += L0 - M0; yi[4] += L1 - M1; yi[5] += L2 - M2;
+  const BeamLoads<T, NX, NY, NZ> o = beam_loads<T, NX, NY, NZ>(c, xi, xj);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    yj[k] += o.F[k];
+    yi[k] -= o.F[k];
+    yj[3 + k] += o.Lj[k];
+    yi[3 + k] += o.Li[k];
+  }
 }
 
 // The same beam's load at one end only: end j (J) or end i, added into y.
 template <class T, int NX, int NY, int NZ, bool J>
 ST_HD void beam_end(const CellCoef<T>& c, const T* xi, const T* xj, T* y) {
-  const T ns = c.ns;
-  const T n0 = (T)NX * ns, n1 = (T)NY * ns, n2 = (T)NZ * ns;
-  const T w0 = xi[3] + xj[3], w1 = xi[4] + xj[4], w2 = xi[5] + xj[5];
-  const T cx0 = n1 * w2 - n2 * w1, cx1 = n2 * w0 - n0 * w2, cx2 = n0 * w1 - n1 * w0;
-  const T d0 = (xj[0] - xi[0]) + c.hd * cx0;
-  const T d1 = (xj[1] - xi[1]) + c.hd * cx1;
-  const T d2 = (xj[2] - xi[2]) + c.hd * cx2;
-  const T nd = n0 * d0 + n1 * d1 + n2 * d2;
-  const T F0 = c.kg * d0 + c.ke * nd * n0;
-  const T F1 = c.kg * d1 + c.ke * nd * n1;
-  const T F2 = c.kg * d2 + c.ke * nd * n2;
-  const T q0 = xj[3] - xi[3], q1 = xj[4] - xi[4], q2 = xj[5] - xi[5];
-  const T nq = n0 * q0 + n1 * q1 + n2 * q2;
-  const T M0 = c.re * q0 + c.rx * nq * n0;
-  const T M1 = c.re * q1 + c.rx * nq * n1;
-  const T M2 = c.re * q2 + c.rx * nq * n2;
-  const T L0 = -c.hk * (n1 * d2 - n2 * d1);
-  const T L1 = -c.hk * (n2 * d0 - n0 * d2);
-  const T L2 = -c.hk * (n0 * d1 - n1 * d0);
-  if (J) {
-    y[0] += F0; y[1] += F1; y[2] += F2;
-    y[3] += L0 + M0; y[4] += L1 + M1; y[5] += L2 + M2;
-  } else {
-    y[0] -= F0; y[1] -= F1; y[2] -= F2;
-    y[3] += L0 - M0; y[4] += L1 - M1; y[5] += L2 - M2;
+  const BeamLoads<T, NX, NY, NZ> o = beam_loads<T, NX, NY, NZ>(c, xi, xj);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (J) {
+      y[k] += o.F[k];
+      y[3 + k] += o.Lj[k];
+    } else {
+      y[k] -= o.F[k];
+      y[3 + k] += o.Li[k];
+    }
   }
 }
 
+// The coefficients of beam Q of a cell: its class's, scaled by the beam's own section when
+// the cells carry sections (kSec: rho holds the cell's 15 factors).
+template <class T, bool kSec, int Q>
+ST_HD CellCoef<T> beam_coef(const CellCoef<T>& c0, const CellCoef<T>& c1, const T* rho) {
+  const CellCoef<T>& cc = pair_opposite(Q) ? c0 : c1;
+  if constexpr (kSec)
+    return scaled<T>(cc, rho[Q]);
+  else
+    return cc;
+}
+
+// Beam Q of a cell applied to its faces' motions x[f][6], its loads added into y[f][6]
+// (FA, FB: the first load of face a or b in this cell, assigned rather than added onto a zero).
+template <class T, int Q, bool FA, bool FB>
+ST_HD void cell_beam(const CellCoef<T>& c, const T (*x)[6], T (*y)[6]) {
+  constexpr int a = kPairA[Q], b = kPairB[Q];
+  const BeamLoads<T, pair_dir(Q, 0), pair_dir(Q, 1), pair_dir(Q, 2)> o =
+      beam_loads<T, pair_dir(Q, 0), pair_dir(Q, 1), pair_dir(Q, 2)>(c, x[a], x[b]);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if constexpr (FB) {
+      y[b][k] = o.F[k];
+      y[b][3 + k] = o.Lj[k];
+    } else {
+      y[b][k] += o.F[k];
+      y[b][3 + k] += o.Lj[k];
+    }
+    if constexpr (FA) {
+      y[a][k] = -o.F[k];
+      y[a][3 + k] = o.Li[k];
+    } else {
+      y[a][k] -= o.F[k];
+      y[a][3 + k] += o.Li[k];
+    }
+  }
+}
+// Every beam of a cell whose 15 beams are all alive: straight-line code, each face's first
+// load assigned (face 0 and 1 first meet beam 0, face f >= 2 beam f - 1).
+template <class T, bool kSec, int Q = 0>
+ST_HD void cell_apply_full(const CellCoef<T>& c0, const CellCoef<T>& c1, const T* rho, const T (*x)[6], T (*y)[6]) {
+  cell_beam<T, Q, Q == 0, Q <= 4>(beam_coef<T, kSec, Q>(c0, c1, rho), x, y);
+  if constexpr (Q < 14) cell_apply_full<T, kSec, Q + 1>(c0, c1, rho, x, y);
+}
+// The live beams of a cell (mask bit q = beam q alive), onto y zeroed before.
+template <class T, bool kSec, int Q = 0>
+ST_HD void cell_apply_masked(const CellCoef<T>& c0, const CellCoef<T>& c1, uint32_t mask, const T* rho,
+                             const T (*x)[6], T (*y)[6]) {
+  if (mask & (1u << Q)) cell_beam<T, Q, false, false>(beam_coef<T, kSec, Q>(c0, c1, rho), x, y);
+  if constexpr (Q < 14) cell_apply_masked<T, kSec, Q + 1>(c0, c1, mask, rho, x, y);
+}
+
 // All live beams of one cell: x[f][6] the motions of its faces (f in VOXEL_FACES
-// order), y[f][6] accumulates. mask bit q = beam q alive. only_face >= 0 restricts
-// to the beams touching that face (the others' loads are still added, but the
-// caller keeps only y[only_face]).
-// rho: the cell's 15 section factors, or NULL (every beam SECTION s^2).
-template <class T>
+// order), y[f][6] receives each face's load (assigned). mask bit q = beam q alive.
+// kSec: rho holds the cell's 15 section factors (else every beam is SECTION s^2).
+template <class T, bool kSec>
 ST_HD void cell_apply(const CellCoef<T>& c0, const CellCoef<T>& c1, uint32_t mask, const T (*x)[6], T (*y)[6],
-                      const T* rho = nullptr) {
-#define ST_BEAM(q, a, b, cc, NX, NY, NZ) \
-  if (mask & (1u << q))                  \
-    beam_apply<T, NX, NY, NZ>(rho ? scaled<T>(cc, rho[q]) : cc, x[a], x[b], y[a], y[b]);
-  ST_BEAM(0, 0, 1, c0, 1, 0, 0)
-  ST_BEAM(1, 0, 2, c1, 1, -1, 0)
-  ST_BEAM(2, 0, 3, c1, 1, 1, 0)
-  ST_BEAM(3, 0, 4, c1, 1, 0, -1)
-  ST_BEAM(4, 0, 5, c1, 1, 0, 1)
-  ST_BEAM(5, 1, 2, c1, -1, -1, 0)
-  ST_BEAM(6, 1, 3, c1, -1, 1, 0)
-  ST_BEAM(7, 1, 4, c1, -1, 0, -1)
-  ST_BEAM(8, 1, 5, c1, -1, 0, 1)
-  ST_BEAM(9, 2, 3, c0, 0, 1, 0)
-  ST_BEAM(10, 2, 4, c1, 0, 1, -1)
-  ST_BEAM(11, 2, 5, c1, 0, 1, 1)
-  ST_BEAM(12, 3, 4, c1, 0, -1, -1)
-  ST_BEAM(13, 3, 5, c1, 0, -1, 1)
-  ST_BEAM(14, 4, 5, c0, 0, 0, 1)
-#undef ST_BEAM
+                      const T* rho) {
+  if (mask == 0x7fffu) {
+    cell_apply_full<T, kSec>(c0, c1, rho, x, y);
+  } else {
+#pragma unroll
+    for (int f = 0; f < 6; f++)
+#pragma unroll
+      for (int k = 0; k < 6; k++) y[f][k] = T(0);
+    cell_apply_masked<T, kSec>(c0, c1, mask, rho, x, y);
+  }
+}
+
+// The load at face F of one cell from its beams (F, f2), onto y: assigned by the first beam
+// (First) or added.
+template <class T, int F, int Q, bool First>
+ST_HD void face_beam_end(const CellCoef<T>& c, const T (*x)[6], T* y) {
+  constexpr bool J = kPairB[Q] == F;
+  const BeamLoads<T, pair_dir(Q, 0), pair_dir(Q, 1), pair_dir(Q, 2)> o =
+      beam_loads<T, pair_dir(Q, 0), pair_dir(Q, 1), pair_dir(Q, 2)>(c, x[kPairA[Q]], x[kPairB[Q]]);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const T f = J ? o.F[k] : -o.F[k];
+    const T l = J ? o.Lj[k] : o.Li[k];
+    if constexpr (First) {
+      y[k] = f;
+      y[3 + k] = l;
+    } else {
+      y[k] += f;
+      y[3 + k] += l;
+    }
+  }
+}
+// Face F's five beams, all alive.
+template <class T, int F, bool kSec, int Q = 0, bool First = true>
+ST_HD void face_row_full(const CellCoef<T>& c0, const CellCoef<T>& c1, const T* rho, const T (*x)[6], T* y) {
+  if constexpr (kPairA[Q] == F || kPairB[Q] == F) {
+    face_beam_end<T, F, Q, First>(beam_coef<T, kSec, Q>(c0, c1, rho), x, y);
+    if constexpr (Q < 14) face_row_full<T, F, kSec, Q + 1, false>(c0, c1, rho, x, y);
+  } else if constexpr (Q < 14) {
+    face_row_full<T, F, kSec, Q + 1, First>(c0, c1, rho, x, y);
+  }
+}
+// Face F's live beams (mask holds only beams of face F), onto y zeroed before.
+template <class T, int F, bool kSec, int Q = 0>
+ST_HD void face_row_masked(const CellCoef<T>& c0, const CellCoef<T>& c1, uint32_t mask, const T* rho,
+                           const T (*x)[6], T* y) {
+  if constexpr (kPairA[Q] == F || kPairB[Q] == F) {
+    if (mask & (1u << Q)) face_beam_end<T, F, Q, false>(beam_coef<T, kSec, Q>(c0, c1, rho), x, y);
+  }
+  if constexpr (Q < 14) face_row_masked<T, F, kSec, Q + 1>(c0, c1, mask, rho, x, y);
 }
 
 // Beam masks of the beams touching face f (5 of 15).

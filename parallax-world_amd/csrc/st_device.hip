@@ -153,6 +153,41 @@ void timing_collect(Ctx& c) {
   c.tev_used = 0;
 }
 
+// One application in schedule kind `tile`; kSec: the cells carry per-beam sections.
+template <class T, class Epi, bool kSec>
+void launch_schedule(Ctx& c, int tile, const GeoD<T>& g, const T* x, const Epi& epi) {
+  const unsigned nb = (unsigned)c.geo.nbricks;
+  switch (tile) {
+    case 0:
+      k_apply<T, Epi, 768, 256, kSec><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
+      break;
+    case 1:
+      k_apply<T, Epi, 0, 0, kSec><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
+      break;
+    default: {
+      // the dynamic LDS limit is raised once per context and kernel instance (a context
+      // owns its device; each keeps what it set, so contexts on other threads or devices
+      // neither share nor race on it)
+      const void* fn = reinterpret_cast<const void*>(&k_apply_cells<T, Epi, kSec>);
+      const DevGeo& d = c.geo;
+      const size_t need = (size_t)std::max(d.class_nodes[0], d.class_nodes[1]) * 6 * sizeof(T);
+      size_t& have = c.lds_attr[fn];
+      if (need > have) {
+        ST_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+        have = need;
+      }
+      const uint32_t* bl = d.blist.as<uint32_t>();
+      for (int k = 0; k < 2; k++) {
+        if (d.nclass[k])
+          k_apply_cells<T, Epi, kSec><<<(unsigned)d.nclass[k], kBlock, (size_t)d.class_nodes[k] * 6 * sizeof(T),
+                                        c.stream>>>(g, bl, x, epi);
+        bl += d.nclass[k];
+      }
+      break;
+    }
+  }
+}
+
 template <class T, class Epi>
 void launch_apply(Ctx& c, const T* x, const Epi& epi) {
   if (!c.geo.nbricks) return;
@@ -172,36 +207,10 @@ void launch_apply(Ctx& c, const T* x, const Epi& epi) {
   GeoD<T> g = geo_view<T>(c);
   const int sched = c.apply_sched[sizeof(T) == 8 ? 1 : 0];
   g.xcd = kSchedules[sched].xcd;
-  const unsigned nb = (unsigned)c.geo.nbricks;
-  switch (kSchedules[sched].tile) {
-    case 0:
-      k_apply<T, Epi, 768, 256><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
-      break;
-    case 1:
-      k_apply<T, Epi, 0, 0><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
-      break;
-    default: {
-      // the dynamic LDS limit is raised once per context and kernel instance (a context
-      // owns its device; each keeps what it set, so contexts on other threads or devices
-      // neither share nor race on it)
-      const void* fn = reinterpret_cast<const void*>(&k_apply_cells<T, Epi>);
-      const DevGeo& d = c.geo;
-      const size_t need = (size_t)std::max(d.class_nodes[0], d.class_nodes[1]) * 6 * sizeof(T);
-      size_t& have = c.lds_attr[fn];
-      if (need > have) {
-        ST_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-        have = need;
-      }
-      const uint32_t* bl = d.blist.as<uint32_t>();
-      for (int k = 0; k < 2; k++) {
-        if (d.nclass[k])
-          k_apply_cells<T, Epi><<<(unsigned)d.nclass[k], kBlock, (size_t)d.class_nodes[k] * 6 * sizeof(T), c.stream>>>(
-              g, bl, x, epi);
-        bl += d.nclass[k];
-      }
-      break;
-    }
-  }
+  if (g.carea)
+    launch_schedule<T, Epi, true>(c, kSchedules[sched].tile, g, x, epi);
+  else
+    launch_schedule<T, Epi, false>(c, kSchedules[sched].tile, g, x, epi);
   ST_HIP(hipGetLastError());
   if (timed) {
     ST_HIP(hipEventRecord(c.tev[2 * c.tev_used + 1], c.stream));

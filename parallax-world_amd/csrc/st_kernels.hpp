@@ -66,22 +66,6 @@ __device__ __forceinline__ void store6<float>(float* __restrict__ p, const float
   q[2] = make_float2(v[4], v[5]);
 }
 
-// The load at face F of one cell from the cell's live beams that end there (the five
-// beams (F, f2)), given the motions xv of its six faces. F is known at compile time,
-// so each beam's direction, class and end are constants.
-template <class T, int F, int Q>
-__device__ __forceinline__ void face_beam(const CellCoef<T>& c0, const CellCoef<T>& c1, uint32_t mask,
-                                          const T (*xv)[6], T* y, const T* rho) {
-  if constexpr (kPairA[Q] == F || kPairB[Q] == F) {
-    if (mask & (1u << Q)) {
-      const CellCoef<T>& cc = pair_opposite(Q) ? c0 : c1;
-      beam_end<T, pair_dir(Q, 0), pair_dir(Q, 1), pair_dir(Q, 2), kPairB[Q] == F>(rho ? scaled<T>(cc, rho[Q]) : cc,
-                                                                                 xv[kPairA[Q]], xv[kPairB[Q]], y);
-    }
-  }
-  if constexpr (Q < 14) face_beam<T, F, Q + 1>(c0, c1, mask, xv, y, rho);
-}
-
 // A brick's share of the operator's inputs staged in LDS: the motions of its first
 // ns nodes and the face tables, e, g and beam masks of its first ncs cells (both
 // read coalesced). Faces and cells outside the staged ranges (the neighbours across
@@ -109,13 +93,20 @@ __device__ __forceinline__ void lds_load6(lds_cptr<T> p, T* v) {
   for (int k = 0; k < 6; k++) v[k] = p[k];
 }
 
-template <class T, int F>
+// The load at face F of cell c from the cell's live beams that end there (the five beams
+// (F, f2)), written into y. F is known at compile time, so each beam's direction, class and
+// end are constants. kSec: the cells carry per-beam sections.
+template <class T, int F, bool kStaged, bool kSec>
 __device__ __forceinline__ void cell_face_row(const GeoD<T>& g, const Tile<T>& t, const BeamGeo& G,
                                               const T* __restrict__ x, uint32_t c, T* y) {
   const uint32_t lc = c - t.c0;
-  const bool staged = lc < t.ncs;
+  const bool staged = kStaged && lc < t.ncs;
   const uint32_t mask = (uint32_t)(staged ? t.cm[lc] : g.cmask[c]) & face_beams(F);
-  if (!mask) return;
+  if (!mask) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) y[k] = T(0);
+    return;
+  }
   uint32_t fi[6];
   if (staged) {
 #pragma unroll
@@ -125,31 +116,46 @@ __device__ __forceinline__ void cell_face_row(const GeoD<T>& g, const Tile<T>& t
     const uint2 f01 = fp[0], f23 = fp[1], f45 = fp[2];
     fi[0] = f01.x; fi[1] = f01.y; fi[2] = f23.x; fi[3] = f23.y; fi[4] = f45.x; fi[5] = f45.y;
   }
+  // Every face is a node (held and gone ones too), so its row is loaded unconditionally
+  // and zeroed after for a node that is not free: the loads issue back to back.
   T xv[6][6];
 #pragma unroll
   for (int f = 0; f < 6; f++) {
-    const uint32_t ln = fi[f] - t.n0;
-    if (fi[f] & kNotFree) {
-#pragma unroll
-      for (int k = 0; k < 6; k++) xv[f][k] = T(0);
-    } else if (ln < t.ns) {
+    const uint32_t v = fi[f] & ~kNotFree;
+    const uint32_t ln = v - t.n0;
+    if (kStaged && ln < t.ns)
       lds_load6<T>(t.xs + 6 * ln, xv[f]);
-    } else {
-      load6<T>(x + 6 * (size_t)fi[f], xv[f]);
-    }
+    else
+      load6<T>(x + 6 * (size_t)v, xv[f]);
   }
+#pragma unroll
+  for (int f = 0; f < 6; f++)
+#pragma unroll
+    for (int k = 0; k < 6; k++) xv[f][k] = (fi[f] & kNotFree) ? T(0) : xv[f][k];
   const T e = staged ? t.ce[lc] : g.ce[c], gg = staged ? t.cg[lc] : g.cg[c];
-  face_beam<T, F, 0>(cell_coef<T>(G, 0, e, gg), cell_coef<T>(G, 1, e, gg), mask, xv, y,
-                     g.carea ? g.carea + 15 * (size_t)c : nullptr);
+  const CellCoef<T> c0 = cell_coef<T>(G, 0, e, gg), c1 = cell_coef<T>(G, 1, e, gg);
+  const T* rho = kSec ? g.carea + 15 * (size_t)c : nullptr;
+  if (mask == face_beams(F)) {
+    face_row_full<T, F, kSec>(c0, c1, rho, xv, y);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; k++) y[k] = T(0);
+    face_row_masked<T, F, kSec>(c0, c1, mask, rho, xv, y);
+  }
 }
 
 // Row of K x at a free node of orientation O: face 2O+1 of the cell below (c-) and
 // face 2O of the cell above (c+), where those cells are free.
-template <class T, int O>
+// (y is written: the sum of the two cells' loads in that order, or the one cell's.)
+template <class T, int O, bool kStaged, bool kSec>
 __device__ __forceinline__ void node_row(const GeoD<T>& g, const Tile<T>& t, const BeamGeo& G,
                                          const T* __restrict__ x, int2 cells, T* y) {
-  if (cells.x >= 0) cell_face_row<T, 2 * O + 1>(g, t, G, x, (uint32_t)cells.x, y);
-  if (cells.y >= 0) cell_face_row<T, 2 * O>(g, t, G, x, (uint32_t)cells.y, y);
+  T ya[6], yb[6];
+  const bool ha = cells.x >= 0, hb = cells.y >= 0;
+  if (ha) cell_face_row<T, 2 * O + 1, kStaged, kSec>(g, t, G, x, (uint32_t)cells.x, ya);
+  if (hb) cell_face_row<T, 2 * O, kStaged, kSec>(g, t, G, x, (uint32_t)cells.y, yb);
+#pragma unroll
+  for (int k = 0; k < 6; k++) y[k] = ha ? (hb ? ya[k] + yb[k] : ya[k]) : (hb ? yb[k] : T(0));
 }
 
 // Block-wide sum of one double per thread; the result is valid in thread 0.
@@ -197,7 +203,7 @@ __device__ __forceinline__ uint32_t xcd_brick(uint32_t w, uint32_t nb) {
 #define ST_APPLY_WAVES 1
 #endif
 
-template <class T, class Epi, int kNodes, int kCells>
+template <class T, class Epi, int kNodes, int kCells, bool kSec>
 __global__ __launch_bounds__(kBlock, ST_APPLY_WAVES) void k_apply(GeoD<T> g, const T* __restrict__ x, Epi epi) {
   using V2 = typename Vec2<T>::type;
   __shared__ V2 xs2[kNodes > 0 ? kNodes * 3 : 1];
@@ -242,11 +248,11 @@ __global__ __launch_bounds__(kBlock, ST_APPLY_WAVES) void k_apply(GeoD<T> g, con
       const int o = g.nkind[v] >> 1;
       const int2 cells = reinterpret_cast<const int2*>(g.ncell)[v];
       if (o == 0)
-        node_row<T, 0>(g, t, G, x, cells, y);
+        node_row<T, 0, (kNodes > 0), kSec>(g, t, G, x, cells, y);
       else if (o == 1)
-        node_row<T, 1>(g, t, G, x, cells, y);
+        node_row<T, 1, (kNodes > 0), kSec>(g, t, G, x, cells, y);
       else
-        node_row<T, 2>(g, t, G, x, cells, y);
+        node_row<T, 2, (kNodes > 0), kSec>(g, t, G, x, cells, y);
     }
     const uint32_t lv = v - t.n0;
     T xo[6];  // the node's own entries of x, for epilogues that use them
@@ -274,7 +280,7 @@ __global__ __launch_bounds__(kBlock, ST_APPLY_WAVES) void k_apply(GeoD<T> g, con
 // faces whose other cell lies in another brick takes that cell's five beams ending on
 // it (the halo). Then the epilogue runs per node. blist: the bricks of this launch
 // (bricks are launched in classes by node count, each with LDS for its largest).
-template <class T, class Epi>
+template <class T, class Epi, bool kSec>
 __global__ __launch_bounds__(kBlock) void k_apply_cells(GeoD<T> g, const uint32_t* __restrict__ blist,
                                                         const T* __restrict__ x, Epi epi) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -298,19 +304,14 @@ __global__ __launch_bounds__(kBlock) void k_apply_cells(GeoD<T> g, const uint32_
     const uint32_t fi[6] = {f01.x, f01.y, f23.x, f23.y, f45.x, f45.y};
     T xv[6][6], yv[6][6];
 #pragma unroll
-    for (int f = 0; f < 6; f++) {
-      if (fi[f] & kNotFree) {
+    for (int f = 0; f < 6; f++) load6<T>(x + 6 * (size_t)(fi[f] & ~kNotFree), xv[f]);  // back to back
 #pragma unroll
-        for (int k = 0; k < 6; k++) xv[f][k] = T(0);
-      } else {
-        load6<T>(x + 6 * (size_t)fi[f], xv[f]);
-      }
+    for (int f = 0; f < 6; f++)
 #pragma unroll
-      for (int k = 0; k < 6; k++) yv[f][k] = T(0);
-    }
+      for (int k = 0; k < 6; k++) xv[f][k] = (fi[f] & kNotFree) ? T(0) : xv[f][k];
     const T e = g.ce[c], gg = g.cg[c];
-    cell_apply<T>(cell_coef<T>(G, 0, e, gg), cell_coef<T>(G, 1, e, gg), (uint32_t)g.cmask[c], xv, yv,
-                  g.carea ? g.carea + 15 * (size_t)c : nullptr);
+    cell_apply<T, kSec>(cell_coef<T>(G, 0, e, gg), cell_coef<T>(G, 1, e, gg), (uint32_t)g.cmask[c], xv, yv,
+                        kSec ? g.carea + 15 * (size_t)c : nullptr);
     if constexpr (Epi::kCellDot) {  // every (cell, face) counted once, in the cell's own brick
 #pragma unroll
       for (int f = 0; f < 6; f++)
@@ -333,14 +334,14 @@ __global__ __launch_bounds__(kBlock) void k_apply_cells(GeoD<T> g, const uint32_
     if ((kind & 1) || (g.nflags[v] & (kHeld | kGone))) continue;
     const int32_t cm = g.ncell[2 * (size_t)v];
     if (cm < 0 || ((uint32_t)cm >= c0 && (uint32_t)cm < c1)) continue;
-    T y6[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    T y6[6];
     const int o = kind >> 1;
     if (o == 0)
-      cell_face_row<T, 1>(g, none, G, x, (uint32_t)cm, y6);
+      cell_face_row<T, 1, false, kSec>(g, none, G, x, (uint32_t)cm, y6);
     else if (o == 1)
-      cell_face_row<T, 3>(g, none, G, x, (uint32_t)cm, y6);
+      cell_face_row<T, 3, false, kSec>(g, none, G, x, (uint32_t)cm, y6);
     else
-      cell_face_row<T, 5>(g, none, G, x, (uint32_t)cm, y6);
+      cell_face_row<T, 5, false, kSec>(g, none, G, x, (uint32_t)cm, y6);
 #pragma unroll
     for (int k = 0; k < 6; k++) atomicAdd(&yl[6 * l + k], y6[k]);
   }
