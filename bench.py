@@ -170,12 +170,14 @@ def node_loads(occ, faces, joined, held, rng, n_points=64):
     return f
 
 
-def cpu_baseline(size, threads, seed=1, points=64):
+def cpu_baseline(size, threads, seed=1, points=64, max_iter=100000, progress=0, spacing=7.0):
     """The oracle (faithful restatement, FP64, assembled blocks, smoothed aggregation) on a size^3
-    sample of the same generator and loads (gravity + point loads)."""
+    sample of the same generator and loads (gravity + point loads). progress > 0: the oracle
+    prints a line per level built and every `progress` CG iterations (stderr)."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as orc
-    occ, theta = structure(size, seed)
+    orc.lib().orc_set_progress(progress)
+    occ, theta = structure(size, seed, spacing=spacing)
     sites, held, e, g = sh.cells_from_occ(occ, theta)
     mats = np.zeros((len(sites), 6))
     mats[:, 0] = 1.0
@@ -183,9 +185,12 @@ def cpu_baseline(size, threads, seed=1, points=64):
     mats[:, 2] = g
     mats[:, 3:] = 1.0
     orc.lib().orc_set_threads(threads)
+    say = (lambda m: print(f"[cpu] {m}", file=sys.stderr, flush=True)) if progress > 0 else (lambda m: None)
+    say(f"{size}^3: {len(sites)} cells, building the lattice")
     t0 = time.perf_counter()
     L = orc.Lattice(0, sites, mats, held)
     t_build = time.perf_counter() - t0
+    say(f"lattice: {L.n_nodes} nodes, {L.n_bonds} bonds ({t_build:.1f} s)")
     t1 = time.perf_counter()
     joined = L.joined_to_held()
     gone = (joined == 0).astype(np.uint8)
@@ -196,11 +201,30 @@ def cpu_baseline(size, threads, seed=1, points=64):
     pick = rng.choice(free, size=min(points, len(free)), replace=False)
     d = rng.normal(size=(len(pick), 3))
     f[pick, :3] += d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(1.0, 10.0, size=(len(pick), 1))
-    x, st = L.motions(f, gone=gone, tol=1e-8)
+    x, st = L.motions(f, gone=gone, tol=1e-8, max_iter=max_iter)
     t_solve = time.perf_counter() - t1
     dof = 6 * int(((L.held == 0) & (gone == 0)).sum())
+    # the true relative residual reached, ||f - K x|| / ||f|| (on samples small enough that assembling K
+    # once more is cheap; the CG itself stops on its recursive residual, bodies.rs:350-352)
+    rel = None
+    if dof < 6_000_000:
+        part = (L.held == 0) & (gone == 0)
+        rel = float(np.linalg.norm(f[part] - L.apply(x, gone=gone)[part]) / np.linalg.norm(f[part]))
     return dict(seconds=t_solve, build_s=t_build, iters=st["iters"], dof=dof, levels=st["levels"],
-                setup_s=st["setup_s"], cpu=orc.host_cpu())
+                setup_s=st["setup_s"], solve_s=st["solve_s"], capped=bool(st["capped"]), rel_residual=rel,
+                cpu=orc.host_cpu(), threads=threads)
+
+
+def cpu_full(args):
+    """The CPU oracle on the whole workload (not a bounded sample): classify + motions to 1e-8 at
+    --cpu-full edge, with progress lines, at most --cpu-max-iter iterations; one JSON line."""
+    import resource
+    t = time.perf_counter()
+    cb = cpu_baseline(args.cpu_full, args.cpu_threads, args.seed, args.points, max_iter=args.cpu_max_iter,
+                      progress=10, spacing=args.spacing)
+    cb.update(edge=args.cpu_full, wall_s=time.perf_counter() - t, nproc=os.cpu_count(),
+              maxrss_gb=resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1e6)
+    print(json.dumps(cb), flush=True)
 
 
 def batch_structures(count, seed):
@@ -358,7 +382,7 @@ def settle_cpu_sample(size, seed, quantile, threads):
     same generator, strengths at the same quantile of its standing used shares."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as orc
-    occ, theta = structure(size, seed)
+    occ, theta = structure(size, seed, spacing=spacing)
     sites, held, e, g = sh.cells_from_occ(occ, theta)
     mats = np.zeros((len(sites), 6))
     mats[:, 0], mats[:, 1], mats[:, 2], mats[:, 3:] = 1.0, e, g, 1.0
@@ -730,6 +754,9 @@ def main():
     ap.add_argument("--points", type=int, default=64, help="point loads on top of gravity (1024: the 512^3 world)")
     ap.add_argument("--cpu-sample", type=int, default=64, help="edge of the CPU oracle's sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-full", type=int, default=0, metavar="N",
+                    help="run the CPU oracle alone on the whole N^3 workload (progress lines, one JSON line)")
+    ap.add_argument("--cpu-max-iter", type=int, default=100000, help="cpu-full: the oracle's iteration cap")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-errors", dest="errors", action="store_false",
                     help="skip the error report (displacements and gradient against tight FP64 solves)")
@@ -769,6 +796,9 @@ def main():
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is single-GPU by design", file=sys.stderr)
         sys.exit(2)
+    if args.cpu_full:
+        cpu_full(args)
+        return
     if args.batch:
         batch_bench(args)
         return

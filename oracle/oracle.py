@@ -105,6 +105,7 @@ def lib():
         L.orc_cell_stiffness.argtypes = [vp, ctypes.c_size_t, vp, vp]
         L.orc_energy.argtypes = [vp, vp, vp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_progress.argtypes = [ctypes.c_int]
         L.orc_threads.restype = ctypes.c_int
         L.mor_new.restype = vp
         L.mor_new.argtypes = [ctypes.c_int64, vp, vp, vp, ctypes.c_int64, vp, vp, ctypes.c_double]

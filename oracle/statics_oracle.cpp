@@ -36,6 +36,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -277,6 +278,9 @@ Six shift(Six x, V3 r) {
 
 // bodies.rs:51-69: contiguous chunks over the host's hardware threads; serial below 256.
 int g_threads = 0;  // 0: std::thread::hardware_concurrency()
+// progress lines on stderr for long runs (an addition of the oracle): every level built,
+// and every g_progress CG iterations; 0: none
+int g_progress = 0;
 int threads_now() {
   int t = g_threads > 0 ? g_threads : (int)std::thread::hardware_concurrency();
   return t < 1 ? 1 : t;
@@ -587,16 +591,35 @@ Level Level::coarser(const std::vector<size_t>& up, size_t count) {
   for (size_t k = 0; k < down.size(); k++)
     for (size_t q = 0; q < down[k].size(); q++) from[down[k][q].first].emplace_back(k, q);
   const Level& fine = *this;
-  std::vector<Row> kp = each<Row>(up.size(), [&](size_t k) { return fine.ask(k, [&](size_t o) { return fine.down[o]; }); });
-  std::vector<Row> rows_c = each<Row>(count, [&](size_t cc) {
-    Row row;
-    for (auto& [k, q] : fine.from[cc]) {
-      M6 pt = transpose(fine.down[k][q].second);
-      for (auto& [d, m] : kp[k])
-        if (d >= cc) merge(row, d, product(pt, m), 1.0);
-    }
-    return row;
-  });
+  // kp[k] = (K P)_k is needed by the coarse rows whose from lists hold k. The reference
+  // keeps it for every fine row at once; here it is formed per run of coarse rows, for the
+  // fine rows that run needs, which bounds its memory (a kp row is ~15 6x6 blocks) and gives
+  // the same values: kp[k] is a function of k alone, and each coarse row sums its from list
+  // in the same order.
+  std::vector<Row> rows_c(count);
+  const size_t run = std::max<size_t>(4096, count / 16);
+  for (size_t a = 0; a < count; a += run) {
+    const size_t b = std::min(count, a + run);
+    std::vector<size_t> need;
+    for (size_t cc = a; cc < b; cc++)
+      for (auto& kq : fine.from[cc]) need.push_back(kq.first);
+    std::sort(need.begin(), need.end());
+    need.erase(std::unique(need.begin(), need.end()), need.end());
+    std::vector<Row> kp =
+        each<Row>(need.size(), [&](size_t i) { return fine.ask(need[i], [&](size_t o) { return fine.down[o]; }); });
+    std::vector<Row> part = each<Row>(b - a, [&](size_t i) {
+      const size_t cc = a + i;
+      Row row;
+      for (auto& [k, q] : fine.from[cc]) {
+        M6 pt = transpose(fine.down[k][q].second);
+        const Row& kr = kp[std::lower_bound(need.begin(), need.end(), k) - need.begin()];
+        for (auto& [d, m] : kr)
+          if (d >= cc) merge(row, d, product(pt, m), 1.0);
+      }
+      return row;
+    });
+    for (size_t i = 0; i < b - a; i++) rows_c[a + i] = std::move(part[i]);
+  }
   std::vector<M6> own_c(count, M6{});
   std::vector<Pair> pairs_c;
   for (size_t cc = 0; cc < count; cc++)
@@ -742,7 +765,14 @@ std::vector<Six> conjugate_gradients(const Level& level, const std::vector<Six>&
   double rz = dot6(r, z);
   double goal = tol * std::sqrt(dot6(loads, loads));
   iters = 0;
+  const auto t0 = std::chrono::steady_clock::now();
   for (int it = 0; it < max_iter; it++) {
+    if (g_progress > 0 && it % g_progress == 0) {
+      std::fprintf(stderr, "oracle: iteration %d, |r|/|f| %.3e, %.1f s\n", it,
+                   std::sqrt(dot6(r, r)) / std::sqrt(dot6(loads, loads)),
+                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+      std::fflush(stderr);
+    }
     if (std::sqrt(dot6(r, r)) <= goal) break;
     std::vector<Six> ap = level.apply(p, work);
     double alpha = rz / dot6(p, ap);
@@ -786,6 +816,11 @@ std::vector<Six> motions(const Lattice& lat, const std::vector<char>& alive, con
   std::vector<int64_t> node = part;
   std::vector<Level> levels;
   levels.push_back(level_new(lat, part, count, live));
+  if (g_progress > 0) {
+    std::fprintf(stderr, "oracle: level 0: %zu parts, %zu pairs, %.1f s\n", count, levels.back().pairs.size(),
+                 std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    std::fflush(stderr);
+  }
   std::vector<Site> key(nodes.size());
   for (size_t v = 0; v < nodes.size(); v++) key[v] = lat.around[v][0];
   uint32_t l = lat.level;
@@ -800,6 +835,12 @@ std::vector<Six> motions(const Lattice& lat, const std::vector<char>& alive, con
     levels.push_back(std::move(up_level));
     part = std::move(next);
     count = bodies;
+    if (g_progress > 0) {
+      std::fprintf(stderr, "oracle: level %zu: %zu parts, %zu pairs, %.1f s\n", levels.size() - 1, count,
+                   levels.back().pairs.size(),
+                   std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+      std::fflush(stderr);
+    }
   }
   Level& top = levels.back();
   top.direct = cholesky_of(top);
@@ -1414,6 +1455,7 @@ extern "C" {
 struct OrcLattice;
 
 void orc_set_threads(int t) { g_threads = t; }
+void orc_set_progress(int every) { g_progress = every; }
 int orc_threads() { return threads_now(); }
 
 // solids: n sites (3 int64 each, level-0 units), mats n x 6 (rho, e, g, ft, fc, fs), held n bytes.

@@ -3,6 +3,8 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
+#include <string>
 #include <cstdlib>
 #include <cstring>
 #include <unordered_map>
@@ -153,38 +155,49 @@ void timing_collect(Ctx& c) {
   c.tev_used = 0;
 }
 
-// One application in schedule kind `tile`; kSec: the cells carry per-beam sections.
+// One application by the cell-centric kernel (kind 2) with kThreads a workgroup.
+template <class T, class Epi, bool kSec, int kThreads>
+void launch_cells(Ctx& c, const GeoD<T>& g, const T* x, const Epi& epi) {
+  // the dynamic LDS limit is raised once per context and kernel instance (a context
+  // owns its device; each keeps what it set, so contexts on other threads or devices
+  // neither share nor race on it)
+  const void* fn = reinterpret_cast<const void*>(&k_apply_cells<T, Epi, kSec, kThreads>);
+  const DevGeo& d = c.geo;
+  const size_t need = (size_t)std::max(d.class_nodes[0], d.class_nodes[1]) * 6 * sizeof(T);
+  size_t& have = c.lds_attr[fn];
+  if (need > have) {
+    ST_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    have = need;
+  }
+  const uint32_t* bl = d.blist.as<uint32_t>();
+  for (int k = 0; k < 2; k++) {
+    if (d.nclass[k])
+      k_apply_cells<T, Epi, kSec, kThreads>
+          <<<(unsigned)d.nclass[k], kThreads, (size_t)d.class_nodes[k] * 6 * sizeof(T), c.stream>>>(g, bl, x, epi);
+    bl += d.nclass[k];
+  }
+}
+
+// One application in schedule s; kSec: the cells carry per-beam sections.
 template <class T, class Epi, bool kSec>
-void launch_schedule(Ctx& c, int tile, const GeoD<T>& g, const T* x, const Epi& epi) {
+void launch_schedule(Ctx& c, const ApplySchedule& s, const GeoD<T>& g, const T* x, const Epi& epi) {
   const unsigned nb = (unsigned)c.geo.nbricks;
-  switch (tile) {
-    case 0:
-      k_apply<T, Epi, 768, 256, kSec><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
-      break;
-    case 1:
+  if (s.tile == 0) {
+    k_apply<T, Epi, 768, 256, kSec><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
+  } else if (s.tile == 1) {
+    if (s.threads == 64)
+      k_apply<T, Epi, 0, 0, kSec, 64><<<nb, 64, 0, c.stream>>>(g, x, epi);
+    else if (s.threads == 128)
+      k_apply<T, Epi, 0, 0, kSec, 128><<<nb, 128, 0, c.stream>>>(g, x, epi);
+    else
       k_apply<T, Epi, 0, 0, kSec><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
-      break;
-    default: {
-      // the dynamic LDS limit is raised once per context and kernel instance (a context
-      // owns its device; each keeps what it set, so contexts on other threads or devices
-      // neither share nor race on it)
-      const void* fn = reinterpret_cast<const void*>(&k_apply_cells<T, Epi, kSec>);
-      const DevGeo& d = c.geo;
-      const size_t need = (size_t)std::max(d.class_nodes[0], d.class_nodes[1]) * 6 * sizeof(T);
-      size_t& have = c.lds_attr[fn];
-      if (need > have) {
-        ST_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
-        have = need;
-      }
-      const uint32_t* bl = d.blist.as<uint32_t>();
-      for (int k = 0; k < 2; k++) {
-        if (d.nclass[k])
-          k_apply_cells<T, Epi, kSec><<<(unsigned)d.nclass[k], kBlock, (size_t)d.class_nodes[k] * 6 * sizeof(T),
-                                        c.stream>>>(g, bl, x, epi);
-        bl += d.nclass[k];
-      }
-      break;
-    }
+  } else {
+    if (s.threads == 64)
+      launch_cells<T, Epi, kSec, 64>(c, g, x, epi);
+    else if (s.threads == 128)
+      launch_cells<T, Epi, kSec, 128>(c, g, x, epi);
+    else
+      launch_cells<T, Epi, kSec, kBlock>(c, g, x, epi);
   }
 }
 
@@ -208,9 +221,9 @@ void launch_apply(Ctx& c, const T* x, const Epi& epi) {
   const int sched = c.apply_sched[sizeof(T) == 8 ? 1 : 0];
   g.xcd = kSchedules[sched].xcd;
   if (g.carea)
-    launch_schedule<T, Epi, true>(c, kSchedules[sched].tile, g, x, epi);
+    launch_schedule<T, Epi, true>(c, kSchedules[sched], g, x, epi);
   else
-    launch_schedule<T, Epi, false>(c, kSchedules[sched].tile, g, x, epi);
+    launch_schedule<T, Epi, false>(c, kSchedules[sched], g, x, epi);
   ST_HIP(hipGetLastError());
   if (timed) {
     ST_HIP(hipEventRecord(c.tev[2 * c.tev_used + 1], c.stream));
@@ -261,8 +274,9 @@ int tune_one(Ctx& c, T* x, T* y) {
     float ms = 0;
     ST_HIP(hipEventElapsedTime(&ms, e0, e1));
     if (std::getenv("STATICS_HIP_VERBOSE"))
-      std::fprintf(stderr, "statics_hip: schedule %d (kind %d, xcd %d) %s: %.3f ms\n", k, kSchedules[k].tile,
-                   kSchedules[k].xcd, sizeof(T) == 8 ? "fp64" : "fp32", ms / 3);
+      std::fprintf(stderr, "statics_hip: schedule %d (kind %d, xcd %d, %d threads) %s: %.3f ms\n", k,
+                   kSchedules[k].tile, kSchedules[k].xcd, kSchedules[k].threads, sizeof(T) == 8 ? "fp64" : "fp32",
+                   ms / 3);
     if (ms < best_ms) {
       best_ms = ms;
       best = k;
@@ -480,13 +494,18 @@ int st_apply_schedule(StCtx* c, int fp64, int* index, const char** kernel, const
     const ApplySchedule& k = kSchedules[sched];
     if (index) *index = sched;
     static const char* const names[3] = {"k_apply", "k_apply", "k_apply_cells"};
-    static const char* const what[3][2] = {
-        {"node-centric, LDS tile of 768 nodes / 256 cells, bricks in launch order",
-         "node-centric, LDS tile of 768 nodes / 256 cells, XCD brick mapping"},
-        {"node-centric, no LDS tile, bricks in launch order", "node-centric, no LDS tile, XCD brick mapping"},
-        {"cell-centric, brick rows in LDS, bricks in launch order", "cell-centric, brick rows in LDS, XCD brick mapping"}};
+    static const char* const what[3] = {"node-centric, LDS tile of 768 nodes / 256 cells",
+                                        "node-centric, no LDS tile", "cell-centric, brick rows in LDS"};
+    static std::string desc[kNumSchedules];
+    static std::once_flag once;
+    std::call_once(once, [] {
+      for (int i = 0; i < kNumSchedules; i++)
+        desc[i] = std::string(what[kSchedules[i].tile]) +
+                  (kSchedules[i].xcd ? ", XCD brick mapping" : ", bricks in launch order") + ", " +
+                  std::to_string(kSchedules[i].threads) + " threads a brick";
+    });
     if (kernel) *kernel = names[k.tile];
-    if (description) *description = what[k.tile][k.xcd];
+    if (description) *description = desc[sched].c_str();
     if (dispatches) *dispatches = k.tile == 2 ? (c->geo.nclass[0] > 0) + (c->geo.nclass[1] > 0) : 1;
   });
 }

@@ -138,11 +138,13 @@ struct DevGeo {
 
 // Schedules of the fine operator: the kind (0: node by node with the brick's
 // 768 nodes / 256 cells staged in LDS, 1: node by node from memory, 2: cell by cell
-// with LDS rows per node) and whether bricks are dealt to XCDs in contiguous runs.
+// with LDS rows per node), whether bricks are dealt to XCDs in contiguous runs, and the
+// workgroup's threads (fewer threads: more workgroups, each a brick, in flight per CU).
 struct ApplySchedule {
-  int tile, xcd;
+  int tile, xcd, threads;
 };
-constexpr ApplySchedule kSchedules[] = {{0, 1}, {1, 1}, {2, 1}, {0, 0}, {1, 0}, {2, 0}};
+constexpr ApplySchedule kSchedules[] = {{0, 1, 256}, {1, 1, 256}, {2, 1, 256}, {0, 0, 256}, {1, 0, 256},
+                                        {2, 0, 256}, {1, 1, 128}, {2, 1, 128}, {2, 1, 64},  {1, 1, 64}};
 constexpr int kNumSchedules = (int)(sizeof(kSchedules) / sizeof(kSchedules[0]));
 
 // One coarse level (aggregates of the level below, 6 DOF each), block-sparse.

@@ -203,8 +203,8 @@ __device__ __forceinline__ uint32_t xcd_brick(uint32_t w, uint32_t nb) {
 #define ST_APPLY_WAVES 1
 #endif
 
-template <class T, class Epi, int kNodes, int kCells, bool kSec>
-__global__ __launch_bounds__(kBlock, ST_APPLY_WAVES) void k_apply(GeoD<T> g, const T* __restrict__ x, Epi epi) {
+template <class T, class Epi, int kNodes, int kCells, bool kSec, int kThreads = kBlock>
+__global__ __launch_bounds__(kThreads, ST_APPLY_WAVES) void k_apply(GeoD<T> g, const T* __restrict__ x, Epi epi) {
   using V2 = typename Vec2<T>::type;
   __shared__ V2 xs2[kNodes > 0 ? kNodes * 3 : 1];
   __shared__ uint2 cf2[kCells > 0 ? kCells * 3 : 1];
@@ -280,8 +280,8 @@ __global__ __launch_bounds__(kBlock, ST_APPLY_WAVES) void k_apply(GeoD<T> g, con
 // faces whose other cell lies in another brick takes that cell's five beams ending on
 // it (the halo). Then the epilogue runs per node. blist: the bricks of this launch
 // (bricks are launched in classes by node count, each with LDS for its largest).
-template <class T, class Epi, bool kSec>
-__global__ __launch_bounds__(kBlock) void k_apply_cells(GeoD<T> g, const uint32_t* __restrict__ blist,
+template <class T, class Epi, bool kSec, int kThreads = kBlock>
+__global__ __launch_bounds__(kThreads) void k_apply_cells(GeoD<T> g, const uint32_t* __restrict__ blist,
                                                         const T* __restrict__ x, Epi epi) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* yl = reinterpret_cast<T*>(smem_raw);

@@ -46,8 +46,13 @@ def test_apply_parity_generated(ctx, seed):
 
 def test_apply_parity_every_schedule(ctx, monkeypatch):
     """Each schedule of the operator (node by node with and without LDS tiles, cell by cell;
-    XCD brick mapping on and off) against the oracle, with dead beams and a gone set."""
-    for k in range(6):
+    XCD brick mapping on and off, workgroup sizes) against the oracle, with dead beams and a
+    gone set."""
+    monkeypatch.setenv("STATICS_HIP_SCHEDULE", "999")  # clamped to the last schedule
+    P = generated(ctx, 12, 4)
+    count = ctx.apply_schedule(True)[0] + 1
+    assert count >= 6
+    for k in range(count):
         monkeypatch.setenv("STATICS_HIP_SCHEDULE", str(k))
         P = generated(ctx, 20, 4)
         rng = np.random.default_rng(k)
@@ -65,8 +70,9 @@ def test_apply_parity_every_schedule(ctx, monkeypatch):
         assert st.status == 0 and rel(P.from_dev(x_dev), x_or) < 1e-8, k
         for fp64 in (True, False):
             idx, kernel, desc, disp = ctx.apply_schedule(fp64)
-            assert idx == k and kernel == ("k_apply_cells" if k % 3 == 2 else "k_apply"), (k, kernel)
-            assert ("XCD" in desc) == (k < 3) and disp in ((1, 2) if k % 3 == 2 else (1,)), (k, desc, disp)
+            cells = desc.startswith("cell-centric")
+            assert idx == k and kernel == ("k_apply_cells" if cells else "k_apply"), (k, kernel)
+            assert disp in ((1, 2) if cells else (1,)) and "threads a brick" in desc, (k, desc, disp)
 
 
 def test_hierarchy_kept_and_dropped(ctx):
