@@ -215,6 +215,23 @@ def cpu_baseline(size, threads, seed=1, points=64, max_iter=100000, progress=0, 
                 cpu=orc.host_cpu(), threads=threads)
 
 
+def cpu_full_record(edge, args):
+    """The committed record of a --cpu-full run on the same workload (edge, spacing, seed, point
+    loads): the JSON line in profiles/cpu_oracle_<n>.txt whose edge is `edge`, or None."""
+    path = os.path.join(HERE, "profiles", f"cpu_oracle_{args.size}.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("{"):
+                    r = json.loads(line)
+                    if (r.get("edge"), r.get("spacing"), r.get("seed"), r.get("points")) == (
+                            edge, args.spacing, args.seed, args.points):
+                        return r
+    except OSError:
+        pass
+    return None
+
+
 def cpu_full(args):
     """The CPU oracle on the whole workload (not a bounded sample): classify + motions to 1e-8 at
     --cpu-full edge, with progress lines, at most --cpu-max-iter iterations; one JSON line."""
@@ -222,7 +239,8 @@ def cpu_full(args):
     t = time.perf_counter()
     cb = cpu_baseline(args.cpu_full, args.cpu_threads, args.seed, args.points, max_iter=args.cpu_max_iter,
                       progress=10, spacing=args.spacing)
-    cb.update(edge=args.cpu_full, wall_s=time.perf_counter() - t, nproc=os.cpu_count(),
+    cb.update(edge=args.cpu_full, spacing=args.spacing, seed=args.seed, points=args.points,
+              wall_s=time.perf_counter() - t, nproc=os.cpu_count(),
               maxrss_gb=resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 1e6)
     print(json.dumps(cb), flush=True)
 
@@ -970,15 +988,31 @@ def main():
     cpu = None
     if args.cpu_sample > 0:
         log(f"[bench] cpu oracle on a {args.cpu_sample}^3 sample, {args.cpu_threads} threads")
-        cb = cpu_baseline(args.cpu_sample, args.cpu_threads, args.seed, args.points)
-        per_s = (1.0 / cb["seconds"]) * (cb["dof"] / dof)
-        cpu = {"value": per_s, "unit": "structure solves/sec (256^3, scaled from the sample)",
-               "cores": args.cpu_threads, "kind": "port", "cpu_model": cb["cpu"], "nproc": os.cpu_count(),
-               "sample": f"oracle (C++ FP64 restatement of statics/src/bodies.rs: assembled blocks, smoothed "
-                         f"aggregation, Cholesky top) classify + motions to 1e-8 on the {args.cpu_sample}^3 structure "
-                         f"of the same generator and loads: {cb['seconds']:.2f} s ({cb['setup_s']:.2f} s setup), "
-                         f"{cb['iters']} iterations, {cb['dof']} DOF; scaled linearly by DOF to the 256^3 "
-                         f"structure's {dof} DOF, which flatters the CPU: its iteration count grows with the size"}
+        cb = cpu_baseline(args.cpu_sample, args.cpu_threads, args.seed, args.points, spacing=args.spacing)
+        what = (f"oracle (C++ FP64 restatement of statics/src/bodies.rs: assembled blocks, smoothed aggregation, "
+                f"Cholesky top) classify + motions to 1e-8 on the {args.cpu_sample}^3 structure of the same generator "
+                f"and loads: {cb['seconds']:.2f} s ({cb['setup_s']:.2f} s setup), {cb['iters']} iterations, "
+                f"{cb['dof']} DOF")
+        full = cpu_full_record(n, args)
+        ref = cpu_full_record(args.cpu_sample, args)
+        if full and ref and not full["capped"]:
+            # the whole workload's time was measured (the oracle on every DOF to 1e-8, on this kind of
+            # host): the live sample is scaled by the measured ratio of the two runs
+            factor = full["seconds"] / ref["seconds"]
+            per_s = 1.0 / (cb["seconds"] * factor)
+            unit = f"structure solves/sec ({n}^3; the live sample times the measured {n}^3 / sample ratio)"
+            what += (f"; the oracle on the whole {n}^3 workload ({full['dof']} DOF, {full['iters']} iterations, "
+                     f"{full['seconds']:.0f} s: {full['setup_s']:.0f} s setup, {full['solve_s']:.0f} s solve, "
+                     f"{full['maxrss_gb']:.0f} GB, {full['threads']} threads, {full['cpu']}) against "
+                     f"{ref['seconds']:.2f} s for the sample in the same run set: ratio {factor:.1f} "
+                     f"(profiles/cpu_oracle_{n}.txt)")
+        else:
+            per_s = (1.0 / cb["seconds"]) * (cb["dof"] / dof)
+            unit = f"structure solves/sec ({n}^3, scaled linearly by DOF from the sample)"
+            what += (f"; scaled linearly by DOF to the {n}^3 structure's {dof} DOF, which flatters the CPU: its "
+                     f"iteration count grows with the size")
+        cpu = {"value": per_s, "unit": unit, "cores": args.cpu_threads, "kind": "port", "cpu_model": cb["cpu"],
+               "nproc": os.cpu_count(), "sample": what}
         log(f"[bench] cpu sample: {cb}")
 
     value = 1e3 / ms_per_step

@@ -182,23 +182,14 @@ void launch_cells(Ctx& c, const GeoD<T>& g, const T* x, const Epi& epi) {
 template <class T, class Epi, bool kSec>
 void launch_schedule(Ctx& c, const ApplySchedule& s, const GeoD<T>& g, const T* x, const Epi& epi) {
   const unsigned nb = (unsigned)c.geo.nbricks;
-  if (s.tile == 0) {
+  if (s.tile == 0)
     k_apply<T, Epi, 768, 256, kSec><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
-  } else if (s.tile == 1) {
-    if (s.threads == 64)
-      k_apply<T, Epi, 0, 0, kSec, 64><<<nb, 64, 0, c.stream>>>(g, x, epi);
-    else if (s.threads == 128)
-      k_apply<T, Epi, 0, 0, kSec, 128><<<nb, 128, 0, c.stream>>>(g, x, epi);
-    else
-      k_apply<T, Epi, 0, 0, kSec><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
-  } else {
-    if (s.threads == 64)
-      launch_cells<T, Epi, kSec, 64>(c, g, x, epi);
-    else if (s.threads == 128)
-      launch_cells<T, Epi, kSec, 128>(c, g, x, epi);
-    else
-      launch_cells<T, Epi, kSec, kBlock>(c, g, x, epi);
-  }
+  else if (s.tile == 1)
+    k_apply<T, Epi, 0, 0, kSec><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
+  else if (s.tile == 3)
+    k_apply_nodes2<T, Epi, kSec><<<nb, kBlock, 0, c.stream>>>(g, x, epi);
+  else
+    launch_cells<T, Epi, kSec, kBlock>(c, g, x, epi);
 }
 
 template <class T, class Epi>
@@ -493,9 +484,10 @@ int st_apply_schedule(StCtx* c, int fp64, int* index, const char** kernel, const
     const int sched = c->apply_sched[fp64 ? 1 : 0];
     const ApplySchedule& k = kSchedules[sched];
     if (index) *index = sched;
-    static const char* const names[3] = {"k_apply", "k_apply", "k_apply_cells"};
-    static const char* const what[3] = {"node-centric, LDS tile of 768 nodes / 256 cells",
-                                        "node-centric, no LDS tile", "cell-centric, brick rows in LDS"};
+    static const char* const names[4] = {"k_apply", "k_apply", "k_apply_cells", "k_apply_nodes2"};
+    static const char* const what[4] = {"node-centric, LDS tile of 768 nodes / 256 cells",
+                                        "node-centric, no LDS tile", "cell-centric, brick rows in LDS",
+                                        "node-centric, no LDS tile, both cells' loads together"};
     static std::string desc[kNumSchedules];
     static std::once_flag once;
     std::call_once(once, [] {

@@ -138,13 +138,16 @@ struct DevGeo {
 
 // Schedules of the fine operator: the kind (0: node by node with the brick's
 // 768 nodes / 256 cells staged in LDS, 1: node by node from memory, 2: cell by cell
-// with LDS rows per node), whether bricks are dealt to XCDs in contiguous runs, and the
+// with LDS rows per node, 3: node by node from memory with both cells' loads issued
+// together), whether bricks are dealt to XCDs in contiguous runs, and the
 // workgroup's threads (fewer threads: more workgroups, each a brick, in flight per CU).
 struct ApplySchedule {
   int tile, xcd, threads;
 };
-constexpr ApplySchedule kSchedules[] = {{0, 1, 256}, {1, 1, 256}, {2, 1, 256}, {0, 0, 256}, {1, 0, 256},
-                                        {2, 0, 256}, {1, 1, 128}, {2, 1, 128}, {2, 1, 64},  {1, 1, 64}};
+// (64- and 128-thread workgroups were measured at 256^3 and never won: 0.58-1.38 ms against
+// 0.58 (FP32) and 0.93 ms (FP64) with 256 threads.)
+constexpr ApplySchedule kSchedules[] = {{0, 1, 256}, {1, 1, 256}, {2, 1, 256}, {0, 0, 256},
+                                        {1, 0, 256}, {2, 0, 256}, {3, 1, 256}, {3, 0, 256}};
 constexpr int kNumSchedules = (int)(sizeof(kSchedules) / sizeof(kSchedules[0]));
 
 // One coarse level (aggregates of the level below, 6 DOF each), block-sparse.

@@ -272,6 +272,111 @@ __global__ __launch_bounds__(kThreads, ST_APPLY_WAVES) void k_apply(GeoD<T> g, c
   }
 }
 
+// Face F's load in one cell from the cell's mask (all beams), its face motions xv (zero on
+// faces that are not free), e and g: written into y.
+template <class T, int F, bool kSec>
+__device__ __forceinline__ void face_row_of(const BeamGeo& G, T e, T gg, uint32_t mask, const T* rho,
+                                            const T (*xv)[6], T* y) {
+  mask &= face_beams(F);
+  const CellCoef<T> c0 = cell_coef<T>(G, 0, e, gg), c1 = cell_coef<T>(G, 1, e, gg);
+  if (mask == face_beams(F)) {
+    face_row_full<T, F, kSec>(c0, c1, rho, xv, y);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; k++) y[k] = T(0);
+    if (mask) face_row_masked<T, F, kSec>(c0, c1, mask, rho, xv, y);
+  }
+}
+
+__device__ __forceinline__ void load_faces(const uint32_t* cfaces, uint32_t c, uint32_t* fi) {
+  const uint2* fp = reinterpret_cast<const uint2*>(cfaces + 6 * (size_t)c);
+  const uint2 f01 = fp[0], f23 = fp[1], f45 = fp[2];
+  fi[0] = f01.x; fi[1] = f01.y; fi[2] = f23.x; fi[3] = f23.y; fi[4] = f45.x; fi[5] = f45.y;
+}
+
+// The operator node by node with all of a node's loads issued before its arithmetic: its
+// two cells' face tables, masks, e and g together, then their twelve face rows together,
+// so that a node waits on three trips to memory (node, cells, faces) where node_row's
+// cell-after-cell order waits on five. One workgroup per brick, no LDS tile.
+template <class T, class Epi, bool kSec>
+__global__ __launch_bounds__(kBlock) void k_apply_nodes2(GeoD<T> g, const T* __restrict__ x, Epi epi) {
+  __shared__ double red[kBlock / 64];
+  const uint32_t b = g.xcd ? xcd_brick(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint32_t s = g.bstruct[b];
+  if (epi.skip(s)) {
+    if (Epi::kReduce && threadIdx.x == 0) epi.partial[b] = 0.0;
+    return;
+  }
+  const uint32_t n0 = g.nbase[b], n_end = g.nbase[b + 1];
+  const BeamGeo G = g.bgeo[s];
+  double part = 0.0;
+  for (uint32_t v = n0 + threadIdx.x; v < n_end; v += blockDim.x) {
+    typename Epi::Pre pr;
+    if constexpr (Epi::kPre) pr = epi.pre(v);  // in flight during the gathers below
+    const uint8_t fl = g.nflags[v], kind = g.nkind[v];
+    const int2 cells = reinterpret_cast<const int2*>(g.ncell)[v];
+    const bool live = !(fl & (kHeld | kGone));
+    const bool ha = live && cells.x >= 0, hb = live && cells.y >= 0;
+    // cell 0 stands in for an absent cell: its loads are in bounds and their values unused
+    const uint32_t ca = ha ? (uint32_t)cells.x : 0u, cb = hb ? (uint32_t)cells.y : 0u;
+    uint32_t fa[6], fb[6];
+    load_faces(g.cfaces, ca, fa);
+    load_faces(g.cfaces, cb, fb);
+    const uint32_t ma = g.cmask[ca], mb = g.cmask[cb];
+    const T ea = g.ce[ca], ga = g.cg[ca], eb = g.ce[cb], gb = g.cg[cb];
+    T xa[6][6], xb[6][6];
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+      load6<T>(x + 6 * (size_t)(fa[f] & ~kNotFree), xa[f]);
+      load6<T>(x + 6 * (size_t)(fb[f] & ~kNotFree), xb[f]);
+    }
+#pragma unroll
+    for (int f = 0; f < 6; f++)
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        xa[f][k] = (fa[f] & kNotFree) ? T(0) : xa[f][k];
+        xb[f][k] = (fb[f] & kNotFree) ? T(0) : xb[f][k];
+      }
+    const T* rhoa = kSec ? g.carea + 15 * (size_t)ca : nullptr;
+    const T* rhob = kSec ? g.carea + 15 * (size_t)cb : nullptr;
+    T ya[6], yb[6], xo[6];
+    const int o = kind >> 1;
+    // the node is face 2o+1 of the cell below it (a) and face 2o of the cell above (b)
+    if (o == 0) {
+      face_row_of<T, 1, kSec>(G, ea, ga, ma, rhoa, xa, ya);
+      face_row_of<T, 0, kSec>(G, eb, gb, mb, rhob, xb, yb);
+#pragma unroll
+      for (int k = 0; k < 6; k++) xo[k] = ha ? xa[1][k] : xb[0][k];
+    } else if (o == 1) {
+      face_row_of<T, 3, kSec>(G, ea, ga, ma, rhoa, xa, ya);
+      face_row_of<T, 2, kSec>(G, eb, gb, mb, rhob, xb, yb);
+#pragma unroll
+      for (int k = 0; k < 6; k++) xo[k] = ha ? xa[3][k] : xb[2][k];
+    } else {
+      face_row_of<T, 5, kSec>(G, ea, ga, ma, rhoa, xa, ya);
+      face_row_of<T, 4, kSec>(G, eb, gb, mb, rhob, xb, yb);
+#pragma unroll
+      for (int k = 0; k < 6; k++) xo[k] = ha ? xa[5][k] : xb[4][k];
+    }
+    T y[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) y[k] = ha ? (hb ? ya[k] + yb[k] : ya[k]) : (hb ? yb[k] : T(0));
+    if (!ha && !hb) load6<T>(x + 6 * (size_t)v, xo);  // a held or gone node (its row of x)
+    if constexpr (Epi::kPre)
+      part += epi.node_pre(v, y, s, xo, pr);
+    else
+      part += epi.node(v, y, s, xo);
+  }
+  if (Epi::kReduce) {
+    const double tot = block_sum(part, red);
+    if (threadIdx.x == 0) epi.partial[b] = tot;
+  }
+  if constexpr (Epi::kTail) {
+    __syncthreads();
+    epi.tail(b, s);
+  }
+}
+
 // The operator cell by cell (the other schedule): one workgroup per brick, one
 // thread per free cell applies the cell's 15 beams once and adds the loads of each
 // face into the LDS row of that face's node when the node is the brick's. A node
@@ -409,6 +514,11 @@ struct EpiBase {
   // cell from the motions and loads it already holds (x^T K x = sum over cells of
   // x_c . (K_c x_c)), so that its epilogue only stores the row (store()).
   static constexpr bool kCellDot = false;
+  // kPre: the epilogue's own reads of node n (Pre, from pre(n)) can be issued before the
+  // operator's gathers and consumed by node_pre(); k_apply_nodes2 does so, overlapping
+  // their latency with the gathers'
+  static constexpr bool kPre = false;
+  struct Pre {};
 };
 
 template <class T>
@@ -479,11 +589,20 @@ struct EpiResidRestrict : EpiBase {
   const uint8_t* nkind;
   const BeamGeo* bgeo;
   T* c;
-  __device__ double node(uint32_t n, const T* a, uint32_t, const T*) const {
-    T bv[6], rv[6];
-    load6<T>(b + 6 * (size_t)n, bv);
+  static constexpr bool kPre = true;
+  struct Pre {
+    T b[6];
+  };
+  __device__ Pre pre(uint32_t n) const {
+    Pre p;
+    load6<T>(b + 6 * (size_t)n, p.b);
+    return p;
+  }
+  __device__ double node(uint32_t n, const T* a, uint32_t s, const T* xn) const { return node_pre(n, a, s, xn, pre(n)); }
+  __device__ double node_pre(uint32_t n, const T* a, uint32_t, const T*, const Pre& p) const {
+    T rv[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) rv[k] = bv[k] - a[k];
+    for (int k = 0; k < 6; k++) rv[k] = p.b[k] - a[k];
     store6<T>(r + 6 * (size_t)n, rv);
     return 0.0;
   }
@@ -528,11 +647,9 @@ __host__ __device__ constexpr int sym_idx(int i, int j) {
 }
 constexpr int kSym = 21;
 
+// out = M v, M symmetric 6x6 given by its upper triangle a (registers)
 template <class T>
-__device__ __forceinline__ void sym6v(const T* __restrict__ m, const T* v, T* out) {
-  T a[kSym];
-#pragma unroll
-  for (int k = 0; k < kSym; k++) a[k] = m[k];
+__device__ __forceinline__ void sym6v_reg(const T* a, const T* v, T* out) {
 #pragma unroll
   for (int i = 0; i < 6; i++) {
     T s = 0;
@@ -540,6 +657,13 @@ __device__ __forceinline__ void sym6v(const T* __restrict__ m, const T* v, T* ou
     for (int j = 0; j < 6; j++) s += a[sym_idx(i, j)] * v[j];
     out[i] = s;
   }
+}
+template <class T>
+__device__ __forceinline__ void sym6v(const T* __restrict__ m, const T* v, T* out) {
+  T a[kSym];
+#pragma unroll
+  for (int k = 0; k < kSym; k++) a[k] = m[k];
+  sym6v_reg<T>(a, v, out);
 }
 
 // 6x6 row-major times vector
@@ -564,13 +688,25 @@ struct EpiSmooth : EpiBase {
   const double* step;  // per structure
   const double* bdot = nullptr;
   T* z;
-  __device__ double node(uint32_t n, const T* a, uint32_t s, const T* xn) const {
-    T bv[6], xv[6], t[6], d[6];
-    load6<T>(b + 6 * (size_t)n, bv);
+  static constexpr bool kPre = true;
+  struct Pre {
+    T b[6], m[kSym];
+  };
+  __device__ Pre pre(uint32_t n) const {
+    Pre p;
+    load6<T>(b + 6 * (size_t)n, p.b);
+#pragma unroll
+    for (int k = 0; k < kSym; k++) p.m[k] = dinv[kSym * (size_t)n + k];
+    return p;
+  }
+  __device__ double node(uint32_t n, const T* a, uint32_t s, const T* xn) const { return node_pre(n, a, s, xn, pre(n)); }
+  __device__ double node_pre(uint32_t n, const T* a, uint32_t s, const T* xn, const Pre& p) const {
+    T xv[6], t[6], d[6];
+    const T* bv = p.b;
     load6<T>(xn, xv);
 #pragma unroll
     for (int k = 0; k < 6; k++) t[k] = bv[k] - a[k];
-    sym6v<T>(dinv + kSym * (size_t)n, t, d);
+    sym6v_reg<T>(p.m, t, d);
     const T w = (T)step[s];
 #pragma unroll
     for (int k = 0; k < 6; k++) xv[k] += w * d[k];
@@ -598,13 +734,24 @@ struct EpiResidJac : EpiBase {
   const double* step;
   T* r1;
   T* sout;
-  __device__ double node(uint32_t n, const T* a, uint32_t s, const T*) const {
-    T bv[6], t[6], d[6];
-    load6<T>(b + 6 * (size_t)n, bv);
+  static constexpr bool kPre = true;
+  struct Pre {
+    T b[6], m[kSym];
+  };
+  __device__ Pre pre(uint32_t n) const {
+    Pre p;
+    load6<T>(b + 6 * (size_t)n, p.b);
 #pragma unroll
-    for (int k = 0; k < 6; k++) t[k] = bv[k] - a[k];
+    for (int k = 0; k < kSym; k++) p.m[k] = dinv[kSym * (size_t)n + k];
+    return p;
+  }
+  __device__ double node(uint32_t n, const T* a, uint32_t s, const T* xn) const { return node_pre(n, a, s, xn, pre(n)); }
+  __device__ double node_pre(uint32_t n, const T* a, uint32_t s, const T*, const Pre& p) const {
+    T t[6], d[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) t[k] = p.b[k] - a[k];
     store6<T>(r1 + 6 * (size_t)n, t);
-    sym6v<T>(dinv + kSym * (size_t)n, t, d);
+    sym6v_reg<T>(p.m, t, d);
     const T w = (T)step[s];
 #pragma unroll
     for (int k = 0; k < 6; k++) d[k] *= w;
@@ -621,11 +768,24 @@ struct EpiProlong : EpiBase {
   const T* dinv;
   const double* step;
   T* x;
-  __device__ double node(uint32_t n, const T* a, uint32_t s, const T* un) const {
+  static constexpr bool kPre = true;
+  struct Pre {
+    T x[6], m[kSym];
+  };
+  __device__ Pre pre(uint32_t n) const {
+    Pre p;
+    load6<T>(x + 6 * (size_t)n, p.x);
+#pragma unroll
+    for (int k = 0; k < kSym; k++) p.m[k] = dinv[kSym * (size_t)n + k];
+    return p;
+  }
+  __device__ double node(uint32_t n, const T* a, uint32_t s, const T* un) const { return node_pre(n, a, s, un, pre(n)); }
+  __device__ double node_pre(uint32_t n, const T* a, uint32_t s, const T* un, const Pre& p) const {
     T uv[6], xv[6], d[6];
     load6<T>(un, uv);
-    load6<T>(x + 6 * (size_t)n, xv);
-    sym6v<T>(dinv + kSym * (size_t)n, a, d);
+#pragma unroll
+    for (int k = 0; k < 6; k++) xv[k] = p.x[k];
+    sym6v_reg<T>(p.m, a, d);
     const T w = (T)step[s];
 #pragma unroll
     for (int k = 0; k < 6; k++) xv[k] += uv[k] - w * d[k];

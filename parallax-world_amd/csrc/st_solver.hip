@@ -2788,6 +2788,8 @@ struct Solver {
   // the levels above it are dropped (visiting them cost hundreds of tiny launches an
   // iteration in the gamma-cycles).
   void make_dense_top() {
+    // (a top of up to 1,024 aggregates, at 256^3 level 5 with 820, saved 2% of an
+    // iteration and took 0.5 s more to invert: not kept)
     constexpr int64_t kDenseMax = 256;
     int ld = -1;
     for (size_t li = 2; li + 1 < c.levels.size(); li++) {

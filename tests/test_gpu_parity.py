@@ -71,7 +71,7 @@ def test_apply_parity_every_schedule(ctx, monkeypatch):
         for fp64 in (True, False):
             idx, kernel, desc, disp = ctx.apply_schedule(fp64)
             cells = desc.startswith("cell-centric")
-            assert idx == k and kernel == ("k_apply_cells" if cells else "k_apply"), (k, kernel)
+            assert idx == k and kernel.startswith("k_apply_cells" if cells else "k_apply"), (k, kernel)
             assert disp in ((1, 2) if cells else (1,)) and "threads a brick" in desc, (k, desc, disp)
 
 
