@@ -215,6 +215,25 @@ def cpu_baseline(size, threads, seed=1, points=64, max_iter=100000, progress=0, 
                 cpu=orc.host_cpu(), threads=threads)
 
 
+def short_solve(args):
+    """The --size structure uploaded, every operator schedule timed (the autotuner) and a mixed solve of
+    --short-solve iterations under the bench's loads: a brief run of every operator kernel, for counters."""
+    import ctypes
+    occ, theta = structure(args.size, args.seed, spacing=args.spacing)
+    sites, held, e, g = sh.cells_from_occ(occ, theta)
+    ctx = sh.Context().upload(sites, held, e, g)
+    joined = ctx.joined_to_held()
+    loads = node_loads(occ, ctx.node_faces(), joined, ctx.node_held(), np.random.default_rng(args.seed),
+                       n_points=args.points)
+    ctx._check(ctx.L.st_set_loads(ctx.h, sh._p(np.ascontiguousarray(loads))), "st_set_loads")
+    c = ctypes.c_int64()
+    ctx._check(ctx.L.st_classify(ctx.h, ctypes.byref(c)), "st_classify")
+    o = sh.StSolveOpts(sh.MIXED, 1e-8, args.short_solve, 0, 0, 1)
+    s = sh.StStats()
+    rc = ctx.L.st_solve_resident(ctx.h, ctypes.byref(o), ctypes.byref(s))
+    print(f"[bench] short solve: rc {rc}, {s.iters} iterations", file=sys.stderr, flush=True)
+
+
 def cpu_full_record(edge, args):
     """The committed record of a --cpu-full run on the same workload (edge, spacing, seed, point
     loads): the JSON line in profiles/cpu_oracle_<n>.txt whose edge is `edge`, or None."""
@@ -799,6 +818,9 @@ def main():
     ap.add_argument("--mechanics", type=int, default=0, metavar="N",
                     help="instead: mechanics::solve::Solver on the device for the N^3 structure's system")
     ap.add_argument("--mechanics-cpu-sample", type=int, default=48, help="mechanics: edge of the oracle's sample")
+    ap.add_argument("--short-solve", type=int, default=0, metavar="ITERS",
+                    help="upload the structure, time every operator schedule and run a mixed solve of ITERS "
+                         "iterations (for counter runs: every operator kernel and epilogue, briefly)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="end with three device copies of 1 GiB (the calibration kernel of the traffic counters)")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -816,6 +838,9 @@ def main():
         sys.exit(2)
     if args.cpu_full:
         cpu_full(args)
+        return
+    if args.short_solve:
+        short_solve(args)
         return
     if args.batch:
         batch_bench(args)
