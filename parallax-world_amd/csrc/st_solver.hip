@@ -64,42 +64,49 @@ __device__ __forceinline__ void shift_add(const T* y, const double* r, T* out) {
   out[4] += y[4];
   out[5] += y[5];
 }
-// 6x6 inverse by Gauss-Jordan with partial pivoting (FP64). Returns false if singular.
+// Inverse of a symmetric positive definite 6x6 block (FP64, row-major) by its Cholesky
+// factor, A^-1 = L^-T L^-1, every loop unrolled so the block stays in registers. Returns
+// false when a pivot is not positive (the block is not SPD).
 __device__ bool invert6(const double* a_in, double* inv) {
-  double a[36];
-  for (int i = 0; i < 36; i++) {
-    a[i] = a_in[i];
-    inv[i] = (i % 7 == 0) ? 1.0 : 0.0;
-  }
-  for (int col = 0; col < 6; col++) {
-    int piv = col;
-    for (int r = col + 1; r < 6; r++)
-      if (fabs(a[6 * r + col]) > fabs(a[6 * piv + col])) piv = r;
-    if (!(fabs(a[6 * piv + col]) > 0.0)) return false;
-    if (piv != col)
-      for (int k = 0; k < 6; k++) {
-        double t = a[6 * col + k];
-        a[6 * col + k] = a[6 * piv + k];
-        a[6 * piv + k] = t;
-        t = inv[6 * col + k];
-        inv[6 * col + k] = inv[6 * piv + k];
-        inv[6 * piv + k] = t;
-      }
-    const double d = 1.0 / a[6 * col + col];
-    for (int k = 0; k < 6; k++) {
-      a[6 * col + k] *= d;
-      inv[6 * col + k] *= d;
-    }
-    for (int r = 0; r < 6; r++) {
-      if (r == col) continue;
-      const double f = a[6 * r + col];
-      if (f != 0.0)
-        for (int k = 0; k < 6; k++) {
-          a[6 * r + k] -= f * a[6 * col + k];
-          inv[6 * r + k] -= f * inv[6 * col + k];
-        }
+  double L[6][6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    double d = a_in[6 * j + j];
+#pragma unroll
+    for (int k = 0; k < j; k++) d -= L[j][k] * L[j][k];
+    if (!(d > 0.0)) return false;
+    L[j][j] = sqrt(d);
+    const double r = 1.0 / L[j][j];
+#pragma unroll
+    for (int i = j + 1; i < 6; i++) {
+      double t = 0.5 * (a_in[6 * i + j] + a_in[6 * j + i]);
+#pragma unroll
+      for (int k = 0; k < j; k++) t -= L[i][k] * L[j][k];
+      L[i][j] = t * r;
     }
   }
+  double M[6][6];  // L^-1, lower triangular
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    M[j][j] = 1.0 / L[j][j];
+#pragma unroll
+    for (int i = j + 1; i < 6; i++) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = j; k < i; k++) t -= L[i][k] * M[k][j];
+      M[i][j] = t / L[i][i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+#pragma unroll
+    for (int j = i; j < 6; j++) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = j; k < 6; k++) t += M[k][i] * M[k][j];
+      inv[6 * i + j] = t;
+      inv[6 * j + i] = t;
+    }
   return true;
 }
 
@@ -325,7 +332,7 @@ __global__ void k_sort_keys(int64_t n, const int* label, int* keys, int* vals) {
 
 // ------------------------------------------------------- fine smoother D^-1
 // D_v = sum of the own blocks of the live beams at node v (statics/src/bodies.rs:151-153).
-__global__ void k_dinv(int64_t nnodes, const uint8_t* nflags, const int32_t* ncell, const uint8_t* nkind,
+__global__ __launch_bounds__(256) void k_dinv(int64_t nnodes, const uint8_t* nflags, const int32_t* ncell, const uint8_t* nkind,
                        const double* ce, const double* cg, const double* csec, const uint16_t* cmask,
                        const uint32_t* cstruct, const BeamGeo* bgeo, double* dinv64, float* dinv32, int* bad) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1448,11 +1455,40 @@ __device__ int beams_at(uint32_t v, const uint8_t* nkind, const int32_t* ncell, 
   return nb;
 }
 
+// The ten beam places at v (five at its face of each of its two cells, in beams_at's order)
+// with dead or absent beams marked q = -1: a list indexed by constants, so it stays in
+// registers.
+__device__ __forceinline__ void beams_at10(uint32_t v, const uint8_t* nkind, const int32_t* ncell,
+                                           const uint32_t* cfaces, const uint16_t* cmask, BeamAt* list) {
+  const int o = nkind[v] >> 1;
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const int c = ncell[2 * v + side];
+    const int f = side == 0 ? 2 * o + 1 : 2 * o;
+    const uint32_t m = c >= 0 ? ((uint32_t)cmask[c] & face_beams(f)) : 0u;
+#pragma unroll
+    for (int jj = 0; jj < 5; jj++) {
+      int q = 0;
+      for (int qq = 0, k = 0; qq < 15; qq++)
+        if (kPairA[qq] == f || kPairB[qq] == f) {
+          if (k == jj) q = qq;
+          k++;
+        }
+      BeamAt& b = list[5 * side + jj];
+      const bool live = (m >> q) & 1u;
+      b.q = live ? q : -1;
+      b.sv = kPairA[q] == f ? -1 : 1;
+      b.c = c;
+      b.wf = live ? cfaces[6 * c + (kPairA[q] == f ? kPairB[q] : kPairA[q])] : kNotFree;
+    }
+  }
+}
+
 // Rows of P0 = (I - w D^-1 K) P_rigid at the fine nodes. pass 0: count blocks; pass 1: fill.
 // (K chi_J)(v), the load at v when aggregate J moves rigidly: for v in J only the beams
 // leaving J count (a beam inside J moves rigidly), K_b,vv Sh(r_v); for v not in J the beams
 // reaching into J, K_b,vw Sh(r_w).
-__global__ void k_p0(int pass, int64_t nn, const uint8_t* nflags, const int* label, const double* centre,
+__global__ __launch_bounds__(256) void k_p0(int pass, int64_t nn, const uint8_t* nflags, const int* label, const double* centre,
                      const uint32_t* nkey, const uint8_t* nkind, const int32_t* ncell, const uint32_t* cfaces,
                      const double* ce, const double* cg, const double* csec, const uint16_t* cmask,
                      const uint32_t* cstruct, const BeamGeo* bgeo, const double* dinv, const double* step, int* cnt,
@@ -1464,13 +1500,14 @@ __global__ void k_p0(int pass, int64_t nn, const uint8_t* nflags, const int* lab
     return;
   }
   BeamAt bl[10];
-  const int nbm = beams_at((uint32_t)v, nkind, ncell, cfaces, cmask, bl);
+  beams_at10((uint32_t)v, nkind, ncell, cfaces, cmask, bl);
   const int I0 = label[v];
   int js[kMaxP];
   int nj = 1;
   js[0] = I0;
-  for (int t = 0; t < nbm; t++) {
-    if (bl[t].wf & kNotFree) continue;
+#pragma unroll
+  for (int t = 0; t < 10; t++) {
+    if (bl[t].q < 0 || (bl[t].wf & kNotFree)) continue;
     const int J = label[bl[t].wf];
     if (J != I0 && slot_of(js, nj, J) < 0) {
       if (nj < kMaxP)
@@ -1486,7 +1523,6 @@ __global__ void k_p0(int pass, int64_t nn, const uint8_t* nflags, const int* lab
   const int base = prow[v];
   double* out = pval + 36 * (int64_t)base;
   for (int t = 0; t < nj; t++) pcol[base + t] = js[t];
-  for (int i = 0; i < 36 * nj; i++) out[i] = 0.0;
   const uint32_t sidx = cstruct[ncell[2 * v] >= 0 ? ncell[2 * v] : ncell[2 * v + 1]];
   const BeamGeo G = bgeo[sidx];
   const double sp = 2.0 * G.hd[0];
@@ -1494,48 +1530,68 @@ __global__ void k_p0(int pass, int64_t nn, const uint8_t* nflags, const int* lab
   node_pos((uint32_t)v, nkey, nkind, sp, pv);
   for (int k = 0; k < 3; k++) rv[k] = pv[k] - centre[3 * (int64_t)I0 + k];
   const double zero[3] = {0, 0, 0};
-  for (int t = 0; t < nbm; t++) {
-    const BeamAt& b = bl[t];
-    const bool wfree = !(b.wf & kNotFree);
-    const int Jw = wfree ? label[b.wf] : -1;
-    if (Jw == I0) continue;  // inside the aggregate: rigid, no load
-    double n[3];
-    int cls;
-    pair_n(b.q, G, n, cls);
-    const double e = ce[b.c], g = cg[b.c];
-    const double rho = beam_sec(csec, b.c, b.q), K4 = G.K4[cls] * rho, R2 = G.R2[cls] * (rho * rho);
-    double Bv[18], Gv[18];
-    gmat(b.sv, n, G.hd[cls], zero, Bv);
-    gmat(b.sv, n, G.hd[cls], rv, Gv);
-    beam_coarse_add(Bv, Gv, b.sv, b.sv, n, K4, R2, e, g, out);  // K_b,vv Sh(r_v)
-    if (Jw >= 0) {
-      double pw[3], rw[3], Gw[18];
-      node_pos(b.wf, nkey, nkind, sp, pw);
-      for (int k = 0; k < 3; k++) rw[k] = pw[k] - centre[3 * (int64_t)Jw + k];
-      gmat(-b.sv, n, G.hd[cls], rw, Gw);
-      beam_coarse_add(Bv, Gw, b.sv, -b.sv, n, K4, R2, e, g, out + 36 * slot_of(js, nj, Jw));
-    }
-  }
-  // P[v, J] = [J = I0] Sh(r_v) - w D^-1 (K chi_J)(v)
   const double w = step[sidx];
-  double D[36];
-  for (int i = 0; i < 6; i++)
-    for (int j = 0; j < 6; j++) D[6 * i + j] = dinv[kSym * v + sym_idx(i, j)];
+  // P[v, J] = [J = I0] Sh(r_v) - w D^-1 (K chi_J)(v), one block at a time in registers: the
+  // beams leaving I0 add K_b,vv Sh(r_v) into I0's block, a beam reaching into J != I0 adds
+  // K_b,vw Sh(r_w) into J's (a beam inside I0 moves rigidly and adds nothing)
   for (int t = 0; t < nj; t++) {
-    double* blk = out + 36 * t;
+    const int J = js[t];
+    double acc[36];
+#pragma unroll
+    for (int i = 0; i < 36; i++) acc[i] = 0.0;
+#pragma unroll 1
+    for (int bt = 0; bt < 10; bt++) {
+      const BeamAt& b = bl[bt];
+      if (b.q < 0) continue;
+      const bool wfree = !(b.wf & kNotFree);
+      const int Jw = wfree ? label[b.wf] : -1;
+      if (Jw == I0) continue;
+      const bool vv = t == 0, vw = Jw == J;
+      if (!vv && !vw) continue;
+      double n[3];
+      int cls;
+      pair_n(b.q, G, n, cls);
+      const double e = ce[b.c], g = cg[b.c];
+      const double rho = beam_sec(csec, b.c, b.q), K4 = G.K4[cls] * rho, R2 = G.R2[cls] * (rho * rho);
+      double Bv[18];
+      gmat(b.sv, n, G.hd[cls], zero, Bv);
+      if (vv) {
+        double Gv[18];
+        gmat(b.sv, n, G.hd[cls], rv, Gv);
+        beam_coarse_add(Bv, Gv, b.sv, b.sv, n, K4, R2, e, g, acc);  // K_b,vv Sh(r_v)
+      }
+      if (vw) {
+        double pw[3], rw[3], Gw[18];
+        node_pos(b.wf, nkey, nkind, sp, pw);
+        for (int k = 0; k < 3; k++) rw[k] = pw[k] - centre[3 * (int64_t)Jw + k];
+        gmat(-b.sv, n, G.hd[cls], rw, Gw);
+        beam_coarse_add(Bv, Gw, b.sv, -b.sv, n, K4, R2, e, g, acc);  // K_b,vw Sh(r_w)
+      }
+    }
+    double D[36];
+    for (int i = 0; i < 6; i++)
+      for (int j = 0; j < 6; j++) D[6 * i + j] = dinv[kSym * v + sym_idx(i, j)];
     double tmp[36];
+#pragma unroll
     for (int i = 0; i < 36; i++) tmp[i] = 0.0;
-    mat6_mul_add(D, blk, tmp);
-    for (int i = 0; i < 36; i++) blk[i] = -w * tmp[i];
+    mat6_mul_add(D, acc, tmp);
+#pragma unroll
+    for (int i = 0; i < 36; i++) acc[i] = -w * tmp[i];
+    if (t == 0) {
+      double S[36];
+      shift_mat(rv, S);
+#pragma unroll
+      for (int i = 0; i < 36; i++) acc[i] += S[i];
+    }
+    double* blk = out + 36 * t;
+#pragma unroll
+    for (int i = 0; i < 36; i++) blk[i] = acc[i];
   }
-  double S[36];
-  shift_mat(rv, S);
-  for (int i = 0; i < 36; i++) out[i] += S[i];
 }
 
 
 // Rows of P = (I - w D^-1 A) P_rigid at an explicit level (rows i, columns the next level's J).
-__global__ void k_pc(int pass, int smooth, int64_t n, const int* rowptr, const int* colA, const double* A,
+__global__ __launch_bounds__(256) void k_pc(int pass, int smooth, int64_t n, const int* rowptr, const int* colA, const double* A,
                      const double* dinv, const double* step, const uint32_t* ast, const int* parent,
                      const double* centre, const double* centre_up, int* cnt, const int* prow, int* pcol, double* pval,
                      int* overflow) {
