@@ -1238,6 +1238,11 @@ __global__ void k_c_init(int64_t n, double* x) {
   if (I >= n) return;
   for (int k = 0; k < 6; k++) x[6 * I + k] = 1.0 + (double)((7 * I + 3 * k) % 11) / 11.0;
 }
+__global__ void k_c_init32(int64_t n, float* x) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  for (int k = 0; k < 6; k++) x[6 * I + k] = 1.0 + (float)((7 * I + 3 * k) % 11) / 11.0;
+}
 
 // restrict fine -> level 1: c[I] = sum over nodes v of I of Sh(r_v)^T t[v]
 // brick_agg[b] = the first level-1 aggregate whose first node lies in brick b or later
@@ -2562,28 +2567,76 @@ struct Solver {
     });
   }
 
+  // The level's FP16 operator for the mixed V-cycle (each block scaled to its own largest
+  // entry): made right after the level is built when the level will run in FP16, so that
+  // its smoother's step is estimated on the same operator.
+  void make_half(Level& lv) {
+    lv.a16.ensure((size_t)lv.nblocks * 36 * sizeof(_Float16));
+    lv.ascale.ensure((size_t)lv.nblocks * sizeof(float));
+    k_to_f16<<<nblk(lv.nblocks), 256, 0, st>>>(lv.nblocks, lv.a64.as<double>(), lv.a16.as<_Float16>(),
+                                              lv.ascale.as<float>());
+    ST_HIP(hipGetLastError());
+    lv.a32.release();
+    lv.half = true;
+  }
+
+  // The smoother's step 4/(3 lambda) per structure, lambda the largest eigenvalue of D^-1 A
+  // by 12 power iterations (statics/src/bodies.rs:180-190). A level that runs in FP16 is
+  // iterated on its FP16 blocks in FP32 (a quarter of the bytes of the FP64 blocks; the
+  // estimate only sets a damping factor).
   void coarse_step(Level& lv, const std::vector<int64_t>& seg) {
     const int64_t n = lv.n;
-    DBuf v, w, sq;
-    v.ensure(n * 6 * sizeof(double));
-    w.ensure(n * 6 * sizeof(double));
+    DBuf v, w, sq, zr, neg;
+    const size_t vb = lv.half ? sizeof(float) : sizeof(double);
+    v.ensure(n * 6 * vb);
+    w.ensure(n * 6 * vb);
     sq.ensure(n * sizeof(double));
     DBuf segd;
     segd.ensure((ns + 1) * sizeof(int64_t));
     ST_HIP(hipMemcpyAsync(segd.p, seg.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
-    double* a = v.as<double>();
-    double* b = w.as<double>();
-    k_c_init<<<nblk(n), 256, 0, st>>>(n, a);
     const int iters = 12;
-    for (int it = 0; it < iters; it++) {
-      k_c_dinvA<double><<<nblk(8 * n), 256, 0, st>>>(n, lv.rowptr.as<int>(), lv.col.as<int>(), lv.a64.as<double>(),
-                                                 lv.dinv64.as<double>(), a, b);
-      if (it >= iters - 2) {
-        k_c_sq<double><<<nblk(n), 256, 0, st>>>(n, b, sq.as<double>());
-        k_reduce_seg<<<ns, kBlock, 0, st>>>(sq.as<double>(), segd.as<int64_t>(), c.scal.as<double>(), kScal,
-                                            it == iters - 2 ? S_LAM_A : S_LAM_B);
+    if (lv.half) {
+      // w = D^-1 A v as the coarse residual kernel's out2 = step D^-1 (r - A v) with r = 0 and
+      // step = -1 for every structure; no structure is skipped (done = NULL)
+      zr.ensure(n * 6 * sizeof(float));
+      ST_HIP(hipMemsetAsync(zr.p, 0, n * 6 * sizeof(float), st));
+      std::vector<double> m1(ns, -1.0);
+      neg.ensure(ns * sizeof(double));
+      ST_HIP(hipMemcpyAsync(neg.p, m1.data(), ns * sizeof(double), hipMemcpyHostToDevice, st));
+      DBuf scratch;
+      scratch.ensure(n * 6 * sizeof(float));
+      float* a = v.as<float>();
+      float* b = w.as<float>();
+      k_c_init32<<<nblk(n), 256, 0, st>>>(n, a);
+      for (int it = 0; it < iters; it++) {
+        k_c_resid<float, _Float16, 0, 4, 0><<<coarse_blocks<float>(n, false), 256, 0, st>>>(
+            0, n, lv.agg_struct.as<uint32_t>(), nullptr, lv.rowptr.as<int>(), lv.col.as<int>(), lv.a16.as<_Float16>(),
+            lv.ascale.as<float>(), lv.dinv32.as<float>(), neg.as<double>(), zr.as<float>(), a, scratch.as<float>(),
+            nullptr, b);
+        if (it >= iters - 2) {
+          k_c_sq<float><<<nblk(n), 256, 0, st>>>(n, b, sq.as<double>());
+          k_reduce_seg<<<ns, kBlock, 0, st>>>(sq.as<double>(), segd.as<int64_t>(), c.scal.as<double>(), kScal,
+                                              it == iters - 2 ? S_LAM_A : S_LAM_B);
+        }
+        std::swap(a, b);
       }
-      std::swap(a, b);
+      ST_HIP(hipGetLastError());
+      ST_HIP(hipStreamSynchronize(st));  // before the scratch vectors go
+      scratch.release();
+    } else {
+      double* a = v.as<double>();
+      double* b = w.as<double>();
+      k_c_init<<<nblk(n), 256, 0, st>>>(n, a);
+      for (int it = 0; it < iters; it++) {
+        k_c_dinvA<double><<<nblk(8 * n), 256, 0, st>>>(n, lv.rowptr.as<int>(), lv.col.as<int>(), lv.a64.as<double>(),
+                                                   lv.dinv64.as<double>(), a, b);
+        if (it >= iters - 2) {
+          k_c_sq<double><<<nblk(n), 256, 0, st>>>(n, b, sq.as<double>());
+          k_reduce_seg<<<ns, kBlock, 0, st>>>(sq.as<double>(), segd.as<int64_t>(), c.scal.as<double>(), kScal,
+                                              it == iters - 2 ? S_LAM_A : S_LAM_B);
+        }
+        std::swap(a, b);
+      }
     }
     std::vector<double> sc((size_t)ns * kScal);
     ST_HIP(hipMemcpyAsync(sc.data(), c.scal.p, sc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -2601,6 +2654,8 @@ struct Solver {
     w.release();
     sq.release();
     segd.release();
+    zr.release();
+    neg.release();
   }
 
   void alloc_level_vectors(Level& lv) {
@@ -2646,6 +2701,20 @@ struct Solver {
       c.last_ndof = ndof;
     }
     if (!ndof) return;
+    // the mixed V-cycle's two largest coarse operators (levels 1 and 2, nearly all of the
+    // coarse blocks) in FP16, each block with its own scale: at 256^3 the same iterations and
+    // 4% less solve time than FP32. FP16 on levels 3 and 4 as well saves 3% more there, but
+    // leaves some structures of the 512-structure batch at 248 iterations instead of 96.
+    // STATICS_HIP_HALF_LEVELS sets how many levels, STATICS_HIP_HALF_BLOCKS keeps the
+    // operators of fewer blocks than it in FP32, STATICS_HIP_HALF=0 keeps them all.
+    int64_t kHalfBlocks = 0;
+    if (const char* e = std::getenv("STATICS_HIP_HALF_BLOCKS")) kHalfBlocks = std::max<int64_t>(0, std::atoll(e));
+    int kHalfLevels = 2;
+    if (const char* e = std::getenv("STATICS_HIP_HALF_LEVELS")) kHalfLevels = std::max(0, std::atoi(e));
+    auto maybe_half = [&](Level& lv) {
+      const int li = (int)c.levels.size();
+      if (mixed && c.half_coarse && li < c.f64_from && li < kHalfLevels && lv.nblocks >= kHalfBlocks) make_half(lv);
+    };
     int64_t prev = ndof;
     DBuf label_tmp;
     label_tmp.ensure(nn * sizeof(int));
@@ -2718,6 +2787,7 @@ struct Solver {
         skeys.release();
         if (c.smoothed) {
           build_sa_coarse(below, lv, (int)c.levels.size() < c.sa_levels);
+          maybe_half(lv);
           coarse_step(lv, seg);
           alloc_level_vectors(lv);
           c.levels.push_back(std::move(lv));
@@ -2742,6 +2812,7 @@ struct Solver {
         finish_rows(lv, cntb, launch);
         cntb.release();
       }
+      maybe_half(lv);
       coarse_step(lv, seg);
       alloc_level_vectors(lv);
       c.levels.push_back(std::move(lv));
@@ -2750,27 +2821,12 @@ struct Solver {
     }
     label_tmp.release();
     if (c.dense_top) make_dense_top();
-    // the mixed V-cycle's two largest coarse operators (levels 1 and 2, nearly all of the
-    // coarse blocks) in FP16, each block with its own scale: at 256^3 the same iterations and
-    // 4% less solve time than FP32. FP16 on levels 3 and 4 as well saves 3% more there, but
-    // leaves some structures of the 512-structure batch at 248 iterations instead of 96.
-    // STATICS_HIP_HALF_LEVELS sets how many levels, STATICS_HIP_HALF_BLOCKS keeps the
-    // operators of fewer blocks than it in FP32, STATICS_HIP_HALF=0 keeps them all.
-    int64_t kHalfBlocks = 0;
-    if (const char* e = std::getenv("STATICS_HIP_HALF_BLOCKS")) kHalfBlocks = std::max<int64_t>(0, std::atoll(e));
-    int kHalfLevels = 2;
-    if (const char* e = std::getenv("STATICS_HIP_HALF_LEVELS")) kHalfLevels = std::max(0, std::atoi(e));
-    if (mixed && c.half_coarse)
-      for (size_t li = 0; li < c.levels.size() && (int)li < c.f64_from && (int)li < kHalfLevels; li++) {
-        Level& lv = c.levels[li];
-        if (lv.dense || lv.nblocks < kHalfBlocks) continue;
-        lv.a16.ensure((size_t)lv.nblocks * 36 * sizeof(_Float16));
-        lv.ascale.ensure((size_t)lv.nblocks * sizeof(float));
-        k_to_f16<<<nblk(lv.nblocks), 256, 0, st>>>(lv.nblocks, lv.a64.as<double>(), lv.a16.as<_Float16>(),
-                                                  lv.ascale.as<float>());
-        ST_HIP(hipGetLastError());
-        lv.a32.release();
-        lv.half = true;
+    // (a level made the dense top keeps its FP64 operator: it is inverted, not iterated)
+    for (Level& lv : c.levels)
+      if (lv.dense && lv.half) {
+        lv.a16.release();
+        lv.ascale.release();
+        lv.half = false;
       }
     if (mixed)  // the FP32 levels' FP64 copies served the Galerkin products and the step estimates only
       for (size_t li = 0; li < c.levels.size() && (int)li < c.f64_from; li++)
@@ -2860,10 +2916,10 @@ struct Solver {
     if (ld < 0) return;
     for (size_t li = ld + 1; li < c.levels.size(); li++) {
       Level& l2 = c.levels[li];
-      for (DBuf* b : {&l2.rowptr, &l2.col, &l2.a64, &l2.a32, &l2.dinv64, &l2.dinv32, &l2.centre, &l2.parent, &l2.rep,
-                      &l2.key, &l2.kid_ptr, &l2.kids, &l2.agg_struct, &l2.step_d, &l2.r64, &l2.x64, &l2.t64, &l2.u64,
-                      &l2.v64, &l2.r32, &l2.x32, &l2.t32, &l2.u32, &l2.v32, &l2.prow, &l2.pcol, &l2.pval64,
-                      &l2.pval32, &l2.pt_ptr, &l2.pt_blk, &l2.pblk_row})
+      for (DBuf* b : {&l2.rowptr, &l2.col, &l2.a64, &l2.a32, &l2.a16, &l2.ascale, &l2.dinv64, &l2.dinv32,
+                      &l2.centre, &l2.parent, &l2.rep, &l2.key, &l2.kid_ptr, &l2.kids, &l2.agg_struct, &l2.step_d,
+                      &l2.r64, &l2.x64, &l2.t64, &l2.u64, &l2.v64, &l2.r32, &l2.x32, &l2.t32, &l2.u32, &l2.v32,
+                      &l2.prow, &l2.pcol, &l2.pval64, &l2.pval32, &l2.pt_ptr, &l2.pt_blk, &l2.pblk_row})
         b->release();
     }
     c.levels.resize(ld + 1);
