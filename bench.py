@@ -231,7 +231,8 @@ def short_solve(args):
     o = sh.StSolveOpts(sh.MIXED, 1e-8, args.short_solve, 0, 0, 1)
     s = sh.StStats()
     rc = ctx.L.st_solve_resident(ctx.h, ctypes.byref(o), ctypes.byref(s))
-    print(f"[bench] short solve: rc {rc}, {s.iters} iterations", file=sys.stderr, flush=True)
+    print(f"[bench] short solve: rc {rc}, {s.iters} iterations, setup {s.setup_ms:.1f} ms", file=sys.stderr,
+          flush=True)
 
 
 def cpu_full_record(edge, args):

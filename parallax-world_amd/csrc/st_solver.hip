@@ -1985,60 +1985,83 @@ __global__ __launch_bounds__(256) void k_sa_a1_w(const int* rows, int64_t n1, co
     return;
   }
   for (int x = threadIdx.x; x < 36 * m; x += blockDim.x) acc[x] = 0.0;
-  // products split by groups of kRG output rows: thread task (row group, node, beam); each
-  // task reads a column block P0[w, J] once for its kRG rows
+  // products split by groups of kRG output rows: thread task (row group, node, j), j < 10
+  // the block K[v, w] of beam j's far end w, j = 10 the node's own block K[v, v], the sum
+  // over its beams (one pass over P0's row of v for all ten); each task reads a column
+  // block P0[., J] once for its kRG rows
   constexpr int NRG = 6 / kRG;
   for (int base = 0; base < nv; base += kV) {
     const int nvc = min(kV, nv - base);
     load_chunk(base, nvc);
-    for (int it = threadIdx.x; it < nvc * 10 * NRG; it += blockDim.x) {
+    for (int it = threadIdx.x; it < nvc * 11 * NRG; it += blockDim.x) {
       const int rg = it % NRG, rest = it / NRG;
       const int vi = rest % nvc, j = rest / nvc;
-      const uint32_t wf = s_wf[j][vi];
-      if (wf == kDeadBeam) continue;
-      const int side = j / 5, o = s_o[vi];
-      const int f = side == 0 ? 2 * o + 1 : 2 * o;
-      const int q = face_beam_q(f, j % 5);
-      const int sv = kPairA[q] == f ? -1 : 1;
-      const bool wfree = !(wf & kNotFree);
-      const int v = s_v[vi];
+      const int o = s_o[vi];
       const double* PvI = pval + 36 * (int64_t)s_blk[vi];
       double pc[kRG][6];  // columns rg kRG .. of P0[v, I]
       for (int rr = 0; rr < kRG; rr++)
         for (int i = 0; i < 6; i++) pc[rr][i] = PvI[6 * i + rg * kRG + rr];
       const BeamGeo G = bgeo[s_sidx[vi]];
-      double n[3];
-      int cls;
-      pair_n(q, G, n, cls);
-      const double e = s_e[side][vi], g = s_g[side][vi];
-      const double rho = beam_sec(csec, s_c[side][vi], q);
-      for (int sd = 0; sd < (wfree ? 2 : 1); sd++) {
-        double qr[kRG][6];  // rows rg kRG .. of P0[v,I]^T K_b
+      double qr[kRG][6];  // rows rg kRG .. of P0[v,I]^T K[v, row]
+      int row;
+      if (j < 10) {
+        const uint32_t wf = s_wf[j][vi];
+        if (wf == kDeadBeam || (wf & kNotFree)) continue;
+        const int side = j / 5, f = side == 0 ? 2 * o + 1 : 2 * o;
+        const int q = face_beam_q(f, j % 5);
+        const int sv = kPairA[q] == f ? -1 : 1;
+        double n[3];
+        int cls;
+        pair_n(q, G, n, cls);
+        const double rho = beam_sec(csec, s_c[side][vi], q);
         for (int rr = 0; rr < kRG; rr++)
-          beam_row(pc[rr], sv, sd == 0 ? sv : -sv, n, G.hd[cls], G.K4[cls] * rho, G.R2[cls] * (rho * rho), e, g,
-                   qr[rr]);
-        const int row = sd == 0 ? v : (int)wf;
-        // start each task's column loop at a different column, so that the tasks of a
-        // wave (mostly nodes of one aggregate) do not add into the same block at once
-        const int k2a = prow[row], len = prow[row + 1] - k2a;
-        for (int kk = 0; kk < len; kk++) {
-          const int k2 = k2a + (kk + rest) % len;
-          const int J = pcol[k2];
-          if (J < I) continue;  // the lower blocks are the transposes of other rows' upper ones
-          const int sl = hfind<kH>(keys, slot, J);
-          const double* Pj = pval + 36 * (int64_t)k2;
-          double o6[kRG][6];
-          for (int rr = 0; rr < kRG; rr++)
-            for (int cc = 0; cc < 6; cc++) o6[rr][cc] = 0.0;
-          for (int k = 0; k < 6; k++)
-            for (int cc = 0; cc < 6; cc++) {
-              const double pk = Pj[6 * k + cc];
-              for (int rr = 0; rr < kRG; rr++) o6[rr][cc] += qr[rr][k] * pk;
-            }
-          double* out = acc + 36 * sl + 6 * rg * kRG;
-          for (int rr = 0; rr < kRG; rr++)
-            for (int cc = 0; cc < 6; cc++) atomicAdd(&out[6 * rr + cc], o6[rr][cc]);
+          beam_row(pc[rr], sv, -sv, n, G.hd[cls], G.K4[cls] * rho, G.R2[cls] * (rho * rho), s_e[side][vi],
+                   s_g[side][vi], qr[rr]);
+        row = (int)wf;
+      } else {
+        bool any = false;
+        for (int rr = 0; rr < kRG; rr++)
+          for (int k = 0; k < 6; k++) qr[rr][k] = 0.0;
+        for (int jb = 0; jb < 10; jb++) {
+          if (s_wf[jb][vi] == kDeadBeam) continue;
+          any = true;
+          const int side = jb / 5, f = side == 0 ? 2 * o + 1 : 2 * o;
+          const int q = face_beam_q(f, jb % 5);
+          const int sv = kPairA[q] == f ? -1 : 1;
+          double n[3];
+          int cls;
+          pair_n(q, G, n, cls);
+          const double rho = beam_sec(csec, s_c[side][vi], q);
+          for (int rr = 0; rr < kRG; rr++) {
+            double t[6];
+            beam_row(pc[rr], sv, sv, n, G.hd[cls], G.K4[cls] * rho, G.R2[cls] * (rho * rho), s_e[side][vi],
+                     s_g[side][vi], t);
+            for (int k = 0; k < 6; k++) qr[rr][k] += t[k];
+          }
         }
+        if (!any) continue;
+        row = s_v[vi];
+      }
+      // start each task's column loop at a different column, so that the tasks of a
+      // wave (mostly nodes of one aggregate) do not add into the same block at once
+      const int k2a = prow[row], len = prow[row + 1] - k2a;
+      for (int kk = 0; kk < len; kk++) {
+        const int k2 = k2a + (kk + rest) % len;
+        const int J = pcol[k2];
+        if (J < I) continue;  // the lower blocks are the transposes of other rows' upper ones
+        const int sl = hfind<kH>(keys, slot, J);
+        const double* Pj = pval + 36 * (int64_t)k2;
+        double o6[kRG][6];
+        for (int rr = 0; rr < kRG; rr++)
+          for (int cc = 0; cc < 6; cc++) o6[rr][cc] = 0.0;
+        for (int k = 0; k < 6; k++)
+          for (int cc = 0; cc < 6; cc++) {
+            const double pk = Pj[6 * k + cc];
+            for (int rr = 0; rr < kRG; rr++) o6[rr][cc] += qr[rr][k] * pk;
+          }
+        double* out = acc + 36 * sl + 6 * rg * kRG;
+        for (int rr = 0; rr < kRG; rr++)
+          for (int cc = 0; cc < 6; cc++) atomicAdd(&out[6 * rr + cc], o6[rr][cc]);
       }
     }
   }
@@ -2475,8 +2498,8 @@ struct Solver {
                                                           pcol, pval, sbad.as<int>());
                          });
     // output rows per task of the A1 product (STATICS_HIP_A1_ROWS: 1, 2, 3 or 6): three rows
-    // read each P0 block once for three; 275 ms at 256^3 with one row, 207 ms with three
-    // (six rows take 256 VGPRs, one wave per SIMD)
+    // read each P0 block once for three; at 256^3 two and three rows within 1% of each other,
+    // six rows (213 VGPRs, two waves per SIMD) 60 ms slower
     int a1_rows = 3;
     if (const char* e = std::getenv("STATICS_HIP_A1_ROWS")) a1_rows = std::atoi(e);
     wave_rows(lv, [&](int pass, int slots, int* cnt, const int* rows, int64_t nr) {
