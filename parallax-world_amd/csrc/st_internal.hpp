@@ -235,6 +235,9 @@ struct Ctx {
   // the coarse residual layout chosen per (level index, FP32, aggregates, blocks), kept
   // for the hierarchies rebuilt on this upload (cleared by st_upload)
   std::vector<std::pair<std::array<int64_t, 4>, int>> coarse_layouts;
+  // the Galerkin product's task layout (1: by pairs, 0: grouped by column) per (level
+  // index, row width class, 0, 0), kept for the context's life (a choice of speed only)
+  std::vector<std::pair<std::array<int64_t, 4>, int>> gal_modes;
   // The hierarchy (fine smoother and coarse levels) is kept between solves of the same
   // operator: anything that changes the operator clears hier_valid; hier_key holds the
   // settings it was built with.

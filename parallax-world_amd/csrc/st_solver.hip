@@ -1729,14 +1729,23 @@ __device__ __forceinline__ int block_scan256(int v, int* wsum) {
 }
 
 // Next level = P^T A P by rows, one workgroup per row I (rows[blockIdx.x], or
-// blockIdx.x): row I += P[i,I]^T sum over blocks (i, j) of A of A[i,j] P[j,:], the
-// (column entry, block of A's row) pairs spread over the threads 256 at a time.
+// blockIdx.x): row I = sum over the nodes i of column I of P, 256 at a time, of
+// P[i,I]^T A[i,j] P[j,:]. Pass 1 groups the chunk's terms by j: one task per column j met
+// in the chunk's rows of A forms T = sum over i of P[i,I]^T A[i,j] (A[i,j] = A[j,i]^T, row
+// j of A read against the chunk's nodes, looked up in an LDS hash), then adds T P[j,J] for
+// the J of P's row j: one pass over P's row j per j instead of one per (i, j). A chunk
+// meeting kWL columns or more, or every chunk when pairs is set, takes one task per (i, j).
 template <int pass, int kSlots, int kH, int kRG = 1>
 __global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, const int* pt_ptr,
                                                   const int* pt_blk, const int* pblk_row, const int* prow,
                                                   const int* pcol, const double* pval, const int* rowptrA,
                                                   const int* colA, const double* Aa, int* rowcnt, const int* rowptr,
-                                                  int* col, double* A, int* overflow) {
+                                                  int* col, double* A, int* overflow, int pairs) {
+  // hashes of the chunk's nodes i (-> their block of P) and of its columns j, and the
+  // columns j one chunk groups by (kWH - kWL > 256); smaller beside the widest rows' LDS
+  constexpr int kIH = pass ? 512 : 1;
+  constexpr int kWH = pass ? (kSlots >= 448 ? 1024 : 2048) : 1;
+  constexpr int kWL = kSlots >= 448 ? 512 : 1024;
   __shared__ int keys[kH];
   __shared__ int slot[kH];
   __shared__ int sorted[kSlots];
@@ -1745,12 +1754,15 @@ __global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, c
   __shared__ int pre[257];
   __shared__ int rowi[256], blki[256];
   __shared__ double acc[36 * kSlots];
+  __shared__ int ikey[kIH], ival[kIH], wkey[kWH], wl[pass ? kWL : 1];
+  __shared__ int nw;
   if ((int64_t)blockIdx.x >= nI) return;
   const int I = rows ? rows[blockIdx.x] : (int)blockIdx.x;
   for (int h = threadIdx.x; h < kH; h += blockDim.x) keys[h] = -1;
   __syncthreads();
   if (threadIdx.x == 0) hinsert<kH>(keys, I);
   const int k0 = pt_ptr[I], k1 = pt_ptr[I + 1];
+  constexpr int NRG = 6 / kRG;
   int m = 0;
   for (int run = 0; run < 2; run++) {  // run 0: columns; run 1 (pass 1): products
     if (run == 1) {
@@ -1768,6 +1780,12 @@ __global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, c
     }
     for (int c0 = k0; c0 < k1; c0 += 256) {
       const int nc = min(256, k1 - c0);
+      if (run == 1) {
+        for (int h = threadIdx.x; h < kIH; h += blockDim.x) ikey[h] = -1;
+        for (int h = threadIdx.x; h < kWH; h += blockDim.x) wkey[h] = -1;
+        if (threadIdx.x == 0) nw = 0;
+        __syncthreads();
+      }
       int len = 0;
       if ((int)threadIdx.x < nc) {
         const int blk = pt_blk[c0 + threadIdx.x];
@@ -1775,15 +1793,20 @@ __global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, c
         blki[threadIdx.x] = blk;
         rowi[threadIdx.x] = i;
         len = rowptrA[i + 1] - rowptrA[i];
+        if (run == 1) {  // a node appears once in a column of P
+          unsigned h = hslot<kIH>(i);
+          while (atomicCAS(&ikey[h], -1, i) != -1) h = (h + 1) & (kIH - 1);
+          ival[h] = blk;
+        }
       }
       const int incl = block_scan256(len, wsum);
       pre[threadIdx.x + 1] = incl;
       if (threadIdx.x == 0) pre[0] = 0;
       __syncthreads();
       const int total = pre[nc];
-      const int split = run == 0 ? 1 : 6 / kRG;  // run 1: one thread per group of kRG output rows
-      for (int it2 = threadIdx.x; it2 < total * split; it2 += blockDim.x) {
-        const int it = it2 / split, rg = it2 % split;
+      // run 0: the columns J of P's rows j; run 1: the distinct j (at most kWL + 255 enter
+      // the hash, which therefore never fills)
+      for (int it = threadIdx.x; it < total; it += blockDim.x) {
         int lo = 0, hi = nc - 1;  // the last t with pre[t] <= it
         while (lo < hi) {
           const int mid = (lo + hi + 1) >> 1;
@@ -1792,43 +1815,119 @@ __global__ __launch_bounds__(256) void k_sa_gal_w(const int* rows, int64_t nI, c
           else
             hi = mid - 1;
         }
-        const int i = rowi[lo];
-        const int b = rowptrA[i] + (it - pre[lo]);
-        const int j = colA[b];
+        const int j = colA[rowptrA[rowi[lo]] + (it - pre[lo])];
         if (run == 0) {
           for (int k2 = prow[j]; k2 < prow[j + 1]; k2++) hinsert<kH>(keys, pcol[k2]);
           continue;
         }
-        // rows rg kRG .. of P[i,I]^T A[i,j], then those rows of its products with P[j,:]
-        // (each block of A and P read once for the group's rows)
-        const double* PiI = pval + 36 * (int64_t)blki[lo];
-        const double* Ab = Aa + 36 * (int64_t)b;
-        double qr[kRG][6];
-        for (int rr = 0; rr < kRG; rr++)
-          for (int k = 0; k < 6; k++) qr[rr][k] = 0.0;
-        for (int i2 = 0; i2 < 6; i2++)
-          for (int k = 0; k < 6; k++) {
-            const double ab = Ab[6 * i2 + k];
-            for (int rr = 0; rr < kRG; rr++) qr[rr][k] += PiI[6 * i2 + rg * kRG + rr] * ab;
+        if (*(volatile int*)&nw >= kWL) continue;
+        unsigned h = hslot<kWH>(j);
+        for (;;) {
+          const int old = atomicCAS(&wkey[h], -1, j);
+          if (old == -1) {
+            const int p = atomicAdd(&nw, 1);
+            if (p < kWL) wl[p] = j;
+            break;
           }
-        const int k2a = prow[j], len = prow[j + 1] - k2a;
-        for (int kk = 0; kk < len; kk++) {
-          const int k2 = k2a + (kk + it) % len;
-          const int J = pcol[k2];
-          if (J < I) continue;  // the lower blocks are the transposes of other rows' upper ones
-          const int sl = hfind<kH>(keys, slot, J);
-          const double* Pj = pval + 36 * (int64_t)k2;
-          double o6[kRG][6];
+          if (old == j) break;
+          h = (h + 1) & (kWH - 1);
+        }
+      }
+      __syncthreads();
+      if (run == 0) continue;
+      const int nwc = nw;
+      if (!pairs && nwc < kWL) {
+        for (int it2 = threadIdx.x; it2 < nwc * NRG; it2 += blockDim.x) {
+          const int rg = it2 % NRG, j = wl[it2 / NRG];
+          // rows rg kRG .. of T = sum over the chunk's i of P[i,I]^T A[i,j]
+          double qr[kRG][6];
           for (int rr = 0; rr < kRG; rr++)
-            for (int cc = 0; cc < 6; cc++) o6[rr][cc] = 0.0;
-          for (int k = 0; k < 6; k++)
-            for (int cc = 0; cc < 6; cc++) {
-              const double pk = Pj[6 * k + cc];
-              for (int rr = 0; rr < kRG; rr++) o6[rr][cc] += qr[rr][k] * pk;
+            for (int k = 0; k < 6; k++) qr[rr][k] = 0.0;
+          for (int b = rowptrA[j]; b < rowptrA[j + 1]; b++) {
+            const int i = colA[b];
+            unsigned h = hslot<kIH>(i);
+            int blk = -1;
+            for (;;) {
+              const int k = ikey[h];
+              if (k == i) {
+                blk = ival[h];
+                break;
+              }
+              if (k == -1) break;
+              h = (h + 1) & (kIH - 1);
             }
-          double* out = acc + 36 * sl + 6 * rg * kRG;
+            if (blk < 0) continue;
+            const double* PiI = pval + 36 * (int64_t)blk;
+            const double* Ab = Aa + 36 * (int64_t)b;  // A[j,i]: A[i,j][i2][k] = Ab[6 k + i2]
+            for (int i2 = 0; i2 < 6; i2++)
+              for (int k = 0; k < 6; k++) {
+                const double ab = Ab[6 * k + i2];
+                for (int rr = 0; rr < kRG; rr++) qr[rr][k] += PiI[6 * i2 + rg * kRG + rr] * ab;
+              }
+          }
+          const int k2a = prow[j], len2 = prow[j + 1] - k2a;
+          for (int kk = 0; kk < len2; kk++) {
+            const int k2 = k2a + (kk + it2) % len2;
+            const int J = pcol[k2];
+            if (J < I) continue;  // the lower blocks are the transposes of other rows' upper ones
+            const int sl = hfind<kH>(keys, slot, J);
+            const double* Pj = pval + 36 * (int64_t)k2;
+            double o6[kRG][6];
+            for (int rr = 0; rr < kRG; rr++)
+              for (int cc = 0; cc < 6; cc++) o6[rr][cc] = 0.0;
+            for (int k = 0; k < 6; k++)
+              for (int cc = 0; cc < 6; cc++) {
+                const double pk = Pj[6 * k + cc];
+                for (int rr = 0; rr < kRG; rr++) o6[rr][cc] += qr[rr][k] * pk;
+              }
+            double* out = acc + 36 * sl + 6 * rg * kRG;
+            for (int rr = 0; rr < kRG; rr++)
+              for (int cc = 0; cc < 6; cc++) atomicAdd(&out[6 * rr + cc], o6[rr][cc]);
+          }
+        }
+      } else {
+        // one task per (i, j): rows of P[i,I]^T A[i,j], then of its products with P[j,:]
+        for (int it2 = threadIdx.x; it2 < total * NRG; it2 += blockDim.x) {
+          const int it = it2 / NRG, rg = it2 % NRG;
+          int lo = 0, hi = nc - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= it)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+          const int b = rowptrA[rowi[lo]] + (it - pre[lo]);
+          const int j = colA[b];
+          const double* PiI = pval + 36 * (int64_t)blki[lo];
+          const double* Ab = Aa + 36 * (int64_t)b;
+          double qr[kRG][6];
           for (int rr = 0; rr < kRG; rr++)
-            for (int cc = 0; cc < 6; cc++) atomicAdd(&out[6 * rr + cc], o6[rr][cc]);
+            for (int k = 0; k < 6; k++) qr[rr][k] = 0.0;
+          for (int i2 = 0; i2 < 6; i2++)
+            for (int k = 0; k < 6; k++) {
+              const double ab = Ab[6 * i2 + k];
+              for (int rr = 0; rr < kRG; rr++) qr[rr][k] += PiI[6 * i2 + rg * kRG + rr] * ab;
+            }
+          const int k2a = prow[j], len2 = prow[j + 1] - k2a;
+          for (int kk = 0; kk < len2; kk++) {
+            const int k2 = k2a + (kk + it) % len2;
+            const int J = pcol[k2];
+            if (J < I) continue;
+            const int sl = hfind<kH>(keys, slot, J);
+            const double* Pj = pval + 36 * (int64_t)k2;
+            double o6[kRG][6];
+            for (int rr = 0; rr < kRG; rr++)
+              for (int cc = 0; cc < 6; cc++) o6[rr][cc] = 0.0;
+            for (int k = 0; k < 6; k++)
+              for (int cc = 0; cc < 6; cc++) {
+                const double pk = Pj[6 * k + cc];
+                for (int rr = 0; rr < kRG; rr++) o6[rr][cc] += qr[rr][k] * pk;
+              }
+            double* out = acc + 36 * sl + 6 * rg * kRG;
+            for (int rr = 0; rr < kRG; rr++)
+              for (int cc = 0; cc < 6; cc++) atomicAdd(&out[6 * rr + cc], o6[rr][cc]);
+          }
         }
       }
       __syncthreads();
@@ -2554,6 +2653,14 @@ struct Solver {
     // 58 ms less than with one
     int gal_rows = 3;
     if (const char* e = std::getenv("STATICS_HIP_GAL_ROWS")) gal_rows = std::atoi(e);
+    // Tasks by (i, j) pairs or grouped by j (the same terms added in another order): the
+    // grouping saves the passes over P's rows where the chunk's nodes share many j, and
+    // costs a scan of row j of A per j, the dearer where A's rows are long (at 256^3 level 2
+    // by j 37 ms against 64, level 3 by pairs 2 ms against 10). Chosen per level and row
+    // width class by timing both the first time this context builds it (kept across
+    // uploads: a batch's chunks time it once); STATICS_HIP_GAL_PAIRS=0 or 1 forces.
+    const char* gp = std::getenv("STATICS_HIP_GAL_PAIRS");
+    const int gal_force = gp ? (std::atoi(gp) ? 1 : 0) : -1;
     below.pval32.ensure((size_t)std::max<int64_t>(below.pblocks, 1) * 36 * sizeof(float));
     k_to_f32<<<nblk(below.pblocks * 36), 256, 0, st>>>(below.pblocks * 36, below.pval64.as<double>(),
                                                        below.pval32.as<float>());
@@ -2564,29 +2671,59 @@ struct Solver {
                                                 below.pcol.as<int>(), below.pval64.as<double>(),                    \
                                                 below.rowptr.as<int>(), below.col.as<int>(), below.a64.as<double>(),\
                                                 cnt, lv.rowptr.as<int>(), lv.col.as<int>(), lv.a64.as<double>(),    \
-                                                sbad.as<int>())
-      if (pass == 0)
-        ST_GAL(0, 1, 2048, 1);
-      else if (slots <= 64) {
-        if (gal_rows == 3)
-          ST_GAL(1, 64, 256, 3);
-        else if (gal_rows == 2)
-          ST_GAL(1, 64, 256, 2);
+                                                sbad.as<int>(), pairs)
+      auto launch = [&](int pairs) {
+        if (pass == 0)
+          ST_GAL(0, 1, 2048, 1);
+        else if (slots <= 64) {
+          if (gal_rows == 3)
+            ST_GAL(1, 64, 256, 3);
+          else if (gal_rows == 2)
+            ST_GAL(1, 64, 256, 2);
+          else
+            ST_GAL(1, 64, 256, 1);
+        } else if (slots <= 128) {
+          if (gal_rows == 3)
+            ST_GAL(1, 128, 512, 3);
+          else if (gal_rows == 2)
+            ST_GAL(1, 128, 512, 2);
+          else
+            ST_GAL(1, 128, 512, 1);
+        } else if (slots <= 256)
+          ST_GAL(1, 256, 1024, 1);
         else
-          ST_GAL(1, 64, 256, 1);
-      } else if (slots <= 128) {
-        if (gal_rows == 3)
-          ST_GAL(1, 128, 512, 3);
-        else if (gal_rows == 2)
-          ST_GAL(1, 128, 512, 2);
-        else
-          ST_GAL(1, 128, 512, 1);
-      } else if (slots <= 256)
-        ST_GAL(1, 256, 1024, 1);
-      else
-        ST_GAL(1, 448, 2048, 1);
+          ST_GAL(1, 448, 2048, 1);
+        ST_HIP(hipGetLastError());
+      };
+      if (pass == 0 || gal_force >= 0) {
+        launch(gal_force > 0 ? 1 : 0);
+        return;
+      }
+      const std::array<int64_t, 4> shape{(int64_t)c.levels.size(), (int64_t)slots, 0, 0};
+      for (auto& kv : c.gal_modes)
+        if (kv.first == shape) {
+          launch(kv.second);
+          return;
+        }
+      hipEvent_t e[3];
+      for (auto& x : e) ST_HIP(hipEventCreate(&x));
+      ST_HIP(hipEventRecord(e[0], st));
+      launch(1);
+      ST_HIP(hipEventRecord(e[1], st));
+      launch(0);
+      ST_HIP(hipEventRecord(e[2], st));
+      ST_HIP(hipEventSynchronize(e[2]));
+      float ms1 = 0, ms0 = 0;
+      ST_HIP(hipEventElapsedTime(&ms1, e[0], e[1]));
+      ST_HIP(hipEventElapsedTime(&ms0, e[1], e[2]));
+      for (auto& x : e) (void)hipEventDestroy(x);
+      const int mode = ms1 < ms0 ? 1 : 0;
+      if (mode == 1) launch(1);  // the rows as every later build of this shape makes them
+      c.gal_modes.emplace_back(shape, mode);
+      if (std::getenv("STATICS_HIP_VERBOSE"))
+        std::fprintf(stderr, "statics_hip: Galerkin rows of width <= %d (%lld rows): by pairs %.2f ms, by j %.2f ms\n",
+                     slots, (long long)nr, ms1, ms0);
 #undef ST_GAL
-      ST_HIP(hipGetLastError());
     });
   }
 
