@@ -150,6 +150,16 @@ constexpr ApplySchedule kSchedules[] = {{0, 1, 256}, {1, 1, 256}, {2, 1, 256}, {
                                         {1, 0, 256}, {2, 0, 256}, {3, 1, 256}, {3, 0, 256}};
 constexpr int kNumSchedules = (int)(sizeof(kSchedules) / sizeof(kSchedules[0]));
 
+// A coarse block in FP16 for the mixed V-cycle: the 36 entries (row-major) over the
+// block's scale (its largest entry maps to 2^14), the scale, and the block's column, in
+// 80 bytes (five aligned 16-byte loads).
+struct alignas(16) HalfBlock {
+  _Float16 a[36];
+  float scale;
+  int col;
+};
+static_assert(sizeof(HalfBlock) == 80, "HalfBlock is five 16-byte words");
+
 // One coarse level (aggregates of the level below, 6 DOF each), block-sparse.
 struct Level {
   int64_t n = 0;        // aggregates
@@ -158,7 +168,7 @@ struct Level {
   DBuf rowptr;          // n+1 (int32): blocks of row I at [rowptr[I], rowptr[I+1]); diagonal first
   DBuf col;             // nblocks (int32)
   DBuf a64, a32;        // 36 per block, row-major
-  DBuf a16, ascale;     // the FP16 copy of the blocks, and each block's scale (when half)
+  DBuf hblk;            // the FP16 copy of the blocks (HalfBlock records, when half)
   bool half = false;    // the mixed V-cycle reads the FP16 copy
   DBuf dinv64, dinv32;  // 36 per aggregate
   DBuf centre;          // 3 per aggregate (double, metres, structure-local)

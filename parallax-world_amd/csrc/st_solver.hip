@@ -833,15 +833,20 @@ __global__ void k_coarse_dinv(int64_t n, const int* rowptr, const double* A, dou
 }
 
 // FP16 copy of a level's blocks, each scaled so that its largest entry is 2^14.
-__global__ void k_to_f16(int64_t nb, const double* a, _Float16* h, float* scale) {
+// FP16 copy of a level's blocks (see HalfBlock): each block over its own scale, its
+// largest entry mapped to 2^14
+__global__ void k_to_half_blocks(int64_t nb, const double* a, const int* col, HalfBlock* h) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   double m = 0.0;
   for (int k = 0; k < 36; k++) m = fmax(m, fabs(a[36 * b + k]));
   const double sc = m > 0.0 ? m / 16384.0 : 1.0;
   const float scf = (float)sc;
-  scale[b] = scf;
-  for (int k = 0; k < 36; k++) h[36 * b + k] = (_Float16)(float)(a[36 * b + k] / (double)scf);
+  HalfBlock out;
+  for (int k = 0; k < 36; k++) out.a[k] = (_Float16)(float)(a[36 * b + k] / (double)scf);
+  out.scale = scf;
+  out.col = col[b];
+  h[b] = out;
 }
 
 __global__ void k_to_f32(int64_t n, const double* a, float* b) {
@@ -1074,25 +1079,11 @@ __device__ __forceinline__ void load_entries_nt<double>(const double* p, double*
   const st_d2 a = __builtin_nontemporal_load(reinterpret_cast<const st_d2*>(p));
   v[0] = a.x; v[1] = a.y;
 }
-// Blocks stored in FP16 (levels whose operator streams most of the V-cycle's bytes in the
-// mixed mode), each with its own FP32 scale: four entries a lane, one 8-byte load.
-typedef _Float16 st_h4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void load_entries_h(const _Float16* p, float* v, bool nt) {
-  const st_h4 a = nt ? __builtin_nontemporal_load(reinterpret_cast<const st_h4*>(p))
-                     : *reinterpret_cast<const st_h4*>(p);
-  v[0] = (float)a.x; v[1] = (float)a.y; v[2] = (float)a.z; v[3] = (float)a.w;
-}
-// Entries E i .. E i + E - 1 of block b, in T: from A in T, or (T float) from FP16 with the
-// block's scale.
-template <class T, class TA, int kNT>
-__device__ __forceinline__ void block_entries(const TA* A, const float* scale, int64_t b, int i, T* v) {
+// Entries E i .. E i + E - 1 of block b.
+template <class T, int kNT>
+__device__ __forceinline__ void block_entries(const T* A, int64_t b, int i, T* v) {
   constexpr int E = CoarseLanes<T>::E;
-  if constexpr (std::is_same<TA, _Float16>::value) {
-    load_entries_h(A + 36 * b + E * i, v, kNT != 0);
-    const float sc = scale[b];
-#pragma unroll
-    for (int k = 0; k < E; k++) v[k] *= sc;
-  } else if (kNT) {
+  if (kNT) {
     load_entries_nt<T>(A + 36 * b + E * i, v);
   } else {
     load_entries<T>(A + 36 * b + E * i, v);
@@ -1126,10 +1117,9 @@ __device__ __forceinline__ void load_columns<double>(const double* xr, int c0, d
 // mode 1: out = x + w D^-1 (r - A x) (+ add, when add is not NULL).
 // kU blocks' loads are issued together per lane (4 or 8); kNT: the blocks' loads are
 // non-temporal. Neither changes the sums or their order.
-template <class T, class TA, int kWide, int kU, int kNT>
+template <class T, int kWide, int kU, int kNT>
 __global__ __launch_bounds__(256) void k_c_resid(int mode, int64_t n, const uint32_t* ast, const uint8_t* done,
-                                                 const int* rowptr, const int* col, const TA* A,
-                                                 const float* ascale, const T* dinv,
+                                                 const int* rowptr, const int* col, const T* A, const T* dinv,
                                                  const double* step, const T* r, const T* x, T* out,
                                                  const T* add, T* out2) {
   using CL = CoarseLanes<T>;
@@ -1159,7 +1149,7 @@ __global__ __launch_bounds__(256) void k_c_resid(int mode, int64_t n, const uint
       for (int q = 0; q < kU; q++) cb[q] = col[b + q * G];
 #pragma unroll
       for (int q = 0; q < kU; q++) {
-        block_entries<T, TA, kNT>(A, ascale, (int64_t)(b + q * G), i, av[q]);
+        block_entries<T, kNT>(A, (int64_t)(b + q * G), i, av[q]);
         load_columns<T>(x + 6 * (int64_t)cb[q], c0, xv[q]);
       }
 #pragma unroll
@@ -1169,7 +1159,7 @@ __global__ __launch_bounds__(256) void k_c_resid(int mode, int64_t n, const uint
     }
     for (; b < b1; b += G) {
       T av[E], xv[E];
-      block_entries<T, TA, kNT>(A, ascale, (int64_t)b, i, av);
+      block_entries<T, kNT>(A, (int64_t)b, i, av);
       load_columns<T>(x + 6 * (int64_t)col[b], c0, xv);
 #pragma unroll
       for (int k = 0; k < E; k++) acc[k] += av[k] * xv[k];
@@ -1209,6 +1199,110 @@ template <class T>
 inline unsigned coarse_blocks(int64_t n, bool wide) {
   const int64_t rpb = wide ? 4 : CoarseLanes<T>::kRowsPerBlock;
   return (unsigned)std::max<int64_t>(1, (n + rpb - 1) / rpb);
+}
+
+// The same residual kernel on FP16 blocks (HalfBlock records, five aligned 16-byte loads
+// each, the column and scale inside): G lanes per row, each lane whole blocks in turn
+// (blocks gl, gl + G, ... of the row, two in flight), y = sum of scale (A_h x[col]) in FP32;
+// the lanes' sums meet by a butterfly (the same bits in every lane of the group), and
+// lanes 0-5 finish components 0-5 as k_c_resid does. A lane load covers a whole block
+// (one load instruction per 16 bytes, against one per 8 bytes and two more for x in the
+// lane layout above), which the texture units address at a quarter of the cost. kX: the
+// workgroups of each XCD take one contiguous run of rows, whose columns (neighbouring
+// aggregates) then meet in that XCD's L2. (Non-temporal block loads ran at half speed.)
+typedef unsigned st_u4 __attribute__((ext_vector_type(4)));
+typedef _Float16 st_h2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void load_half_block(const HalfBlock* p, st_u4* q) {
+  const st_u4* s = reinterpret_cast<const st_u4*>(p);
+#pragma unroll
+  for (int k = 0; k < 5; k++) q[k] = s[k];
+}
+__device__ __forceinline__ void load_x6(const float* p, float* v) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float2 a = *reinterpret_cast<const float2*>(p + 2 * k);
+    v[2 * k] = a.x;
+    v[2 * k + 1] = a.y;
+  }
+}
+__device__ __forceinline__ void half_block_madd(const st_u4* q, const float* xv, float* y) {
+  unsigned w[20];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    w[4 * k] = q[k].x;
+    w[4 * k + 1] = q[k].y;
+    w[4 * k + 2] = q[k].z;
+    w[4 * k + 3] = q[k].w;
+  }
+  const float sc = __builtin_bit_cast(float, w[18]);
+#pragma unroll
+  for (int r = 0; r < 6; r++) {
+    float t = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      const int e = 6 * r + j;
+      const st_h2 h = __builtin_bit_cast(st_h2, w[e >> 1]);
+      t += (float)((e & 1) ? h.y : h.x) * xv[j];
+    }
+    y[r] += sc * t;
+  }
+}
+template <int G, int kX>
+__global__ __launch_bounds__(256) void k_c_resid_h(int mode, int64_t n, const uint32_t* ast, const uint8_t* done,
+                                                   const int* rowptr, const HalfBlock* A, const float* dinv,
+                                                   const double* step, const float* r, const float* x, float* out,
+                                                   const float* add, float* out2) {
+  static_assert(G >= 8 && G <= 64 && (G & (G - 1)) == 0, "G: 8 to 64 lanes, a power of two");
+  const int gl = threadIdx.x & (G - 1);
+  const uint32_t wg = kX ? xcd_brick(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int64_t I = ((int64_t)wg * blockDim.x + threadIdx.x) / G;
+  const bool live = I < n && !(done && done[ast[I]]);
+  float y[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (live) {
+    int b = rowptr[I] + gl;
+    const int b1 = rowptr[I + 1];
+    for (; b + G < b1; b += 2 * G) {
+      st_u4 q0[5], q1[5];
+      load_half_block(A + b, q0);
+      load_half_block(A + b + G, q1);
+      float x0[6], x1[6];
+      load_x6(x + 6 * (int64_t)(int)q0[4].w, x0);
+      load_x6(x + 6 * (int64_t)(int)q1[4].w, x1);
+      half_block_madd(q0, x0, y);
+      half_block_madd(q1, x1, y);
+    }
+    if (b < b1) {
+      st_u4 q0[5];
+      load_half_block(A + b, q0);
+      float x0[6];
+      load_x6(x + 6 * (int64_t)(int)q0[4].w, x0);
+      half_block_madd(q0, x0, y);
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1)
+#pragma unroll
+    for (int k = 0; k < 6; k++) y[k] += __shfl_xor(y[k], o, 64);
+  if (!live || gl >= 6) return;
+  const int i = gl;
+  float v[6];
+  load_x6(r + 6 * I, v);
+#pragma unroll
+  for (int k = 0; k < 6; k++) v[k] -= y[k];
+  if (mode == 0) out[6 * I + i] = v[i];
+  if (mode == 0 && !out2) return;
+  float d = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 6; j++) d += dinv[36 * I + 6 * i + j] * v[j];
+  d *= (float)step[ast[I]];
+  if (mode == 0)
+    out2[6 * I + i] = d;
+  else
+    out[6 * I + i] = x[6 * I + i] + d + (add ? add[6 * I + i] : 0.0f);
+}
+template <int G>
+inline unsigned half_blocks_grid(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, (n * G + 255) / 256);
 }
 
 // y = A x (for power iterations), then partial sums of x.x per aggregate are not needed: w = D^-1 A x
@@ -2731,10 +2825,9 @@ struct Solver {
   // entry): made right after the level is built when the level will run in FP16, so that
   // its smoother's step is estimated on the same operator.
   void make_half(Level& lv) {
-    lv.a16.ensure((size_t)lv.nblocks * 36 * sizeof(_Float16));
-    lv.ascale.ensure((size_t)lv.nblocks * sizeof(float));
-    k_to_f16<<<nblk(lv.nblocks), 256, 0, st>>>(lv.nblocks, lv.a64.as<double>(), lv.a16.as<_Float16>(),
-                                              lv.ascale.as<float>());
+    lv.hblk.ensure((size_t)lv.nblocks * sizeof(HalfBlock));
+    k_to_half_blocks<<<nblk(lv.nblocks), 256, 0, st>>>(lv.nblocks, lv.a64.as<double>(), lv.col.as<int>(),
+                                                       lv.hblk.as<HalfBlock>());
     ST_HIP(hipGetLastError());
     lv.a32.release();
     lv.half = true;
@@ -2769,10 +2862,9 @@ struct Solver {
       float* b = w.as<float>();
       k_c_init32<<<nblk(n), 256, 0, st>>>(n, a);
       for (int it = 0; it < iters; it++) {
-        k_c_resid<float, _Float16, 0, 4, 0><<<coarse_blocks<float>(n, false), 256, 0, st>>>(
-            0, n, lv.agg_struct.as<uint32_t>(), nullptr, lv.rowptr.as<int>(), lv.col.as<int>(), lv.a16.as<_Float16>(),
-            lv.ascale.as<float>(), lv.dinv32.as<float>(), neg.as<double>(), zr.as<float>(), a, scratch.as<float>(),
-            nullptr, b);
+        k_c_resid_h<8, 0><<<half_blocks_grid<8>(n), 256, 0, st>>>(
+            0, n, lv.agg_struct.as<uint32_t>(), nullptr, lv.rowptr.as<int>(), lv.hblk.as<HalfBlock>(),
+            lv.dinv32.as<float>(), neg.as<double>(), zr.as<float>(), a, scratch.as<float>(), nullptr, b);
         if (it >= iters - 2) {
           k_c_sq<float><<<nblk(n), 256, 0, st>>>(n, b, sq.as<double>());
           k_reduce_seg<<<ns, kBlock, 0, st>>>(sq.as<double>(), segd.as<int64_t>(), c.scal.as<double>(), kScal,
@@ -2984,8 +3076,7 @@ struct Solver {
     // (a level made the dense top keeps its FP64 operator: it is inverted, not iterated)
     for (Level& lv : c.levels)
       if (lv.dense && lv.half) {
-        lv.a16.release();
-        lv.ascale.release();
+        lv.hblk.release();
         lv.half = false;
       }
     if (mixed)  // the FP32 levels' FP64 copies served the Galerkin products and the step estimates only
@@ -2994,10 +3085,11 @@ struct Solver {
     tune_coarse_layout();
   }
 
-  // Per coarse level, the fastest layout of its residual kernel (narrow or wide rows, four
-  // or eight blocks in flight, plain or non-temporal block loads, see k_c_resid) in the
-  // precision the solve runs it in, timed on the level itself: three launches each after a
-  // warm one. STATICS_HIP_WIDE=k (0-7) forces layout k.
+  // Per coarse level, the fastest layout of its residual kernel in the precision the solve
+  // runs it in (FP64 and FP32: narrow or wide rows, four or eight blocks in flight, plain or
+  // non-temporal block loads, see k_c_resid; FP16: lanes per row and XCD row runs, see
+  // k_c_resid_h), timed on the level itself after a warm launch. STATICS_HIP_WIDE=k (0-7)
+  // forces layout k.
   void tune_coarse_layout() {
     const char* force = std::getenv("STATICS_HIP_WIDE");
     hipEvent_t e0, e1;
@@ -3013,7 +3105,7 @@ struct Solver {
       }
       // a rebuilt hierarchy of the same upload has levels of the same shapes: the choice
       // made for a level of this index, precision and shape stands
-      const std::array<int64_t, 4> shape{(int64_t)li, f32 ? 1 : 0, lv.n, lv.nblocks};
+      const std::array<int64_t, 4> shape{(int64_t)li, f32 ? (lv.half ? 2 : 1) : 0, lv.n, lv.nblocks};
       bool known = false;
       for (auto& kv : c.coarse_layouts)
         if (kv.first == shape) {
@@ -3022,7 +3114,8 @@ struct Solver {
         }
       if (known) continue;
       float ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      for (int w = 0; w < 8; w++) {
+      const int nlay = f32 && lv.half ? 6 : 8;  // the FP16 kernel's layouts, or the lane layouts
+      for (int w = 0; w < nlay; w++) {
         (f32 ? lv.layout32 : lv.layout64) = w;
         auto run = [&] {
           if (f32) {
@@ -3033,23 +3126,32 @@ struct Solver {
             c_resid<double>(lv, 0, lv.r64.as<double>(), lv.x64.as<double>(), lv.t64.as<double>());
           }
         };
-        run();
+        // one warm launch, timed: a level whose launch takes under 50 us is timed over 30
+        // launches (its few microseconds vary from launch to launch), others over 3
         ST_HIP(hipEventRecord(e0, st));
-        for (int k = 0; k < 3; k++) run();
+        run();
+        ST_HIP(hipEventRecord(e1, st));
+        ST_HIP(hipEventSynchronize(e1));
+        float warm = 0;
+        ST_HIP(hipEventElapsedTime(&warm, e0, e1));
+        const int reps = warm < 0.05f ? 30 : 3;
+        ST_HIP(hipEventRecord(e0, st));
+        for (int k = 0; k < reps; k++) run();
         ST_HIP(hipEventRecord(e1, st));
         ST_HIP(hipEventSynchronize(e1));
         ST_HIP(hipEventElapsedTime(&ms[w], e0, e1));
+        ms[w] /= (float)reps;
       }
       int best = 0;
-      for (int w = 1; w < 8; w++)
+      for (int w = 1; w < nlay; w++)
         if (ms[w] < ms[best]) best = w;
       (f32 ? lv.layout32 : lv.layout64) = best;
       c.coarse_layouts.emplace_back(shape, best);
       if (std::getenv("STATICS_HIP_VERBOSE"))
         std::fprintf(stderr,
                      "statics_hip: level %zu residual layout %d (%s): %.1f %.1f %.1f %.1f %.1f %.1f %.1f %.1f us\n",
-                     li + 1, best, f32 ? "fp32" : "fp64", 1e3 * ms[0] / 3, 1e3 * ms[1] / 3, 1e3 * ms[2] / 3,
-                     1e3 * ms[3] / 3, 1e3 * ms[4] / 3, 1e3 * ms[5] / 3, 1e3 * ms[6] / 3, 1e3 * ms[7] / 3);
+                     li + 1, best, f32 ? (lv.half ? "fp16" : "fp32") : "fp64", 1e3 * ms[0], 1e3 * ms[1],
+                     1e3 * ms[2], 1e3 * ms[3], 1e3 * ms[4], 1e3 * ms[5], 1e3 * ms[6], 1e3 * ms[7]);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -3076,7 +3178,7 @@ struct Solver {
     if (ld < 0) return;
     for (size_t li = ld + 1; li < c.levels.size(); li++) {
       Level& l2 = c.levels[li];
-      for (DBuf* b : {&l2.rowptr, &l2.col, &l2.a64, &l2.a32, &l2.a16, &l2.ascale, &l2.dinv64, &l2.dinv32,
+      for (DBuf* b : {&l2.rowptr, &l2.col, &l2.a64, &l2.a32, &l2.hblk, &l2.dinv64, &l2.dinv32,
                       &l2.centre, &l2.parent, &l2.rep, &l2.key, &l2.kid_ptr, &l2.kids, &l2.agg_struct, &l2.step_d,
                       &l2.r64, &l2.x64, &l2.t64, &l2.u64, &l2.v64, &l2.r32, &l2.x32, &l2.t32, &l2.u32, &l2.v32,
                       &l2.prow, &l2.pcol, &l2.pval64, &l2.pval32, &l2.pt_ptr, &l2.pt_blk, &l2.pblk_row})
@@ -3166,23 +3268,35 @@ struct Solver {
   template <class T, int W, int U, int NT>
   void c_resid_v(Level& lv, int mode, const T* r, const T* x, T* out, const T* add, T* out2, const T* dinv,
                  const double* step) {
-    if constexpr (std::is_same<T, float>::value) {
-      if (lv.half) {
-        k_c_resid<T, _Float16, W, U, NT><<<coarse_blocks<T>(lv.n, W != 0), 256, 0, st>>>(
-            mode, lv.n, lv.agg_struct.as<uint32_t>(), c.done.as<uint8_t>(), lv.rowptr.as<int>(), lv.col.as<int>(),
-            lv.a16.as<_Float16>(), lv.ascale.as<float>(), dinv, step, r, x, out, add, out2);
-        return;
-      }
-    }
-    k_c_resid<T, T, W, U, NT><<<coarse_blocks<T>(lv.n, W != 0), 256, 0, st>>>(
+    k_c_resid<T, W, U, NT><<<coarse_blocks<T>(lv.n, W != 0), 256, 0, st>>>(
           mode, lv.n, lv.agg_struct.as<uint32_t>(), c.done.as<uint8_t>(), lv.rowptr.as<int>(), lv.col.as<int>(),
-          lvA<T>(lv), nullptr, dinv, step, r, x, out, add, out2);
+          lvA<T>(lv), dinv, step, r, x, out, add, out2);
+  }
+  template <int G, int X>
+  void c_resid_h(Level& lv, int mode, const float* r, const float* x, float* out, const float* add, float* out2,
+                 const float* dinv, const double* step) {
+    k_c_resid_h<G, X><<<half_blocks_grid<G>(lv.n), 256, 0, st>>>(
+        mode, lv.n, lv.agg_struct.as<uint32_t>(), c.done.as<uint8_t>(), lv.rowptr.as<int>(), lv.hblk.as<HalfBlock>(),
+        dinv, step, r, x, out, add, out2);
   }
   template <class T>
   void c_resid(Level& lv, int mode, const T* r, const T* x, T* out, const T* add = nullptr, T* out2 = nullptr) {
     const int layout = std::is_same<T, double>::value ? lv.layout64 : lv.layout32;
     const T* dinv = (mode == 1 || out2) ? lvD<T>(lv) : nullptr;
     const double* step = (mode == 1 || out2) ? lv.step_d.as<double>() : nullptr;
+    if constexpr (std::is_same<T, float>::value) {
+      if (lv.half) {
+        switch (layout) {  // lanes per row 8, 16, 32; from 3 on each XCD a run of rows
+          case 0: c_resid_h<8, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          case 1: c_resid_h<16, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          case 2: c_resid_h<32, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          case 3: c_resid_h<8, 1>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          case 4: c_resid_h<16, 1>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          default: c_resid_h<32, 1>(lv, mode, r, x, out, add, out2, dinv, step); break;
+        }
+        return;
+      }
+    }
     switch (layout) {  // bit 0: wide rows; bit 1: eight blocks in flight; bit 2: non-temporal blocks
       case 0: c_resid_v<T, 0, 4, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
       case 1: c_resid_v<T, 1, 4, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
@@ -3650,7 +3764,7 @@ void release_levels(Ctx& c) {
   c.hier_valid = false;
   c.restrict_fused = false;
   for (auto& lv : c.levels)
-    for (DBuf* b : {&lv.rowptr, &lv.col, &lv.a64, &lv.a32, &lv.a16, &lv.ascale, &lv.dinv64, &lv.dinv32, &lv.centre,
+    for (DBuf* b : {&lv.rowptr, &lv.col, &lv.a64, &lv.a32, &lv.hblk, &lv.dinv64, &lv.dinv32, &lv.centre,
                     &lv.parent, &lv.rep,
                     &lv.key, &lv.kid_ptr, &lv.kids, &lv.agg_struct, &lv.step_d, &lv.r64, &lv.x64, &lv.t64, &lv.u64,
                     &lv.v64, &lv.r32, &lv.x32, &lv.t32, &lv.u32, &lv.v32, &lv.dense_a, &lv.dense_off,
