@@ -965,31 +965,26 @@ __global__ void k_dense_sym(int nact, const int* ord, const int64_t* pre2, const
   Ds[i * m + j] = a;
   Ds[j * m + i] = a;
 }
-// x = D r per structure (D symmetric: row i read as column i). A 1024-thread block takes
-// 32 rows; its 32 thread groups split the sum over j and add their partial sums in a
-// fixed order.
+// x = D r per structure (D symmetric: row i read as column i, contiguous). One wave per
+// row: its 64 lanes take every 64th j, then meet by a butterfly (a fixed order), so the
+// rows of a 1536-row top fill 384 workgroups (a 1024-thread block per 32 rows filled 48).
 template <class T>
-__global__ __launch_bounds__(1024) void k_dense_apply(const int64_t* seg, const int64_t* off, const double* D,
-                                                      const uint8_t* done, const T* r, T* x) {
-  __shared__ double part[32][33];
+__global__ __launch_bounds__(256) void k_dense_apply(const int64_t* seg, const int64_t* off, const double* D,
+                                                     const uint8_t* done, const T* r, T* x) {
   const int s = blockIdx.y;
   if (done && done[s]) return;
   const int64_t a0 = seg[s], m = 6 * (seg[s + 1] - a0);
-  const int il = threadIdx.x & 31, jg = threadIdx.x >> 5;
-  const int64_t i = (int64_t)blockIdx.x * 32 + il;
-  if ((int64_t)blockIdx.x * 32 >= m) return;
-  const double* Ds = D + off[s];
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= m) return;  // the whole wave
+  const int lane = threadIdx.x & 63;
+  const double* Di = D + off[s] + i * m;
   const T* rs = r + 6 * a0;
   double acc = 0.0;
-  if (i < m)
-    for (int64_t j = jg; j < m; j += 32) acc += Ds[j * m + i] * (double)rs[j];
-  part[jg][il] = acc;
-  __syncthreads();
-  if (jg == 0 && i < m) {
-    double t = 0.0;
-    for (int k = 0; k < 32; k++) t += part[k][il];
-    x[6 * a0 + i] = (T)t;
-  }
+#pragma unroll 4
+  for (int64_t j = lane; j < m; j += 64) acc += Di[j] * (double)rs[j];
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) x[6 * a0 + i] = (T)acc;
 }
 
 // Coarse block rows are worked by groups of 8 lanes: lane i < 6 of a group computes
@@ -2321,6 +2316,44 @@ __global__ void k_restrict_p(int64_t nJ, const uint32_t* ast, const uint8_t* don
     if (a < 6) x0[6 * J + a] = (T)step[ast[J]] * row6_dot<T>(dinv + 36 * J + 6 * a, all);
   }
 }
+// The same restriction for a level of few aggregates J (each with a long column of P):
+// one wave per J, its lanes in ten groups of six, group g taking blocks g, g + 10, ... of
+// the column; the groups' sums meet in LDS and are added in group order.
+template <class T>
+__global__ __launch_bounds__(256) void k_restrict_pw(int64_t nJ, const uint32_t* ast, const uint8_t* done,
+                                                     const int* pt_ptr, const int* pt_blk, const int* pblk_row,
+                                                     const T* pval, const T* t, T* c, const T* dinv,
+                                                     const double* step, T* x0) {
+  __shared__ T part[4][64];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t J = (int64_t)blockIdx.x * 4 + wv;
+  const bool live = J < nJ && !(done && done[ast[J]]);
+  const int grp = lane / 6, a = lane - 6 * grp;
+  T acc = 0;
+  if (live && grp < 10) {
+    const int k1 = pt_ptr[J + 1];
+    for (int kk = pt_ptr[J] + grp; kk < k1; kk += 10) {
+      const int blk = pt_blk[kk];
+      const T* M = pval + 36 * (int64_t)blk + a;
+      T tv[6];
+      load6<T>(t + 6 * (int64_t)pblk_row[blk], tv);
+#pragma unroll
+      for (int b = 0; b < 6; b++) acc += M[6 * b] * tv[b];
+    }
+  }
+  part[wv][lane] = acc;
+  __syncthreads();
+  if (!live || lane >= 6) return;
+  T all[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    T sum = 0;
+    for (int g2 = 0; g2 < 10; g2++) sum += part[wv][6 * g2 + j];
+    all[j] = sum;
+  }
+  c[6 * J + lane] = all[lane];
+  if (x0) x0[6 * J + lane] = (T)step[ast[J]] * row6_dot<T>(dinv + 36 * J + 6 * lane, all);
+}
 // prolong: x[i] += sum over row i of P y
 template <class T>
 __global__ void k_prolong_p(int64_t n, const uint32_t* ast, const uint8_t* done, const int* prow, const int* pcol,
@@ -3341,8 +3374,8 @@ struct Solver {
     const int64_t n = lv.n;
     const uint32_t* ast = lv.agg_struct.as<uint32_t>();
     if (lv.dense) {  // the dense top: exact
-      const dim3 gd((unsigned)((lv.dense_mmax + 31) / 32), (unsigned)ns);
-      k_dense_apply<T><<<gd, 1024, 0, st>>>(lv.dense_seg.as<int64_t>(), lv.dense_off.as<int64_t>(),
+      const dim3 gd((unsigned)((lv.dense_mmax + 3) / 4), (unsigned)ns);
+      k_dense_apply<T><<<gd, 256, 0, st>>>(lv.dense_seg.as<int64_t>(), lv.dense_off.as<int64_t>(),
                                           lv.dense_a.as<double>(), done, r, x);
       return x;
     }
@@ -3362,6 +3395,11 @@ struct Solver {
     }
     return res;
   }
+
+  // Restrictions onto a level of at most this many aggregates take a wave per aggregate
+  // (k_restrict_pw): such levels are deep, with long columns of P, and few enough that the
+  // eight-lane groups of k_restrict_p leave most of the GPU idle.
+  static constexpr int64_t kWideRestrict = 8192;
 
   // One cycle at level li on rhs from the damped Jacobi start x (updated in place): the
   // residual, restricted; the coarser solve, prolonged onto x; the post-smoothing
@@ -3397,7 +3435,11 @@ struct Solver {
     T* up_x0 = fuse ? lvbuf<T>(up, 1) : nullptr;
     const T* up_d = fuse ? lvD<T>(up) : nullptr;
     const double* up_step = fuse ? up.step_d.as<double>() : nullptr;
-    if (lv.pblocks)
+    if (lv.pblocks && up.n <= kWideRestrict)
+      k_restrict_pw<T><<<(unsigned)((up.n + 3) / 4), 256, 0, st>>>(
+          up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(), lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv,
+          u, up_r, up_d, up_step, up_x0);
+    else if (lv.pblocks)
       k_restrict_p<T><<<nblk(8 * up.n), 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(),
                                                      lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv, u, up_r, up_d,
                                                      up_step, up_x0);
