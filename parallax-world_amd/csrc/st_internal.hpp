@@ -160,6 +160,15 @@ struct alignas(16) HalfBlock {
 };
 static_assert(sizeof(HalfBlock) == 80, "HalfBlock is five 16-byte words");
 
+// A block of a prolongator in FP32 with the index of the other end of its row or column
+// (160 bytes: ten aligned 16-byte loads).
+struct alignas(16) PBlock32 {
+  float a[36];
+  int idx;
+  int pad[3];
+};
+static_assert(sizeof(PBlock32) == 160, "PBlock32 is ten 16-byte words");
+
 // One coarse level (aggregates of the level below, 6 DOF each), block-sparse.
 struct Level {
   int64_t n = 0;        // aggregates
@@ -180,6 +189,9 @@ struct Level {
   // block-sparse by rows, and the same blocks grouped by column (the next level's aggregates)
   int64_t pblocks = 0;
   DBuf prow, pcol, pval64, pval32, pt_ptr, pt_blk, pblk_row;
+  // (mixed, large levels) the FP32 blocks as PBlock32 records in row order (column index
+  // inside) and in column order (row index inside), for the V-cycle's transfers; pval32 goes
+  DBuf prr, prc;
   DBuf agg_struct;      // per aggregate: structure
   int layout32 = 0, layout64 = 0;  // the residual kernel's tuned layout per precision (see k_c_resid)
   std::vector<double> step;  // smoother step per structure

@@ -468,12 +468,26 @@ __global__ void k_mask_rows(int64_t n, const uint8_t* nflags, double* v) {
 }
 
 // out[s * stride + slot] = sum of partial over [seg[s], seg[s+1]) (fixed order)
-__global__ __launch_bounds__(kBlock) void k_reduce_seg(const double* partial, const int64_t* seg, double* out, int stride,
-                                                       int slot) {
-  __shared__ double red[kBlock / 64];
+// (kReduceThreads threads a structure, each adding every kReduceThreads-th entry in order,
+// four loads in flight: a 256^3 structure has 32,000 bricks, which 256 threads took 54 us
+// to read one after another)
+constexpr int kReduceThreads = 1024;
+__global__ __launch_bounds__(kReduceThreads) void k_reduce_seg(const double* partial, const int64_t* seg, double* out,
+                                                               int stride, int slot) {
+  __shared__ double red[kReduceThreads / 64];
   const int s = blockIdx.x;
+  constexpr int K = kReduceThreads;
+  const int64_t e = seg[s + 1];
   double acc = 0.0;
-  for (int64_t i = seg[s] + threadIdx.x; i < seg[s + 1]; i += blockDim.x) acc += partial[i];
+  int64_t i = seg[s] + threadIdx.x;
+  for (; i + 3 * K < e; i += 4 * K) {
+    const double v0 = partial[i], v1 = partial[i + K], v2 = partial[i + 2 * K], v3 = partial[i + 3 * K];
+    acc += v0;
+    acc += v1;
+    acc += v2;
+    acc += v3;
+  }
+  for (; i < e; i += K) acc += partial[i];
   const double t = block_sum(acc, red);
   if (threadIdx.x == 0) out[(int64_t)s * stride + slot] = t;
 }
@@ -2354,6 +2368,102 @@ __global__ __launch_bounds__(256) void k_restrict_pw(int64_t nJ, const uint32_t*
   c[6 * J + lane] = all[lane];
   if (x0) x0[6 * J + lane] = (T)step[ast[J]] * row6_dot<T>(dinv + 36 * J + 6 * lane, all);
 }
+// The transfers of a large FP32 level through PBlock32 records, G lanes per aggregate,
+// each lane whole blocks in turn (ten 16-byte loads and the other end's six entries),
+// the lanes' sums meeting by a butterfly; lanes 0-5 finish as k_restrict_p and
+// k_prolong_p do. Restriction: c[J] = sum over column J of P^T t (x0 = w D^-1 c when set).
+typedef float st_f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void load_pblock(const PBlock32* p, float* a, int& idx) {
+  const st_f4v* q = reinterpret_cast<const st_f4v*>(p);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const st_f4v v = q[k];
+    a[4 * k] = v.x;
+    a[4 * k + 1] = v.y;
+    a[4 * k + 2] = v.z;
+    a[4 * k + 3] = v.w;
+  }
+  idx = __builtin_bit_cast(int, q[9].x);
+}
+template <int G>
+__global__ __launch_bounds__(256) void k_restrict_rec(int64_t nJ, const uint32_t* ast, const uint8_t* done,
+                                                      const int* pt_ptr, const PBlock32* pc, const float* t, float* c,
+                                                      const float* dinv, const double* step, float* x0) {
+  const int gl = threadIdx.x & (G - 1);
+  const int64_t J = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const bool live = J < nJ && !(done && done[ast[J]]);
+  float y[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (live) {
+    const int k1 = pt_ptr[J + 1];
+    for (int kk = pt_ptr[J] + gl; kk < k1; kk += G) {
+      float a[36], tv[6];
+      int i;
+      load_pblock(pc + kk, a, i);
+      load6<float>(t + 6 * (int64_t)i, tv);
+#pragma unroll
+      for (int b = 0; b < 6; b++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) y[j] += a[6 * b + j] * tv[b];
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1)
+#pragma unroll
+    for (int k = 0; k < 6; k++) y[k] += __shfl_xor(y[k], o, 64);
+  if (!live || gl >= 6) return;
+  c[6 * J + gl] = y[gl];
+  if (x0) x0[6 * J + gl] = (float)step[ast[J]] * row6_dot<float>(dinv + 36 * J + 6 * gl, y);
+}
+// Prolongation: x[i] += sum over row i of P y.
+template <int G>
+__global__ __launch_bounds__(256) void k_prolong_rec(int64_t n, const uint32_t* ast, const uint8_t* done,
+                                                     const int* prow, const PBlock32* pr, const float* y, float* x) {
+  const int gl = threadIdx.x & (G - 1);
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const bool live = i < n && !(done && done[ast[i]]);
+  float acc[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (live) {
+    const int k1 = prow[i + 1];
+    for (int k = prow[i] + gl; k < k1; k += G) {
+      float a[36], yv[6];
+      int J;
+      load_pblock(pr + k, a, J);
+      load6<float>(y + 6 * (int64_t)J, yv);
+#pragma unroll
+      for (int r = 0; r < 6; r++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) acc[r] += a[6 * r + j] * yv[j];
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1)
+#pragma unroll
+    for (int k = 0; k < 6; k++) acc[k] += __shfl_xor(acc[k], o, 64);
+  if (!live || gl >= 6) return;
+  x[6 * i + gl] += acc[gl];
+}
+// The records: rows (block k in storage order, its column) or columns (position kk of
+// the column grouping, block pt_blk[kk], its row).
+__global__ void k_pblocks_rows(int64_t nb, const int* pcol, const float* pval, PBlock32* out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  PBlock32 r;
+  for (int e = 0; e < 36; e++) r.a[e] = pval[36 * k + e];
+  r.idx = pcol[k];
+  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+  out[k] = r;
+}
+__global__ void k_pblocks_cols(int64_t nb, const int* pt_blk, const int* pblk_row, const float* pval,
+                               PBlock32* out) {
+  const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= nb) return;
+  const int blk = pt_blk[kk];
+  PBlock32 r;
+  for (int e = 0; e < 36; e++) r.a[e] = pval[36 * (int64_t)blk + e];
+  r.idx = pblk_row[blk];
+  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+  out[kk] = r;
+}
 // prolong: x[i] += sum over row i of P y
 template <class T>
 __global__ void k_prolong_p(int64_t n, const uint32_t* ast, const uint8_t* done, const int* prow, const int* pcol,
@@ -2454,7 +2564,7 @@ struct Solver {
   }
 
   void reduce_bricks(DBuf& partial, int slot) {
-    k_reduce_seg<<<ns, kBlock, 0, st>>>(partial.as<double>(), sbrick_d.as<int64_t>(), c.scal.as<double>(), kScal, slot);
+    k_reduce_seg<<<ns, kReduceThreads, 0, st>>>(partial.as<double>(), sbrick_d.as<int64_t>(), c.scal.as<double>(), kScal, slot);
   }
 
   void scan(int* in, int* out, int64_t n) {
@@ -2900,7 +3010,7 @@ struct Solver {
             lv.dinv32.as<float>(), neg.as<double>(), zr.as<float>(), a, scratch.as<float>(), nullptr, b);
         if (it >= iters - 2) {
           k_c_sq<float><<<nblk(n), 256, 0, st>>>(n, b, sq.as<double>());
-          k_reduce_seg<<<ns, kBlock, 0, st>>>(sq.as<double>(), segd.as<int64_t>(), c.scal.as<double>(), kScal,
+          k_reduce_seg<<<ns, kReduceThreads, 0, st>>>(sq.as<double>(), segd.as<int64_t>(), c.scal.as<double>(), kScal,
                                               it == iters - 2 ? S_LAM_A : S_LAM_B);
         }
         std::swap(a, b);
@@ -2917,7 +3027,7 @@ struct Solver {
                                                    lv.dinv64.as<double>(), a, b);
         if (it >= iters - 2) {
           k_c_sq<double><<<nblk(n), 256, 0, st>>>(n, b, sq.as<double>());
-          k_reduce_seg<<<ns, kBlock, 0, st>>>(sq.as<double>(), segd.as<int64_t>(), c.scal.as<double>(), kScal,
+          k_reduce_seg<<<ns, kReduceThreads, 0, st>>>(sq.as<double>(), segd.as<int64_t>(), c.scal.as<double>(), kScal,
                                               it == iters - 2 ? S_LAM_A : S_LAM_B);
         }
         std::swap(a, b);
@@ -3115,6 +3225,22 @@ struct Solver {
     if (mixed)  // the FP32 levels' FP64 copies served the Galerkin products and the step estimates only
       for (size_t li = 0; li < c.levels.size() && (int)li < c.f64_from; li++)
         for (DBuf* b : {&c.levels[li].a64, &c.levels[li].dinv64, &c.levels[li].pval64}) b->release();
+    // the FP32 transfers below levels of more than kWideRestrict aggregates through records
+    // (STATICS_HIP_PREC=0: through the block arrays)
+    const char* prec_env = std::getenv("STATICS_HIP_PREC");
+    if (mixed && !(prec_env && std::atoi(prec_env) == 0))
+      for (size_t li = 0; li + 1 < c.levels.size() && (int)li < c.f64_from; li++) {
+        Level& lv = c.levels[li];
+        if (!lv.pblocks || c.levels[li + 1].n <= kWideRestrict) continue;
+        lv.prr.ensure((size_t)lv.pblocks * sizeof(PBlock32));
+        lv.prc.ensure((size_t)lv.pblocks * sizeof(PBlock32));
+        k_pblocks_rows<<<nblk(lv.pblocks), 256, 0, st>>>(lv.pblocks, lv.pcol.as<int>(), lv.pval32.as<float>(),
+                                                         lv.prr.as<PBlock32>());
+        k_pblocks_cols<<<nblk(lv.pblocks), 256, 0, st>>>(lv.pblocks, lv.pt_blk.as<int>(), lv.pblk_row.as<int>(),
+                                                         lv.pval32.as<float>(), lv.prc.as<PBlock32>());
+        ST_HIP(hipGetLastError());
+        lv.pval32.release();
+      }
     tune_coarse_layout();
   }
 
@@ -3214,7 +3340,7 @@ struct Solver {
       for (DBuf* b : {&l2.rowptr, &l2.col, &l2.a64, &l2.a32, &l2.hblk, &l2.dinv64, &l2.dinv32,
                       &l2.centre, &l2.parent, &l2.rep, &l2.key, &l2.kid_ptr, &l2.kids, &l2.agg_struct, &l2.step_d,
                       &l2.r64, &l2.x64, &l2.t64, &l2.u64, &l2.v64, &l2.r32, &l2.x32, &l2.t32, &l2.u32, &l2.v32,
-                      &l2.prow, &l2.pcol, &l2.pval64, &l2.pval32, &l2.pt_ptr, &l2.pt_blk, &l2.pblk_row})
+                      &l2.prow, &l2.pcol, &l2.pval64, &l2.pval32, &l2.prr, &l2.prc, &l2.pt_ptr, &l2.pt_blk, &l2.pblk_row})
         b->release();
     }
     c.levels.resize(ld + 1);
@@ -3435,7 +3561,14 @@ struct Solver {
     T* up_x0 = fuse ? lvbuf<T>(up, 1) : nullptr;
     const T* up_d = fuse ? lvD<T>(up) : nullptr;
     const double* up_step = fuse ? up.step_d.as<double>() : nullptr;
-    if (lv.pblocks && up.n <= kWideRestrict)
+    bool recs = false;
+    if constexpr (std::is_same<T, float>::value) recs = lv.prc.p != nullptr;
+    if (recs) {
+      if constexpr (std::is_same<T, float>::value)
+        k_restrict_rec<16><<<(unsigned)((up.n * 16 + 255) / 256), 256, 0, st>>>(
+            up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(), lv.prc.as<PBlock32>(), u, up_r, up_d,
+            up_step, up_x0);
+    } else if (lv.pblocks && up.n <= kWideRestrict)
       k_restrict_pw<T><<<(unsigned)((up.n + 3) / 4), 256, 0, st>>>(
           up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(), lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv,
           u, up_r, up_d, up_step, up_x0);
@@ -3448,7 +3581,11 @@ struct Solver {
                                                   up.kids.as<int>(), up.centre.as<double>(), lv.centre.as<double>(), u,
                                                   up_r, up_d, up_step, up_x0);
     const T* y = descend<T>(li + 1, fuse);
-    if (lv.pblocks)
+    if (recs) {
+      if constexpr (std::is_same<T, float>::value)
+        k_prolong_rec<8><<<(unsigned)((n * 8 + 255) / 256), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(),
+                                                                          lv.prr.as<PBlock32>(), y, x);
+    } else if (lv.pblocks)
       k_prolong_p<T><<<nblk(8 * n), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.pcol.as<int>(), pv, y, x);
     else
       k_prolong_c<T><<<nblk(n), 256, 0, st>>>(n, ast, done, lv.parent.as<int>(), lv.centre.as<double>(),
@@ -3810,7 +3947,7 @@ void release_levels(Ctx& c) {
                     &lv.parent, &lv.rep,
                     &lv.key, &lv.kid_ptr, &lv.kids, &lv.agg_struct, &lv.step_d, &lv.r64, &lv.x64, &lv.t64, &lv.u64,
                     &lv.v64, &lv.r32, &lv.x32, &lv.t32, &lv.u32, &lv.v32, &lv.dense_a, &lv.dense_off,
-                    &lv.dense_seg, &lv.prow, &lv.pcol, &lv.pval64, &lv.pval32, &lv.pt_ptr, &lv.pt_blk,
+                    &lv.dense_seg, &lv.prow, &lv.pcol, &lv.pval64, &lv.pval32, &lv.prr, &lv.prc, &lv.pt_ptr, &lv.pt_blk,
                     &lv.pblk_row})
       b->release();
   c.levels.clear();
