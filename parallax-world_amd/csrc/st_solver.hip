@@ -1216,9 +1216,10 @@ inline unsigned coarse_blocks(int64_t n, bool wide) {
 // the lanes' sums meet by a butterfly (the same bits in every lane of the group), and
 // lanes 0-5 finish components 0-5 as k_c_resid does. A lane load covers a whole block
 // (one load instruction per 16 bytes, against one per 8 bytes and two more for x in the
-// lane layout above), which the texture units address at a quarter of the cost. kX: the
-// workgroups of each XCD take one contiguous run of rows, whose columns (neighbouring
-// aggregates) then meet in that XCD's L2. (Non-temporal block loads ran at half speed.)
+// lane layout above), which the texture units address at a quarter of the cost. kU
+// blocks' loads are issued together per lane. (Non-temporal block loads ran at half
+// speed; dealing each XCD one contiguous run of rows, so that neighbouring aggregates'
+// x meet in its L2, changed nothing.)
 typedef unsigned st_u4 __attribute__((ext_vector_type(4)));
 typedef _Float16 st_h2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void load_half_block(const HalfBlock* p, st_u4* q) {
@@ -1256,31 +1257,30 @@ __device__ __forceinline__ void half_block_madd(const st_u4* q, const float* xv,
     y[r] += sc * t;
   }
 }
-template <int G, int kX>
+template <int G, int kU>
 __global__ __launch_bounds__(256) void k_c_resid_h(int mode, int64_t n, const uint32_t* ast, const uint8_t* done,
                                                    const int* rowptr, const HalfBlock* A, const float* dinv,
                                                    const double* step, const float* r, const float* x, float* out,
                                                    const float* add, float* out2) {
   static_assert(G >= 8 && G <= 64 && (G & (G - 1)) == 0, "G: 8 to 64 lanes, a power of two");
   const int gl = threadIdx.x & (G - 1);
-  const uint32_t wg = kX ? xcd_brick(blockIdx.x, gridDim.x) : blockIdx.x;
-  const int64_t I = ((int64_t)wg * blockDim.x + threadIdx.x) / G;
+  const int64_t I = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const bool live = I < n && !(done && done[ast[I]]);
   float y[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (live) {
     int b = rowptr[I] + gl;
     const int b1 = rowptr[I + 1];
-    for (; b + G < b1; b += 2 * G) {
-      st_u4 q0[5], q1[5];
-      load_half_block(A + b, q0);
-      load_half_block(A + b + G, q1);
-      float x0[6], x1[6];
-      load_x6(x + 6 * (int64_t)(int)q0[4].w, x0);
-      load_x6(x + 6 * (int64_t)(int)q1[4].w, x1);
-      half_block_madd(q0, x0, y);
-      half_block_madd(q1, x1, y);
+    for (; b + (kU - 1) * G < b1; b += kU * G) {
+      st_u4 q[kU][5];
+      float xv[kU][6];
+#pragma unroll
+      for (int u = 0; u < kU; u++) load_half_block(A + b + u * G, q[u]);
+#pragma unroll
+      for (int u = 0; u < kU; u++) load_x6(x + 6 * (int64_t)(int)q[u][4].w, xv[u]);
+#pragma unroll
+      for (int u = 0; u < kU; u++) half_block_madd(q[u], xv[u], y);
     }
-    if (b < b1) {
+    for (; b < b1; b += G) {
       st_u4 q0[5];
       load_half_block(A + b, q0);
       float x0[6];
@@ -3005,7 +3005,7 @@ struct Solver {
       float* b = w.as<float>();
       k_c_init32<<<nblk(n), 256, 0, st>>>(n, a);
       for (int it = 0; it < iters; it++) {
-        k_c_resid_h<8, 0><<<half_blocks_grid<8>(n), 256, 0, st>>>(
+        k_c_resid_h<8, 2><<<half_blocks_grid<8>(n), 256, 0, st>>>(
             0, n, lv.agg_struct.as<uint32_t>(), nullptr, lv.rowptr.as<int>(), lv.hblk.as<HalfBlock>(),
             lv.dinv32.as<float>(), neg.as<double>(), zr.as<float>(), a, scratch.as<float>(), nullptr, b);
         if (it >= iters - 2) {
@@ -3246,7 +3246,7 @@ struct Solver {
 
   // Per coarse level, the fastest layout of its residual kernel in the precision the solve
   // runs it in (FP64 and FP32: narrow or wide rows, four or eight blocks in flight, plain or
-  // non-temporal block loads, see k_c_resid; FP16: lanes per row and XCD row runs, see
+  // non-temporal block loads, see k_c_resid; FP16: lanes per row and blocks in flight, see
   // k_c_resid_h), timed on the level itself after a warm launch. STATICS_HIP_WIDE=k (0-7)
   // forces layout k.
   void tune_coarse_layout() {
@@ -3273,7 +3273,7 @@ struct Solver {
         }
       if (known) continue;
       float ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      const int nlay = f32 && lv.half ? 6 : 8;  // the FP16 kernel's layouts, or the lane layouts
+      const int nlay = 8;  // the FP16 kernel's layouts, or the lane layouts
       for (int w = 0; w < nlay; w++) {
         (f32 ? lv.layout32 : lv.layout64) = w;
         auto run = [&] {
@@ -3431,10 +3431,10 @@ struct Solver {
           mode, lv.n, lv.agg_struct.as<uint32_t>(), c.done.as<uint8_t>(), lv.rowptr.as<int>(), lv.col.as<int>(),
           lvA<T>(lv), dinv, step, r, x, out, add, out2);
   }
-  template <int G, int X>
+  template <int G, int U>
   void c_resid_h(Level& lv, int mode, const float* r, const float* x, float* out, const float* add, float* out2,
                  const float* dinv, const double* step) {
-    k_c_resid_h<G, X><<<half_blocks_grid<G>(lv.n), 256, 0, st>>>(
+    k_c_resid_h<G, U><<<half_blocks_grid<G>(lv.n), 256, 0, st>>>(
         mode, lv.n, lv.agg_struct.as<uint32_t>(), c.done.as<uint8_t>(), lv.rowptr.as<int>(), lv.hblk.as<HalfBlock>(),
         dinv, step, r, x, out, add, out2);
   }
@@ -3445,13 +3445,15 @@ struct Solver {
     const double* step = (mode == 1 || out2) ? lv.step_d.as<double>() : nullptr;
     if constexpr (std::is_same<T, float>::value) {
       if (lv.half) {
-        switch (layout) {  // lanes per row 8, 16, 32; from 3 on each XCD a run of rows
-          case 0: c_resid_h<8, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
-          case 1: c_resid_h<16, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
-          case 2: c_resid_h<32, 0>(lv, mode, r, x, out, add, out2, dinv, step); break;
-          case 3: c_resid_h<8, 1>(lv, mode, r, x, out, add, out2, dinv, step); break;
-          case 4: c_resid_h<16, 1>(lv, mode, r, x, out, add, out2, dinv, step); break;
-          default: c_resid_h<32, 1>(lv, mode, r, x, out, add, out2, dinv, step); break;
+        switch (layout) {  // (lanes per row, blocks in flight a lane)
+          case 0: c_resid_h<8, 2>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          case 1: c_resid_h<16, 2>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          case 2: c_resid_h<32, 2>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          case 3: c_resid_h<8, 4>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          case 4: c_resid_h<16, 4>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          case 5: c_resid_h<32, 4>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          case 6: c_resid_h<64, 4>(lv, mode, r, x, out, add, out2, dinv, step); break;
+          default: c_resid_h<16, 8>(lv, mode, r, x, out, add, out2, dinv, step); break;
         }
         return;
       }
