@@ -415,7 +415,7 @@ def fd_check(args, eps=1e-4, beta=20.0):
     print(json.dumps(out), flush=True)
 
 
-def settle_cpu_sample(size, seed, quantile, threads):
+def settle_cpu_sample(size, seed, quantile, threads, spacing=7.0):
     """The oracle's settle (bodies.rs:511-541, gravity -y, tol 1e-6) on a size^3 structure of the
     same generator, strengths at the same quantile of its standing used shares."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
@@ -475,7 +475,8 @@ def settle_bench(args):
     dof0 = 6 * int((nheld == 0).sum())
     cpu = None
     if args.settle_cpu_sample > 0:
-        cb = settle_cpu_sample(args.settle_cpu_sample, args.seed, args.settle_quantile, args.cpu_threads)
+        cb = settle_cpu_sample(args.settle_cpu_sample, args.seed, args.settle_quantile, args.cpu_threads,
+                               spacing=args.spacing)
         per_s = (1.0 / cb["seconds"]) * (cb["dof"] * cb["rounds"]) / (dof0 * rounds)
         cpu = {"value": per_s, "unit": f"settles/sec ({n}^3, extrapolated)", "cores": args.cpu_threads,
                "kind": "port",

@@ -3096,15 +3096,15 @@ struct Solver {
       c.last_ndof = ndof;
     }
     if (!ndof) return;
-    // the mixed V-cycle's two largest coarse operators (levels 1 and 2, nearly all of the
+    // the mixed V-cycle's three largest coarse operators (levels 1-3, nearly all of the
     // coarse blocks) in FP16, each block with its own scale: at 256^3 the same iterations and
-    // 4% less solve time than FP32. FP16 on levels 3 and 4 as well saves 3% more there, but
-    // leaves some structures of the 512-structure batch at 248 iterations instead of 96.
+    // 3.14 s a step against 3.20 s with level 3 in FP32; the 512-structure batch stays at 96
+    // iterations at most. FP16 on level 4 as well left some of the batch's structures at 248.
     // STATICS_HIP_HALF_LEVELS sets how many levels, STATICS_HIP_HALF_BLOCKS keeps the
     // operators of fewer blocks than it in FP32, STATICS_HIP_HALF=0 keeps them all.
     int64_t kHalfBlocks = 0;
     if (const char* e = std::getenv("STATICS_HIP_HALF_BLOCKS")) kHalfBlocks = std::max<int64_t>(0, std::atoll(e));
-    int kHalfLevels = 2;
+    int kHalfLevels = 3;
     if (const char* e = std::getenv("STATICS_HIP_HALF_LEVELS")) kHalfLevels = std::max(0, std::atoi(e));
     auto maybe_half = [&](Level& lv) {
       const int li = (int)c.levels.size();
