@@ -247,6 +247,7 @@ struct Ctx {
   std::vector<int> gammas;  // if set: cycles per visit of each level index (1 beyond the list);
                             // if empty, gamma on the indices (gamma_from, gamma_to]
   int f64_from = 4;      // first level (0: level 1) kept in FP64 by the mixed V-cycle
+  bool fold_switch = true;  // the switch to FP64 folded into the restriction and prolongation
   bool half_coarse = true;  // the mixed V-cycle's largest coarse operators stored in FP16
   bool graphs = false;   // replay each CG iteration's tail as a captured graph
   bool dense_top = true; // invert a small coarse level densely and drop the levels above

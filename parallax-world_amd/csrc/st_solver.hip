@@ -1021,8 +1021,20 @@ __device__ __forceinline__ void group_gather6(T a, T* all) {
 
 // Row i of block row I times x, summed over the row's blocks b0..b1 in order; four
 // blocks' loads are issued before their products, to keep more loads in flight.
-template <class T>
-__device__ __forceinline__ T block_row_dot(int b0, int b1, const int* col, const T* A, const T* x, int i) {
+// six entries of a TX vector into T (widened or narrowed as a separate conversion would)
+template <class T, class TX>
+__device__ __forceinline__ void load6_as(const TX* p, T* v) {
+  if constexpr (std::is_same<T, TX>::value) {
+    load6<T>(p, v);
+  } else {
+    TX w[6];
+    load6<TX>(p, w);
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = (T)w[k];
+  }
+}
+template <class T, class TX = T>
+__device__ __forceinline__ T block_row_dot(int b0, int b1, const int* col, const T* A, const TX* x, int i) {
   T acc = 0;
   int b = b0;
   for (; b + 4 <= b1; b += 4) {
@@ -1032,7 +1044,7 @@ __device__ __forceinline__ T block_row_dot(int b0, int b1, const int* col, const
     for (int k = 0; k < 4; k++) cb[k] = col[b + k];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      load6<T>(x + 6 * (int64_t)cb[k], xv[k]);
+      load6_as<T, TX>(x + 6 * (int64_t)cb[k], xv[k]);
       load6<T>(A + 36 * (int64_t)(b + k) + 6 * i, av[k]);
     }
 #pragma unroll
@@ -1040,7 +1052,7 @@ __device__ __forceinline__ T block_row_dot(int b0, int b1, const int* col, const
   }
   for (; b < b1; b++) {
     T xv[6];
-    load6<T>(x + 6 * (int64_t)col[b], xv);
+    load6_as<T, TX>(x + 6 * (int64_t)col[b], xv);
     acc += row6_dot<T>(A + 36 * (int64_t)b + 6 * i, xv);
   }
   return acc;
@@ -2332,12 +2344,14 @@ __global__ void k_restrict_p(int64_t nJ, const uint32_t* ast, const uint8_t* don
 }
 // The same restriction for a level of few aggregates J (each with a long column of P):
 // one wave per J, its lanes in ten groups of six, group g taking blocks g, g + 10, ... of
-// the column; the groups' sums meet in LDS and are added in group order.
-template <class T>
+// the column; the groups' sums meet in LDS and are added in group order. U: the
+// coarse level's precision (wider than T where the level above switches to FP64: the
+// sums are widened as a separate conversion would, and x0 formed as k_c_jacobi does).
+template <class T, class U = T>
 __global__ __launch_bounds__(256) void k_restrict_pw(int64_t nJ, const uint32_t* ast, const uint8_t* done,
                                                      const int* pt_ptr, const int* pt_blk, const int* pblk_row,
-                                                     const T* pval, const T* t, T* c, const T* dinv,
-                                                     const double* step, T* x0) {
+                                                     const T* pval, const T* t, U* c, const U* dinv,
+                                                     const double* step, U* x0) {
   __shared__ T part[4][64];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t J = (int64_t)blockIdx.x * 4 + wv;
@@ -2358,15 +2372,15 @@ __global__ __launch_bounds__(256) void k_restrict_pw(int64_t nJ, const uint32_t*
   part[wv][lane] = acc;
   __syncthreads();
   if (!live || lane >= 6) return;
-  T all[6];
+  U all[6];
 #pragma unroll
   for (int j = 0; j < 6; j++) {
     T sum = 0;
     for (int g2 = 0; g2 < 10; g2++) sum += part[wv][6 * g2 + j];
-    all[j] = sum;
+    all[j] = (U)sum;
   }
   c[6 * J + lane] = all[lane];
-  if (x0) x0[6 * J + lane] = (T)step[ast[J]] * row6_dot<T>(dinv + 36 * J + 6 * lane, all);
+  if (x0) x0[6 * J + lane] = row6_dot<U>(dinv + 36 * J + 6 * lane, all) * (U)step[ast[J]];
 }
 // The transfers of a large FP32 level through PBlock32 records, G lanes per aggregate,
 // each lane whole blocks in turn (ten 16-byte loads and the other end's six entries),
@@ -2465,15 +2479,15 @@ __global__ void k_pblocks_cols(int64_t nb, const int* pt_blk, const int* pblk_ro
   out[kk] = r;
 }
 // prolong: x[i] += sum over row i of P y
-template <class T>
+template <class T, class TY = T>
 __global__ void k_prolong_p(int64_t n, const uint32_t* ast, const uint8_t* done, const int* prow, const int* pcol,
-                            const T* pval, const T* y, T* x) {
+                            const T* pval, const TY* y, T* x) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t i = g >> 3;
   const int a = (int)(g & 7);
   if (i >= n || a >= 6) return;
   if (done && done[ast[i]]) return;
-  x[6 * i + a] += block_row_dot<T>(prow[i], prow[i + 1], pcol, pval, y, a);
+  x[6 * i + a] += block_row_dot<T, TY>(prow[i], prow[i + 1], pcol, pval, y, a);
 }
 
 // rigid prolongation level 1 -> fine, overwriting: x[v] = Sh(r_v) y[label[v]] (0 off aggregates)
@@ -3556,6 +3570,24 @@ struct Solver {
     const T* pv = std::is_same<T, double>::value ? (const T*)lv.pval64.p : (const T*)lv.pval32.p;
     T* u = out;
     c_resid<T>(lv, 0, rhs, x, u);
+    if constexpr (std::is_same<T, float>::value) {
+      // the level above runs in FP64 and is small: the restriction writes its FP64
+      // right-hand side and damped Jacobi start, and the prolongation reads its FP64
+      // result (the conversions and the start's launch folded in, the same values)
+      if (c.fold_switch && (int)(li + 1) >= c.f64_from && lv.pblocks && lv.prc.p == nullptr &&
+          up.n <= kWideRestrict) {
+        const bool fx = !exact_level(li + 1);
+        k_restrict_pw<float, double><<<(unsigned)((up.n + 3) / 4), 256, 0, st>>>(
+            up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(), lv.pt_blk.as<int>(), lv.pblk_row.as<int>(),
+            pv, u, lvbuf<double>(up, 0), fx ? lvD<double>(up) : nullptr, fx ? up.step_d.as<double>() : nullptr,
+            fx ? lvbuf<double>(up, 1) : nullptr);
+        const double* y64 = coarse_cycle<double>(li + 1, fx);
+        k_prolong_p<float, double><<<nblk(8 * n), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.pcol.as<int>(),
+                                                                pv, y64, x);
+        c_resid<T>(lv, 1, rhs, x, out, add);
+        return out;
+      }
+    }
     // the coarser level's first cycle starts from w D^-1 of its right-hand side: written
     // with it, unless that level is solved exactly or switches precision
     const bool fuse = !exact_level(li + 1) && !(std::is_same<T, float>::value && (int)(li + 1) >= c.f64_from);
@@ -4073,6 +4105,8 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   // up, and the FP32 levels cut 7% of the solve
   c.f64_from = 4;
   if (const char* e = std::getenv("STATICS_HIP_F64FROM")) c.f64_from = std::max(0, std::atoi(e));
+  c.fold_switch = true;  // STATICS_HIP_FOLD=0: the switch to FP64 through separate conversions
+  if (const char* e = std::getenv("STATICS_HIP_FOLD")) c.fold_switch = std::atoi(e) != 0;
   c.half_coarse = true;
   if (const char* e = std::getenv("STATICS_HIP_HALF")) c.half_coarse = std::atoi(e) != 0;
   autotune_apply(c);
