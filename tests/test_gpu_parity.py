@@ -225,16 +225,21 @@ def test_fused_restriction_matches(ctx, monkeypatch):
         assert p1 == p2 and i1 == i2 and rel(x1, x2) < 1e-8, (p1, i1, i2, rel(x1, x2))
 
 
+def _device_structure(ctx, n, seed):
+    occ, theta = sh.generate(n, n, n, seed=seed, sigma=0.5)
+    ctx.upload(*sh.cells_from_occ(occ, theta))
+    ctx.joined_to_held()  # sets the gone nodes
+    f = np.zeros((ctx.nodes, 6))
+    f[:, 2] = -1.0
+    return f
+
+
 def test_galerkin_layouts_agree(ctx, monkeypatch):
     """The coarse Galerkin products with tasks by (i, j) pairs and grouped by column j (the same
     terms added in another order) against each other: the same iterations and the same solution
     to 1e-10, in both precisions. A 64^3 structure, whose hierarchy has five coarse levels, and
     each solve rebuilding it."""
-    occ, theta = sh.generate(64, 64, 64, seed=1, sigma=0.5)
-    ctx.upload(*sh.cells_from_occ(occ, theta))
-    ctx.joined_to_held()  # sets the gone nodes
-    f = np.zeros((ctx.nodes, 6))
-    f[:, 2] = -1.0
+    f = _device_structure(ctx, 64, 1)
     out = {}
     for pairs in ("1", "0"):
         monkeypatch.setenv("STATICS_HIP_GAL_PAIRS", pairs)
@@ -245,6 +250,36 @@ def test_galerkin_layouts_agree(ctx, monkeypatch):
     for prec in (sh.FP64, sh.MIXED):
         (i1, x1), (i0, x0) = out["1", prec], out["0", prec]
         assert i1 == i0 and rel(x0, x1) < 1e-10, (prec, i1, i0, rel(x0, x1))
+
+
+def test_coarse_layouts_agree(ctx, monkeypatch):
+    """Every layout of the coarse residual kernels forced on every level (STATICS_HIP_WIDE=k:
+    on the FP16 levels the record kernel's lanes per row and blocks in flight, elsewhere the
+    lane layouts) against the tuned choice: mixed solves to 1e-10 on a 48^3 structure reach
+    the same solution to 1e-8, within two iterations."""
+    f = _device_structure(ctx, 48, 2)
+    x0, st0 = ctx.solve(f, precision=sh.MIXED, tol=1e-10, release_hierarchy=True)
+    assert st0.status == 0
+    for k in range(8):
+        monkeypatch.setenv("STATICS_HIP_WIDE", str(k))
+        x, st = ctx.solve(f, precision=sh.MIXED, tol=1e-10, release_hierarchy=True)
+        assert st.status == 0 and abs(st.iters - st0.iters) <= 2 and rel(x, x0) < 1e-8, (k, st.iters, rel(x, x0))
+
+
+def test_transfer_variants_agree(ctx, monkeypatch):
+    """The transfers below large FP32 levels through prolongator records against the block
+    arrays (STATICS_HIP_PREC=0), and the switch to FP64 folded into the restriction and
+    prolongation against separate conversions (STATICS_HIP_FOLD=0): mixed solves to 1e-10 on
+    a 128^3 structure (level 2 above 8,192 aggregates) reach the same solution to 1e-8, within
+    one iteration."""
+    f = _device_structure(ctx, 128, 1)
+    x0, st0 = ctx.solve(f, precision=sh.MIXED, tol=1e-10, release_hierarchy=True)
+    assert st0.status == 0
+    for var in ("STATICS_HIP_PREC", "STATICS_HIP_FOLD"):
+        monkeypatch.setenv(var, "0")
+        x, st = ctx.solve(f, precision=sh.MIXED, tol=1e-10, release_hierarchy=True)
+        monkeypatch.delenv(var)
+        assert st.status == 0 and abs(st.iters - st0.iters) <= 1 and rel(x, x0) < 1e-8, (var, st.iters, rel(x, x0))
 
 
 def test_reference_stopping_rule(ctx):
