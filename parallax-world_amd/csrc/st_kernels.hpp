@@ -646,6 +646,34 @@ __host__ __device__ constexpr int sym_idx(int i, int j) {
   return i <= j ? 6 * i - i * (i - 1) / 2 + (j - i) : 6 * j - j * (j - 1) / 2 + (i - j);
 }
 constexpr int kSym = 21;
+// A node's packed D^-1 (the 21 entries of the upper triangle): FP64 as 21 doubles; in
+// the FP32 V-cycle as a 48-byte record of the entries in FP16 over the node's own scale
+// (its largest entry maps to 2^14), read as three 16-byte loads instead of 21 4-byte
+// ones (84 bytes).
+struct alignas(16) SymHalf {
+  _Float16 e[22];  // e[21] unused
+  float scale;
+};
+static_assert(sizeof(SymHalf) == 48, "SymHalf is three 16-byte words");
+template <class T>
+__device__ __forceinline__ void load_sym(const T* __restrict__ dinv, size_t n, T* m) {
+#pragma unroll
+  for (int k = 0; k < kSym; k++) m[k] = dinv[kSym * n + k];
+}
+template <>
+__device__ __forceinline__ void load_sym<float>(const float* __restrict__ dinv, size_t n, float* m) {
+  typedef unsigned st_sym_u4 __attribute__((ext_vector_type(4)));
+  typedef _Float16 st_sym_h2 __attribute__((ext_vector_type(2)));
+  const st_sym_u4* q = reinterpret_cast<const st_sym_u4*>(reinterpret_cast<const SymHalf*>(dinv) + n);
+  const st_sym_u4 a = q[0], b = q[1], c = q[2];
+  const unsigned w[12] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w};
+  const float sc = __builtin_bit_cast(float, w[11]);
+#pragma unroll
+  for (int k = 0; k < kSym; k++) {
+    const st_sym_h2 h = __builtin_bit_cast(st_sym_h2, w[k >> 1]);
+    m[k] = (float)((k & 1) ? h.y : h.x) * sc;
+  }
+}
 
 // out = M v, M symmetric 6x6 given by its upper triangle a (registers)
 template <class T>
@@ -658,11 +686,11 @@ __device__ __forceinline__ void sym6v_reg(const T* a, const T* v, T* out) {
     out[i] = s;
   }
 }
+// D^-1 of node n (packed, see SymHalf) times v
 template <class T>
-__device__ __forceinline__ void sym6v(const T* __restrict__ m, const T* v, T* out) {
+__device__ __forceinline__ void sym6v(const T* __restrict__ dinv, size_t n, const T* v, T* out) {
   T a[kSym];
-#pragma unroll
-  for (int k = 0; k < kSym; k++) a[k] = m[k];
+  load_sym<T>(dinv, n, a);
   sym6v_reg<T>(a, v, out);
 }
 
@@ -695,8 +723,7 @@ struct EpiSmooth : EpiBase {
   __device__ Pre pre(uint32_t n) const {
     Pre p;
     load6<T>(b + 6 * (size_t)n, p.b);
-#pragma unroll
-    for (int k = 0; k < kSym; k++) p.m[k] = dinv[kSym * (size_t)n + k];
+    load_sym<T>(dinv, n, p.m);
     return p;
   }
   __device__ double node(uint32_t n, const T* a, uint32_t s, const T* xn) const { return node_pre(n, a, s, xn, pre(n)); }
@@ -741,8 +768,7 @@ struct EpiResidJac : EpiBase {
   __device__ Pre pre(uint32_t n) const {
     Pre p;
     load6<T>(b + 6 * (size_t)n, p.b);
-#pragma unroll
-    for (int k = 0; k < kSym; k++) p.m[k] = dinv[kSym * (size_t)n + k];
+    load_sym<T>(dinv, n, p.m);
     return p;
   }
   __device__ double node(uint32_t n, const T* a, uint32_t s, const T* xn) const { return node_pre(n, a, s, xn, pre(n)); }
@@ -775,8 +801,7 @@ struct EpiProlong : EpiBase {
   __device__ Pre pre(uint32_t n) const {
     Pre p;
     load6<T>(x + 6 * (size_t)n, p.x);
-#pragma unroll
-    for (int k = 0; k < kSym; k++) p.m[k] = dinv[kSym * (size_t)n + k];
+    load_sym<T>(dinv, n, p.m);
     return p;
   }
   __device__ double node(uint32_t n, const T* a, uint32_t s, const T* un) const { return node_pre(n, a, s, un, pre(n)); }

@@ -334,7 +334,7 @@ __global__ void k_sort_keys(int64_t n, const int* label, int* keys, int* vals) {
 // D_v = sum of the own blocks of the live beams at node v (statics/src/bodies.rs:151-153).
 __global__ __launch_bounds__(256) void k_dinv(int64_t nnodes, const uint8_t* nflags, const int32_t* ncell, const uint8_t* nkind,
                        const double* ce, const double* cg, const double* csec, const uint16_t* cmask,
-                       const uint32_t* cstruct, const BeamGeo* bgeo, double* dinv64, float* dinv32, int* bad) {
+                       const uint32_t* cstruct, const BeamGeo* bgeo, double* dinv64, SymHalf* dinv32, int* bad) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nnodes) return;
   double D[36], inv[36];
@@ -359,13 +359,22 @@ __global__ __launch_bounds__(256) void k_dinv(int64_t nnodes, const uint8_t* nfl
   } else {
     for (int i = 0; i < 36; i++) inv[i] = 0.0;
   }
+  double m = 0.0;
   for (int i = 0; i < 6; i++)
     for (int j = i; j < 6; j++) {
       // the inverse of a symmetric block, symmetrised against rounding
       const double a = 0.5 * (inv[6 * i + j] + inv[6 * j + i]);
       dinv64[kSym * v + sym_idx(i, j)] = a;
-      dinv32[kSym * v + sym_idx(i, j)] = (float)a;
+      m = fmax(m, fabs(a));
     }
+  // the FP32 V-cycle's copy: SymHalf (FP16 entries over the node's scale)
+  const double sc = m > 0.0 ? m / 16384.0 : 1.0;
+  const float scf = (float)sc;
+  SymHalf h;
+  for (int k = 0; k < 22; k++) h.e[k] = (_Float16)0.0f;
+  for (int k = 0; k < kSym; k++) h.e[k] = (_Float16)(float)(dinv64[kSym * v + k] / (double)scf);
+  h.scale = scf;
+  dinv32[v] = h;
 }
 
 // ----------------------------------------------------------- brick vector ops
@@ -381,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_fine(const uint32_t* nbase, c
   for (uint32_t n = nbase[b] + threadIdx.x; n < nbase[b + 1]; n += blockDim.x) {
     T rv[6], d[6];
     load6<T>(r + 6 * (size_t)n, rv);
-    sym6v<T>(dinv + kSym * (size_t)n, rv, d);
+    sym6v<T>(dinv, n, rv, d);
     for (int k = 0; k < 6; k++) d[k] *= w;
     store6<T>(x + 6 * (size_t)n, d);
   }
@@ -2629,13 +2638,13 @@ struct Solver {
   void setup_fine() {
     const int64_t nn = g.nnodes;
     c.dinv64.ensure(std::max<int64_t>(nn, 1) * kSym * sizeof(double));
-    c.dinv32.ensure(std::max<int64_t>(nn, 1) * kSym * sizeof(float));
+    c.dinv32.ensure(std::max<int64_t>(nn, 1) * sizeof(SymHalf));
     ST_HIP(hipMemsetAsync(sbad.p, 0, sizeof(int), st));
     if (nn)
       k_dinv<<<nblk(nn), 256, 0, st>>>(nn, g.nflags.as<uint8_t>(), g.ncell.as<int32_t>(), g.nkind.as<uint8_t>(),
                                        g.ce64.as<double>(), g.cg64.as<double>(), g.csec64.as<double>(), g.cmask.as<uint16_t>(),
                                        g.cstruct.as<uint32_t>(), g.bgeo.as<BeamGeo>(), c.dinv64.as<double>(),
-                                       c.dinv32.as<float>(), sbad.as<int>());
+                                       c.dinv32.as<SymHalf>(), sbad.as<int>());
     ST_HIP(hipGetLastError());
     if (read_int(sbad.as<int>())) throw StError{ST_NOT_SPD, "a free node has a singular stiffness block"};
     // smoother step: 4/3 over the greatest eigenvalue of D^-1 K (statics/src/bodies.rs:174, 180-190)
