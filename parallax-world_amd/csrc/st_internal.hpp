@@ -192,6 +192,7 @@ struct Level {
   // (mixed, large levels) the FP32 blocks as PBlock32 records in row order (column index
   // inside) and in column order (row index inside), for the V-cycle's transfers; pval32 goes
   DBuf prr, prc;
+  bool prh = false;  // prr and prc hold HalfBlock records (FP16 over a per-block scale)
   DBuf agg_struct;      // per aggregate: structure
   int layout32 = 0, layout64 = 0;  // the residual kernel's tuned layout per precision (see k_c_resid)
   std::vector<double> step;  // smoother step per structure
@@ -248,6 +249,7 @@ struct Ctx {
                             // if empty, gamma on the indices (gamma_from, gamma_to]
   int f64_from = 4;      // first level (0: level 1) kept in FP64 by the mixed V-cycle
   bool fold_switch = true;  // the switch to FP64 folded into the restriction and prolongation
+  bool p_half = true;       // the V-cycle's prolongator records in FP16
   bool half_coarse = true;  // the mixed V-cycle's largest coarse operators stored in FP16
   bool graphs = false;   // replay each CG iteration's tail as a captured graph
   bool dense_top = true; // invert a small coarse level densely and drop the levels above

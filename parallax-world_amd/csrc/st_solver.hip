@@ -2408,9 +2408,29 @@ __device__ __forceinline__ void load_pblock(const PBlock32* p, float* a, int& id
   }
   idx = __builtin_bit_cast(int, q[9].x);
 }
-template <int G>
+// (the records in FP16: HalfBlock, the other end's index in its column field)
+__device__ __forceinline__ void load_pblock(const HalfBlock* p, float* a, int& idx) {
+  st_u4 q[5];
+  load_half_block(p, q);
+  unsigned w[20];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    w[4 * k] = q[k].x;
+    w[4 * k + 1] = q[k].y;
+    w[4 * k + 2] = q[k].z;
+    w[4 * k + 3] = q[k].w;
+  }
+  const float sc = __builtin_bit_cast(float, w[18]);
+#pragma unroll
+  for (int e = 0; e < 36; e++) {
+    const st_h2 h = __builtin_bit_cast(st_h2, w[e >> 1]);
+    a[e] = (float)((e & 1) ? h.y : h.x) * sc;
+  }
+  idx = (int)w[19];
+}
+template <int G, class PB>
 __global__ __launch_bounds__(256) void k_restrict_rec(int64_t nJ, const uint32_t* ast, const uint8_t* done,
-                                                      const int* pt_ptr, const PBlock32* pc, const float* t, float* c,
+                                                      const int* pt_ptr, const PB* pc, const float* t, float* c,
                                                       const float* dinv, const double* step, float* x0) {
   const int gl = threadIdx.x & (G - 1);
   const int64_t J = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
@@ -2438,9 +2458,9 @@ __global__ __launch_bounds__(256) void k_restrict_rec(int64_t nJ, const uint32_t
   if (x0) x0[6 * J + gl] = (float)step[ast[J]] * row6_dot<float>(dinv + 36 * J + 6 * gl, y);
 }
 // Prolongation: x[i] += sum over row i of P y.
-template <int G>
+template <int G, class PB>
 __global__ __launch_bounds__(256) void k_prolong_rec(int64_t n, const uint32_t* ast, const uint8_t* done,
-                                                     const int* prow, const PBlock32* pr, const float* y, float* x) {
+                                                     const int* prow, const PB* pr, const float* y, float* x) {
   const int gl = threadIdx.x & (G - 1);
   const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const bool live = i < n && !(done && done[ast[i]]);
@@ -2467,24 +2487,34 @@ __global__ __launch_bounds__(256) void k_prolong_rec(int64_t n, const uint32_t* 
 }
 // The records: rows (block k in storage order, its column) or columns (position kk of
 // the column grouping, block pt_blk[kk], its row).
-__global__ void k_pblocks_rows(int64_t nb, const int* pcol, const float* pval, PBlock32* out) {
+__device__ __forceinline__ void fill_pblock(const float* a, int idx, PBlock32& r) {
+  for (int e = 0; e < 36; e++) r.a[e] = a[e];
+  r.idx = idx;
+  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+}
+__device__ __forceinline__ void fill_pblock(const float* a, int idx, HalfBlock& r) {
+  float m = 0.0f;
+  for (int e = 0; e < 36; e++) m = fmaxf(m, fabsf(a[e]));
+  const float sc = m > 0.0f ? m / 16384.0f : 1.0f;
+  for (int e = 0; e < 36; e++) r.a[e] = (_Float16)(a[e] / sc);
+  r.scale = sc;
+  r.col = idx;
+}
+template <class PB>
+__global__ void k_pblocks_rows(int64_t nb, const int* pcol, const float* pval, PB* out) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nb) return;
-  PBlock32 r;
-  for (int e = 0; e < 36; e++) r.a[e] = pval[36 * k + e];
-  r.idx = pcol[k];
-  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+  PB r;
+  fill_pblock(pval + 36 * k, pcol[k], r);
   out[k] = r;
 }
-__global__ void k_pblocks_cols(int64_t nb, const int* pt_blk, const int* pblk_row, const float* pval,
-                               PBlock32* out) {
+template <class PB>
+__global__ void k_pblocks_cols(int64_t nb, const int* pt_blk, const int* pblk_row, const float* pval, PB* out) {
   const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (kk >= nb) return;
   const int blk = pt_blk[kk];
-  PBlock32 r;
-  for (int e = 0; e < 36; e++) r.a[e] = pval[36 * (int64_t)blk + e];
-  r.idx = pblk_row[blk];
-  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+  PB r;
+  fill_pblock(pval + 36 * (int64_t)blk, pblk_row[blk], r);
   out[kk] = r;
 }
 // prolong: x[i] += sum over row i of P y
@@ -3255,12 +3285,21 @@ struct Solver {
       for (size_t li = 0; li + 1 < c.levels.size() && (int)li < c.f64_from; li++) {
         Level& lv = c.levels[li];
         if (!lv.pblocks || c.levels[li + 1].n <= kWideRestrict) continue;
-        lv.prr.ensure((size_t)lv.pblocks * sizeof(PBlock32));
-        lv.prc.ensure((size_t)lv.pblocks * sizeof(PBlock32));
-        k_pblocks_rows<<<nblk(lv.pblocks), 256, 0, st>>>(lv.pblocks, lv.pcol.as<int>(), lv.pval32.as<float>(),
-                                                         lv.prr.as<PBlock32>());
-        k_pblocks_cols<<<nblk(lv.pblocks), 256, 0, st>>>(lv.pblocks, lv.pt_blk.as<int>(), lv.pblk_row.as<int>(),
-                                                         lv.pval32.as<float>(), lv.prc.as<PBlock32>());
+        lv.prh = c.p_half && lv.half;
+        const size_t rb = lv.prh ? sizeof(HalfBlock) : sizeof(PBlock32);
+        lv.prr.ensure((size_t)lv.pblocks * rb);
+        lv.prc.ensure((size_t)lv.pblocks * rb);
+        if (lv.prh) {
+          k_pblocks_rows<<<nblk(lv.pblocks), 256, 0, st>>>(lv.pblocks, lv.pcol.as<int>(), lv.pval32.as<float>(),
+                                                           lv.prr.as<HalfBlock>());
+          k_pblocks_cols<<<nblk(lv.pblocks), 256, 0, st>>>(lv.pblocks, lv.pt_blk.as<int>(), lv.pblk_row.as<int>(),
+                                                           lv.pval32.as<float>(), lv.prc.as<HalfBlock>());
+        } else {
+          k_pblocks_rows<<<nblk(lv.pblocks), 256, 0, st>>>(lv.pblocks, lv.pcol.as<int>(), lv.pval32.as<float>(),
+                                                           lv.prr.as<PBlock32>());
+          k_pblocks_cols<<<nblk(lv.pblocks), 256, 0, st>>>(lv.pblocks, lv.pt_blk.as<int>(), lv.pblk_row.as<int>(),
+                                                           lv.pval32.as<float>(), lv.prc.as<PBlock32>());
+        }
         ST_HIP(hipGetLastError());
         lv.pval32.release();
       }
@@ -3607,10 +3646,15 @@ struct Solver {
     bool recs = false;
     if constexpr (std::is_same<T, float>::value) recs = lv.prc.p != nullptr;
     if (recs) {
-      if constexpr (std::is_same<T, float>::value)
-        k_restrict_rec<16><<<(unsigned)((up.n * 16 + 255) / 256), 256, 0, st>>>(
-            up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(), lv.prc.as<PBlock32>(), u, up_r, up_d,
-            up_step, up_x0);
+      if constexpr (std::is_same<T, float>::value) {
+        const unsigned gr = (unsigned)((up.n * 16 + 255) / 256);
+        if (lv.prh)
+          k_restrict_rec<16><<<gr, 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(),
+                                                 lv.prc.as<HalfBlock>(), u, up_r, up_d, up_step, up_x0);
+        else
+          k_restrict_rec<16><<<gr, 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(),
+                                                 lv.prc.as<PBlock32>(), u, up_r, up_d, up_step, up_x0);
+      }
     } else if (lv.pblocks && up.n <= kWideRestrict)
       k_restrict_pw<T><<<(unsigned)((up.n + 3) / 4), 256, 0, st>>>(
           up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(), lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv,
@@ -3625,9 +3669,13 @@ struct Solver {
                                                   up_r, up_d, up_step, up_x0);
     const T* y = descend<T>(li + 1, fuse);
     if (recs) {
-      if constexpr (std::is_same<T, float>::value)
-        k_prolong_rec<8><<<(unsigned)((n * 8 + 255) / 256), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(),
-                                                                          lv.prr.as<PBlock32>(), y, x);
+      if constexpr (std::is_same<T, float>::value) {
+        const unsigned gp = (unsigned)((n * 8 + 255) / 256);
+        if (lv.prh)
+          k_prolong_rec<8><<<gp, 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.prr.as<HalfBlock>(), y, x);
+        else
+          k_prolong_rec<8><<<gp, 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.prr.as<PBlock32>(), y, x);
+      }
     } else if (lv.pblocks)
       k_prolong_p<T><<<nblk(8 * n), 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.pcol.as<int>(), pv, y, x);
     else
@@ -4114,6 +4162,10 @@ void solve_all(Ctx& c, const StSolveOpts& o, std::vector<StStats>& stats) {
   // up, and the FP32 levels cut 7% of the solve
   c.f64_from = 4;
   if (const char* e = std::getenv("STATICS_HIP_F64FROM")) c.f64_from = std::max(0, std::atoi(e));
+  // the prolongator records in FP16 (block-scaled as the coarse operators) where the level
+  // below runs its operator in FP16; STATICS_HIP_PHALF=0 keeps them FP32
+  c.p_half = true;
+  if (const char* e = std::getenv("STATICS_HIP_PHALF")) c.p_half = std::atoi(e) != 0;
   c.fold_switch = true;  // STATICS_HIP_FOLD=0: the switch to FP64 through separate conversions
   if (const char* e = std::getenv("STATICS_HIP_FOLD")) c.fold_switch = std::atoi(e) != 0;
   c.half_coarse = true;

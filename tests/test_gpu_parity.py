@@ -267,12 +267,15 @@ def test_coarse_layouts_agree(ctx, monkeypatch):
 
 
 def test_transfer_variants_agree(ctx, monkeypatch):
-    """The transfers below large FP32 levels through prolongator records against the block
+    """The transfers below large FP32 levels through FP32 prolongator records against the block
     arrays (STATICS_HIP_PREC=0), and the switch to FP64 folded into the restriction and
-    prolongation against separate conversions (STATICS_HIP_FOLD=0): mixed solves to 1e-10 on
-    a 128^3 structure (level 2 above 8,192 aggregates) reach the same solution to 1e-8, within
-    one iteration."""
+    prolongation against separate conversions (STATICS_HIP_FOLD=0): the same sums in another
+    order or the same values, so mixed solves to 1e-10 on a 128^3 structure (level 2 above
+    8,192 aggregates) reach the same solution to 1e-8 within one iteration. The records in
+    FP16 (the default) change the preconditioner slightly: the same solution to 1e-7 within
+    three iterations."""
     f = _device_structure(ctx, 128, 1)
+    monkeypatch.setenv("STATICS_HIP_PHALF", "0")
     x0, st0 = ctx.solve(f, precision=sh.MIXED, tol=1e-10, release_hierarchy=True)
     assert st0.status == 0
     for var in ("STATICS_HIP_PREC", "STATICS_HIP_FOLD"):
@@ -280,6 +283,9 @@ def test_transfer_variants_agree(ctx, monkeypatch):
         x, st = ctx.solve(f, precision=sh.MIXED, tol=1e-10, release_hierarchy=True)
         monkeypatch.delenv(var)
         assert st.status == 0 and abs(st.iters - st0.iters) <= 1 and rel(x, x0) < 1e-8, (var, st.iters, rel(x, x0))
+    monkeypatch.delenv("STATICS_HIP_PHALF")
+    x, st = ctx.solve(f, precision=sh.MIXED, tol=1e-10, release_hierarchy=True)
+    assert st.status == 0 and abs(st.iters - st0.iters) <= 3 and rel(x, x0) < 1e-7, (st.iters, st0.iters, rel(x, x0))
 
 
 def test_reference_stopping_rule(ctx):
