@@ -3058,7 +3058,7 @@ struct Solver {
       float* b = w.as<float>();
       k_c_init32<<<nblk(n), 256, 0, st>>>(n, a);
       for (int it = 0; it < iters; it++) {
-        k_c_resid_h<8, 2><<<half_blocks_grid<8>(n), 256, 0, st>>>(
+        k_c_resid_h<16, 4><<<half_blocks_grid<16>(n), 256, 0, st>>>(
             0, n, lv.agg_struct.as<uint32_t>(), nullptr, lv.rowptr.as<int>(), lv.hblk.as<HalfBlock>(),
             lv.dinv32.as<float>(), neg.as<double>(), zr.as<float>(), a, scratch.as<float>(), nullptr, b);
         if (it >= iters - 2) {
