@@ -3564,7 +3564,7 @@ struct Solver {
     const int64_t n = lv.n;
     const uint32_t* ast = lv.agg_struct.as<uint32_t>();
     if (lv.dense) {  // the dense top: exact
-      const dim3 gd((unsigned)((lv.dense_mmax + 3) / 4), (unsigned)ns);
+      const dim3 gd(nblk(lv.dense_mmax, 4), (unsigned)ns);
       k_dense_apply<T><<<gd, 256, 0, st>>>(lv.dense_seg.as<int64_t>(), lv.dense_off.as<int64_t>(),
                                           lv.dense_a.as<double>(), done, r, x);
       return x;
@@ -3625,7 +3625,7 @@ struct Solver {
       if (c.fold_switch && (int)(li + 1) >= c.f64_from && lv.pblocks && lv.prc.p == nullptr &&
           up.n <= kWideRestrict) {
         const bool fx = !exact_level(li + 1);
-        k_restrict_pw<float, double><<<(unsigned)((up.n + 3) / 4), 256, 0, st>>>(
+        k_restrict_pw<float, double><<<nblk(up.n, 4), 256, 0, st>>>(
             up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(), lv.pt_blk.as<int>(), lv.pblk_row.as<int>(),
             pv, u, lvbuf<double>(up, 0), fx ? lvD<double>(up) : nullptr, fx ? up.step_d.as<double>() : nullptr,
             fx ? lvbuf<double>(up, 1) : nullptr);
@@ -3647,7 +3647,7 @@ struct Solver {
     if constexpr (std::is_same<T, float>::value) recs = lv.prc.p != nullptr;
     if (recs) {
       if constexpr (std::is_same<T, float>::value) {
-        const unsigned gr = (unsigned)((up.n * 16 + 255) / 256);
+        const unsigned gr = nblk(up.n * 16);
         if (lv.prh)
           k_restrict_rec<16><<<gr, 256, 0, st>>>(up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(),
                                                  lv.prc.as<HalfBlock>(), u, up_r, up_d, up_step, up_x0);
@@ -3656,7 +3656,7 @@ struct Solver {
                                                  lv.prc.as<PBlock32>(), u, up_r, up_d, up_step, up_x0);
       }
     } else if (lv.pblocks && up.n <= kWideRestrict)
-      k_restrict_pw<T><<<(unsigned)((up.n + 3) / 4), 256, 0, st>>>(
+      k_restrict_pw<T><<<nblk(up.n, 4), 256, 0, st>>>(
           up.n, up.agg_struct.as<uint32_t>(), done, lv.pt_ptr.as<int>(), lv.pt_blk.as<int>(), lv.pblk_row.as<int>(), pv,
           u, up_r, up_d, up_step, up_x0);
     else if (lv.pblocks)
@@ -3670,7 +3670,7 @@ struct Solver {
     const T* y = descend<T>(li + 1, fuse);
     if (recs) {
       if constexpr (std::is_same<T, float>::value) {
-        const unsigned gp = (unsigned)((n * 8 + 255) / 256);
+        const unsigned gp = nblk(n * 8);
         if (lv.prh)
           k_prolong_rec<8><<<gp, 256, 0, st>>>(n, ast, done, lv.prow.as<int>(), lv.prr.as<HalfBlock>(), y, x);
         else
