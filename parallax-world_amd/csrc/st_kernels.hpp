@@ -752,14 +752,14 @@ struct EpiSmooth : EpiBase {
   }
 };
 
-// r1 = b - K x and s = step D^-1 r1 (the restriction's smoothing step, fused)
+// sout = x + step D^-1 (b - K x) without the dot (the V-cycle's smoothed restriction
+// forms its argument b - K sout from it)
 template <class T>
 struct EpiResidJac : EpiBase {
   static constexpr bool kReduce = false;
   const T* b;
   const T* dinv;
   const double* step;
-  T* r1;
   T* sout;
   static constexpr bool kPre = true;
   struct Pre {
@@ -772,16 +772,16 @@ struct EpiResidJac : EpiBase {
     return p;
   }
   __device__ double node(uint32_t n, const T* a, uint32_t s, const T* xn) const { return node_pre(n, a, s, xn, pre(n)); }
-  __device__ double node_pre(uint32_t n, const T* a, uint32_t s, const T*, const Pre& p) const {
-    T t[6], d[6];
+  __device__ double node_pre(uint32_t n, const T* a, uint32_t s, const T* xn, const Pre& p) const {
+    T t[6], d[6], xv[6];
+    load6<T>(xn, xv);
 #pragma unroll
     for (int k = 0; k < 6; k++) t[k] = p.b[k] - a[k];
-    store6<T>(r1 + 6 * (size_t)n, t);
     sym6v_reg<T>(p.m, t, d);
     const T w = (T)step[s];
 #pragma unroll
-    for (int k = 0; k < 6; k++) d[k] *= w;
-    store6<T>(sout + 6 * (size_t)n, d);
+    for (int k = 0; k < 6; k++) xv[k] += w * d[k];
+    store6<T>(sout + 6 * (size_t)n, xv);
     return 0.0;
   }
 };

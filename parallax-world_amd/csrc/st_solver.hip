@@ -3742,18 +3742,19 @@ struct Solver {
     };
     smooth_steps(c.nu - 1);
     if (!c.levels.empty() && c.smoothed) {
-      // r1 = r - K x1, s1 = w D^-1 r1; u = r1 - K s1 (= the smoothed restriction's argument);
-      // c = P_rigid^T u; coarse; s1 = P_rigid y; x1 += s1 - w D^-1 K s1; z = x1 + w D^-1 (r - K x1)
+      // The smoothed restriction's argument is u = r1 - K w D^-1 r1 with r1 = r - K x1, that
+      // is u = r - K s1 for s1 = x1 + w D^-1 (r - K x1): one operator pass writes s1 (and
+      // no r1), the next forms u from r. Then c = P_rigid^T u; coarse; s1 = P_rigid y;
+      // x1 += s1 - w D^-1 K s1; z = x1 + w D^-1 (r - K x1)
       EpiResidJac<T> ej;
       ej.done = done;
       ej.b = r;
       ej.dinv = dinv;
       ej.step = c.step0_d.as<double>();
-      ej.r1 = r1;
       ej.sout = s1;
       launch_apply<T>(c, x1, ej);
       Level& l1 = c.levels[0];
-      residual_restrict<T>(s1, r1, r1, l1);
+      residual_restrict<T>(s1, r, r1, l1);
       const T* y1 = descend<T>(0, false);
       k_prolong1_set<T><<<nb, kBlock, 0, st>>>(g.nbase.as<uint32_t>(), g.bstruct.as<uint32_t>(), done,
                                                c.label1.as<int>(), l1.centre.as<double>(), g.nkey.as<uint32_t>(),
