@@ -3846,11 +3846,15 @@ struct Solver {
       // (mixed: r32 is kept equal to r by the residual update; z stays FP32 in z32)
       float* r32 = mixed ? c.r32.as<float>() : nullptr;
       float* z32 = mixed ? c.z32.as<float>() : nullptr;
+      // r.z of the mixed CG from the rounded r32 the V-cycle reads anyway (STATICS_HIP_RZ64=1:
+      // from the FP64 r, 48 more bytes a node in the last smoothing pass)
+      const char* rz = std::getenv("STATICS_HIP_RZ64");
+      const double* rdot = rz && std::atoi(rz) ? r : nullptr;
       auto precond = [&](int slot) {
         if (!mixed)
           vcycle<double>(r, z, c.t64.as<double>(), c.u64.as<double>(), c.w64.as<double>());
         else
-          vcycle<float>(r32, z32, c.t32.as<float>(), c.p32.as<float>(), c.w32.as<float>(), r);
+          vcycle<float>(r32, z32, c.t32.as<float>(), c.p32.as<float>(), c.w32.as<float>(), rdot);
         reduce_bricks(c.partial, slot);
       };
       if (mixed) k_copy_fine<float, double><<<nb, kBlock, 0, st>>>(nbase, bst, done, r, r32);
